@@ -1,0 +1,131 @@
+/*
+ * rdp_mi.h - C ABI of the MI355X RDP rendering library (librdp_mi.so).
+ *
+ * Drop-in boundary for the reference's RDP::CommandProcessor
+ * (parallel-rdp/rdp_device.hpp:111-198).  One opaque processor per emulated
+ * RDP; plain pointers and sizes only.  Each entry point cites the reference
+ * interface it replaces.
+ *
+ * Memory model: RDRAM, hidden RDRAM and TMEM stay resident in GPU memory.
+ * rdp_mi_begin_read_* copies the device state into a host mirror (call it
+ * after rdp_mi_idle or rdp_mi_wait_for_timeline); rdp_mi_end_write_* uploads
+ * the host mirror back.  When the caller passes its own RDRAM pointer at
+ * creation, that buffer is the host mirror.
+ *
+ * Errors: entry points returning int return 0 on success and a negative
+ * RDP_MI_ERROR_* code otherwise; rdp_mi_last_error() gives a message.
+ * Nothing throws across the ABI.
+ */
+#ifndef RDP_MI_H
+#define RDP_MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rdp_mi_processor rdp_mi_processor;
+
+enum {
+	RDP_MI_OK = 0,
+	RDP_MI_ERROR_INVALID_ARGUMENT = -1,
+	RDP_MI_ERROR_OUT_OF_MEMORY = -2,
+	RDP_MI_ERROR_DEVICE = -3,
+	RDP_MI_ERROR_UNSUPPORTED = -4,
+};
+
+/* Flags accepted by rdp_mi_create (rdp_device.hpp:43-51).  Upscaling flags
+ * are rejected with RDP_MI_ERROR_UNSUPPORTED. */
+enum {
+	RDP_MI_FLAG_HOST_VISIBLE_HIDDEN_RDRAM = 1 << 0,
+	RDP_MI_FLAG_HOST_VISIBLE_TMEM = 1 << 1,
+	RDP_MI_FLAG_UPSCALING_2X = 1 << 2,
+	RDP_MI_FLAG_UPSCALING_4X = 1 << 3,
+	RDP_MI_FLAG_UPSCALING_8X = 1 << 4,
+};
+
+/* VI registers (rdp_common.hpp:324-341). */
+enum {
+	RDP_MI_VI_CONTROL = 0, RDP_MI_VI_ORIGIN, RDP_MI_VI_WIDTH, RDP_MI_VI_INTR, RDP_MI_VI_V_CURRENT_LINE,
+	RDP_MI_VI_TIMING, RDP_MI_VI_V_SYNC, RDP_MI_VI_H_SYNC, RDP_MI_VI_LEAP, RDP_MI_VI_H_START, RDP_MI_VI_V_START,
+	RDP_MI_VI_V_BURST, RDP_MI_VI_X_SCALE, RDP_MI_VI_Y_SCALE, RDP_MI_VI_COUNT
+};
+
+/* CommandProcessor::CommandProcessor(device, rdram_ptr, rdram_offset,
+ * rdram_size, hidden_rdram_size, flags), rdp_device.cpp:44-162.
+ * rdram_size must be a power of two; hidden_rdram_size must be
+ * rdram_size / 2.  RDRAM starts from *rdram_ptr (or zero), hidden RDRAM at
+ * 0x03 per byte, TMEM at zero. */
+int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, size_t rdram_size,
+                  size_t hidden_rdram_size, uint32_t flags);
+
+/* CommandProcessor::~CommandProcessor, rdp_device.cpp:164-167 (implies idle). */
+void rdp_mi_destroy(rdp_mi_processor *p);
+
+/* CommandProcessor::device_is_supported, rdp_device.cpp:225-228. */
+int rdp_mi_is_supported(const rdp_mi_processor *p);
+
+/* CommandProcessor::enqueue_command / enqueue_command_direct,
+ * rdp_device.cpp:874-959.  words holds one complete command. */
+void rdp_mi_enqueue_command(rdp_mi_processor *p, unsigned num_words, const uint32_t *words);
+void rdp_mi_enqueue_command_direct(rdp_mi_processor *p, unsigned num_words, const uint32_t *words);
+
+/* Enqueue a packed run of complete commands (no reference counterpart: the
+ * reference's callers loop over enqueue_command, which this performs in one
+ * call).  Returns the number of words consumed; a truncated trailing command
+ * is left unconsumed, a word with an undefined opcode is skipped. */
+size_t rdp_mi_enqueue_command_list(rdp_mi_processor *p, size_t num_words, const uint32_t *words);
+
+/* CommandProcessor::set_quirks, rdp_device.cpp:961-968 (in-band). */
+void rdp_mi_set_quirks(rdp_mi_processor *p, uint32_t quirks);
+
+/* CommandProcessor::flush / signal_timeline / wait_for_timeline / idle /
+ * begin_frame_context, rdp_device.cpp:169-174, 1039-1080. */
+void rdp_mi_flush(rdp_mi_processor *p);
+uint64_t rdp_mi_signal_timeline(rdp_mi_processor *p);
+int rdp_mi_wait_for_timeline(rdp_mi_processor *p, uint64_t value);
+int rdp_mi_idle(rdp_mi_processor *p);
+void rdp_mi_begin_frame_context(rdp_mi_processor *p);
+
+/* Memory interaction, rdp_device.cpp:997-1037. */
+void *rdp_mi_begin_read_rdram(rdp_mi_processor *p);
+void rdp_mi_end_write_rdram(rdp_mi_processor *p);
+void *rdp_mi_begin_read_hidden_rdram(rdp_mi_processor *p);
+void rdp_mi_end_write_hidden_rdram(rdp_mi_processor *p);
+size_t rdp_mi_get_rdram_size(const rdp_mi_processor *p);
+size_t rdp_mi_get_hidden_rdram_size(const rdp_mi_processor *p);
+void *rdp_mi_get_tmem(rdp_mi_processor *p);
+
+/* CommandProcessor::set_vi_register, rdp_device.cpp:970-974. */
+void rdp_mi_set_vi_register(rdp_mi_processor *p, unsigned reg, uint32_t value);
+
+/* CommandProcessor::scanout_sync, rdp_device.cpp:1159-1182: drains work,
+ * scans out through the VI and writes width*height RGBA8 words (alpha 255)
+ * into out, which must hold 640 * 576 words.  A blank or invalid frame
+ * gives 0x0. */
+int rdp_mi_scanout_sync(rdp_mi_processor *p, uint32_t *out, unsigned *width, unsigned *height);
+
+/* ValidationInterface (rdp_renderer.hpp:75-95), installed with
+ * CommandProcessor::set_validation_interface (rdp_device.cpp:230-233).
+ * Reports RDP states that would hang real hardware; never aborts. */
+typedef void (*rdp_mi_validation_fn)(uint32_t error, const char *message, void *user);
+void rdp_mi_set_validation_callback(rdp_mi_processor *p, rdp_mi_validation_fn fn, void *user);
+
+/* Batched submission across processors (no reference counterpart; the
+ * reference runs one processor per device queue).  Submits the pending
+ * render passes of all processors with one launch per stage per pass round,
+ * so independent streams share every kernel launch.  Results are identical
+ * to flushing each processor on its own. */
+int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count);
+
+/* Library diagnostics. */
+const char *rdp_mi_last_error(void);
+uint64_t rdp_mi_primitive_count(const rdp_mi_processor *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,131 @@
+"""ctypes binding for the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module.  It loads oracle/build/librdp_oracle.so, building it with make
+when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "librdp_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    vp, u32, u64, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t
+    lib.rdpo_create.restype = vp
+    lib.rdpo_create.argtypes = [u32]
+    lib.rdpo_destroy.argtypes = [vp]
+    lib.rdpo_rdram.restype = vp
+    lib.rdpo_rdram.argtypes = [vp]
+    lib.rdpo_hidden_rdram.restype = vp
+    lib.rdpo_hidden_rdram.argtypes = [vp]
+    lib.rdpo_tmem.restype = vp
+    lib.rdpo_tmem.argtypes = [vp]
+    lib.rdpo_process.restype = sz
+    lib.rdpo_process.argtypes = [vp, vp, sz]
+    lib.rdpo_set_vi_register.argtypes = [vp, ctypes.c_uint, u32]
+    lib.rdpo_scanout.restype = ctypes.c_int
+    lib.rdpo_scanout.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
+    lib.rdpo_fragment_count.restype = u64
+    lib.rdpo_fragment_count.argtypes = [vp]
+    lib.rdpo_primitive_count.restype = u64
+    lib.rdpo_primitive_count.argtypes = [vp]
+    lib.rdpo_command_length.restype = ctypes.c_uint
+    lib.rdpo_command_length.argtypes = [u32]
+    lib.rdpo_blender_divider_table.argtypes = [vp]
+    lib.rdpo_gamma_table.argtypes = [vp]
+    lib.rdpo_perspective_table.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+class Oracle:
+    """One emulated RDP: RDRAM, hidden RDRAM, TMEM and register state."""
+
+    def __init__(self, rdram_size: int = 8 << 20):
+        self._lib = _load()
+        self._h = self._lib.rdpo_create(rdram_size)
+        if not self._h:
+            raise RuntimeError("rdpo_create failed")
+        self.rdram_size = rdram_size
+        buf = (ctypes.c_uint8 * rdram_size).from_address(self._lib.rdpo_rdram(self._h))
+        self.rdram = np.frombuffer(buf, dtype=np.uint32)
+        hbuf = (ctypes.c_uint8 * (rdram_size // 2)).from_address(self._lib.rdpo_hidden_rdram(self._h))
+        self.hidden = np.frombuffer(hbuf, dtype=np.uint8)
+        tbuf = (ctypes.c_uint16 * 2048).from_address(self._lib.rdpo_tmem(self._h))
+        self.tmem = np.frombuffer(tbuf, dtype=np.uint16)
+
+    def close(self):
+        if self._h:
+            self._lib.rdpo_destroy(self._h)
+            self._h = None
+            self.rdram = self.hidden = self.tmem = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def process(self, words: np.ndarray) -> int:
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        return int(self._lib.rdpo_process(self._h, w.ctypes.data, w.size))
+
+    def set_vi_register(self, reg: int, value: int):
+        self._lib.rdpo_set_vi_register(self._h, reg, value & 0xFFFFFFFF)
+
+    def scanout(self):
+        out = np.zeros(640 * 576, dtype=np.uint32)
+        w = ctypes.c_uint(0)
+        h = ctypes.c_uint(0)
+        self._lib.rdpo_scanout(self._h, out.ctypes.data, ctypes.byref(w), ctypes.byref(h))
+        return out[: w.value * h.value].reshape(h.value, w.value) if w.value and h.value else out[:0].reshape(0, 0)
+
+    @property
+    def fragments(self) -> int:
+        return int(self._lib.rdpo_fragment_count(self._h))
+
+    @property
+    def primitives(self) -> int:
+        return int(self._lib.rdpo_primitive_count(self._h))
+
+
+def command_length(word: int) -> int:
+    return int(_load().rdpo_command_length(word & 0xFFFFFFFF))
+
+
+def blender_divider_table() -> np.ndarray:
+    out = np.zeros(0x8000, dtype=np.uint8)
+    _load().rdpo_blender_divider_table(out.ctypes.data)
+    return out
+
+
+def gamma_table() -> np.ndarray:
+    out = np.zeros(256 + 256 * 64, dtype=np.uint8)
+    _load().rdpo_gamma_table(out.ctypes.data)
+    return out
+
+
+def perspective_table() -> np.ndarray:
+    out = np.zeros((64, 2), dtype=np.int16)
+    _load().rdpo_perspective_table(out.ctypes.data)
+    return out

@@ -1,0 +1,74 @@
+/*
+ * rdp_oracle.h - CPU restatement of the paraLLEl-RDP rendering semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the correctness oracle for the MI355X
+ * implementation under parallel-rdp_amd/.  Only the test-suite, the smoke
+ * check and the CPU-baseline leg of the benchmark may load it; the product
+ * path never calls into it.
+ *
+ * The oracle is an immediate-mode renderer: every draw command is rasterised
+ * and merged into RDRAM before the next command is decoded.  It follows the
+ * integer semantics of the reference's shaders and host setup code (citations
+ * at each function in rdp_oracle.c), evaluated in plain 32-bit two's
+ * complement arithmetic (build with -fwrapv).
+ *
+ * Memory layout matches the reference: RDRAM is an array of host-endian
+ * 32-bit words holding big-endian N64 data, so byte address a lives at
+ * host byte a^3 and halfword h at host halfword h^1.  Hidden RDRAM holds one
+ * byte (2 significant bits) per RDRAM halfword.  TMEM is 2048 halfwords with
+ * the same ^1 halfword swizzle.
+ */
+#ifndef RDP_ORACLE_H
+#define RDP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rdpo rdpo;
+
+/* rdram_size must be a power of two (4 or 8 MiB in practice).  Hidden RDRAM
+ * is rdram_size/2 bytes.  RDRAM starts zeroed, hidden RDRAM starts at 0x03
+ * per byte and TMEM at zero (reference: rdp_device.cpp:235-243). */
+rdpo *rdpo_create(uint32_t rdram_size);
+void rdpo_destroy(rdpo *o);
+
+uint8_t *rdpo_rdram(rdpo *o);
+uint8_t *rdpo_hidden_rdram(rdpo *o);
+uint16_t *rdpo_tmem(rdpo *o);
+uint32_t rdpo_rdram_size(const rdpo *o);
+
+/* Number of 32-bit words the command starting with this word occupies
+ * (2..44), or 0 for an opcode the RDP does not define. */
+unsigned rdpo_command_length(uint32_t first_word);
+
+/* Process a stream of RDP commands.  Returns the number of words consumed
+ * (stops early on a truncated trailing command). */
+size_t rdpo_process(rdpo *o, const uint32_t *words, size_t num_words);
+
+/* VI register file (reference: rdp_common.hpp VIRegister). */
+void rdpo_set_vi_register(rdpo *o, unsigned reg, uint32_t value);
+
+/* Scan out the current frame through the VI filters.  out must hold
+ * 640 x 576 RGBA8 words; width/height receive the output size, 0x0 for a
+ * blank or invalid frame.  Returns 0 on success. */
+int rdpo_scanout(rdpo *o, uint32_t *out, unsigned *width, unsigned *height);
+
+/* Instrumentation: number of (pixel, primitive) fragments that reached the
+ * depth/blend stage, and number of draw commands seen. */
+uint64_t rdpo_fragment_count(const rdpo *o);
+uint64_t rdpo_primitive_count(const rdpo *o);
+
+/* Generated lookup tables, exposed so tests can pin them. */
+void rdpo_blender_divider_table(uint8_t out[0x8000]);
+void rdpo_gamma_table(uint8_t out[256 + 256 * 64]);
+void rdpo_perspective_table(int16_t out[64][2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

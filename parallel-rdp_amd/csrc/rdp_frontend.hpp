@@ -1,0 +1,111 @@
+// Host front end: RDP command decode, state tracking and render-pass building.
+//
+// Mirrors the sequential half of the reference's CommandProcessor and
+// Renderer (parallel-rdp/rdp_device.cpp:268-959, rdp_renderer.cpp:679-1590,
+// 3051-3486): state commands update register state, draw commands are
+// recorded into the open render pass together with a snapshot of the
+// resolved draw state, and TMEM loads are planned into UploadInfo records.
+// Triangle decode and everything per pixel run on the GPU.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <functional>
+#include <vector>
+
+#include "rdp_types.hpp"
+
+namespace rdp_mi {
+
+// Validation errors that would hang a real RDP (rdp_renderer.hpp:75-95).
+enum ValidationError : uint32_t {
+	VALIDATION_FILL_4BPP = 0,
+	VALIDATION_FILL_DEPTH_TEST = 1,
+	VALIDATION_FILL_DEPTH_WRITE = 2,
+	VALIDATION_FILL_IMAGE_READ = 3,
+	VALIDATION_COPY_32BPP = 4,
+	VALIDATION_LOAD_TILE_4BPP = 5,
+	VALIDATION_MULTILINE_TLUT = 6,
+};
+
+struct HostPass {
+	std::vector<uint32_t> words;
+	std::vector<DrawState> states;
+	std::vector<PrimRecord> prims;
+	std::vector<UploadInfo> uploads;
+	uint32_t fb_addr = 0, fb_width = 0, fb_fmt = 0, depth_addr = 0, fb_height = 0;
+	uint32_t num_spans = 0, num_tilerows = 0;
+	bool empty() const { return prims.empty() && uploads.empty(); }
+};
+
+class Frontend {
+public:
+	explicit Frontend(uint32_t rdram_size);
+
+	// Number of words of the command that starts with this word (0: unknown).
+	static unsigned command_length(uint32_t first_word);
+
+	// Execute one complete command.
+	void command(const uint32_t *words);
+
+	// Close the open pass (if it holds any work).
+	void flush();
+
+	// Completed passes, in order; the caller takes ownership.
+	std::vector<HostPass> &completed() { return done; }
+
+	bool has_pending_work() const { return !done.empty() || !cur.empty(); }
+
+	void set_validation_callback(std::function<void(uint32_t, const char *)> cb) { validation = std::move(cb); }
+
+	uint64_t primitive_counter() const { return prim_counter; }
+
+	// Largest pass size the front end will build (tunable for tests).
+	uint32_t max_prims_per_pass = kMaxPrimsPerPass;
+	uint32_t max_spans_per_pass = 1u << 20;
+
+private:
+	uint32_t rdram_size;
+
+	// Register state (rdp_device.hpp:236-246, rdp_renderer.hpp:294-312).
+	uint32_t scissor[4] = {};
+	uint32_t ras_flags = 0, dither = 0;
+	uint8_t comb[2][8] = {};
+	uint32_t db_flags = 0;
+	uint8_t blend[2][4] = {};
+	uint32_t coverage_mode = 0, z_mode = 0;
+	struct { uint32_t addr, width, fmt, size; } teximg = {};
+	TileDesc tiles[8] = {};
+	uint32_t blend_color = 0, fog_color = 0, env_color = 0, prim_color = 0, fill_color = 0;
+	uint32_t min_level = 0, prim_lod_frac = 0;
+	int32_t prim_depth = 0;
+	uint32_t prim_dz = 0;
+	bool use_prim_depth = false;
+	int32_t convert[6] = {};
+	uint32_t key_width[3] = {}, key_center[3] = {}, key_scale[3] = {};
+	uint32_t fb_addr = 0, fb_width = 0, fb_fmt = 0, depth_addr = 0;
+	uint32_t quirks = 1;
+
+	uint64_t prim_counter = 0;
+	bool state_dirty = true;
+	bool color_write_pending = false, depth_write_pending = false;
+
+	HostPass cur;
+	std::vector<HostPass> done;
+	std::function<void(uint32_t, const char *)> validation;
+
+	void draw(unsigned op, const uint32_t *words, unsigned len);
+	void resolve_state(DrawState &d) const;
+	void load(uint32_t mode, const uint32_t *words);
+	void load_tile_iteration(uint32_t tile, uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi,
+	                         uint32_t tmem_offset);
+	bool load_needs_flush(uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi) const;
+	void validate_draw() const;
+	void report(uint32_t err, const char *msg) const
+	{
+		if (validation) validation(err, msg);
+	}
+};
+
+}  // namespace rdp_mi

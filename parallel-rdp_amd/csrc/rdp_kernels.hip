@@ -1,0 +1,1985 @@
+// gfx950 kernels for the RDP rendering hot path.
+//
+// Four stages per render pass (any number of passes, from any number of
+// streams, per launch; every block finds its pass through a work item):
+//   k_setup   one wave64 per primitive: decodes the raw command words with the
+//             pass draw state, writes PrimSetup, walks the edges for every
+//             covered scanline (SpanSetup) and reduces the per-8-line x extent
+//             into tile-row ranges.
+//   k_tmem    one lane per TMEM halfword: replays the pass's uploads in order
+//             and writes a TMEM snapshot after each one.
+//   k_bin     one wave64 per (tile row, group of 64 primitives): a ballot per
+//             tile column produces the ordered 64-bit primitive masks.
+//   k_raster  one wave64 per 8x8 screen tile: colour/depth/coverage of the 64
+//             pixels live in registers while every binned primitive is shaded
+//             (coverage, interpolation, texturing, combiner) and merged (Z test,
+//             blender, dither, coverage update) in submission order; RDRAM is
+//             read once and written back once, masked to dirty pixels.
+//
+// All arithmetic is the RDP's integer fixed point, evaluated in 32-bit two's
+// complement (compiled with -fwrapv).  Semantics follow the reference shaders
+// cited at each function.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rdp_types.hpp"
+
+namespace rdp_mi {
+
+#define DEV static __device__ __forceinline__
+#define MDEV __device__ __forceinline__
+
+// --------------------------------------------------------------------------
+// Integer helpers with GLSL semantics.
+// --------------------------------------------------------------------------
+DEV int32_t sext(int32_t v, int bits) { return (int32_t)((uint32_t)v << (32 - bits)) >> (32 - bits); }
+DEV int find_msb(int32_t v)
+{
+	uint32_t u = (uint32_t)(v < 0 ? ~v : v);
+	return u ? 31 - __clz((int)u) : -1;
+}
+DEV int find_lsb(uint32_t v) { return __ffs((int)v) - 1; }
+DEV int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+DEV int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+DEV int32_t iclamp(int32_t v, int32_t lo, int32_t hi) { return imin(imax(v, lo), hi); }
+
+DEV uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// --------------------------------------------------------------------------
+// RDRAM access: host-endian 32-bit words holding big-endian data.
+// --------------------------------------------------------------------------
+struct Mem {
+	uint8_t *rdram;
+	uint8_t *hidden;
+	uint32_t mask8;
+	MDEV uint32_t rd8(uint32_t addr) const { return rdram[(addr & mask8) ^ 3]; }
+	MDEV uint32_t rd16i(uint32_t index) const
+	{
+		return reinterpret_cast<const uint16_t *>(rdram)[(index & (mask8 >> 1)) ^ 1];
+	}
+	MDEV void wr16i(uint32_t index, uint32_t v) const
+	{
+		reinterpret_cast<uint16_t *>(rdram)[(index & (mask8 >> 1)) ^ 1] = (uint16_t)v;
+	}
+	MDEV uint32_t rd32i(uint32_t index) const { return reinterpret_cast<const uint32_t *>(rdram)[index & (mask8 >> 2)]; }
+	MDEV void wr32i(uint32_t index, uint32_t v) const { reinterpret_cast<uint32_t *>(rdram)[index & (mask8 >> 2)] = v; }
+	MDEV uint32_t rd16a(uint32_t addr) const
+	{
+		if ((addr & 1) == 0)
+			return rd16i(addr >> 1);
+		return (rd8(addr) << 8) | rd8(addr + 1);
+	}
+	MDEV uint32_t hid(uint32_t index) const { return hidden[index & (mask8 >> 1)]; }
+	MDEV void set_hid(uint32_t index, uint32_t v) const { hidden[index & (mask8 >> 1)] = (uint8_t)v; }
+};
+
+// ==========================================================================
+// Stage 1: primitive decode + span setup.
+// ==========================================================================
+
+// Renderer::build_derived_attributes dz part, rdp_renderer.cpp:717-740, 910-928.
+DEV int dz_compress_host(int dz)
+{
+	int val = 0;
+	if (dz & 0xff00) val |= 8;
+	if (dz & 0xf0f0) val |= 4;
+	if (dz & 0xcccc) val |= 2;
+	if (dz & 0xaaaa) val |= 1;
+	return val;
+}
+
+DEV int normalize_dzpix(int dz)
+{
+	if (dz >= 0x8000) return 0x8000;
+	if (dz == 0) return 1;
+	return 1 << (find_msb(dz) + 1);
+}
+
+// CommandProcessor decoders, rdp_device.cpp:268-344 and 713-821, plus
+// Renderer::fixup_triangle_setup (rdp_renderer.cpp:1402-1423) and the
+// primitive-depth override (rdp_renderer.cpp:1489-1497).
+DEV void decode_primitive(const uint32_t *w, uint32_t op, const DrawState &st, PrimSetup &s)
+{
+	const uint32_t raw = st.raw_ras_flags;
+	for (int i = 0; i < 4; i++) {
+		s.rgba[i] = s.drgba_dx[i] = s.drgba_de[i] = s.drgba_dy[i] = 0;
+		s.stzw[i] = s.dstzw_dx[i] = s.dstzw_de[i] = s.dstzw_dy[i] = 0;
+	}
+	s.flags = 0;
+	s.tile = 0;
+	if (op >= 0x08 && op <= 0x0f) {
+		uint32_t w0 = w[0], w1 = w[1];
+		bool flip = (w0 & 0x800000u) != 0;
+		bool sign_dxhdy = (w[5] & 0x80000000u) != 0;
+		if (flip) s.flags |= TRI_FLIP;
+		if (flip == sign_dxhdy) s.flags |= TRI_DO_OFFSET;
+		if (raw & RAS_COPY) s.flags |= TRI_SKIP_XFRAC;
+		if (st.quirks & 0x100) s.flags |= TRI_NATIVE_LOD;
+		s.tile = (w0 >> 16) & 63;
+		s.yl = sext((int32_t)w0, 14);
+		s.ym = sext((int32_t)(w1 >> 16), 14);
+		s.yh = sext((int32_t)w1, 14);
+		s.xl = sext((int32_t)w[2], 28) >> 1;
+		s.xh = sext((int32_t)w[4], 28) >> 1;
+		s.xm = sext((int32_t)w[6], 28) >> 1;
+		s.dxldy = sext((int32_t)(w[3] >> 2), 28) >> 1;
+		s.dxhdy = sext((int32_t)(w[5] >> 2), 28) >> 1;
+		s.dxmdy = sext((int32_t)(w[7] >> 2), 28) >> 1;
+		const uint32_t *p = w + 8;
+		if (op & 4) {
+			for (int i = 0; i < 2; i++) {
+				s.rgba[2 * i + 0] = (int32_t)((p[i] & 0xffff0000u) | ((p[4 + i] >> 16) & 0xffff));
+				s.rgba[2 * i + 1] = (int32_t)((p[i] << 16) | (p[4 + i] & 0xffff));
+				s.drgba_dx[2 * i + 0] = (int32_t)((p[2 + i] & 0xffff0000u) | ((p[6 + i] >> 16) & 0xffff));
+				s.drgba_dx[2 * i + 1] = (int32_t)((p[2 + i] << 16) | (p[6 + i] & 0xffff));
+				s.drgba_de[2 * i + 0] = (int32_t)((p[8 + i] & 0xffff0000u) | ((p[12 + i] >> 16) & 0xffff));
+				s.drgba_de[2 * i + 1] = (int32_t)((p[8 + i] << 16) | (p[12 + i] & 0xffff));
+				s.drgba_dy[2 * i + 0] = (int32_t)((p[10 + i] & 0xffff0000u) | ((p[14 + i] >> 16) & 0xffff));
+				s.drgba_dy[2 * i + 1] = (int32_t)((p[10 + i] << 16) | (p[14 + i] & 0xffff));
+			}
+			p += 16;
+		}
+		if (op & 2) {
+			s.stzw[0] = (int32_t)((p[0] & 0xffff0000u) | ((p[4] >> 16) & 0xffff));
+			s.stzw[1] = (int32_t)(((p[0] << 16) & 0xffff0000u) | (p[4] & 0xffff));
+			s.stzw[3] = (int32_t)((p[1] & 0xffff0000u) | ((p[5] >> 16) & 0xffff));
+			s.dstzw_dx[0] = (int32_t)((p[2] & 0xffff0000u) | ((p[6] >> 16) & 0xffff));
+			s.dstzw_dx[1] = (int32_t)(((p[2] << 16) & 0xffff0000u) | (p[6] & 0xffff));
+			s.dstzw_dx[3] = (int32_t)((p[3] & 0xffff0000u) | ((p[7] >> 16) & 0xffff));
+			s.dstzw_de[0] = (int32_t)((p[8] & 0xffff0000u) | ((p[12] >> 16) & 0xffff));
+			s.dstzw_de[1] = (int32_t)(((p[8] << 16) & 0xffff0000u) | (p[12] & 0xffff));
+			s.dstzw_de[3] = (int32_t)((p[9] & 0xffff0000u) | ((p[13] >> 16) & 0xffff));
+			s.dstzw_dy[0] = (int32_t)((p[10] & 0xffff0000u) | ((p[14] >> 16) & 0xffff));
+			s.dstzw_dy[1] = (int32_t)(((p[10] << 16) & 0xffff0000u) | (p[14] & 0xffff));
+			s.dstzw_dy[3] = (int32_t)((p[11] & 0xffff0000u) | ((p[15] >> 16) & 0xffff));
+			p += 16;
+		}
+		if (op & 1) {
+			s.stzw[2] = (int32_t)p[0];
+			s.dstzw_dx[2] = (int32_t)p[1];
+			s.dstzw_de[2] = (int32_t)p[2];
+			s.dstzw_dy[2] = (int32_t)p[3];
+		}
+	} else {
+		uint32_t xl = (w[0] >> 12) & 0xfff, yl = w[0] & 0xfff;
+		uint32_t xh = (w[1] >> 12) & 0xfff, yh = w[1] & 0xfff;
+		if (raw & (RAS_COPY | RAS_FILL))
+			yl |= 3;
+		s.xh = (int32_t)(xh << 13);
+		s.xl = (int32_t)(xl << 13);
+		s.xm = (int32_t)(xl << 13);
+		s.ym = (int32_t)yl;
+		s.yl = (int32_t)yl;
+		s.yh = (int32_t)yh;
+		s.dxhdy = s.dxmdy = s.dxldy = 0;
+		if (op == 0x36) {
+			s.flags = TRI_FLIP | TRI_DISABLE_UPSCALING;
+		} else {
+			int32_t ss = (int32_t)((w[2] >> 16) & 0xffff), tt = (int32_t)(w[2] & 0xffff);
+			int32_t dsdx = sext((int32_t)((w[3] >> 16) & 0xffff), 16);
+			int32_t dtdy = sext((int32_t)(w[3] & 0xffff), 16);
+			s.flags = TRI_FLIP;
+			if (op == 0x25 || (st.quirks & 1)) s.flags |= TRI_DISABLE_UPSCALING;
+			if (st.quirks & 0x100) s.flags |= TRI_NATIVE_LOD;
+			s.tile = (w[1] >> 24) & 7;
+			s.stzw[0] = ss << 16;
+			s.stzw[1] = tt << 16;
+			if (op == 0x24) {
+				s.dstzw_dx[0] = dsdx << 11;
+				s.dstzw_de[1] = dtdy << 11;
+				s.dstzw_dy[1] = dtdy << 11;
+			} else {
+				s.dstzw_dx[1] = dtdy << 11;
+				s.dstzw_de[0] = dsdx << 11;
+				s.dstzw_dy[0] = dsdx << 11;
+			}
+			if (raw & RAS_COPY) s.flags |= TRI_SKIP_XFRAC;
+		}
+	}
+
+	// fixup_triangle_setup
+	int start_y = s.yh & ~3;
+	if (s.ym < start_y) s.ym = 0x7fff;
+	if (raw & RAS_INTERLACE_FIELD) {
+		s.flags |= TRI_INTERLACE_FIELD;
+		if (raw & RAS_INTERLACE_KEEP_ODD) s.flags |= TRI_INTERLACE_KEEP_ODD;
+	}
+	if (raw & (RAS_COPY | RAS_FILL)) s.flags |= TRI_FILL_COPY_RASTER;
+
+	// dz from the decoded gradients (before any primitive-depth override)
+	if (st.use_prim_depth) {
+		s.dz = (int32_t)st.prim_dz;
+		s.dz_compressed = dz_compress_host((int)st.prim_dz);
+		s.stzw[2] = st.prim_depth;
+		s.dstzw_dx[2] = s.dstzw_de[2] = s.dstzw_dy[2] = 0;
+	} else {
+		int dzdx = s.dstzw_dx[2] >> 16;
+		int dzdy = s.dstzw_dy[2] >> 16;
+		int dzpix = (dzdx < 0 ? (~dzdx & 0x7fff) : dzdx) + (dzdy < 0 ? (~dzdy & 0x7fff) : dzdy);
+		dzpix = normalize_dzpix(dzpix);
+		s.dz = dzpix;
+		s.dz_compressed = dz_compress_host(dzpix);
+	}
+}
+
+DEV int32_t quantize_x(int32_t x) { return (x >> 12) | ((x & 0xfff) != 0); }
+
+// span_setup.comp:87-227 for one scanline (no upscaling).
+DEV void compute_span(const PrimSetup &s, const uint32_t sc[4], int y, SpanSetup &out)
+{
+	const bool flip = (s.flags & TRI_FLIP) != 0;
+	{
+		const bool do_offset = (s.flags & TRI_DO_OFFSET) != 0;
+		const bool skip_xfrac = (s.flags & TRI_SKIP_XFRAC) != 0;
+		int dy = y - (s.yh >> 2);
+		int32_t xh = s.xh + dy * (s.dxhdy << 2);
+		int32_t diff_c[4] = { 0, 0, 0, 0 }, diff_t[4] = { 0, 0, 0, 0 };
+		if (do_offset) {
+			xh += 3 * s.dxhdy;
+#pragma unroll
+			for (int i = 0; i < 4; i++) {
+				int32_t deh = s.drgba_de[i] & ~0x1ff, dyh = s.drgba_dy[i] & ~0x1ff;
+				diff_c[i] = deh - (deh >> 2) - dyh + (dyh >> 2);
+				deh = s.dstzw_de[i] & ~0x1ff;
+				dyh = s.dstzw_dy[i] & ~0x1ff;
+				diff_t[i] = deh - (deh >> 2) - dyh + (dyh >> 2);
+			}
+		}
+		out.base_x = xh >> 15;
+		int32_t xfrac = skip_xfrac ? 0 : ((xh >> 7) & 0xff);
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			int32_t v = s.rgba[i] + dy * s.drgba_de[i];
+			out.rgba[i] = ((v & ~0x1ff) + diff_c[i] - xfrac * ((s.drgba_dx[i] >> 8) & ~1)) & ~0x3ff;
+			v = s.stzw[i] + dy * s.dstzw_de[i];
+			out.stzw[i] = ((v & ~0x1ff) + diff_t[i] - xfrac * ((s.dstzw_dx[i] >> 8) & ~1)) & ~0x3ff;
+		}
+	}
+
+	const int yh_base = s.yh & ~3;
+	const int ylo = imax(s.yh, (int)sc[1]);
+	const int yhi = imin(s.yl, (int)sc[3]);
+	const int32_t lo_sc = (int32_t)(sc[0] << 1), hi_sc = (int32_t)(sc[2] << 1);
+	bool all_over = true, all_under = true, all_invalid = true;
+	int32_t start_x = 0x7fffffff, end_x = (int32_t)0x80000000;
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		int ys = y * 4 + i;
+		bool clip_y = ys < ylo || ys >= yhi;
+		int32_t xh = s.xh + (ys - yh_base) * s.dxhdy;
+		int32_t xm = s.xm + (ys - yh_base) * s.dxmdy;
+		int32_t xl = s.xl + (ys - s.ym) * s.dxldy;
+		if (ys < s.ym) xl = xm;
+		xl = sext(xl, 27);
+		xh = sext(xh, 27);
+		int32_t xh_s = quantize_x(xh), xl_s = quantize_x(xl);
+		int32_t l = flip ? xh_s : xl_s;
+		int32_t r = flip ? xl_s : xh_s;
+		bool invalid = (l >> 1) > (r >> 1);
+		if (!(imin(l, r) >= hi_sc)) all_over = false;
+		if (!(imax(l, r) < lo_sc)) all_under = false;
+		l = imin(imax(l, lo_sc), hi_sc);
+		r = imin(imax(r, lo_sc), hi_sc);
+		invalid = invalid || clip_y;
+		if (invalid) {
+			l = 0xffff;
+			r = 0;
+		} else
+			all_invalid = false;
+		out.xleft[i] = (uint16_t)l;
+		out.xright[i] = (uint16_t)r;
+		start_x = imin(start_x, l);
+		end_x = imax(end_x, r);
+	}
+	out.start_x = start_x >> 3;
+	out.end_x = end_x >> 3;
+	bool valid = !all_invalid && !all_over && !all_under;
+	if ((s.flags & TRI_INTERLACE_FIELD) && ((y & 1) != (int)((s.flags & TRI_INTERLACE_KEEP_ODD) != 0)))
+		valid = false;
+	out.valid = valid ? 1 : 0;
+	out.lodlength = (int16_t)(flip ? (out.end_x - out.base_x) : (out.base_x - out.start_x));
+}
+
+__global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 12];
+	const uint32_t prim = item & 0xfff;
+	const PrimRecord rec = P.prims[prim];
+	const DrawState &st = P.states[rec.state_index];
+	const int lane = threadIdx.x;
+
+	PrimSetup s;
+	decode_primitive(P.words + rec.word_offset, rec.opcode, st, s);
+	s.state_index = rec.state_index;
+	s.tmem_instance = rec.tmem_instance;
+	s.prim_index = rec.prim_index;
+	s.ylo = rec.ylo;
+	s.yhi = rec.yhi;
+	s.span_offset = rec.span_offset;
+	s.pad = 0;
+	if (lane == 0)
+		P.setups[prim] = s;
+
+	const int ylo = rec.ylo, yhi = rec.yhi;
+	const int row0 = ylo >> 3, row1 = yhi >> 3;
+	const int fbw = (int)P.fb_width;
+	for (int yb = ylo & ~7; yb <= yhi + 1; yb += 64) {
+		const int y = yb + lane;
+		int tlo = 0x7fffffff, thi = -1;
+		if (y >= ylo && y <= yhi + 1) {
+			SpanSetup sp;
+			compute_span(s, st.scissor, y, sp);
+			P.spans[rec.span_offset + (y - ylo)] = sp;
+			if (y <= yhi && sp.valid) {
+				int lo = imax(sp.start_x, 0), hi = imin(sp.end_x, fbw - 1);
+				if (lo <= hi) {
+					tlo = lo >> 3;
+					thi = hi >> 3;
+				}
+			}
+		}
+		// Reduce over the 8 lanes of one tile row.
+#pragma unroll
+		for (int m = 1; m < 8; m <<= 1) {
+			tlo = imin(tlo, __shfl_xor(tlo, m, 64));
+			thi = imax(thi, __shfl_xor(thi, m, 64));
+		}
+		const int row = (yb >> 3) + (lane >> 3);
+		if ((lane & 7) == 0 && row >= row0 && row <= row1) {
+			uint32_t v = thi >= tlo ? ((uint32_t)tlo | ((uint32_t)thi << 16)) : 0xffffu;
+			P.tilerows[rec.tilerow_offset + (row - row0)] = v;
+		}
+	}
+}
+
+// ==========================================================================
+// Stage 2: TMEM snapshots, tmem_update.comp:108-577 (one lane per halfword).
+// ==========================================================================
+DEV int div_floor(int num, int den)
+{
+	if (den <= 0) return 0;
+	if (num >= 0) return num / den;
+	return -((-num + den - 1) / den);
+}
+
+DEV bool tmem_update_32(const Mem &m, const UploadInfo &u, int idx, bool upper, bool yuv, uint32_t &value)
+{
+	const int off16 = (u.tmem_offset & 0x7ff) >> 1;
+	const int stride = u.tmem_stride_words;
+	int pixel_offset = (idx - off16) & 0x3ff;
+	int ux, uy, ux_xor = 0;
+	if (u.mode == UP_BLOCK) {
+		int word_offset = pixel_offset >> 1;
+		if (stride == 0) {
+			int c0 = word_offset & ~1, c1 = c0 + 1;
+			int t0 = (c0 * u.dxt) >> 16, t1 = (c1 * u.dxt) >> 16;
+			if (t0 != t1) {
+				int w0 = c0 ^ (t0 & 1), w1 = c1 ^ (t1 & 1);
+				if (w1 == word_offset) ux_xor = (t1 & 1) << 1;
+				else if (w0 == word_offset) ux_xor = (t0 & 1) << 1;
+				else return false;
+			} else
+				ux_xor ^= (t0 & 1) << 1;
+		} else {
+			int min_t = div_floor(word_offset & ~1, u.max_line_stride);
+			int max_t = div_floor(word_offset | 1, u.min_line_stride);
+			int max_cand = (word_offset | 1) - stride * min_t;
+			int min_cand = (word_offset & ~1) - stride * max_t;
+			min_t = imax(min_t, (min_cand * u.dxt) >> 16);
+			max_t = imin(max_t, (max_cand * u.dxt) >> 16);
+			bool found = false;
+			for (int t = max_t; t >= min_t; t--) {
+				int cs0 = (word_offset & ~1) - stride * t, cs1 = (word_offset | 1) - stride * t;
+				int ct0 = (cs0 * u.dxt) >> 16, ct1 = (cs1 * u.dxt) >> 16;
+				if (((cs1 + ct1 * stride) ^ (ct1 & 1)) == word_offset) {
+					found = true;
+					pixel_offset = (cs1 << 1) + (pixel_offset & 1);
+					break;
+				} else if (((cs0 + ct0 * stride) ^ (ct0 & 1)) == word_offset) {
+					found = true;
+					pixel_offset = (cs0 << 1) + (pixel_offset & 1);
+					break;
+				}
+			}
+			if (!found) return false;
+		}
+		ux = pixel_offset;
+		uy = 0;
+	} else if (stride == 0) {
+		ux = pixel_offset;
+		uy = u.height - 1;
+	} else {
+		uy = div_floor(pixel_offset, stride);
+		ux = pixel_offset - uy * stride;
+		if (uy >= u.height) {
+			ux += stride * (uy - u.height + 1);
+			uy = u.height - 1;
+		}
+	}
+	int last_line_x = ux ^ ((uy & 1) << 1);
+	if (last_line_x >= u.width && uy > 0) {
+		uy--;
+		ux += stride;
+	}
+	int iter_off = 0;
+	ux ^= ((uy & 1) << 1) | ux_xor;
+	if (u.vram_size == 3 || yuv)
+		iter_off = 4 * (ux & ~1);
+	else if (u.vram_size == 2) {
+		if (ux & 2) {
+			if (uy > 0) {
+				uy--;
+				ux += stride;
+				ux ^= 2;
+			} else
+				return false;
+		}
+		iter_off = 2 * (ux & ~1);
+	} else if (u.vram_size == 1) {
+		for (int i = 0; i < 4 && uy > 0 && (ux & 6) != 0; i++) {
+			uy--;
+			ux += stride;
+			ux ^= 2;
+		}
+		if (ux & 6) return false;
+		iter_off = ux & ~1;
+	}
+	if (ux >= u.width) return false;
+	int line_addr = u.vram_addr + ((uy * u.vram_width) << (u.vram_size - 1));
+	uint32_t addr = (uint32_t)(line_addr + iter_off + 4 * (ux & 1));
+	uint32_t word;
+	if ((addr & 3) == 0)
+		word = m.rd32i(addr >> 2);
+	else
+		word = (m.rd8(addr) << 24) | (m.rd8(addr + 1) << 16) | (m.rd8(addr + 2) << 8) | m.rd8(addr + 3);
+	if (yuv) {
+		if (upper)
+			word = (((word >> 16) & 0xff) << 8) | (word & 0xff);
+		else
+			word = (((word >> 24) & 0xff) << 8) | ((word >> 8) & 0xff);
+	} else {
+		word >>= 16 - 16 * (uint32_t)upper;
+		word &= 0xffff;
+	}
+	value = word;
+	return true;
+}
+
+DEV bool tmem_update_16(const Mem &m, const UploadInfo &u, int idx, uint32_t &value)
+{
+	const int off16 = (u.tmem_offset & 0xfff) >> 1;
+	const int stride = u.tmem_stride_words;
+	int pixel_offset = (idx - off16) & 0x7ff;
+	int ux, uy, ux_xor = 0;
+	if (u.mode == UP_BLOCK) {
+		int word_offset = pixel_offset >> 2;
+		if (stride == 0)
+			ux_xor = (((word_offset * u.dxt) >> 16) & 1) << 1;
+		else {
+			int min_t = div_floor(word_offset, u.max_line_stride);
+			int max_t = div_floor(word_offset, u.min_line_stride);
+			int max_cand = word_offset - stride * min_t;
+			int min_cand = word_offset - stride * max_t;
+			min_t = imax(min_t, (min_cand * u.dxt) >> 16);
+			max_t = imin(max_t, (max_cand * u.dxt) >> 16);
+			bool found = false;
+			for (int t = max_t; t >= min_t; t--) {
+				int cand = word_offset - stride * t;
+				int ct = (cand * u.dxt) >> 16;
+				if (cand + ct * stride == word_offset) {
+					found = true;
+					ux_xor = (ct & 1) << 1;
+					pixel_offset = (cand << 2) + (pixel_offset & 3);
+				}
+			}
+			if (!found) return false;
+		}
+		ux = pixel_offset;
+		uy = 0;
+	} else if (stride == 0) {
+		ux = pixel_offset;
+		uy = u.height - 1;
+	} else {
+		uy = div_floor(pixel_offset, stride);
+		ux = pixel_offset - uy * stride;
+		if (uy >= u.height) {
+			ux += stride * (uy - u.height + 1);
+			uy = u.height - 1;
+		}
+	}
+	int iter_off = 0;
+	if (u.tmem_size != u.vram_size) {
+		if (u.vram_size - u.tmem_size == 1) {
+			iter_off = (ux & ~3) * 4;
+			if ((ux & ~3) + 2 < (u.vram_effective_width >> (3 - u.vram_size)))
+				iter_off += 8;
+		} else if (u.tmem_size == 2 && u.vram_size == 1) {
+			if (ux & 4) {
+				if ((stride & 4) != 0 && uy > 0) {
+					uy--;
+					ux += stride;
+				} else
+					return false;
+			}
+			iter_off = ux & ~3;
+		}
+	} else
+		iter_off = (ux & ~3) * 2;
+	if (ux >= u.width) return false;
+	int line_addr = u.vram_addr + ((uy * u.vram_width) << (u.vram_size - 1));
+	ux ^= ((uy & 1) << 1) | ux_xor;
+	value = m.rd16a((uint32_t)(line_addr + iter_off + 2 * (ux & 3)));
+	return true;
+}
+
+DEV bool tmem_update_lut(const Mem &m, const UploadInfo &u, int idx, uint32_t &value)
+{
+	const int off16 = (u.tmem_offset & 0xfff) >> 1;
+	const int p = (idx - off16) & 0x7ff;
+	const int diff = u.vram_size - u.tmem_size;
+	int splat;
+	if (diff == 2) {
+		splat = (p >> 2) << (u.vram_size - 2);
+		if (splat >= u.vram_effective_width) return false;
+	} else if (diff == 1) {
+		if (p & 4) return false;
+		splat = (p & ~7) >> (u.tmem_size + (u.vram_size == 2 ? 2 : 0));
+		if (splat >= u.vram_effective_width) return false;
+	} else if (diff == 0) {
+		if (p & 0xc) return false;
+		splat = (p & ~3) >> (u.tmem_size + (u.vram_size == 2 ? 2 : 0));
+		if (splat >= u.vram_effective_width) return false;
+	} else if (diff == -1) {
+		if (p & 0x1c) return false;
+		splat = (p >> u.tmem_size) & ~7;
+		if (splat >= u.vram_effective_width) return false;
+	} else {
+		int span_pixel = (p >> 2) * 4;
+		if (span_pixel + 2 < u.vram_effective_width) span_pixel += 2;
+		if (span_pixel >= u.vram_effective_width) return false;
+		splat = span_pixel;
+	}
+	uint32_t addr = (uint32_t)(u.vram_addr + (splat << (u.vram_size - 1)));
+	addr += 2 * (addr & 1) * (uint32_t)(p & 3);
+	value = m.rd16a(addr);
+	return true;
+}
+
+__global__ __launch_bounds__(256) void k_tmem(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 12];
+	const int g = (int)((item & 0xfff) * 256 + threadIdx.x);  // 0..2047
+	const Mem m{ P.rdram, P.hidden, P.rdram_mask };
+	uint32_t cur = P.tmem[g];
+	const int idx = g ^ 1;
+	const bool upper = idx >= 0x400;
+	P.tmem_instances[g] = (uint16_t)cur;
+	const int n = (int)P.num_uploads;
+	for (int i = 0; i < n; i++) {
+		const UploadInfo u = P.uploads[i];
+		uint32_t v;
+		bool written;
+		if (u.mode == UP_TLUT)
+			written = tmem_update_lut(m, u, idx, v);
+		else {
+			bool yuv = u.tmem_fmt == TF_YUV;
+			if (u.tmem_size == 3 || yuv)
+				written = tmem_update_32(m, u, idx & 0x3ff, upper, yuv, v);
+			else
+				written = tmem_update_16(m, u, idx, v);
+		}
+		if (written) cur = v;
+		P.tmem_instances[(i + 1) * 2048 + g] = (uint16_t)cur;
+	}
+	P.tmem[g] = (uint16_t)cur;
+}
+
+// ==========================================================================
+// Stage 3: binning with wave64 ballots.
+// ==========================================================================
+__global__ __launch_bounds__(64) void k_bin(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 13];
+	const uint32_t ty = (item >> 6) & 0x7f, group = item & 0x3f;
+	const int lane = threadIdx.x;
+	const uint32_t prim = group * 64 + lane;
+	uint32_t lo = 0xffff, hi = 0;
+	if (prim < P.num_prims) {
+		const PrimRecord &r = P.prims[prim];
+		if (r.yhi >= r.ylo) {
+			int row0 = r.ylo >> 3, row1 = r.yhi >> 3;
+			if ((int)ty >= row0 && (int)ty <= row1) {
+				uint32_t v = P.tilerows[r.tilerow_offset + (ty - row0)];
+				lo = v & 0xffff;
+				hi = v >> 16;
+			}
+		}
+	}
+	const uint32_t tiles_x = P.tiles_x, G = P.num_groups;
+	uint64_t *out = P.bin_fine + (size_t)ty * tiles_x * G + group;
+	for (uint32_t tx = 0; tx < tiles_x; tx++) {
+		uint64_t mask = __ballot(lo <= tx && tx <= hi);
+		if (lane == 0)
+			out[(size_t)tx * G] = mask;
+	}
+}
+
+// ==========================================================================
+// Stage 4: per-tile shading and depth/blend.
+// ==========================================================================
+
+// coverage.h:31-44.  Sample layout per sub-scanline: 0,4 / 2,6 / 0,4 / 2,6.
+DEV int compute_coverage(const SpanSetup &sp, int x)
+{
+	int cov = 0;
+	const int xs = x << 3;
+#pragma unroll
+	for (int k = 0; k < 8; k++) {
+		const int line = k >> 1;
+		const int off = ((k & 1) ? 4 : 0) + ((line & 1) ? 2 : 0);
+		const int xv = xs + off;
+		if (xv >= (int)sp.xleft[line] && xv < (int)sp.xright[line])
+			cov |= 1 << k;
+	}
+	return cov;
+}
+
+DEV int32_t clamp_9bit(int32_t c) { return iclamp(sext(c - 0x80, 9) + 0x80, 0, 0xff); }
+
+DEV int32_t clamp_z(int32_t z)
+{
+	z = sext(z - (1 << 17), 19) + (1 << 17);
+	return iclamp(z, 0, 0x3ffff);
+}
+
+// perspective.h:46-112; the table is generated from r_i = round(2^20 / (64+i)).
+DEV int32_t persp_base(int i) { return ((1 << 20) + (64 + i) / 2) / (64 + i); }
+
+DEV void perspective_divide(int32_t s, int32_t t, int32_t w, int32_t &os, int32_t &ot, bool &overflow)
+{
+	bool w_carry = w <= 0;
+	w &= 0x7fff;
+	int shift = imin(14 - find_msb(w), 14);
+	int32_t normout = (w << shift) & 0x3fff;
+	int32_t wnorm = normout & 0xff;
+	int i = normout >> 8;
+	int32_t base = persp_base(i);
+	int32_t slope = -4 * (base - persp_base(i + 1));
+	int32_t rcp = ((slope * wnorm) >> 10) + base;
+	int32_t prod[2] = { s * rcp, t * rcp };
+	int32_t temp_mask = ((1 << 30) - 1) & -((1 << 29) >> shift);
+	int32_t out[2];
+#pragma unroll
+	for (int k = 0; k < 2; k++) {
+		int32_t oob = prod[k] & temp_mask;
+		if (shift != 14) {
+			prod[k] >>= (13 - shift);
+			out[k] = prod[k];
+		} else
+			out[k] = prod[k] << 1;
+		if (oob != temp_mask && oob != 0) {
+			out[k] = (prod[k] & (1 << 29)) == 0 ? 0x7fff : -0x8000;
+			overflow = true;
+		}
+	}
+	if (w_carry) {
+		out[0] = out[1] = 0x7fff;
+		overflow = true;
+	}
+	os = iclamp(out[0], -0x10000, 0xffff);
+	ot = iclamp(out[1], -0x10000, 0xffff);
+}
+
+DEV void stw_to_st(int32_t s, int32_t t, int32_t w, bool persp, int32_t &os, int32_t &ot, bool &ovf)
+{
+	s >>= 16;
+	t >>= 16;
+	w >>= 16;
+	if (persp)
+		perspective_divide(s, t, w, os, ot, ovf);
+	else {
+		os = s;
+		ot = t;
+	}
+}
+
+// noise.h:29-37
+DEV uint32_t reseed_noise(uint32_t x, uint32_t y, uint32_t prim)
+{
+	const uint32_t P = 1103515245u;
+	uint32_t s0 = x, s1 = y, s2 = prim;
+#pragma unroll
+	for (int r = 0; r < 3; r++) {
+		uint32_t n0 = ((s0 >> 8) ^ s1) * P;
+		uint32_t n1 = ((s1 >> 8) ^ s2) * P;
+		uint32_t n2 = ((s2 >> 8) ^ s0) * P;
+		s0 = n0;
+		s1 = n1;
+		s2 = n2;
+	}
+	return (s0 >> 16) & 0xffff;
+}
+
+// ---- texture sampling: texture.h -----------------------------------------
+struct Tex {
+	const uint16_t *tmem;
+	MDEV uint32_t t16(uint32_t i) const { return tmem[i & 0x7ff]; }
+	MDEV uint32_t t8(uint32_t i) const { return reinterpret_cast<const uint8_t *>(tmem)[i & 0xfff]; }
+};
+
+DEV int mask_coord(int v, uint32_t mask_bits, bool mirror)
+{
+	if (mask_bits) {
+		int mask = 1 << mask_bits;
+		if (mirror) v ^= imax((v & mask) - 1, 0);
+		v &= mask - 1;
+	}
+	return v;
+}
+
+DEV void rgba16_to(uint32_t w, int32_t o[4])
+{
+#pragma unroll
+	for (int i = 0; i < 3; i++) {
+		uint32_t c = (w >> (11 - 5 * i)) & 31;
+		o[i] = (int32_t)((c << 3) | (c >> 2));
+	}
+	o[3] = (w & 1) ? 0xff : 0;
+}
+
+DEV void ia16_to(uint32_t w, int32_t o[4])
+{
+	o[0] = o[1] = o[2] = (int32_t)(w >> 8);
+	o[3] = (int32_t)(w & 0xff);
+}
+
+DEV uint32_t tidx8(const TileDesc &t, uint32_t v, uint32_t wrap, uint32_t add)
+{
+	uint32_t b = (t.offset + t.stride * v + add) & wrap;
+	return (b ^ ((v & 1) << 2)) ^ 3;
+}
+
+DEV uint32_t tidx16(const TileDesc &t, uint32_t s, uint32_t v, uint32_t wrap)
+{
+	uint32_t b = (t.offset + t.stride * v + s * 2) & wrap;
+	return ((b >> 1) ^ ((v & 1) << 1)) ^ 1;
+}
+
+// sample_texel_* of texture.h:90-315 for one texel.
+DEV void fetch_texel(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, bool tlut, bool tlut_type,
+                     uint32_t lut_offset, uint32_t addr_xor, int32_t o[4])
+{
+	const uint32_t size = t.size;
+	if (tlut) {
+		uint32_t lut_entry;
+		if (size == 0) {
+			uint32_t w = (tx.t8(tidx8(t, v, 0x7ff, s >> 1)) >> ((~s & 1) * 4)) & 0xf;
+			w |= (uint32_t)t.palette << 4;
+			lut_entry = (w << 2) + lut_offset;
+		} else if (size == 1) {
+			lut_entry = (tx.t8(tidx8(t, v, 0x7ff, s)) << 2) + lut_offset;
+		} else {
+			lut_entry = ((tx.t16(tidx16(t, s, v, 0x7ff)) >> 6) & ~3u) + lut_offset;
+		}
+		uint32_t w = tx.t16(0x400 | (lut_entry ^ addr_xor));
+		if (tlut_type) ia16_to(w, o);
+		else rgba16_to(w, o);
+		return;
+	}
+	uint32_t fmt = t.fmt;
+	int kind;  // 0: 4-bit replicate, 1: 8-bit, 2: 16-bit 2x8 (ci32), 3: rgba16, 4: rgba32, 5: ia4, 6: ia8, 7: ia16, 8: ci4
+	switch (fmt) {
+	case TF_RGBA: kind = size == 0 ? 0 : size == 1 ? 1 : size == 2 ? 3 : 4; break;
+	case TF_CI: kind = size == 0 ? 8 : size == 1 ? 1 : 2; break;
+	case TF_IA: kind = size == 0 ? 5 : size == 1 ? 6 : size == 2 ? 7 : 2; break;
+	case TF_I: kind = size == 0 ? 0 : size == 1 ? 1 : 2; break;
+	default: kind = -1; break;
+	}
+	switch (kind) {
+	case 0: {
+		uint32_t w = (tx.t8(tidx8(t, v, 0xfff, s >> 1)) >> ((~s & 1) * 4)) & 0xf;
+		w |= w << 4;
+		o[0] = o[1] = o[2] = o[3] = (int32_t)w;
+		break;
+	}
+	case 1: {
+		uint32_t w = tx.t8(tidx8(t, v, 0xfff, s));
+		o[0] = o[1] = o[2] = o[3] = (int32_t)w;
+		break;
+	}
+	case 2: {
+		uint32_t w = tx.t16(tidx16(t, s, v, 0xfff));
+		o[0] = o[2] = (int32_t)(w >> 8);
+		o[1] = o[3] = (int32_t)(w & 0xff);
+		break;
+	}
+	case 3: rgba16_to(tx.t16(tidx16(t, s, v, 0xfff)), o); break;
+	case 4: {
+		uint32_t i = tidx16(t, s, v, 0x7ff);
+		uint32_t lo = tx.t16(i), hi = tx.t16(i | 0x400);
+		o[0] = (int32_t)(lo >> 8);
+		o[1] = (int32_t)(lo & 0xff);
+		o[2] = (int32_t)(hi >> 8);
+		o[3] = (int32_t)(hi & 0xff);
+		break;
+	}
+	case 5: {
+		uint32_t w = (tx.t8(tidx8(t, v, 0xfff, s >> 1)) >> ((~s & 1) * 4)) & 0xf;
+		uint32_t in = w & 0xe;
+		in = (in << 4) | (in << 1) | (in >> 2);
+		o[0] = o[1] = o[2] = (int32_t)in;
+		o[3] = (w & 1) ? 0xff : 0;
+		break;
+	}
+	case 6: {
+		uint32_t w = tx.t8(tidx8(t, v, 0xfff, s));
+		uint32_t in = w >> 4, al = w & 0xf;
+		o[0] = o[1] = o[2] = (int32_t)(in | (in << 4));
+		o[3] = (int32_t)(al | (al << 4));
+		break;
+	}
+	case 7: ia16_to(tx.t16(tidx16(t, s, v, 0xfff)), o); break;
+	case 8: {
+		uint32_t w = (tx.t8(tidx8(t, v, 0xfff, s >> 1)) >> ((~s & 1) * 4)) & 0xf;
+		w |= (uint32_t)t.palette << 4;
+		o[0] = o[1] = o[2] = o[3] = (int32_t)w;
+		break;
+	}
+	default: o[0] = o[1] = o[2] = o[3] = 0; break;
+	}
+}
+
+DEV void fetch_yuv16(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, uint32_t cx, int32_t o[4])
+{
+	uint32_t bo = t.offset + t.stride * v;
+	uint32_t bl = (bo + s) & 0x7ff, bc = (bo + cx * 2) & 0x7ff;
+	uint32_t il = (bl ^ ((v & 1) << 2)) ^ 3;
+	uint32_t ic = ((bc >> 1) ^ ((v & 1) << 1)) ^ 1;
+	uint32_t luma = tx.t8(il | 0x800);
+	uint32_t chroma = tx.t16(ic);
+	o[0] = (int32_t)((chroma >> 8) & 0xff) - 0x80;
+	o[1] = (int32_t)(chroma & 0xff) - 0x80;
+	o[2] = o[3] = (int32_t)luma;
+}
+
+DEV int clamp_and_shift_coord(bool clamp_bit, int c, int lo, int hi, int shift)
+{
+	c = iclamp(c, -0x8000, 0x7fff);
+	if (shift < 11) c >>= shift;
+	else c = (c << (32 - shift)) >> 16;
+	if (clamp_bit) {
+		if ((c >> 3) >= hi) c = (((hi >> 2) - (lo >> 2)) & 0x3ff) << 5;
+		else c = imax(c - (lo << 3), 0);
+	} else
+		c -= lo << 3;
+	return c;
+}
+
+DEV int shift_coord(int c, int lo, int shift)
+{
+	c = iclamp(c, -0x8000, 0x7fff);
+	if (shift < 11) c >>= shift;
+	else c = (c << (32 - shift)) >> 16;
+	return c - (lo << 3);
+}
+
+DEV void convert_factors(const int32_t in[4], const int32_t f[4], int32_t o[4])
+{
+	int32_t t0 = sext(in[0], 9), t1 = sext(in[1], 9), t2 = sext(in[2], 9);
+	o[0] = t2 + ((f[0] * t1 + 0x80) >> 8);
+	o[1] = t2 + ((f[1] * t0 + f[2] * t1 + 0x80) >> 8);
+	o[2] = t2 + ((f[3] * t0 + 0x80) >> 8);
+	o[3] = t2;
+}
+
+DEV void bilinear_3tap2(const int32_t *t00, const int32_t *t10, const int32_t *t01, const int32_t *t11, int fx, int fy,
+                        int32_t *o)
+{
+	int sum = fx + fy;
+	const int32_t *b = sum >= 32 ? t11 : t00;
+	int ffx = sum >= 32 ? 32 - fy : fx;
+	int ffy = sum >= 32 ? 32 - fx : fy;
+#pragma unroll
+	for (int i = 0; i < 2; i++)
+		o[i] = ((((t10[i] - b[i]) * ffx + (t01[i] - b[i]) * ffy) + 0x10) >> 5) + b[i];
+}
+
+// texture.h:498-923
+DEV void sample_texture(const Tex &tx, const TileDesc &tile, int32_t st_s, int32_t st_t, bool tlut, bool tlut_type,
+                        bool sample_quad, bool mid_state, bool convert_one, bool bilerp, const int32_t factors[4],
+                        const int32_t prev[4], int32_t out[4])
+{
+	int s = clamp_and_shift_coord((tile.flags & TILE_CLAMP_S) != 0, st_s, (int)tile.slo, (int)tile.shi, tile.shift_s);
+	int t = clamp_and_shift_coord((tile.flags & TILE_CLAMP_T) != 0, st_t, (int)tile.tlo, (int)tile.thi, tile.shift_t);
+	int fx = 0, fy = 0;
+	if (sample_quad || tlut) {
+		fx = s & 31;
+		fy = t & 31;
+	}
+	int sum_frac = fx + fy;
+	s >>= 5;
+	t >>= 5;
+	const bool ms = (tile.flags & TILE_MIRROR_S) != 0, mt = (tile.flags & TILE_MIRROR_T) != 0;
+	int s0 = mask_coord(s, tile.mask_s, ms);
+	int t0 = mask_coord(t, tile.mask_t, mt);
+	int s1 = mask_coord(s + 1, tile.mask_s, ms);
+	int t1 = mask_coord(t + 1, tile.mask_t, mt);
+	int tdiff = imax(t1 - t0, -255);
+	t1 = (t0 & 0xff) + tdiff;
+	t0 &= 0xff;
+
+	int32_t tb[4] = { 0, 0, 0, 0 }, t10[4] = { 0, 0, 0, 0 }, t01[4] = { 0, 0, 0, 0 }, t11[4] = { 0, 0, 0, 0 };
+	const bool mid_texel = mid_state && bilerp && fx == 0x10 && fy == 0x10;
+	const bool upper_lut = sum_frac >= 0x20;
+	if (mid_texel) sum_frac = 0;
+	const bool yuv = tile.fmt == TF_YUV;
+	int bs = sum_frac >= 0x20 ? s1 : s0;
+	int bt = sum_frac >= 0x20 ? t1 : t0;
+	const int chroma_frac = ((s0 & 1) << 4) | (fx >> 1);
+
+	if (tlut) {
+		if (!sample_quad) {
+			bs = s0;
+			bt = t0;
+			s1 = s0;
+			t1 = t0;
+		}
+		if (!yuv) {
+			const bool upper = sum_frac >= 0x20;
+			const uint32_t axor = upper_lut ? 2 : 1;
+			fetch_texel(tx, tile, (uint32_t)bs, (uint32_t)bt, true, tlut_type, upper ? 3 : 0, axor, tb);
+			if (bilerp) {
+				fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t0, true, tlut_type, 1, axor, t10);
+				fetch_texel(tx, tile, (uint32_t)s0, (uint32_t)t1, true, tlut_type, 2, axor, t01);
+			}
+			if (mid_texel)
+				fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t1, true, tlut_type, 3, axor, t11);
+		}
+	} else if (yuv) {
+		uint32_t cx0 = (uint32_t)(s0 >> 1), cx1 = (uint32_t)((s1 + (s1 - s0)) >> 1);
+		fetch_yuv16(tx, tile, (uint32_t)s0, (uint32_t)t0, cx0, tb);
+		if (sample_quad) {
+			fetch_yuv16(tx, tile, (uint32_t)s1, (uint32_t)t0, cx1, t10);
+			fetch_yuv16(tx, tile, (uint32_t)s0, (uint32_t)t1, cx0, t01);
+			fetch_yuv16(tx, tile, (uint32_t)s1, (uint32_t)t1, cx1, t11);
+		}
+	} else {
+		fetch_texel(tx, tile, (uint32_t)bs, (uint32_t)bt, false, false, 0, 0, tb);
+		if (sample_quad) {
+			fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t0, false, false, 0, 0, t10);
+			fetch_texel(tx, tile, (uint32_t)s0, (uint32_t)t1, false, false, 0, 0, t01);
+		}
+		if (mid_texel)
+			fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t1, false, false, 0, 0, t11);
+	}
+
+	int32_t acc[4];
+	if (convert_one) {
+		int32_t ps[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++) ps[i] = sext(prev[i], 9);
+		if (sample_quad) {
+			const bool mid_rg = yuv ? (mid_state && chroma_frac == 0x10 && fy == 0x10) : mid_texel;
+			const bool mid_ba = mid_texel;
+			const bool upper_ba = sum_frac >= 32;
+			const bool upper_rg = yuv ? ((chroma_frac + fy) >= 32 && !mid_rg) : upper_ba;
+			int32_t frg0 = upper_rg ? ps[1] : ps[0], frg1 = upper_rg ? ps[0] : ps[1];
+			int32_t fba0 = upper_ba ? ps[1] : ps[0], fba1 = upper_ba ? ps[0] : ps[1];
+#pragma unroll
+			for (int i = 0; i < 4; i++) {
+				const bool rg = i < 2;
+				const bool mid = rg ? mid_rg : mid_ba;
+				const int32_t f0 = rg ? frg0 : fba0, f1 = rg ? frg1 : fba1;
+				int32_t conv;
+				if (mid)
+					conv = f0 * (t01[i] - t11[i]) + f1 * (t10[i] - t11[i]) + ((tb[i] - t11[i]) << 6) + 0x80;
+				else {
+					const bool up = rg ? upper_rg : upper_ba;
+					int32_t base = (up && yuv) ? t11[i] : tb[i];
+					conv = f0 * (t10[i] - base) + f1 * (t01[i] - base) + 0x80;
+				}
+				acc[i] = (conv >> 8) + ps[2];
+			}
+		} else
+			acc[0] = acc[1] = acc[2] = acc[3] = ps[2];
+	} else if (yuv) {
+		if (sample_quad) {
+			int32_t ac[2], al[2];
+			if (bilerp) {
+				if (mid_state && chroma_frac == 0x10 && fy == 0x10) {
+					ac[0] = (tb[0] + t10[0] + t11[0] + t01[0] + 2) >> 2;
+					ac[1] = (tb[1] + t10[1] + t11[1] + t01[1] + 2) >> 2;
+				} else
+					bilinear_3tap2(tb, t10, t01, t11, chroma_frac, fy, ac);
+				if (mid_texel) {
+					al[0] = (tb[2] + t10[2] + t11[2] + t01[2] + 2) >> 2;
+					al[1] = (tb[3] + t10[3] + t11[3] + t01[3] + 2) >> 2;
+				} else
+					bilinear_3tap2(tb + 2, t10 + 2, t01 + 2, t11 + 2, fx, fy, al);
+			} else {
+				const int32_t *l = (fx + fy >= 32) ? t11 : tb;
+				const int32_t *c = (chroma_frac + fy >= 32) ? t11 : tb;
+				al[0] = l[2];
+				al[1] = l[3];
+				ac[0] = c[0];
+				ac[1] = c[1];
+			}
+			acc[0] = ac[0];
+			acc[1] = ac[1];
+			acc[2] = al[0];
+			acc[3] = al[1];
+		} else {
+#pragma unroll
+			for (int i = 0; i < 4; i++) acc[i] = tb[i];
+		}
+	} else if (mid_texel) {
+#pragma unroll
+		for (int i = 0; i < 4; i++) acc[i] = (tb[i] + t01[i] + t10[i] + t11[i] + 2) >> 2;
+	} else if (bilerp && (sample_quad || tlut)) {
+		const int ffx = sum_frac >= 32 ? 32 - fy : fx;
+		const int ffy = sum_frac >= 32 ? 32 - fx : fy;
+#pragma unroll
+		for (int i = 0; i < 4; i++) acc[i] = ((((t10[i] - tb[i]) * ffx + (t01[i] - tb[i]) * ffy) + 0x10) >> 5) + tb[i];
+	} else {
+#pragma unroll
+		for (int i = 0; i < 4; i++) acc[i] = tb[i];
+	}
+
+	if (!bilerp && !convert_one)
+		convert_factors(acc, factors, out);
+	else {
+#pragma unroll
+		for (int i = 0; i < 4; i++) out[i] = acc[i];
+	}
+}
+
+// texture.h:925-1003
+DEV void compute_lod(uint32_t &tile0, uint32_t &tile1, int32_t &lod_frac, uint32_t max_level, int min_lod,
+                     const int32_t st[2], const int32_t stdx[2], const int32_t stdy[2], bool ovf, bool tex_lod_en,
+                     bool sharpen, bool detail)
+{
+	bool magnify = false, distant = false;
+	uint32_t tile_offset = 0;
+	if (ovf) {
+		distant = true;
+		lod_frac = 0xff;
+	} else {
+		int32_t dx0 = stdx[0] - st[0], dx1 = stdx[1] - st[1];
+		int32_t dy0 = stdy[0] - st[0], dy1 = stdy[1] - st[1];
+		dx0 ^= dx0 >> 31;
+		dx1 ^= dx1 >> 31;
+		dy0 ^= dy0 >> 31;
+		dy1 ^= dy1 >> 31;
+		int32_t max_d = imax(imax(dx0, dy0), imax(dx1, dy1));
+		if (max_d >= 0x4000) {
+			distant = true;
+			lod_frac = 0xff;
+			tile_offset = max_level;
+		} else if (max_d < 32) {
+			distant = max_level == 0;
+			magnify = true;
+			if (!sharpen && !detail)
+				lod_frac = distant ? 0xff : 0;
+			else
+				lod_frac = (imax(min_lod, max_d) << 3) + (sharpen ? -0x100 : 0);
+		} else {
+			int mip_base = imax(find_msb(max_d >> 5), 0);
+			distant = (uint32_t)mip_base >= max_level;
+			if (distant && !sharpen && !detail)
+				lod_frac = 0xff;
+			else {
+				lod_frac = ((max_d << 3) >> mip_base) & 0xff;
+				tile_offset = (uint32_t)mip_base;
+			}
+		}
+	}
+	if (tex_lod_en) {
+		if (distant) tile_offset = max_level;
+		if (!detail) {
+			tile0 = (tile0 + tile_offset) & 7;
+			tile1 = (distant || (!sharpen && magnify)) ? tile0 : ((tile0 + 1) & 7);
+		} else {
+			tile1 = (tile0 + tile_offset + ((distant || magnify) ? 1 : 2)) & 7;
+			tile0 = (tile0 + tile_offset + (magnify ? 0 : 1)) & 7;
+		}
+	}
+}
+
+// texture.h:360-472
+DEV int sample_copy_word(const Tex &tx, const TileDesc &tile, int st_s, int st_t, int s_offset, bool tlut)
+{
+	const bool high_word = s_offset < 2;
+	const bool replicate = high_word && tile.size != 2 && !tlut;
+	const int shamt = imin((int)tile.size, 2);
+	const uint32_t idx_mask = (tile.size == 3 || tlut) ? 0x3ff : 0x7ff;
+	const bool ms = (tile.flags & TILE_MIRROR_S) != 0, mt = (tile.flags & TILE_MIRROR_T) != 0;
+	if (replicate) {
+		int sx = st_s + 2 * s_offset;
+		int t = mask_coord(st_t, tile.mask_t, mt);
+		uint32_t tbase = tile.offset + tile.stride * (uint32_t)t;
+		uint32_t sam[2];
+#pragma unroll
+		for (int i = 0; i < 2; i++) {
+			int sv = mask_coord(sx + i, tile.mask_s, ms);
+			uint32_t nib = (tbase * 2 + ((uint32_t)sv << shamt)) & 0x1fff;
+			nib ^= ((uint32_t)t & 1) * 8;
+			uint32_t v = tx.t16(((nib >> 2) & idx_mask) ^ 1);
+			if (tile.size == 1)
+				v = (v >> (8 - 4 * (nib & 2))) & 0xff;
+			else if (tile.size == 0)
+				v = ((v >> (12 - 4 * (nib & 3))) & 0xf) * 0x11;
+			else
+				v >>= 8;
+			sam[i] = v;
+		}
+		return (int)((sam[0] << 8) | sam[1]);
+	}
+	int sv = mask_coord(st_s + s_offset, tile.mask_s, ms);
+	int t = mask_coord(st_t, tile.mask_t, mt);
+	uint32_t tbase = tile.offset + tile.stride * (uint32_t)t;
+	uint32_t nib = (tbase * 2 + ((uint32_t)sv << shamt)) & 0x1fff;
+	nib ^= ((uint32_t)t & 1) * 8;
+	int samp = (int)tx.t16(((nib >> 2) & idx_mask) ^ 1);
+	if (tlut) {
+		if (tile.size == 0) {
+			samp >>= 12 - 4 * (nib & 3);
+			samp &= 0xf;
+			samp |= (int)tile.palette << 4;
+		} else {
+			samp >>= 8 - 4 * (nib & 2);
+			samp &= 0xff;
+		}
+		samp = (samp << 2) + s_offset;
+		samp = (int)tx.t16(((uint32_t)samp | 0x400) ^ 1);
+	}
+	return samp;
+}
+
+// ---- combiner: combiner.h -------------------------------------------------
+DEV int32_t special_expand(int32_t v) { return sext(v - 0x80, 9) + 0x80; }
+
+struct CombIn {
+	int32_t shade[4], combined[4], texel0[4], texel1[4];
+	int32_t lod_frac, noise;
+};
+
+// Selector decoding follows the reference's select_* helpers: values that
+// do not name a per-pixel input read the constant slot.
+DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const uint8_t k[4])
+{
+	switch (which) {
+	case 0:  // muladd
+		switch (sel) {
+		case 0: return in.combined[ch];
+		case 1: return in.texel0[ch];
+		case 2: return in.texel1[ch];
+		case 4: return in.shade[ch];
+		case 6: return 0x100;
+		case 7: return in.noise;
+		default: return k[ch];
+		}
+	case 1:  // mulsub
+		switch (sel) {
+		case 0: return in.combined[ch];
+		case 1: return in.texel0[ch];
+		case 2: return in.texel1[ch];
+		case 4: return in.shade[ch];
+		case 7: return ((int32_t)k[1] << 8) | k[2];
+		default: return k[ch];
+		}
+	case 2:  // mul
+		switch (sel) {
+		case 0: return in.combined[ch];
+		case 1: return in.texel0[ch];
+		case 2: return in.texel1[ch];
+		case 4: return in.shade[ch];
+		case 7: return in.combined[3];
+		case 8: return in.texel0[3];
+		case 9: return in.texel1[3];
+		case 11: return in.shade[3];
+		case 13: return in.lod_frac;
+		case 15: return ((int32_t)k[1] << 8) | k[2];
+		default: return k[ch];
+		}
+	default:  // add
+		switch (sel) {
+		case 0: return in.combined[ch];
+		case 1: return in.texel0[ch];
+		case 2: return in.texel1[ch];
+		case 4: return in.shade[ch];
+		case 6: return 0x100;
+		default: return k[ch];
+		}
+	}
+}
+
+DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const uint8_t k[4])
+{
+	if (which == 2) {
+		switch (sel) {
+		case 0: return in.lod_frac;
+		case 1: return in.texel0[3];
+		case 2: return in.texel1[3];
+		case 4: return in.shade[3];
+		default: return k[3];
+		}
+	}
+	switch (sel) {
+	case 0: return in.combined[3];
+	case 1: return in.texel0[3];
+	case 2: return in.texel1[3];
+	case 4: return in.shade[3];
+	case 6: return 0x100;
+	default: return k[3];
+	}
+}
+
+DEV void combine(const CombIn &in, const uint8_t sel[8], const uint8_t k[4][4], int32_t out[4])
+{
+#pragma unroll
+	for (int ch = 0; ch < 4; ch++) {
+		int32_t a, b, c, d;
+		if (ch < 3) {
+			a = sel_rgb(0, sel[0], ch, in, k[0]);
+			b = sel_rgb(1, sel[1], ch, in, k[1]);
+			c = sel_rgb(2, sel[2], ch, in, k[2]);
+			d = sel_rgb(3, sel[3], ch, in, k[3]);
+		} else {
+			a = sel_alpha(0, sel[4], in, k[0]);
+			b = sel_alpha(1, sel[5], in, k[1]);
+			c = sel_alpha(2, sel[6], in, k[2]);
+			d = sel_alpha(3, sel[7], in, k[3]);
+		}
+		c = sext(c, 9);
+		a = special_expand(a);
+		b = special_expand(b);
+		d = special_expand(d);
+		out[ch] = (((a - b) * c + 0x80) >> 8) + d;
+	}
+}
+
+DEV void dither_coefficients(int x, int y, int mode_rgb, int mode_alpha, uint32_t noise, int &rgb_dith, int &alpha_dith)
+{
+	// dither.h:26-68: magic-square and Bayer matrices packed 3 bits per entry.
+	const uint64_t magic = 0x1ull * 0 | 06ull << 3 | 01ull << 6 | 07ull << 9 | 04ull << 12 | 02ull << 15 | 05ull << 18 |
+	                       03ull << 21 | 03ull << 24 | 05ull << 27 | 02ull << 30 | 04ull << 33 | 07ull << 36 |
+	                       01ull << 39 | 06ull << 42 | 00ull << 45;
+	const uint64_t bayer = 00ull | 04ull << 3 | 01ull << 6 | 05ull << 9 | 04ull << 12 | 00ull << 15 | 05ull << 18 |
+	                       01ull << 21 | 03ull << 24 | 07ull << 27 | 02ull << 30 | 06ull << 33 | 07ull << 36 |
+	                       03ull << 39 | 06ull << 42 | 02ull << 45;
+	const int idx = (y & 3) * 4 + (x & 3);
+	const int mag = (int)((magic >> (3 * idx)) & 7);
+	const int bay = (int)((bayer >> (3 * idx)) & 7);
+	if (mode_rgb == 0) rgb_dith = mag * 0x49;
+	else if (mode_rgb == 1) rgb_dith = bay * 0x49;
+	else if (mode_rgb == 2) rgb_dith = (int)(noise & 0x1ff);
+	else rgb_dith = 0;
+	if (mode_alpha == 3) alpha_dith = 0;
+	else if (mode_alpha == 2) alpha_dith = (int)(noise & 7);
+	else {
+		int a = mode_rgb >= 2 ? ((mode_rgb & 1) ? bay : mag) : (rgb_dith & 7);
+		if (mode_alpha == 1) a = ~a & 7;
+		alpha_dith = a;
+	}
+}
+
+// ---- depth / blend: z_encode.h, depth_test.h, blender.h, coverage.h --------
+DEV int32_t z_decompress(uint32_t z)
+{
+	int32_t e = (int32_t)(z >> 11), man = (int32_t)(z & 0x7ff);
+	return (man << imax(6 - e, 0)) + (0x40000 - (0x40000 >> e));
+}
+
+DEV uint32_t z_compress(int32_t z)
+{
+	int32_t e = iclamp(17 - find_msb(imax(0x3ffff - z, 1)), 0, 7);
+	int32_t man = (z >> imax(6 - e, 0)) & 0x7ff;
+	return (uint32_t)((e << 11) + man) & 0xffff;
+}
+
+struct Pixel {
+	int32_t c[4];
+	uint32_t depth, dz;
+	bool color_dirty, depth_dirty;
+};
+
+DEV void write_color(Pixel &p, uint32_t fmt, int32_t r, int32_t g, int32_t b, int32_t a)
+{
+	p.c[0] = r;
+	p.c[1] = g;
+	p.c[2] = b;
+	if (fmt != FB_I4) p.c[3] = a;
+	p.color_dirty = true;
+}
+
+DEV void alias_color_to_depth(Pixel &p, uint32_t fmt)
+{
+	if (fmt == FB_RGBA5551) {
+		p.dz = (((uint32_t)p.c[3] >> 3) | ((uint32_t)p.c[2] & 8)) & 0xff;
+		p.depth = ((((uint32_t)p.c[0] & 0xf8) << 6) | (((uint32_t)p.c[1] & 0xf8) << 1) | (((uint32_t)p.c[2] & 0xf8) >> 4)) & 0xffff;
+	} else if (fmt == FB_IA88) {
+		uint32_t w = (((uint32_t)p.c[0] & 0xff) << 8) | ((uint32_t)p.c[3] & 0xff);
+		p.depth = (w >> 2) & 0xffff;
+		p.dz = (((w & 3) << 2) | ((w & 1) * 3)) & 0xff;
+	}
+}
+
+DEV void alias_depth_to_color(Pixel &p, uint32_t fmt)
+{
+	uint32_t w = (p.depth << 4) | p.dz;
+	if (fmt == FB_RGBA5551) {
+		p.c[0] = (int32_t)((w >> 10) & 0xf8);
+		p.c[1] = (int32_t)((w >> 5) & 0xf8);
+		p.c[2] = (int32_t)(w & 0xf8);
+		p.c[3] = (int32_t)((w & 7) << 5);
+	} else if (fmt == FB_IA88) {
+		p.c[0] = (int32_t)((w >> 10) & 0xff);
+		p.c[3] = (int32_t)((w >> 2) & 0xff);
+	}
+	p.color_dirty = true;
+}
+
+DEV void blender(const uint8_t *lut, const int32_t pixel[4], const int32_t memory[4], const uint8_t fog[4],
+                 const uint8_t blendc[4], int32_t shade_alpha, const uint8_t modes[4], bool force_blend, bool blend_en,
+                 bool color_on_cvg, bool cvg_wrap, int shift0, int shift1, bool final_cycle, int32_t out[3])
+{
+	int32_t src1[3], src0[3];
+#pragma unroll
+	for (int i = 0; i < 3; i++) {
+		switch (modes[2]) {
+		case 0: src1[i] = pixel[i]; break;
+		case 1: src1[i] = memory[i]; break;
+		case 2: src1[i] = blendc[i]; break;
+		default: src1[i] = fog[i]; break;
+		}
+		switch (modes[0]) {
+		case 0: src0[i] = pixel[i]; break;
+		case 1: src0[i] = memory[i]; break;
+		case 2: src0[i] = blendc[i]; break;
+		default: src0[i] = fog[i]; break;
+		}
+	}
+	if (final_cycle && color_on_cvg && !cvg_wrap) {
+		out[0] = src1[0];
+		out[1] = src1[1];
+		out[2] = src1[2];
+		return;
+	}
+	if (final_cycle && (!blend_en || (modes[1] == 0 && modes[3] == 0 && pixel[3] == 0xff))) {
+		out[0] = src0[0];
+		out[1] = src0[1];
+		out[2] = src0[2];
+		return;
+	}
+	int32_t a0, a1;
+	switch (modes[1]) {
+	case 0: a0 = pixel[3]; break;
+	case 1: a0 = fog[3]; break;
+	case 2: a0 = shade_alpha; break;
+	default: a0 = 0; break;
+	}
+	switch (modes[3]) {
+	case 0: a1 = ~a0 & 0xff; break;
+	case 1: a1 = memory[3]; break;
+	case 2: a1 = 0xff; break;
+	default: a1 = 0; break;
+	}
+	a0 >>= 3;
+	a1 >>= 3;
+	if (modes[3] == 1) {
+		a0 = (a0 >> shift0) & 0x3c;
+		a1 = (a1 >> shift1) | 3;
+	}
+	if (!final_cycle || force_blend) {
+#pragma unroll
+		for (int i = 0; i < 3; i++) out[i] = ((src0[i] * a0 + src1[i] * (a1 + 1)) >> 5) & 0xff;
+	} else {
+		const uint32_t sum = (uint32_t)((a0 >> 2) + (a1 >> 2) + 1);
+#pragma unroll
+		for (int i = 0; i < 3; i++) {
+			uint32_t b = (uint32_t)((src0[i] * a0 + src1[i] * (a1 + 1)) >> 2) & 0x7ff;
+			uint32_t idx = (sum << 11) | b;
+			out[i] = idx < 0x8000 ? lut[idx] : 0;
+		}
+	}
+}
+
+DEV void load_color(const Mem &m, uint32_t fmt, uint32_t index, Pixel &p)
+{
+	switch (fmt) {
+	case FB_I4:
+	case FB_I8: {
+		uint32_t i = index & m.mask8;
+		uint32_t w = m.rdram[i ^ 3];
+		p.c[0] = p.c[1] = p.c[2] = (int32_t)w;
+		p.c[3] = (int32_t)m.hid(i >> 1);
+		break;
+	}
+	case FB_RGBA5551: {
+		uint32_t w = m.rd16i(index);
+		p.c[0] = (int32_t)((w >> 8) & 0xf8);
+		p.c[1] = (int32_t)((w >> 3) & 0xf8);
+		p.c[2] = (int32_t)((w << 2) & 0xf8);
+		p.c[3] = (int32_t)(((m.hid(index) << 5) | ((w & 1) << 7)) & 0xff);
+		break;
+	}
+	case FB_IA88: {
+		uint32_t w = m.rd16i(index);
+		p.c[0] = p.c[1] = p.c[2] = (int32_t)((w >> 8) & 0xff);
+		p.c[3] = (int32_t)(w & 0xff);
+		break;
+	}
+	default: {
+		uint32_t w = m.rd32i(index);
+		p.c[0] = (int32_t)(w >> 24);
+		p.c[1] = (int32_t)((w >> 16) & 0xff);
+		p.c[2] = (int32_t)((w >> 8) & 0xff);
+		p.c[3] = (int32_t)(w & 0xff);
+		break;
+	}
+	}
+}
+
+DEV void store_color(const Mem &m, uint32_t fmt, uint32_t index, const Pixel &p)
+{
+	switch (fmt) {
+	case FB_I4: {
+		uint32_t i = index & m.mask8;
+		m.rdram[i ^ 3] = 0;
+		if (i & 1) m.set_hid(i >> 1, (uint32_t)p.c[3]);
+		break;
+	}
+	case FB_I8: {
+		uint32_t i = index & m.mask8;
+		uint32_t col = (uint32_t)((i & 1) ? p.c[1] : p.c[0]) & 0xff;
+		m.rdram[i ^ 3] = (uint8_t)col;
+		if (i & 1) m.set_hid(i >> 1, (col & 1) * 3);
+		break;
+	}
+	case FB_RGBA5551: {
+		uint32_t cov = ((uint32_t)p.c[3] & 0xff) >> 5;
+		uint32_t w = (((uint32_t)p.c[0] & 0xf8) << 8) | (((uint32_t)p.c[1] & 0xf8) << 3) | (((uint32_t)p.c[2] & 0xf8) >> 2) |
+		             (cov >> 2);
+		m.wr16i(index, w);
+		m.set_hid(index, cov & 3);
+		break;
+	}
+	case FB_IA88: {
+		uint32_t w = (((uint32_t)p.c[0] & 0xff) << 8) | ((uint32_t)p.c[3] & 0xff);
+		m.wr16i(index, w);
+		m.set_hid(index, ((uint32_t)p.c[3] & 1) * 3);
+		break;
+	}
+	default: {
+		uint32_t w = (((uint32_t)p.c[0] & 0xff) << 24) | (((uint32_t)p.c[1] & 0xff) << 16) | (((uint32_t)p.c[2] & 0xff) << 8) |
+		             ((uint32_t)p.c[3] & 0xff);
+		m.wr32i(index, w);
+		uint32_t i = index & (m.mask8 >> 2);
+		m.set_hid(2 * i, ((uint32_t)p.c[1] & 1) * 3);
+		m.set_hid(2 * i + 1, ((uint32_t)p.c[3] & 1) * 3);
+		break;
+	}
+	}
+}
+
+struct Shaded {
+	int32_t combined[4];
+	int32_t z_dith;
+	int32_t coverage;
+	int32_t shade_alpha;
+};
+
+#define COVERAGE_FILL 0x40
+#define COVERAGE_COPY 0x20
+
+// shading.h:38-359 for one pixel of one primitive (no upscaling).
+DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st, const SpanSetup &span, int x, int y,
+                     Shaded &out)
+{
+	const uint32_t f = st.ras_flags;
+	const bool tlut = f & RAS_TLUT, tlut_type = f & RAS_TLUT_TYPE, sample_quad = f & RAS_SAMPLE_QUAD;
+	const bool cvg_times_alpha = f & RAS_CVG_TIMES_ALPHA, alpha_cvg_select = f & RAS_ALPHA_CVG_SELECT;
+	const bool perspective = f & RAS_PERSPECTIVE, aa = f & RAS_AA, multi = f & RAS_MULTI_CYCLE;
+	const bool interlace = f & RAS_INTERLACE_FIELD, alpha_test = f & RAS_ALPHA_TEST;
+	const bool mid_texel = f & RAS_MID_TEXEL, convert_one = f & RAS_CONVERT_ONE;
+	const bool bilerp0 = f & RAS_BILERP0, bilerp1 = f & RAS_BILERP1;
+	bool uses_texel1 = f & RAS_USES_TEXEL1;
+
+	uint32_t noise = 0;
+	if (f & RAS_NEED_NOISE)
+		noise = reseed_noise((uint32_t)x, (uint32_t)y, ps.prim_index);
+
+	const bool flip = (ps.flags & TRI_FLIP) != 0;
+	const Tex tx{ P.tmem_instances + (size_t)ps.tmem_instance * 2048 };
+
+	if (f & RAS_COPY) {
+		if (x < span.start_x || x > span.end_x) return false;
+		int dxp = flip ? (x - span.start_x) : (span.end_x - x);
+		uint32_t dx_mask, dx_shift, fb_size;
+		switch (P.fb_fmt) {
+		case FB_I4: fb_size = 0; dx_mask = 0; dx_shift = 0; break;
+		case FB_I8: fb_size = 1; dx_mask = ~7u; dx_shift = 3; break;
+		case FB_RGBA5551:
+		case FB_IA88: fb_size = 2; dx_mask = ~3u; dx_shift = 2; break;
+		default: fb_size = 4; dx_mask = ~1u; dx_shift = 1; break;
+		}
+		int s_offset = dxp - (dxp & (int)dx_mask);
+		int lerp_dx = (dxp >> dx_shift) * (flip ? 1 : -1);
+		int32_t ss, tt;
+		bool ovf = false;
+		stw_to_st(span.stzw[0] + (ps.dstzw_dx[0] & ~0x1f) * lerp_dx, span.stzw[1] + (ps.dstzw_dx[1] & ~0x1f) * lerp_dx,
+		          span.stzw[3] + (ps.dstzw_dx[3] & ~0x1f) * lerp_dx, perspective, ss, tt, ovf);
+		const TileDesc &t0 = st.tiles[ps.tile & 7];
+		int s = shift_coord(ss, (int)t0.slo, t0.shift_s) >> 5;
+		int t = shift_coord(tt, (int)t0.tlo, t0.shift_t) >> 5;
+		int texel;
+		if (fb_size == 0)
+			texel = 0;
+		else if (fb_size == 1)
+			texel = (sample_copy_word(tx, t0, s, t, s_offset >> 1, tlut) >> (8 - 8 * (s_offset & 1))) & 0xff;
+		else
+			texel = sample_copy_word(tx, t0, s, t, s_offset, tlut);
+		out.z_dith = texel;
+		out.coverage = COVERAGE_COPY;
+		if (alpha_test && fb_size == 2 && (texel & 1) == 0) return false;
+		return true;
+	} else if (f & RAS_FILL) {
+		out.coverage = COVERAGE_FILL;
+		return x >= span.start_x && x <= span.end_x;
+	}
+
+	const int coverage = compute_coverage(span, x);
+	if (coverage == 0) return false;
+	int cvg_count = __popc(coverage);
+	if (!aa && (coverage & 1) == 0) return false;
+
+	const int dx = x - span.base_x;
+	const int dir = flip ? 1 : -1;
+	const int first = find_lsb((uint32_t)coverage);
+	const int yoff = first >> 1, xoff = ((first & 1) << 1) + (yoff & 1);
+
+	int32_t shade[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		int32_t v = span.rgba[i] + (ps.drgba_dx[i] & ~0x1f) * dx;
+		int32_t sn = (v >> 14) << 2;
+		sn += xoff * (ps.drgba_dx[i] >> 14) + yoff * (ps.drgba_dy[i] >> 14);
+		shade[i] = clamp_9bit(sn >> 4);
+	}
+
+	const bool uses_lod = f & RAS_USES_LOD;
+	int32_t st0[2], stdx[2] = { 0, 0 }, stdy[2] = { 0, 0 };
+	bool povf = false;
+	{
+		int32_t s = span.stzw[0] + (ps.dstzw_dx[0] & ~0x1f) * dx;
+		int32_t t = span.stzw[1] + (ps.dstzw_dx[1] & ~0x1f) * dx;
+		int32_t w = span.stzw[3] + (ps.dstzw_dx[3] & ~0x1f) * dx;
+		stw_to_st(s, t, w, perspective, st0[0], st0[1], povf);
+		if (uses_lod) {
+			stw_to_st(s + dir * (ps.dstzw_dx[0] & ~0x1f), t + dir * (ps.dstzw_dx[1] & ~0x1f),
+			          w + dir * (ps.dstzw_dx[3] & ~0x1f), perspective, stdx[0], stdx[1], povf);
+			stw_to_st(s + (ps.dstzw_dy[0] & ~0x7fff), t + (ps.dstzw_dy[1] & ~0x7fff), w + (ps.dstzw_dy[3] & ~0x7fff),
+			          perspective, stdy[0], stdy[1], povf);
+		}
+	}
+	int32_t z;
+	{
+		int32_t zz = span.stzw[2] + ps.dstzw_dx[2] * dx;
+		int32_t sn = (zz >> 10) << 2;
+		sn += xoff * (ps.dstzw_dx[2] >> 10) + yoff * (ps.dstzw_dy[2] >> 10);
+		z = clamp_z(sn >> 5);
+	}
+
+	uint32_t tile0 = ps.tile & 7, tile1 = (tile0 + 1) & 7;
+	int32_t lod_frac = 0;
+	if (uses_lod)
+		compute_lod(tile0, tile1, lod_frac, ps.tile >> 3, (int)st.min_lod, st0, stdx, stdy, povf, f & RAS_TEX_LOD,
+		            f & RAS_SHARPEN_LOD, f & RAS_DETAIL_LOD);
+
+	int32_t texel0[4] = { 0, 0, 0, 0 }, texel1[4] = { 0, 0, 0, 0 };
+	if (f & RAS_USES_TEXEL0)
+		sample_texture(tx, st.tiles[tile0], st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, false, bilerp0,
+		               st.factors, texel1, texel0);
+
+	if (f & RAS_USES_PIPELINED_TEXEL1) {
+		const SpanSetup &next = (&span)[1];
+		const bool long_span = span.lodlength >= 8;
+		const bool end_span = x == (flip ? span.end_x : span.start_x);
+		if (end_span && long_span && next.valid) {
+			int32_t s = next.stzw[0] >> 16, t = next.stzw[1] >> 16, w = next.stzw[3] >> 16;
+			if (perspective) {
+				bool o2 = false;
+				perspective_divide(s, t, w, st0[0], st0[1], o2);
+			} else {
+				st0[0] = s;
+				st0[1] = t;
+			}
+		} else {
+			bool o2 = false;
+			const int d2 = dx + dir;
+			stw_to_st(span.stzw[0] + (ps.dstzw_dx[0] & ~0x1f) * d2, span.stzw[1] + (ps.dstzw_dx[1] & ~0x1f) * d2,
+			          span.stzw[3] + (ps.dstzw_dx[3] & ~0x1f) * d2, perspective, st0[0], st0[1], o2);
+		}
+		tile1 = tile0;
+		uses_texel1 = true;
+	}
+
+	if (uses_texel1) {
+		if (convert_one && !bilerp1)
+			convert_factors(texel0, st.factors, texel1);
+		else
+			sample_texture(tx, st.tiles[tile1], st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, convert_one,
+			               bilerp1, st.factors, texel0, texel1);
+	}
+
+	int rgb_dith, alpha_dith;
+	dither_coefficients(x, y >> (interlace ? 1 : 0), (int)(st.dither >> 2), (int)(st.dither & 3), noise, rgb_dith,
+	                    alpha_dith);
+
+	CombIn ci;
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		ci.shade[i] = shade[i];
+		ci.texel0[i] = texel0[i];
+		ci.texel1[i] = texel1[i];
+		ci.combined[i] = 0;
+	}
+	ci.lod_frac = lod_frac;
+	ci.noise = (int32_t)(((noise & 7) << 6) | 0x20);
+
+	int32_t combined[4];
+	int32_t alpha_ref;
+	if (multi) {
+		int32_t c0[4];
+		combine(ci, st.comb[0], st.constants[0], c0);
+		if (alpha_test) {
+			int32_t cl = clamp_9bit(c0[3]);
+			int32_t ex = cl + ((cl + 1) >> 8);
+			if (alpha_cvg_select)
+				ex = cvg_times_alpha ? ((ex * cvg_count + 4) >> 3) : (cvg_count << 5);
+			else
+				ex += alpha_dith;
+			alpha_ref = iclamp(ex, 0, 0xff);
+		} else
+			alpha_ref = 0;
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			ci.combined[i] = c0[i];
+			int32_t tmp = ci.texel0[i];
+			ci.texel0[i] = ci.texel1[i];
+			ci.texel1[i] = tmp;
+		}
+		if (f & RAS_NEED_NOISE_DUAL) {
+			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), ps.prim_index + 11);
+			ci.noise = (int32_t)(((n2 & 7) << 6) | 0x20);
+		}
+		combine(ci, st.comb[1], st.constants[1], combined);
+	} else {
+		combine(ci, st.comb[1], st.constants[1], combined);
+	}
+	// combiner_cycle1 tail (combiner.h:256-281)
+#pragma unroll
+	for (int i = 0; i < 4; i++) combined[i] = clamp_9bit(combined[i]);
+	{
+		int32_t ex = combined[3] + ((combined[3] + 1) >> 8);
+		int32_t mod;
+		if (cvg_times_alpha) {
+			mod = (ex * cvg_count + 4) >> 3;
+			cvg_count = mod >> 5;
+		} else
+			mod = cvg_count << 5;
+		ex = alpha_cvg_select ? mod : ex + alpha_dith;
+		combined[3] = iclamp(ex, 0, 0xff);
+	}
+	if (!multi) alpha_ref = combined[3];
+
+	if (aa && cvg_count == 0) return false;
+	if (alpha_test) {
+		int32_t thr = (f & RAS_ALPHA_TEST_DITHER) ? (int32_t)(noise & 0xff) : st.blendc[3];
+		if (alpha_ref < thr) return false;
+	}
+#pragma unroll
+	for (int i = 0; i < 4; i++) out.combined[i] = combined[i];
+	out.z_dith = (z << 9) | rgb_dith;
+	out.coverage = cvg_count;
+	out.shade_alpha = imin(shade[3] + alpha_dith, 0xff);
+	return true;
+}
+
+// memory_interfacing.h:488-571
+DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st, uint32_t fmt, bool alias, Pixel &p,
+                     const Shaded &sh)
+{
+	const int32_t z = sh.z_dith >> 9;
+	const int dith = sh.z_dith & 0x1ff;
+	int cvg = sh.coverage;
+	const uint32_t f = st.db_flags;
+	const bool force_blend = f & DB_FORCE_BLEND, z_compare = f & DB_DEPTH_TEST, z_update = f & DB_DEPTH_UPDATE;
+	const bool image_read = f & DB_IMAGE_READ, color_on_cvg = f & DB_COLOR_ON_CVG, multi = f & DB_MULTI_CYCLE;
+	const bool aa = f & DB_AA, dither_en = f & DB_DITHER;
+
+	int32_t memory[4];
+	int32_t mem_cov = image_read ? (p.c[3] & 0xe0) : 0xe0;
+	switch (fmt) {
+	case FB_I4: memory[0] = memory[1] = memory[2] = 0; mem_cov = 0xe0; break;
+	case FB_I8: memory[0] = memory[1] = memory[2] = p.c[0] & 0xff; mem_cov = 0xe0; break;
+	case FB_RGBA5551:
+		memory[0] = p.c[0] & 0xf8;
+		memory[1] = p.c[1] & 0xf8;
+		memory[2] = p.c[2] & 0xf8;
+		break;
+	case FB_IA88: memory[0] = memory[1] = memory[2] = p.c[0] & 0xff; break;
+	default:
+		memory[0] = p.c[0] & 0xff;
+		memory[1] = p.c[1] & 0xff;
+		memory[2] = p.c[2] & 0xff;
+		break;
+	}
+	memory[3] = mem_cov;
+	const int memory_coverage = mem_cov >> 5;
+
+	bool blend_en, cvg_wrap, pass;
+	int shift0, shift1;
+	if (z_compare) {
+		int32_t memory_z = z_decompress(p.depth);
+		int32_t memory_dz = 1 << p.dz;
+		int32_t precision = ((int32_t)p.depth >> 11) & 0xf;
+		bool coplanar = false;
+		shift0 = iclamp(ps.dz_compressed - (int32_t)p.dz, 0, 4);
+		shift1 = iclamp((int32_t)p.dz - ps.dz_compressed, 0, 4);
+		if (precision < 3) {
+			if (memory_dz != 0x8000) memory_dz = imax(memory_dz << 1, 16 >> precision);
+			else {
+				coplanar = true;
+				memory_dz = 0xffff;
+			}
+		}
+		int32_t comb = ps.dz | memory_dz;
+		if (comb != 0) comb = 1 << find_msb(comb);
+		const int32_t comb_ip = comb;
+		comb <<= 3;
+		const bool farther = coplanar || ((z + comb) >= memory_z);
+		const bool overflow = (cvg + memory_coverage) >= 8;
+		blend_en = force_blend || (!overflow && aa && farther);
+		cvg_wrap = overflow;
+		const bool max_z = memory_z == 0x3ffff;
+		const bool front = z < memory_z;
+		const bool nearer = coplanar || ((z - comb) <= memory_z);
+		switch (st.z_mode) {
+		case 0: pass = max_z || (overflow ? front : nearer); break;
+		case 1:
+			if (!front || !farther || !overflow)
+				pass = max_z || (overflow ? front : nearer);
+			else {
+				int32_t c = imax(find_msb(comb_ip & 0xffff), 0);
+				int32_t coeff = ((memory_z >> c) - (z >> c)) & 0xf;
+				cvg = imin((coeff * cvg) >> 3, 8);
+				pass = true;
+			}
+			break;
+		case 2: pass = front || max_z; break;
+		default: pass = farther && nearer && !max_z; break;
+		}
+	} else {
+		shift0 = 0;
+		shift1 = imin(0xf - ps.dz_compressed, 4);
+		const bool overflow = (cvg + memory_coverage) >= 8;
+		blend_en = force_blend || (!overflow && aa);
+		cvg_wrap = overflow;
+		pass = true;
+	}
+
+	if (pass && (!aa || cvg != 0)) {
+		int32_t pixel[4] = { sh.combined[0], sh.combined[1], sh.combined[2], sh.combined[3] };
+		const uint8_t *modes = st.blend[0];
+		int32_t rgb[3];
+		if (multi) {
+			blender(P.blender_lut, pixel, memory, st.fog, st.blendc, sh.shade_alpha, modes, force_blend, blend_en,
+			        color_on_cvg, cvg_wrap, shift0, shift1, false, rgb);
+			pixel[0] = rgb[0];
+			pixel[1] = rgb[1];
+			pixel[2] = rgb[2];
+			modes = st.blend[1];
+		}
+		blender(P.blender_lut, pixel, memory, st.fog, st.blendc, sh.shade_alpha, modes, force_blend, blend_en,
+		        color_on_cvg, cvg_wrap, shift0, shift1, true, rgb);
+		if (dither_en) {
+#pragma unroll
+			for (int i = 0; i < 3; i++) {
+				int d = (dith >> (3 * i)) & 7;
+				int32_t orig = rgb[i];
+				int32_t v = orig > 247 ? 255 : (orig & 0xf8) + 8;
+				int32_t sign = (d - (orig & 7)) >> 31;
+				rgb[i] = (orig + ((v - orig) & sign)) & 0xff;
+			}
+		}
+		int new_cov;
+		switch (st.coverage_mode) {
+		case 0: new_cov = blend_en ? imin(7, memory_coverage + cvg) : ((cvg - 1) & 7); break;
+		case 1: new_cov = (cvg + memory_coverage) & 7; break;
+		case 2: new_cov = 7; break;
+		default: new_cov = memory_coverage; break;
+		}
+		write_color(p, fmt, rgb[0], rgb[1], rgb[2], new_cov << 5);
+		if (z_update) {
+			p.depth = z_compress(z);
+			p.dz = (uint32_t)ps.dz_compressed;
+			p.depth_dirty = true;
+			if (alias) alias_depth_to_color(p, fmt);
+		} else if (alias)
+			alias_color_to_depth(p, fmt);
+	}
+}
+
+DEV void fill_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t color_index, uint32_t col)
+{
+	switch (fmt) {
+	case FB_RGBA8888:
+		write_color(p, fmt, (col >> 24) & 0xff, (col >> 16) & 0xff, (col >> 8) & 0xff, col & 0xff);
+		break;
+	case FB_RGBA5551:
+		col >>= ((color_index & 1) ^ 1) * 16;
+		write_color(p, fmt, (col >> 8) & 0xf8, (col >> 3) & 0xf8, (col << 2) & 0xf8, (col & 1) * 0xe0);
+		break;
+	case FB_IA88: {
+		col = (col >> (((color_index & 1) ^ 1) * 16)) & 0xffff;
+		int32_t r = (int32_t)((col >> 8) & 0xff);
+		write_color(p, fmt, r, r, r, (int32_t)(col & 0xff));
+		break;
+	}
+	case FB_I8: {
+		int32_t v = (int32_t)((col >> (((color_index & 3) ^ 3) * 8)) & 0xff);
+		write_color(p, fmt, v, v, v, v);
+		break;
+	}
+	default: break;
+	}
+	if (alias) alias_color_to_depth(p, fmt);
+}
+
+DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
+{
+	switch (fmt) {
+	case FB_I4:
+		p.c[0] = p.c[1] = p.c[2] = p.c[3] = 0;
+		p.color_dirty = true;
+		break;
+	case FB_I8: {
+		int32_t v = (int32_t)(word & 0xff);
+		write_color(p, fmt, v, v, v, v);
+		break;
+	}
+	case FB_RGBA5551:
+		write_color(p, fmt, (word >> 8) & 0xf8, (word >> 3) & 0xf8, (word << 2) & 0xf8, (word & 1) * 0xe0);
+		break;
+	default: break;
+	}
+	if (alias) alias_color_to_depth(p, fmt);
+}
+
+__global__ __launch_bounds__(64) void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 14];
+	const uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
+	const int lane = threadIdx.x;
+	const int x = (int)(tx * 8) + (lane & 7);
+	const int y = (int)(ty * 8) + (lane >> 3);
+	const uint32_t fmt = P.fb_fmt;
+	const bool in_fb = x < (int)P.fb_width && y < (int)P.fb_height;
+	const bool alias = P.fb_addr == P.depth_addr;
+	const Mem m{ P.rdram, P.hidden, P.rdram_mask };
+
+	uint32_t addr_index;
+	switch (fmt) {
+	case FB_I4:
+	case FB_I8: addr_index = P.fb_addr; break;
+	case FB_RGBA5551:
+	case FB_IA88: addr_index = P.fb_addr >> 1; break;
+	default: addr_index = P.fb_addr >> 2; break;
+	}
+	const uint32_t ci = addr_index + P.fb_width * (uint32_t)y + (uint32_t)x;
+	const uint32_t di = (P.depth_addr >> 1) + P.fb_width * (uint32_t)y + (uint32_t)x;
+
+	Pixel px;
+	px.color_dirty = px.depth_dirty = false;
+	load_color(m, fmt, ci, px);
+	{
+		uint32_t w = m.rd16i(di);
+		px.depth = w >> 2;
+		px.dz = (m.hid(di) | ((w & 3) << 2)) & 0xff;
+	}
+
+	const uint32_t G = P.num_groups;
+	const uint32_t tile = ty * P.tiles_x + tx;
+	uint64_t my_mask = (uint32_t)lane < G ? P.bin_fine[(size_t)tile * G + lane] : 0;
+	uint64_t coarse = __ballot(my_mask != 0);
+	while (coarse) {
+		const int g = __ffsll((unsigned long long)coarse) - 1;
+		coarse &= coarse - 1;
+		uint32_t lo = __builtin_amdgcn_readlane((uint32_t)my_mask, g);
+		uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(my_mask >> 32), g);
+		uint64_t mask = ((uint64_t)hi << 32) | lo;
+		while (mask) {
+			const int bit = __ffsll((unsigned long long)mask) - 1;
+			mask &= mask - 1;
+			const uint32_t prim = (uint32_t)(g * 64 + bit);
+			const PrimSetup &ps = P.setups[prim];
+			const int ylo = ps.ylo, yhi = ps.yhi;
+			if (!in_fb || y < ylo || y > yhi) continue;
+			const SpanSetup &span = P.spans[ps.span_offset + (uint32_t)(y - ylo)];
+			if (!span.valid) continue;
+			const DrawState &st = P.states[ps.state_index];
+			Shaded sh;
+			if (!shade_pixel(P, ps, st, span, x, y, sh)) continue;
+			if (sh.coverage & COVERAGE_FILL)
+				fill_pixel(px, fmt, alias, ci, st.fill_color);
+			else if (sh.coverage & COVERAGE_COPY)
+				copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
+			else
+				depth_blend(P, ps, st, fmt, alias, px, sh);
+		}
+	}
+
+	if (in_fb && px.color_dirty) store_color(m, fmt, ci, px);
+	if (in_fb && !alias && px.depth_dirty) {
+		m.wr16i(di, (px.depth << 2) | (px.dz >> 2));
+		m.set_hid(di, px.dz & 3);
+	}
+}
+
+// ==========================================================================
+// Launchers (host side).
+// ==========================================================================
+hipError_t launch_setup(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_setup, dim3(n), dim3(64), 0, s, passes, items);
+	return hipGetLastError();
+}
+
+hipError_t launch_tmem(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_tmem, dim3(n), dim3(256), 0, s, passes, items);
+	return hipGetLastError();
+}
+
+hipError_t launch_bin(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_bin, dim3(n), dim3(64), 0, s, passes, items);
+	return hipGetLastError();
+}
+
+hipError_t launch_raster(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_raster, dim3(n), dim3(64), 0, s, passes, items);
+	return hipGetLastError();
+}
+
+}  // namespace rdp_mi

@@ -1,0 +1,664 @@
+// Device runtime and C ABI of librdp_mi.so.
+//
+// One Context per process owns the HIP stream, the generated blender divider
+// table, the staging/input/scratch arenas and the launch plumbing.  A
+// Processor is one emulated RDP: its front end (rdp_frontend.cpp) and its
+// device-resident RDRAM, hidden RDRAM and TMEM.
+//
+// Submission groups pending render passes into rounds: round r holds the
+// r-th pending pass of every participating processor.  All inputs of all
+// rounds go to the device in one copy; each round is then four launches
+// (setup, TMEM, bin, raster) whose blocks cover every pass of the round.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rdp_mi.h"
+#include "rdp_frontend.hpp"
+#include "rdp_types.hpp"
+
+namespace rdp_mi {
+
+hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_raster(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+int vi_scanout(struct Processor &p, hipStream_t stream, uint32_t *out, unsigned *width, unsigned *height);
+
+static thread_local std::string g_last_error;
+
+static int set_error(int code, const char *fmt, ...)
+{
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_last_error = buf;
+	return code;
+}
+
+#define HIP_CHECK(expr)                                                                                        \
+	do {                                                                                                       \
+		hipError_t err_ = (expr);                                                                              \
+		if (err_ != hipSuccess)                                                                                \
+			return set_error(RDP_MI_ERROR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(err_));            \
+	} while (0)
+
+static void blender_divider_table(uint8_t *out)
+{
+	// Bit-serial non-restoring divider (11-bit dividend, 4-bit divisor,
+	// 3-bit partial remainder): the reference carries its output as data in
+	// luts.hpp:7-1032 and indexes it at blender.h:133-139.
+	for (unsigned i = 0; i < 0x8000; i++) {
+		unsigned d = (i >> 11) & 0xf, n = i & 0x7ff, nd = (~d) & 0xf;
+		unsigned part = (nd + (n >> 8) + 1) & 7, q = 0, prev = 0;
+		for (unsigned k = 0; k < 8; k++) {
+			unsigned b = (n >> (7 - k)) & 1;
+			unsigned t = prev ? nd + (part << 1) + b + 1 : d + (part << 1) + b;
+			part = t & 7;
+			prev = (t >> 4) & 1;
+			q |= prev << (7 - k);
+		}
+		out[i] = (uint8_t)q;
+	}
+}
+
+struct DeviceBuffer {
+	uint8_t *ptr = nullptr;
+	size_t size = 0;
+	int reserve(size_t want, hipStream_t stream)
+	{
+		if (want <= size) return 0;
+		size_t ns = std::max(want, size * 2);
+		ns = (ns + 0xfffff) & ~(size_t)0xfffff;
+		if (ptr) {
+			// Work already queued may still read the old allocation.
+			if (hipStreamSynchronize(stream) != hipSuccess) return RDP_MI_ERROR_DEVICE;
+			if (hipFree(ptr) != hipSuccess) return RDP_MI_ERROR_DEVICE;
+			ptr = nullptr;
+			size = 0;
+		}
+		if (hipMalloc(&ptr, ns) != hipSuccess) return RDP_MI_ERROR_OUT_OF_MEMORY;
+		size = ns;
+		return 0;
+	}
+};
+
+struct PinnedBuffer {
+	uint8_t *ptr = nullptr;
+	size_t size = 0;
+	int reserve(size_t want)
+	{
+		if (want <= size) return 0;
+		size_t ns = std::max(want, size * 2);
+		ns = (ns + 0xfffff) & ~(size_t)0xfffff;
+		if (ptr) {
+			if (hipHostFree(ptr) != hipSuccess) return RDP_MI_ERROR_DEVICE;
+			ptr = nullptr;
+			size = 0;
+		}
+		if (hipHostMalloc(&ptr, ns, hipHostMallocDefault) != hipSuccess) return RDP_MI_ERROR_OUT_OF_MEMORY;
+		size = ns;
+		return 0;
+	}
+};
+
+struct Context {
+	bool ok = false;
+	hipStream_t stream = nullptr;
+	uint8_t *d_lut = nullptr;
+	DeviceBuffer input, scratch;
+	PinnedBuffer staging;
+	hipEvent_t staging_done = nullptr;
+	bool staging_busy = false;
+	std::mutex lock;
+
+	int init()
+	{
+		if (ok) return 0;
+		int n = 0;
+		if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(RDP_MI_ERROR_DEVICE, "no HIP device");
+		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+		HIP_CHECK(hipEventCreateWithFlags(&staging_done, hipEventDisableTiming));
+		std::vector<uint8_t> lut(0x8000);
+		blender_divider_table(lut.data());
+		HIP_CHECK(hipMalloc(&d_lut, 0x8000));
+		HIP_CHECK(hipMemcpy(d_lut, lut.data(), 0x8000, hipMemcpyHostToDevice));
+		ok = true;
+		return 0;
+	}
+};
+
+static Context &ctx()
+{
+	static Context c;
+	return c;
+}
+
+struct Processor {
+	Frontend fe;
+	uint32_t rdram_size = 0;
+	uint8_t *d_rdram = nullptr;
+	uint8_t *d_hidden = nullptr;
+	uint16_t *d_tmem = nullptr;
+	uint8_t *user_rdram = nullptr;
+	std::vector<uint8_t> rdram_mirror, hidden_mirror;
+	uint16_t tmem_mirror[2048];
+	uint32_t vi_regs[RDP_MI_VI_COUNT] = {};
+	uint64_t timeline = 0;
+	std::map<uint64_t, hipEvent_t> timeline_events;
+	rdp_mi_validation_fn validation_fn = nullptr;
+	void *validation_user = nullptr;
+	bool supported = false;
+
+	explicit Processor(uint32_t size) : fe(size), rdram_size(size) { memset(tmem_mirror, 0, sizeof(tmem_mirror)); }
+	uint8_t *host_rdram() { return user_rdram ? user_rdram : rdram_mirror.data(); }
+};
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
+
+struct PassRef {
+	Processor *proc;
+	HostPass *pass;
+};
+
+// Byte sizes of the per-pass device scratch.
+struct ScratchLayout {
+	size_t setups, spans, tilerows, bins, tmem, total;
+};
+
+static ScratchLayout scratch_layout(const HostPass &hp)
+{
+	ScratchLayout l;
+	const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
+	const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
+	l.setups = align_up(hp.prims.size() * sizeof(PrimSetup), 256);
+	l.spans = align_up((size_t)hp.num_spans * sizeof(SpanSetup), 256);
+	l.tilerows = align_up((size_t)hp.num_tilerows * 4, 256);
+	l.bins = align_up((size_t)tx * ty * groups * 8, 256);
+	l.tmem = align_up((hp.uploads.size() + 1) * 4096, 256);
+	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem;
+	return l;
+}
+
+// Submit rounds of passes.  Caller holds the context lock.
+static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &rounds)
+{
+	if (rounds.empty()) return 0;
+	if (c.staging_busy) {
+		HIP_CHECK(hipEventSynchronize(c.staging_done));
+		c.staging_busy = false;
+	}
+
+	// Input layout: per round [PassDesc x n][items: setup|tmem|bin|raster], then per pass data.
+	struct RoundLayout {
+		size_t descs, items_setup, items_tmem, items_bin, items_raster;
+		uint32_t n_setup, n_tmem, n_bin, n_raster;
+		size_t scratch;
+	};
+	std::vector<RoundLayout> rl(rounds.size());
+	struct PassInputs {
+		size_t words, states, prims, uploads;
+		ScratchLayout sl;
+		size_t scratch_base;
+	};
+	std::vector<std::vector<PassInputs>> pl(rounds.size());
+
+	size_t off = 0, max_scratch = 0;
+	for (size_t r = 0; r < rounds.size(); r++) {
+		RoundLayout &L = rl[r];
+		L.n_setup = L.n_tmem = L.n_bin = L.n_raster = 0;
+		size_t scr = 0;
+		pl[r].resize(rounds[r].size());
+		for (size_t i = 0; i < rounds[r].size(); i++) {
+			const HostPass &hp = *rounds[r][i].pass;
+			const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
+			const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
+			L.n_setup += (uint32_t)hp.prims.size();
+			L.n_tmem += 8;
+			if (!hp.prims.empty()) {
+				L.n_bin += ty * groups;
+				L.n_raster += tx * ty;
+			}
+			PassInputs &pi = pl[r][i];
+			pi.sl = scratch_layout(hp);
+			pi.scratch_base = scr;
+			scr += pi.sl.total;
+		}
+		max_scratch = std::max(max_scratch, scr);
+		L.scratch = scr;
+		L.descs = off;
+		off = align_up(off + rounds[r].size() * sizeof(PassDesc), 256);
+		L.items_setup = off;
+		off = align_up(off + L.n_setup * 4ull, 256);
+		L.items_tmem = off;
+		off = align_up(off + L.n_tmem * 4ull, 256);
+		L.items_bin = off;
+		off = align_up(off + L.n_bin * 4ull, 256);
+		L.items_raster = off;
+		off = align_up(off + L.n_raster * 4ull, 256);
+		for (size_t i = 0; i < rounds[r].size(); i++) {
+			const HostPass &hp = *rounds[r][i].pass;
+			PassInputs &pi = pl[r][i];
+			pi.words = off;
+			off = align_up(off + hp.words.size() * 4, 256);
+			pi.states = off;
+			off = align_up(off + hp.states.size() * sizeof(DrawState), 256);
+			pi.prims = off;
+			off = align_up(off + hp.prims.size() * sizeof(PrimRecord), 256);
+			pi.uploads = off;
+			off = align_up(off + std::max<size_t>(hp.uploads.size(), 1) * sizeof(UploadInfo), 256);
+		}
+	}
+	const size_t total_input = off;
+
+	int err;
+	if ((err = c.staging.reserve(total_input))) return set_error(err, "staging allocation of %zu bytes failed", total_input);
+	if ((err = c.input.reserve(total_input, c.stream)))
+		return set_error(err, "input allocation of %zu bytes failed", total_input);
+	if ((err = c.scratch.reserve(std::max<size_t>(max_scratch, 256), c.stream)))
+		return set_error(err, "scratch allocation of %zu bytes failed", max_scratch);
+
+	uint8_t *h = c.staging.ptr;
+	uint8_t *d = c.input.ptr;
+	for (size_t r = 0; r < rounds.size(); r++) {
+		RoundLayout &L = rl[r];
+		PassDesc *descs = reinterpret_cast<PassDesc *>(h + L.descs);
+		uint32_t *it_setup = reinterpret_cast<uint32_t *>(h + L.items_setup);
+		uint32_t *it_tmem = reinterpret_cast<uint32_t *>(h + L.items_tmem);
+		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
+		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
+		uint32_t ns = 0, nt = 0, nb = 0, nr = 0;
+		for (size_t i = 0; i < rounds[r].size(); i++) {
+			const HostPass &hp = *rounds[r][i].pass;
+			Processor &pr = *rounds[r][i].proc;
+			PassInputs &pi = pl[r][i];
+			memcpy(h + pi.words, hp.words.data(), hp.words.size() * 4);
+			memcpy(h + pi.states, hp.states.data(), hp.states.size() * sizeof(DrawState));
+			memcpy(h + pi.prims, hp.prims.data(), hp.prims.size() * sizeof(PrimRecord));
+			if (!hp.uploads.empty())
+				memcpy(h + pi.uploads, hp.uploads.data(), hp.uploads.size() * sizeof(UploadInfo));
+
+			PassDesc &D = descs[i];
+			memset(&D, 0, sizeof(D));
+			D.words = reinterpret_cast<const uint32_t *>(d + pi.words);
+			D.states = reinterpret_cast<const DrawState *>(d + pi.states);
+			D.prims = reinterpret_cast<const PrimRecord *>(d + pi.prims);
+			D.uploads = reinterpret_cast<const UploadInfo *>(d + pi.uploads);
+			uint8_t *sb = c.scratch.ptr + pi.scratch_base;
+			D.setups = reinterpret_cast<PrimSetup *>(sb);
+			D.spans = reinterpret_cast<SpanSetup *>(sb + pi.sl.setups);
+			D.tilerows = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans);
+			D.bin_fine = reinterpret_cast<uint64_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows);
+			D.bin_coarse = nullptr;
+			D.tmem_instances = reinterpret_cast<uint16_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins);
+			D.rdram = pr.d_rdram;
+			D.hidden = pr.d_hidden;
+			D.tmem = pr.d_tmem;
+			D.blender_lut = c.d_lut;
+			D.rdram_mask = pr.rdram_size - 1;
+			D.num_prims = (uint32_t)hp.prims.size();
+			D.num_uploads = (uint32_t)hp.uploads.size();
+			D.fb_addr = hp.fb_addr;
+			D.fb_width = hp.fb_width;
+			D.fb_height = hp.fb_height;
+			D.fb_fmt = hp.fb_fmt;
+			D.depth_addr = hp.depth_addr;
+			D.tiles_x = (hp.fb_width + 7) / 8;
+			D.tiles_y = (hp.fb_height + 7) / 8;
+			D.num_spans = hp.num_spans;
+			D.num_tilerows = hp.num_tilerows;
+			D.num_groups = ((uint32_t)hp.prims.size() + 63) / 64;
+
+			const uint32_t pidx = (uint32_t)i;
+			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 12) | p;
+			for (uint32_t k = 0; k < 8; k++) it_tmem[nt++] = (pidx << 12) | k;
+			if (D.num_prims) {
+				for (uint32_t y = 0; y < D.tiles_y; y++)
+					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 13) | (y << 6) | g;
+				for (uint32_t y = 0; y < D.tiles_y; y++)
+					for (uint32_t x = 0; x < D.tiles_x; x++) it_raster[nr++] = (pidx << 14) | (x << 7) | y;
+			}
+		}
+	}
+
+	HIP_CHECK(hipMemcpyAsync(d, h, total_input, hipMemcpyHostToDevice, c.stream));
+	for (size_t r = 0; r < rounds.size(); r++) {
+		RoundLayout &L = rl[r];
+		const PassDesc *descs = reinterpret_cast<const PassDesc *>(d + L.descs);
+		HIP_CHECK(launch_setup(descs, reinterpret_cast<const uint32_t *>(d + L.items_setup), L.n_setup, c.stream));
+		HIP_CHECK(launch_tmem(descs, reinterpret_cast<const uint32_t *>(d + L.items_tmem), L.n_tmem, c.stream));
+		HIP_CHECK(launch_bin(descs, reinterpret_cast<const uint32_t *>(d + L.items_bin), L.n_bin, c.stream));
+		HIP_CHECK(launch_raster(descs, reinterpret_cast<const uint32_t *>(d + L.items_raster), L.n_raster, c.stream));
+	}
+	HIP_CHECK(hipEventRecord(c.staging_done, c.stream));
+	c.staging_busy = true;
+	return 0;
+}
+
+// Submit every completed pass of the given processors (plain path: one
+// processor, one pass per round).
+static int submit_processors(Processor *const *procs, unsigned count, bool batched)
+{
+	Context &c = ctx();
+	std::vector<std::vector<PassRef>> rounds;
+	for (unsigned i = 0; i < count; i++) {
+		auto &passes = procs[i]->fe.completed();
+		for (size_t k = 0; k < passes.size(); k++) {
+			if (!batched) {
+				rounds.push_back({ { procs[i], &passes[k] } });
+				continue;
+			}
+			if (rounds.size() <= k) rounds.resize(k + 1);
+			rounds[k].push_back({ procs[i], &passes[k] });
+		}
+	}
+	int ret = 0;
+	if (batched) {
+		// Keep each launch's pass count within the work-item encoding.
+		ret = submit_rounds(c, rounds);
+	} else {
+		for (auto &r : rounds) {
+			std::vector<std::vector<PassRef>> one{ r };
+			if ((ret = submit_rounds(c, one))) break;
+		}
+	}
+	for (unsigned i = 0; i < count; i++) procs[i]->fe.completed().clear();
+	return ret;
+}
+
+// Stream-ordered synchronous copy (the context stream does not synchronise
+// with the null stream).
+static int copy_sync(void *dst, const void *src, size_t size, hipMemcpyKind kind)
+{
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	HIP_CHECK(hipMemcpyAsync(dst, src, size, kind, c.stream));
+	HIP_CHECK(hipStreamSynchronize(c.stream));
+	return 0;
+}
+
+}  // namespace rdp_mi
+
+using namespace rdp_mi;
+
+struct rdp_mi_processor {
+	Processor p;
+	explicit rdp_mi_processor(uint32_t size) : p(size) {}
+};
+
+extern "C" {
+
+const char *rdp_mi_last_error(void) { return g_last_error.c_str(); }
+
+int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, size_t rdram_size,
+                  size_t hidden_rdram_size, uint32_t flags)
+{
+	if (!out) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "out is null");
+	*out = nullptr;
+	if (rdram_size < 0x10000 || (rdram_size & (rdram_size - 1)) || rdram_size > (1u << 30))
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "rdram_size must be a power of two");
+	if (hidden_rdram_size != rdram_size / 2)
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "hidden_rdram_size must be rdram_size / 2");
+	if (flags & (RDP_MI_FLAG_UPSCALING_2X | RDP_MI_FLAG_UPSCALING_4X | RDP_MI_FLAG_UPSCALING_8X))
+		return set_error(RDP_MI_ERROR_UNSUPPORTED, "upscaling is not supported");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	int err = c.init();
+	if (err) return err;
+
+	rdp_mi_processor *h = new (std::nothrow) rdp_mi_processor((uint32_t)rdram_size);
+	if (!h) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	Processor &p = h->p;
+	auto fail = [&](int code, const char *msg) {
+		if (p.d_rdram) (void)hipFree(p.d_rdram);
+		if (p.d_hidden) (void)hipFree(p.d_hidden);
+		if (p.d_tmem) (void)hipFree(p.d_tmem);
+		delete h;
+		return set_error(code, "%s", msg);
+	};
+	if (hipMalloc(&p.d_rdram, rdram_size) != hipSuccess) return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "RDRAM allocation failed");
+	if (hipMalloc(&p.d_hidden, hidden_rdram_size) != hipSuccess)
+		return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "hidden RDRAM allocation failed");
+	if (hipMalloc(&p.d_tmem, 4096) != hipSuccess) return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "TMEM allocation failed");
+	if (rdram_ptr) {
+		p.user_rdram = static_cast<uint8_t *>(rdram_ptr) + rdram_offset;
+		if (hipMemcpyAsync(p.d_rdram, p.user_rdram, rdram_size, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+			return fail(RDP_MI_ERROR_DEVICE, "RDRAM upload failed");
+	} else {
+		p.rdram_mirror.assign(rdram_size, 0);
+		if (hipMemsetAsync(p.d_rdram, 0, rdram_size, c.stream) != hipSuccess)
+			return fail(RDP_MI_ERROR_DEVICE, "RDRAM clear failed");
+	}
+	p.hidden_mirror.assign(hidden_rdram_size, 0x03);
+	if (hipMemsetAsync(p.d_hidden, 0x03, hidden_rdram_size, c.stream) != hipSuccess)
+		return fail(RDP_MI_ERROR_DEVICE, "hidden RDRAM clear failed");
+	if (hipMemsetAsync(p.d_tmem, 0, 4096, c.stream) != hipSuccess) return fail(RDP_MI_ERROR_DEVICE, "TMEM clear failed");
+	if (hipStreamSynchronize(c.stream) != hipSuccess) return fail(RDP_MI_ERROR_DEVICE, "device synchronisation failed");
+	p.supported = true;
+	*out = h;
+	return 0;
+}
+
+void rdp_mi_destroy(rdp_mi_processor *h)
+{
+	if (!h) return;
+	rdp_mi_idle(h);
+	Processor &p = h->p;
+	for (auto &kv : p.timeline_events) (void)hipEventDestroy(kv.second);
+	(void)hipFree(p.d_rdram);
+	(void)hipFree(p.d_hidden);
+	(void)hipFree(p.d_tmem);
+	delete h;
+}
+
+int rdp_mi_is_supported(const rdp_mi_processor *h) { return h && h->p.supported ? 1 : 0; }
+
+void rdp_mi_enqueue_command_direct(rdp_mi_processor *h, unsigned num_words, const uint32_t *words)
+{
+	if (!h || !words || num_words == 0) return;
+	unsigned len = Frontend::command_length(words[0]);
+	if (len == 0 || len > num_words) return;
+	h->p.fe.command(words);
+	// Bound host memory: hand completed passes to the GPU once enough queue up.
+	if (h->p.fe.completed().size() >= 32) {
+		Context &c = ctx();
+		std::lock_guard<std::mutex> g(c.lock);
+		Processor *pp = &h->p;
+		(void)submit_processors(&pp, 1, false);
+	}
+}
+
+void rdp_mi_enqueue_command(rdp_mi_processor *h, unsigned num_words, const uint32_t *words)
+{
+	rdp_mi_enqueue_command_direct(h, num_words, words);
+}
+
+size_t rdp_mi_enqueue_command_list(rdp_mi_processor *h, size_t num_words, const uint32_t *words)
+{
+	if (!h || !words) return 0;
+	size_t i = 0;
+	while (i < num_words) {
+		unsigned len = Frontend::command_length(words[i]);
+		if (len == 0) {
+			i++;
+			continue;
+		}
+		if (i + len > num_words) break;
+		rdp_mi_enqueue_command_direct(h, len, words + i);
+		i += len;
+	}
+	return i;
+}
+
+void rdp_mi_set_quirks(rdp_mi_processor *h, uint32_t quirks)
+{
+	if (!h) return;
+	const uint32_t w[2] = { 4u << 24, quirks };
+	h->p.fe.command(w);
+}
+
+void rdp_mi_flush(rdp_mi_processor *h)
+{
+	if (!h) return;
+	h->p.fe.flush();
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Processor *pp = &h->p;
+	(void)submit_processors(&pp, 1, false);
+}
+
+uint64_t rdp_mi_signal_timeline(rdp_mi_processor *h)
+{
+	if (!h) return 0;
+	rdp_mi_flush(h);
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Processor &p = h->p;
+	uint64_t v = ++p.timeline;
+	hipEvent_t ev;
+	if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return v;
+	(void)hipEventRecord(ev, c.stream);
+	p.timeline_events[v] = ev;
+	return v;
+}
+
+int rdp_mi_wait_for_timeline(rdp_mi_processor *h, uint64_t value)
+{
+	if (!h) return RDP_MI_ERROR_INVALID_ARGUMENT;
+	Processor &p = h->p;
+	hipEvent_t ev = nullptr;
+	{
+		Context &c = ctx();
+		std::lock_guard<std::mutex> g(c.lock);
+		auto it = p.timeline_events.find(value);
+		if (it == p.timeline_events.end()) return 0;
+		ev = it->second;
+	}
+	HIP_CHECK(hipEventSynchronize(ev));
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	for (auto it = p.timeline_events.begin(); it != p.timeline_events.end();) {
+		if (it->first <= value) {
+			(void)hipEventDestroy(it->second);
+			it = p.timeline_events.erase(it);
+		} else
+			++it;
+	}
+	return 0;
+}
+
+int rdp_mi_idle(rdp_mi_processor *h)
+{
+	if (!h) return RDP_MI_ERROR_INVALID_ARGUMENT;
+	return rdp_mi_wait_for_timeline(h, rdp_mi_signal_timeline(h));
+}
+
+void rdp_mi_begin_frame_context(rdp_mi_processor *h) { rdp_mi_flush(h); }
+
+void *rdp_mi_begin_read_rdram(rdp_mi_processor *h)
+{
+	if (!h) return nullptr;
+	Processor &p = h->p;
+	if (copy_sync(p.host_rdram(), p.d_rdram, p.rdram_size, hipMemcpyDeviceToHost)) return nullptr;
+	return p.host_rdram();
+}
+
+void rdp_mi_end_write_rdram(rdp_mi_processor *h)
+{
+	if (!h) return;
+	rdp_mi_idle(h);
+	Processor &p = h->p;
+	(void)copy_sync(p.d_rdram, p.host_rdram(), p.rdram_size, hipMemcpyHostToDevice);
+}
+
+void *rdp_mi_begin_read_hidden_rdram(rdp_mi_processor *h)
+{
+	if (!h) return nullptr;
+	Processor &p = h->p;
+	if (copy_sync(p.hidden_mirror.data(), p.d_hidden, p.hidden_mirror.size(), hipMemcpyDeviceToHost)) return nullptr;
+	return p.hidden_mirror.data();
+}
+
+void rdp_mi_end_write_hidden_rdram(rdp_mi_processor *h)
+{
+	if (!h) return;
+	rdp_mi_idle(h);
+	Processor &p = h->p;
+	(void)copy_sync(p.d_hidden, p.hidden_mirror.data(), p.hidden_mirror.size(), hipMemcpyHostToDevice);
+}
+
+size_t rdp_mi_get_rdram_size(const rdp_mi_processor *h) { return h ? h->p.rdram_size : 0; }
+size_t rdp_mi_get_hidden_rdram_size(const rdp_mi_processor *h) { return h ? h->p.rdram_size / 2 : 0; }
+
+void *rdp_mi_get_tmem(rdp_mi_processor *h)
+{
+	if (!h) return nullptr;
+	Processor &p = h->p;
+	if (copy_sync(p.tmem_mirror, p.d_tmem, 4096, hipMemcpyDeviceToHost)) return nullptr;
+	return p.tmem_mirror;
+}
+
+void rdp_mi_set_vi_register(rdp_mi_processor *h, unsigned reg, uint32_t value)
+{
+	if (!h || reg >= RDP_MI_VI_COUNT) return;
+	h->p.vi_regs[reg] = value;
+}
+
+int rdp_mi_scanout_sync(rdp_mi_processor *h, uint32_t *out, unsigned *width, unsigned *height)
+{
+	if (!h || !out || !width || !height) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	int err = rdp_mi_idle(h);
+	if (err) return err;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	return vi_scanout(h->p, c.stream, out, width, height);
+}
+
+void rdp_mi_set_validation_callback(rdp_mi_processor *h, rdp_mi_validation_fn fn, void *user)
+{
+	if (!h) return;
+	h->p.validation_fn = fn;
+	h->p.validation_user = user;
+	if (fn)
+		h->p.fe.set_validation_callback([fn, user](uint32_t e, const char *m) { fn(e, m, user); });
+	else
+		h->p.fe.set_validation_callback(nullptr);
+}
+
+int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count)
+{
+	if (!procs && count) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor list");
+	std::vector<Processor *> ps;
+	ps.reserve(count);
+	for (unsigned i = 0; i < count; i++) {
+		if (!procs[i]) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
+		procs[i]->p.fe.flush();
+		ps.push_back(&procs[i]->p);
+	}
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	return submit_processors(ps.data(), count, true);
+}
+
+uint64_t rdp_mi_primitive_count(const rdp_mi_processor *h) { return h ? h->p.fe.primitive_counter() : 0; }
+
+}  // extern "C"
+
+namespace rdp_mi {
+// VI scanout is implemented in rdp_vi.hip; until then a blank frame.
+__attribute__((weak)) int vi_scanout(Processor &, hipStream_t, uint32_t *, unsigned *width, unsigned *height)
+{
+	*width = 0;
+	*height = 0;
+	return 0;
+}
+}  // namespace rdp_mi

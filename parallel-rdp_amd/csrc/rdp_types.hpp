@@ -1,0 +1,212 @@
+// Data layouts shared by the host front end and the gfx950 kernels.
+//
+// A render pass is the unit of GPU work: a run of primitives that target one
+// colour/depth framebuffer, plus the TMEM uploads interleaved with them.  The
+// host front end records, per pass:
+//   * the raw command words of every primitive (decoded on the GPU),
+//   * a table of resolved draw states (appended whenever a state command
+//     changed something since the previous primitive),
+//   * one PrimRecord per primitive (word offset, state index, TMEM snapshot,
+//     span-line range),
+//   * the TMEM uploads (UploadInfo) in command order.
+// Everything else (triangle decode, span setup, binning, TMEM snapshots,
+// shading, depth/blend) is computed on the device.
+#pragma once
+
+#include <stdint.h>
+
+namespace rdp_mi {
+
+// Triangle setup flags (parallel-rdp/rdp_data_structures.hpp:33-43).
+enum : uint32_t {
+	TRI_FLIP = 1u << 0,
+	TRI_DO_OFFSET = 1u << 1,
+	TRI_SKIP_XFRAC = 1u << 2,
+	TRI_INTERLACE_FIELD = 1u << 3,
+	TRI_INTERLACE_KEEP_ODD = 1u << 4,
+	TRI_DISABLE_UPSCALING = 1u << 5,
+	TRI_NATIVE_LOD = 1u << 6,
+	TRI_FILL_COPY_RASTER = 1u << 7,
+};
+
+// Rasterisation flags; bit positions follow rdp_data_structures.hpp:45-78.
+enum : uint32_t {
+	RAS_INTERLACE_FIELD = 1u << 0,
+	RAS_INTERLACE_KEEP_ODD = 1u << 1,
+	RAS_AA = 1u << 2,
+	RAS_PERSPECTIVE = 1u << 3,
+	RAS_TLUT = 1u << 4,
+	RAS_TLUT_TYPE = 1u << 5,
+	RAS_CVG_TIMES_ALPHA = 1u << 6,
+	RAS_ALPHA_CVG_SELECT = 1u << 7,
+	RAS_MULTI_CYCLE = 1u << 8,
+	RAS_TEX_LOD = 1u << 9,
+	RAS_SHARPEN_LOD = 1u << 10,
+	RAS_DETAIL_LOD = 1u << 11,
+	RAS_FILL = 1u << 12,
+	RAS_COPY = 1u << 13,
+	RAS_SAMPLE_QUAD = 1u << 14,
+	RAS_ALPHA_TEST = 1u << 15,
+	RAS_ALPHA_TEST_DITHER = 1u << 16,
+	RAS_MID_TEXEL = 1u << 17,
+	RAS_USES_TEXEL0 = 1u << 18,
+	RAS_USES_TEXEL1 = 1u << 19,
+	RAS_USES_LOD = 1u << 20,
+	RAS_USES_PIPELINED_TEXEL1 = 1u << 21,
+	RAS_CONVERT_ONE = 1u << 22,
+	RAS_BILERP0 = 1u << 23,
+	RAS_BILERP1 = 1u << 24,
+	RAS_NEED_NOISE_DUAL = 1u << 25,
+	RAS_NEED_NOISE = 1u << 28,
+};
+
+// Depth/blend flags (rdp_data_structures.hpp:80-90).
+enum : uint32_t {
+	DB_DEPTH_TEST = 1u << 0,
+	DB_DEPTH_UPDATE = 1u << 1,
+	DB_FORCE_BLEND = 1u << 3,
+	DB_IMAGE_READ = 1u << 4,
+	DB_COLOR_ON_CVG = 1u << 5,
+	DB_MULTI_CYCLE = 1u << 6,
+	DB_AA = 1u << 7,
+	DB_DITHER = 1u << 8,
+};
+
+enum : uint32_t { FB_I4 = 0, FB_I8 = 1, FB_RGBA5551 = 2, FB_IA88 = 3, FB_RGBA8888 = 4 };
+enum : uint32_t { TF_RGBA = 0, TF_YUV = 1, TF_CI = 2, TF_IA = 3, TF_I = 4 };
+enum : uint32_t { UP_TILE = 0, UP_TLUT = 1, UP_BLOCK = 2 };
+enum : uint32_t { TILE_CLAMP_S = 1, TILE_MIRROR_S = 2, TILE_CLAMP_T = 4, TILE_MIRROR_T = 8 };
+
+constexpr int kTileSize = 8;                 // 8x8 pixel screen tile = one wave64
+constexpr int kMaxPrimsPerPass = 4096;       // 64 bin groups x 64 primitives
+constexpr int kBinGroups = kMaxPrimsPerPass / 64;
+constexpr int kMaxUploadsPerPass = 255;      // TMEM snapshots per pass (u8 index)
+constexpr int kMaxWidth = 1024;
+constexpr int kMaxHeight = 1024;
+constexpr int kMaxTilesX = kMaxWidth / kTileSize;
+constexpr int kMaxTilesY = kMaxHeight / kTileSize;
+
+// Texture tile descriptor (rdp_common.hpp:263-288), 32 bytes.
+struct TileDesc {
+	uint32_t slo, shi, tlo, thi;
+	uint32_t offset, stride;
+	uint8_t fmt, size, palette, mask_s;
+	uint8_t shift_s, mask_t, shift_t, flags;
+};
+static_assert(sizeof(TileDesc) == 32, "TileDesc layout");
+
+// Resolved draw state, snapshotted by the host whenever any state command
+// ran since the previous primitive.  Combiner constants are already resolved
+// per selector the way Renderer::build_combiner_constants does it.
+struct DrawState {
+	uint32_t scissor[4];              // xlo, ylo, xhi, yhi (10.2)
+	uint32_t ras_flags;               // normalised static flags incl. USES_* / NEED_NOISE*
+	uint32_t dither;                  // rgb mode << 2 | alpha mode
+	uint32_t db_flags;
+	uint32_t coverage_mode;
+	uint32_t z_mode;
+	uint8_t comb[2][8];               // per cycle: rgb muladd, mulsub, mul, add, alpha muladd, mulsub, mul, add
+	uint8_t blend[2][4];              // per cycle: 1a, 1b, 2a, 2b
+	uint8_t constants[2][4][4];       // per cycle: muladd, mulsub, mul, add constant RGBA
+	uint8_t fog[4], blendc[4];
+	uint32_t fill_color;
+	int32_t factors[4];               // convert K0..K3 as 2*sext9(k)+1
+	uint32_t min_lod;
+	uint32_t use_prim_depth;
+	int32_t prim_depth;               // already << 16
+	uint32_t prim_dz;
+	uint32_t quirks;
+	uint32_t raw_ras_flags;           // static flags before normalisation (interlace bits)
+	uint32_t pad[2];
+	TileDesc tiles[8];
+};
+
+// One primitive of a pass.
+struct PrimRecord {
+	uint32_t word_offset;   // into the pass word buffer
+	uint16_t state_index;   // into the pass draw-state table
+	uint8_t tmem_instance;  // TMEM snapshot index (0 = state at pass start)
+	uint8_t opcode;
+	int32_t ylo, yhi;       // active scanline range (yhi < ylo: nothing to draw)
+	uint32_t span_offset;   // first SpanSetup record (lines ylo .. yhi+1)
+	uint32_t tilerow_offset;// first tile-row range record (rows ylo>>3 .. yhi>>3)
+	uint32_t prim_index;    // stream-global primitive counter (noise seed)
+	uint32_t pad;
+};
+static_assert(sizeof(PrimRecord) == 32, "PrimRecord layout");
+
+// Decoded per-primitive setup written by the setup kernel.
+struct PrimSetup {
+	int32_t xh, xm, xl;
+	int32_t dxhdy, dxmdy, dxldy;
+	int32_t yh, ym, yl;
+	uint32_t flags;
+	uint32_t tile;
+	uint32_t state_index;
+	int32_t rgba[4], drgba_dx[4], drgba_de[4], drgba_dy[4];
+	int32_t stzw[4], dstzw_dx[4], dstzw_de[4], dstzw_dy[4];
+	int32_t dz, dz_compressed;
+	uint32_t tmem_instance;
+	uint32_t prim_index;
+	int32_t ylo, yhi;
+	uint32_t span_offset;
+	uint32_t pad;
+};
+
+// One scanline of one primitive (span_setup.comp output), 64 bytes.
+struct SpanSetup {
+	int32_t rgba[4];
+	int32_t stzw[4];
+	uint16_t xleft[4];
+	uint16_t xright[4];
+	int32_t base_x;
+	int32_t start_x;
+	int32_t end_x;
+	int16_t lodlength;
+	uint16_t valid;
+};
+static_assert(sizeof(SpanSetup) == 64, "SpanSetup layout");
+
+// TMEM upload descriptor (rdp_data_structures.hpp:182-202), with the float
+// reciprocals replaced by the integer divisors they stand for.
+struct UploadInfo {
+	int32_t width, height;
+	int32_t min_line_stride, max_line_stride;
+	int32_t vram_addr, vram_width, vram_size, vram_effective_width;
+	int32_t tmem_offset, tmem_stride_words, tmem_size, tmem_fmt;
+	int32_t mode, dxt;
+	int32_t pad[2];
+};
+static_assert(sizeof(UploadInfo) == 64, "UploadInfo layout");
+
+// Per-pass device descriptor.  All pointers are device addresses.
+struct PassDesc {
+	// inputs
+	const uint32_t *words;
+	const DrawState *states;
+	const PrimRecord *prims;
+	const UploadInfo *uploads;
+	// per-pass scratch / outputs
+	PrimSetup *setups;
+	SpanSetup *spans;
+	uint32_t *tilerows;        // per (prim, tile row): lo tile x | hi tile x << 16, 0xffff lo = empty
+	uint64_t *bin_fine;        // [tile][group] primitive masks
+	uint64_t *bin_coarse;      // [tile] group masks
+	uint16_t *tmem_instances;  // (num_uploads + 1) x 2048 halfwords
+	// stream memory
+	uint8_t *rdram;
+	uint8_t *hidden;
+	uint16_t *tmem;            // persistent TMEM of the stream (2048 halfwords)
+	const uint8_t *blender_lut;
+	uint32_t rdram_mask;       // rdram_size - 1
+	uint32_t num_prims;
+	uint32_t num_uploads;
+	uint32_t fb_addr, fb_width, fb_height, fb_fmt, depth_addr;
+	uint32_t tiles_x, tiles_y;
+	uint32_t num_spans;
+	uint32_t num_tilerows;
+	uint32_t num_groups;       // ceil(num_prims / 64)
+	uint32_t pad[2];
+};
+
+}  // namespace rdp_mi

@@ -1,0 +1,212 @@
+"""Python binding of librdp_mi.so, shaped like RDP::CommandProcessor.
+
+The methods mirror the reference's surface (parallel-rdp/rdp_device.hpp:111-198):
+enqueue_command, flush, signal_timeline, wait_for_timeline, idle,
+begin_frame_context, set_vi_register, scanout_sync, begin_read_rdram, ...
+
+The library must be built (python parallel-rdp_amd/build.py).  Loading
+fails loudly when it is missing; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librdp_mi.so")
+
+_lib = None
+
+VALIDATION_FN = ctypes.CFUNCTYPE(None, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_void_p)
+
+
+class RDPError(RuntimeError):
+    pass
+
+
+def load_library():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RDPError(f"{LIB_PATH} is not built; run python parallel-rdp_amd/build.py")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, sz, cu = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint
+    lib.rdp_mi_create.restype = ctypes.c_int
+    lib.rdp_mi_create.argtypes = [ctypes.POINTER(vp), vp, sz, sz, sz, u32]
+    lib.rdp_mi_destroy.argtypes = [vp]
+    lib.rdp_mi_is_supported.restype = ctypes.c_int
+    lib.rdp_mi_is_supported.argtypes = [vp]
+    lib.rdp_mi_enqueue_command.argtypes = [vp, cu, vp]
+    lib.rdp_mi_enqueue_command_direct.argtypes = [vp, cu, vp]
+    lib.rdp_mi_enqueue_command_list.restype = sz
+    lib.rdp_mi_enqueue_command_list.argtypes = [vp, sz, vp]
+    lib.rdp_mi_set_quirks.argtypes = [vp, u32]
+    lib.rdp_mi_flush.argtypes = [vp]
+    lib.rdp_mi_signal_timeline.restype = u64
+    lib.rdp_mi_signal_timeline.argtypes = [vp]
+    lib.rdp_mi_wait_for_timeline.restype = ctypes.c_int
+    lib.rdp_mi_wait_for_timeline.argtypes = [vp, u64]
+    lib.rdp_mi_idle.restype = ctypes.c_int
+    lib.rdp_mi_idle.argtypes = [vp]
+    lib.rdp_mi_begin_frame_context.argtypes = [vp]
+    for name in ("rdp_mi_begin_read_rdram", "rdp_mi_begin_read_hidden_rdram", "rdp_mi_get_tmem"):
+        getattr(lib, name).restype = vp
+        getattr(lib, name).argtypes = [vp]
+    lib.rdp_mi_end_write_rdram.argtypes = [vp]
+    lib.rdp_mi_end_write_hidden_rdram.argtypes = [vp]
+    lib.rdp_mi_get_rdram_size.restype = sz
+    lib.rdp_mi_get_rdram_size.argtypes = [vp]
+    lib.rdp_mi_get_hidden_rdram_size.restype = sz
+    lib.rdp_mi_get_hidden_rdram_size.argtypes = [vp]
+    lib.rdp_mi_set_vi_register.argtypes = [vp, cu, u32]
+    lib.rdp_mi_scanout_sync.restype = ctypes.c_int
+    lib.rdp_mi_scanout_sync.argtypes = [vp, vp, ctypes.POINTER(cu), ctypes.POINTER(cu)]
+    lib.rdp_mi_set_validation_callback.argtypes = [vp, VALIDATION_FN, vp]
+    lib.rdp_mi_batch_flush.restype = ctypes.c_int
+    lib.rdp_mi_batch_flush.argtypes = [ctypes.POINTER(vp), cu]
+    lib.rdp_mi_last_error.restype = ctypes.c_char_p
+    lib.rdp_mi_last_error.argtypes = []
+    lib.rdp_mi_primitive_count.restype = u64
+    lib.rdp_mi_primitive_count.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load_library().rdp_mi_last_error().decode(errors="replace")
+
+
+def _check(ret: int, what: str):
+    if ret != 0:
+        raise RDPError(f"{what} failed ({ret}): {last_error()}")
+
+
+class CommandProcessor:
+    """One emulated RDP with device-resident RDRAM, hidden RDRAM and TMEM."""
+
+    def __init__(self, rdram_size: int = 8 << 20, flags: int = 0):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.rdp_mi_create(ctypes.byref(h), None, 0, rdram_size, rdram_size // 2, flags), "rdp_mi_create")
+        self._h = h
+        self.rdram_size = rdram_size
+        self._validation_cb = None
+        self.validation_errors = []
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            self._lib.rdp_mi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_is_supported(self) -> bool:
+        return bool(self._lib.rdp_mi_is_supported(self._h))
+
+    # -- commands ---------------------------------------------------------
+    def enqueue_command(self, words: Sequence[int]):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        self._lib.rdp_mi_enqueue_command(self._h, w.size, w.ctypes.data)
+
+    def enqueue_commands(self, words: np.ndarray) -> int:
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        return int(self._lib.rdp_mi_enqueue_command_list(self._h, w.size, w.ctypes.data))
+
+    def set_quirks(self, quirks: int):
+        self._lib.rdp_mi_set_quirks(self._h, quirks)
+
+    # -- synchronisation --------------------------------------------------
+    def flush(self):
+        self._lib.rdp_mi_flush(self._h)
+
+    def signal_timeline(self) -> int:
+        return int(self._lib.rdp_mi_signal_timeline(self._h))
+
+    def wait_for_timeline(self, value: int):
+        _check(self._lib.rdp_mi_wait_for_timeline(self._h, value), "wait_for_timeline")
+
+    def idle(self):
+        _check(self._lib.rdp_mi_idle(self._h), "idle")
+
+    def begin_frame_context(self):
+        self._lib.rdp_mi_begin_frame_context(self._h)
+
+    # -- memory -----------------------------------------------------------
+    def read_rdram(self) -> np.ndarray:
+        """Copy of RDRAM as host-endian uint32 words (valid after idle())."""
+        p = self._lib.rdp_mi_begin_read_rdram(self._h)
+        if not p:
+            raise RDPError("begin_read_rdram failed: " + last_error())
+        buf = (ctypes.c_uint8 * self.rdram_size).from_address(p)
+        return np.frombuffer(buf, dtype=np.uint32).copy()
+
+    def read_hidden_rdram(self) -> np.ndarray:
+        p = self._lib.rdp_mi_begin_read_hidden_rdram(self._h)
+        if not p:
+            raise RDPError("begin_read_hidden_rdram failed: " + last_error())
+        buf = (ctypes.c_uint8 * (self.rdram_size // 2)).from_address(p)
+        return np.frombuffer(buf, dtype=np.uint8).copy()
+
+    def write_rdram(self, words: np.ndarray):
+        p = self._lib.rdp_mi_begin_read_rdram(self._h)
+        if not p:
+            raise RDPError("begin_read_rdram failed: " + last_error())
+        buf = (ctypes.c_uint8 * self.rdram_size).from_address(p)
+        np.frombuffer(buf, dtype=np.uint32)[:] = np.asarray(words, dtype=np.uint32)
+        self._lib.rdp_mi_end_write_rdram(self._h)
+
+    def write_hidden_rdram(self, data: np.ndarray):
+        p = self._lib.rdp_mi_begin_read_hidden_rdram(self._h)
+        if not p:
+            raise RDPError("begin_read_hidden_rdram failed: " + last_error())
+        buf = (ctypes.c_uint8 * (self.rdram_size // 2)).from_address(p)
+        np.frombuffer(buf, dtype=np.uint8)[:] = np.asarray(data, dtype=np.uint8)
+        self._lib.rdp_mi_end_write_hidden_rdram(self._h)
+
+    def read_tmem(self) -> np.ndarray:
+        p = self._lib.rdp_mi_get_tmem(self._h)
+        if not p:
+            raise RDPError("get_tmem failed: " + last_error())
+        return np.frombuffer((ctypes.c_uint16 * 2048).from_address(p), dtype=np.uint16).copy()
+
+    # -- VI ---------------------------------------------------------------
+    def set_vi_register(self, reg: int, value: int):
+        self._lib.rdp_mi_set_vi_register(self._h, reg, value & 0xFFFFFFFF)
+
+    def scanout_sync(self) -> np.ndarray:
+        out = np.zeros(640 * 576, dtype=np.uint32)
+        w = ctypes.c_uint(0)
+        h = ctypes.c_uint(0)
+        _check(self._lib.rdp_mi_scanout_sync(self._h, out.ctypes.data, ctypes.byref(w), ctypes.byref(h)), "scanout")
+        if not w.value or not h.value:
+            return out[:0].reshape(0, 0)
+        return out[: w.value * h.value].reshape(h.value, w.value)
+
+    # -- validation -------------------------------------------------------
+    def set_validation_callback(self, record: bool = True):
+        def cb(err, msg, user):
+            self.validation_errors.append((int(err), msg.decode(errors="replace") if msg else ""))
+        self._validation_cb = VALIDATION_FN(cb) if record else VALIDATION_FN()
+        self._lib.rdp_mi_set_validation_callback(self._h, self._validation_cb, None)
+
+    @property
+    def primitive_count(self) -> int:
+        return int(self._lib.rdp_mi_primitive_count(self._h))
+
+
+def batch_flush(processors: Sequence[CommandProcessor]):
+    lib = load_library()
+    arr = (ctypes.c_void_p * len(processors))(*[p.handle.value for p in processors])
+    _check(lib.rdp_mi_batch_flush(arr, len(processors)), "batch_flush")

@@ -1,0 +1,591 @@
+"""Seeded RDP command-stream builder for the hot-path workloads.
+
+Plays the role of the reference's CommandBuilder and triangle converter
+(rdp_command_builder.cpp:278-419, triangle_converter.cpp:129-273): it turns
+screen-space vertices into raw RDP command words.  Triangle setup is
+vectorised over whole batches with numpy; edge setup is integer-exact (quarter
+pixel vertices, round-away-from-zero slopes masked with ~7, sub-scanline
+start fixup) and attribute gradients follow the same plane equations,
+quantised to the RDP's 16.16 formats.
+
+Everything here produces input data only.  Rendering semantics live in the
+library and in the oracle.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+# ----------------------------------------------------------------------------
+# Opcodes (rdp_common.hpp:37-81)
+# ----------------------------------------------------------------------------
+OP_FILL_TRIANGLE = 0x08
+OP_SHADE_TRIANGLE = 0x0C
+OP_SHADE_Z_TRIANGLE = 0x0D
+OP_SHADE_TEX_TRIANGLE = 0x0E
+OP_SHADE_TEX_Z_TRIANGLE = 0x0F
+OP_TEX_Z_TRIANGLE = 0x0B
+OP_TEXTURE_RECTANGLE = 0x24
+OP_TEXTURE_RECTANGLE_FLIP = 0x25
+OP_SYNC_LOAD = 0x26
+OP_SYNC_PIPE = 0x27
+OP_SYNC_TILE = 0x28
+OP_SYNC_FULL = 0x29
+OP_SET_CONVERT = 0x2C
+OP_SET_SCISSOR = 0x2D
+OP_SET_PRIM_DEPTH = 0x2E
+OP_SET_OTHER_MODES = 0x2F
+OP_LOAD_TLUT = 0x30
+OP_SET_TILE_SIZE = 0x32
+OP_LOAD_BLOCK = 0x33
+OP_LOAD_TILE = 0x34
+OP_SET_TILE = 0x35
+OP_FILL_RECTANGLE = 0x36
+OP_SET_FILL_COLOR = 0x37
+OP_SET_FOG_COLOR = 0x38
+OP_SET_BLEND_COLOR = 0x39
+OP_SET_PRIM_COLOR = 0x3A
+OP_SET_ENV_COLOR = 0x3B
+OP_SET_COMBINE = 0x3C
+OP_SET_TEXTURE_IMAGE = 0x3D
+OP_SET_MASK_IMAGE = 0x3E
+OP_SET_COLOR_IMAGE = 0x3F
+
+# Texture formats / sizes (rdp_common.hpp:176-190)
+FMT_RGBA, FMT_YUV, FMT_CI, FMT_IA, FMT_I = 0, 1, 2, 3, 4
+SIZ_4, SIZ_8, SIZ_16, SIZ_32 = 0, 1, 2, 3
+
+# VI registers (rdp_common.hpp:324-341)
+VI_CONTROL, VI_ORIGIN, VI_WIDTH, VI_INTR, VI_V_CURRENT_LINE, VI_TIMING, VI_V_SYNC, \
+    VI_H_SYNC, VI_LEAP, VI_H_START, VI_V_START, VI_V_BURST, VI_X_SCALE, VI_Y_SCALE = range(14)
+
+RDRAM_SIZE = 8 * 1024 * 1024
+
+COMMAND_LENGTHS = np.zeros(64, dtype=np.int32)
+COMMAND_LENGTHS[0] = 2
+COMMAND_LENGTHS[0x08:0x10] = [8, 12, 24, 28, 24, 28, 40, 44]
+COMMAND_LENGTHS[0x24:0x40] = 2
+COMMAND_LENGTHS[0x24] = 4
+COMMAND_LENGTHS[0x25] = 4
+COMMAND_LENGTHS[0x31] = 0
+
+
+def _u32(v) -> int:
+    return int(v) & 0xFFFFFFFF
+
+
+@dataclasses.dataclass
+class OtherModes:
+    """Fields of SetOtherModes (rdp_command_builder.cpp:143-205 layout)."""
+    cycle_type: int = 0          # 0 1-cycle, 1 2-cycle, 2 copy, 3 fill
+    perspective: bool = False
+    detail_lod: bool = False
+    sharpen_lod: bool = False
+    tex_lod: bool = False
+    tlut: bool = False
+    tlut_ia: bool = False
+    sample_quad: bool = False
+    mid_texel: bool = False
+    bilerp0: bool = False
+    bilerp1: bool = False
+    convert_one: bool = False
+    rgb_dither: int = 3          # 0 magic, 1 bayer, 2 noise, 3 off
+    alpha_dither: int = 3        # 0 pattern, 1 inv pattern, 2 noise, 3 off
+    blend: Sequence[Sequence[int]] = ((0, 0, 0, 0), (0, 0, 0, 0))   # per cycle: 1a, 1b, 2a, 2b
+    force_blend: bool = False
+    alpha_cvg_select: bool = False
+    cvg_times_alpha: bool = False
+    z_mode: int = 0
+    coverage_mode: int = 0
+    color_on_coverage: bool = False
+    image_read: bool = False
+    depth_write: bool = False
+    depth_test: bool = False
+    aa: bool = False
+    prim_depth: bool = False
+    alpha_test_dither: bool = False
+    alpha_test: bool = False
+
+    def words(self) -> List[int]:
+        w0 = OP_SET_OTHER_MODES << 24
+        w0 |= (self.cycle_type & 3) << 20
+        for bit, flag in ((19, self.perspective), (18, self.detail_lod), (17, self.sharpen_lod),
+                          (16, self.tex_lod), (15, self.tlut), (14, self.tlut_ia), (13, self.sample_quad),
+                          (12, self.mid_texel), (11, self.bilerp0), (10, self.bilerp1), (9, self.convert_one)):
+            if flag:
+                w0 |= 1 << bit
+        w0 |= (self.rgb_dither & 3) << 6
+        w0 |= (self.alpha_dither & 3) << 4
+        b0, b1 = self.blend
+        w1 = ((b0[0] & 3) << 30) | ((b1[0] & 3) << 28) | ((b0[1] & 3) << 26) | ((b1[1] & 3) << 24)
+        w1 |= ((b0[2] & 3) << 22) | ((b1[2] & 3) << 20) | ((b0[3] & 3) << 18) | ((b1[3] & 3) << 16)
+        for bit, flag in ((14, self.force_blend), (13, self.alpha_cvg_select), (12, self.cvg_times_alpha),
+                          (7, self.color_on_coverage), (6, self.image_read), (5, self.depth_write),
+                          (4, self.depth_test), (3, self.aa), (2, self.prim_depth),
+                          (1, self.alpha_test_dither), (0, self.alpha_test)):
+            if flag:
+                w1 |= 1 << bit
+        w1 |= (self.z_mode & 3) << 10
+        w1 |= (self.coverage_mode & 3) << 8
+        return [_u32(w0), _u32(w1)]
+
+
+def combine_words(c0_rgb, c0_alpha, c1_rgb, c1_alpha) -> List[int]:
+    """SetCombine from (muladd, mulsub, mul, add) selectors per cycle.
+
+    Bit layout inverts rdp_device.cpp:661-684."""
+    a0, b0, c0, d0 = c0_rgb
+    aa0, ab0, ac0, ad0 = c0_alpha
+    a1, b1, c1, d1 = c1_rgb
+    aa1, ab1, ac1, ad1 = c1_alpha
+    w0 = OP_SET_COMBINE << 24
+    w0 |= (a0 & 0xF) << 20 | (c0 & 0x1F) << 15 | (aa0 & 7) << 12 | (ac0 & 7) << 9 | (a1 & 0xF) << 5 | (c1 & 0x1F)
+    w1 = (b0 & 0xF) << 28 | (b1 & 0xF) << 24 | (aa1 & 7) << 21 | (ac1 & 7) << 18 | (d0 & 7) << 15
+    w1 |= (ab0 & 7) << 12 | (ad0 & 7) << 9 | (d1 & 7) << 6 | (ab1 & 7) << 3 | (ad1 & 7)
+    return [_u32(w0), _u32(w1)]
+
+
+# Combiner selector values (rdp_common.hpp:83-174).
+RGB_COMBINED, RGB_TEXEL0, RGB_TEXEL1, RGB_PRIM, RGB_SHADE, RGB_ENV = 0, 1, 2, 3, 4, 5
+MULADD_ONE, MULADD_NOISE, MULADD_ZERO = 6, 7, 8
+MULSUB_KEY_CENTER, MULSUB_K4, MULSUB_ZERO = 6, 7, 8
+MUL_KEY_SCALE, MUL_COMBINED_ALPHA, MUL_TEXEL0_ALPHA, MUL_TEXEL1_ALPHA = 6, 7, 8, 9
+MUL_PRIM_ALPHA, MUL_SHADE_ALPHA, MUL_ENV_ALPHA, MUL_LOD_FRAC, MUL_PRIM_LOD_FRAC, MUL_K5, MUL_ZERO = \
+    10, 11, 12, 13, 14, 15, 16
+ADD_ONE, ADD_ZERO = 6, 7
+A_COMBINED, A_TEXEL0, A_TEXEL1, A_PRIM, A_SHADE, A_ENV, A_ONE, A_ZERO = range(8)
+AMUL_LOD_FRAC, AMUL_PRIM_LOD_FRAC, AMUL_ZERO = 0, 6, 7
+
+
+class CommandList:
+    """Accumulates raw RDP command words."""
+
+    def __init__(self):
+        self._chunks: List[np.ndarray] = []
+        self._pending: List[int] = []
+
+    def _flush_pending(self):
+        if self._pending:
+            self._chunks.append(np.asarray(self._pending, dtype=np.uint32))
+            self._pending = []
+
+    def raw(self, words: Sequence[int]):
+        self._pending.extend(_u32(w) for w in words)
+
+    def extend_array(self, arr: np.ndarray):
+        self._flush_pending()
+        self._chunks.append(np.ascontiguousarray(arr, dtype=np.uint32).ravel())
+
+    def words(self) -> np.ndarray:
+        self._flush_pending()
+        if not self._chunks:
+            return np.zeros(0, dtype=np.uint32)
+        return np.concatenate(self._chunks)
+
+    # --- state commands ----------------------------------------------------
+    def set_color_image(self, addr: int, width: int, size: int = SIZ_16, fmt: int = FMT_RGBA):
+        self.raw([OP_SET_COLOR_IMAGE << 24 | (fmt & 7) << 21 | (size & 3) << 19 | ((width - 1) & 1023),
+                  addr & 0xFFFFFF])
+
+    def set_mask_image(self, addr: int):
+        self.raw([OP_SET_MASK_IMAGE << 24, addr & 0xFFFFFF])
+
+    def set_scissor(self, xlo: int, ylo: int, xhi: int, yhi: int, field: bool = False, keep_odd: bool = False):
+        """Edges in 10.2 fixed point."""
+        w1 = (xhi & 0xFFF) << 12 | (yhi & 0xFFF)
+        if field:
+            w1 |= 1 << 25
+        if keep_odd:
+            w1 |= 1 << 24
+        self.raw([OP_SET_SCISSOR << 24 | (xlo & 0xFFF) << 12 | (ylo & 0xFFF), w1])
+
+    def set_other_modes(self, modes: OtherModes):
+        self.raw(modes.words())
+
+    def set_combine(self, c0_rgb, c0_alpha, c1_rgb=None, c1_alpha=None):
+        if c1_rgb is None:
+            c1_rgb, c1_alpha = c0_rgb, c0_alpha
+        self.raw(combine_words(c0_rgb, c0_alpha, c1_rgb, c1_alpha))
+
+    def set_fill_color(self, color: int):
+        self.raw([OP_SET_FILL_COLOR << 24, color])
+
+    def set_fog_color(self, color: int):
+        self.raw([OP_SET_FOG_COLOR << 24, color])
+
+    def set_blend_color(self, color: int):
+        self.raw([OP_SET_BLEND_COLOR << 24, color])
+
+    def set_prim_color(self, color: int, min_level: int = 0, lod_frac: int = 0):
+        self.raw([OP_SET_PRIM_COLOR << 24 | (min_level & 31) << 8 | (lod_frac & 0xFF), color])
+
+    def set_env_color(self, color: int):
+        self.raw([OP_SET_ENV_COLOR << 24, color])
+
+    def set_prim_depth(self, z: int, dz: int):
+        self.raw([OP_SET_PRIM_DEPTH << 24, (z & 0xFFFF) << 16 | (dz & 0xFFFF)])
+
+    def set_convert(self, k: Sequence[int]):
+        m = 0
+        for i, v in enumerate(k):
+            m |= (v & 0x1FF) << (45 - 9 * i)
+        self.raw([OP_SET_CONVERT << 24 | (m >> 32) & 0xFFFFFF, m & 0xFFFFFFFF])
+
+    def sync_full(self):
+        self.raw([OP_SYNC_FULL << 24, 0])
+
+    def sync_pipe(self):
+        self.raw([OP_SYNC_PIPE << 24, 0])
+
+    def sync_load(self):
+        self.raw([OP_SYNC_LOAD << 24, 0])
+
+    def sync_tile(self):
+        self.raw([OP_SYNC_TILE << 24, 0])
+
+    # --- texture commands --------------------------------------------------
+    def set_texture_image(self, addr: int, width: int, size: int, fmt: int = FMT_RGBA):
+        self.raw([OP_SET_TEXTURE_IMAGE << 24 | (fmt & 7) << 21 | (size & 3) << 19 | ((width - 1) & 0x3FF),
+                  addr & 0xFFFFFF])
+
+    def set_tile(self, tile: int, fmt: int, size: int, tmem_offset: int, stride: int, palette: int = 0,
+                 mask_s: int = 0, shift_s: int = 0, mask_t: int = 0, shift_t: int = 0,
+                 clamp_s: bool = False, mirror_s: bool = False, clamp_t: bool = False, mirror_t: bool = False):
+        """tmem_offset and stride in bytes (multiples of 8)."""
+        w0 = OP_SET_TILE << 24 | (fmt & 7) << 21 | (size & 3) << 19 | ((stride >> 3) & 511) << 9 | ((tmem_offset >> 3) & 511)
+        w1 = (tile & 7) << 24 | (palette & 15) << 20 | int(clamp_t) << 19 | int(mirror_t) << 18
+        w1 |= (mask_t & 15) << 14 | (shift_t & 15) << 10 | int(clamp_s) << 9 | int(mirror_s) << 8
+        w1 |= (mask_s & 15) << 4 | (shift_s & 15)
+        self.raw([w0, w1])
+
+    def _tile_cmd(self, op: int, tile: int, s0: int, t0: int, s1: int, t1: int):
+        self.raw([op << 24 | (s0 & 0xFFF) << 12 | (t0 & 0xFFF), (tile & 7) << 24 | (s1 & 0xFFF) << 12 | (t1 & 0xFFF)])
+
+    def load_tile(self, tile: int, s0: int, t0: int, s1: int, t1: int):
+        """Coordinates in 10.2 fixed point."""
+        self._tile_cmd(OP_LOAD_TILE, tile, s0, t0, s1, t1)
+
+    def load_block(self, tile: int, s0: int, t0: int, s1: int, dxt: int):
+        self._tile_cmd(OP_LOAD_BLOCK, tile, s0, t0, s1, dxt)
+
+    def load_tlut(self, tile: int, s0: int, t0: int, s1: int, t1: int):
+        self._tile_cmd(OP_LOAD_TLUT, tile, s0, t0, s1, t1)
+
+    def set_tile_size(self, tile: int, s0: int, t0: int, s1: int, t1: int):
+        self._tile_cmd(OP_SET_TILE_SIZE, tile, s0, t0, s1, t1)
+
+    # --- primitives --------------------------------------------------------
+    def fill_rectangle(self, x0: int, y0: int, x1: int, y1: int):
+        """Upper-left (x0, y0) and lower-right (x1, y1) in 10.2 fixed point."""
+        self.raw([OP_FILL_RECTANGLE << 24 | (x1 & 0xFFF) << 12 | (y1 & 0xFFF), (x0 & 0xFFF) << 12 | (y0 & 0xFFF)])
+
+    def texture_rectangle(self, tile: int, x0: int, y0: int, x1: int, y1: int, s: int, t: int, dsdx: int, dtdy: int,
+                          flip: bool = False):
+        op = OP_TEXTURE_RECTANGLE_FLIP if flip else OP_TEXTURE_RECTANGLE
+        self.raw([op << 24 | (x1 & 0xFFF) << 12 | (y1 & 0xFFF), (tile & 7) << 24 | (x0 & 0xFFF) << 12 | (y0 & 0xFFF),
+                  (s & 0xFFFF) << 16 | (t & 0xFFFF), (dsdx & 0xFFFF) << 16 | (dtdy & 0xFFFF)])
+
+
+# ----------------------------------------------------------------------------
+# Triangle setup
+# ----------------------------------------------------------------------------
+def _round_away_div(num: np.ndarray, den: np.ndarray) -> np.ndarray:
+    """Integer division rounding away from zero (triangle_converter.cpp:121-130)."""
+    mag = (np.abs(num) + den - 1) // den
+    return np.where(num < 0, -mag, mag)
+
+
+def _pack_pair(a: np.ndarray, b: np.ndarray):
+    """Two 16.16 values -> (integer-halves word, fraction-halves word)."""
+    a = a.astype(np.int64) & 0xFFFFFFFF
+    b = b.astype(np.int64) & 0xFFFFFFFF
+    hi = (a & 0xFFFF0000) | ((b >> 16) & 0xFFFF)
+    lo = ((a << 16) & 0xFFFF0000) | (b & 0xFFFF)
+    return hi, lo
+
+
+def triangle_commands(xy_q: np.ndarray, rgba: Optional[np.ndarray] = None, z: Optional[np.ndarray] = None,
+                      uv: Optional[np.ndarray] = None, w: Optional[np.ndarray] = None, perspective: bool = False,
+                      tile: int = 0, max_level: int = 0) -> np.ndarray:
+    """Build triangle commands for a batch.
+
+    xy_q:  (N, 3, 2) integer vertex positions in quarter pixels.
+    rgba:  (N, 3, 4) float colour in [0, 255] (shade triangles).
+    z:     (N, 3) float depth in [0, 0x3ffff] (z-buffered triangles).
+    uv:    (N, 3, 2) float texel coordinates (textured triangles).
+    w:     (N, 3) float clip w > 0, only with perspective=True.
+
+    Returns a (M, L) uint32 array, one command per row, where degenerate
+    (zero-area) triangles have been dropped.
+    """
+    xy_q = np.asarray(xy_q, dtype=np.int64)
+    n = xy_q.shape[0]
+    shade = rgba is not None
+    zbuf = z is not None
+    tex = uv is not None
+    op = OP_FILL_TRIANGLE | (4 if shade else 0) | (2 if tex else 0) | (1 if zbuf else 0)
+    length = int(COMMAND_LENGTHS[op])
+
+    xs = xy_q[:, :, 0]
+    ys = xy_q[:, :, 1]
+    # Sort vertices by Y, tie-break on X (triangle_converter.cpp:145-158).
+    key = ys * 65536 + (xs + 32768)
+    order = np.argsort(key, axis=1, kind="stable")
+    ar = np.arange(n)[:, None]
+    xs = xs[ar, order]
+    ys = ys[ar, order]
+    xa, xb, xc = xs[:, 0], xs[:, 1], xs[:, 2]
+    y_lo, y_mid, y_hi = ys[:, 0], ys[:, 1], ys[:, 2]
+
+    # Signed area on sorted vertices; zero area is dropped.
+    ab_x, bc_x, ca_x = xb - xa, xc - xb, xa - xc
+    ab_y, bc_y, ca_y = y_mid - y_lo, y_hi - y_mid, y_lo - y_hi
+    area = ab_x * bc_y - ab_y * bc_x
+    keep = area != 0
+
+    x_a = xa << 14
+    x_b = xa << 14
+    x_c = xb << 14
+    dxdy_a = _round_away_div((xc - xa) << 16, np.maximum(1, y_hi - y_lo)) & ~7
+    dxdy_b = _round_away_div((xb - xa) << 16, np.maximum(1, y_mid - y_lo)) & ~7
+    dxdy_c = _round_away_div((xc - xb) << 16, np.maximum(1, y_hi - y_mid)) & ~7
+    sub = y_lo & 3
+    x_a = x_a - (dxdy_a >> 2) * sub
+    x_b = x_b - (dxdy_b >> 2) * sub
+    right_major = dxdy_b < dxdy_a
+
+    words = np.zeros((n, length), dtype=np.int64)
+    words[:, 0] = (op << 24) | (np.where(right_major, 0, 1) << 23) | ((max_level & 7) << 19) | ((tile & 7) << 16) | (y_hi & 0x3FFF)
+    words[:, 1] = ((y_mid & 0x3FFF) << 16) | (y_lo & 0x3FFF)
+    words[:, 2] = x_c & 0x0FFFFFFF
+    words[:, 3] = dxdy_c & 0x3FFFFFFF
+    words[:, 4] = x_a & 0x0FFFFFFF
+    words[:, 5] = dxdy_a & 0xFFFFFFFF
+    words[:, 6] = x_b & 0x0FFFFFFF
+    words[:, 7] = dxdy_b & 0x3FFFFFFF
+
+    safe_area = np.where(keep, area, 1).astype(np.float64)
+    inv_area = 4.0 / safe_area
+    dxdy = dxdy_a.astype(np.float64) / 65536.0
+    yfrac = (y_lo & 3).astype(np.float64) / 4.0
+
+    def plane(vals: np.ndarray, scale: float):
+        """vals: (N, 3) unsorted per-vertex values -> start, dx, de, dy (int64)."""
+        v = vals[ar, order].astype(np.float64)
+        va, vb, vc = v[:, 0], v[:, 1], v[:, 2]
+        ddx = -inv_area * (ab_y * vc + ca_y * vb + bc_y * va)
+        ddy = inv_area * (ab_x * vc + ca_x * vb + bc_x * va)
+        dde = ddy + ddx * dxdy
+        start = va - yfrac * dde
+        q = lambda a: np.round(a * scale).astype(np.int64)
+        return q(start), q(ddx), q(dde), q(ddy)
+
+    base = 8
+    if shade:
+        comps = [plane(np.asarray(rgba, dtype=np.float64)[:, :, c], 65536.0) for c in range(4)]
+        # comps[c] = (start, dx, de, dy)
+        for k, (blk_hi, blk_lo) in enumerate(((0, 4), (2, 6), (8, 12), (10, 14))):
+            r_hi, r_lo = _pack_pair(comps[0][k], comps[1][k])
+            b_hi, b_lo = _pack_pair(comps[2][k], comps[3][k])
+            words[:, base + blk_hi] = r_hi
+            words[:, base + blk_hi + 1] = b_hi
+            words[:, base + blk_lo] = r_lo
+            words[:, base + blk_lo + 1] = b_lo
+        base += 16
+    if tex:
+        uv = np.asarray(uv, dtype=np.float64)
+        if perspective:
+            ww = np.asarray(w, dtype=np.float64)
+            min_w = ww.min(axis=1, keepdims=True) * 0.49
+            iw = min_w / ww
+            s_vals = uv[:, :, 0] * iw * 64.0
+            t_vals = uv[:, :, 1] * iw * 64.0
+            w_vals = iw
+            s_p = plane(s_vals, 65536.0)
+            t_p = plane(t_vals, 65536.0)
+            w_p = plane(w_vals, 4294967296.0)
+        else:
+            s_p = plane(uv[:, :, 0] * 32.0, 65536.0)
+            t_p = plane(uv[:, :, 1] * 32.0, 65536.0)
+            w_p = tuple(np.zeros(n, dtype=np.int64) for _ in range(4))
+        for k, (blk_hi, blk_lo) in enumerate(((0, 4), (2, 6), (8, 12), (10, 14))):
+            st_hi, st_lo = _pack_pair(s_p[k], t_p[k])
+            w_hi, w_lo = _pack_pair(w_p[k], np.zeros(n, dtype=np.int64))
+            words[:, base + blk_hi] = st_hi
+            words[:, base + blk_hi + 1] = w_hi
+            words[:, base + blk_lo] = st_lo
+            words[:, base + blk_lo + 1] = w_lo
+        base += 16
+    if zbuf:
+        zp = plane(np.asarray(z, dtype=np.float64), 8192.0)
+        for k in range(4):
+            words[:, base + k] = zp[k] & 0xFFFFFFFF
+        base += 4
+
+    return (words[keep] & 0xFFFFFFFF).astype(np.uint32)
+
+
+# ----------------------------------------------------------------------------
+# RDRAM helpers: the N64 address space stored as host-endian 32-bit words.
+# ----------------------------------------------------------------------------
+def rdram_write16(rdram: np.ndarray, addr: int, values: np.ndarray):
+    """Write big-endian halfwords starting at byte address addr (2-aligned)."""
+    h = rdram.view(np.uint16)
+    idx = (addr >> 1) + np.arange(len(values))
+    h[idx ^ 1] = values.astype(np.uint16)
+
+
+def rdram_write8(rdram: np.ndarray, addr: int, values: np.ndarray):
+    b = rdram.view(np.uint8)
+    idx = addr + np.arange(len(values))
+    b[idx ^ 3] = values.astype(np.uint8)
+
+
+def rdram_read16(rdram: np.ndarray, addr: int, count: int) -> np.ndarray:
+    h = rdram.view(np.uint16)
+    idx = (addr >> 1) + np.arange(count)
+    return h[idx ^ 1]
+
+
+# ----------------------------------------------------------------------------
+# Workloads
+# ----------------------------------------------------------------------------
+@dataclasses.dataclass
+class Frame:
+    """One frame of a stream: RDP words plus the VI register state for scanout."""
+    words: np.ndarray
+    vi: Optional[Dict[int, int]] = None
+
+
+@dataclasses.dataclass
+class Stream:
+    """A seeded command stream with its initial RDRAM contents."""
+    name: str
+    seed: int
+    frames: List[Frame]
+    rdram_init: Optional[np.ndarray] = None      # uint32 words, len RDRAM_SIZE/4 (None = zero)
+    hidden_init: int = 0                          # fill value for hidden RDRAM
+    color_addr: int = 0
+    depth_addr: int = 0
+    width: int = 0
+    height: int = 0
+    color_size: int = SIZ_16
+    num_triangles: int = 0
+
+    def all_words(self) -> np.ndarray:
+        return np.concatenate([f.words for f in self.frames]) if self.frames else np.zeros(0, np.uint32)
+
+
+COLOR_ADDR = 0x100000
+DEPTH_ADDR = 0x300000
+TEXTURE_ADDR = 0x500000
+
+
+def _clear_passes(cl: CommandList, width: int, height: int, color_size: int, clear_color: bool,
+                  color_addr: int = COLOR_ADDR, depth_addr: int = DEPTH_ADDR):
+    """Standard frame start: fill the Z buffer with the far value, optionally
+    fill the colour buffer, then bind colour + depth for drawing."""
+    fill = OtherModes(cycle_type=3)
+    cl.set_other_modes(fill)
+    cl.set_scissor(0, 0, width << 2, height << 2)
+    cl.set_color_image(depth_addr, width, SIZ_16)
+    cl.set_fill_color(0xFFFCFFFC)
+    cl.fill_rectangle(0, 0, (width - 1) << 2, (height - 1) << 2)
+    cl.sync_pipe()
+    cl.set_color_image(color_addr, width, color_size)
+    if clear_color:
+        cl.set_fill_color(0x00010001 if color_size == SIZ_16 else 0x000000FF)
+        cl.fill_rectangle(0, 0, (width - 1) << 2, (height - 1) << 2)
+        cl.sync_pipe()
+    cl.set_mask_image(depth_addr)
+
+
+def config0_stream(seed: int = 0x52445000, num_triangles: int = 256, width: int = 320, height: int = 240) -> Stream:
+    """Plain method: 256 Gouraud triangles, 1-cycle SHADE combiner, Z compare
+    and update, RGBA5551 colour + 16-bit Z, one frame."""
+    rng = np.random.default_rng(seed)
+    cl = CommandList()
+    _clear_passes(cl, width, height, SIZ_16, clear_color=False)
+    modes = OtherModes(cycle_type=0, depth_test=True, depth_write=True, z_mode=0,
+                       blend=((0, 0, 1, 0), (0, 0, 1, 0)))
+    cl.set_other_modes(modes)
+    shade = ((MULADD_ZERO, MULSUB_ZERO, MUL_ZERO, RGB_SHADE), (A_ZERO, A_ZERO, AMUL_ZERO, A_SHADE))
+    cl.set_combine(shade[0], shade[1])
+    xy = np.stack([rng.integers(0, width * 4, size=(num_triangles, 3)),
+                   rng.integers(0, height * 4, size=(num_triangles, 3))], axis=-1)
+    rgba = rng.integers(0, 256, size=(num_triangles, 3, 4)).astype(np.float64)
+    z = rng.integers(0, 0x40000, size=(num_triangles, 3)).astype(np.float64)
+    tris = triangle_commands(xy, rgba=rgba, z=z)
+    cl.extend_array(tris)
+    cl.sync_full()
+    return Stream(name="c0", seed=seed, frames=[Frame(cl.words())], hidden_init=0, color_addr=COLOR_ADDR,
+                  depth_addr=DEPTH_ADDR, width=width, height=height, color_size=SIZ_16,
+                  num_triangles=int(tris.shape[0]))
+
+
+def _log_uniform_triangles(rng, n: int, width: int, height: int, lo: float, hi: float):
+    """Triangles whose extent is log-uniform in [lo, hi] pixels at a uniform
+    position; vertices in quarter pixels, clamped to the guard band."""
+    size = np.exp(rng.uniform(math.log(lo), math.log(hi), size=n))
+    cx = rng.uniform(0, width, size=n)
+    cy = rng.uniform(0, height, size=n)
+    ang = rng.uniform(0, 2 * math.pi, size=(n, 3))
+    rad = size[:, None] * rng.uniform(0.35, 0.6, size=(n, 3))
+    x = cx[:, None] + rad * np.cos(ang)
+    y = cy[:, None] + rad * np.sin(ang)
+    xq = np.clip(np.round(x * 4), -1024 * 4, 1023 * 4).astype(np.int64)
+    yq = np.clip(np.round(y * 4), 0, (height - 1) * 4 + 3).astype(np.int64)
+    return np.stack([xq, yq], axis=-1)
+
+
+def _random_texture_rgba16(rng, tex_w: int, tex_h: int) -> np.ndarray:
+    return rng.integers(0, 1 << 16, size=tex_w * tex_h, dtype=np.int64).astype(np.uint16)
+
+
+def config1_stream(seed: int = 0x52446000, frames: int = 60, tris_per_frame: int = 4096, width: int = 640,
+                   height: int = 480, num_textures: int = 8, tris_per_texture: int = 64) -> Stream:
+    """Textured 1-cycle pass: RGBA8888 colour + Z, 32x32 RGBA16 textures loaded
+    with LoadTile, bilinear + perspective, TEX0*SHADE, alpha blend, Z on."""
+    rng = np.random.default_rng(seed)
+    rdram = np.zeros(RDRAM_SIZE // 4, dtype=np.uint32)
+    tex_w = tex_h = 32
+    tex_bytes = tex_w * tex_h * 2
+    for i in range(num_textures):
+        rdram_write16(rdram, TEXTURE_ADDR + i * tex_bytes, _random_texture_rgba16(rng, tex_w, tex_h))
+
+    out_frames = []
+    total = 0
+    shade_tex = ((RGB_TEXEL0, MULSUB_ZERO, RGB_SHADE, ADD_ZERO), (A_TEXEL0, A_ZERO, A_SHADE, A_ZERO))
+    modes = OtherModes(cycle_type=0, perspective=True, sample_quad=True, bilerp0=True, bilerp1=True,
+                       depth_test=True, depth_write=True, z_mode=0, force_blend=True,
+                       blend=((0, 0, 1, 0), (0, 0, 1, 0)))
+    for f in range(frames):
+        cl = CommandList()
+        _clear_passes(cl, width, height, SIZ_32, clear_color=True)
+        cl.set_other_modes(modes)
+        cl.set_combine(shade_tex[0], shade_tex[1])
+        n = tris_per_frame
+        xy = _log_uniform_triangles(rng, n, width, height, 2.0, 200.0)
+        rgba = rng.integers(0, 256, size=(n, 3, 4)).astype(np.float64)
+        z = rng.uniform(0, 0x3FFFF, size=(n, 3))
+        uv = rng.uniform(-8.0, 40.0, size=(n, 3, 2))
+        w = rng.uniform(0.3, 1.0, size=(n, 3))
+        tris = triangle_commands(xy, rgba=rgba, z=z, uv=uv, w=w, perspective=True, tile=0, max_level=0)
+        total += tris.shape[0]
+        for start in range(0, tris.shape[0], tris_per_texture):
+            t = int(rng.integers(0, num_textures))
+            cl.set_texture_image(TEXTURE_ADDR + t * tex_bytes, tex_w, SIZ_16, FMT_RGBA)
+            cl.set_tile(7, FMT_RGBA, SIZ_16, 0, tex_w * 2)
+            cl.sync_load()
+            cl.load_tile(7, 0, 0, (tex_w - 1) << 2, (tex_h - 1) << 2)
+            cl.sync_tile()
+            cl.set_tile(0, FMT_RGBA, SIZ_16, 0, tex_w * 2, mask_s=5, mask_t=5)
+            cl.set_tile_size(0, 0, 0, (tex_w - 1) << 2, (tex_h - 1) << 2)
+            cl.extend_array(tris[start:start + tris_per_texture])
+        cl.sync_full()
+        out_frames.append(Frame(cl.words()))
+    return Stream(name="c1", seed=seed, frames=out_frames, rdram_init=rdram, hidden_init=0, color_addr=COLOR_ADDR,
+                  depth_addr=DEPTH_ADDR, width=width, height=height, color_size=SIZ_32, num_triangles=total)

@@ -1,0 +1,63 @@
+"""Shared helpers for the parity tests: run one stream through the oracle and
+through the MI355X library, then compare RDRAM, hidden RDRAM and TMEM."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for sub in ("oracle", "parallel-rdp_amd"):
+    p = os.path.join(ROOT, sub)
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import rdp_streams as rs  # noqa: E402
+
+
+def run_oracle(stream, frames=None):
+    import oracle
+    o = oracle.Oracle(rs.RDRAM_SIZE)
+    if stream.rdram_init is not None:
+        o.rdram[:] = stream.rdram_init
+    o.hidden[:] = stream.hidden_init
+    for f in (stream.frames if frames is None else stream.frames[:frames]):
+        o.process(f.words)
+    out = (o.rdram.copy(), o.hidden.copy(), o.tmem.copy(), o.fragments)
+    o.close()
+    return out
+
+
+def run_gpu(stream, frames=None, proc=None):
+    import rdp_mi
+    p = proc or rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+    p.write_rdram(stream.rdram_init if stream.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
+    p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, stream.hidden_init, np.uint8))
+    for f in (stream.frames if frames is None else stream.frames[:frames]):
+        p.enqueue_commands(f.words)
+    p.idle()
+    out = (p.read_rdram(), p.read_hidden_rdram(), p.read_tmem())
+    if proc is None:
+        p.close()
+    return out
+
+
+def mismatch_report(a, b, name):
+    a8 = a.view(np.uint8)
+    b8 = b.view(np.uint8)
+    diff = np.nonzero(a8 != b8)[0]
+    if diff.size == 0:
+        return 0, f"{name}: identical"
+    return int(diff.size), f"{name}: {diff.size} mismatched bytes, first at {hex(int(diff[0]))}"
+
+
+def compare(gpu, ref):
+    """Returns (total mismatched bytes, report lines)."""
+    total = 0
+    lines = []
+    for name, g, r in (("rdram", gpu[0], ref[0]), ("hidden", gpu[1], ref[1]), ("tmem", gpu[2], ref[2])):
+        n, line = mismatch_report(g, r, name)
+        total += n
+        lines.append(line)
+    return total, lines
