@@ -120,6 +120,24 @@ void rdp_mi_set_validation_callback(rdp_mi_processor *p, rdp_mi_validation_fn fn
  * to flushing each processor on its own. */
 int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count);
 
+/* Recorded submissions (no reference counterpart; the analogue of a
+ * pre-recorded command buffer).  rdp_mi_record takes the pending render
+ * passes of the processors (batched != 0: one launch per stage per pass
+ * round across all processors; 0: one launch per stage per pass), uploads
+ * their inputs once and keeps them device-resident.  A replay re-executes
+ * the recorded passes on the processors' current RDRAM and TMEM, either as
+ * direct launches or as one captured hipGraph; iteration_ms (optional)
+ * receives each replay's GPU time.  rdp_mi_recording_profile times the four
+ * stages (setup, TMEM, bin, raster) with events between direct launches. */
+typedef struct rdp_mi_recording rdp_mi_recording;
+int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, rdp_mi_recording **out);
+int rdp_mi_recording_replay(rdp_mi_recording *r, unsigned iterations, int use_graph, float *iteration_ms);
+int rdp_mi_recording_profile(rdp_mi_recording *r, unsigned iterations, float stage_ms[4]);
+uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *r);
+unsigned rdp_mi_recording_rounds(const rdp_mi_recording *r);
+unsigned rdp_mi_recording_passes(const rdp_mi_recording *r);
+void rdp_mi_recording_destroy(rdp_mi_recording *r);
+
 /* Library diagnostics. */
 const char *rdp_mi_last_error(void);
 uint64_t rdp_mi_primitive_count(const rdp_mi_processor *p);

@@ -189,9 +189,52 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 	return l;
 }
 
-// Submit rounds of passes.  Caller holds the context lock.
-static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &rounds)
+// One round: the four stage launches covering every pass of the round.
+struct Launch {
+	const PassDesc *descs;
+	const uint32_t *items[4];
+	uint32_t n[4];
+	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
+	uint32_t passes;
+};
+
+static uint64_t framebuffer_bytes(const HostPass &hp)
 {
+	// Colour value + hidden bits, and 16-bit depth + hidden bits, per pixel.
+	uint64_t color;
+	switch (hp.fb_fmt) {
+	case FB_I4:
+	case FB_I8: color = 2; break;              // 1 byte + 1 hidden byte per 2 pixels, rounded up
+	case FB_RGBA8888: color = 6; break;        // 4 bytes + 2 hidden bytes
+	default: color = 3; break;                 // 2 bytes + 1 hidden byte
+	}
+	const uint64_t depth = hp.fb_addr == hp.depth_addr ? 0 : 3;
+	const uint64_t tiles = (uint64_t)((hp.fb_width + 7) / 8) * ((hp.fb_height + 7) / 8);
+	return hp.prims.empty() ? 0 : 2 * tiles * 64 * (color + depth);
+}
+
+static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, hipEvent_t *stage_events = nullptr)
+{
+	for (size_t r = 0; r < launches.size(); r++) {
+		const Launch &L = launches[r];
+		HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
+		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
+		HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], s));
+		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
+		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
+		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
+		HIP_CHECK(launch_raster(L.descs, L.items[3], L.n[3], s));
+		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
+	}
+	return 0;
+}
+
+// Pack the inputs of all rounds into `input` (one async copy on the context
+// stream) and describe the launches.  Caller holds the context lock.
+static int prepare_rounds(Context &c, const std::vector<std::vector<PassRef>> &rounds, DeviceBuffer &input,
+                          DeviceBuffer &scratch, std::vector<Launch> &launches)
+{
+	launches.clear();
 	if (rounds.empty()) return 0;
 	if (c.staging_busy) {
 		HIP_CHECK(hipEventSynchronize(c.staging_done));
@@ -262,13 +305,14 @@ static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &ro
 
 	int err;
 	if ((err = c.staging.reserve(total_input))) return set_error(err, "staging allocation of %zu bytes failed", total_input);
-	if ((err = c.input.reserve(total_input, c.stream)))
+	if ((err = input.reserve(total_input, c.stream)))
 		return set_error(err, "input allocation of %zu bytes failed", total_input);
-	if ((err = c.scratch.reserve(std::max<size_t>(max_scratch, 256), c.stream)))
+	if ((err = scratch.reserve(std::max<size_t>(max_scratch, 256), c.stream)))
 		return set_error(err, "scratch allocation of %zu bytes failed", max_scratch);
 
 	uint8_t *h = c.staging.ptr;
-	uint8_t *d = c.input.ptr;
+	uint8_t *d = input.ptr;
+	launches.resize(rounds.size());
 	for (size_t r = 0; r < rounds.size(); r++) {
 		RoundLayout &L = rl[r];
 		PassDesc *descs = reinterpret_cast<PassDesc *>(h + L.descs);
@@ -277,7 +321,20 @@ static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &ro
 		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
 		uint32_t ns = 0, nt = 0, nb = 0, nr = 0;
+		Launch &LA = launches[r];
+		LA.descs = reinterpret_cast<const PassDesc *>(d + L.descs);
+		LA.items[0] = reinterpret_cast<const uint32_t *>(d + L.items_setup);
+		LA.items[1] = reinterpret_cast<const uint32_t *>(d + L.items_tmem);
+		LA.items[2] = reinterpret_cast<const uint32_t *>(d + L.items_bin);
+		LA.items[3] = reinterpret_cast<const uint32_t *>(d + L.items_raster);
+		LA.n[0] = L.n_setup;
+		LA.n[1] = L.n_tmem;
+		LA.n[2] = L.n_bin;
+		LA.n[3] = L.n_raster;
+		LA.raster_bytes = 0;
+		LA.passes = (uint32_t)rounds[r].size();
 		for (size_t i = 0; i < rounds[r].size(); i++) {
+			LA.raster_bytes += framebuffer_bytes(*rounds[r][i].pass);
 			const HostPass &hp = *rounds[r][i].pass;
 			Processor &pr = *rounds[r][i].proc;
 			PassInputs &pi = pl[r][i];
@@ -293,7 +350,7 @@ static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &ro
 			D.states = reinterpret_cast<const DrawState *>(d + pi.states);
 			D.prims = reinterpret_cast<const PrimRecord *>(d + pi.prims);
 			D.uploads = reinterpret_cast<const UploadInfo *>(d + pi.uploads);
-			uint8_t *sb = c.scratch.ptr + pi.scratch_base;
+			uint8_t *sb = scratch.ptr + pi.scratch_base;
 			D.setups = reinterpret_cast<PrimSetup *>(sb);
 			D.spans = reinterpret_cast<SpanSetup *>(sb + pi.sl.setups);
 			D.tilerows = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans);
@@ -331,17 +388,35 @@ static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &ro
 	}
 
 	HIP_CHECK(hipMemcpyAsync(d, h, total_input, hipMemcpyHostToDevice, c.stream));
-	for (size_t r = 0; r < rounds.size(); r++) {
-		RoundLayout &L = rl[r];
-		const PassDesc *descs = reinterpret_cast<const PassDesc *>(d + L.descs);
-		HIP_CHECK(launch_setup(descs, reinterpret_cast<const uint32_t *>(d + L.items_setup), L.n_setup, c.stream));
-		HIP_CHECK(launch_tmem(descs, reinterpret_cast<const uint32_t *>(d + L.items_tmem), L.n_tmem, c.stream));
-		HIP_CHECK(launch_bin(descs, reinterpret_cast<const uint32_t *>(d + L.items_bin), L.n_bin, c.stream));
-		HIP_CHECK(launch_raster(descs, reinterpret_cast<const uint32_t *>(d + L.items_raster), L.n_raster, c.stream));
-	}
 	HIP_CHECK(hipEventRecord(c.staging_done, c.stream));
 	c.staging_busy = true;
 	return 0;
+}
+
+// Submit rounds of passes through the shared arenas.  Caller holds the lock.
+static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &rounds)
+{
+	std::vector<Launch> launches;
+	int err = prepare_rounds(c, rounds, c.input, c.scratch, launches);
+	if (err) return err;
+	return execute_launches(launches, c.stream);
+}
+
+static void build_round_list(Processor *const *procs, unsigned count, bool batched,
+                             std::vector<std::vector<PassRef>> &rounds)
+{
+	rounds.clear();
+	for (unsigned i = 0; i < count; i++) {
+		auto &passes = procs[i]->fe.completed();
+		for (size_t k = 0; k < passes.size(); k++) {
+			if (!batched) {
+				rounds.push_back({ { procs[i], &passes[k] } });
+				continue;
+			}
+			if (rounds.size() <= k) rounds.resize(k + 1);
+			rounds[k].push_back({ procs[i], &passes[k] });
+		}
+	}
 }
 
 // Submit every completed pass of the given processors (plain path: one
@@ -386,9 +461,24 @@ static int copy_sync(void *dst, const void *src, size_t size, hipMemcpyKind kind
 	return 0;
 }
 
+// A recorded submission: device-resident inputs and scratch for a fixed set
+// of passes, replayable as direct launches or as one captured hipGraph.
+struct Recording {
+	std::vector<Launch> launches;
+	DeviceBuffer input, scratch;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	uint64_t raster_bytes = 0;
+	uint32_t passes = 0;
+};
+
 }  // namespace rdp_mi
 
 using namespace rdp_mi;
+
+struct rdp_mi_recording {
+	Recording r;
+};
 
 struct rdp_mi_processor {
 	Processor p;
@@ -650,6 +740,126 @@ int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count)
 }
 
 uint64_t rdp_mi_primitive_count(const rdp_mi_processor *h) { return h ? h->p.fe.primitive_counter() : 0; }
+
+int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, rdp_mi_recording **out)
+{
+	if (!out || (!procs && count)) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	*out = nullptr;
+	std::vector<Processor *> ps;
+	for (unsigned i = 0; i < count; i++) {
+		if (!procs[i]) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
+		procs[i]->p.fe.flush();
+		ps.push_back(&procs[i]->p);
+	}
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	rdp_mi_recording *h = new (std::nothrow) rdp_mi_recording();
+	if (!h) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	std::vector<std::vector<PassRef>> rounds;
+	build_round_list(ps.data(), count, batched != 0, rounds);
+	int err = prepare_rounds(c, rounds, h->r.input, h->r.scratch, h->r.launches);
+	for (auto *p : ps) p->fe.completed().clear();
+	if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "upload failed");
+	if (err) {
+		delete h;
+		return err;
+	}
+	for (auto &L : h->r.launches) {
+		h->r.raster_bytes += L.raster_bytes;
+		h->r.passes += L.passes;
+	}
+	*out = h;
+	return 0;
+}
+
+static int ensure_graph(Context &c, Recording &r)
+{
+	if (r.exec) return 0;
+	HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+	int err = execute_launches(r.launches, c.stream);
+	hipGraph_t graph = nullptr;
+	hipError_t e = hipStreamEndCapture(c.stream, &graph);
+	if (err) return err;
+	if (e != hipSuccess) return set_error(RDP_MI_ERROR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+	r.graph = graph;
+	HIP_CHECK(hipGraphInstantiate(&r.exec, graph, nullptr, nullptr, 0));
+	return 0;
+}
+
+int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_graph, float *iteration_ms)
+{
+	if (!h || iterations == 0) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Recording &r = h->r;
+	if (use_graph) {
+		int err = ensure_graph(c, r);
+		if (err) return err;
+	}
+	std::vector<hipEvent_t> ev(iteration_ms ? iterations + 1 : 0);
+	for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+	int err = 0;
+	if (!ev.empty()) HIP_CHECK(hipEventRecord(ev[0], c.stream));
+	for (unsigned i = 0; i < iterations && !err; i++) {
+		if (use_graph) {
+			hipError_t e = hipGraphLaunch(r.exec, c.stream);
+			if (e != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
+		} else
+			err = execute_launches(r.launches, c.stream);
+		if (!err && !ev.empty()) HIP_CHECK(hipEventRecord(ev[i + 1], c.stream));
+	}
+	hipError_t se = hipStreamSynchronize(c.stream);
+	if (!err && se != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "replay failed: %s", hipGetErrorString(se));
+	if (!err && iteration_ms)
+		for (unsigned i = 0; i < iterations; i++) (void)hipEventElapsedTime(&iteration_ms[i], ev[i], ev[i + 1]);
+	for (auto &e : ev) (void)hipEventDestroy(e);
+	return err;
+}
+
+int rdp_mi_recording_profile(rdp_mi_recording *h, unsigned iterations, float stage_ms[4])
+{
+	if (!h || iterations == 0 || !stage_ms) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Recording &r = h->r;
+	const size_t nev = r.launches.size() * 4;
+	std::vector<hipEvent_t> ev(nev + 1);
+	for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+	double acc[4] = { 0, 0, 0, 0 };
+	int err = 0;
+	for (unsigned it = 0; it < iterations && !err; it++) {
+		HIP_CHECK(hipEventRecord(ev[0], c.stream));
+		err = execute_launches(r.launches, c.stream, ev.data() + 1);
+		HIP_CHECK(hipStreamSynchronize(c.stream));
+		for (size_t k = 0; k < nev && !err; k++) {
+			float ms = 0;
+			(void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+			acc[k % 4] += ms;
+		}
+	}
+	for (auto &e : ev) (void)hipEventDestroy(e);
+	for (int k = 0; k < 4; k++) stage_ms[k] = (float)(acc[k] / iterations);
+	return err;
+}
+
+uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *h) { return h ? h->r.raster_bytes : 0; }
+
+unsigned rdp_mi_recording_rounds(const rdp_mi_recording *h) { return h ? (unsigned)h->r.launches.size() : 0; }
+
+unsigned rdp_mi_recording_passes(const rdp_mi_recording *h) { return h ? h->r.passes : 0; }
+
+void rdp_mi_recording_destroy(rdp_mi_recording *h)
+{
+	if (!h) return;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	(void)hipStreamSynchronize(c.stream);
+	if (h->r.exec) (void)hipGraphExecDestroy(h->r.exec);
+	if (h->r.graph) (void)hipGraphDestroy(h->r.graph);
+	if (h->r.input.ptr) (void)hipFree(h->r.input.ptr);
+	if (h->r.scratch.ptr) (void)hipFree(h->r.scratch.ptr);
+	delete h;
+}
 
 }  // extern "C"
 

@@ -210,3 +210,63 @@ def batch_flush(processors: Sequence[CommandProcessor]):
     lib = load_library()
     arr = (ctypes.c_void_p * len(processors))(*[p.handle.value for p in processors])
     _check(lib.rdp_mi_batch_flush(arr, len(processors)), "batch_flush")
+
+
+class Recording:
+    """Device-resident recorded submission of pending passes (see rdp_mi_record)."""
+
+    def __init__(self, processors: Sequence[CommandProcessor], batched: bool = True):
+        lib = load_library()
+        lib.rdp_mi_record.restype = ctypes.c_int
+        lib.rdp_mi_record.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_void_p)]
+        lib.rdp_mi_recording_replay.restype = ctypes.c_int
+        lib.rdp_mi_recording_replay.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]
+        lib.rdp_mi_recording_profile.restype = ctypes.c_int
+        lib.rdp_mi_recording_profile.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
+        lib.rdp_mi_recording_raster_bytes.restype = ctypes.c_uint64
+        lib.rdp_mi_recording_raster_bytes.argtypes = [ctypes.c_void_p]
+        lib.rdp_mi_recording_rounds.restype = ctypes.c_uint
+        lib.rdp_mi_recording_rounds.argtypes = [ctypes.c_void_p]
+        lib.rdp_mi_recording_passes.restype = ctypes.c_uint
+        lib.rdp_mi_recording_passes.argtypes = [ctypes.c_void_p]
+        lib.rdp_mi_recording_destroy.argtypes = [ctypes.c_void_p]
+        self._lib = lib
+        arr = (ctypes.c_void_p * len(processors))(*[p.handle.value for p in processors])
+        h = ctypes.c_void_p()
+        _check(lib.rdp_mi_record(arr, len(processors), int(batched), ctypes.byref(h)), "rdp_mi_record")
+        self._h = h
+        self.processors = list(processors)
+
+    def replay(self, iterations: int = 1, use_graph: bool = True) -> np.ndarray:
+        ms = np.zeros(iterations, dtype=np.float32)
+        _check(self._lib.rdp_mi_recording_replay(self._h, iterations, int(use_graph), ms.ctypes.data), "replay")
+        return ms
+
+    def profile(self, iterations: int = 1) -> np.ndarray:
+        ms = np.zeros(4, dtype=np.float32)
+        _check(self._lib.rdp_mi_recording_profile(self._h, iterations, ms.ctypes.data), "profile")
+        return ms
+
+    @property
+    def raster_bytes(self) -> int:
+        return int(self._lib.rdp_mi_recording_raster_bytes(self._h))
+
+    @property
+    def rounds(self) -> int:
+        return int(self._lib.rdp_mi_recording_rounds(self._h))
+
+    @property
+    def passes(self) -> int:
+        return int(self._lib.rdp_mi_recording_passes(self._h))
+
+    def close(self):
+        if self._h:
+            self._lib.rdp_mi_recording_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -589,3 +589,87 @@ def config1_stream(seed: int = 0x52446000, frames: int = 60, tris_per_frame: int
         out_frames.append(Frame(cl.words()))
     return Stream(name="c1", seed=seed, frames=out_frames, rdram_init=rdram, hidden_init=0, color_addr=COLOR_ADDR,
                   depth_addr=DEPTH_ADDR, width=width, height=height, color_size=SIZ_32, num_triangles=total)
+
+
+# ----------------------------------------------------------------------------
+# Mixed batch (64 independent streams)
+# ----------------------------------------------------------------------------
+SHADE_COMBINE = ((MULADD_ZERO, MULSUB_ZERO, MUL_ZERO, RGB_SHADE), (A_ZERO, A_ZERO, AMUL_ZERO, A_SHADE))
+TEX_SHADE_COMBINE = ((RGB_TEXEL0, MULSUB_ZERO, RGB_SHADE, ADD_ZERO), (A_TEXEL0, A_ZERO, A_SHADE, A_ZERO))
+
+
+def _emit_shade_z(cl: CommandList, rng, n: int, width: int, height: int, uniform_vertices: bool) -> int:
+    cl.set_other_modes(OtherModes(cycle_type=0, depth_test=True, depth_write=True,
+                                  blend=((0, 0, 1, 0), (0, 0, 1, 0))))
+    cl.set_combine(*SHADE_COMBINE)
+    if uniform_vertices:
+        xy = np.stack([rng.integers(0, width * 4, size=(n, 3)), rng.integers(0, height * 4, size=(n, 3))], axis=-1)
+    else:
+        xy = _log_uniform_triangles(rng, n, width, height, 2.0, 200.0)
+    rgba = rng.integers(0, 256, size=(n, 3, 4)).astype(np.float64)
+    z = rng.integers(0, 0x40000, size=(n, 3)).astype(np.float64)
+    tris = triangle_commands(xy, rgba=rgba, z=z)
+    cl.extend_array(tris)
+    return int(tris.shape[0])
+
+
+def _emit_textured(cl: CommandList, rng, n: int, width: int, height: int, num_textures: int,
+                   tris_per_texture: int = 64) -> int:
+    tex_w = tex_h = 32
+    tex_bytes = tex_w * tex_h * 2
+    cl.set_other_modes(OtherModes(cycle_type=0, perspective=True, sample_quad=True, bilerp0=True, bilerp1=True,
+                                  depth_test=True, depth_write=True, force_blend=True,
+                                  blend=((0, 0, 1, 0), (0, 0, 1, 0))))
+    cl.set_combine(*TEX_SHADE_COMBINE)
+    xy = _log_uniform_triangles(rng, n, width, height, 2.0, 200.0)
+    rgba = rng.integers(0, 256, size=(n, 3, 4)).astype(np.float64)
+    z = rng.uniform(0, 0x3FFFF, size=(n, 3))
+    uv = rng.uniform(-8.0, 40.0, size=(n, 3, 2))
+    w = rng.uniform(0.3, 1.0, size=(n, 3))
+    tris = triangle_commands(xy, rgba=rgba, z=z, uv=uv, w=w, perspective=True)
+    for start in range(0, tris.shape[0], tris_per_texture):
+        t = int(rng.integers(0, num_textures))
+        cl.set_texture_image(TEXTURE_ADDR + t * tex_bytes, tex_w, SIZ_16, FMT_RGBA)
+        cl.set_tile(7, FMT_RGBA, SIZ_16, 0, tex_w * 2)
+        cl.sync_load()
+        cl.load_tile(7, 0, 0, (tex_w - 1) << 2, (tex_h - 1) << 2)
+        cl.sync_tile()
+        cl.set_tile(0, FMT_RGBA, SIZ_16, 0, tex_w * 2, mask_s=5, mask_t=5)
+        cl.set_tile_size(0, 0, 0, (tex_w - 1) << 2, (tex_h - 1) << 2)
+        cl.extend_array(tris[start:start + tris_per_texture])
+    return int(tris.shape[0])
+
+
+BATCH_MODES = ("c0", "c1")
+
+
+def batch_stream(index: int, seed_base: int = 0x52449000, modes: Sequence[str] = BATCH_MODES,
+                 min_tris: int = 32, max_tris: int = 8192) -> Stream:
+    """One stream of the mixed batch: log-uniform triangle count, 320x240 or
+    640x480 at 16 or 32 bits per pixel, render mode drawn from the plain
+    shaded (c0) and textured (c1) workloads."""
+    seed = seed_base + index
+    rng = np.random.default_rng(seed)
+    n = int(round(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))))
+    width, height = ((320, 240), (640, 480))[int(rng.integers(0, 2))]
+    size = (SIZ_16, SIZ_32)[int(rng.integers(0, 2))]
+    mode = modes[int(rng.integers(0, len(modes)))]
+    rdram = None
+    cl = CommandList()
+    _clear_passes(cl, width, height, size, clear_color=(mode != "c0"))
+    if mode == "c1":
+        rdram = np.zeros(RDRAM_SIZE // 4, dtype=np.uint32)
+        num_textures = 8
+        for i in range(num_textures):
+            rdram_write16(rdram, TEXTURE_ADDR + i * 2048, _random_texture_rgba16(rng, 32, 32))
+        tris = _emit_textured(cl, rng, n, width, height, num_textures)
+    else:
+        tris = _emit_shade_z(cl, rng, n, width, height, uniform_vertices=(n <= 512))
+    cl.sync_full()
+    return Stream(name=f"b{index}:{mode}:{width}x{height}x{4 << size}", seed=seed, frames=[Frame(cl.words())],
+                  rdram_init=rdram, hidden_init=0, color_addr=COLOR_ADDR, depth_addr=DEPTH_ADDR, width=width,
+                  height=height, color_size=size, num_triangles=tris)
+
+
+def batch_streams(count: int = 64, **kw) -> List[Stream]:
+    return [batch_stream(i, **kw) for i in range(count)]
