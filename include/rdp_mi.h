@@ -133,6 +133,8 @@ typedef struct rdp_mi_recording rdp_mi_recording;
 int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, rdp_mi_recording **out);
 int rdp_mi_recording_replay(rdp_mi_recording *r, unsigned iterations, int use_graph, float *iteration_ms);
 int rdp_mi_recording_profile(rdp_mi_recording *r, unsigned iterations, float stage_ms[4]);
+/* Per-round variant: round_stage_ms holds rounds * 4 entries. */
+int rdp_mi_recording_profile_rounds(rdp_mi_recording *r, unsigned iterations, float *round_stage_ms);
 uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_rounds(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *r);

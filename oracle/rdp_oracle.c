@@ -240,7 +240,8 @@ void rdpo_gamma_table(uint8_t out[256 + 256 * 64])
 static inline int32_t persp_base(int i);
 
 /* Perspective reciprocal table (reference data: perspective.h:27-44).
- * Entry i: r_i = round(2^20 / (64 + i)) and slope -4 * (r_i - r_{i+1}). */
+ * Entry i: r_i = round(2^20 / (64 + i)), except r_6 which is one lower, and
+ * slope -4 * (r_i - r_{i+1}). */
 void rdpo_perspective_table(int16_t out[64][2])
 {
 	int32_t r[65];
@@ -499,7 +500,8 @@ static void interpolate_rgba(const span_t *s, const attr_t *a, int dx, int cover
 /* perspective.h:46-112 */
 static inline int32_t persp_base(int i)
 {
-	return ((1 << 20) + (64 + i) / 2) / (64 + i);
+	/* round(2^20 / (64 + i)); the hardware table's entry 6 is one lower. */
+	return ((1 << 20) + (64 + i) / 2) / (64 + i) - (i == 6);
 }
 
 static void perspective_get_lut(int32_t w, int32_t *rcp, int32_t *shift)
@@ -3180,3 +3182,40 @@ void rdpo_set_vi_register(rdpo *o, unsigned reg, uint32_t value)
 	if (reg < 14)
 		o->vi_regs[reg] = value;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Hooks that let the test-suite pin individual functions.                  */
+/* ------------------------------------------------------------------------ */
+uint32_t rdpo_test_z_compress(int32_t z) { return z_compress(z); }
+int32_t rdpo_test_z_decompress(uint32_t z) { return z_decompress(z); }
+int32_t rdpo_test_clamp_9bit(int32_t c) { return clamp_9bit(c); }
+int32_t rdpo_test_clamp_z(int32_t z) { return clamp_z(z); }
+int32_t rdpo_test_dz_compress_host(int32_t dz) { return dz_compress_host(dz); }
+int32_t rdpo_test_normalize_dzpix(int32_t dz) { return normalize_dzpix(dz); }
+int32_t rdpo_test_combiner_equation(int32_t a, int32_t b, int32_t c, int32_t d)
+{
+	int32_t A[4] = { a, a, a, a }, B[4] = { b, b, b, b }, C[4] = { c, c, c, c }, D[4] = { d, d, d, d }, out[4];
+	combiner_equation(A, B, C, D, out);
+	return out[0];
+}
+uint32_t rdpo_test_noise(uint32_t x, uint32_t y, uint32_t prim) { return reseed_noise(x, y, prim); }
+void rdpo_test_perspective_divide(int32_t s, int32_t t, int32_t w, int32_t out[3])
+{
+	int32_t stw[3] = { s, t, w }, st[2];
+	int ovf = 0;
+	perspective_divide(stw, st, &ovf);
+	out[0] = st[0];
+	out[1] = st[1];
+	out[2] = ovf;
+}
+int rdpo_test_coverage(const uint16_t xleft[4], const uint16_t xright[4], int x)
+{
+	span_t s;
+	memset(&s, 0, sizeof(s));
+	for (int i = 0; i < 4; i++) {
+		s.xleft[i] = xleft[i];
+		s.xright[i] = xright[i];
+	}
+	return compute_coverage(&s, x);
+}
+void rdpo_test_rgb_dither(int32_t rgb[3], int dith) { rgb_dither(rgb, dith); }

@@ -67,6 +67,18 @@ void rdpo_blender_divider_table(uint8_t out[0x8000]);
 void rdpo_gamma_table(uint8_t out[256 + 256 * 64]);
 void rdpo_perspective_table(int16_t out[64][2]);
 
+/* Test hooks onto individual functions of the restatement. */
+uint32_t rdpo_test_z_compress(int32_t z);
+int32_t rdpo_test_z_decompress(uint32_t z);
+int32_t rdpo_test_clamp_9bit(int32_t c);
+int32_t rdpo_test_clamp_z(int32_t z);
+int32_t rdpo_test_dz_compress_host(int32_t dz);
+int32_t rdpo_test_normalize_dzpix(int32_t dz);
+int32_t rdpo_test_combiner_equation(int32_t a, int32_t b, int32_t c, int32_t d);
+uint32_t rdpo_test_noise(uint32_t x, uint32_t y, uint32_t prim);
+void rdpo_test_perspective_divide(int32_t s, int32_t t, int32_t w, int32_t out[3]);
+int rdpo_test_coverage(const uint16_t xleft[4], const uint16_t xright[4], int x);
+void rdpo_test_rgb_dither(int32_t rgb[3], int dith);
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,40 @@ namespace rdp_mi {
 #define DEV static __device__ __forceinline__
 #define MDEV __device__ __forceinline__
 
+// Explicit address spaces: read-only per-pass tables go through the constant
+// address space so uniform reads become scalar loads; everything else is
+// global memory (never flat).
+#define AS_GLOBAL __attribute__((address_space(1)))
+#define AS_CONST __attribute__((address_space(4)))
+#define AS_LDS __attribute__((address_space(3)))
+template <typename T> DEV const AS_CONST T *cptr(const T *p) { return (const AS_CONST T *)p; }
+template <typename T> DEV AS_GLOBAL T *gptr(T *p) { return (AS_GLOBAL T *)p; }
+template <typename T> DEV const AS_GLOBAL T *gptr(const T *p) { return (const AS_GLOBAL T *)p; }
+
+// Whole-record loads (the implicit copy constructors cannot read from a
+// non-generic address space).
+template <typename T> DEV T ldc(const AS_CONST T *p)
+{
+	static_assert(sizeof(T) % 4 == 0, "record size");
+	T v;
+	const AS_CONST uint32_t *src = (const AS_CONST uint32_t *)p;
+	uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+	for (unsigned i = 0; i < sizeof(T) / 4; i++) dst[i] = src[i];
+	return v;
+}
+
+template <typename T> DEV T ldg16(const AS_GLOBAL T *p)
+{
+	static_assert(sizeof(T) % 16 == 0, "record size");
+	T v;
+	const AS_GLOBAL uint32_t *src = (const AS_GLOBAL uint32_t *)p;
+	uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+	for (unsigned i = 0; i < sizeof(T) / 4; i++) dst[i] = src[i];
+	return v;
+}
+
 // --------------------------------------------------------------------------
 // Integer helpers with GLSL semantics.
 // --------------------------------------------------------------------------
@@ -49,20 +83,20 @@ DEV uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v);
 // RDRAM access: host-endian 32-bit words holding big-endian data.
 // --------------------------------------------------------------------------
 struct Mem {
-	uint8_t *rdram;
-	uint8_t *hidden;
+	AS_GLOBAL uint8_t *rdram;
+	AS_GLOBAL uint8_t *hidden;
 	uint32_t mask8;
 	MDEV uint32_t rd8(uint32_t addr) const { return rdram[(addr & mask8) ^ 3]; }
 	MDEV uint32_t rd16i(uint32_t index) const
 	{
-		return reinterpret_cast<const uint16_t *>(rdram)[(index & (mask8 >> 1)) ^ 1];
+		return reinterpret_cast<const AS_GLOBAL uint16_t *>(rdram)[(index & (mask8 >> 1)) ^ 1];
 	}
 	MDEV void wr16i(uint32_t index, uint32_t v) const
 	{
-		reinterpret_cast<uint16_t *>(rdram)[(index & (mask8 >> 1)) ^ 1] = (uint16_t)v;
+		reinterpret_cast<AS_GLOBAL uint16_t *>(rdram)[(index & (mask8 >> 1)) ^ 1] = (uint16_t)v;
 	}
-	MDEV uint32_t rd32i(uint32_t index) const { return reinterpret_cast<const uint32_t *>(rdram)[index & (mask8 >> 2)]; }
-	MDEV void wr32i(uint32_t index, uint32_t v) const { reinterpret_cast<uint32_t *>(rdram)[index & (mask8 >> 2)] = v; }
+	MDEV uint32_t rd32i(uint32_t index) const { return reinterpret_cast<const AS_GLOBAL uint32_t *>(rdram)[index & (mask8 >> 2)]; }
+	MDEV void wr32i(uint32_t index, uint32_t v) const { reinterpret_cast<AS_GLOBAL uint32_t *>(rdram)[index & (mask8 >> 2)] = v; }
 	MDEV uint32_t rd16a(uint32_t addr) const
 	{
 		if ((addr & 1) == 0)
@@ -318,8 +352,45 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 	s.yhi = rec.yhi;
 	s.span_offset = rec.span_offset;
 	s.pad = 0;
-	if (lane == 0)
-		P.setups[prim] = s;
+	if (lane == 0) {
+		RasterPrim r;
+		for (int i = 0; i < 4; i++) {
+			r.drgba_dx[i] = s.drgba_dx[i];
+			r.drgba_dy[i] = s.drgba_dy[i];
+			r.dstzw_dx[i] = s.dstzw_dx[i];
+			r.dstzw_dy[i] = s.dstzw_dy[i];
+			r.factors[i] = st.factors[i];
+			r.fog[i] = st.fog[i];
+			r.blendc[i] = st.blendc[i];
+		}
+		r.flags = s.flags;
+		r.tile = s.tile;
+		r.ylo = s.ylo;
+		r.yhi = s.yhi;
+		r.span_offset = s.span_offset;
+		r.tmem_instance = s.tmem_instance;
+		r.prim_index = s.prim_index;
+		r.state_index = s.state_index;
+		r.dz = s.dz;
+		r.dz_compressed = s.dz_compressed;
+		r.ras_flags = st.ras_flags;
+		r.dither = st.dither;
+		r.db_flags = st.db_flags;
+		r.coverage_mode = st.coverage_mode;
+		r.z_mode = st.z_mode;
+		r.fill_color = st.fill_color;
+		r.min_lod = st.min_lod;
+		r.pad0[0] = r.pad0[1] = r.pad0[2] = 0;
+		for (int c = 0; c < 2; c++) {
+			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
+			for (int i = 0; i < 4; i++) {
+				r.blend[c][i] = st.blend[c][i];
+				for (int k = 0; k < 4; k++) r.constants[c][i][k] = st.constants[c][i][k];
+			}
+		}
+		for (int i = 0; i < 8; i++) r.pad1[i] = 0;
+		P.rprims[prim] = r;
+	}
 
 	const int ylo = rec.ylo, yhi = rec.yhi;
 	const int row0 = ylo >> 3, row1 = yhi >> 3;
@@ -571,7 +642,7 @@ __global__ __launch_bounds__(256) void k_tmem(const PassDesc *__restrict__ passe
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 12];
 	const int g = (int)((item & 0xfff) * 256 + threadIdx.x);  // 0..2047
-	const Mem m{ P.rdram, P.hidden, P.rdram_mask };
+	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
 	uint32_t cur = P.tmem[g];
 	const int idx = g ^ 1;
 	const bool upper = idx >= 0x400;
@@ -655,8 +726,9 @@ DEV int32_t clamp_z(int32_t z)
 	return iclamp(z, 0, 0x3ffff);
 }
 
-// perspective.h:46-112; the table is generated from r_i = round(2^20 / (64+i)).
-DEV int32_t persp_base(int i) { return ((1 << 20) + (64 + i) / 2) / (64 + i); }
+// perspective.h:46-112.  Reciprocal table r_i = round(2^20 / (64+i)), except
+// entry 6 which the hardware table holds one lower; slopes are -4 * (r_i - r_i+1).
+DEV int32_t persp_base(int i) { return ((1 << 20) + (64 + i) / 2) / (64 + i) - (i == 6); }
 
 DEV void perspective_divide(int32_t s, int32_t t, int32_t w, int32_t &os, int32_t &ot, bool &overflow)
 {
@@ -725,9 +797,9 @@ DEV uint32_t reseed_noise(uint32_t x, uint32_t y, uint32_t prim)
 
 // ---- texture sampling: texture.h -----------------------------------------
 struct Tex {
-	const uint16_t *tmem;
+	const AS_LDS uint16_t *tmem;
 	MDEV uint32_t t16(uint32_t i) const { return tmem[i & 0x7ff]; }
-	MDEV uint32_t t8(uint32_t i) const { return reinterpret_cast<const uint8_t *>(tmem)[i & 0xfff]; }
+	MDEV uint32_t t8(uint32_t i) const { return reinterpret_cast<const AS_LDS uint8_t *>(tmem)[i & 0xfff]; }
 };
 
 DEV int mask_coord(int v, uint32_t mask_bits, bool mirror)
@@ -1168,7 +1240,7 @@ struct CombIn {
 
 // Selector decoding follows the reference's select_* helpers: values that
 // do not name a per-pixel input read the constant slot.
-DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const uint8_t k[4])
+DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const AS_CONST uint8_t *k)
 {
 	switch (which) {
 	case 0:  // muladd
@@ -1216,7 +1288,7 @@ DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const uint8_t 
 	}
 }
 
-DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const uint8_t k[4])
+DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const AS_CONST uint8_t *k)
 {
 	if (which == 2) {
 		switch (sel) {
@@ -1237,7 +1309,7 @@ DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const uint8_t k[4])
 	}
 }
 
-DEV void combine(const CombIn &in, const uint8_t sel[8], const uint8_t k[4][4], int32_t out[4])
+DEV void combine(const CombIn &in, const AS_CONST uint8_t *sel, const AS_CONST uint8_t (*k)[4], int32_t out[4])
 {
 #pragma unroll
 	for (int ch = 0; ch < 4; ch++) {
@@ -1342,8 +1414,8 @@ DEV void alias_depth_to_color(Pixel &p, uint32_t fmt)
 	p.color_dirty = true;
 }
 
-DEV void blender(const uint8_t *lut, const int32_t pixel[4], const int32_t memory[4], const uint8_t fog[4],
-                 const uint8_t blendc[4], int32_t shade_alpha, const uint8_t modes[4], bool force_blend, bool blend_en,
+DEV void blender(const AS_GLOBAL uint8_t *lut, const int32_t pixel[4], const int32_t memory[4], const AS_CONST uint8_t *fog,
+                 const AS_CONST uint8_t *blendc, int32_t shade_alpha, const AS_CONST uint8_t *modes, bool force_blend, bool blend_en,
                  bool color_on_cvg, bool cvg_wrap, int shift0, int shift1, bool final_cycle, int32_t out[3])
 {
 	int32_t src1[3], src0[3];
@@ -1496,10 +1568,10 @@ struct Shaded {
 #define COVERAGE_COPY 0x20
 
 // shading.h:38-359 for one pixel of one primitive (no upscaling).
-DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st, const SpanSetup &span, int x, int y,
-                     Shaded &out)
+DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *tiles, const Tex &tx, uint32_t fb_fmt,
+                     const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, Shaded &out)
 {
-	const uint32_t f = st.ras_flags;
+	const uint32_t f = ps->ras_flags;
 	const bool tlut = f & RAS_TLUT, tlut_type = f & RAS_TLUT_TYPE, sample_quad = f & RAS_SAMPLE_QUAD;
 	const bool cvg_times_alpha = f & RAS_CVG_TIMES_ALPHA, alpha_cvg_select = f & RAS_ALPHA_CVG_SELECT;
 	const bool perspective = f & RAS_PERSPECTIVE, aa = f & RAS_AA, multi = f & RAS_MULTI_CYCLE;
@@ -1510,16 +1582,16 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 
 	uint32_t noise = 0;
 	if (f & RAS_NEED_NOISE)
-		noise = reseed_noise((uint32_t)x, (uint32_t)y, ps.prim_index);
+		noise = reseed_noise((uint32_t)x, (uint32_t)y, ps->prim_index);
 
-	const bool flip = (ps.flags & TRI_FLIP) != 0;
-	const Tex tx{ P.tmem_instances + (size_t)ps.tmem_instance * 2048 };
+	const bool flip = (ps->flags & TRI_FLIP) != 0;
+	const int32_t factors[4] = { ps->factors[0], ps->factors[1], ps->factors[2], ps->factors[3] };
 
 	if (f & RAS_COPY) {
 		if (x < span.start_x || x > span.end_x) return false;
 		int dxp = flip ? (x - span.start_x) : (span.end_x - x);
 		uint32_t dx_mask, dx_shift, fb_size;
-		switch (P.fb_fmt) {
+		switch (fb_fmt) {
 		case FB_I4: fb_size = 0; dx_mask = 0; dx_shift = 0; break;
 		case FB_I8: fb_size = 1; dx_mask = ~7u; dx_shift = 3; break;
 		case FB_RGBA5551:
@@ -1530,9 +1602,9 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 		int lerp_dx = (dxp >> dx_shift) * (flip ? 1 : -1);
 		int32_t ss, tt;
 		bool ovf = false;
-		stw_to_st(span.stzw[0] + (ps.dstzw_dx[0] & ~0x1f) * lerp_dx, span.stzw[1] + (ps.dstzw_dx[1] & ~0x1f) * lerp_dx,
-		          span.stzw[3] + (ps.dstzw_dx[3] & ~0x1f) * lerp_dx, perspective, ss, tt, ovf);
-		const TileDesc &t0 = st.tiles[ps.tile & 7];
+		stw_to_st(span.stzw[0] + (ps->dstzw_dx[0] & ~0x1f) * lerp_dx, span.stzw[1] + (ps->dstzw_dx[1] & ~0x1f) * lerp_dx,
+		          span.stzw[3] + (ps->dstzw_dx[3] & ~0x1f) * lerp_dx, perspective, ss, tt, ovf);
+		const TileDesc t0 = ldc(tiles + (ps->tile & 7));
 		int s = shift_coord(ss, (int)t0.slo, t0.shift_s) >> 5;
 		int t = shift_coord(tt, (int)t0.tlo, t0.shift_t) >> 5;
 		int texel;
@@ -1564,9 +1636,9 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 	int32_t shade[4];
 #pragma unroll
 	for (int i = 0; i < 4; i++) {
-		int32_t v = span.rgba[i] + (ps.drgba_dx[i] & ~0x1f) * dx;
+		int32_t v = span.rgba[i] + (ps->drgba_dx[i] & ~0x1f) * dx;
 		int32_t sn = (v >> 14) << 2;
-		sn += xoff * (ps.drgba_dx[i] >> 14) + yoff * (ps.drgba_dy[i] >> 14);
+		sn += xoff * (ps->drgba_dx[i] >> 14) + yoff * (ps->drgba_dy[i] >> 14);
 		shade[i] = clamp_9bit(sn >> 4);
 	}
 
@@ -1574,38 +1646,41 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 	int32_t st0[2], stdx[2] = { 0, 0 }, stdy[2] = { 0, 0 };
 	bool povf = false;
 	{
-		int32_t s = span.stzw[0] + (ps.dstzw_dx[0] & ~0x1f) * dx;
-		int32_t t = span.stzw[1] + (ps.dstzw_dx[1] & ~0x1f) * dx;
-		int32_t w = span.stzw[3] + (ps.dstzw_dx[3] & ~0x1f) * dx;
+		int32_t s = span.stzw[0] + (ps->dstzw_dx[0] & ~0x1f) * dx;
+		int32_t t = span.stzw[1] + (ps->dstzw_dx[1] & ~0x1f) * dx;
+		int32_t w = span.stzw[3] + (ps->dstzw_dx[3] & ~0x1f) * dx;
 		stw_to_st(s, t, w, perspective, st0[0], st0[1], povf);
 		if (uses_lod) {
-			stw_to_st(s + dir * (ps.dstzw_dx[0] & ~0x1f), t + dir * (ps.dstzw_dx[1] & ~0x1f),
-			          w + dir * (ps.dstzw_dx[3] & ~0x1f), perspective, stdx[0], stdx[1], povf);
-			stw_to_st(s + (ps.dstzw_dy[0] & ~0x7fff), t + (ps.dstzw_dy[1] & ~0x7fff), w + (ps.dstzw_dy[3] & ~0x7fff),
+			stw_to_st(s + dir * (ps->dstzw_dx[0] & ~0x1f), t + dir * (ps->dstzw_dx[1] & ~0x1f),
+			          w + dir * (ps->dstzw_dx[3] & ~0x1f), perspective, stdx[0], stdx[1], povf);
+			stw_to_st(s + (ps->dstzw_dy[0] & ~0x7fff), t + (ps->dstzw_dy[1] & ~0x7fff), w + (ps->dstzw_dy[3] & ~0x7fff),
 			          perspective, stdy[0], stdy[1], povf);
 		}
 	}
 	int32_t z;
 	{
-		int32_t zz = span.stzw[2] + ps.dstzw_dx[2] * dx;
+		int32_t zz = span.stzw[2] + ps->dstzw_dx[2] * dx;
 		int32_t sn = (zz >> 10) << 2;
-		sn += xoff * (ps.dstzw_dx[2] >> 10) + yoff * (ps.dstzw_dy[2] >> 10);
+		sn += xoff * (ps->dstzw_dx[2] >> 10) + yoff * (ps->dstzw_dy[2] >> 10);
 		z = clamp_z(sn >> 5);
 	}
 
-	uint32_t tile0 = ps.tile & 7, tile1 = (tile0 + 1) & 7;
+	uint32_t tile0 = ps->tile & 7, tile1 = (tile0 + 1) & 7;
 	int32_t lod_frac = 0;
 	if (uses_lod)
-		compute_lod(tile0, tile1, lod_frac, ps.tile >> 3, (int)st.min_lod, st0, stdx, stdy, povf, f & RAS_TEX_LOD,
+		compute_lod(tile0, tile1, lod_frac, ps->tile >> 3, (int)ps->min_lod, st0, stdx, stdy, povf, f & RAS_TEX_LOD,
 		            f & RAS_SHARPEN_LOD, f & RAS_DETAIL_LOD);
 
 	int32_t texel0[4] = { 0, 0, 0, 0 }, texel1[4] = { 0, 0, 0, 0 };
 	if (f & RAS_USES_TEXEL0)
-		sample_texture(tx, st.tiles[tile0], st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, false, bilerp0,
-		               st.factors, texel1, texel0);
+	{
+		const TileDesc td = ldc(tiles + tile0);
+		sample_texture(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, false, bilerp0, factors,
+		               texel1, texel0);
+	}
 
 	if (f & RAS_USES_PIPELINED_TEXEL1) {
-		const SpanSetup &next = (&span)[1];
+		const SpanSetup next = ldg16(next_span);
 		const bool long_span = span.lodlength >= 8;
 		const bool end_span = x == (flip ? span.end_x : span.start_x);
 		if (end_span && long_span && next.valid) {
@@ -1620,8 +1695,8 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 		} else {
 			bool o2 = false;
 			const int d2 = dx + dir;
-			stw_to_st(span.stzw[0] + (ps.dstzw_dx[0] & ~0x1f) * d2, span.stzw[1] + (ps.dstzw_dx[1] & ~0x1f) * d2,
-			          span.stzw[3] + (ps.dstzw_dx[3] & ~0x1f) * d2, perspective, st0[0], st0[1], o2);
+			stw_to_st(span.stzw[0] + (ps->dstzw_dx[0] & ~0x1f) * d2, span.stzw[1] + (ps->dstzw_dx[1] & ~0x1f) * d2,
+			          span.stzw[3] + (ps->dstzw_dx[3] & ~0x1f) * d2, perspective, st0[0], st0[1], o2);
 		}
 		tile1 = tile0;
 		uses_texel1 = true;
@@ -1629,14 +1704,16 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 
 	if (uses_texel1) {
 		if (convert_one && !bilerp1)
-			convert_factors(texel0, st.factors, texel1);
-		else
-			sample_texture(tx, st.tiles[tile1], st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, convert_one,
-			               bilerp1, st.factors, texel0, texel1);
+			convert_factors(texel0, factors, texel1);
+		else {
+			const TileDesc td = ldc(tiles + tile1);
+			sample_texture(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, convert_one, bilerp1,
+			               factors, texel0, texel1);
+		}
 	}
 
 	int rgb_dith, alpha_dith;
-	dither_coefficients(x, y >> (interlace ? 1 : 0), (int)(st.dither >> 2), (int)(st.dither & 3), noise, rgb_dith,
+	dither_coefficients(x, y >> (interlace ? 1 : 0), (int)(ps->dither >> 2), (int)(ps->dither & 3), noise, rgb_dith,
 	                    alpha_dith);
 
 	CombIn ci;
@@ -1654,7 +1731,7 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 	int32_t alpha_ref;
 	if (multi) {
 		int32_t c0[4];
-		combine(ci, st.comb[0], st.constants[0], c0);
+		combine(ci, ps->comb[0], ps->constants[0], c0);
 		if (alpha_test) {
 			int32_t cl = clamp_9bit(c0[3]);
 			int32_t ex = cl + ((cl + 1) >> 8);
@@ -1673,12 +1750,12 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 			ci.texel1[i] = tmp;
 		}
 		if (f & RAS_NEED_NOISE_DUAL) {
-			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), ps.prim_index + 11);
+			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), ps->prim_index + 11);
 			ci.noise = (int32_t)(((n2 & 7) << 6) | 0x20);
 		}
-		combine(ci, st.comb[1], st.constants[1], combined);
+		combine(ci, ps->comb[1], ps->constants[1], combined);
 	} else {
-		combine(ci, st.comb[1], st.constants[1], combined);
+		combine(ci, ps->comb[1], ps->constants[1], combined);
 	}
 	// combiner_cycle1 tail (combiner.h:256-281)
 #pragma unroll
@@ -1698,7 +1775,7 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 
 	if (aa && cvg_count == 0) return false;
 	if (alpha_test) {
-		int32_t thr = (f & RAS_ALPHA_TEST_DITHER) ? (int32_t)(noise & 0xff) : st.blendc[3];
+		int32_t thr = (f & RAS_ALPHA_TEST_DITHER) ? (int32_t)(noise & 0xff) : ps->blendc[3];
 		if (alpha_ref < thr) return false;
 	}
 #pragma unroll
@@ -1710,13 +1787,13 @@ DEV bool shade_pixel(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 }
 
 // memory_interfacing.h:488-571
-DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st, uint32_t fmt, bool alias, Pixel &p,
+DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps, uint32_t fmt, bool alias, Pixel &p,
                      const Shaded &sh)
 {
 	const int32_t z = sh.z_dith >> 9;
 	const int dith = sh.z_dith & 0x1ff;
 	int cvg = sh.coverage;
-	const uint32_t f = st.db_flags;
+	const uint32_t f = ps->db_flags;
 	const bool force_blend = f & DB_FORCE_BLEND, z_compare = f & DB_DEPTH_TEST, z_update = f & DB_DEPTH_UPDATE;
 	const bool image_read = f & DB_IMAGE_READ, color_on_cvg = f & DB_COLOR_ON_CVG, multi = f & DB_MULTI_CYCLE;
 	const bool aa = f & DB_AA, dither_en = f & DB_DITHER;
@@ -1748,8 +1825,8 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 		int32_t memory_dz = 1 << p.dz;
 		int32_t precision = ((int32_t)p.depth >> 11) & 0xf;
 		bool coplanar = false;
-		shift0 = iclamp(ps.dz_compressed - (int32_t)p.dz, 0, 4);
-		shift1 = iclamp((int32_t)p.dz - ps.dz_compressed, 0, 4);
+		shift0 = iclamp(ps->dz_compressed - (int32_t)p.dz, 0, 4);
+		shift1 = iclamp((int32_t)p.dz - ps->dz_compressed, 0, 4);
 		if (precision < 3) {
 			if (memory_dz != 0x8000) memory_dz = imax(memory_dz << 1, 16 >> precision);
 			else {
@@ -1757,7 +1834,7 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 				memory_dz = 0xffff;
 			}
 		}
-		int32_t comb = ps.dz | memory_dz;
+		int32_t comb = ps->dz | memory_dz;
 		if (comb != 0) comb = 1 << find_msb(comb);
 		const int32_t comb_ip = comb;
 		comb <<= 3;
@@ -1768,7 +1845,7 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 		const bool max_z = memory_z == 0x3ffff;
 		const bool front = z < memory_z;
 		const bool nearer = coplanar || ((z - comb) <= memory_z);
-		switch (st.z_mode) {
+		switch (ps->z_mode) {
 		case 0: pass = max_z || (overflow ? front : nearer); break;
 		case 1:
 			if (!front || !farther || !overflow)
@@ -1785,7 +1862,7 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 		}
 	} else {
 		shift0 = 0;
-		shift1 = imin(0xf - ps.dz_compressed, 4);
+		shift1 = imin(0xf - ps->dz_compressed, 4);
 		const bool overflow = (cvg + memory_coverage) >= 8;
 		blend_en = force_blend || (!overflow && aa);
 		cvg_wrap = overflow;
@@ -1794,17 +1871,17 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 
 	if (pass && (!aa || cvg != 0)) {
 		int32_t pixel[4] = { sh.combined[0], sh.combined[1], sh.combined[2], sh.combined[3] };
-		const uint8_t *modes = st.blend[0];
+		const AS_CONST uint8_t *modes = ps->blend[0];
 		int32_t rgb[3];
 		if (multi) {
-			blender(P.blender_lut, pixel, memory, st.fog, st.blendc, sh.shade_alpha, modes, force_blend, blend_en,
+			blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, modes, force_blend, blend_en,
 			        color_on_cvg, cvg_wrap, shift0, shift1, false, rgb);
 			pixel[0] = rgb[0];
 			pixel[1] = rgb[1];
 			pixel[2] = rgb[2];
-			modes = st.blend[1];
+			modes = ps->blend[1];
 		}
-		blender(P.blender_lut, pixel, memory, st.fog, st.blendc, sh.shade_alpha, modes, force_blend, blend_en,
+		blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, modes, force_blend, blend_en,
 		        color_on_cvg, cvg_wrap, shift0, shift1, true, rgb);
 		if (dither_en) {
 #pragma unroll
@@ -1817,7 +1894,7 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 			}
 		}
 		int new_cov;
-		switch (st.coverage_mode) {
+		switch (ps->coverage_mode) {
 		case 0: new_cov = blend_en ? imin(7, memory_coverage + cvg) : ((cvg - 1) & 7); break;
 		case 1: new_cov = (cvg + memory_coverage) & 7; break;
 		case 2: new_cov = 7; break;
@@ -1826,7 +1903,7 @@ DEV void depth_blend(const PassDesc &P, const PrimSetup &ps, const DrawState &st
 		write_color(p, fmt, rgb[0], rgb[1], rgb[2], new_cov << 5);
 		if (z_update) {
 			p.depth = z_compress(z);
-			p.dz = (uint32_t)ps.dz_compressed;
+			p.dz = (uint32_t)ps->dz_compressed;
 			p.depth_dirty = true;
 			if (alias) alias_depth_to_color(p, fmt);
 		} else if (alias)
@@ -1880,8 +1957,14 @@ DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
 	if (alias) alias_color_to_depth(p, fmt);
 }
 
-__global__ __launch_bounds__(64) void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+#ifndef RDP_RASTER_WAVES_PER_EU
+#define RDP_RASTER_WAVES_PER_EU 4
+#endif
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_RASTER_WAVES_PER_EU)))
+void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
+	__shared__ uint16_t s_tmem[2048];
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 14];
 	const uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
@@ -1891,7 +1974,13 @@ __global__ __launch_bounds__(64) void k_raster(const PassDesc *__restrict__ pass
 	const uint32_t fmt = P.fb_fmt;
 	const bool in_fb = x < (int)P.fb_width && y < (int)P.fb_height;
 	const bool alias = P.fb_addr == P.depth_addr;
-	const Mem m{ P.rdram, P.hidden, P.rdram_mask };
+	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
+	const AS_CONST RasterPrim *rprims = cptr(P.rprims);
+	const AS_CONST DrawState *states = cptr(P.states);
+	const AS_GLOBAL SpanSetup *spans = gptr(P.spans);
+	const AS_GLOBAL uint8_t *lut = gptr(P.blender_lut);
+	const AS_GLOBAL uint32_t *tmem_src = (const AS_GLOBAL uint32_t *)gptr(P.tmem_instances);
+	const Tex tex{ (const AS_LDS uint16_t *)(s_tmem) };
 
 	uint32_t addr_index;
 	switch (fmt) {
@@ -1915,32 +2004,44 @@ __global__ __launch_bounds__(64) void k_raster(const PassDesc *__restrict__ pass
 
 	const uint32_t G = P.num_groups;
 	const uint32_t tile = ty * P.tiles_x + tx;
-	uint64_t my_mask = (uint32_t)lane < G ? P.bin_fine[(size_t)tile * G + lane] : 0;
+	uint64_t my_mask = (uint32_t)lane < G ? gptr(P.bin_fine)[(size_t)tile * G + lane] : 0;
 	uint64_t coarse = __ballot(my_mask != 0);
+	uint32_t staged = 0xffffffffu;
+	const uint32_t needs_tmem = RAS_USES_TEXEL0 | RAS_USES_TEXEL1 | RAS_USES_PIPELINED_TEXEL1 | RAS_COPY;
 	while (coarse) {
 		const int g = __ffsll((unsigned long long)coarse) - 1;
 		coarse &= coarse - 1;
-		uint32_t lo = __builtin_amdgcn_readlane((uint32_t)my_mask, g);
-		uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(my_mask >> 32), g);
+		const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)my_mask, g);
+		const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(my_mask >> 32), g);
 		uint64_t mask = ((uint64_t)hi << 32) | lo;
 		while (mask) {
 			const int bit = __ffsll((unsigned long long)mask) - 1;
 			mask &= mask - 1;
-			const uint32_t prim = (uint32_t)(g * 64 + bit);
-			const PrimSetup &ps = P.setups[prim];
-			const int ylo = ps.ylo, yhi = ps.yhi;
-			if (!in_fb || y < ylo || y > yhi) continue;
-			const SpanSetup &span = P.spans[ps.span_offset + (uint32_t)(y - ylo)];
+			const AS_CONST RasterPrim *ps = rprims + (g * 64 + bit);
+
+			// Stage the primitive's TMEM snapshot in LDS (wave-uniform).
+			if ((ps->ras_flags & needs_tmem) && ps->tmem_instance != staged) {
+				__syncthreads();
+				const AS_GLOBAL uint32_t *src = tmem_src + (size_t)ps->tmem_instance * 1024;
+				uint32_t *dst = reinterpret_cast<uint32_t *>(s_tmem);
+#pragma unroll
+				for (int k = 0; k < 16; k++) dst[lane * 4 + (k & 3) + 256 * (k >> 2)] = src[lane * 4 + (k & 3) + 256 * (k >> 2)];
+				__syncthreads();
+				staged = ps->tmem_instance;
+			}
+
+			if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
+			const AS_GLOBAL SpanSetup *sp = spans + (ps->span_offset + (uint32_t)(y - ps->ylo));
+			const SpanSetup span = ldg16(sp);
 			if (!span.valid) continue;
-			const DrawState &st = P.states[ps.state_index];
 			Shaded sh;
-			if (!shade_pixel(P, ps, st, span, x, y, sh)) continue;
+			if (!shade_pixel(ps, states[ps->state_index].tiles, tex, fmt, span, sp + 1, x, y, sh)) continue;
 			if (sh.coverage & COVERAGE_FILL)
-				fill_pixel(px, fmt, alias, ci, st.fill_color);
+				fill_pixel(px, fmt, alias, ci, ps->fill_color);
 			else if (sh.coverage & COVERAGE_COPY)
 				copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
 			else
-				depth_blend(P, ps, st, fmt, alias, px, sh);
+				depth_blend(lut, ps, fmt, alias, px, sh);
 		}
 	}
 

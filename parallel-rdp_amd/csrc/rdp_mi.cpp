@@ -180,7 +180,7 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 	ScratchLayout l;
 	const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
 	const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
-	l.setups = align_up(hp.prims.size() * sizeof(PrimSetup), 256);
+	l.setups = align_up(hp.prims.size() * sizeof(RasterPrim), 256);
 	l.spans = align_up((size_t)hp.num_spans * sizeof(SpanSetup), 256);
 	l.tilerows = align_up((size_t)hp.num_tilerows * 4, 256);
 	l.bins = align_up((size_t)tx * ty * groups * 8, 256);
@@ -351,7 +351,7 @@ static int prepare_rounds(Context &c, const std::vector<std::vector<PassRef>> &r
 			D.prims = reinterpret_cast<const PrimRecord *>(d + pi.prims);
 			D.uploads = reinterpret_cast<const UploadInfo *>(d + pi.uploads);
 			uint8_t *sb = scratch.ptr + pi.scratch_base;
-			D.setups = reinterpret_cast<PrimSetup *>(sb);
+			D.rprims = reinterpret_cast<RasterPrim *>(sb);
 			D.spans = reinterpret_cast<SpanSetup *>(sb + pi.sl.setups);
 			D.tilerows = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans);
 			D.bin_fine = reinterpret_cast<uint64_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows);
@@ -813,6 +813,32 @@ int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_gr
 	if (!err && iteration_ms)
 		for (unsigned i = 0; i < iterations; i++) (void)hipEventElapsedTime(&iteration_ms[i], ev[i], ev[i + 1]);
 	for (auto &e : ev) (void)hipEventDestroy(e);
+	return err;
+}
+
+int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, float *round_stage_ms)
+{
+	if (!h || iterations == 0 || !round_stage_ms) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Recording &r = h->r;
+	const size_t nev = r.launches.size() * 4;
+	std::vector<hipEvent_t> ev(nev + 1);
+	for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+	std::vector<double> acc(nev, 0.0);
+	int err = 0;
+	for (unsigned it = 0; it < iterations && !err; it++) {
+		HIP_CHECK(hipEventRecord(ev[0], c.stream));
+		err = execute_launches(r.launches, c.stream, ev.data() + 1);
+		HIP_CHECK(hipStreamSynchronize(c.stream));
+		for (size_t k = 0; k < nev && !err; k++) {
+			float ms = 0;
+			(void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+			acc[k] += ms;
+		}
+	}
+	for (auto &e : ev) (void)hipEventDestroy(e);
+	for (size_t k = 0; k < nev; k++) round_stage_ms[k] = (float)(acc[k] / iterations);
 	return err;
 }
 

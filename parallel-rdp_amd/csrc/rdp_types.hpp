@@ -153,6 +153,37 @@ struct PrimSetup {
 	uint32_t pad;
 };
 
+// Everything the per-tile kernel needs about one primitive, 256 bytes, read
+// with scalar loads: the per-pixel attribute gradients, the span range and
+// the resolved draw state of DrawState minus the texture tiles.
+struct RasterPrim {
+	int32_t drgba_dx[4], drgba_dy[4];
+	int32_t dstzw_dx[4], dstzw_dy[4];
+	uint32_t flags;          // TRI_* flags
+	uint32_t tile;           // tile index | max level << 3
+	int32_t ylo, yhi;
+	uint32_t span_offset;
+	uint32_t tmem_instance;
+	uint32_t prim_index;
+	uint32_t state_index;
+	int32_t dz, dz_compressed;
+	uint32_t ras_flags;
+	uint32_t dither;
+	uint32_t db_flags;
+	uint32_t coverage_mode;
+	uint32_t z_mode;
+	uint32_t fill_color;
+	uint32_t min_lod;
+	uint32_t pad0[3];
+	uint8_t comb[2][8];
+	uint8_t blend[2][4];
+	uint8_t constants[2][4][4];
+	uint8_t fog[4], blendc[4];
+	int32_t factors[4];
+	uint32_t pad1[8];
+};
+static_assert(sizeof(RasterPrim) == 256, "RasterPrim layout");
+
 // One scanline of one primitive (span_setup.comp output), 64 bytes.
 struct SpanSetup {
 	int32_t rgba[4];
@@ -187,7 +218,7 @@ struct PassDesc {
 	const PrimRecord *prims;
 	const UploadInfo *uploads;
 	// per-pass scratch / outputs
-	PrimSetup *setups;
+	RasterPrim *rprims;
 	SpanSetup *spans;
 	uint32_t *tilerows;        // per (prim, tile row): lo tile x | hi tile x << 16, 0xffff lo = empty
 	uint64_t *bin_fine;        // [tile][group] primitive masks

@@ -248,6 +248,13 @@ class Recording:
         _check(self._lib.rdp_mi_recording_profile(self._h, iterations, ms.ctypes.data), "profile")
         return ms
 
+    def profile_rounds(self, iterations: int = 1) -> np.ndarray:
+        self._lib.rdp_mi_recording_profile_rounds.restype = ctypes.c_int
+        self._lib.rdp_mi_recording_profile_rounds.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
+        ms = np.zeros((self.rounds, 4), dtype=np.float32)
+        _check(self._lib.rdp_mi_recording_profile_rounds(self._h, iterations, ms.ctypes.data), "profile_rounds")
+        return ms
+
     @property
     def raster_bytes(self) -> int:
         return int(self._lib.rdp_mi_recording_raster_bytes(self._h))

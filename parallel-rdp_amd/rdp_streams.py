@@ -496,11 +496,11 @@ def _clear_passes(cl: CommandList, width: int, height: int, color_size: int, cle
     cl.fill_rectangle(0, 0, (width - 1) << 2, (height - 1) << 2)
     cl.sync_pipe()
     cl.set_color_image(color_addr, width, color_size)
+    cl.set_mask_image(depth_addr)
     if clear_color:
         cl.set_fill_color(0x00010001 if color_size == SIZ_16 else 0x000000FF)
         cl.fill_rectangle(0, 0, (width - 1) << 2, (height - 1) << 2)
         cl.sync_pipe()
-    cl.set_mask_image(depth_addr)
 
 
 def config0_stream(seed: int = 0x52445000, num_triangles: int = 256, width: int = 320, height: int = 240) -> Stream:
