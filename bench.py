@@ -68,6 +68,7 @@ def oracle_results(streams):
         o.hidden[:] = s.hidden_init
         for f in s.frames:
             o.process(f.words)
+        o.flush()
         results.append((o.rdram.copy(), o.hidden.copy(), o.tmem.copy()))
         o.close()
     return results, time.perf_counter() - t0

@@ -42,6 +42,9 @@ def _load():
     lib.rdpo_tmem.argtypes = [vp]
     lib.rdpo_process.restype = sz
     lib.rdpo_process.argtypes = [vp, vp, sz]
+    lib.rdpo_flush.argtypes = [vp]
+    lib.rdpo_pass_ends.restype = sz
+    lib.rdpo_pass_ends.argtypes = [vp, vp, sz]
     lib.rdpo_set_vi_register.argtypes = [vp, ctypes.c_uint, u32]
     lib.rdpo_scanout.restype = ctypes.c_int
     lib.rdpo_scanout.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
@@ -87,8 +90,15 @@ class Oracle:
             pass
 
     def process(self, words: np.ndarray) -> int:
+        """Decode and render commands.  Pixels of a render pass that is still
+        open stay in the pass cache until flush() or a command that ends the
+        pass (SyncFull, a new colour/depth image, a flush rule)."""
         w = np.ascontiguousarray(words, dtype=np.uint32)
         return int(self._lib.rdpo_process(self._h, w.ctypes.data, w.size))
+
+    def flush(self):
+        """End the open render pass (the reference's idle())."""
+        self._lib.rdpo_flush(self._h)
 
     def set_vi_register(self, reg: int, value: int):
         self._lib.rdpo_set_vi_register(self._h, reg, value & 0xFFFFFFFF)
@@ -99,6 +109,13 @@ class Oracle:
         h = ctypes.c_uint(0)
         self._lib.rdpo_scanout(self._h, out.ctypes.data, ctypes.byref(w), ctypes.byref(h))
         return out[: w.value * h.value].reshape(h.value, w.value) if w.value and h.value else out[:0].reshape(0, 0)
+
+    def pass_ends(self) -> np.ndarray:
+        """Primitive counter at the end of every render pass that held a draw."""
+        n = int(self._lib.rdpo_pass_ends(self._h, None, 0))
+        out = np.zeros(n, dtype=np.uint64)
+        self._lib.rdpo_pass_ends(self._h, out.ctypes.data, n)
+        return out
 
     @property
     def fragments(self) -> int:

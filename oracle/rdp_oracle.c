@@ -54,6 +54,15 @@ enum {
 	RAS_BILERP0 = 1 << 23,
 	RAS_BILERP1 = 1 << 24,
 	RAS_CONVERT_ONE = 1 << 22,
+	/* Derived per draw by the host (deduce_static_texture_state and
+	 * deduce_noise_state); part of the state-cache key only. */
+	RAS_USES_TEXEL0 = 1 << 18,
+	RAS_USES_TEXEL1 = 1 << 19,
+	RAS_USES_LOD = 1 << 20,
+	RAS_USES_PIPELINED_TEXEL1 = 1 << 21,
+	RAS_NEED_NOISE_DUAL = 1 << 25,
+	RAS_NEED_NOISE = 1 << 28,
+	RAS_USE_STATIC_TEXTURE_SIZE_FORMAT = 1 << 29,
 };
 
 enum {
@@ -123,6 +132,36 @@ typedef struct {
 	int need_noise, need_noise_dual;
 } draw_t;
 
+/* Keys of the reference renderer's per-pass state caches
+ * (StaticRasterizationState, DepthBlendState, TileInfo;
+ * rdp_data_structures.hpp:154-172, rdp_common.hpp:264-290).  Two draws share
+ * a cache slot exactly when their keys compare equal byte for byte. */
+typedef struct {
+	uint8_t comb[2][8];
+	uint32_t flags, dither, texture_size, texture_fmt;
+} static_key_t;
+
+typedef struct {
+	uint8_t blend[2][4];
+	uint32_t flags, coverage_mode, z_mode;
+} depth_blend_key_t;
+
+/* Bookkeeping of the open render pass, mirroring what Renderer::need_flush
+ * and tmem_upload_needs_flush read (rdp_renderer.cpp:1464-1590, 3051-3098,
+ * 3484-3485). */
+typedef struct {
+	static_key_t static_keys[64];
+	depth_blend_key_t depth_blend_keys[64];
+	tile_t tile_keys[256];
+	unsigned num_static, num_depth_blend, num_tiles;
+	unsigned num_prims, num_span_jobs, num_uploads;
+	uint32_t max_shaded_tiles;
+	uint32_t deduced_height;
+	int color_write_pending, depth_write_pending;
+} pass_track_t;
+
+struct pixel_state;
+
 struct rdpo {
 	uint8_t *rdram;
 	uint8_t *hidden;
@@ -156,6 +195,20 @@ struct rdpo {
 	uint32_t vi_regs[14];
 
 	uint8_t blender_lut[0x8000];
+
+	/* Open render pass: colour/depth state of every pixel it has touched,
+	 * indexed y * 1024 + x, loaded on first touch and written back when the
+	 * pass ends (the reference's init_tile/finish_tile, memory_interfacing.h:
+	 * 286-362). */
+	pass_track_t pass;
+	struct pixel_state *px;
+	uint8_t *px_loaded;
+	uint32_t *px_touched;
+	size_t num_touched;
+
+	/* Primitive counter at the end of every pass that held a draw. */
+	uint64_t *pass_log;
+	size_t pass_log_len, pass_log_cap;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -344,6 +397,10 @@ void rdpo_destroy(rdpo *o)
 		return;
 	free(o->rdram);
 	free(o->hidden);
+	free(o->px);
+	free(o->px_loaded);
+	free(o->px_touched);
+	free(o->pass_log);
 	free(o);
 }
 
@@ -1535,11 +1592,11 @@ static int blend_coverage(int coverage, int memory_coverage, int blend_en, int m
 /* ------------------------------------------------------------------------ */
 /* Framebuffer memory: memory_interfacing.h:56-282, 364-486.                */
 /* ------------------------------------------------------------------------ */
-typedef struct {
+typedef struct pixel_state {
 	int32_t color[4];
 	uint32_t depth, dz;
 	int color_dirty, depth_dirty;
-	uint32_t color_index;
+	uint32_t color_index, depth_index;
 } pixel_state_t;
 
 static void load_color(const rdpo *o, uint32_t fmt, uint32_t index, int32_t c[4])
@@ -2044,7 +2101,7 @@ static void copy_pipeline(pixel_state_t *p, uint32_t fmt, int alias, uint32_t wo
 
 /* ------------------------------------------------------------------------ */
 /* Draw path: Renderer::draw_shaded_primitive and helpers,                  */
-/* rdp_renderer.cpp:717-1554, executed immediately.                         */
+/* rdp_renderer.cpp:717-1554.                                              */
 /* ------------------------------------------------------------------------ */
 static int normalize_dzpix(int dz)
 {
@@ -2233,6 +2290,318 @@ static void resolve_draw_state(rdpo *o, const attr_t *attr, draw_t *d)
 	d->need_noise_dual = need_dual;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Render passes.  The reference merges draws into render passes and keeps    */
+/* each touched pixel's colour and depth in registers for the whole pass,     */
+/* loading it once and writing it back once (depth_blend.comp:86-149 with     */
+/* init_tile/finish_tile).  For I8 colour images and for colour/depth         */
+/* aliasing that differs from writing every fragment through to RDRAM, so the */
+/* oracle keeps the same pass-resident pixel state and ends passes where the  */
+/* reference's Renderer does (rdp_renderer.cpp:679-693, 1464-1590,            */
+/* 2909-2987, 3051-3111, 3484-3485; rdp_device.cpp:263-266).  Flushes the     */
+/* reference issues when its command thread runs idle                          */
+/* (maintain_queues_idle, rdp_renderer.cpp:2403-2411) depend on timing and are */
+/* not modelled: the oracle behaves as enqueue_command_direct.                */
+/* ------------------------------------------------------------------------ */
+#define MAX_FB_LINES 1024
+#define MAX_FB_WIDTH 1024
+
+static void pass_write_back(rdpo *o)
+{
+	uint32_t fmt = o->fb_fmt;
+	int alias = o->fb_addr == o->depth_addr;
+	for (size_t i = 0; i < o->num_touched; i++) {
+		uint32_t key = o->px_touched[i];
+		pixel_state_t *p = &o->px[key];
+		if (p->color_dirty)
+			store_color(o, fmt, p->color_index, p->color);
+		if (!alias && p->depth_dirty)
+			store_depth(o, p->depth_index, p->depth, p->dz);
+		o->px_loaded[key] = 0;
+	}
+	o->num_touched = 0;
+}
+
+/* Renderer::flush_queues + reset_context. */
+static void pass_flush(rdpo *o)
+{
+	pass_write_back(o);
+	if (o->pass.num_prims) {
+		if (o->pass_log_len == o->pass_log_cap) {
+			o->pass_log_cap = o->pass_log_cap ? 2 * o->pass_log_cap : 256;
+			o->pass_log = (uint64_t *)realloc(o->pass_log, o->pass_log_cap * sizeof(uint64_t));
+			if (!o->pass_log)
+				abort();
+		}
+		o->pass_log[o->pass_log_len++] = o->prim_counter;
+	}
+	memset(&o->pass, 0, sizeof(o->pass));
+}
+
+/* normalize_combiner, rdp_renderer.cpp:1171-1292: selectors that only read
+ * per-draw constants fold into the "zero" value of their slot. */
+static uint8_t keep_or(uint8_t v, uint32_t keep_mask, uint8_t zero)
+{
+	return (v < 32 && ((keep_mask >> v) & 1)) ? v : zero;
+}
+
+static void static_state_key(const rdpo *o, uint32_t setup_tile, static_key_t *k)
+{
+	memset(k, 0, sizeof(*k));
+	uint32_t f = o->ras_flags;
+	unsigned tile = setup_tile & 7;
+
+	/* normalize_static_state, rdp_renderer.cpp:1294-1331 (fill and copy). */
+	if (f & RAS_FILL) {
+		k->flags = RAS_FILL;
+		return;
+	}
+	if (f & RAS_COPY) {
+		/* deduce_static_texture_state always finds a static format in copy
+		 * mode; the USES_* bits are never set there (the stream state is
+		 * re-copied from the command processor on every mode change). */
+		k->flags = (f & (RAS_COPY | RAS_TLUT | RAS_TLUT_TYPE | RAS_TEX_LOD | RAS_DETAIL_LOD | RAS_PERSPECTIVE |
+		                 RAS_ALPHA_TEST)) |
+		           RAS_USE_STATIC_TEXTURE_SIZE_FORMAT;
+		k->texture_fmt = o->tiles[tile].fmt;
+		k->texture_size = o->tiles[tile].size;
+		return;
+	}
+
+	/* deduce_static_texture_state, rdp_renderer.cpp:1333-1400. */
+	int multi = (f & RAS_MULTI_CYCLE) != 0;
+	int t0, t1, pt1, lodf;
+	if (multi) {
+		t0 = accesses_texel0(o->comb[0]) || accesses_texel1(o->comb[1]);
+		t1 = accesses_texel1(o->comb[0]) || accesses_texel0(o->comb[1]);
+		pt1 = 0;
+		lodf = accesses_lod_frac(o->comb[0]) || accesses_lod_frac(o->comb[1]);
+	} else {
+		t0 = accesses_texel0(o->comb[1]);
+		t1 = 0;
+		pt1 = accesses_texel1(o->comb[1]);
+		lodf = 0;
+	}
+	if (t1 && (f & RAS_CONVERT_ONE))
+		t0 = 1;
+	if (t0) f |= RAS_USES_TEXEL0;
+	if (t1) f |= RAS_USES_TEXEL1;
+	if (pt1) f |= RAS_USES_PIPELINED_TEXEL1;
+	if (lodf || (f & RAS_TEX_LOD)) f |= RAS_USES_LOD;
+	if (t0 || t1 || pt1) {
+		int physical_t1 = t1 && (!(f & RAS_CONVERT_ONE) || (f & RAS_BILERP1));
+		unsigned max_lod = setup_tile >> 3;
+		if (!(f & RAS_TEX_LOD))
+			max_lod = physical_t1 ? 1 : 0;
+		if (f & RAS_DETAIL_LOD)
+			max_lod++;
+		if (max_lod > 7)
+			max_lod = 7;
+		int same = 1;
+		for (unsigned i = 1; i <= max_lod && same; i++) {
+			const tile_t *t = &o->tiles[(tile + i) & 7];
+			same = t->fmt == o->tiles[tile].fmt && t->size == o->tiles[tile].size;
+		}
+		if (same) {
+			f |= RAS_USE_STATIC_TEXTURE_SIZE_FORMAT;
+			k->texture_fmt = o->tiles[tile].fmt;
+			k->texture_size = o->tiles[tile].size;
+		}
+	}
+
+	/* deduce_noise_state, rdp_renderer.cpp:1136-1169. */
+	if ((o->dither & 3) == 2 || ((o->dither >> 2) & 3) == 2)
+		f |= RAS_NEED_NOISE;
+	else {
+		if (multi && o->comb[0][0] == 7)
+			f |= RAS_NEED_NOISE;
+		if (o->comb[1][0] == 7)
+			f |= RAS_NEED_NOISE;
+		if (multi && o->comb[0][0] == 7 && o->comb[1][0] == 7)
+			f |= RAS_NEED_NOISE_DUAL;
+		if ((f & (RAS_ALPHA_TEST | RAS_ALPHA_TEST_DITHER)) == (RAS_ALPHA_TEST | RAS_ALPHA_TEST_DITHER))
+			f |= RAS_NEED_NOISE;
+	}
+
+	/* normalize_static_state, rdp_renderer.cpp:1321-1330. */
+	if (!(f & (RAS_MULTI_CYCLE | RAS_USES_PIPELINED_TEXEL1)))
+		f &= ~(uint32_t)(RAS_BILERP1 | RAS_CONVERT_ONE);
+	for (int c = 0; c < 2; c++) {
+		const uint8_t *in = o->comb[c];
+		uint8_t *out = k->comb[c];
+		out[0] = keep_or(in[0], 0xd7u, 8);      /* combined texel0 texel1 shade one noise */
+		out[1] = keep_or(in[1], 0x97u, 8);      /* combined texel0 texel1 shade k4 */
+		out[2] = keep_or(in[2], 0xab97u, 16);   /* + combined/texel alphas, shade alpha, lod frac, k5 */
+		out[3] = keep_or(in[3], 0x57u, 7);      /* combined texel0 texel1 shade one */
+		out[4] = keep_or(in[4], 0x57u, 7);
+		out[5] = keep_or(in[5], 0x57u, 7);
+		out[6] = keep_or(in[6], 0x17u, 7);      /* lod frac, texel0/1 alpha, shade alpha */
+		out[7] = keep_or(in[7], 0x57u, 7);
+	}
+	k->flags = f;
+	k->dither = o->dither;
+}
+
+static unsigned cache_add(void *keys, unsigned *count, const void *key, size_t size)
+{
+	uint8_t *base = (uint8_t *)keys;
+	for (unsigned i = *count; i-- > 0;)
+		if (memcmp(base + i * size, key, size) == 0)
+			return i;
+	memcpy(base + (*count) * size, key, size);
+	return (*count)++;
+}
+
+static void interpolate_x_64(const tri_t *s, int y, int flip, int64_t *left, int64_t *right)
+{
+	/* interpolate_x, rdp_renderer.cpp:962-992 (no upscaling). */
+	int64_t yh_base = s->yh & ~3;
+	int64_t ym_base = s->ym;
+	int64_t xh = (int64_t)s->xh + ((int64_t)y - yh_base) * s->dxhdy;
+	int64_t xm = (int64_t)s->xm + ((int64_t)y - yh_base) * s->dxmdy;
+	int64_t xl = (int64_t)s->xl + ((int64_t)y - ym_base) * s->dxldy;
+	if (y < s->ym)
+		xl = xm;
+	int64_t hs = xh >> 15, ls = xl >> 15;
+	int64_t l = flip ? hs : ls, r = flip ? ls : hs;
+	if (l < INT32_MIN) l = INT32_MIN;
+	if (l > INT32_MAX) l = INT32_MAX;
+	if (r < INT32_MIN) r = INT32_MIN;
+	if (r > INT32_MAX) r = INT32_MAX;
+	*left = l;
+	*right = r;
+}
+
+/* compute_conservative_max_num_tiles, rdp_renderer.cpp:994-1060. */
+static uint32_t conservative_tile_count(const rdpo *o, const tri_t *s)
+{
+	if (s->yl <= s->yh)
+		return 0;
+	int start_y = imax(s->yh & ~3, (int)o->scissor.ylo);
+	int end_y = imin((s->yl - 1) | 3, (int)o->scissor.yhi - 1);
+	if (end_y < start_y)
+		return 0;
+	int flip = (s->flags & TRI_FLIP) != 0;
+	int64_t l[4], r[4];
+	interpolate_x_64(s, start_y, flip, &l[0], &r[0]);
+	interpolate_x_64(s, end_y, flip, &l[1], &r[1]);
+	l[2] = l[3] = l[0];
+	r[2] = r[3] = r[0];
+	if (s->ym > start_y && s->ym < end_y) {
+		interpolate_x_64(s, s->ym, flip, &l[2], &r[2]);
+		interpolate_x_64(s, s->ym - 1, flip, &l[3], &r[3]);
+	}
+	int64_t sx = l[0], ex = r[0];
+	for (int i = 1; i < 4; i++) {
+		if (l[i] < sx) sx = l[i];
+		if (r[i] > ex) ex = r[i];
+	}
+	int64_t range = (sx < 0 ? -sx : sx) > (ex < 0 ? -ex : ex) ? (sx < 0 ? -sx : sx) : (ex < 0 ? -ex : ex);
+	if (range > 2047) {
+		sx = 0;
+		ex = 0x7fffffff;
+	}
+	if (sx < (int64_t)(o->scissor.xlo >> 2))
+		sx = o->scissor.xlo >> 2;
+	if (ex > (int64_t)((o->scissor.xhi + 3) >> 2) - 1)
+		ex = (int64_t)((o->scissor.xhi + 3) >> 2) - 1;
+	if (ex < sx)
+		return 0;
+	return (uint32_t)((ex / 8 - sx / 8 + 1) * (end_y / 32 - start_y / 32 + 1));
+}
+
+/* Per-draw part of draw_shaded_primitive + need_flush,
+ * rdp_renderer.cpp:1464-1521, 1523-1590. */
+static void pass_account_draw(rdpo *o, const tri_t *setup)
+{
+	pass_track_t *t = &o->pass;
+	t->max_shaded_tiles += conservative_tile_count(o, setup);
+
+	int max_sub = imin(setup->yl - 1, (int)o->scissor.yhi - 1);
+	int height = imax((max_sub >> 2) + 1, 0);
+	if ((uint32_t)height > t->deduced_height)
+		t->deduced_height = (uint32_t)height;
+
+	int min_line = imax(setup->yh, (int)o->scissor.ylo) >> 2;
+	int max_line = max_sub >> 2;
+	if (max_line >= min_line) {
+		int h = imin(imax(max_line - min_line + 2, 0), 1024);
+		t->num_span_jobs += (unsigned)((h + 63) / 64);
+	}
+
+	static_key_t sk;
+	static_state_key(o, setup->tile, &sk);
+	cache_add(t->static_keys, &t->num_static, &sk, sizeof(sk));
+	depth_blend_key_t dk;
+	memset(&dk, 0, sizeof(dk));
+	memcpy(dk.blend, o->blend, sizeof(dk.blend));
+	dk.flags = o->db_flags;
+	dk.coverage_mode = o->coverage_mode;
+	dk.z_mode = o->z_mode;
+	cache_add(t->depth_blend_keys, &t->num_depth_blend, &dk, sizeof(dk));
+	for (int i = 0; i < 8; i++)
+		cache_add(t->tile_keys, &t->num_tiles, &o->tiles[i], sizeof(tile_t));
+
+	t->color_write_pending = 1;
+	if (o->db_flags & DB_DEPTH_UPDATE)
+		t->depth_write_pending = 1;
+	t->num_prims++;
+
+	int cache_full = t->num_static == 64 || t->num_depth_blend == 64 || t->num_tiles + 8 > 256;
+	int prims_full = t->num_prims == 256;
+	int spans_full = t->num_span_jobs * 64 + 1024 > 32 * 1024;
+	int tiles_full = t->max_shaded_tiles + 128 * 128 > 0x8000;
+	if (cache_full || prims_full || spans_full || tiles_full)
+		pass_flush(o);
+}
+
+/* Renderer::tmem_upload_needs_flush, rdp_renderer.cpp:3051-3097 (compares
+ * only the start address of the texture image with the pending images). */
+static int tmem_upload_needs_flush(const rdpo *o, uint32_t addr)
+{
+	const pass_track_t *t = &o->pass;
+	uint32_t mask = o->rdram_size - 1;
+	if (t->color_write_pending) {
+		uint32_t offset = (addr - o->fb_addr) & mask;
+		uint32_t pending = t->deduced_height * o->fb_width;
+		if (o->fb_fmt == FB_RGBA5551 || o->fb_fmt == FB_I8)
+			offset >>= 1;
+		else if (o->fb_fmt == FB_RGBA8888)
+			offset >>= 2;
+		if (offset < pending)
+			return 1;
+	}
+	if (t->depth_write_pending) {
+		uint32_t offset = ((addr - o->depth_addr) & mask) >> 1;
+		if (offset < t->deduced_height * o->fb_width)
+			return 1;
+	}
+	return 0;
+}
+
+static pixel_state_t *pass_pixel(rdpo *o, uint32_t fmt, int x, int y, uint32_t ci, uint32_t di)
+{
+	if (!o->px) {
+		o->px = (pixel_state_t *)malloc(sizeof(pixel_state_t) * MAX_FB_LINES * MAX_FB_WIDTH);
+		o->px_loaded = (uint8_t *)calloc(MAX_FB_LINES * MAX_FB_WIDTH, 1);
+		o->px_touched = (uint32_t *)malloc(sizeof(uint32_t) * MAX_FB_LINES * MAX_FB_WIDTH);
+		if (!o->px || !o->px_loaded || !o->px_touched)
+			abort();
+	}
+	uint32_t key = (uint32_t)y * MAX_FB_WIDTH + (uint32_t)x;
+	pixel_state_t *p = &o->px[key];
+	if (!o->px_loaded[key]) {
+		memset(p, 0, sizeof(*p));
+		p->color_index = ci;
+		p->depth_index = di;
+		load_color(o, fmt, ci, p->color);
+		load_depth(o, di, &p->depth, &p->dz);
+		o->px_loaded[key] = 1;
+		o->px_touched[o->num_touched++] = key;
+	}
+	return p;
+}
+
 static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 {
 	uint32_t prim_index = (uint32_t)o->prim_counter++;
@@ -2259,21 +2628,20 @@ static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 	resolve_draw_state(o, attr_in, &d);
 	d.prim_index = prim_index;
 
-	if (o->fb_width == 0)
-		return;
-
 	uint32_t scissor[4] = { o->scissor.xlo, o->scissor.ylo, o->scissor.xhi, o->scissor.yhi };
 
 	/* allocate_span_jobs, rdp_renderer.cpp:1523-1554 */
 	int ylo = imax(setup->yh, (int)o->scissor.ylo) >> 2;
 	int yhi = imin(setup->yl - 1, (int)o->scissor.yhi - 1) >> 2;
-	if (yhi < ylo)
+	if (o->fb_width == 0 || yhi < ylo) {
+		pass_account_draw(o, setup);
 		return;
+	}
 
 	int nlines = yhi - ylo + 2;
 	span_t *spans = (span_t *)malloc(sizeof(span_t) * (size_t)nlines);
 	if (!spans)
-		return;
+		abort();
 	for (int i = 0; i < nlines; i++)
 		span_setup(setup, &attr, scissor, ylo + i, &spans[i]);
 
@@ -2317,26 +2685,19 @@ static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 			if (!shade_pixel(o, &pc, x, y, &sh))
 				continue;
 			o->fragments++;
-			pixel_state_t p;
-			memset(&p, 0, sizeof(p));
 			uint32_t ci = addr_index + o->fb_width * (uint32_t)y + (uint32_t)x;
 			uint32_t di = depth_index + o->fb_width * (uint32_t)y + (uint32_t)x;
-			p.color_index = ci;
-			load_color(o, fmt, ci, p.color);
-			load_depth(o, di, &p.depth, &p.dz);
+			pixel_state_t *p = pass_pixel(o, fmt, x, y, ci, di);
 			if (sh.coverage_count & COVERAGE_FILL)
-				fill_color(&p, fmt, alias, d.fill_color);
+				fill_color(p, fmt, alias, d.fill_color);
 			else if (sh.coverage_count & COVERAGE_COPY)
-				copy_pipeline(&p, fmt, alias, (uint32_t)sh.z_dith);
+				copy_pipeline(p, fmt, alias, (uint32_t)sh.z_dith);
 			else
-				depth_blend(o, &d, fmt, alias, &p, &sh);
-			if (p.color_dirty)
-				store_color(o, fmt, ci, p.color);
-			if (!alias && p.depth_dirty)
-				store_depth(o, di, p.depth, p.dz);
+				depth_blend(o, &d, fmt, alias, p, &sh);
 		}
 	}
 	free(spans);
+	pass_account_draw(o, setup);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2743,10 +3104,15 @@ static void load_tile_iteration(rdpo *o, uint32_t tile, const load_info_t *info,
 	up.tmem_fmt = (int32_t)t->fmt;
 	up.mode = (int32_t)mode;
 	apply_upload(o, &up);
+	/* One TMEM instance per upload; rdp_renderer.cpp:3482-3485. */
+	if (++o->pass.num_uploads + 1 >= 256)
+		pass_flush(o);
 }
 
 static void load_tile(rdpo *o, uint32_t tile, const load_info_t *info)
 {
+	if (tmem_upload_needs_flush(o, info->tex_addr))
+		pass_flush(o);
 	if (info->mode != UP_BLOCK) {
 		if ((info->thi >> 2) < (info->tlo >> 2))
 			return;
@@ -3137,19 +3503,28 @@ static void execute(rdpo *o, const uint32_t *w)
 		o->teximg.width = (w[0] & 0x3ff) + 1;
 		o->teximg.addr = w[1] & 0x00ffffffu;
 		break;
-	case 0x3e:
+	case 0x29: /* SyncFull: flush_and_signal, rdp_device.cpp:263-266 */
+		pass_flush(o);
+		break;
+	case 0x3e: /* set_depth_framebuffer, rdp_renderer.cpp:688-694 */
+		if ((w[1] & 0xffffff) != o->depth_addr)
+			pass_flush(o);
 		o->depth_addr = w[1] & 0xffffff;
 		break;
-	case 0x3f: {
+	case 0x3f: { /* set_color_framebuffer, rdp_renderer.cpp:679-686 */
 		unsigned fmt = (w[0] >> 21) & 7, size = (w[0] >> 19) & 3;
-		o->fb_width = (w[0] & 1023) + 1;
-		o->fb_addr = w[1] & 0xffffff;
+		uint32_t width = (w[0] & 1023) + 1, addr = w[1] & 0xffffff, ff;
 		switch (size) {
-		case 0: o->fb_fmt = FB_I4; break;
-		case 1: o->fb_fmt = FB_I8; break;
-		case 2: o->fb_fmt = fmt ? FB_IA88 : FB_RGBA5551; break;
-		default: o->fb_fmt = FB_RGBA8888; break;
+		case 0: ff = FB_I4; break;
+		case 1: ff = FB_I8; break;
+		case 2: ff = fmt ? FB_IA88 : FB_RGBA5551; break;
+		default: ff = FB_RGBA8888; break;
 		}
+		if (addr != o->fb_addr || width != o->fb_width || ff != o->fb_fmt)
+			pass_flush(o);
+		o->fb_width = width;
+		o->fb_addr = addr;
+		o->fb_fmt = ff;
 		break;
 	}
 	case 0x04: /* MetaSetQuirks */
@@ -3175,6 +3550,15 @@ size_t rdpo_process(rdpo *o, const uint32_t *words, size_t num_words)
 		i += len;
 	}
 	return i;
+}
+
+void rdpo_flush(rdpo *o) { pass_flush(o); }
+
+size_t rdpo_pass_ends(const rdpo *o, uint64_t *out, size_t cap)
+{
+	for (size_t i = 0; i < o->pass_log_len && i < cap; i++)
+		out[i] = o->pass_log[i];
+	return o->pass_log_len;
 }
 
 void rdpo_set_vi_register(rdpo *o, unsigned reg, uint32_t value)

@@ -6,8 +6,11 @@
  * check and the CPU-baseline leg of the benchmark may load it; the product
  * path never calls into it.
  *
- * The oracle is an immediate-mode renderer: every draw command is rasterised
- * and merged into RDRAM before the next command is decoded.  It follows the
+ * Draw commands are rasterised and shaded as they are decoded; their
+ * framebuffer merge is kept in a per-pass pixel cache and written to RDRAM
+ * when the render pass ends, at the same points where the reference's
+ * renderer ends its passes (see the render-pass section of rdp_oracle.c).
+ * It follows the
  * integer semantics of the reference's shaders and host setup code (citations
  * at each function in rdp_oracle.c), evaluated in plain 32-bit two's
  * complement arithmetic (build with -fwrapv).
@@ -48,6 +51,16 @@ unsigned rdpo_command_length(uint32_t first_word);
 /* Process a stream of RDP commands.  Returns the number of words consumed
  * (stops early on a truncated trailing command). */
 size_t rdpo_process(rdpo *o, const uint32_t *words, size_t num_words);
+
+/* End the open render pass: write back the colour and depth state of every
+ * pixel it touched.  Equivalent to the reference's idle()/flush; RDRAM is
+ * only complete after this (or after a SyncFull that closed the pass). */
+void rdpo_flush(rdpo *o);
+
+/* Render-pass log: the primitive counter (number of draw commands seen) at
+ * the end of every pass that contained a draw.  Returns the total number of
+ * entries; copies at most cap of them. */
+size_t rdpo_pass_ends(const rdpo *o, uint64_t *out, size_t cap);
 
 /* VI register file (reference: rdp_common.hpp VIRegister). */
 void rdpo_set_vi_register(rdpo *o, unsigned reg, uint32_t value);

@@ -9,6 +9,10 @@ namespace rdp_mi {
 
 static inline int32_t sext32(int32_t v, int bits) { return (int32_t)((uint32_t)v << (32 - bits)) >> (32 - bits); }
 
+static bool accesses_texel0(const uint8_t c[8]);
+static bool accesses_texel1(const uint8_t c[8]);
+static bool accesses_lod_frac(const uint8_t c[8]);
+
 Frontend::Frontend(uint32_t size) : rdram_size(size) {}
 
 unsigned Frontend::command_length(uint32_t first_word)
@@ -29,12 +33,305 @@ void Frontend::flush()
 		cur.fb_width = fb_width;
 		cur.fb_fmt = fb_fmt;
 		cur.depth_addr = depth_addr;
+		cur.marks.clear();
 		done.push_back(std::move(cur));
 	}
 	cur = HostPass();
 	state_dirty = true;
 	color_write_pending = false;
 	depth_write_pending = false;
+	if (pass_end_log && ref.prims) pass_end_log->push_back(prim_counter);
+	ref.reset();
+}
+
+// ---------------------------------------------------------------------------
+// Render-pass boundaries.
+//
+// The reference renders each pass tile by tile and keeps every pixel's colour
+// and depth in registers from the pass's first primitive to its last
+// (depth_blend.comp:86-149).  Ending a pass writes those registers back and
+// the next pass reads RDRAM again.  For RGBA5551, IA88, RGBA8888 and I4 that
+// round trip leaves every value a later primitive reads unchanged, so passes
+// can be merged; for I8 colour images (odd pixels store the green channel)
+// and for colour/depth aliasing it does not, and a TMEM load issued after a
+// boundary sees what the earlier passes wrote.  The front end therefore
+// tracks where the reference ends its passes (RefPassState) and either ends
+// its own pass there (I8, aliasing) or records a PassMark that can later be
+// turned into a real boundary (a load that reads pixels written before the
+// mark, or a device-side size limit).
+// ---------------------------------------------------------------------------
+void RefPassState::reset()
+{
+	statics.clear();
+	depth_blends.clear();
+	tiles.clear();
+	prims = span_jobs = uploads = max_shaded_tiles = deduced_height = 0;
+	color_write_pending = depth_write_pending = false;
+}
+
+template <typename T>
+static void cache_add(std::vector<T> &cache, const T &key)
+{
+	for (size_t i = cache.size(); i-- > 0;)
+		if (memcmp(&cache[i], &key, sizeof(T)) == 0) return;
+	cache.push_back(key);
+}
+
+static uint8_t keep_or(uint8_t v, uint32_t keep_mask, uint8_t zero)
+{
+	return (v < 32 && ((keep_mask >> v) & 1)) ? v : zero;
+}
+
+struct EdgeSetup {
+	int32_t xh, xm, xl, yh, ym, yl, dxhdy, dxmdy, dxldy;
+	bool flip;
+};
+
+static void interpolate_x_64(const EdgeSetup &s, int y, int64_t &left, int64_t &right)
+{
+	// interpolate_x, rdp_renderer.cpp:962-992 (no upscaling).
+	const int64_t yh_base = s.yh & ~3, ym_base = s.ym;
+	const int64_t xh = (int64_t)s.xh + ((int64_t)y - yh_base) * s.dxhdy;
+	const int64_t xm = (int64_t)s.xm + ((int64_t)y - yh_base) * s.dxmdy;
+	int64_t xl = (int64_t)s.xl + ((int64_t)y - ym_base) * s.dxldy;
+	if (y < s.ym) xl = xm;
+	const int64_t hs = xh >> 15, ls = xl >> 15;
+	left = std::min<int64_t>(std::max<int64_t>(s.flip ? hs : ls, INT32_MIN), INT32_MAX);
+	right = std::min<int64_t>(std::max<int64_t>(s.flip ? ls : hs, INT32_MIN), INT32_MAX);
+}
+
+// Renderer::compute_conservative_max_num_tiles, rdp_renderer.cpp:994-1060.
+static uint32_t conservative_tile_count(const EdgeSetup &s, const uint32_t scissor[4])
+{
+	if (s.yl <= s.yh) return 0;
+	const int start_y = std::max(s.yh & ~3, (int)scissor[1]);
+	const int end_y = std::min((s.yl - 1) | 3, (int)scissor[3] - 1);
+	if (end_y < start_y) return 0;
+	int64_t l[4], r[4];
+	interpolate_x_64(s, start_y, l[0], r[0]);
+	interpolate_x_64(s, end_y, l[1], r[1]);
+	l[2] = l[3] = l[0];
+	r[2] = r[3] = r[0];
+	if (s.ym > start_y && s.ym < end_y) {
+		interpolate_x_64(s, s.ym, l[2], r[2]);
+		interpolate_x_64(s, s.ym - 1, l[3], r[3]);
+	}
+	int64_t sx = std::min(std::min(l[0], l[1]), std::min(l[2], l[3]));
+	int64_t ex = std::max(std::max(r[0], r[1]), std::max(r[2], r[3]));
+	if (std::max(sx < 0 ? -sx : sx, ex < 0 ? -ex : ex) > 2047) {
+		sx = 0;
+		ex = 0x7fffffff;
+	}
+	sx = std::max<int64_t>(sx, scissor[0] >> 2);
+	ex = std::min<int64_t>(ex, (int64_t)((scissor[2] + 3) >> 2) - 1);
+	if (ex < sx) return 0;
+	return (uint32_t)((ex / 8 - sx / 8 + 1) * (end_y / 32 - start_y / 32 + 1));
+}
+
+// The per-draw half of Renderer::draw_shaded_primitive that feeds
+// need_flush (rdp_renderer.cpp:1464-1521, 1556-1590).  Returns true when the
+// reference ends its render pass after this draw.
+bool Frontend::ref_account_draw(unsigned op, const uint32_t *w)
+{
+	// Edge setup as decoded by rdp_device.cpp:268-294 and 713-821, after
+	// fixup_triangle_setup (rdp_renderer.cpp:1402-1409).
+	EdgeSetup e;
+	if (op >= 0x08 && op <= 0x0f) {
+		e.flip = (w[0] & 0x800000u) != 0;
+		e.yl = sext32((int32_t)w[0], 14);
+		e.ym = sext32((int32_t)(w[1] >> 16), 14);
+		e.yh = sext32((int32_t)w[1], 14);
+		e.xl = sext32((int32_t)w[2], 28) >> 1;
+		e.dxldy = sext32((int32_t)(w[3] >> 2), 28) >> 1;
+		e.xh = sext32((int32_t)w[4], 28) >> 1;
+		e.dxhdy = sext32((int32_t)(w[5] >> 2), 28) >> 1;
+		e.xm = sext32((int32_t)w[6], 28) >> 1;
+		e.dxmdy = sext32((int32_t)(w[7] >> 2), 28) >> 1;
+	} else {
+		uint32_t yl = w[0] & 0xfff;
+		if (ras_flags & (RAS_COPY | RAS_FILL)) yl |= 3;
+		e.flip = true;
+		e.xh = (int32_t)(((w[1] >> 12) & 0xfff) << 13);
+		e.xl = e.xm = (int32_t)(((w[0] >> 12) & 0xfff) << 13);
+		e.yl = e.ym = (int32_t)yl;
+		e.yh = (int32_t)(w[1] & 0xfff);
+		e.dxhdy = e.dxmdy = e.dxldy = 0;
+	}
+	if (e.ym < (e.yh & ~3)) e.ym = 0x7fff;
+
+	ref.max_shaded_tiles += conservative_tile_count(e, scissor);
+	const int max_sub = std::min(e.yl - 1, (int)scissor[3] - 1);
+	ref.deduced_height = std::max<uint32_t>(ref.deduced_height, (uint32_t)std::max((max_sub >> 2) + 1, 0));
+	const int min_line = std::max(e.yh, (int)scissor[1]) >> 2, max_line = max_sub >> 2;
+	if (max_line >= min_line) ref.span_jobs += (uint32_t)((std::min(max_line - min_line + 2, 1024) + 63) / 64);
+
+	// normalize_static_state after deduce_static_texture_state and
+	// deduce_noise_state (rdp_renderer.cpp:1136-1400).
+	RefPassState::StaticKey k;
+	memset(&k, 0, sizeof(k));
+	const unsigned setup_tile = op >= 0x08 && op <= 0x0f ? (w[0] >> 16) & 63 : (op == 0x36 ? 0 : (w[1] >> 24) & 7);
+	const unsigned tile = setup_tile & 7;
+	uint32_t f = ras_flags;
+	if (f & RAS_FILL) {
+		k.flags = RAS_FILL;
+	} else if (f & RAS_COPY) {
+		k.flags = (f & (RAS_COPY | RAS_TLUT | RAS_TLUT_TYPE | RAS_TEX_LOD | RAS_DETAIL_LOD | RAS_PERSPECTIVE |
+		                RAS_ALPHA_TEST)) |
+		          RAS_USE_STATIC_TEXTURE_SIZE_FORMAT;
+		k.texture_fmt = tiles[tile].fmt;
+		k.texture_size = tiles[tile].size;
+	} else {
+		const bool multi = (f & RAS_MULTI_CYCLE) != 0;
+		bool t0, t1 = false, pt1 = false, lodf = false;
+		if (multi) {
+			t0 = accesses_texel0(comb[0]) || accesses_texel1(comb[1]);
+			t1 = accesses_texel1(comb[0]) || accesses_texel0(comb[1]);
+			lodf = accesses_lod_frac(comb[0]) || accesses_lod_frac(comb[1]);
+		} else {
+			t0 = accesses_texel0(comb[1]);
+			pt1 = accesses_texel1(comb[1]);
+		}
+		if (t1 && (f & RAS_CONVERT_ONE)) t0 = true;
+		if (t0) f |= RAS_USES_TEXEL0;
+		if (t1) f |= RAS_USES_TEXEL1;
+		if (pt1) f |= RAS_USES_PIPELINED_TEXEL1;
+		if (lodf || (f & RAS_TEX_LOD)) f |= RAS_USES_LOD;
+		if (t0 || t1 || pt1) {
+			const bool physical_t1 = t1 && (!(f & RAS_CONVERT_ONE) || (f & RAS_BILERP1));
+			unsigned max_lod = setup_tile >> 3;
+			if (!(f & RAS_TEX_LOD)) max_lod = physical_t1 ? 1 : 0;
+			if (f & RAS_DETAIL_LOD) max_lod++;
+			max_lod = std::min(max_lod, 7u);
+			bool same = true;
+			for (unsigned i = 1; i <= max_lod && same; i++) {
+				const TileDesc &t = tiles[(tile + i) & 7];
+				same = t.fmt == tiles[tile].fmt && t.size == tiles[tile].size;
+			}
+			if (same) {
+				f |= RAS_USE_STATIC_TEXTURE_SIZE_FORMAT;
+				k.texture_fmt = tiles[tile].fmt;
+				k.texture_size = tiles[tile].size;
+			}
+		}
+		if ((dither & 3) == 2 || ((dither >> 2) & 3) == 2)
+			f |= RAS_NEED_NOISE;
+		else {
+			if (multi && comb[0][0] == 7) f |= RAS_NEED_NOISE;
+			if (comb[1][0] == 7) f |= RAS_NEED_NOISE;
+			if (multi && comb[0][0] == 7 && comb[1][0] == 7) f |= RAS_NEED_NOISE_DUAL;
+			if ((f & (RAS_ALPHA_TEST | RAS_ALPHA_TEST_DITHER)) == (RAS_ALPHA_TEST | RAS_ALPHA_TEST_DITHER))
+				f |= RAS_NEED_NOISE;
+		}
+		if (!(f & (RAS_MULTI_CYCLE | RAS_USES_PIPELINED_TEXEL1))) f &= ~(RAS_BILERP1 | RAS_CONVERT_ONE);
+		// normalize_combiner, rdp_renderer.cpp:1171-1292.
+		for (int c = 0; c < 2; c++) {
+			k.comb[c][0] = keep_or(comb[c][0], 0xd7u, 8);
+			k.comb[c][1] = keep_or(comb[c][1], 0x97u, 8);
+			k.comb[c][2] = keep_or(comb[c][2], 0xab97u, 16);
+			k.comb[c][3] = keep_or(comb[c][3], 0x57u, 7);
+			k.comb[c][4] = keep_or(comb[c][4], 0x57u, 7);
+			k.comb[c][5] = keep_or(comb[c][5], 0x57u, 7);
+			k.comb[c][6] = keep_or(comb[c][6], 0x17u, 7);
+			k.comb[c][7] = keep_or(comb[c][7], 0x57u, 7);
+		}
+		k.flags = f;
+		k.dither = dither;
+	}
+	cache_add(ref.statics, k);
+
+	RefPassState::DepthBlendKey dk;
+	memset(&dk, 0, sizeof(dk));
+	memcpy(dk.blend, blend, sizeof(dk.blend));
+	dk.flags = db_flags;
+	dk.coverage_mode = coverage_mode;
+	dk.z_mode = z_mode;
+	cache_add(ref.depth_blends, dk);
+	for (int i = 0; i < 8; i++) cache_add(ref.tiles, tiles[i]);
+
+	ref.color_write_pending = true;
+	if (db_flags & DB_DEPTH_UPDATE) ref.depth_write_pending = true;
+	ref.prims++;
+
+	return ref.statics.size() == 64 || ref.depth_blends.size() == 64 || ref.tiles.size() + 8 > 256 ||
+	       ref.prims == 256 || ref.span_jobs * 64 + 1024 > 32 * 1024 || ref.max_shaded_tiles + 128 * 128 > 0x8000;
+}
+
+// The reference ends a render pass here (flush_queues).
+void Frontend::ref_pass_end()
+{
+	if (fb_fmt == FB_I8 || fb_addr == depth_addr) {
+		flush();
+		return;
+	}
+	if (!cur.empty()) {
+		PassMark m;
+		m.prims = (uint32_t)cur.prims.size();
+		m.uploads = (uint32_t)cur.uploads.size();
+		m.words = (uint32_t)cur.words.size();
+		m.states = (uint32_t)cur.states.size();
+		m.spans = cur.num_spans;
+		m.tilerows = cur.num_tilerows;
+		m.fb_height = cur.fb_height;
+		m.color_written = color_write_pending;
+		m.depth_written = depth_write_pending;
+		cur.marks.push_back(m);
+	}
+	if (pass_end_log && ref.prims) pass_end_log->push_back(prim_counter);
+	ref.reset();
+}
+
+// Turn the last PassMark into a real boundary: everything before it becomes
+// a completed pass, the rest stays open.
+void Frontend::split_at_last_mark()
+{
+	const PassMark m = cur.marks.back();
+	HostPass head;
+	head.words.assign(cur.words.begin(), cur.words.begin() + m.words);
+	head.states.assign(cur.states.begin(), cur.states.begin() + m.states);
+	head.prims.assign(cur.prims.begin(), cur.prims.begin() + m.prims);
+	head.uploads.assign(cur.uploads.begin(), cur.uploads.begin() + m.uploads);
+	head.fb_addr = fb_addr;
+	head.fb_width = fb_width;
+	head.fb_fmt = fb_fmt;
+	head.depth_addr = depth_addr;
+	head.fb_height = m.fb_height;
+	head.num_spans = m.spans;
+	head.num_tilerows = m.tilerows;
+	if (!head.empty()) done.push_back(std::move(head));
+
+	HostPass tail;
+	tail.words.assign(cur.words.begin() + m.words, cur.words.end());
+	tail.uploads.assign(cur.uploads.begin() + m.uploads, cur.uploads.end());
+	std::vector<int32_t> remap(cur.states.size(), -1);
+	color_write_pending = depth_write_pending = false;
+	for (size_t i = m.prims; i < cur.prims.size(); i++) {
+		PrimRecord r = cur.prims[i];
+		if (remap[r.state_index] < 0) {
+			remap[r.state_index] = (int32_t)tail.states.size();
+			tail.states.push_back(cur.states[r.state_index]);
+		}
+		if (cur.states[r.state_index].db_flags & DB_DEPTH_UPDATE) depth_write_pending = true;
+		color_write_pending = true;
+		r.state_index = (uint16_t)remap[r.state_index];
+		r.word_offset -= m.words;
+		r.tmem_instance = (uint8_t)(r.tmem_instance - m.uploads);
+		r.span_offset -= m.spans;
+		r.tilerow_offset -= m.tilerows;
+		tail.fb_height = std::max(tail.fb_height, (uint32_t)(r.yhi + 1));
+		tail.prims.push_back(r);
+	}
+	tail.num_spans = cur.num_spans - m.spans;
+	tail.num_tilerows = cur.num_tilerows - m.tilerows;
+	cur = std::move(tail);
+	state_dirty = true;
+}
+
+void Frontend::split_or_flush()
+{
+	if (!cur.marks.empty())
+		split_at_last_mark();
+	else
+		flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -194,7 +491,6 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 {
 	validate_draw();
 	const uint32_t prim_index = (uint32_t)prim_counter++;
-	if (fb_width == 0) return;
 
 	int32_t yl, yh;
 	if (op >= 0x08 && op <= 0x0f) {
@@ -209,39 +505,40 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 	// allocate_span_jobs, rdp_renderer.cpp:1523-1554
 	const int32_t ylo = std::max(yh, (int32_t)scissor[1]) >> 2;
 	const int32_t yhi = std::min(yl - 1, (int32_t)scissor[3] - 1) >> 2;
-	if (yhi < ylo) return;
+	if (fb_width != 0 && yhi >= ylo) {
+		const uint32_t lines = (uint32_t)(yhi - ylo + 2);
+		const uint32_t rows = (uint32_t)((yhi >> 3) - (ylo >> 3) + 1);
+		if (!cur.prims.empty() && cur.num_spans + lines > max_spans_per_pass) split_or_flush();
 
-	const uint32_t lines = (uint32_t)(yhi - ylo + 2);
-	const uint32_t rows = (uint32_t)((yhi >> 3) - (ylo >> 3) + 1);
-	if (!cur.prims.empty() && cur.num_spans + lines > max_spans_per_pass) flush();
+		if (state_dirty || cur.states.empty()) {
+			DrawState d;
+			resolve_state(d);
+			cur.states.push_back(d);
+			state_dirty = false;
+		}
 
-	if (state_dirty || cur.states.empty()) {
-		DrawState d;
-		resolve_state(d);
-		cur.states.push_back(d);
-		state_dirty = false;
+		PrimRecord r;
+		memset(&r, 0, sizeof(r));
+		r.word_offset = (uint32_t)cur.words.size();
+		r.state_index = (uint16_t)(cur.states.size() - 1);
+		r.tmem_instance = (uint8_t)cur.uploads.size();
+		r.opcode = (uint8_t)op;
+		r.ylo = ylo;
+		r.yhi = yhi;
+		r.span_offset = cur.num_spans;
+		r.tilerow_offset = cur.num_tilerows;
+		r.prim_index = prim_index;
+		cur.words.insert(cur.words.end(), w, w + len);
+		cur.prims.push_back(r);
+		cur.num_spans += lines;
+		cur.num_tilerows += rows;
+		cur.fb_height = std::max(cur.fb_height, (uint32_t)(yhi + 1));
+		color_write_pending = true;
+		if (db_flags & DB_DEPTH_UPDATE) depth_write_pending = true;
 	}
 
-	PrimRecord r;
-	memset(&r, 0, sizeof(r));
-	r.word_offset = (uint32_t)cur.words.size();
-	r.state_index = (uint16_t)(cur.states.size() - 1);
-	r.tmem_instance = (uint8_t)cur.uploads.size();
-	r.opcode = (uint8_t)op;
-	r.ylo = ylo;
-	r.yhi = yhi;
-	r.span_offset = cur.num_spans;
-	r.tilerow_offset = cur.num_tilerows;
-	r.prim_index = prim_index;
-	cur.words.insert(cur.words.end(), w, w + len);
-	cur.prims.push_back(r);
-	cur.num_spans += lines;
-	cur.num_tilerows += rows;
-	cur.fb_height = std::max(cur.fb_height, (uint32_t)(yhi + 1));
-	color_write_pending = true;
-	if (db_flags & DB_DEPTH_UPDATE) depth_write_pending = true;
-
-	if (cur.prims.size() >= max_prims_per_pass || cur.states.size() >= 65535) flush();
+	if (ref_account_draw(op, w)) ref_pass_end();
+	if (cur.prims.size() >= max_prims_per_pass || cur.states.size() >= 65535) split_or_flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -249,13 +546,32 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 // (rdp_renderer.cpp:3099-3486); float reciprocals become integer divisors.
 // ---------------------------------------------------------------------------
 
-// True when the load reads RDRAM bytes that a draw already recorded in the
-// open pass writes (colour or depth image), so the pass must complete first.
-// This is a full byte-range overlap test; the reference only tests the start
-// address (rdp_renderer.cpp:3051-3097).
-bool Frontend::load_needs_flush(uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi) const
+// Renderer::tmem_upload_needs_flush, rdp_renderer.cpp:3051-3097: only the
+// texture image's start address is compared with the pending images.
+bool Frontend::ref_upload_needs_flush(uint32_t addr) const
 {
-	if (!color_write_pending && !depth_write_pending) return false;
+	const uint32_t mask = rdram_size - 1;
+	const uint32_t pending = ref.deduced_height * fb_width;
+	if (ref.color_write_pending) {
+		uint32_t offset = (addr - fb_addr) & mask;
+		if (fb_fmt == FB_RGBA5551 || fb_fmt == FB_I8)
+			offset >>= 1;
+		else if (fb_fmt == FB_RGBA8888)
+			offset >>= 2;
+		if (offset < pending) return true;
+	}
+	if (ref.depth_write_pending && (((addr - depth_addr) & mask) >> 1) < pending) return true;
+	return false;
+}
+
+// True when the load may read RDRAM bytes written by primitives recorded
+// before the open pass's last PassMark (a conservative byte-range test,
+// modulo the RDRAM size).  The reference has written those pixels back by the
+// time it runs the upload, so the pass has to be cut at that mark.
+bool Frontend::load_reads_marked_writes(uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi) const
+{
+	const PassMark &m = cur.marks.back();
+	if (!m.color_written && !m.depth_written) return false;
 	if (teximg.size == 0) return true;
 	uint64_t pixel_count, offset_pixels;
 	if (mode == UP_BLOCK) {
@@ -269,13 +585,13 @@ bool Frontend::load_needs_flush(uint32_t mode, uint32_t slo, uint32_t tlo, uint3
 		offset_pixels = (slo >> 2) + (uint64_t)teximg.width * (tlo >> 2);
 	}
 	const uint32_t shift = teximg.size - 1;
-	uint64_t start = teximg.addr + (offset_pixels << shift);
-	uint64_t end = start + (((pixel_count << shift) + 7) & ~7ull) + 16;
+	const uint64_t start = teximg.addr + (offset_pixels << shift);
+	const uint64_t end = start + (((pixel_count << shift) + 7) & ~7ull) + 16;
 	const uint64_t mask = rdram_size - 1;
 
 	auto overlaps = [&](uint64_t a0, uint64_t a1) {
 		if (a1 - a0 >= rdram_size || end - start >= rdram_size) return true;
-		// Compare modulo the RDRAM size by testing both wrapped copies.
+		// Compare modulo the RDRAM size by testing the wrapped copies.
 		uint64_t s = start & mask, e = s + (end - start);
 		uint64_t b0 = a0 & mask, b1 = b0 + (a1 - a0);
 		for (int k = -1; k <= 1; k++) {
@@ -286,9 +602,9 @@ bool Frontend::load_needs_flush(uint32_t mode, uint32_t slo, uint32_t tlo, uint3
 	};
 
 	uint32_t bpp_shift = fb_fmt == FB_RGBA8888 ? 2 : (fb_fmt == FB_RGBA5551 || fb_fmt == FB_IA88) ? 1 : 0;
-	uint64_t pixels = (uint64_t)fb_width * cur.fb_height;
-	if (color_write_pending && overlaps(fb_addr, fb_addr + ((pixels << bpp_shift) | 1))) return true;
-	if (depth_write_pending && overlaps(depth_addr, depth_addr + (pixels << 1) + 1)) return true;
+	uint64_t pixels = (uint64_t)fb_width * m.fb_height;
+	if (m.color_written && overlaps(fb_addr, fb_addr + ((pixels << bpp_shift) | 1))) return true;
+	if (m.depth_written && overlaps(depth_addr, depth_addr + (pixels << 1) + 1)) return true;
 	return false;
 }
 
@@ -301,7 +617,10 @@ void Frontend::load(uint32_t mode, const uint32_t *w)
 	if (mode == UP_TLUT && (thi >> 2) > (tlo >> 2))
 		report(VALIDATION_MULTILINE_TLUT, "Attempting to load multiple lines in TLUT.");
 
-	if (load_needs_flush(mode, slo, tlo, shi, thi)) flush();
+	if (ref_upload_needs_flush(teximg.addr))
+		flush();
+	else if (!cur.marks.empty() && load_reads_marked_writes(mode, slo, tlo, shi, thi))
+		split_at_last_mark();
 
 	if (mode != UP_BLOCK) {
 		if ((thi >> 2) < (tlo >> 2)) return;
@@ -439,7 +758,8 @@ void Frontend::load_tile_iteration(uint32_t tile, uint32_t mode, uint32_t slo, u
 	u.mode = (int32_t)m;
 	cur.uploads.push_back(u);
 	state_dirty = true;  // draws after this upload see a new TMEM snapshot
-	if (cur.uploads.size() + 1 >= (size_t)kMaxUploadsPerPass + 1) flush();
+	if (++ref.uploads + 1 >= 256) ref_pass_end();  // rdp_renderer.cpp:3484-3485
+	if (cur.uploads.size() >= (size_t)kMaxUploadsPerPass) split_or_flush();
 }
 
 // ---------------------------------------------------------------------------

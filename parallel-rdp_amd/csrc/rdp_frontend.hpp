@@ -29,14 +29,44 @@ enum ValidationError : uint32_t {
 	VALIDATION_MULTILINE_TLUT = 6,
 };
 
+// A point inside an open pass where the reference's renderer ends a render
+// pass but this front end kept going (the pass boundary is not observable
+// there).  Enough is recorded to cut the pass at that point later.
+struct PassMark {
+	uint32_t prims, uploads, words, states, spans, tilerows, fb_height;
+	bool color_written, depth_written;
+};
+
 struct HostPass {
 	std::vector<uint32_t> words;
 	std::vector<DrawState> states;
 	std::vector<PrimRecord> prims;
 	std::vector<UploadInfo> uploads;
+	std::vector<PassMark> marks;
 	uint32_t fb_addr = 0, fb_width = 0, fb_fmt = 0, depth_addr = 0, fb_height = 0;
 	uint32_t num_spans = 0, num_tilerows = 0;
 	bool empty() const { return prims.empty() && uploads.empty(); }
+};
+
+// What the reference's Renderer tracks about its open render pass to decide
+// when to end it (Renderer::need_flush and tmem_upload_needs_flush,
+// rdp_renderer.cpp:1464-1590, 3051-3097, 3484-3485; the state caches are
+// StateCache<T, N>, rdp_data_structures.hpp:256-312).
+struct RefPassState {
+	struct StaticKey {
+		uint8_t comb[2][8];
+		uint32_t flags, dither, texture_size, texture_fmt;
+	};
+	struct DepthBlendKey {
+		uint8_t blend[2][4];
+		uint32_t flags, coverage_mode, z_mode;
+	};
+	std::vector<StaticKey> statics;
+	std::vector<DepthBlendKey> depth_blends;
+	std::vector<TileDesc> tiles;
+	uint32_t prims = 0, span_jobs = 0, uploads = 0, max_shaded_tiles = 0, deduced_height = 0;
+	bool color_write_pending = false, depth_write_pending = false;
+	void reset();
 };
 
 class Frontend {
@@ -64,6 +94,10 @@ public:
 	// Largest pass size the front end will build (tunable for tests).
 	uint32_t max_prims_per_pass = kMaxPrimsPerPass;
 	uint32_t max_spans_per_pass = 1u << 20;
+
+	// When set, receives the primitive counter at the end of every render
+	// pass of the reference's renderer that held a draw (for tests).
+	std::vector<uint64_t> *pass_end_log = nullptr;
 
 private:
 	uint32_t rdram_size;
@@ -93,6 +127,7 @@ private:
 
 	HostPass cur;
 	std::vector<HostPass> done;
+	RefPassState ref;
 	std::function<void(uint32_t, const char *)> validation;
 
 	void draw(unsigned op, const uint32_t *words, unsigned len);
@@ -100,7 +135,12 @@ private:
 	void load(uint32_t mode, const uint32_t *words);
 	void load_tile_iteration(uint32_t tile, uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi,
 	                         uint32_t tmem_offset);
-	bool load_needs_flush(uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi) const;
+	bool load_reads_marked_writes(uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi) const;
+	bool ref_upload_needs_flush(uint32_t addr) const;
+	bool ref_account_draw(unsigned op, const uint32_t *words);
+	void ref_pass_end();
+	void split_at_last_mark();
+	void split_or_flush();
 	void validate_draw() const;
 	void report(uint32_t err, const char *msg) const
 	{

@@ -58,6 +58,7 @@ enum : uint32_t {
 	RAS_BILERP1 = 1u << 24,
 	RAS_NEED_NOISE_DUAL = 1u << 25,
 	RAS_NEED_NOISE = 1u << 28,
+	RAS_USE_STATIC_TEXTURE_SIZE_FORMAT = 1u << 29,
 };
 
 // Depth/blend flags (rdp_data_structures.hpp:80-90).

@@ -673,3 +673,170 @@ def batch_stream(index: int, seed_base: int = 0x52449000, modes: Sequence[str] =
 
 def batch_streams(count: int = 64, **kw) -> List[Stream]:
     return [batch_stream(i, **kw) for i in range(count)]
+
+
+# ----------------------------------------------------------------------------
+# Random-state streams: every mode bit, selector, format and primitive kind
+# drawn at random on a small framebuffer, for parity testing.
+# ----------------------------------------------------------------------------
+def _set_key_r(cl: CommandList, width: int, center: int, scale: int):
+    cl.raw([0x2B << 24, (width & 0xFFF) << 16 | (center & 0xFF) << 8 | (scale & 0xFF)])
+
+
+def _set_key_gb(cl: CommandList, width_g: int, width_b: int, center_g: int, scale_g: int, center_b: int,
+                scale_b: int):
+    cl.raw([0x2A << 24 | (width_g & 0xFFF) << 12 | (width_b & 0xFFF),
+            (center_g & 0xFF) << 24 | (scale_g & 0xFF) << 16 | (center_b & 0xFF) << 8 | (scale_b & 0xFF)])
+
+
+# (fmt, size) pairs a tile or texture image can take.
+_TEX_FORMATS = [(FMT_RGBA, SIZ_16), (FMT_RGBA, SIZ_32), (FMT_YUV, SIZ_16), (FMT_CI, SIZ_4), (FMT_CI, SIZ_8),
+                (FMT_IA, SIZ_4), (FMT_IA, SIZ_8), (FMT_IA, SIZ_16), (FMT_I, SIZ_4), (FMT_I, SIZ_8),
+                (FMT_RGBA, SIZ_8), (FMT_I, SIZ_16)]
+# (fmt, size) of colour images: I4, I8, RGBA5551, IA88, RGBA8888.
+_FB_FORMATS = [(FMT_I, SIZ_4), (FMT_I, SIZ_8), (FMT_RGBA, SIZ_16), (FMT_IA, SIZ_16), (FMT_RGBA, SIZ_32)]
+
+
+def _random_modes(rng, cycle_type: int) -> OtherModes:
+    b = lambda p=0.5: bool(rng.random() < p)
+    r4 = lambda: int(rng.integers(0, 4))
+    return OtherModes(
+        cycle_type=cycle_type, perspective=b(), detail_lod=b(0.2), sharpen_lod=b(0.2), tex_lod=b(0.3),
+        tlut=b(0.3), tlut_ia=b(0.3), sample_quad=b(), mid_texel=b(0.2), bilerp0=b(), bilerp1=b(),
+        convert_one=b(0.2), rgb_dither=r4(), alpha_dither=r4(),
+        blend=tuple(tuple(r4() for _ in range(4)) for _ in range(2)), force_blend=b(), alpha_cvg_select=b(0.3),
+        cvg_times_alpha=b(0.3), z_mode=r4(), coverage_mode=r4(), color_on_coverage=b(0.3), image_read=b(),
+        depth_write=b(), depth_test=b(), aa=b(), prim_depth=b(0.2), alpha_test_dither=b(0.2), alpha_test=b(0.3))
+
+
+def _random_combine(rng, cl: CommandList):
+    sel = lambda n: int(rng.integers(0, n))
+    c = [(sel(16), sel(16), sel(32), sel(8)) for _ in range(2)]
+    a = [(sel(8), sel(8), sel(8), sel(8)) for _ in range(2)]
+    cl.set_combine(c[0], a[0], c[1], a[1])
+
+
+def _random_colors(rng, cl: CommandList):
+    u32 = lambda: int(rng.integers(0, 1 << 32, dtype=np.int64))
+    cl.set_prim_color(u32(), min_level=int(rng.integers(0, 32)), lod_frac=int(rng.integers(0, 256)))
+    cl.set_env_color(u32())
+    cl.set_fog_color(u32())
+    cl.set_blend_color(u32())
+    cl.set_fill_color(u32())
+    cl.set_prim_depth(int(rng.integers(0, 1 << 16)), int(rng.integers(0, 1 << 16)))
+    cl.set_convert([int(v) for v in rng.integers(0, 512, size=6)])
+    _set_key_r(cl, int(rng.integers(0, 4096)), int(rng.integers(0, 256)), int(rng.integers(0, 256)))
+    _set_key_gb(cl, *[int(rng.integers(0, 4096)) for _ in range(2)], *[int(rng.integers(0, 256)) for _ in range(4)])
+
+
+def _random_tiles(rng, cl: CommandList):
+    for tile in range(8):
+        fmt, size = _TEX_FORMATS[int(rng.integers(0, len(_TEX_FORMATS)))]
+        cl.set_tile(tile, fmt, size, int(rng.integers(0, 512)) * 8, int(rng.integers(0, 48)) * 8,
+                    palette=int(rng.integers(0, 16)), mask_s=int(rng.integers(0, 9)), shift_s=int(rng.integers(0, 16)),
+                    mask_t=int(rng.integers(0, 9)), shift_t=int(rng.integers(0, 16)), clamp_s=bool(rng.random() < 0.5),
+                    mirror_s=bool(rng.random() < 0.3), clamp_t=bool(rng.random() < 0.5),
+                    mirror_t=bool(rng.random() < 0.3))
+        s0, t0 = (int(v) for v in rng.integers(0, 64, size=2))
+        s1, t1 = s0 + int(rng.integers(0, 192)), t0 + int(rng.integers(0, 192))
+        cl.set_tile_size(tile, s0, t0, s1, t1)
+
+
+def _random_load(rng, cl: CommandList):
+    fmt, size = _TEX_FORMATS[int(rng.integers(0, len(_TEX_FORMATS)))]
+    width = int(rng.integers(1, 128))
+    cl.set_texture_image(TEXTURE_ADDR + int(rng.integers(0, 4096)) * 8, width, size, fmt)
+    tile = int(rng.integers(0, 8))
+    tfmt, tsize = _TEX_FORMATS[int(rng.integers(0, len(_TEX_FORMATS)))]
+    cl.set_tile(tile, tfmt, tsize, int(rng.integers(0, 512)) * 8, int(rng.integers(0, 40)) * 8,
+                palette=int(rng.integers(0, 16)))
+    kind = int(rng.integers(0, 3))
+    s0, t0 = int(rng.integers(0, 64)), int(rng.integers(0, 64))
+    if kind == 0:
+        cl.load_tile(tile, s0, t0, s0 + int(rng.integers(0, 160)), t0 + int(rng.integers(0, 96)))
+    elif kind == 1:
+        cl.load_block(tile, s0 >> 2, t0 >> 2, (s0 >> 2) + int(rng.integers(0, 2048)), int(rng.integers(0, 4096)))
+    else:
+        cl.load_tlut(tile, s0, t0, s0 + (int(rng.integers(0, 256)) << 2), t0 + (4 if rng.random() < 0.1 else 0))
+    cl.set_tile_size(tile, 0, 0, int(rng.integers(4, 256)), int(rng.integers(4, 256)))
+
+
+def _random_triangles(rng, cl: CommandList, n: int, width: int, height: int):
+    op_bits = int(rng.integers(0, 8))
+    shade, tex, zbuf = bool(op_bits & 4), bool(op_bits & 2), bool(op_bits & 1)
+    xy = _log_uniform_triangles(rng, n, width, height, 1.0, 1.5 * max(width, height))
+    xy[:, :, 0] += rng.integers(-40, 40, size=(n, 1))
+    rgba = rng.uniform(-40.0, 300.0, size=(n, 3, 4)) if shade else None
+    z = rng.uniform(-2000.0, 0x40000 + 2000.0, size=(n, 3)) if zbuf else None
+    uv = rng.uniform(-64.0, 160.0, size=(n, 3, 2)) if tex else None
+    w = rng.uniform(0.05, 1.0, size=(n, 3)) if tex else None
+    persp = bool(tex and rng.random() < 0.6)
+    tris = triangle_commands(xy, rgba=rgba, z=z, uv=uv, w=w, perspective=persp,
+                             tile=int(rng.integers(0, 8)), max_level=int(rng.integers(0, 8)))
+    if tris.shape[0] and rng.random() < 0.3:
+        # Perturb the edge slopes and starting points a little.
+        k = tris.shape[0]
+        for col in (2, 3, 4, 5, 6, 7):
+            noise = rng.integers(-0x8000, 0x8000, size=k).astype(np.int64)
+            tris[:, col] = ((tris[:, col].astype(np.int64) + noise) & 0xFFFFFFFF).astype(np.uint32)
+    cl.extend_array(tris)
+    return int(tris.shape[0])
+
+
+def _random_rects(rng, cl: CommandList, n: int, width: int, height: int):
+    for _ in range(n):
+        x0, y0 = int(rng.integers(-16, width * 4)), int(rng.integers(-16, height * 4))
+        x1, y1 = x0 + int(rng.integers(0, width * 4)), y0 + int(rng.integers(0, height * 3))
+        x0, y0 = max(x0, 0), max(y0, 0)
+        if rng.random() < 0.4:
+            cl.fill_rectangle(x0, y0, x1, y1)
+        else:
+            cl.texture_rectangle(int(rng.integers(0, 8)), x0, y0, x1, y1, int(rng.integers(-2048, 8192)),
+                                 int(rng.integers(-2048, 8192)), int(rng.integers(-4096, 4096)),
+                                 int(rng.integers(-4096, 4096)), flip=bool(rng.random() < 0.3))
+
+
+def fuzz_stream(seed: int, segments: int = 12, width: Optional[int] = None, height: Optional[int] = None) -> Stream:
+    """A stream whose draw state is re-randomised every segment: cycle type,
+    combiner and blender selectors, depth/coverage/dither modes, constant
+    colours, tile descriptors, TMEM loads (tile, block, TLUT), triangles of
+    every opcode and rectangles.  RDRAM starts random; the framebuffer format
+    is one of I4, I8, RGBA5551, IA88, RGBA8888."""
+    rng = np.random.default_rng(seed)
+    width = width or int(rng.choice([32, 64, 96, 160]))
+    height = height or int(rng.choice([24, 48, 72, 120]))
+    fb_fmt, fb_size = _FB_FORMATS[int(rng.integers(0, len(_FB_FORMATS)))]
+    rdram = rng.integers(0, 1 << 32, size=RDRAM_SIZE // 4, dtype=np.uint64).astype(np.uint32)
+    cl = CommandList()
+    cl.set_color_image(COLOR_ADDR, width, fb_size, fb_fmt)
+    cl.set_mask_image(DEPTH_ADDR)
+    total = 0
+    for _ in range(segments):
+        if rng.random() < 0.3:
+            sx0, sy0 = int(rng.integers(0, width * 2)), int(rng.integers(0, height * 2))
+            cl.set_scissor(sx0, sy0, sx0 + int(rng.integers(0, width * 4 + 64)),
+                           sy0 + int(rng.integers(0, height * 4 + 64)), field=bool(rng.random() < 0.2),
+                           keep_odd=bool(rng.random() < 0.5))
+        else:
+            cl.set_scissor(0, 0, width << 2, height << 2)
+        cycle = int(rng.choice([0, 1, 2, 3], p=[0.4, 0.35, 0.1, 0.15]))
+        cl.set_other_modes(_random_modes(rng, cycle))
+        _random_combine(rng, cl)
+        _random_colors(rng, cl)
+        _random_tiles(rng, cl)
+        for _ in range(int(rng.integers(0, 3))):
+            _random_load(rng, cl)
+        _random_tiles(rng, cl) if rng.random() < 0.3 else None
+        if cycle >= 2:
+            _random_rects(rng, cl, int(rng.integers(1, 6)), width, height)
+            total += 1
+        else:
+            total += _random_triangles(rng, cl, int(rng.integers(1, 24)), width, height)
+            if rng.random() < 0.5:
+                _random_rects(rng, cl, int(rng.integers(1, 4)), width, height)
+        if rng.random() < 0.3:
+            cl.sync_full()
+    cl.sync_full()
+    return Stream(name=f"fuzz{seed:#x}:{width}x{height}:{fb_fmt}/{fb_size}", seed=seed, frames=[Frame(cl.words())],
+                  rdram_init=rdram, hidden_init=int(rng.integers(0, 4)), color_addr=COLOR_ADDR,
+                  depth_addr=DEPTH_ADDR, width=width, height=height, color_size=fb_size, num_triangles=total)

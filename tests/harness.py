@@ -24,6 +24,7 @@ def run_oracle(stream, frames=None):
     o.hidden[:] = stream.hidden_init
     for f in (stream.frames if frames is None else stream.frames[:frames]):
         o.process(f.words)
+    o.flush()
     out = (o.rdram.copy(), o.hidden.copy(), o.tmem.copy(), o.fragments)
     o.close()
     return out

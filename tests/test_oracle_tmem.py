@@ -30,6 +30,7 @@ def run_load(oracle_mod, words):
     o = oracle_mod.Oracle(rs.RDRAM_SIZE)
     o.rdram[:] = ramp_rdram()
     o.process(np.asarray(words, dtype=np.uint32))
+    o.flush()
     t = o.tmem.copy()
     rdram = o.rdram.copy()
     o.close()
