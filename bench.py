@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -142,26 +143,36 @@ def main():
         p.close()
 
     # ---- plain path: each pass launched on its own, no graph
-    procs = setup_processors(rdp_mi, streams)
-    enqueue_all(procs, streams)
-    plain = rdp_mi.Recording(procs, batched=False)
-    plain.replay(max(args.warmup, 1), use_graph=False)
-    plain_steps = max(3, args.steps // 5)
-    plain_ms = plain.replay(plain_steps, use_graph=False)
-    plain_exact, plain_mism = exact_frames(procs, refs)
-    log(f"[bench] plain: {plain_ms.mean():.3f} ms/step, bit-exact {plain_exact}/{len(streams)}")
-    plain.close()
-    for p in procs:
-        p.close()
+    plain_ms, plain_exact = None, 0
+    if not args.no_plain:
+        procs = setup_processors(rdp_mi, streams)
+        enqueue_all(procs, streams)
+        plain = rdp_mi.Recording(procs, batched=False)
+        plain.replay(max(args.warmup, 1), use_graph=False)
+        plain_steps = max(3, args.steps // 5)
+        plain_ms = plain.replay(plain_steps, use_graph=False)
+        plain_exact, plain_mism = exact_frames(procs, refs)
+        log(f"[bench] plain: {plain_ms.mean():.3f} ms/step, bit-exact {plain_exact}/{len(streams)}")
+        plain.close()
+        for p in procs:
+            p.close()
 
     frames_per_step = len(streams)
     ms_per_step = float(step_ms.mean())
     value = exact / (ms_per_step / 1e3)
-    plain_value = plain_exact / (float(plain_ms.mean()) / 1e3)
+    plain_value = plain_exact / (float(plain_ms.mean()) / 1e3) if plain_ms is not None else 0.0
 
     raster_ms_per_launch = float(stage_ms[3]) / max(rounds, 1)
     bytes_per_launch = raster_bytes / max(rounds, 1)
     achieved = bytes_per_launch / (raster_ms_per_launch / 1e3) / 1e9 if raster_ms_per_launch > 0 else 0.0
+
+    # HBM bytes per k_raster launch from the committed rocprof counter runs
+    # (FETCH_SIZE and WRITE_SIZE, scaled by a plain-copy calibration).
+    traffic = None
+    traffic_path = os.path.join(ROOT, "profiles", "k_raster_batch64_traffic.json")
+    if os.path.exists(traffic_path):
+        with open(traffic_path) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -189,14 +200,14 @@ def main():
         "mismatched_bytes": mism,
         "p50_frame_latency_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_frame_latency_ms": round(float(np.percentile(step_ms, 99)), 4),
-        "plain_path": {"value": round(plain_value, 3), "unit": "frames/s",
-                       "ms_per_step": round(float(plain_ms.mean()), 4),
-                       "bit_exact_frames": f"{plain_exact}/{frames_per_step}"},
+        "plain_path": None if plain_ms is None else {
+            "value": round(plain_value, 3), "unit": "frames/s", "ms_per_step": round(float(plain_ms.mean()), 4),
+            "bit_exact_frames": f"{plain_exact}/{frames_per_step}"},
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value > 0 else None,
         "stage_ms_per_step": {"setup": round(float(stage_ms[0]), 4), "tmem": round(float(stage_ms[1]), 4),
                               "bin": round(float(stage_ms[2]), 4), "raster": round(float(stage_ms[3]), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_raster", "bytes_per_launch": int(bytes_per_launch),
                      "ms_per_launch": round(raster_ms_per_launch, 4)},
         "cpu_baseline": cpu,

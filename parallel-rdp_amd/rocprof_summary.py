@@ -1,12 +1,13 @@
-"""Summarise rocprofv3 SQLite output (run_results.db) per kernel.
+"""Summarise rocprofv3 SQLite output (<name>_results.db) per kernel.
 
-    python parallel-rdp_amd/rocprof_summary.py stats  <kernel-trace .db>
+    python parallel-rdp_amd/rocprof_summary.py stats  <kernel-trace .db> [kernel-substring]
     python parallel-rdp_amd/rocprof_summary.py pmc    <pmc .db> [kernel-substring]
 
-`stats` prints call count, total and average duration per kernel.  `pmc`
+`stats` prints, per kernel, the call count and total/average duration, then
+for kernels matching the substring a breakdown by launch grid size.  `pmc`
 prints, per kernel and counter, the number of dispatches, the sum over
-dispatches and the per-dispatch average (counter values are summed over the
-per-SE/per-instance rows of one dispatch first).
+dispatches and the per-dispatch average (counter values are first summed over
+the per-instance rows of one dispatch).  FETCH_SIZE and WRITE_SIZE are in KiB.
 """
 from __future__ import annotations
 
@@ -15,40 +16,53 @@ import sqlite3
 import sys
 
 
-def stats(path: str):
+def stats(path: str, kernel_filter: str = ""):
     db = sqlite3.connect(path)
-    rows = db.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
-    print(f"{'kernel':60s} {'calls':>7s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s}")
-    for name, calls, total, avg, pct in rows:
-        print(f"{name[:60]:60s} {calls:7d} {total / 1e3:12.1f} {avg / 1e3:10.3f} {pct:6.2f}")
+    rows = db.execute("select name, start, end, grid_x from kernels").fetchall()
+    per = collections.defaultdict(list)
+    grids = collections.defaultdict(list)
+    for name, start, end, grid in rows:
+        short = name.split("(")[0]
+        per[short].append(end - start)
+        if kernel_filter and kernel_filter in name:
+            grids[(short, grid)].append(end - start)
+    total_all = sum(sum(v) for v in per.values())
+    print(f"{'kernel':48s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>10s} {'pct':>6s}")
+    for short, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        print(f"{short[:48]:48s} {len(d):7d} {sum(d) / 1e6:10.3f} {sum(d) / len(d) / 1e3:10.2f} "
+              f"{100.0 * sum(d) / total_all:6.2f}")
+    if grids:
+        print(f"\n{'kernel':32s} {'grid_x (threads)':>16s} {'calls':>7s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s}")
+        for (short, grid), d in sorted(grids.items(), key=lambda kv: (kv[0][0], -kv[0][1])):
+            print(f"{short[:32]:32s} {grid:16d} {len(d):7d} {sum(d) / len(d) / 1e3:10.2f} {min(d) / 1e3:10.2f} "
+                  f"{max(d) / 1e3:10.2f}")
 
 
 def pmc(path: str, kernel_filter: str = ""):
     db = sqlite3.connect(path)
     per_dispatch = collections.defaultdict(float)
-    names = {}
     for disp, kname, counter, value in db.execute(
             "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
         if kernel_filter and kernel_filter not in kname:
             continue
         per_dispatch[(kname, counter, disp)] += value
-        names[kname] = True
     agg = collections.defaultdict(lambda: [0, 0.0])
     for (kname, counter, disp), v in per_dispatch.items():
         a = agg[(kname, counter)]
         a[0] += 1
         a[1] += v
-    print(f"{'kernel':48s} {'counter':22s} {'dispatches':>10s} {'sum':>16s} {'avg/dispatch':>16s}")
+    print(f"{'kernel':48s} {'counter':14s} {'dispatches':>10s} {'sum':>16s} {'avg/dispatch':>16s}")
     for (kname, counter), (n, total) in sorted(agg.items()):
         short = kname.split("(")[0]
-        print(f"{short[:48]:48s} {counter:22s} {n:10d} {total:16.1f} {total / n:16.1f}")
+        print(f"{short[:48]:48s} {counter:14s} {n:10d} {total:16.1f} {total / n:16.1f}")
 
 
 if __name__ == "__main__":
     if len(sys.argv) < 3:
         print(__doc__)
         sys.exit(1)
+    flt = sys.argv[3] if len(sys.argv) > 3 else ""
     if sys.argv[1] == "stats":
-        stats(sys.argv[2])
+        stats(sys.argv[2], flt)
     else:
-        pmc(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
+        pmc(sys.argv[2], flt)
