@@ -1961,11 +1961,43 @@ DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
 #define RDP_RASTER_WAVES_PER_EU 4
 #endif
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b runs on XCD
+// b % 8) and every XCD has its own L2.  Neighbouring tiles read the same
+// primitive and span records, so consecutive chunks of RDP_XCD_CHUNK work
+// items are kept on one XCD: chunk c goes to XCD c % 8.  0 gives each XCD one
+// contiguous eighth of the items; a negative value keeps block order.
+#ifndef RDP_XCD_CHUNK
+#define RDP_XCD_CHUNK 64
+#endif
+#define RDP_NUM_XCDS 8
+
+DEV uint32_t xcd_local_item(uint32_t b, uint32_t n)
+{
+	const uint32_t xcd = b % RDP_NUM_XCDS, k = b / RDP_NUM_XCDS;
+#if RDP_XCD_CHUNK < 0
+	(void)xcd;
+	(void)k;
+	(void)n;
+	return b;
+#elif RDP_XCD_CHUNK == 0
+	const uint32_t q = n / RDP_NUM_XCDS, r = n % RDP_NUM_XCDS;
+	return xcd * q + (xcd < r ? xcd : r) + k;
+#else
+	// Block b is the k-th block of its XCD; that XCD's k-th item is in its
+	// (k / CHUNK)-th chunk.  The last, partial round of chunks keeps the
+	// identity mapping.
+	const uint32_t per_round = RDP_NUM_XCDS * RDP_XCD_CHUNK;
+	if (b >= (n / per_round) * per_round) return b;
+	const uint32_t chunk = (k / RDP_XCD_CHUNK) * RDP_NUM_XCDS + xcd;
+	return chunk * RDP_XCD_CHUNK + k % RDP_XCD_CHUNK;
+#endif
+}
+
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_RASTER_WAVES_PER_EU)))
 void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	__shared__ uint16_t s_tmem[2048];
-	const uint32_t item = items[blockIdx.x];
+	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x)];
 	const PassDesc &P = passes[item >> 14];
 	const uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
 	const int lane = threadIdx.x;
