@@ -107,6 +107,41 @@ void rdp_mi_set_vi_register(rdp_mi_processor *p, unsigned reg, uint32_t value);
  * gives 0x0. */
 int rdp_mi_scanout_sync(rdp_mi_processor *p, uint32_t *out, unsigned *width, unsigned *height);
 
+/* ScanoutOptions (video_interface.hpp:31-80) without upscaling, downscaling
+ * or external memory.  rdp_mi_scanout_options_default gives the reference's
+ * defaults: no crop, no persistence, no blending with the previous frame,
+ * upscale deinterlacing (each field scanned out progressively, 240 or 288
+ * lines) and every VI filter enabled.  The reference's conformance driver
+ * uses blend_previous_frame = 1, upscale_deinterlacing = 0
+ * (replayer_driver_parallel.cpp:194-206). */
+typedef struct rdp_mi_scanout_options {
+	unsigned crop_overscan_pixels;
+	unsigned crop_left, crop_right, crop_top, crop_bottom;
+	int crop_enable;
+	int persist_frame_on_invalid_input;
+	int blend_previous_frame;
+	int upscale_deinterlacing;
+	int vi_aa, vi_scale, vi_serrate, vi_dither_filter, vi_divot_filter, vi_gamma_dither;
+} rdp_mi_scanout_options;
+
+void rdp_mi_scanout_options_default(rdp_mi_scanout_options *opts);
+
+/* CommandProcessor::scanout_sync(colors, width, height, opts),
+ * rdp_device.cpp:1159-1182, with explicit options. */
+int rdp_mi_scanout_sync_ex(rdp_mi_processor *p, const rdp_mi_scanout_options *opts, uint32_t *out, unsigned *width,
+                           unsigned *height);
+
+/* Per-scanline HStart / XScale registers: begin_vi_register_per_scanline,
+ * set_vi_register_for_scanline, latch_vi_register_for_scanline,
+ * end_vi_register_per_scanline (rdp_device.cpp:977-995,
+ * video_interface.cpp:1126-1213).  flags / reg_bit: 1 = HStart, 2 = XScale.
+ * Latched lines must increase monotonically; the state applies to the next
+ * scanout only. */
+void rdp_mi_begin_vi_register_per_scanline(rdp_mi_processor *p, uint32_t flags);
+void rdp_mi_set_vi_register_for_scanline(rdp_mi_processor *p, uint32_t reg_bit, uint32_t value);
+void rdp_mi_latch_vi_register_for_scanline(rdp_mi_processor *p, unsigned vi_line);
+void rdp_mi_end_vi_register_per_scanline(rdp_mi_processor *p);
+
 /* ValidationInterface (rdp_renderer.hpp:75-95), installed with
  * CommandProcessor::set_validation_interface (rdp_device.cpp:230-233).
  * Reports RDP states that would hang real hardware; never aborts. */

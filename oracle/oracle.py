@@ -24,6 +24,23 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+class ScanoutOptions(ctypes.Structure):
+    """rdpo_scanout_options (the reference's ScanoutOptions without upscaling)."""
+    _fields_ = [(n, ctypes.c_uint) for n in ("crop_overscan_pixels", "crop_left", "crop_right", "crop_top",
+                                             "crop_bottom")] + \
+               [(n, ctypes.c_int) for n in ("crop_enable", "persist_frame_on_invalid_input", "blend_previous_frame",
+                                            "upscale_deinterlacing", "vi_aa", "vi_scale", "vi_serrate",
+                                            "vi_dither_filter", "vi_divot_filter", "vi_gamma_dither")]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        _load().rdpo_scanout_options_default(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
 def _load():
     global _lib
     if _lib is not None:
@@ -43,6 +60,14 @@ def _load():
     lib.rdpo_process.restype = sz
     lib.rdpo_process.argtypes = [vp, vp, sz]
     lib.rdpo_flush.argtypes = [vp]
+    lib.rdpo_scanout_ex.restype = ctypes.c_int
+    lib.rdpo_scanout_ex.argtypes = [vp, ctypes.POINTER(ScanoutOptions), vp, ctypes.POINTER(ctypes.c_uint),
+                                    ctypes.POINTER(ctypes.c_uint)]
+    lib.rdpo_scanout_options_default.argtypes = [ctypes.POINTER(ScanoutOptions)]
+    lib.rdpo_begin_vi_register_per_scanline.argtypes = [vp, u32]
+    lib.rdpo_set_vi_register_for_scanline.argtypes = [vp, u32, u32]
+    lib.rdpo_latch_vi_register_for_scanline.argtypes = [vp, ctypes.c_uint]
+    lib.rdpo_end_vi_register_per_scanline.argtypes = [vp]
     lib.rdpo_pass_ends.restype = sz
     lib.rdpo_pass_ends.argtypes = [vp, vp, sz]
     lib.rdpo_set_vi_register.argtypes = [vp, ctypes.c_uint, u32]
@@ -103,12 +128,26 @@ class Oracle:
     def set_vi_register(self, reg: int, value: int):
         self._lib.rdpo_set_vi_register(self._h, reg, value & 0xFFFFFFFF)
 
-    def scanout(self):
+    def scanout(self, options: "ScanoutOptions | None" = None):
+        """VI scanout: (height, width) uint32 RGBA8 words (R in the low byte)."""
+        opts = options if options is not None else ScanoutOptions.default()
         out = np.zeros(640 * 576, dtype=np.uint32)
         w = ctypes.c_uint(0)
         h = ctypes.c_uint(0)
-        self._lib.rdpo_scanout(self._h, out.ctypes.data, ctypes.byref(w), ctypes.byref(h))
+        self._lib.rdpo_scanout_ex(self._h, ctypes.byref(opts), out.ctypes.data, ctypes.byref(w), ctypes.byref(h))
         return out[: w.value * h.value].reshape(h.value, w.value) if w.value and h.value else out[:0].reshape(0, 0)
+
+    def begin_vi_register_per_scanline(self, flags: int = 3):
+        self._lib.rdpo_begin_vi_register_per_scanline(self._h, flags)
+
+    def set_vi_register_for_scanline(self, reg_bit: int, value: int):
+        self._lib.rdpo_set_vi_register_for_scanline(self._h, reg_bit, value & 0xFFFFFFFF)
+
+    def latch_vi_register_for_scanline(self, vi_line: int):
+        self._lib.rdpo_latch_vi_register_for_scanline(self._h, vi_line)
+
+    def end_vi_register_per_scanline(self):
+        self._lib.rdpo_end_vi_register_per_scanline(self._h)
 
     def pass_ends(self) -> np.ndarray:
         """Primitive counter at the end of every render pass that held a draw."""

@@ -13,6 +13,7 @@
  * 2^32 and shifts of negative values are arithmetic.
  */
 #include "rdp_oracle.h"
+#include "vi_oracle.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -206,6 +207,10 @@ struct rdpo {
 	uint32_t *px_touched;
 	size_t num_touched;
 
+	/* Video interface state (frame counter, previous image, per-scanline
+	 * register latches). */
+	vi_state_t *vi;
+
 	/* Primitive counter at the end of every pass that held a draw. */
 	uint64_t *pass_log;
 	size_t pass_log_len, pass_log_cap;
@@ -386,6 +391,12 @@ rdpo *rdpo_create(uint32_t rdram_size)
 		return NULL;
 	}
 	memset(o->hidden, 0x03, rdram_size / 2);
+	o->vi = (vi_state_t *)malloc(sizeof(vi_state_t));
+	if (!o->vi) {
+		rdpo_destroy(o);
+		return NULL;
+	}
+	vi_state_init(o->vi);
 	o->quirks = 1; /* native_resolution_tex_rect defaults on (rdp_device.hpp:80) */
 	rdpo_blender_divider_table(o->blender_lut);
 	return o;
@@ -401,6 +412,7 @@ void rdpo_destroy(rdpo *o)
 	free(o->px_loaded);
 	free(o->px_touched);
 	free(o->pass_log);
+	free(o->vi);
 	free(o);
 }
 
@@ -3560,6 +3572,28 @@ size_t rdpo_pass_ends(const rdpo *o, uint64_t *out, size_t cap)
 		out[i] = o->pass_log[i];
 	return o->pass_log_len;
 }
+
+int rdpo_scanout_ex(rdpo *o, const rdpo_scanout_options *opts, uint32_t *out, unsigned *width, unsigned *height)
+{
+	/* CommandProcessor::scanout flushes the renderer first (rdp_device.cpp:1095-1106). */
+	pass_flush(o);
+	return vi_scanout(o->vi, o->vi_regs, o->rdram, o->hidden, o->rdram_size, opts, out, width, height);
+}
+
+int rdpo_scanout(rdpo *o, uint32_t *out, unsigned *width, unsigned *height)
+{
+	rdpo_scanout_options opts;
+	rdpo_scanout_options_default(&opts);
+	return rdpo_scanout_ex(o, &opts, out, width, height);
+}
+
+void rdpo_begin_vi_register_per_scanline(rdpo *o, uint32_t flags) { vi_begin_per_scanline(o->vi, o->vi_regs, flags); }
+void rdpo_set_vi_register_for_scanline(rdpo *o, uint32_t reg_bit, uint32_t value)
+{
+	vi_set_for_scanline(o->vi, reg_bit, value);
+}
+void rdpo_latch_vi_register_for_scanline(rdpo *o, unsigned vi_line) { vi_latch_for_scanline(o->vi, vi_line); }
+void rdpo_end_vi_register_per_scanline(rdpo *o) { vi_end_per_scanline(o->vi); }
 
 void rdpo_set_vi_register(rdpo *o, unsigned reg, uint32_t value)
 {

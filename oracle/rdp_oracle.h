@@ -65,10 +65,35 @@ size_t rdpo_pass_ends(const rdpo *o, uint64_t *out, size_t cap);
 /* VI register file (reference: rdp_common.hpp VIRegister). */
 void rdpo_set_vi_register(rdpo *o, unsigned reg, uint32_t value);
 
-/* Scan out the current frame through the VI filters.  out must hold
- * 640 x 576 RGBA8 words; width/height receive the output size, 0x0 for a
- * blank or invalid frame.  Returns 0 on success. */
+/* Scanout options (ScanoutOptions, video_interface.hpp:31-80; upscaling,
+ * downscaling and external memory are not part of the path). */
+typedef struct {
+	unsigned crop_overscan_pixels;
+	unsigned crop_left, crop_right, crop_top, crop_bottom;
+	int crop_enable;
+	int persist_frame_on_invalid_input;
+	int blend_previous_frame;
+	int upscale_deinterlacing;
+	int vi_aa, vi_scale, vi_serrate, vi_dither_filter, vi_divot_filter, vi_gamma_dither;
+} rdpo_scanout_options;
+
+/* The reference's defaults: no crop, no persistence or blending, upscale
+ * deinterlacing (fields come out progressive), every VI filter on. */
+void rdpo_scanout_options_default(rdpo_scanout_options *opts);
+
+/* Scan out the current frame through the VI filters (ends the open render
+ * pass first).  out must hold 640 x 576 RGBA8 words (byte order R, G, B, A);
+ * width/height receive the output size, 0x0 for a blank or invalid frame.
+ * rdpo_scanout uses the default options.  Returns 0. */
 int rdpo_scanout(rdpo *o, uint32_t *out, unsigned *width, unsigned *height);
+int rdpo_scanout_ex(rdpo *o, const rdpo_scanout_options *opts, uint32_t *out, unsigned *width, unsigned *height);
+
+/* Per-scanline HStart / XScale registers (video_interface.cpp:1126-1213);
+ * flags and reg_bit: 1 = HStart, 2 = XScale. */
+void rdpo_begin_vi_register_per_scanline(rdpo *o, uint32_t flags);
+void rdpo_set_vi_register_for_scanline(rdpo *o, uint32_t reg_bit, uint32_t value);
+void rdpo_latch_vi_register_for_scanline(rdpo *o, unsigned vi_line);
+void rdpo_end_vi_register_per_scanline(rdpo *o);
 
 /* Instrumentation: number of (pixel, primitive) fragments that reached the
  * depth/blend stage, and number of draw commands seen. */

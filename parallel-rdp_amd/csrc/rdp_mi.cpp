@@ -23,6 +23,7 @@
 #include "../../include/rdp_mi.h"
 #include "rdp_frontend.hpp"
 #include "rdp_types.hpp"
+#include "rdp_vi.hpp"
 
 namespace rdp_mi {
 
@@ -30,7 +31,6 @@ hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, hipStream_
 hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_raster(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
-int vi_scanout(struct Processor &p, hipStream_t stream, uint32_t *out, unsigned *width, unsigned *height);
 
 static thread_local std::string g_last_error;
 
@@ -115,6 +115,7 @@ struct Context {
 	bool ok = false;
 	hipStream_t stream = nullptr;
 	uint8_t *d_lut = nullptr;
+	uint8_t *d_gamma = nullptr;
 	DeviceBuffer input, scratch;
 	PinnedBuffer staging;
 	hipEvent_t staging_done = nullptr;
@@ -132,6 +133,10 @@ struct Context {
 		blender_divider_table(lut.data());
 		HIP_CHECK(hipMalloc(&d_lut, 0x8000));
 		HIP_CHECK(hipMemcpy(d_lut, lut.data(), 0x8000, hipMemcpyHostToDevice));
+		std::vector<uint8_t> gamma(256 + 256 * 64);
+		vi_gamma_table(gamma.data());
+		HIP_CHECK(hipMalloc(&d_gamma, gamma.size()));
+		HIP_CHECK(hipMemcpy(d_gamma, gamma.data(), gamma.size(), hipMemcpyHostToDevice));
 		ok = true;
 		return 0;
 	}
@@ -153,6 +158,7 @@ struct Processor {
 	std::vector<uint8_t> rdram_mirror, hidden_mirror;
 	uint16_t tmem_mirror[2048];
 	uint32_t vi_regs[RDP_MI_VI_COUNT] = {};
+	ViState vi;
 	uint64_t timeline = 0;
 	std::map<uint64_t, hipEvent_t> timeline_events;
 	rdp_mi_validation_fn validation_fn = nullptr;
@@ -544,6 +550,7 @@ void rdp_mi_destroy(rdp_mi_processor *h)
 	rdp_mi_idle(h);
 	Processor &p = h->p;
 	for (auto &kv : p.timeline_events) (void)hipEventDestroy(kv.second);
+	p.vi.release();
 	(void)hipFree(p.d_rdram);
 	(void)hipFree(p.d_hidden);
 	(void)hipFree(p.d_tmem);
@@ -703,14 +710,55 @@ void rdp_mi_set_vi_register(rdp_mi_processor *h, unsigned reg, uint32_t value)
 	h->p.vi_regs[reg] = value;
 }
 
-int rdp_mi_scanout_sync(rdp_mi_processor *h, uint32_t *out, unsigned *width, unsigned *height)
+void rdp_mi_scanout_options_default(rdp_mi_scanout_options *o)
 {
-	if (!h || !out || !width || !height) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	if (!o) return;
+	memset(o, 0, sizeof(*o));
+	o->upscale_deinterlacing = 1;
+	o->vi_aa = o->vi_scale = o->vi_serrate = o->vi_dither_filter = o->vi_divot_filter = o->vi_gamma_dither = 1;
+}
+
+int rdp_mi_scanout_sync_ex(rdp_mi_processor *h, const rdp_mi_scanout_options *opts, uint32_t *out, unsigned *width,
+                           unsigned *height)
+{
+	if (!h || !opts || !out || !width || !height) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	// CommandProcessor::scanout drains and flushes rendering first (rdp_device.cpp:1082-1106).
 	int err = rdp_mi_idle(h);
 	if (err) return err;
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
-	return vi_scanout(h->p, c.stream, out, width, height);
+	Processor &p = h->p;
+	err = vi_scanout(p.vi, p.vi_regs, p.d_rdram, p.d_hidden, p.rdram_size, c.d_gamma, c.stream, *opts, out, width,
+	                 height);
+	if (err) return set_error(err, "VI scanout failed");
+	return 0;
+}
+
+int rdp_mi_scanout_sync(rdp_mi_processor *h, uint32_t *out, unsigned *width, unsigned *height)
+{
+	rdp_mi_scanout_options opts;
+	rdp_mi_scanout_options_default(&opts);
+	return rdp_mi_scanout_sync_ex(h, &opts, out, width, height);
+}
+
+void rdp_mi_begin_vi_register_per_scanline(rdp_mi_processor *h, uint32_t flags)
+{
+	if (h) h->p.vi.begin_per_scanline(h->p.vi_regs, flags);
+}
+
+void rdp_mi_set_vi_register_for_scanline(rdp_mi_processor *h, uint32_t reg_bit, uint32_t value)
+{
+	if (h) h->p.vi.set_for_scanline(reg_bit, value);
+}
+
+void rdp_mi_latch_vi_register_for_scanline(rdp_mi_processor *h, unsigned vi_line)
+{
+	if (h) h->p.vi.latch_for_scanline(vi_line);
+}
+
+void rdp_mi_end_vi_register_per_scanline(rdp_mi_processor *h)
+{
+	if (h) h->p.vi.end_per_scanline();
 }
 
 void rdp_mi_set_validation_callback(rdp_mi_processor *h, rdp_mi_validation_fn fn, void *user)
@@ -889,12 +937,3 @@ void rdp_mi_recording_destroy(rdp_mi_recording *h)
 
 }  // extern "C"
 
-namespace rdp_mi {
-// VI scanout is implemented in rdp_vi.hip; until then a blank frame.
-__attribute__((weak)) int vi_scanout(Processor &, hipStream_t, uint32_t *, unsigned *width, unsigned *height)
-{
-	*width = 0;
-	*height = 0;
-	return 0;
-}
-}  // namespace rdp_mi

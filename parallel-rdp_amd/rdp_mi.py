@@ -27,6 +27,23 @@ class RDPError(RuntimeError):
     pass
 
 
+class ScanoutOptions(ctypes.Structure):
+    """rdp_mi_scanout_options (the reference's ScanoutOptions without upscaling)."""
+    _fields_ = [(n, ctypes.c_uint) for n in ("crop_overscan_pixels", "crop_left", "crop_right", "crop_top",
+                                             "crop_bottom")] + \
+               [(n, ctypes.c_int) for n in ("crop_enable", "persist_frame_on_invalid_input", "blend_previous_frame",
+                                            "upscale_deinterlacing", "vi_aa", "vi_scale", "vi_serrate",
+                                            "vi_dither_filter", "vi_divot_filter", "vi_gamma_dither")]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        load_library().rdp_mi_scanout_options_default(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
 def load_library():
     global _lib
     if _lib is not None:
@@ -65,6 +82,14 @@ def load_library():
     lib.rdp_mi_set_vi_register.argtypes = [vp, cu, u32]
     lib.rdp_mi_scanout_sync.restype = ctypes.c_int
     lib.rdp_mi_scanout_sync.argtypes = [vp, vp, ctypes.POINTER(cu), ctypes.POINTER(cu)]
+    lib.rdp_mi_scanout_options_default.argtypes = [ctypes.POINTER(ScanoutOptions)]
+    lib.rdp_mi_scanout_sync_ex.restype = ctypes.c_int
+    lib.rdp_mi_scanout_sync_ex.argtypes = [vp, ctypes.POINTER(ScanoutOptions), vp, ctypes.POINTER(cu),
+                                           ctypes.POINTER(cu)]
+    lib.rdp_mi_begin_vi_register_per_scanline.argtypes = [vp, u32]
+    lib.rdp_mi_set_vi_register_for_scanline.argtypes = [vp, u32, u32]
+    lib.rdp_mi_latch_vi_register_for_scanline.argtypes = [vp, cu]
+    lib.rdp_mi_end_vi_register_per_scanline.argtypes = [vp]
     lib.rdp_mi_set_validation_callback.argtypes = [vp, VALIDATION_FN, vp]
     lib.rdp_mi_batch_flush.restype = ctypes.c_int
     lib.rdp_mi_batch_flush.argtypes = [ctypes.POINTER(vp), cu]
@@ -185,14 +210,30 @@ class CommandProcessor:
     def set_vi_register(self, reg: int, value: int):
         self._lib.rdp_mi_set_vi_register(self._h, reg, value & 0xFFFFFFFF)
 
-    def scanout_sync(self) -> np.ndarray:
+    def scanout_sync(self, options: Optional[ScanoutOptions] = None) -> np.ndarray:
+        """VI scanout as a (height, width) array of RGBA8 words (R in the low byte);
+        (0, 0) for a blank or invalid frame."""
+        opts = options if options is not None else ScanoutOptions.default()
         out = np.zeros(640 * 576, dtype=np.uint32)
         w = ctypes.c_uint(0)
         h = ctypes.c_uint(0)
-        _check(self._lib.rdp_mi_scanout_sync(self._h, out.ctypes.data, ctypes.byref(w), ctypes.byref(h)), "scanout")
+        _check(self._lib.rdp_mi_scanout_sync_ex(self._h, ctypes.byref(opts), out.ctypes.data, ctypes.byref(w),
+                                                ctypes.byref(h)), "scanout")
         if not w.value or not h.value:
             return out[:0].reshape(0, 0)
         return out[: w.value * h.value].reshape(h.value, w.value)
+
+    def begin_vi_register_per_scanline(self, flags: int = 3):
+        self._lib.rdp_mi_begin_vi_register_per_scanline(self._h, flags)
+
+    def set_vi_register_for_scanline(self, reg_bit: int, value: int):
+        self._lib.rdp_mi_set_vi_register_for_scanline(self._h, reg_bit, value & 0xFFFFFFFF)
+
+    def latch_vi_register_for_scanline(self, vi_line: int):
+        self._lib.rdp_mi_latch_vi_register_for_scanline(self._h, vi_line)
+
+    def end_vi_register_per_scanline(self):
+        self._lib.rdp_mi_end_vi_register_per_scanline(self._h)
 
     # -- validation -------------------------------------------------------
     def set_validation_callback(self, record: bool = True):
