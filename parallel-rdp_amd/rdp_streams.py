@@ -592,6 +592,177 @@ def config1_stream(seed: int = 0x52446000, frames: int = 60, tris_per_frame: int
 
 
 # ----------------------------------------------------------------------------
+# 2-cycle stress workload
+# ----------------------------------------------------------------------------
+VI_TYPE_RGBA5551 = 2
+VI_AA_RESAMP_EXTRA = 1 << 8
+VI_DIVOT = 1 << 4
+VI_DITHER_FILTER = 1 << 16
+
+
+def vi_start_register(start: int, end: int) -> int:
+    """make_vi_start_register, rdp_common.hpp:365-368."""
+    return ((start & 0x3FF) << 16) | (end & 0x3FF)
+
+
+def vi_scale_register(scale: int, bias: int) -> int:
+    """make_vi_scale_register, rdp_common.hpp:370-373."""
+    return ((bias & 0xFFF) << 16) | (scale & 0xFFF)
+
+
+MIP_SIZES = (32, 16, 8, 4)
+TEXTURE_STRIDE = 0x2000       # RDRAM bytes per CI texture (4 levels + TLUT)
+TLUT_OFFSET = 0x1800
+
+
+def _ci_level_layout(size_bits: int):
+    """TMEM byte offset and line stride of each mip level (lower TMEM)."""
+    offs, strides, off = [], [], 0
+    for s in MIP_SIZES:
+        line = max(8, (s * (8 if size_bits == SIZ_8 else 4) // 8 + 7) & ~7)
+        offs.append(off)
+        strides.append(line)
+        off += line * s
+    return offs, strides
+
+
+def _write_ci_texture(rdram: np.ndarray, rng, index: int, size_bits: int) -> int:
+    """Random CI4/CI8 mip chain and its TLUT (RGBA5551 entries) in RDRAM."""
+    base = TEXTURE_ADDR + index * TEXTURE_STRIDE
+    addr = base
+    for s in MIP_SIZES:
+        nbytes = s * s if size_bits == SIZ_8 else s * s // 2
+        rdram_write8(rdram, addr, rng.integers(0, 256, size=nbytes))
+        addr += (nbytes + 7) & ~7
+    entries = 256 if size_bits == SIZ_8 else 256  # CI4 uses 16 palettes of 16
+    rdram_write16(rdram, base + TLUT_OFFSET, rng.integers(0, 1 << 16, size=entries))
+    return base
+
+
+def _emit_ci_texture(cl: CommandList, index: int, size_bits: int, palette: int):
+    """Load the 4 mip levels with LoadTile (CI4 as 8-bit texels of half the
+    width) and the TLUT with LoadTLUT, then set render tiles 0-3."""
+    base = TEXTURE_ADDR + index * TEXTURE_STRIDE
+    offs, strides = _ci_level_layout(size_bits)
+    addr = base
+    cl.sync_load()
+    for lvl, s in enumerate(MIP_SIZES):
+        nbytes = s * s if size_bits == SIZ_8 else s * s // 2
+        w = s if size_bits == SIZ_8 else max(s // 2, 1)
+        cl.set_texture_image(addr, w, SIZ_8, FMT_CI)
+        cl.set_tile(7, FMT_CI, SIZ_8, offs[lvl], strides[lvl])
+        cl.load_tile(7, 0, 0, (w - 1) << 2, (s - 1) << 2)
+        addr += (nbytes + 7) & ~7
+    n = 256 if size_bits == SIZ_8 else 16
+    first = 0 if size_bits == SIZ_8 else palette * 16
+    cl.set_texture_image(base + TLUT_OFFSET + 2 * first, n, SIZ_16, FMT_RGBA)
+    cl.set_tile(7, FMT_RGBA, SIZ_4, 0x800 + first * 8, 0)
+    cl.load_tlut(7, 0, 0, (n - 1) << 2, 0)
+    cl.sync_tile()
+    for lvl, s in enumerate(MIP_SIZES):
+        m = s.bit_length() - 1
+        cl.set_tile(lvl, FMT_CI, size_bits, offs[lvl], strides[lvl], palette=palette if size_bits == SIZ_4 else 0,
+                    mask_s=m, mask_t=m, shift_s=lvl, shift_t=lvl)
+        cl.set_tile_size(lvl, 0, 0, (s - 1) << 2, (s - 1) << 2)
+
+
+C2_MODES = OtherModes(cycle_type=1, perspective=True, tex_lod=True, tlut=True, sample_quad=True, bilerp0=True,
+                      bilerp1=True, blend=((3, 2, 0, 0), (0, 0, 1, 1)), aa=True, image_read=True,
+                      alpha_cvg_select=True, depth_test=True, depth_write=True, z_mode=0, coverage_mode=0)
+# cycle 0: lerp(TEXEL0, TEXEL1, LOD_FRAC); cycle 1: COMBINED * SHADE
+C2_COMBINE = (((RGB_TEXEL1, RGB_TEXEL0, MUL_LOD_FRAC, RGB_TEXEL0), (A_TEXEL1, A_TEXEL0, AMUL_LOD_FRAC, A_TEXEL0)),
+              ((RGB_COMBINED, MULSUB_ZERO, RGB_SHADE, ADD_ZERO), (A_COMBINED, A_ZERO, A_SHADE, A_ZERO)))
+# Sprites (texture rectangles): no depth, 2-cycle, cycle 1 passes the texture
+# through scaled by the environment colour.
+C2_RECT_MODES = OtherModes(cycle_type=1, tlut=True, sample_quad=True, bilerp0=True, bilerp1=True,
+                           blend=((0, 0, 0, 0), (0, 0, 1, 1)), aa=True, image_read=True, alpha_cvg_select=True,
+                           coverage_mode=0)
+C2_RECT_COMBINE = (((MULADD_ZERO, MULSUB_ZERO, MUL_ZERO, RGB_TEXEL0), (A_ZERO, A_ZERO, AMUL_ZERO, A_TEXEL0)),
+                   ((RGB_COMBINED, MULSUB_ZERO, RGB_ENV, ADD_ZERO), (A_ZERO, A_ZERO, AMUL_ZERO, A_COMBINED)))
+
+
+def c2_vi_registers(width: int, height: int, origin: int = COLOR_ADDR) -> Dict[int, int]:
+    """RGBA5551 scanout with AA (resample extra), divot and dither filter."""
+    return {VI_CONTROL: VI_TYPE_RGBA5551 | VI_AA_RESAMP_EXTRA | VI_DIVOT | VI_DITHER_FILTER,
+            VI_ORIGIN: origin, VI_WIDTH: width, VI_V_SYNC: 525,
+            VI_V_START: vi_start_register(34, 34 + 480), VI_H_START: vi_start_register(108, 108 + 640),
+            VI_X_SCALE: vi_scale_register((width * 1024) // 640, 0),
+            VI_Y_SCALE: vi_scale_register((height * 1024) // 240, 0)}
+
+
+def _emit_c2(cl: CommandList, rng, n: int, width: int, height: int, num_textures: int,
+             prims_per_texture: int = 64) -> int:
+    """2-cycle CI textured triangles with mip LOD, fog and coverage AA, plus
+    20 % texture rectangles drawn as sprites; returns the number of
+    primitives emitted."""
+    cl.set_fog_color(int(rng.integers(0, 1 << 32, dtype=np.int64)))
+    cl.set_env_color(int(rng.integers(0x40404000, 1 << 32, dtype=np.int64)))
+    n_rect = n // 5
+    n_tri = n - n_rect
+    xy = _log_uniform_triangles(rng, n_tri, width, height, 4.0, 160.0)
+    rgba = rng.integers(32, 256, size=(n_tri, 3, 4)).astype(np.float64)
+    z = rng.uniform(0, 0x3FFFF, size=(n_tri, 3))
+    uv = rng.uniform(-64.0, 256.0, size=(n_tri, 3, 2))
+    w = rng.uniform(0.3, 1.0, size=(n_tri, 3))
+    tris = triangle_commands(xy, rgba=rgba, z=z, uv=uv, w=w, perspective=True, tile=0, max_level=3)
+    rects = []
+    for _ in range(n_rect):
+        x0 = int(rng.integers(0, width * 4))
+        y0 = int(rng.integers(0, height * 4))
+        rw, rh = int(rng.integers(8, 48)) * 4, int(rng.integers(8, 48)) * 4
+        rects.append((x0, y0, x0 + rw, y0 + rh, int(rng.integers(-512, 2048)), int(rng.integers(-512, 2048)),
+                      int(rng.choice([512, 1024, 2048])), int(rng.choice([512, 1024, 2048]))))
+    ti = ri = 0
+    emitted = 0
+    per_group_rects = max(prims_per_texture // 5, 1)
+    while ti < tris.shape[0] or ri < len(rects):
+        t = int(rng.integers(0, num_textures))
+        size_bits = SIZ_8 if t % 2 == 0 else SIZ_4
+        _emit_ci_texture(cl, t, size_bits, palette=int(rng.integers(0, 16)))
+        k = min(prims_per_texture - per_group_rects, tris.shape[0] - ti)
+        if k > 0:
+            cl.sync_pipe()
+            cl.set_other_modes(C2_MODES)
+            cl.set_combine(C2_COMBINE[0][0], C2_COMBINE[0][1], C2_COMBINE[1][0], C2_COMBINE[1][1])
+            cl.extend_array(tris[ti:ti + k])
+            ti += k
+            emitted += k
+        k = min(per_group_rects, len(rects) - ri)
+        if k > 0:
+            cl.sync_pipe()
+            cl.set_other_modes(C2_RECT_MODES)
+            cl.set_combine(C2_RECT_COMBINE[0][0], C2_RECT_COMBINE[0][1], C2_RECT_COMBINE[1][0],
+                           C2_RECT_COMBINE[1][1])
+            for x0, y0, x1, y1, s0, t0, dsdx, dtdy in rects[ri:ri + k]:
+                cl.texture_rectangle(0, x0, y0, x1, y1, s0, t0, dsdx, dtdy)
+            ri += k
+            emitted += k
+    return emitted
+
+
+def config2_stream(seed: int = 0x52447000, frames: int = 30, prims_per_frame: int = 2560, width: int = 640,
+                   height: int = 480, num_textures: int = 8) -> Stream:
+    """2-cycle stress pass: 640x480 RGBA5551 + Z, CI4/CI8 textures with random
+    16/256-entry TLUTs, mip LOD over 4 levels, fog blender in cycle 0 and
+    coverage AA in cycle 1, 20 % texture rectangles; every frame is scanned
+    out through the VI with AA, divot and dither filters."""
+    rng = np.random.default_rng(seed)
+    rdram = np.zeros(RDRAM_SIZE // 4, dtype=np.uint32)
+    for t in range(num_textures):
+        _write_ci_texture(rdram, rng, t, SIZ_8 if t % 2 == 0 else SIZ_4)
+    out_frames = []
+    total = 0
+    for f in range(frames):
+        cl = CommandList()
+        _clear_passes(cl, width, height, SIZ_16, clear_color=True)
+        total += _emit_c2(cl, rng, prims_per_frame, width, height, num_textures)
+        cl.sync_full()
+        out_frames.append(Frame(cl.words(), vi=c2_vi_registers(width, height)))
+    return Stream(name="c2", seed=seed, frames=out_frames, rdram_init=rdram, hidden_init=0, color_addr=COLOR_ADDR,
+                  depth_addr=DEPTH_ADDR, width=width, height=height, color_size=SIZ_16, num_triangles=total)
+
+
+# ----------------------------------------------------------------------------
 # Mixed batch (64 independent streams)
 # ----------------------------------------------------------------------------
 SHADE_COMBINE = ((MULADD_ZERO, MULSUB_ZERO, MUL_ZERO, RGB_SHADE), (A_ZERO, A_ZERO, AMUL_ZERO, A_SHADE))
@@ -640,14 +811,14 @@ def _emit_textured(cl: CommandList, rng, n: int, width: int, height: int, num_te
     return int(tris.shape[0])
 
 
-BATCH_MODES = ("c0", "c1")
+BATCH_MODES = ("c0", "c1", "c2")
 
 
 def batch_stream(index: int, seed_base: int = 0x52449000, modes: Sequence[str] = BATCH_MODES,
                  min_tris: int = 32, max_tris: int = 8192) -> Stream:
-    """One stream of the mixed batch: log-uniform triangle count, 320x240 or
+    """One stream of the mixed batch: log-uniform primitive count, 320x240 or
     640x480 at 16 or 32 bits per pixel, render mode drawn from the plain
-    shaded (c0) and textured (c1) workloads."""
+    shaded (c0), textured (c1) and 2-cycle stress (c2) workloads."""
     seed = seed_base + index
     rng = np.random.default_rng(seed)
     n = int(round(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))))
@@ -663,6 +834,11 @@ def batch_stream(index: int, seed_base: int = 0x52449000, modes: Sequence[str] =
         for i in range(num_textures):
             rdram_write16(rdram, TEXTURE_ADDR + i * 2048, _random_texture_rgba16(rng, 32, 32))
         tris = _emit_textured(cl, rng, n, width, height, num_textures)
+    elif mode == "c2":
+        rdram = np.zeros(RDRAM_SIZE // 4, dtype=np.uint32)
+        for i in range(8):
+            _write_ci_texture(rdram, rng, i, SIZ_8 if i % 2 == 0 else SIZ_4)
+        tris = _emit_c2(cl, rng, n, width, height, 8)
     else:
         tris = _emit_shade_z(cl, rng, n, width, height, uniform_vertices=(n <= 512))
     cl.sync_full()

@@ -16,7 +16,10 @@ for sub in ("oracle", "parallel-rdp_amd"):
 import rdp_streams as rs  # noqa: E402
 
 
-def run_oracle(stream, frames=None):
+def run_oracle(stream, frames=None, scanouts=None):
+    """Renders the stream on the oracle.  With a list for `scanouts`, frames
+    that carry VI registers are scanned out after their commands and the
+    images appended."""
     import oracle
     o = oracle.Oracle(rs.RDRAM_SIZE)
     if stream.rdram_init is not None:
@@ -24,19 +27,27 @@ def run_oracle(stream, frames=None):
     o.hidden[:] = stream.hidden_init
     for f in (stream.frames if frames is None else stream.frames[:frames]):
         o.process(f.words)
+        if scanouts is not None and f.vi:
+            for r, v in f.vi.items():
+                o.set_vi_register(r, v)
+            scanouts.append(o.scanout())
     o.flush()
     out = (o.rdram.copy(), o.hidden.copy(), o.tmem.copy(), o.fragments)
     o.close()
     return out
 
 
-def run_gpu(stream, frames=None, proc=None):
+def run_gpu(stream, frames=None, proc=None, scanouts=None):
     import rdp_mi
     p = proc or rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
     p.write_rdram(stream.rdram_init if stream.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
     p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, stream.hidden_init, np.uint8))
     for f in (stream.frames if frames is None else stream.frames[:frames]):
         p.enqueue_commands(f.words)
+        if scanouts is not None and f.vi:
+            for r, v in f.vi.items():
+                p.set_vi_register(r, v)
+            scanouts.append(p.scanout_sync())
     p.idle()
     out = (p.read_rdram(), p.read_hidden_rdram(), p.read_tmem())
     if proc is None:

@@ -37,7 +37,18 @@ def test_config1_textured_two_frames(rdp_mi):
     assert_same(harness.run_gpu(s), harness.run_oracle(s), s.name)
 
 
-@pytest.mark.parametrize("start", [0, 8])
+def test_config2_two_cycle_with_vi(rdp_mi):
+    s = rs.config2_stream(frames=3)
+    say(f"[case] {s.name} 3 frames, {s.num_triangles} primitives, VI scanout per frame")
+    gimg, oimg = [], []
+    assert_same(harness.run_gpu(s, scanouts=gimg), harness.run_oracle(s, scanouts=oimg), s.name)
+    assert len(gimg) == len(oimg) == 3
+    for k, (g, o) in enumerate(zip(gimg, oimg)):
+        assert g.shape == o.shape == (240, 640), (k, g.shape, o.shape)
+        assert np.array_equal(g, o), f"VI frame {k}: {int(np.count_nonzero(g != o))} pixels differ"
+
+
+@pytest.mark.parametrize("start", [0, 8, 16, 24])
 def test_batch_streams_immediate(rdp_mi, start):
     for i in range(start, start + 8):
         s = rs.batch_stream(i)

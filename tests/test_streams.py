@@ -30,7 +30,8 @@ def test_batch_streams_are_deterministic_and_well_formed():
         assert np.array_equal(x.frames[0].words, y.frames[0].words)
         ops = walk(x.frames[0].words)
         assert ops[-1] == rs.OP_SYNC_FULL
-        assert sum(1 for o in ops if 0x08 <= o <= 0x0F) == x.num_triangles
+        # num_triangles counts triangles and texture rectangles (clears excluded)
+        assert sum(1 for o in ops if 0x08 <= o <= 0x0F or o in (0x24, 0x25)) == x.num_triangles
 
 
 def test_fuzz_streams_are_deterministic_and_well_formed():
