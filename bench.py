@@ -4,7 +4,8 @@
 Workload (the batched form): 64 independent seeded command streams, each with
 its own 8 MB device RDRAM; triangle counts log-uniform in [32, 8192];
 framebuffers 320x240 or 640x480 at 16 or 32 bits; render modes drawn from the
-plain shaded and textured workloads.  One step renders one frame of every
+plain shaded, textured and 2-cycle (CI textures, mip LOD, fog, coverage AA)
+workloads.  One step renders one frame of every
 stream.
 
 The stream commands are decoded by the host front end and uploaded once
@@ -194,7 +195,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched form: {frames_per_step} seeded streams (8 MB RDRAM each), "
                                f"{total_tris} triangles per step, 320x240/640x480 at 16/32 bpp, "
-                               "shaded and textured modes",
+                               "modes drawn from the shaded, textured and 2-cycle CI/LOD/fog/AA workloads",
                    "streams": frames_per_step, "triangles_per_step": total_tris},
         "bit_exact_frames": f"{exact}/{frames_per_step}",
         "mismatched_bytes": mism,

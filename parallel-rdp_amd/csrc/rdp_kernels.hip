@@ -380,7 +380,8 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 		r.z_mode = st.z_mode;
 		r.fill_color = st.fill_color;
 		r.min_lod = st.min_lod;
-		r.pad0[0] = r.pad0[1] = r.pad0[2] = 0;
+		r.variant = st.variant;
+		r.pad0[0] = r.pad0[1] = 0;
 		for (int c = 0; c < 2; c++) {
 			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
 			for (int i = 0; i < 4; i++) {
@@ -1309,7 +1310,33 @@ DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const AS_CONST uint8
 	}
 }
 
-DEV void combine(const CombIn &in, const AS_CONST uint8_t *sel, const AS_CONST uint8_t (*k)[4], int32_t out[4])
+// Selector sources for combine(): read from the primitive at run time, or a
+// combiner known at compile time (kKnownCombiners), which lets every selector
+// switch fold away.
+struct RuntimeSel {
+	const AS_CONST uint8_t *p;
+	MDEV int operator[](int i) const { return p[i]; }
+};
+
+template <int C>
+struct KnownSel {
+	MDEV int operator[](int i) const
+	{
+		constexpr uint8_t v[4][8] = {
+			{ 0, 0, 0, 0, 0, 0, 0, 0 },
+			{ kKnownCombiners[1][0], kKnownCombiners[1][1], kKnownCombiners[1][2], kKnownCombiners[1][3],
+			  kKnownCombiners[1][4], kKnownCombiners[1][5], kKnownCombiners[1][6], kKnownCombiners[1][7] },
+			{ kKnownCombiners[2][0], kKnownCombiners[2][1], kKnownCombiners[2][2], kKnownCombiners[2][3],
+			  kKnownCombiners[2][4], kKnownCombiners[2][5], kKnownCombiners[2][6], kKnownCombiners[2][7] },
+			{ kKnownCombiners[3][0], kKnownCombiners[3][1], kKnownCombiners[3][2], kKnownCombiners[3][3],
+			  kKnownCombiners[3][4], kKnownCombiners[3][5], kKnownCombiners[3][6], kKnownCombiners[3][7] },
+		};
+		return v[C][i];
+	}
+};
+
+template <typename Sel>
+DEV void combine(const CombIn &in, const Sel &sel, const AS_CONST uint8_t (*k)[4], int32_t out[4])
 {
 #pragma unroll
 	for (int ch = 0; ch < 4; ch++) {
@@ -1568,26 +1595,36 @@ struct Shaded {
 #define COVERAGE_COPY 0x20
 
 // shading.h:38-359 for one pixel of one primitive (no upscaling).
+// A raster code variant: flag bits in KNOWN are the compile-time constants in
+// VALUE, everything else comes from the primitive (see RDP_RASTER_VARIANTS).
+template <uint32_t KNOWN, uint32_t VALUE, int COMB>
+struct RasterVariant {
+	static constexpr int comb = COMB;
+	static MDEV bool ras(uint32_t f, uint32_t bit) { return (KNOWN & bit) ? ((VALUE & bit) != 0) : ((f & bit) != 0); }
+};
+using GenericVariant = RasterVariant<0u, 0u, (int)COMB_ANY>;
+
+template <class V>
 DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *tiles, const Tex &tx, uint32_t fb_fmt,
                      const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, Shaded &out)
 {
 	const uint32_t f = ps->ras_flags;
-	const bool tlut = f & RAS_TLUT, tlut_type = f & RAS_TLUT_TYPE, sample_quad = f & RAS_SAMPLE_QUAD;
-	const bool cvg_times_alpha = f & RAS_CVG_TIMES_ALPHA, alpha_cvg_select = f & RAS_ALPHA_CVG_SELECT;
-	const bool perspective = f & RAS_PERSPECTIVE, aa = f & RAS_AA, multi = f & RAS_MULTI_CYCLE;
-	const bool interlace = f & RAS_INTERLACE_FIELD, alpha_test = f & RAS_ALPHA_TEST;
-	const bool mid_texel = f & RAS_MID_TEXEL, convert_one = f & RAS_CONVERT_ONE;
-	const bool bilerp0 = f & RAS_BILERP0, bilerp1 = f & RAS_BILERP1;
-	bool uses_texel1 = f & RAS_USES_TEXEL1;
+	const bool tlut = V::ras(f, RAS_TLUT), tlut_type = V::ras(f, RAS_TLUT_TYPE), sample_quad = V::ras(f, RAS_SAMPLE_QUAD);
+	const bool cvg_times_alpha = V::ras(f, RAS_CVG_TIMES_ALPHA), alpha_cvg_select = V::ras(f, RAS_ALPHA_CVG_SELECT);
+	const bool perspective = V::ras(f, RAS_PERSPECTIVE), aa = V::ras(f, RAS_AA), multi = V::ras(f, RAS_MULTI_CYCLE);
+	const bool interlace = V::ras(f, RAS_INTERLACE_FIELD), alpha_test = V::ras(f, RAS_ALPHA_TEST);
+	const bool mid_texel = V::ras(f, RAS_MID_TEXEL), convert_one = V::ras(f, RAS_CONVERT_ONE);
+	const bool bilerp0 = V::ras(f, RAS_BILERP0), bilerp1 = V::ras(f, RAS_BILERP1);
+	bool uses_texel1 = V::ras(f, RAS_USES_TEXEL1);
 
 	uint32_t noise = 0;
-	if (f & RAS_NEED_NOISE)
+	if (V::ras(f, RAS_NEED_NOISE))
 		noise = reseed_noise((uint32_t)x, (uint32_t)y, ps->prim_index);
 
 	const bool flip = (ps->flags & TRI_FLIP) != 0;
 	const int32_t factors[4] = { ps->factors[0], ps->factors[1], ps->factors[2], ps->factors[3] };
 
-	if (f & RAS_COPY) {
+	if (V::ras(f, RAS_COPY)) {
 		if (x < span.start_x || x > span.end_x) return false;
 		int dxp = flip ? (x - span.start_x) : (span.end_x - x);
 		uint32_t dx_mask, dx_shift, fb_size;
@@ -1618,7 +1655,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 		out.coverage = COVERAGE_COPY;
 		if (alpha_test && fb_size == 2 && (texel & 1) == 0) return false;
 		return true;
-	} else if (f & RAS_FILL) {
+	} else if (V::ras(f, RAS_FILL)) {
 		out.coverage = COVERAGE_FILL;
 		return x >= span.start_x && x <= span.end_x;
 	}
@@ -1642,7 +1679,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 		shade[i] = clamp_9bit(sn >> 4);
 	}
 
-	const bool uses_lod = f & RAS_USES_LOD;
+	const bool uses_lod = V::ras(f, RAS_USES_LOD);
 	int32_t st0[2], stdx[2] = { 0, 0 }, stdy[2] = { 0, 0 };
 	bool povf = false;
 	{
@@ -1668,18 +1705,18 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	uint32_t tile0 = ps->tile & 7, tile1 = (tile0 + 1) & 7;
 	int32_t lod_frac = 0;
 	if (uses_lod)
-		compute_lod(tile0, tile1, lod_frac, ps->tile >> 3, (int)ps->min_lod, st0, stdx, stdy, povf, f & RAS_TEX_LOD,
-		            f & RAS_SHARPEN_LOD, f & RAS_DETAIL_LOD);
+		compute_lod(tile0, tile1, lod_frac, ps->tile >> 3, (int)ps->min_lod, st0, stdx, stdy, povf, V::ras(f, RAS_TEX_LOD),
+		            V::ras(f, RAS_SHARPEN_LOD), V::ras(f, RAS_DETAIL_LOD));
 
 	int32_t texel0[4] = { 0, 0, 0, 0 }, texel1[4] = { 0, 0, 0, 0 };
-	if (f & RAS_USES_TEXEL0)
+	if (V::ras(f, RAS_USES_TEXEL0))
 	{
 		const TileDesc td = ldc(tiles + tile0);
 		sample_texture(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, false, bilerp0, factors,
 		               texel1, texel0);
 	}
 
-	if (f & RAS_USES_PIPELINED_TEXEL1) {
+	if (V::ras(f, RAS_USES_PIPELINED_TEXEL1)) {
 		const SpanSetup next = ldg16(next_span);
 		const bool long_span = span.lodlength >= 8;
 		const bool end_span = x == (flip ? span.end_x : span.start_x);
@@ -1731,7 +1768,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	int32_t alpha_ref;
 	if (multi) {
 		int32_t c0[4];
-		combine(ci, ps->comb[0], ps->constants[0], c0);
+		combine(ci, RuntimeSel{ ps->comb[0] }, ps->constants[0], c0);
 		if (alpha_test) {
 			int32_t cl = clamp_9bit(c0[3]);
 			int32_t ex = cl + ((cl + 1) >> 8);
@@ -1749,13 +1786,15 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 			ci.texel0[i] = ci.texel1[i];
 			ci.texel1[i] = tmp;
 		}
-		if (f & RAS_NEED_NOISE_DUAL) {
+		if (V::ras(f, RAS_NEED_NOISE_DUAL)) {
 			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), ps->prim_index + 11);
 			ci.noise = (int32_t)(((n2 & 7) << 6) | 0x20);
 		}
-		combine(ci, ps->comb[1], ps->constants[1], combined);
+		combine(ci, RuntimeSel{ ps->comb[1] }, ps->constants[1], combined);
+	} else if (V::comb != COMB_ANY) {
+		combine(ci, KnownSel<V::comb>{}, ps->constants[1], combined);
 	} else {
-		combine(ci, ps->comb[1], ps->constants[1], combined);
+		combine(ci, RuntimeSel{ ps->comb[1] }, ps->constants[1], combined);
 	}
 	// combiner_cycle1 tail (combiner.h:256-281)
 #pragma unroll
@@ -1775,7 +1814,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 
 	if (aa && cvg_count == 0) return false;
 	if (alpha_test) {
-		int32_t thr = (f & RAS_ALPHA_TEST_DITHER) ? (int32_t)(noise & 0xff) : ps->blendc[3];
+		int32_t thr = V::ras(f, RAS_ALPHA_TEST_DITHER) ? (int32_t)(noise & 0xff) : ps->blendc[3];
 		if (alpha_ref < thr) return false;
 	}
 #pragma unroll
@@ -1957,6 +1996,26 @@ DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
 	if (alias) alias_color_to_depth(p, fmt);
 }
 
+// One primitive at one pixel: span lookup, shading, then fill / copy /
+// depth test + blend into the pixel's registers.
+template <class V>
+DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *states, const AS_GLOBAL SpanSetup *spans,
+                     const Tex &tex, const AS_GLOBAL uint8_t *lut, uint32_t fmt, bool alias, uint32_t ci, int x, int y,
+                     Pixel &px)
+{
+	const AS_GLOBAL SpanSetup *sp = spans + (ps->span_offset + (uint32_t)(y - ps->ylo));
+	const SpanSetup span = ldg16(sp);
+	if (!span.valid) return;
+	Shaded sh;
+	if (!shade_pixel<V>(ps, states[ps->state_index].tiles, tex, fmt, span, sp + 1, x, y, sh)) return;
+	if (sh.coverage & COVERAGE_FILL)
+		fill_pixel(px, fmt, alias, ci, ps->fill_color);
+	else if (sh.coverage & COVERAGE_COPY)
+		copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
+	else
+		depth_blend(lut, ps, fmt, alias, px, sh);
+}
+
 #ifndef RDP_RASTER_WAVES_PER_EU
 #define RDP_RASTER_WAVES_PER_EU 4
 #endif
@@ -2063,17 +2122,15 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 			}
 
 			if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
-			const AS_GLOBAL SpanSetup *sp = spans + (ps->span_offset + (uint32_t)(y - ps->ylo));
-			const SpanSetup span = ldg16(sp);
-			if (!span.valid) continue;
-			Shaded sh;
-			if (!shade_pixel(ps, states[ps->state_index].tiles, tex, fmt, span, sp + 1, x, y, sh)) continue;
-			if (sh.coverage & COVERAGE_FILL)
-				fill_pixel(px, fmt, alias, ci, ps->fill_color);
-			else if (sh.coverage & COVERAGE_COPY)
-				copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
-			else
-				depth_blend(lut, ps, fmt, alias, px, sh);
+			switch (ps->variant) {
+#define RDP_DISPATCH_VARIANT(id, known, value, comb)                                                            \
+	case id:                                                                                                    \
+		raster_prim<RasterVariant<(known), (value), (int)(comb)>>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px); \
+		break;
+				RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
+#undef RDP_DISPATCH_VARIANT
+			default: raster_prim<GenericVariant>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px); break;
+			}
 		}
 	}
 

@@ -96,6 +96,57 @@ struct TileDesc {
 };
 static_assert(sizeof(TileDesc) == 32, "TileDesc layout");
 
+// Raster code variants.  The raster kernel carries one instantiation of its
+// per-primitive shading and blending code per variant; in each, the flag bits
+// in RAS_KNOWN hold the values in RAS_VALUE as compile-time constants (all
+// other bits are read from the primitive at run time), and a known combiner,
+// when given, replaces the run-time selector decode.  The host assigns every
+// draw the first variant whose constants equal its resolved state, so the
+// result is the same as the generic code's by construction.
+enum : uint32_t {
+	COMB_ANY = 0,        // selectors read at run time
+	COMB_SHADE = 1,      // 1 cycle: rgb = SHADE, alpha = SHADE
+	COMB_TEX0_SHADE = 2, // 1 cycle: TEXEL0 * SHADE (rgb and alpha)
+	COMB_TEX0 = 3,       // 1 cycle: TEXEL0 (decal)
+};
+
+// Raw SetCombine selectors of cycle 1 for the known combiners
+// (rgb muladd, mulsub, mul, add, alpha muladd, mulsub, mul, add).
+constexpr uint8_t kKnownCombiners[4][8] = {
+	{ 0, 0, 0, 0, 0, 0, 0, 0 },
+	{ 8, 8, 16, 4, 7, 7, 7, 4 },
+	{ 1, 8, 4, 7, 1, 7, 4, 7 },
+	{ 8, 8, 16, 1, 7, 7, 7, 1 },
+};
+
+constexpr uint32_t RAS_STRUCT_1CYC = RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL | RAS_USES_TEXEL0 | RAS_USES_TEXEL1 |
+                                     RAS_USES_PIPELINED_TEXEL1 | RAS_USES_LOD;
+
+// X(id, ras_known, ras_value, combiner)
+#define RDP_RASTER_VARIANTS(X)                                                        \
+	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY)                                     \
+	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY)                                     \
+	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE)                                             \
+	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY)                                               \
+	X(5, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE)                           \
+	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0)                                 \
+	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY)                                  \
+	X(8, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY)
+
+inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t comb_cycle1[8])
+{
+#define RDP_MATCH_VARIANT(id, known, value, comb)                                                   \
+	if ((ras_flags & (known)) == (value)) {                                                         \
+		bool ok = true;                                                                              \
+		if ((comb) != COMB_ANY)                                                                      \
+			for (int i = 0; i < 8; i++) ok = ok && comb_cycle1[i] == kKnownCombiners[(comb)][i];     \
+		if (ok) return (id);                                                                         \
+	}
+	RDP_RASTER_VARIANTS(RDP_MATCH_VARIANT)
+#undef RDP_MATCH_VARIANT
+	return 0;
+}
+
 // Resolved draw state, snapshotted by the host whenever any state command
 // ran since the previous primitive.  Combiner constants are already resolved
 // per selector the way Renderer::build_combiner_constants does it.
@@ -118,7 +169,8 @@ struct DrawState {
 	uint32_t prim_dz;
 	uint32_t quirks;
 	uint32_t raw_ras_flags;           // static flags before normalisation (interlace bits)
-	uint32_t pad[2];
+	uint32_t variant;                 // raster code variant (select_raster_variant)
+	uint32_t pad;
 	TileDesc tiles[8];
 };
 
@@ -175,7 +227,8 @@ struct RasterPrim {
 	uint32_t z_mode;
 	uint32_t fill_color;
 	uint32_t min_lod;
-	uint32_t pad0[3];
+	uint32_t variant;        // raster code variant (select_raster_variant)
+	uint32_t pad0[2];
 	uint8_t comb[2][8];
 	uint8_t blend[2][4];
 	uint8_t constants[2][4][4];
