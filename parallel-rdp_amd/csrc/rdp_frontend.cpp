@@ -469,7 +469,6 @@ void Frontend::resolve_state(DrawState &d) const
 		if (need_dual) f |= RAS_NEED_NOISE_DUAL;
 	}
 	d.ras_flags = f;
-	d.variant = select_raster_variant(f, d.comb[1]);
 }
 
 // Renderer::validate_draw_state, rdp_renderer.cpp:1425-1462.

@@ -380,7 +380,7 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 		r.z_mode = st.z_mode;
 		r.fill_color = st.fill_color;
 		r.min_lod = st.min_lod;
-		r.variant = st.variant;
+		r.variant = select_raster_variant(st.ras_flags, st.comb[1], st.tiles, s.tile);
 		r.pad0[0] = r.pad0[1] = 0;
 		for (int c = 0; c < 2; c++) {
 			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
@@ -842,10 +842,12 @@ DEV uint32_t tidx16(const TileDesc &t, uint32_t s, uint32_t v, uint32_t wrap)
 }
 
 // sample_texel_* of texture.h:90-315 for one texel.
+// TF >= 0 is the tile's format * 4 + size known at compile time.
+template <int TF>
 DEV void fetch_texel(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, bool tlut, bool tlut_type,
                      uint32_t lut_offset, uint32_t addr_xor, int32_t o[4])
 {
-	const uint32_t size = t.size;
+	const uint32_t size = TF >= 0 ? (uint32_t)(TF & 3) : (uint32_t)t.size;
 	if (tlut) {
 		uint32_t lut_entry;
 		if (size == 0) {
@@ -862,7 +864,7 @@ DEV void fetch_texel(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, b
 		else rgba16_to(w, o);
 		return;
 	}
-	uint32_t fmt = t.fmt;
+	const uint32_t fmt = TF >= 0 ? (uint32_t)(TF >> 2) : (uint32_t)t.fmt;
 	int kind;  // 0: 4-bit replicate, 1: 8-bit, 2: 16-bit 2x8 (ci32), 3: rgba16, 4: rgba32, 5: ia4, 6: ia8, 7: ia16, 8: ci4
 	switch (fmt) {
 	case TF_RGBA: kind = size == 0 ? 0 : size == 1 ? 1 : size == 2 ? 3 : 4; break;
@@ -981,6 +983,7 @@ DEV void bilinear_3tap2(const int32_t *t00, const int32_t *t10, const int32_t *t
 }
 
 // texture.h:498-923
+template <int TF>
 DEV void sample_texture(const Tex &tx, const TileDesc &tile, int32_t st_s, int32_t st_t, bool tlut, bool tlut_type,
                         bool sample_quad, bool mid_state, bool convert_one, bool bilerp, const int32_t factors[4],
                         const int32_t prev[4], int32_t out[4])
@@ -1008,7 +1011,7 @@ DEV void sample_texture(const Tex &tx, const TileDesc &tile, int32_t st_s, int32
 	const bool mid_texel = mid_state && bilerp && fx == 0x10 && fy == 0x10;
 	const bool upper_lut = sum_frac >= 0x20;
 	if (mid_texel) sum_frac = 0;
-	const bool yuv = tile.fmt == TF_YUV;
+	const bool yuv = (TF >= 0 ? (uint32_t)(TF >> 2) : (uint32_t)tile.fmt) == TF_YUV;
 	int bs = sum_frac >= 0x20 ? s1 : s0;
 	int bt = sum_frac >= 0x20 ? t1 : t0;
 	const int chroma_frac = ((s0 & 1) << 4) | (fx >> 1);
@@ -1023,13 +1026,13 @@ DEV void sample_texture(const Tex &tx, const TileDesc &tile, int32_t st_s, int32
 		if (!yuv) {
 			const bool upper = sum_frac >= 0x20;
 			const uint32_t axor = upper_lut ? 2 : 1;
-			fetch_texel(tx, tile, (uint32_t)bs, (uint32_t)bt, true, tlut_type, upper ? 3 : 0, axor, tb);
+			fetch_texel<TF>(tx, tile, (uint32_t)bs, (uint32_t)bt, true, tlut_type, upper ? 3 : 0, axor, tb);
 			if (bilerp) {
-				fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t0, true, tlut_type, 1, axor, t10);
-				fetch_texel(tx, tile, (uint32_t)s0, (uint32_t)t1, true, tlut_type, 2, axor, t01);
+				fetch_texel<TF>(tx, tile, (uint32_t)s1, (uint32_t)t0, true, tlut_type, 1, axor, t10);
+				fetch_texel<TF>(tx, tile, (uint32_t)s0, (uint32_t)t1, true, tlut_type, 2, axor, t01);
 			}
 			if (mid_texel)
-				fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t1, true, tlut_type, 3, axor, t11);
+				fetch_texel<TF>(tx, tile, (uint32_t)s1, (uint32_t)t1, true, tlut_type, 3, axor, t11);
 		}
 	} else if (yuv) {
 		uint32_t cx0 = (uint32_t)(s0 >> 1), cx1 = (uint32_t)((s1 + (s1 - s0)) >> 1);
@@ -1040,13 +1043,13 @@ DEV void sample_texture(const Tex &tx, const TileDesc &tile, int32_t st_s, int32
 			fetch_yuv16(tx, tile, (uint32_t)s1, (uint32_t)t1, cx1, t11);
 		}
 	} else {
-		fetch_texel(tx, tile, (uint32_t)bs, (uint32_t)bt, false, false, 0, 0, tb);
+		fetch_texel<TF>(tx, tile, (uint32_t)bs, (uint32_t)bt, false, false, 0, 0, tb);
 		if (sample_quad) {
-			fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t0, false, false, 0, 0, t10);
-			fetch_texel(tx, tile, (uint32_t)s0, (uint32_t)t1, false, false, 0, 0, t01);
+			fetch_texel<TF>(tx, tile, (uint32_t)s1, (uint32_t)t0, false, false, 0, 0, t10);
+			fetch_texel<TF>(tx, tile, (uint32_t)s0, (uint32_t)t1, false, false, 0, 0, t01);
 		}
 		if (mid_texel)
-			fetch_texel(tx, tile, (uint32_t)s1, (uint32_t)t1, false, false, 0, 0, t11);
+			fetch_texel<TF>(tx, tile, (uint32_t)s1, (uint32_t)t1, false, false, 0, 0, t11);
 	}
 
 	int32_t acc[4];
@@ -1597,12 +1600,13 @@ struct Shaded {
 // shading.h:38-359 for one pixel of one primitive (no upscaling).
 // A raster code variant: flag bits in KNOWN are the compile-time constants in
 // VALUE, everything else comes from the primitive (see RDP_RASTER_VARIANTS).
-template <uint32_t KNOWN, uint32_t VALUE, int COMB>
+template <uint32_t KNOWN, uint32_t VALUE, int COMB, int TF>
 struct RasterVariant {
 	static constexpr int comb = COMB;
+	static constexpr int texfmt = TF;
 	static MDEV bool ras(uint32_t f, uint32_t bit) { return (KNOWN & bit) ? ((VALUE & bit) != 0) : ((f & bit) != 0); }
 };
-using GenericVariant = RasterVariant<0u, 0u, (int)COMB_ANY>;
+using GenericVariant = RasterVariant<0u, 0u, (int)COMB_ANY, TEXFMT_ANY>;
 
 template <class V>
 DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *tiles, const Tex &tx, uint32_t fb_fmt,
@@ -1712,7 +1716,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	if (V::ras(f, RAS_USES_TEXEL0))
 	{
 		const TileDesc td = ldc(tiles + tile0);
-		sample_texture(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, false, bilerp0, factors,
+		sample_texture<V::texfmt>(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, false, bilerp0, factors,
 		               texel1, texel0);
 	}
 
@@ -1744,7 +1748,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 			convert_factors(texel0, factors, texel1);
 		else {
 			const TileDesc td = ldc(tiles + tile1);
-			sample_texture(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, convert_one, bilerp1,
+			sample_texture<V::texfmt>(tx, td, st0[0], st0[1], tlut, tlut_type, sample_quad, mid_texel, convert_one, bilerp1,
 			               factors, texel0, texel1);
 		}
 	}
@@ -2123,9 +2127,10 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 
 			if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
 			switch (ps->variant) {
-#define RDP_DISPATCH_VARIANT(id, known, value, comb)                                                            \
+#define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt)                                                      \
 	case id:                                                                                                    \
-		raster_prim<RasterVariant<(known), (value), (int)(comb)>>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px); \
+		raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt)>>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, \
+		                                                                    px);                              \
 		break;
 				RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
 #undef RDP_DISPATCH_VARIANT

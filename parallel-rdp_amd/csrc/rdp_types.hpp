@@ -99,10 +99,17 @@ static_assert(sizeof(TileDesc) == 32, "TileDesc layout");
 // Raster code variants.  The raster kernel carries one instantiation of its
 // per-primitive shading and blending code per variant; in each, the flag bits
 // in RAS_KNOWN hold the values in RAS_VALUE as compile-time constants (all
-// other bits are read from the primitive at run time), and a known combiner,
-// when given, replaces the run-time selector decode.  The host assigns every
-// draw the first variant whose constants equal its resolved state, so the
-// result is the same as the generic code's by construction.
+// other bits are read from the primitive at run time), a known combiner
+// replaces the run-time selector decode, and a known texture format
+// (fmt * 4 + size) replaces the per-texel format decode.  Every primitive is
+// given the first variant whose constants equal its resolved state and
+// tiles, so the result is the same as the generic code's by construction.
+#if defined(__HIPCC__)
+#define RDP_HD __host__ __device__
+#else
+#define RDP_HD
+#endif
+
 enum : uint32_t {
 	COMB_ANY = 0,        // selectors read at run time
 	COMB_SHADE = 1,      // 1 cycle: rgb = SHADE, alpha = SHADE
@@ -119,28 +126,65 @@ constexpr uint8_t kKnownCombiners[4][8] = {
 	{ 8, 8, 16, 1, 7, 7, 7, 1 },
 };
 
+constexpr int TEXFMT_ANY = -1;
+constexpr int TEXFMT_RGBA16 = 0 * 4 + 2;
+constexpr int TEXFMT_CI4 = 2 * 4 + 0;
+constexpr int TEXFMT_CI8 = 2 * 4 + 1;
+
 constexpr uint32_t RAS_STRUCT_1CYC = RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL | RAS_USES_TEXEL0 | RAS_USES_TEXEL1 |
                                      RAS_USES_PIPELINED_TEXEL1 | RAS_USES_LOD;
+constexpr uint32_t RAS_SAMPLING = RAS_PERSPECTIVE | RAS_SAMPLE_QUAD | RAS_BILERP0 | RAS_BILERP1 | RAS_MID_TEXEL |
+                                  RAS_CONVERT_ONE | RAS_TLUT | RAS_TLUT_TYPE | RAS_TEX_LOD | RAS_SHARPEN_LOD |
+                                  RAS_DETAIL_LOD;
+// 1 cycle, texel 0 only, perspective bilinear without TLUT.
+constexpr uint32_t RAS_TEX0_BILINEAR = RAS_USES_TEXEL0 | RAS_PERSPECTIVE | RAS_SAMPLE_QUAD | RAS_BILERP0;
+// 2 cycles, two texels, mip LOD, perspective bilinear with an RGBA16 TLUT.
+constexpr uint32_t RAS_2CYC_LOD_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_USES_TEXEL1 | RAS_USES_LOD |
+                                       RAS_PERSPECTIVE | RAS_SAMPLE_QUAD | RAS_BILERP0 | RAS_BILERP1 | RAS_TLUT |
+                                       RAS_TEX_LOD;
 
-// X(id, ras_known, ras_value, combiner)
-#define RDP_RASTER_VARIANTS(X)                                                        \
-	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY)                                     \
-	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY)                                     \
-	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE)                                             \
-	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY)                                               \
-	X(5, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE)                           \
-	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0)                                 \
-	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY)                                  \
-	X(8, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY)
+// X(id, ras_known, ras_value, combiner, texture format)
+#define RDP_RASTER_VARIANTS(X)                                                                                     \
+	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY)                                                      \
+	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY)                                                      \
+	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY)                                                              \
+	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY)                                                                \
+	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16)                        \
+	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY)                                            \
+	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY)                                                  \
+	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY)                                                   \
+	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI8)                                  \
+	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI4)                                 \
+	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY)
 
-inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t comb_cycle1[8])
+// The format every tile a primitive can sample shares, or TEXFMT_ANY.
+// Without texture LOD the sampled tiles are base (texel 0, pipelined texel 1)
+// and base + 1 (texel 1 in two-cycle mode); with it, compute_lod
+// (texture.h:925-1003) selects tiles base .. base + max_level, one further
+// with detail LOD.
+RDP_HD inline int static_texture_format(uint32_t ras_flags, const TileDesc *tiles, uint32_t setup_tile)
 {
-#define RDP_MATCH_VARIANT(id, known, value, comb)                                                   \
-	if ((ras_flags & (known)) == (value)) {                                                         \
-		bool ok = true;                                                                              \
-		if ((comb) != COMB_ANY)                                                                      \
-			for (int i = 0; i < 8; i++) ok = ok && comb_cycle1[i] == kKnownCombiners[(comb)][i];     \
-		if (ok) return (id);                                                                         \
+	const uint32_t base = setup_tile & 7, max_level = setup_tile >> 3;
+	uint32_t range = (ras_flags & RAS_TEX_LOD) ? max_level + ((ras_flags & RAS_DETAIL_LOD) ? 1u : 0u)
+	                                           : ((ras_flags & RAS_USES_TEXEL1) ? 1u : 0u);
+	if (range > 7) range = 7;
+	for (uint32_t i = 1; i <= range; i++) {
+		const TileDesc &t = tiles[(base + i) & 7];
+		if (t.fmt != tiles[base].fmt || t.size != tiles[base].size) return TEXFMT_ANY;
+	}
+	return (int)tiles[base].fmt * 4 + (int)tiles[base].size;
+}
+
+RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t comb_cycle1[8], const TileDesc *tiles,
+                                             uint32_t setup_tile)
+{
+	const int tf = static_texture_format(ras_flags, tiles, setup_tile);
+#define RDP_MATCH_VARIANT(id, known, value, comb, texfmt)                                                  \
+	if ((ras_flags & (known)) == (value) && ((texfmt) == TEXFMT_ANY || (texfmt) == tf)) {                 \
+		bool ok = true;                                                                                     \
+		if ((comb) != COMB_ANY)                                                                             \
+			for (int i = 0; i < 8; i++) ok = ok && comb_cycle1[i] == kKnownCombiners[(comb)][i];            \
+		if (ok) return (id);                                                                                \
 	}
 	RDP_RASTER_VARIANTS(RDP_MATCH_VARIANT)
 #undef RDP_MATCH_VARIANT
@@ -169,8 +213,7 @@ struct DrawState {
 	uint32_t prim_dz;
 	uint32_t quirks;
 	uint32_t raw_ras_flags;           // static flags before normalisation (interlace bits)
-	uint32_t variant;                 // raster code variant (select_raster_variant)
-	uint32_t pad;
+	uint32_t pad[2];
 	TileDesc tiles[8];
 };
 
