@@ -16,6 +16,13 @@ for comparison.  After the timed loop every stream's RDRAM, hidden RDRAM and
 TMEM is compared byte for byte with the CPU oracle; only bit-exact frames
 count toward frames/s.
 
+Serving form (reported under "serving"): 256 instances with device-resident
+state send frame requests to the continuous-batching server (host decode on
+a thread pool, batched submission, admission by a device-byte budget) as a
+Poisson process at 25/50/90% of the measured peak; p50/p99 latency is
+completion minus arrival, and every instance's final state is checked
+against the oracle.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -87,6 +94,63 @@ def exact_frames(procs, refs):
     return exact, mismatched_bytes
 
 
+def serving(rdp_mi, streams, refs, args):
+    """Serving form: `instances` emulator instances, each with its own
+    processor (8 MB RDRAM, hidden RDRAM and TMEM on the device); instance i
+    renders the frame of batch stream i mod len(streams) over and over.
+    Requests arrive as a Poisson process at 25/50/90% of the peak rate the
+    server sustains with every request queued at once; latency is completion
+    minus arrival.  Every batch stream's frame is idempotent (rendering it
+    again leaves RDRAM, hidden RDRAM and TMEM unchanged), so after all runs
+    each instance must hold exactly the oracle result of its stream."""
+    rng = np.random.default_rng(0x5E5E)
+    n = args.instances
+    tmpl = [i % len(streams) for i in range(n)]
+    procs = setup_processors(rdp_mi, [streams[t] for t in tmpl])
+    words = [s.frames[0].words for s in streams]
+    budget = args.hbm_budget_mib << 20
+
+    def run(arrivals, instances):
+        with rdp_mi.Server(procs, hbm_budget_bytes=budget, max_batch=args.max_batch) as srv:
+            for a, i in zip(arrivals, instances):
+                srv.submit(int(i), words[tmpl[int(i)]], arrival_us=float(a))
+            srv.start()
+            srv.drain()
+            return srv.results(), srv.stats()
+
+    # Peak: every request queued at time 0.
+    count = n * args.peak_frames_per_instance
+    res, st = run(np.zeros(count), np.tile(np.arange(n), args.peak_frames_per_instance))
+    peak = count / (res["complete_us"].max() * 1e-6)
+    log(f"[serve] peak {peak:.0f} frames/s over {count} frames, {st['batches']} batches "
+        f"(largest {st['max_batch_frames']} frames, {st['max_arena_bytes'] / 2**20:.1f} MiB arena)")
+    out = {"instances": n, "distinct_streams": len(streams), "hbm_budget_bytes": budget,
+           "resident_bytes_per_instance": rs.RDRAM_SIZE + rs.RDRAM_SIZE // 2 + 4096,
+           "peak_frames_per_s": round(peak, 1), "peak_stats": st, "loads": []}
+    for frac in (0.25, 0.5, 0.9):
+        rate = frac * peak
+        m = int(min(args.serve_max_frames, max(256, rate * args.serve_seconds)))
+        arrivals = np.cumsum(rng.exponential(1e6 / rate, size=m))
+        inst = rng.integers(0, n, size=m)
+        res, st = run(arrivals, inst)
+        lat = (res["complete_us"] - res["arrival_us"]) / 1e3
+        span = (res["complete_us"].max() - res["arrival_us"].min()) * 1e-6
+        entry = {"offered_frac": frac, "offered_frames_per_s": round(rate, 1), "frames": m,
+                 "achieved_frames_per_s": round(m / span, 1),
+                 "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                 "max_ms": round(float(lat.max()), 4), "mean_batch_frames": round(st["frames"] / max(st["batches"], 1), 2),
+                 "max_arena_bytes": st["max_arena_bytes"]}
+        log(f"[serve] {frac:.0%} load: {entry}")
+        out["loads"].append(entry)
+    exact, mism = exact_frames(procs, [refs[t] for t in tmpl])
+    out["bit_exact_instances"] = f"{exact}/{n}"
+    out["mismatched_bytes"] = mism
+    log(f"[serve] final state bit-exact {exact}/{n} ({mism} mismatched bytes)")
+    for p in procs:
+        p.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +159,14 @@ def main():
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison (profiling runs)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--peak-frames-per-instance", type=int, default=4)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-max-frames", type=int, default=4096)
+    ap.add_argument("--max-batch", type=int, default=64, help="frames per serving batch at most")
+    ap.add_argument("--hbm-budget-mib", type=int, default=2048,
+                    help="device bytes one serving batch may use for packed inputs and scratch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -158,6 +230,8 @@ def main():
         for p in procs:
             p.close()
 
+    serve = None if args.no_serve else serving(rdp_mi, streams, refs, args)
+
     frames_per_step = len(streams)
     ms_per_step = float(step_ms.mean())
     value = exact / (ms_per_step / 1e3)
@@ -199,8 +273,9 @@ def main():
                    "streams": frames_per_step, "triangles_per_step": total_tris},
         "bit_exact_frames": f"{exact}/{frames_per_step}",
         "mismatched_bytes": mism,
-        "p50_frame_latency_ms": round(float(np.percentile(step_ms, 50)), 4),
-        "p99_frame_latency_ms": round(float(np.percentile(step_ms, 99)), 4),
+        "p50_frame_latency_ms": None if serve is None else serve["loads"][1]["p50_ms"],
+        "p99_frame_latency_ms": None if serve is None else serve["loads"][1]["p99_ms"],
+        "latency_note": "serving form, Poisson arrivals at 50% of peak offered load, completion minus arrival",
         "plain_path": None if plain_ms is None else {
             "value": round(plain_value, 3), "unit": "frames/s", "ms_per_step": round(float(plain_ms.mean()), 4),
             "bit_exact_frames": f"{plain_exact}/{frames_per_step}"},
@@ -212,6 +287,7 @@ def main():
                      "kernel": "k_raster", "bytes_per_launch": int(bytes_per_launch),
                      "ms_per_launch": round(raster_ms_per_launch, 4)},
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -67,6 +67,7 @@ void RefPassState::reset()
 	tiles.clear();
 	prims = span_jobs = uploads = max_shaded_tiles = deduced_height = 0;
 	color_write_pending = depth_write_pending = false;
+	key_gen = ~0ull;
 }
 
 template <typename T>
@@ -165,11 +166,30 @@ bool Frontend::ref_account_draw(unsigned op, const uint32_t *w)
 	const int min_line = std::max(e.yh, (int)scissor[1]) >> 2, max_line = max_sub >> 2;
 	if (max_line >= min_line) ref.span_jobs += (uint32_t)((std::min(max_line - min_line + 2, 1024) + 63) / 64);
 
-	// normalize_static_state after deduce_static_texture_state and
-	// deduce_noise_state (rdp_renderer.cpp:1136-1400).
+	const unsigned setup_tile = op >= 0x08 && op <= 0x0f ? (w[0] >> 16) & 63 : (op == 0x36 ? 0 : (w[1] >> 24) & 7);
+	if (ref.key_gen != state_gen || ref.key_setup_tile != setup_tile) {
+		add_ref_keys(setup_tile);
+		ref.key_gen = state_gen;
+		ref.key_setup_tile = setup_tile;
+	}
+
+	ref.color_write_pending = true;
+	if (db_flags & DB_DEPTH_UPDATE) ref.depth_write_pending = true;
+	ref.prims++;
+
+	return ref.statics.size() == 64 || ref.depth_blends.size() == 64 || ref.tiles.size() + 8 > 256 ||
+	       ref.prims == 256 || ref.span_jobs * 64 + 1024 > 32 * 1024 || ref.max_shaded_tiles + 128 * 128 > 0x8000;
+}
+
+// Adds the draw's static, depth/blend and tile keys to the pass caches:
+// normalize_static_state after deduce_static_texture_state and
+// deduce_noise_state (rdp_renderer.cpp:1136-1400), and the depth/blend and
+// tile caches of draw_shaded_primitive.  The keys depend only on the draw
+// state and the setup tile.
+void Frontend::add_ref_keys(unsigned setup_tile)
+{
 	RefPassState::StaticKey k;
 	memset(&k, 0, sizeof(k));
-	const unsigned setup_tile = op >= 0x08 && op <= 0x0f ? (w[0] >> 16) & 63 : (op == 0x36 ? 0 : (w[1] >> 24) & 7);
 	const unsigned tile = setup_tile & 7;
 	uint32_t f = ras_flags;
 	if (f & RAS_FILL) {
@@ -247,13 +267,6 @@ bool Frontend::ref_account_draw(unsigned op, const uint32_t *w)
 	dk.z_mode = z_mode;
 	cache_add(ref.depth_blends, dk);
 	for (int i = 0; i < 8; i++) cache_add(ref.tiles, tiles[i]);
-
-	ref.color_write_pending = true;
-	if (db_flags & DB_DEPTH_UPDATE) ref.depth_write_pending = true;
-	ref.prims++;
-
-	return ref.statics.size() == 64 || ref.depth_blends.size() == 64 || ref.tiles.size() + 8 > 256 ||
-	       ref.prims == 256 || ref.span_jobs * 64 + 1024 > 32 * 1024 || ref.max_shaded_tiles + 128 * 128 > 0x8000;
 }
 
 // The reference ends a render pass here (flush_queues).
@@ -774,6 +787,7 @@ void Frontend::load_tile_iteration(uint32_t tile, uint32_t mode, uint32_t slo, u
 void Frontend::command(const uint32_t *w)
 {
 	const unsigned op = (w[0] >> 24) & 63;
+	if (!((op >= 0x08 && op <= 0x0f) || op == 0x24 || op == 0x25 || op == 0x36)) state_gen++;
 	switch (op) {
 	case 0x08: case 0x09: case 0x0a: case 0x0b: case 0x0c: case 0x0d: case 0x0e: case 0x0f:
 	case 0x24: case 0x25: case 0x36:

@@ -66,6 +66,10 @@ struct RefPassState {
 	std::vector<TileDesc> tiles;
 	uint32_t prims = 0, span_jobs = 0, uploads = 0, max_shaded_tiles = 0, deduced_height = 0;
 	bool color_write_pending = false, depth_write_pending = false;
+	// The caches already hold the keys of a draw made at state generation
+	// key_gen with this setup tile (all-ones: none since the reset).
+	uint64_t key_gen = ~0ull;
+	unsigned key_setup_tile = 0;
 	void reset();
 };
 
@@ -123,6 +127,7 @@ private:
 
 	uint64_t prim_counter = 0;
 	bool state_dirty = true;
+	uint64_t state_gen = 0;  // bumped by every command that is not a draw
 	bool color_write_pending = false, depth_write_pending = false;
 
 	HostPass cur;
@@ -138,6 +143,7 @@ private:
 	bool load_reads_marked_writes(uint32_t mode, uint32_t slo, uint32_t tlo, uint32_t shi, uint32_t thi) const;
 	bool ref_upload_needs_flush(uint32_t addr) const;
 	bool ref_account_draw(unsigned op, const uint32_t *words);
+	void add_ref_keys(unsigned setup_tile);
 	void ref_pass_end();
 	void split_at_last_mark();
 	void split_or_flush();

@@ -15,9 +15,15 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rdp_mi.h"
@@ -118,6 +124,7 @@ struct Context {
 	uint8_t *d_gamma = nullptr;
 	DeviceBuffer input, scratch;
 	PinnedBuffer staging;
+	size_t last_arena_bytes = 0;  // packed inputs + scratch of the last submission
 	hipEvent_t staging_done = nullptr;
 	bool staging_busy = false;
 	std::mutex lock;
@@ -308,6 +315,7 @@ static int prepare_rounds(Context &c, const std::vector<std::vector<PassRef>> &r
 		}
 	}
 	const size_t total_input = off;
+	c.last_arena_bytes = total_input + max_scratch;
 
 	int err;
 	if ((err = c.staging.reserve(total_input))) return set_error(err, "staging allocation of %zu bytes failed", total_input);
@@ -478,9 +486,253 @@ struct Recording {
 	uint32_t passes = 0;
 };
 
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of frame requests from many processors.
+//
+// Requests (one frame of commands for one processor, with an arrival time)
+// queue per processor.  A scheduler thread repeatedly
+//   1. admits every arrived request whose processor has nothing in flight, in
+//      round-robin order, while the batch's estimated device bytes (packed
+//      inputs plus kernel scratch) fit in the HBM budget (and up to
+//      max_batch).  Batches in flight share one stream-ordered input/scratch
+//      arena, so the budget bounds each batch rather than their sum;
+//   2. decodes the admitted frames on a pool of host threads (one front end
+//      per processor, so they are independent);
+//   3. submits all their passes as one batched submission and records an
+//      event, then goes straight back to 1 while the GPU works.
+// A completion thread waits on the batch events in order and stamps each
+// request's completion time.  Results are identical to rendering each frame
+// alone: processors share launches, never state.
+// ---------------------------------------------------------------------------
+class ThreadPool {
+public:
+	explicit ThreadPool(unsigned n)
+	{
+		for (unsigned i = 0; i < n; i++) workers.emplace_back([this] { run(); });
+	}
+	~ThreadPool()
+	{
+		{
+			std::lock_guard<std::mutex> g(m);
+			stop = true;
+		}
+		cv.notify_all();
+		for (auto &t : workers) t.join();
+	}
+	// Runs every job and returns when all have finished.
+	void run_all(std::vector<std::function<void()>> &jobs)
+	{
+		if (jobs.empty()) return;
+		std::unique_lock<std::mutex> g(m);
+		pending = jobs.size();
+		for (auto &j : jobs) queue.push_back(&j);
+		cv.notify_all();
+		done_cv.wait(g, [this] { return pending == 0; });
+	}
+
+private:
+	void run()
+	{
+		for (;;) {
+			std::function<void()> *job;
+			{
+				std::unique_lock<std::mutex> g(m);
+				cv.wait(g, [this] { return stop || !queue.empty(); });
+				if (stop && queue.empty()) return;
+				job = queue.front();
+				queue.pop_front();
+			}
+			(*job)();
+			std::lock_guard<std::mutex> g(m);
+			if (--pending == 0) done_cv.notify_all();
+		}
+	}
+	std::vector<std::thread> workers;
+	std::deque<std::function<void()> *> queue;
+	std::mutex m;
+	std::condition_variable cv, done_cv;
+	size_t pending = 0;
+	bool stop = false;
+};
+
+struct ServeRequest {
+	uint64_t ticket;
+	uint32_t instance;
+	double arrival_us;
+	std::vector<uint32_t> words;
+};
+
+struct ServeResult {
+	uint64_t ticket;
+	uint32_t instance;
+	double arrival_us, admit_us, complete_us;
+};
+
+struct Server {
+	std::vector<Processor *> procs;
+	size_t budget = 0;
+	unsigned max_batch = 0;
+	std::unique_ptr<ThreadPool> pool;
+
+	std::mutex m;
+	std::condition_variable cv;
+	std::vector<std::deque<ServeRequest>> queues;
+	std::vector<bool> busy;
+	uint64_t next_ticket = 1;
+	size_t outstanding = 0;  // submitted but not completed
+	std::vector<ServeResult> results;
+	bool started = false, stopping = false;
+	int error = 0;
+	std::string error_msg;
+	std::chrono::steady_clock::time_point t0;
+
+	struct Batch {
+		hipEvent_t ev = nullptr;
+		std::vector<ServeResult> reqs;
+	};
+	std::deque<Batch> inflight;
+	std::condition_variable inflight_cv;
+	std::thread scheduler, completer;
+	unsigned rr = 0;
+	// Statistics: batches, frames, largest batch, largest estimated and
+	// actual device bytes of one batch.
+	uint64_t n_batches = 0, n_frames = 0, max_frames = 0, max_est_bytes = 0, max_arena_bytes = 0;
+
+	double now_us() const { return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count(); }
+
+	// Device bytes a frame's submission needs: its inputs (words, states,
+	// primitive records) and scratch (setup records, spans, bins, TMEM
+	// snapshots), estimated from the command words before decoding.
+	static size_t estimate_bytes(const ServeRequest &r) { return r.words.size() * 4 * 24 + (1u << 20); }
+
+	void scheduler_loop();
+	void completion_loop();
+};
+
+void Server::scheduler_loop()
+{
+	Context &c = ctx();
+	std::unique_lock<std::mutex> g(m);
+	for (;;) {
+		if (stopping && outstanding == 0) break;
+		const double now = now_us();
+		// Admission.
+		std::vector<ServeRequest> batch;
+		size_t batch_bytes = 0;
+		double next_arrival = 1e300;
+		const unsigned n = (unsigned)procs.size();
+		for (unsigned k = 0; k < n && batch.size() < max_batch; k++) {
+			const unsigned i = (rr + k) % n;
+			if (busy[i] || queues[i].empty()) continue;
+			const ServeRequest &r = queues[i].front();
+			if (r.arrival_us > now) {
+				next_arrival = std::min(next_arrival, r.arrival_us);
+				continue;
+			}
+			const size_t est = estimate_bytes(r);
+			if (!batch.empty() && batch_bytes + est > budget) break;
+			batch_bytes += est;
+			busy[i] = true;
+			batch.push_back(std::move(queues[i].front()));
+			queues[i].pop_front();
+		}
+		if (batch.empty()) {
+			if (stopping && outstanding == 0) break;
+			if (next_arrival < 1e300)
+				cv.wait_for(g, std::chrono::microseconds((int64_t)std::max(1.0, next_arrival - now)));
+			else
+				cv.wait(g);
+			continue;
+		}
+		rr = (rr + 1) % n;
+		g.unlock();
+
+		// Decode on the host pool.
+		const double admit = now_us();
+		std::vector<std::function<void()>> jobs;
+		jobs.reserve(batch.size());
+		for (auto &r : batch) {
+			Processor *p = procs[r.instance];
+			const std::vector<uint32_t> *w = &r.words;
+			jobs.emplace_back([p, w] {
+				size_t i = 0;
+				while (i < w->size()) {
+					const unsigned len = Frontend::command_length((*w)[i]);
+					if (len == 0) {
+						i++;
+						continue;
+					}
+					if (i + len > w->size()) break;
+					p->fe.command(w->data() + i);
+					i += len;
+				}
+				p->fe.flush();
+			});
+		}
+		pool->run_all(jobs);
+
+		// Submit as one batched submission.
+		Batch b;
+		int err = 0;
+		{
+			std::lock_guard<std::mutex> lk(c.lock);
+			std::vector<Processor *> ps;
+			for (auto &r : batch) ps.push_back(procs[r.instance]);
+			err = submit_processors(ps.data(), (unsigned)ps.size(), true);
+			if (!err && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess)
+				err = set_error(RDP_MI_ERROR_DEVICE, "event creation failed");
+			if (!err && hipEventRecord(b.ev, c.stream) != hipSuccess)
+				err = set_error(RDP_MI_ERROR_DEVICE, "event record failed");
+			n_batches++;
+			n_frames += batch.size();
+			max_frames = std::max<uint64_t>(max_frames, batch.size());
+			max_est_bytes = std::max<uint64_t>(max_est_bytes, batch_bytes);
+			max_arena_bytes = std::max<uint64_t>(max_arena_bytes, c.last_arena_bytes);
+		}
+		for (auto &r : batch) b.reqs.push_back(ServeResult{ r.ticket, r.instance, r.arrival_us, admit, 0.0 });
+
+		g.lock();
+		if (err) {
+			error = err;
+			error_msg = rdp_mi_last_error();
+		}
+		inflight.push_back(std::move(b));
+		inflight_cv.notify_all();
+	}
+	g.unlock();
+}
+
+void Server::completion_loop()
+{
+	std::unique_lock<std::mutex> g(m);
+	for (;;) {
+		inflight_cv.wait(g, [this] { return !inflight.empty() || (stopping && outstanding == 0); });
+		if (inflight.empty()) break;
+		hipEvent_t ev = inflight.front().ev;
+		g.unlock();
+		if (ev) (void)hipEventSynchronize(ev);
+		const double t = now_us();
+		g.lock();
+		Batch b = std::move(inflight.front());
+		inflight.pop_front();
+		if (b.ev) (void)hipEventDestroy(b.ev);
+		for (auto &r : b.reqs) {
+			r.complete_us = t;
+			results.push_back(r);
+			busy[r.instance] = false;
+			outstanding--;
+		}
+		cv.notify_all();
+	}
+}
+
 }  // namespace rdp_mi
 
 using namespace rdp_mi;
+
+struct rdp_mi_server {
+	Server s;
+};
 
 struct rdp_mi_recording {
 	Recording r;
@@ -921,6 +1173,135 @@ uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *h) { return h ? h
 unsigned rdp_mi_recording_rounds(const rdp_mi_recording *h) { return h ? (unsigned)h->r.launches.size() : 0; }
 
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *h) { return h ? h->r.passes : 0; }
+
+int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t hbm_budget_bytes, unsigned max_batch,
+                         unsigned decode_threads, rdp_mi_server **out)
+{
+	if (!out || !procs || !count) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	*out = nullptr;
+	rdp_mi_server *h = new (std::nothrow) rdp_mi_server();
+	if (!h) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	Server &s = h->s;
+	for (unsigned i = 0; i < count; i++) {
+		if (!procs[i]) {
+			delete h;
+			return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
+		}
+		s.procs.push_back(&procs[i]->p);
+	}
+	s.budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)1 << 32);
+	s.max_batch = max_batch ? max_batch : count;
+	unsigned threads = decode_threads ? decode_threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+	s.pool.reset(new ThreadPool(threads));
+	s.queues.resize(count);
+	s.busy.assign(count, false);
+	*out = h;
+	return 0;
+}
+
+int rdp_mi_server_submit(rdp_mi_server *h, unsigned instance, const uint32_t *words, size_t num_words,
+                         double arrival_us, uint64_t *ticket)
+{
+	if (!h || !words || instance >= h->s.procs.size()) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	if (s.stopping) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "server is draining");
+	ServeRequest r;
+	r.ticket = s.next_ticket++;
+	r.instance = instance;
+	r.arrival_us = s.started ? std::max(arrival_us, s.now_us()) : arrival_us;
+	r.words.assign(words, words + num_words);
+	if (ticket) *ticket = r.ticket;
+	// Keep each processor's queue in arrival order.
+	auto &q = s.queues[instance];
+	auto it = q.end();
+	while (it != q.begin() && (it - 1)->arrival_us > r.arrival_us) --it;
+	q.insert(it, std::move(r));
+	s.outstanding++;
+	s.cv.notify_all();
+	return 0;
+}
+
+int rdp_mi_server_start(rdp_mi_server *h)
+{
+	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null server");
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	if (s.started) return 0;
+	Context &c = ctx();
+	{
+		std::lock_guard<std::mutex> lk(c.lock);
+		int err = c.init();
+		if (err) return err;
+	}
+	s.t0 = std::chrono::steady_clock::now();
+	s.started = true;
+	s.scheduler = std::thread([&s] { s.scheduler_loop(); });
+	s.completer = std::thread([&s] { s.completion_loop(); });
+	return 0;
+}
+
+int rdp_mi_server_drain(rdp_mi_server *h)
+{
+	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null server");
+	Server &s = h->s;
+	{
+		std::unique_lock<std::mutex> g(s.m);
+		if (!s.started) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "server not started");
+		s.cv.wait(g, [&s] { return s.outstanding == 0; });
+		if (s.error) return set_error(s.error, "%s", s.error_msg.c_str());
+	}
+	return 0;
+}
+
+size_t rdp_mi_server_results(rdp_mi_server *h, uint64_t *tickets, uint32_t *instances, double *arrival_us,
+                             double *admit_us, double *complete_us, size_t max)
+{
+	if (!h) return 0;
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	const size_t n = s.results.size();
+	for (size_t i = 0; i < n && i < max; i++) {
+		const ServeResult &r = s.results[i];
+		if (tickets) tickets[i] = r.ticket;
+		if (instances) instances[i] = r.instance;
+		if (arrival_us) arrival_us[i] = r.arrival_us;
+		if (admit_us) admit_us[i] = r.admit_us;
+		if (complete_us) complete_us[i] = r.complete_us;
+	}
+	return n;
+}
+
+void rdp_mi_server_stats(rdp_mi_server *h, uint64_t out[5])
+{
+	if (!h || !out) return;
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	out[0] = s.n_batches;
+	out[1] = s.n_frames;
+	out[2] = s.max_frames;
+	out[3] = s.max_est_bytes;
+	out[4] = s.max_arena_bytes;
+}
+
+void rdp_mi_server_destroy(rdp_mi_server *h)
+{
+	if (!h) return;
+	Server &s = h->s;
+	{
+		std::lock_guard<std::mutex> g(s.m);
+		s.stopping = true;
+		s.cv.notify_all();
+		s.inflight_cv.notify_all();
+	}
+	if (s.scheduler.joinable()) s.scheduler.join();
+	{
+		std::lock_guard<std::mutex> g(s.m);
+		s.inflight_cv.notify_all();
+	}
+	if (s.completer.joinable()) s.completer.join();
+	delete h;
+}
 
 void rdp_mi_recording_destroy(rdp_mi_recording *h)
 {

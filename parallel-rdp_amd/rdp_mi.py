@@ -318,3 +318,96 @@ class Recording:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching frame server over a fixed set of processors.
+
+    Each request is one frame of complete commands for one processor with an
+    arrival time (microseconds after start()).  Arrived requests are admitted
+    whenever their processor is idle and the batch fits the device-byte
+    budget; their frames are decoded on host threads and submitted as one
+    batched launch set while earlier batches still run.  Per processor,
+    frames render in arrival order and the results equal rendering them alone.
+    """
+
+    def __init__(self, processors: Sequence[CommandProcessor], hbm_budget_bytes: int = 0, max_batch: int = 0,
+                 decode_threads: int = 0):
+        lib = load_library()
+        vp, sz, cu = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.rdp_mi_server_create.restype = ctypes.c_int
+        lib.rdp_mi_server_create.argtypes = [ctypes.POINTER(vp), cu, sz, cu, cu, ctypes.POINTER(vp)]
+        lib.rdp_mi_server_submit.restype = ctypes.c_int
+        lib.rdp_mi_server_submit.argtypes = [vp, cu, vp, sz, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]
+        lib.rdp_mi_server_start.restype = ctypes.c_int
+        lib.rdp_mi_server_start.argtypes = [vp]
+        lib.rdp_mi_server_drain.restype = ctypes.c_int
+        lib.rdp_mi_server_drain.argtypes = [vp]
+        lib.rdp_mi_server_results.restype = sz
+        lib.rdp_mi_server_results.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
+                                              dp, dp, dp, sz]
+        lib.rdp_mi_server_stats.argtypes = [vp, vp]
+        lib.rdp_mi_server_destroy.argtypes = [vp]
+        self._lib = lib
+        self.processors = list(processors)
+        arr = (vp * len(processors))(*[p.handle.value for p in processors])
+        h = vp()
+        _check(lib.rdp_mi_server_create(arr, len(processors), hbm_budget_bytes, max_batch, decode_threads,
+                                        ctypes.byref(h)), "rdp_mi_server_create")
+        self._h = h
+
+    def submit(self, instance: int, words: np.ndarray, arrival_us: float = 0.0) -> int:
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        t = ctypes.c_uint64()
+        _check(self._lib.rdp_mi_server_submit(self._h, instance, w.ctypes.data, w.size, float(arrival_us),
+                                              ctypes.byref(t)), "rdp_mi_server_submit")
+        return int(t.value)
+
+    def start(self):
+        _check(self._lib.rdp_mi_server_start(self._h), "rdp_mi_server_start")
+
+    def drain(self):
+        _check(self._lib.rdp_mi_server_drain(self._h), "rdp_mi_server_drain")
+
+    def results(self) -> dict:
+        """Completed requests in completion order: arrays ticket, instance,
+        arrival_us, admit_us, complete_us."""
+        n = int(self._lib.rdp_mi_server_results(self._h, None, None, None, None, None, 0))
+        out = {
+            "ticket": np.zeros(n, np.uint64),
+            "instance": np.zeros(n, np.uint32),
+            "arrival_us": np.zeros(n, np.float64),
+            "admit_us": np.zeros(n, np.float64),
+            "complete_us": np.zeros(n, np.float64),
+        }
+        dp = ctypes.POINTER(ctypes.c_double)
+        got = int(self._lib.rdp_mi_server_results(
+            self._h, out["ticket"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            out["instance"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            out["arrival_us"].ctypes.data_as(dp), out["admit_us"].ctypes.data_as(dp),
+            out["complete_us"].ctypes.data_as(dp), n))
+        return {k: v[:min(n, got)] for k, v in out.items()}
+
+    def stats(self) -> dict:
+        v = np.zeros(5, np.uint64)
+        self._lib.rdp_mi_server_stats(self._h, v.ctypes.data)
+        return dict(zip(("batches", "frames", "max_batch_frames", "max_estimated_bytes", "max_arena_bytes"),
+                        (int(x) for x in v)))
+
+    def close(self):
+        if self._h:
+            self._lib.rdp_mi_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -851,6 +851,23 @@ def batch_streams(count: int = 64, **kw) -> List[Stream]:
     return [batch_stream(i, **kw) for i in range(count)]
 
 
+def split_at_sync_full(words: np.ndarray) -> List[np.ndarray]:
+    """Splits a command stream into frames that each end with a SyncFull
+    (trailing commands after the last SyncFull form a final frame)."""
+    frames = []
+    start = i = 0
+    n = int(words.size)
+    while i < n:
+        op = (int(words[i]) >> 24) & 0x3F
+        i += int(COMMAND_LENGTHS[op]) or 1
+        if op == 0x29:
+            frames.append(words[start:i])
+            start = i
+    if start < n:
+        frames.append(words[start:n])
+    return frames
+
+
 # ----------------------------------------------------------------------------
 # Random-state streams: every mode bit, selector, format and primitive kind
 # drawn at random on a small framebuffer, for parity testing.
