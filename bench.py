@@ -111,7 +111,7 @@ def serving(rdp_mi, streams, refs, args):
     budget = args.hbm_budget_mib << 20
 
     def run(arrivals, instances):
-        with rdp_mi.Server(procs, hbm_budget_bytes=budget, max_batch=args.max_batch) as srv:
+        with rdp_mi.Server(procs, hbm_budget_bytes=budget, max_batch=args.max_batch, streams=args.serve_streams) as srv:
             for a, i in zip(arrivals, instances):
                 srv.submit(int(i), words[tmpl[int(i)]], arrival_us=float(a))
             srv.start()
@@ -125,6 +125,7 @@ def serving(rdp_mi, streams, refs, args):
     log(f"[serve] peak {peak:.0f} frames/s over {count} frames, {st['batches']} batches "
         f"(largest {st['max_batch_frames']} frames, {st['max_arena_bytes'] / 2**20:.1f} MiB arena)")
     out = {"instances": n, "distinct_streams": len(streams), "hbm_budget_bytes": budget,
+           "submission_streams": args.serve_streams, "max_batch": args.max_batch,
            "resident_bytes_per_instance": rs.RDRAM_SIZE + rs.RDRAM_SIZE // 2 + 4096,
            "peak_frames_per_s": round(peak, 1), "peak_stats": st, "loads": []}
     for frac in (0.25, 0.5, 0.9):
@@ -140,6 +141,13 @@ def serving(rdp_mi, streams, refs, args):
                  "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
                  "max_ms": round(float(lat.max()), 4), "mean_batch_frames": round(st["frames"] / max(st["batches"], 1), 2),
                  "max_arena_bytes": st["max_arena_bytes"]}
+        for key, d in (("queue", res["admit_us"] - res["arrival_us"]), ("decode", res["submit_us"] - res["admit_us"]),
+                       ("gpu", res["complete_us"] - res["submit_us"])):
+            entry[f"{key}_p50_ms"] = round(float(np.percentile(d, 50)) / 1e3, 4)
+            entry[f"{key}_p99_ms"] = round(float(np.percentile(d, 99)) / 1e3, 4)
+        if args.serve_dump:
+            np.savez(f"{args.serve_dump}_{int(frac * 100)}.npz", words=np.array([words[tmpl[int(i)]].size
+                                                                                 for i in res["instance"]]), **res)
         log(f"[serve] {frac:.0%} load: {entry}")
         out["loads"].append(entry)
     exact, mism = exact_frames(procs, [refs[t] for t in tmpl])
@@ -165,6 +173,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-max-frames", type=int, default=4096)
     ap.add_argument("--max-batch", type=int, default=64, help="frames per serving batch at most")
+    ap.add_argument("--serve-streams", type=int, default=2, help="submission streams of the server")
+    ap.add_argument("--serve-dump", default="", help="write per-request serving times to PREFIX_<load>.npz")
     ap.add_argument("--hbm-budget-mib", type=int, default=2048,
                     help="device bytes one serving batch may use for packed inputs and scratch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

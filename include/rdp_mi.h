@@ -176,19 +176,23 @@ unsigned rdp_mi_recording_passes(const rdp_mi_recording *r);
 void rdp_mi_recording_destroy(rdp_mi_recording *r);
 
 /* Serving (no reference counterpart: the reference serves one emulator per
- * device queue).  A server takes ownership of the scheduling of `count`
- * processors and renders frame requests with continuous batching:
- * requests that have arrived are admitted whenever their processor is idle
- * and the estimated device bytes of the batch fit in hbm_budget_bytes next to
- * the batches still on the GPU (at most max_batch per batch, 0 = count); the
- * admitted frames are decoded on decode_threads host threads (0 = up to 16)
- * and submitted as one batched launch set, while the previous batch runs.
- * Each processor renders its requests in arrival order and its results are
- * identical to rendering the frames alone.  While a server exists, its
- * processors must not be driven through the other entry points. */
+ * device queue).  A server takes over the scheduling of `count` processors
+ * and renders frame requests with continuous batching:
+ *  - a request that has arrived is admitted as soon as its processor is idle
+ *    and the estimated device bytes of admitted, not yet submitted frames
+ *    stay within hbm_budget_bytes (0 = 4 GiB); it is decoded at once on one
+ *    of decode_threads host threads (0 = up to 16);
+ *  - decoded frames are submitted together (at most max_batch per batch,
+ *    0 = count, within the budget) whenever one of `streams` submission
+ *    streams (0 = 2, at most 8) is free, so new work starts while earlier
+ *    batches still run; each stream keeps its own input/scratch arena.
+ * Each processor renders its requests in arrival order, one at a time, and
+ * its results are identical to rendering the frames alone.  While a server
+ * exists, its processors must not be driven through the other entry
+ * points. */
 typedef struct rdp_mi_server rdp_mi_server;
 int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t hbm_budget_bytes, unsigned max_batch,
-                         unsigned decode_threads, rdp_mi_server **out);
+                         unsigned decode_threads, unsigned streams, rdp_mi_server **out);
 /* Queue one frame (complete commands) for processor `instance`, arriving
  * arrival_us microseconds after rdp_mi_server_start (requests queued before
  * the start keep their schedule; after the start, earlier times mean now). */
@@ -198,10 +202,11 @@ int rdp_mi_server_start(rdp_mi_server *s);
 /* Block until every submitted request has completed. */
 int rdp_mi_server_drain(rdp_mi_server *s);
 /* Completed requests in completion order: ticket, processor, arrival,
- * admission and completion times (microseconds since the start).  Returns
- * the number completed so far; copies at most max. */
+ * admission, submission (decoded and queued to the GPU) and completion times
+ * (microseconds since the start).  Returns the number completed so far;
+ * copies at most max.  Any output pointer may be null. */
 size_t rdp_mi_server_results(rdp_mi_server *s, uint64_t *tickets, uint32_t *instances, double *arrival_us,
-                             double *admit_us, double *complete_us, size_t max);
+                             double *admit_us, double *submit_us, double *complete_us, size_t max);
 /* Counters so far: out[0] batches submitted, out[1] frames, out[2] frames in
  * the largest batch, out[3] largest estimated batch bytes (the admission
  * figure), out[4] largest actual packed-input plus scratch bytes of a batch. */

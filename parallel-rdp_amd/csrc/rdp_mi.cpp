@@ -117,16 +117,40 @@ struct PinnedBuffer {
 	}
 };
 
-struct Context {
-	bool ok = false;
+// A submission lane: a stream with its own pinned staging buffer and device
+// input/scratch arenas.  Work on one lane is stream ordered, so the arenas
+// are reused by consecutive submissions without host waits (except for the
+// staging buffer, which waits for the previous upload).
+struct Lane {
 	hipStream_t stream = nullptr;
-	uint8_t *d_lut = nullptr;
-	uint8_t *d_gamma = nullptr;
 	DeviceBuffer input, scratch;
 	PinnedBuffer staging;
 	size_t last_arena_bytes = 0;  // packed inputs + scratch of the last submission
 	hipEvent_t staging_done = nullptr;
 	bool staging_busy = false;
+
+	int create()
+	{
+		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+		HIP_CHECK(hipEventCreateWithFlags(&staging_done, hipEventDisableTiming));
+		return 0;
+	}
+	void release()
+	{
+		if (stream) (void)hipStreamSynchronize(stream);
+		if (input.ptr) (void)hipFree(input.ptr);
+		if (scratch.ptr) (void)hipFree(scratch.ptr);
+		if (staging.ptr) (void)hipHostFree(staging.ptr);
+		if (staging_done) (void)hipEventDestroy(staging_done);
+		if (stream) (void)hipStreamDestroy(stream);
+		*this = Lane();
+	}
+};
+
+struct Context : Lane {
+	bool ok = false;
+	uint8_t *d_lut = nullptr;
+	uint8_t *d_gamma = nullptr;
 	std::mutex lock;
 
 	int init()
@@ -134,8 +158,7 @@ struct Context {
 		if (ok) return 0;
 		int n = 0;
 		if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(RDP_MI_ERROR_DEVICE, "no HIP device");
-		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-		HIP_CHECK(hipEventCreateWithFlags(&staging_done, hipEventDisableTiming));
+		if (int err = create()) return err;
 		std::vector<uint8_t> lut(0x8000);
 		blender_divider_table(lut.data());
 		HIP_CHECK(hipMalloc(&d_lut, 0x8000));
@@ -244,7 +267,7 @@ static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, 
 
 // Pack the inputs of all rounds into `input` (one async copy on the context
 // stream) and describe the launches.  Caller holds the context lock.
-static int prepare_rounds(Context &c, const std::vector<std::vector<PassRef>> &rounds, DeviceBuffer &input,
+static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &rounds, DeviceBuffer &input,
                           DeviceBuffer &scratch, std::vector<Launch> &launches)
 {
 	launches.clear();
@@ -374,7 +397,7 @@ static int prepare_rounds(Context &c, const std::vector<std::vector<PassRef>> &r
 			D.rdram = pr.d_rdram;
 			D.hidden = pr.d_hidden;
 			D.tmem = pr.d_tmem;
-			D.blender_lut = c.d_lut;
+			D.blender_lut = ctx().d_lut;
 			D.rdram_mask = pr.rdram_size - 1;
 			D.num_prims = (uint32_t)hp.prims.size();
 			D.num_uploads = (uint32_t)hp.uploads.size();
@@ -407,8 +430,9 @@ static int prepare_rounds(Context &c, const std::vector<std::vector<PassRef>> &r
 	return 0;
 }
 
-// Submit rounds of passes through the shared arenas.  Caller holds the lock.
-static int submit_rounds(Context &c, const std::vector<std::vector<PassRef>> &rounds)
+// Submit rounds of passes through a lane's arenas.  The caller owns the lane
+// (for the context's own lane: holds the context lock).
+static int submit_rounds(Lane &c, const std::vector<std::vector<PassRef>> &rounds)
 {
 	std::vector<Launch> launches;
 	int err = prepare_rounds(c, rounds, c.input, c.scratch, launches);
@@ -434,10 +458,10 @@ static void build_round_list(Processor *const *procs, unsigned count, bool batch
 }
 
 // Submit every completed pass of the given processors (plain path: one
-// processor, one pass per round).
-static int submit_processors(Processor *const *procs, unsigned count, bool batched)
+// processor, one pass per round) on a lane (default: the context's).
+static int submit_processors(Processor *const *procs, unsigned count, bool batched, Lane *lane = nullptr)
 {
-	Context &c = ctx();
+	Lane &c = lane ? *lane : ctx();
 	std::vector<std::vector<PassRef>> rounds;
 	for (unsigned i = 0; i < count; i++) {
 		auto &passes = procs[i]->fe.completed();
@@ -490,19 +514,22 @@ struct Recording {
 // Serving: continuous batching of frame requests from many processors.
 //
 // Requests (one frame of commands for one processor, with an arrival time)
-// queue per processor.  A scheduler thread repeatedly
-//   1. admits every arrived request whose processor has nothing in flight, in
-//      round-robin order, while the batch's estimated device bytes (packed
-//      inputs plus kernel scratch) fit in the HBM budget (and up to
-//      max_batch).  Batches in flight share one stream-ordered input/scratch
-//      arena, so the budget bounds each batch rather than their sum;
-//   2. decodes the admitted frames on a pool of host threads (one front end
-//      per processor, so they are independent);
-//   3. submits all their passes as one batched submission and records an
-//      event, then goes straight back to 1 while the GPU works.
-// A completion thread waits on the batch events in order and stamps each
-// request's completion time.  Results are identical to rendering each frame
-// alone: processors share launches, never state.
+// queue per processor.  Three kinds of threads move them along:
+//   - the scheduler admits every arrived request whose processor has nothing
+//     in flight, in round-robin order, while the estimated device bytes of
+//     admitted-but-unsubmitted frames (packed inputs plus kernel scratch) fit
+//     in the HBM budget, and hands each admitted frame to the decode pool at
+//     once;
+//   - decode workers run the host front end of one frame each (front ends
+//     are per processor, so frames decode in parallel) and mark it ready;
+//   - the submitter takes every ready frame (up to max_batch, within the
+//     budget) whenever a submission lane is free, and submits their passes
+//     as one batched launch set on that lane, so a batch can start on one
+//     lane while a long one still runs on the other;
+//   - one completion thread per lane waits on its batches' events in order,
+//     stamps completion times and frees the processors and the lane.
+// Results are identical to rendering each frame alone: a processor has at
+// most one frame in flight and processors share launches, never state.
 // ---------------------------------------------------------------------------
 class ThreadPool {
 public:
@@ -519,39 +546,34 @@ public:
 		cv.notify_all();
 		for (auto &t : workers) t.join();
 	}
-	// Runs every job and returns when all have finished.
-	void run_all(std::vector<std::function<void()>> &jobs)
+	void post(std::function<void()> job)
 	{
-		if (jobs.empty()) return;
-		std::unique_lock<std::mutex> g(m);
-		pending = jobs.size();
-		for (auto &j : jobs) queue.push_back(&j);
-		cv.notify_all();
-		done_cv.wait(g, [this] { return pending == 0; });
+		{
+			std::lock_guard<std::mutex> g(m);
+			queue.push_back(std::move(job));
+		}
+		cv.notify_one();
 	}
 
 private:
 	void run()
 	{
 		for (;;) {
-			std::function<void()> *job;
+			std::function<void()> job;
 			{
 				std::unique_lock<std::mutex> g(m);
 				cv.wait(g, [this] { return stop || !queue.empty(); });
-				if (stop && queue.empty()) return;
-				job = queue.front();
+				if (queue.empty()) return;
+				job = std::move(queue.front());
 				queue.pop_front();
 			}
-			(*job)();
-			std::lock_guard<std::mutex> g(m);
-			if (--pending == 0) done_cv.notify_all();
+			job();
 		}
 	}
 	std::vector<std::thread> workers;
-	std::deque<std::function<void()> *> queue;
+	std::deque<std::function<void()>> queue;
 	std::mutex m;
-	std::condition_variable cv, done_cv;
-	size_t pending = 0;
+	std::condition_variable cv;
 	bool stop = false;
 };
 
@@ -565,34 +587,49 @@ struct ServeRequest {
 struct ServeResult {
 	uint64_t ticket;
 	uint32_t instance;
-	double arrival_us, admit_us, complete_us;
+	double arrival_us, admit_us, submit_us, complete_us;
 };
 
 struct Server {
 	std::vector<Processor *> procs;
 	size_t budget = 0;
 	unsigned max_batch = 0;
+	unsigned decode_threads = 0;
 	std::unique_ptr<ThreadPool> pool;
 
 	std::mutex m;
-	std::condition_variable cv;
+	std::condition_variable cv;  // scheduler: arrivals, freed processors, freed budget
+	std::condition_variable submit_cv;  // submitter: ready frames, free lanes
 	std::vector<std::deque<ServeRequest>> queues;
 	std::vector<bool> busy;
 	uint64_t next_ticket = 1;
-	size_t outstanding = 0;  // submitted but not completed
+	size_t outstanding = 0;    // submitted by the caller, not completed
+	size_t pending_bytes = 0;  // admitted, not yet submitted to the GPU
 	std::vector<ServeResult> results;
 	bool started = false, stopping = false;
 	int error = 0;
 	std::string error_msg;
 	std::chrono::steady_clock::time_point t0;
 
+	struct Ready {
+		ServeResult r;
+		size_t bytes;
+	};
+	std::deque<Ready> ready;
+
 	struct Batch {
 		hipEvent_t ev = nullptr;
 		std::vector<ServeResult> reqs;
 	};
-	std::deque<Batch> inflight;
-	std::condition_variable inflight_cv;
-	std::thread scheduler, completer;
+	struct LaneState {
+		Lane lane;
+		bool busy = false;
+		std::deque<Batch> inflight;
+		std::condition_variable cv;
+		std::thread completer;
+	};
+	std::vector<std::unique_ptr<LaneState>> lanes;
+	std::thread scheduler, submitter;
 	unsigned rr = 0;
 	// Statistics: batches, frames, largest batch, largest estimated and
 	// actual device bytes of one batch.
@@ -605,23 +642,43 @@ struct Server {
 	// snapshots), estimated from the command words before decoding.
 	static size_t estimate_bytes(const ServeRequest &r) { return r.words.size() * 4 * 24 + (1u << 20); }
 
+	bool done() const { return stopping && outstanding == 0; }
+	void decode(ServeRequest req, size_t bytes, double admit);
 	void scheduler_loop();
-	void completion_loop();
+	void submitter_loop();
+	void completion_loop(LaneState &ls);
 };
+
+void Server::decode(ServeRequest req, size_t bytes, double admit)
+{
+	Processor *p = procs[req.instance];
+	const std::vector<uint32_t> &w = req.words;
+	size_t i = 0;
+	while (i < w.size()) {
+		const unsigned len = Frontend::command_length(w[i]);
+		if (len == 0) {
+			i++;
+			continue;
+		}
+		if (i + len > w.size()) break;
+		p->fe.command(w.data() + i);
+		i += len;
+	}
+	p->fe.flush();
+	std::lock_guard<std::mutex> g(m);
+	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0 }, bytes });
+	submit_cv.notify_one();
+}
 
 void Server::scheduler_loop()
 {
-	Context &c = ctx();
 	std::unique_lock<std::mutex> g(m);
 	for (;;) {
-		if (stopping && outstanding == 0) break;
+		if (done()) break;
 		const double now = now_us();
-		// Admission.
-		std::vector<ServeRequest> batch;
-		size_t batch_bytes = 0;
 		double next_arrival = 1e300;
 		const unsigned n = (unsigned)procs.size();
-		for (unsigned k = 0; k < n && batch.size() < max_batch; k++) {
+		for (unsigned k = 0; k < n; k++) {
 			const unsigned i = (rr + k) % n;
 			if (busy[i] || queues[i].empty()) continue;
 			const ServeRequest &r = queues[i].front();
@@ -630,91 +687,99 @@ void Server::scheduler_loop()
 				continue;
 			}
 			const size_t est = estimate_bytes(r);
-			if (!batch.empty() && batch_bytes + est > budget) break;
-			batch_bytes += est;
+			if (pending_bytes != 0 && pending_bytes + est > budget) break;
+			pending_bytes += est;
 			busy[i] = true;
-			batch.push_back(std::move(queues[i].front()));
+			ServeRequest req = std::move(queues[i].front());
 			queues[i].pop_front();
-		}
-		if (batch.empty()) {
-			if (stopping && outstanding == 0) break;
-			if (next_arrival < 1e300)
-				cv.wait_for(g, std::chrono::microseconds((int64_t)std::max(1.0, next_arrival - now)));
-			else
-				cv.wait(g);
-			continue;
+			const double admit = now_us();
+			pool->post([this, req = std::move(req), est, admit]() mutable { decode(std::move(req), est, admit); });
 		}
 		rr = (rr + 1) % n;
-		g.unlock();
-
-		// Decode on the host pool.
-		const double admit = now_us();
-		std::vector<std::function<void()>> jobs;
-		jobs.reserve(batch.size());
-		for (auto &r : batch) {
-			Processor *p = procs[r.instance];
-			const std::vector<uint32_t> *w = &r.words;
-			jobs.emplace_back([p, w] {
-				size_t i = 0;
-				while (i < w->size()) {
-					const unsigned len = Frontend::command_length((*w)[i]);
-					if (len == 0) {
-						i++;
-						continue;
-					}
-					if (i + len > w->size()) break;
-					p->fe.command(w->data() + i);
-					i += len;
-				}
-				p->fe.flush();
-			});
-		}
-		pool->run_all(jobs);
-
-		// Submit as one batched submission.
-		Batch b;
-		int err = 0;
-		{
-			std::lock_guard<std::mutex> lk(c.lock);
-			std::vector<Processor *> ps;
-			for (auto &r : batch) ps.push_back(procs[r.instance]);
-			err = submit_processors(ps.data(), (unsigned)ps.size(), true);
-			if (!err && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess)
-				err = set_error(RDP_MI_ERROR_DEVICE, "event creation failed");
-			if (!err && hipEventRecord(b.ev, c.stream) != hipSuccess)
-				err = set_error(RDP_MI_ERROR_DEVICE, "event record failed");
-			n_batches++;
-			n_frames += batch.size();
-			max_frames = std::max<uint64_t>(max_frames, batch.size());
-			max_est_bytes = std::max<uint64_t>(max_est_bytes, batch_bytes);
-			max_arena_bytes = std::max<uint64_t>(max_arena_bytes, c.last_arena_bytes);
-		}
-		for (auto &r : batch) b.reqs.push_back(ServeResult{ r.ticket, r.instance, r.arrival_us, admit, 0.0 });
-
-		g.lock();
-		if (err) {
-			error = err;
-			error_msg = rdp_mi_last_error();
-		}
-		inflight.push_back(std::move(b));
-		inflight_cv.notify_all();
+		if (next_arrival < 1e300)
+			cv.wait_for(g, std::chrono::microseconds((int64_t)std::max(1.0, next_arrival - now_us())));
+		else
+			cv.wait(g);
 	}
-	g.unlock();
 }
 
-void Server::completion_loop()
+void Server::submitter_loop()
 {
 	std::unique_lock<std::mutex> g(m);
 	for (;;) {
-		inflight_cv.wait(g, [this] { return !inflight.empty() || (stopping && outstanding == 0); });
-		if (inflight.empty()) break;
-		hipEvent_t ev = inflight.front().ev;
+		LaneState *ls = nullptr;
+		submit_cv.wait(g, [&] {
+			if (done()) return true;
+			if (ready.empty()) return false;
+			for (auto &l : lanes)
+				if (!l->busy) return true;
+			return false;
+		});
+		if (ready.empty() && done()) break;
+		for (auto &l : lanes)
+			if (!l->busy) {
+				ls = l.get();
+				break;
+			}
+		if (!ls) continue;
+
+		// Take ready frames in order, within max_batch and the budget.
+		std::vector<Ready> batch;
+		size_t bytes = 0;
+		while (!ready.empty() && batch.size() < max_batch) {
+			if (!batch.empty() && bytes + ready.front().bytes > budget) break;
+			bytes += ready.front().bytes;
+			batch.push_back(ready.front());
+			ready.pop_front();
+		}
+		ls->busy = true;
+		g.unlock();
+
+		std::vector<Processor *> ps;
+		ps.reserve(batch.size());
+		for (auto &r : batch) ps.push_back(procs[r.r.instance]);
+		Batch b;
+		int err = submit_processors(ps.data(), (unsigned)ps.size(), true, &ls->lane);
+		if (!err && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess)
+			err = set_error(RDP_MI_ERROR_DEVICE, "event creation failed");
+		if (!err && hipEventRecord(b.ev, ls->lane.stream) != hipSuccess)
+			err = set_error(RDP_MI_ERROR_DEVICE, "event record failed");
+		const double submitted = now_us();
+		for (auto &r : batch) {
+			r.r.submit_us = submitted;
+			b.reqs.push_back(r.r);
+		}
+
+		g.lock();
+		if (err && !error) {
+			error = err;
+			error_msg = rdp_mi_last_error();
+		}
+		n_batches++;
+		n_frames += batch.size();
+		max_frames = std::max<uint64_t>(max_frames, batch.size());
+		max_est_bytes = std::max<uint64_t>(max_est_bytes, bytes);
+		max_arena_bytes = std::max<uint64_t>(max_arena_bytes, ls->lane.last_arena_bytes);
+		pending_bytes -= bytes;
+		ls->inflight.push_back(std::move(b));
+		ls->cv.notify_all();
+		cv.notify_all();  // budget freed
+	}
+}
+
+void Server::completion_loop(LaneState &ls)
+{
+	std::unique_lock<std::mutex> g(m);
+	for (;;) {
+		ls.cv.wait(g, [&] { return !ls.inflight.empty() || done(); });
+		if (ls.inflight.empty()) break;
+		hipEvent_t ev = ls.inflight.front().ev;
 		g.unlock();
 		if (ev) (void)hipEventSynchronize(ev);
 		const double t = now_us();
 		g.lock();
-		Batch b = std::move(inflight.front());
-		inflight.pop_front();
+		Batch b = std::move(ls.inflight.front());
+		ls.inflight.pop_front();
 		if (b.ev) (void)hipEventDestroy(b.ev);
 		for (auto &r : b.reqs) {
 			r.complete_us = t;
@@ -722,7 +787,13 @@ void Server::completion_loop()
 			busy[r.instance] = false;
 			outstanding--;
 		}
+		ls.busy = !ls.inflight.empty();
 		cv.notify_all();
+		submit_cv.notify_all();
+		if (done()) {
+			// Wake the other threads so they can see the end.
+			for (auto &l : lanes) l->cv.notify_all();
+		}
 	}
 }
 
@@ -1175,7 +1246,7 @@ unsigned rdp_mi_recording_rounds(const rdp_mi_recording *h) { return h ? (unsign
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *h) { return h ? h->r.passes : 0; }
 
 int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t hbm_budget_bytes, unsigned max_batch,
-                         unsigned decode_threads, rdp_mi_server **out)
+                         unsigned decode_threads, unsigned streams, rdp_mi_server **out)
 {
 	if (!out || !procs || !count) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
 	*out = nullptr;
@@ -1191,8 +1262,10 @@ int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t 
 	}
 	s.budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)1 << 32);
 	s.max_batch = max_batch ? max_batch : count;
-	unsigned threads = decode_threads ? decode_threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-	s.pool.reset(new ThreadPool(threads));
+	s.decode_threads = decode_threads ? decode_threads
+	                                  : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+	for (unsigned i = 0; i < (streams ? std::min(streams, 8u) : 2u); i++)
+		s.lanes.emplace_back(new Server::LaneState());
 	s.queues.resize(count);
 	s.busy.assign(count, false);
 	*out = h;
@@ -1233,11 +1306,30 @@ int rdp_mi_server_start(rdp_mi_server *h)
 		std::lock_guard<std::mutex> lk(c.lock);
 		int err = c.init();
 		if (err) return err;
+		// Work queued on the context stream for these processors finishes
+		// before the server's lanes touch them.
+		if (hipStreamSynchronize(c.stream) != hipSuccess) return set_error(RDP_MI_ERROR_DEVICE, "synchronisation failed");
 	}
+	// Size each lane's arenas up front (growing one later waits for the
+	// device): packed inputs run at about a quarter of a batch's estimate.
+	const size_t scratch_bytes = std::min<size_t>(s.budget, (size_t)256 << 20);
+	const size_t input_bytes = scratch_bytes / 4;
+	for (auto &l : s.lanes) {
+		Lane &ln = l->lane;
+		if (int err = ln.create()) return err;
+		if (ln.staging.reserve(input_bytes) || ln.input.reserve(input_bytes, ln.stream) ||
+		    ln.scratch.reserve(scratch_bytes, ln.stream))
+			return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "lane arenas of %zu bytes failed", input_bytes * 2 + scratch_bytes);
+	}
+	s.pool.reset(new ThreadPool(s.decode_threads));
 	s.t0 = std::chrono::steady_clock::now();
 	s.started = true;
 	s.scheduler = std::thread([&s] { s.scheduler_loop(); });
-	s.completer = std::thread([&s] { s.completion_loop(); });
+	s.submitter = std::thread([&s] { s.submitter_loop(); });
+	for (auto &l : s.lanes) {
+		Server::LaneState *ls = l.get();
+		ls->completer = std::thread([&s, ls] { s.completion_loop(*ls); });
+	}
 	return 0;
 }
 
@@ -1255,7 +1347,7 @@ int rdp_mi_server_drain(rdp_mi_server *h)
 }
 
 size_t rdp_mi_server_results(rdp_mi_server *h, uint64_t *tickets, uint32_t *instances, double *arrival_us,
-                             double *admit_us, double *complete_us, size_t max)
+                             double *admit_us, double *submit_us, double *complete_us, size_t max)
 {
 	if (!h) return 0;
 	Server &s = h->s;
@@ -1267,6 +1359,7 @@ size_t rdp_mi_server_results(rdp_mi_server *h, uint64_t *tickets, uint32_t *inst
 		if (instances) instances[i] = r.instance;
 		if (arrival_us) arrival_us[i] = r.arrival_us;
 		if (admit_us) admit_us[i] = r.admit_us;
+		if (submit_us) submit_us[i] = r.submit_us;
 		if (complete_us) complete_us[i] = r.complete_us;
 	}
 	return n;
@@ -1289,17 +1382,19 @@ void rdp_mi_server_destroy(rdp_mi_server *h)
 	if (!h) return;
 	Server &s = h->s;
 	{
+		// Requests still queued are rendered before the threads end.
 		std::lock_guard<std::mutex> g(s.m);
 		s.stopping = true;
 		s.cv.notify_all();
-		s.inflight_cv.notify_all();
+		s.submit_cv.notify_all();
+		for (auto &l : s.lanes) l->cv.notify_all();
 	}
 	if (s.scheduler.joinable()) s.scheduler.join();
-	{
-		std::lock_guard<std::mutex> g(s.m);
-		s.inflight_cv.notify_all();
-	}
-	if (s.completer.joinable()) s.completer.join();
+	if (s.submitter.joinable()) s.submitter.join();
+	for (auto &l : s.lanes)
+		if (l->completer.joinable()) l->completer.join();
+	s.pool.reset();
+	for (auto &l : s.lanes) l->lane.release();
 	delete h;
 }
 

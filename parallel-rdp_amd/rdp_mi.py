@@ -325,19 +325,20 @@ class Server:
 
     Each request is one frame of complete commands for one processor with an
     arrival time (microseconds after start()).  Arrived requests are admitted
-    whenever their processor is idle and the batch fits the device-byte
-    budget; their frames are decoded on host threads and submitted as one
-    batched launch set while earlier batches still run.  Per processor,
-    frames render in arrival order and the results equal rendering them alone.
+    whenever their processor is idle and the admitted, unsubmitted frames fit
+    the device-byte budget; each is decoded at once on a host thread, and
+    decoded frames are submitted together whenever one of the submission
+    streams is free, while earlier batches still run.  Per processor, frames
+    render in arrival order and the results equal rendering them alone.
     """
 
     def __init__(self, processors: Sequence[CommandProcessor], hbm_budget_bytes: int = 0, max_batch: int = 0,
-                 decode_threads: int = 0):
+                 decode_threads: int = 0, streams: int = 0):
         lib = load_library()
         vp, sz, cu = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint
         dp = ctypes.POINTER(ctypes.c_double)
         lib.rdp_mi_server_create.restype = ctypes.c_int
-        lib.rdp_mi_server_create.argtypes = [ctypes.POINTER(vp), cu, sz, cu, cu, ctypes.POINTER(vp)]
+        lib.rdp_mi_server_create.argtypes = [ctypes.POINTER(vp), cu, sz, cu, cu, cu, ctypes.POINTER(vp)]
         lib.rdp_mi_server_submit.restype = ctypes.c_int
         lib.rdp_mi_server_submit.argtypes = [vp, cu, vp, sz, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]
         lib.rdp_mi_server_start.restype = ctypes.c_int
@@ -346,14 +347,14 @@ class Server:
         lib.rdp_mi_server_drain.argtypes = [vp]
         lib.rdp_mi_server_results.restype = sz
         lib.rdp_mi_server_results.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
-                                              dp, dp, dp, sz]
+                                              dp, dp, dp, dp, sz]
         lib.rdp_mi_server_stats.argtypes = [vp, vp]
         lib.rdp_mi_server_destroy.argtypes = [vp]
         self._lib = lib
         self.processors = list(processors)
         arr = (vp * len(processors))(*[p.handle.value for p in processors])
         h = vp()
-        _check(lib.rdp_mi_server_create(arr, len(processors), hbm_budget_bytes, max_batch, decode_threads,
+        _check(lib.rdp_mi_server_create(arr, len(processors), hbm_budget_bytes, max_batch, decode_threads, streams,
                                         ctypes.byref(h)), "rdp_mi_server_create")
         self._h = h
 
@@ -372,13 +373,14 @@ class Server:
 
     def results(self) -> dict:
         """Completed requests in completion order: arrays ticket, instance,
-        arrival_us, admit_us, complete_us."""
-        n = int(self._lib.rdp_mi_server_results(self._h, None, None, None, None, None, 0))
+        arrival_us, admit_us, submit_us, complete_us."""
+        n = int(self._lib.rdp_mi_server_results(self._h, None, None, None, None, None, None, 0))
         out = {
             "ticket": np.zeros(n, np.uint64),
             "instance": np.zeros(n, np.uint32),
             "arrival_us": np.zeros(n, np.float64),
             "admit_us": np.zeros(n, np.float64),
+            "submit_us": np.zeros(n, np.float64),
             "complete_us": np.zeros(n, np.float64),
         }
         dp = ctypes.POINTER(ctypes.c_double)
@@ -386,7 +388,7 @@ class Server:
             self._h, out["ticket"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
             out["instance"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
             out["arrival_us"].ctypes.data_as(dp), out["admit_us"].ctypes.data_as(dp),
-            out["complete_us"].ctypes.data_as(dp), n))
+            out["submit_us"].ctypes.data_as(dp), out["complete_us"].ctypes.data_as(dp), n))
         return {k: v[:min(n, got)] for k, v in out.items()}
 
     def stats(self) -> dict:

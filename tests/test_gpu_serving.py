@@ -44,12 +44,12 @@ def _init(p, s):
     p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
 
 
-def _serve(streams, spacing_us, **server_kw):
+def _serve(instance_streams, spacing_us, **server_kw):
     import rdp_mi
-    procs = [rdp_mi.CommandProcessor(rs.RDRAM_SIZE) for _ in streams]
-    for p, s in zip(procs, streams):
+    procs = [rdp_mi.CommandProcessor(rs.RDRAM_SIZE) for _ in instance_streams]
+    for p, s in zip(procs, instance_streams):
         _init(p, s)
-    reqs = _requests(streams)
+    reqs = _requests(instance_streams)
     # Interleave the instances: request k of every instance before request k+1.
     rank, seen = [], {}
     for inst, _ in reqs:
@@ -74,8 +74,10 @@ def _serve(streams, spacing_us, **server_kw):
     (0.0, {}),                                   # everything queued at once
     (40.0, {}),                                  # arrivals while batches run
     (10.0, {"max_batch": 3}),                    # small slots
-    (10.0, {"hbm_budget_bytes": 1}),             # budget admits one frame per batch
+    (10.0, {"hbm_budget_bytes": 1}),             # budget admits one frame at a time
     (0.0, {"decode_threads": 1}),
+    (20.0, {"streams": 1}),
+    (20.0, {"streams": 4, "decode_threads": 3}),
 ])
 def test_serving_matches_alone_and_oracle(spacing_us, kw):
     streams = _instances()
