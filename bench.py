@@ -41,6 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "parallel-rdp_amd"))
 import rdp_streams as rs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 39.3  # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, 32-bit integer lane operations
 
 
 def log(msg):
@@ -253,11 +254,20 @@ def main():
 
     # HBM bytes per k_raster launch from the committed rocprof counter runs
     # (FETCH_SIZE and WRITE_SIZE, scaled by a plain-copy calibration).
-    traffic = None
+    traffic, valu = None, None
     traffic_path = os.path.join(ROOT, "profiles", "k_raster_batch64_traffic.json")
     if os.path.exists(traffic_path):
         with open(traffic_path) as fh:
-            traffic = json.load(fh).get("traffic_bytes_per_launch")
+            prof = json.load(fh)
+        traffic = prof.get("traffic_bytes_per_launch")
+        # VALU issue rate: the committed SQ_INSTS_VALU count per launch of this
+        # deterministic workload over the live launch duration.
+        if prof.get("valu_insts_per_launch") and raster_ms_per_launch > 0 and args.streams == 64:
+            ops = prof["valu_insts_per_launch"] * 64 / (raster_ms_per_launch / 1e3)
+            valu = {"achieved": round(ops / 1e12, 3), "peak": VALU_PEAK_TOPS, "unit": "T lane-ops/s",
+                    "frac": round(ops / 1e12 / VALU_PEAK_TOPS, 4),
+                    "insts_per_launch": prof["valu_insts_per_launch"],
+                    "salu_insts_per_launch": prof.get("salu_insts_per_launch")}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -295,7 +305,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_raster", "bytes_per_launch": int(bytes_per_launch),
-                     "ms_per_launch": round(raster_ms_per_launch, 4)},
+                     "ms_per_launch": round(raster_ms_per_launch, 4), "valu_issue": valu},
         "cpu_baseline": cpu,
         "serving": serve,
     }
