@@ -310,6 +310,7 @@ void Frontend::split_at_last_mark()
 	head.fb_height = m.fb_height;
 	head.num_spans = m.spans;
 	head.num_tilerows = m.tilerows;
+	for (const PrimRecord &r : head.prims) head.raster_class = std::max(head.raster_class, r.raster_class);
 	if (!head.empty()) done.push_back(std::move(head));
 
 	HostPass tail;
@@ -331,12 +332,14 @@ void Frontend::split_at_last_mark()
 		r.span_offset -= m.spans;
 		r.tilerow_offset -= m.tilerows;
 		tail.fb_height = std::max(tail.fb_height, (uint32_t)(r.yhi + 1));
+		tail.raster_class = std::max(tail.raster_class, r.raster_class);
 		tail.prims.push_back(r);
 	}
 	tail.num_spans = cur.num_spans - m.spans;
 	tail.num_tilerows = cur.num_tilerows - m.tilerows;
 	cur = std::move(tail);
 	state_dirty = true;
+	state_serial++;
 }
 
 void Frontend::split_or_flush()
@@ -528,6 +531,16 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 			resolve_state(d);
 			cur.states.push_back(d);
 			state_dirty = false;
+			state_serial++;
+		}
+		// The kernel's variant choice (k_setup) from the same state and setup
+		// tile, reduced to its occupancy class.
+		const uint32_t setup_tile = (op >= 0x08 && op <= 0x0f) ? (w[0] >> 16) & 63 : (op == 0x36 ? 0 : (w[1] >> 24) & 7);
+		if (class_state_serial != state_serial || class_setup_tile != setup_tile) {
+			const DrawState &st = cur.states.back();
+			class_value = raster_variant_class(select_raster_variant(st.ras_flags, st.comb[1], st.tiles, setup_tile));
+			class_state_serial = state_serial;
+			class_setup_tile = setup_tile;
 		}
 
 		PrimRecord r;
@@ -541,6 +554,8 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 		r.span_offset = cur.num_spans;
 		r.tilerow_offset = cur.num_tilerows;
 		r.prim_index = prim_index;
+		r.raster_class = class_value;
+		cur.raster_class = std::max(cur.raster_class, class_value);
 		cur.words.insert(cur.words.end(), w, w + len);
 		cur.prims.push_back(r);
 		cur.num_spans += lines;

@@ -45,6 +45,7 @@ struct HostPass {
 	std::vector<PassMark> marks;
 	uint32_t fb_addr = 0, fb_width = 0, fb_fmt = 0, depth_addr = 0, fb_height = 0;
 	uint32_t num_spans = 0, num_tilerows = 0;
+	uint32_t raster_class = 0;  // highest raster occupancy class of its primitives
 	bool empty() const { return prims.empty() && uploads.empty(); }
 };
 
@@ -128,6 +129,9 @@ private:
 	uint64_t prim_counter = 0;
 	bool state_dirty = true;
 	uint64_t state_gen = 0;  // bumped by every command that is not a draw
+	// Raster class of the last (draw state, setup tile) pair.
+	uint64_t class_state_serial = 0, state_serial = 1;
+	uint32_t class_setup_tile = 0, class_value = 0;
 	bool color_write_pending = false, depth_write_pending = false;
 
 	HostPass cur;

@@ -2021,7 +2021,7 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *st
 }
 
 #ifndef RDP_RASTER_WAVES_PER_EU
-#define RDP_RASTER_WAVES_PER_EU 4
+#define RDP_RASTER_WAVES_PER_EU 5
 #endif
 
 // Workgroups are dispatched to the 8 XCDs round robin (block b runs on XCD
@@ -2056,7 +2056,11 @@ DEV uint32_t xcd_local_item(uint32_t b, uint32_t n)
 #endif
 }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_RASTER_WAVES_PER_EU)))
+// CLS: occupancy class; the kernel holds the variants of classes <= CLS (and
+// the generic code in the top class).  The host launches each pass in the
+// class of its heaviest primitive.
+template <uint32_t CLS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CLS == 0 ? 8 : RDP_RASTER_WAVES_PER_EU)))
 void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	__shared__ uint16_t s_tmem[2048];
@@ -2127,14 +2131,18 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 
 			if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
 			switch (ps->variant) {
-#define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt)                                                      \
+#define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt, cls)                                                 \
 	case id:                                                                                                    \
-		raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt)>>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, \
-		                                                                    px);                              \
+		if constexpr ((cls) <= CLS)                                                                             \
+			raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt)>>(ps, states, spans, tex, lut, fmt, alias, ci, \
+			                                                                    x, y, px);                        \
 		break;
 				RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
 #undef RDP_DISPATCH_VARIANT
-			default: raster_prim<GenericVariant>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px); break;
+			default:
+				if constexpr (CLS == kRasterClasses - 1)
+					raster_prim<GenericVariant>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px);
+				break;
 			}
 		}
 	}
@@ -2170,10 +2178,14 @@ hipError_t launch_bin(const PassDesc *passes, const uint32_t *items, uint32_t n,
 	return hipGetLastError();
 }
 
-hipError_t launch_raster(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+hipError_t launch_raster(uint32_t cls, const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
 {
 	if (!n) return hipSuccess;
-	hipLaunchKernelGGL(k_raster, dim3(n), dim3(64), 0, s, passes, items);
+	switch (cls) {
+	case 0: hipLaunchKernelGGL(k_raster<0>, dim3(n), dim3(64), 0, s, passes, items); break;
+	case 1: hipLaunchKernelGGL(k_raster<1>, dim3(n), dim3(64), 0, s, passes, items); break;
+	default: hipLaunchKernelGGL(k_raster<2>, dim3(n), dim3(64), 0, s, passes, items); break;
+	}
 	return hipGetLastError();
 }
 

@@ -36,7 +36,7 @@ namespace rdp_mi {
 hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
-hipError_t launch_raster(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 
 static thread_local std::string g_last_error;
 
@@ -225,11 +225,14 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 	return l;
 }
 
-// One round: the four stage launches covering every pass of the round.
+// One round: the four stage launches covering every pass of the round (the
+// raster stage as one launch per occupancy class present).
 struct Launch {
 	const PassDesc *descs;
 	const uint32_t *items[4];
 	uint32_t n[4];
+	const uint32_t *raster_items[kRasterClasses];
+	uint32_t raster_n[kRasterClasses];
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
 	uint32_t passes;
 };
@@ -259,7 +262,8 @@ static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, 
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
 		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
-		HIP_CHECK(launch_raster(L.descs, L.items[3], L.n[3], s));
+		for (uint32_t c = 0; c < kRasterClasses; c++)
+			HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
 	}
 	return 0;
@@ -281,6 +285,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	struct RoundLayout {
 		size_t descs, items_setup, items_tmem, items_bin, items_raster;
 		uint32_t n_setup, n_tmem, n_bin, n_raster;
+		uint32_t n_raster_class[kRasterClasses];
 		size_t scratch;
 	};
 	std::vector<RoundLayout> rl(rounds.size());
@@ -295,6 +300,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	for (size_t r = 0; r < rounds.size(); r++) {
 		RoundLayout &L = rl[r];
 		L.n_setup = L.n_tmem = L.n_bin = L.n_raster = 0;
+		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = 0;
 		size_t scr = 0;
 		pl[r].resize(rounds[r].size());
 		for (size_t i = 0; i < rounds[r].size(); i++) {
@@ -306,6 +312,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			if (!hp.prims.empty()) {
 				L.n_bin += ty * groups;
 				L.n_raster += tx * ty;
+				L.n_raster_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
 			}
 			PassInputs &pi = pl[r][i];
 			pi.sl = scratch_layout(hp);
@@ -357,7 +364,13 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		uint32_t *it_tmem = reinterpret_cast<uint32_t *>(h + L.items_tmem);
 		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
-		uint32_t ns = 0, nt = 0, nb = 0, nr = 0;
+		uint32_t ns = 0, nt = 0, nb = 0;
+		uint32_t nr[kRasterClasses], rbase[kRasterClasses];
+		for (uint32_t c = 0, acc = 0; c < kRasterClasses; c++) {
+			nr[c] = 0;
+			rbase[c] = acc;
+			acc += L.n_raster_class[c];
+		}
 		Launch &LA = launches[r];
 		LA.descs = reinterpret_cast<const PassDesc *>(d + L.descs);
 		LA.items[0] = reinterpret_cast<const uint32_t *>(d + L.items_setup);
@@ -368,6 +381,10 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		LA.n[1] = L.n_tmem;
 		LA.n[2] = L.n_bin;
 		LA.n[3] = L.n_raster;
+		for (uint32_t c = 0; c < kRasterClasses; c++) {
+			LA.raster_items[c] = LA.items[3] + rbase[c];
+			LA.raster_n[c] = L.n_raster_class[c];
+		}
 		LA.raster_bytes = 0;
 		LA.passes = (uint32_t)rounds[r].size();
 		for (size_t i = 0; i < rounds[r].size(); i++) {
@@ -418,8 +435,10 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)
 					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 13) | (y << 6) | g;
+				const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
+				uint32_t *out = it_raster + rbase[cls];
 				for (uint32_t y = 0; y < D.tiles_y; y++)
-					for (uint32_t x = 0; x < D.tiles_x; x++) it_raster[nr++] = (pidx << 14) | (x << 7) | y;
+					for (uint32_t x = 0; x < D.tiles_x; x++) out[nr[cls]++] = (pidx << 14) | (x << 7) | y;
 			}
 		}
 	}

@@ -143,19 +143,25 @@ constexpr uint32_t RAS_2CYC_LOD_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_U
                                        RAS_PERSPECTIVE | RAS_SAMPLE_QUAD | RAS_BILERP0 | RAS_BILERP1 | RAS_TLUT |
                                        RAS_TEX_LOD;
 
-// X(id, ras_known, ras_value, combiner, texture format)
+// X(id, ras_known, ras_value, combiner, texture format, occupancy class).
+// The raster kernel is built once per occupancy class with only the variants
+// of that class and below (the top class adds the generic code), and a pass
+// runs in the lowest class that holds all its primitives' variants: light
+// passes then get the register budget, and so the waves per SIMD, of their
+// own code rather than of the largest variant.
 #define RDP_RASTER_VARIANTS(X)                                                                                     \
-	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY)                                                      \
-	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY)                                                      \
-	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY)                                                              \
-	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY)                                                                \
-	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16)                        \
-	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY)                                            \
-	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY)                                                  \
-	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY)                                                   \
-	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI8)                                  \
-	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI4)                                 \
-	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY)
+	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY, 0)                                                   \
+	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY, 0)                                                   \
+	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY, 0)                                                           \
+	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY, 0)                                                             \
+	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, 0)                     \
+	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY, 1)                                         \
+	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY, 1)                                               \
+	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY, 1)                                                \
+	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI8, 1)                               \
+	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI4, 1)                              \
+	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, 2)
+constexpr uint32_t kRasterClasses = 3;  // the generic variant (id 0) is in the top class
 
 // The format every tile a primitive can sample shares, or TEXFMT_ANY.
 // Without texture LOD the sampled tiles are base (texel 0, pipelined texel 1)
@@ -179,7 +185,7 @@ RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t c
                                              uint32_t setup_tile)
 {
 	const int tf = static_texture_format(ras_flags, tiles, setup_tile);
-#define RDP_MATCH_VARIANT(id, known, value, comb, texfmt)                                                  \
+#define RDP_MATCH_VARIANT(id, known, value, comb, texfmt, cls)                                             \
 	if ((ras_flags & (known)) == (value) && ((texfmt) == TEXFMT_ANY || (texfmt) == tf)) {                 \
 		bool ok = true;                                                                                     \
 		if ((comb) != COMB_ANY)                                                                             \
@@ -189,6 +195,15 @@ RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t c
 	RDP_RASTER_VARIANTS(RDP_MATCH_VARIANT)
 #undef RDP_MATCH_VARIANT
 	return 0;
+}
+
+RDP_HD inline uint32_t raster_variant_class(uint32_t variant)
+{
+#define RDP_VARIANT_CLASS(id, known, value, comb, texfmt, cls) \
+	if (variant == (id)) return (cls);
+	RDP_RASTER_VARIANTS(RDP_VARIANT_CLASS)
+#undef RDP_VARIANT_CLASS
+	return kRasterClasses - 1;
 }
 
 // Resolved draw state, snapshotted by the host whenever any state command
@@ -227,7 +242,7 @@ struct PrimRecord {
 	uint32_t span_offset;   // first SpanSetup record (lines ylo .. yhi+1)
 	uint32_t tilerow_offset;// first tile-row range record (rows ylo>>3 .. yhi>>3)
 	uint32_t prim_index;    // stream-global primitive counter (noise seed)
-	uint32_t pad;
+	uint32_t raster_class;  // occupancy class of the primitive's raster variant
 };
 static_assert(sizeof(PrimRecord) == 32, "PrimRecord layout");
 
