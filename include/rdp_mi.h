@@ -170,6 +170,20 @@ int rdp_mi_recording_replay(rdp_mi_recording *r, unsigned iterations, int use_gr
 int rdp_mi_recording_profile(rdp_mi_recording *r, unsigned iterations, float stage_ms[4]);
 /* Per-round variant: round_stage_ms holds rounds * 4 entries. */
 int rdp_mi_recording_profile_rounds(rdp_mi_recording *r, unsigned iterations, float *round_stage_ms);
+/* Launch-shape autotuner (no reference counterpart; the reference picks its
+ * shader variants and workgroup layout statically, rdp_renderer.cpp:1790-1822).
+ * Times the recorded submission under every candidate raster launch shape:
+ * the XCD chunk of the tile-to-workgroup mapping (-1 block order, 0 one
+ * contiguous eighth per XCD, 32/64/128 tiles per chunk) and the lowest
+ * occupancy class passes run in (0-2); keeps the fastest for later replays
+ * and reports it.  Each candidate is timed over `iterations` replays, best of
+ * two sweeps. */
+int rdp_mi_recording_autotune(rdp_mi_recording *r, unsigned iterations, int32_t *xcd_chunk, uint32_t *min_class,
+                              float *best_ms);
+/* Launch shape for immediate submissions (flush, batch flush, serving), for
+ * example the one an autotuned recording of the same workload picked.
+ * Default: xcd_chunk 64, min_class 0. */
+void rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class);
 uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_rounds(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *r);

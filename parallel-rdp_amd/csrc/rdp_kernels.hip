@@ -2020,51 +2020,56 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *st
 		depth_blend(lut, ps, fmt, alias, px, sh);
 }
 
-#ifndef RDP_RASTER_WAVES_PER_EU
-#define RDP_RASTER_WAVES_PER_EU 5
+// Waves per SIMD each occupancy class is compiled for (the register budget
+// follows: 512 / waves, in steps of 8).
+#ifndef RDP_RASTER_WAVES_CLASS0
+#define RDP_RASTER_WAVES_CLASS0 8
+#endif
+#ifndef RDP_RASTER_WAVES_CLASS1
+#define RDP_RASTER_WAVES_CLASS1 5
+#endif
+#ifndef RDP_RASTER_WAVES_CLASS2
+#define RDP_RASTER_WAVES_CLASS2 6
 #endif
 
 // Workgroups are dispatched to the 8 XCDs round robin (block b runs on XCD
 // b % 8) and every XCD has its own L2.  Neighbouring tiles read the same
-// primitive and span records, so consecutive chunks of RDP_XCD_CHUNK work
-// items are kept on one XCD: chunk c goes to XCD c % 8.  0 gives each XCD one
-// contiguous eighth of the items; a negative value keeps block order.
-#ifndef RDP_XCD_CHUNK
-#define RDP_XCD_CHUNK 64
-#endif
+// primitive and span records, so consecutive chunks of `chunk` work items are
+// kept on one XCD: chunk c goes to XCD c % 8.  chunk 0 gives each XCD one
+// contiguous eighth of the items; a negative chunk keeps block order.  The
+// chunk is a launch parameter chosen by the host (autotuned per workload).
 #define RDP_NUM_XCDS 8
 
-DEV uint32_t xcd_local_item(uint32_t b, uint32_t n)
+DEV uint32_t xcd_local_item(uint32_t b, uint32_t n, int32_t chunk)
 {
 	const uint32_t xcd = b % RDP_NUM_XCDS, k = b / RDP_NUM_XCDS;
-#if RDP_XCD_CHUNK < 0
-	(void)xcd;
-	(void)k;
-	(void)n;
-	return b;
-#elif RDP_XCD_CHUNK == 0
-	const uint32_t q = n / RDP_NUM_XCDS, r = n % RDP_NUM_XCDS;
-	return xcd * q + (xcd < r ? xcd : r) + k;
-#else
+	if (chunk < 0) return b;
+	if (chunk == 0) {
+		const uint32_t q = n / RDP_NUM_XCDS, r = n % RDP_NUM_XCDS;
+		// Item order: XCD x owns items [x*q + min(x, r), ...) of length q (+1 if x < r).
+		const uint32_t len = q + (xcd < r ? 1u : 0u);
+		return k < len ? xcd * q + (xcd < r ? xcd : r) + k : b;
+	}
 	// Block b is the k-th block of its XCD; that XCD's k-th item is in its
-	// (k / CHUNK)-th chunk.  The last, partial round of chunks keeps the
+	// (k / chunk)-th chunk.  The last, partial round of chunks keeps the
 	// identity mapping.
-	const uint32_t per_round = RDP_NUM_XCDS * RDP_XCD_CHUNK;
+	const uint32_t c = (uint32_t)chunk;
+	const uint32_t per_round = RDP_NUM_XCDS * c;
 	if (b >= (n / per_round) * per_round) return b;
-	const uint32_t chunk = (k / RDP_XCD_CHUNK) * RDP_NUM_XCDS + xcd;
-	return chunk * RDP_XCD_CHUNK + k % RDP_XCD_CHUNK;
-#endif
+	const uint32_t ch = (k / c) * RDP_NUM_XCDS + xcd;
+	return ch * c + k % c;
 }
 
 // CLS: occupancy class; the kernel holds the variants of classes <= CLS (and
 // the generic code in the top class).  The host launches each pass in the
 // class of its heaviest primitive.
 template <uint32_t CLS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CLS == 0 ? 8 : RDP_RASTER_WAVES_PER_EU)))
-void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
+	CLS == 0 ? RDP_RASTER_WAVES_CLASS0 : (CLS == 1 ? RDP_RASTER_WAVES_CLASS1 : RDP_RASTER_WAVES_CLASS2))))
+void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
 {
 	__shared__ uint16_t s_tmem[2048];
-	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x)];
+	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)];
 	const PassDesc &P = passes[item >> 14];
 	const uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
 	const int lane = threadIdx.x;
@@ -2178,13 +2183,14 @@ hipError_t launch_bin(const PassDesc *passes, const uint32_t *items, uint32_t n,
 	return hipGetLastError();
 }
 
-hipError_t launch_raster(uint32_t cls, const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+hipError_t launch_raster(uint32_t cls, const PassDesc *passes, const uint32_t *items, uint32_t n, int32_t xcd_chunk,
+                         hipStream_t s)
 {
 	if (!n) return hipSuccess;
 	switch (cls) {
-	case 0: hipLaunchKernelGGL(k_raster<0>, dim3(n), dim3(64), 0, s, passes, items); break;
-	case 1: hipLaunchKernelGGL(k_raster<1>, dim3(n), dim3(64), 0, s, passes, items); break;
-	default: hipLaunchKernelGGL(k_raster<2>, dim3(n), dim3(64), 0, s, passes, items); break;
+	case 0: hipLaunchKernelGGL(k_raster<0>, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+	case 1: hipLaunchKernelGGL(k_raster<1>, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+	default: hipLaunchKernelGGL(k_raster<2>, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
 	}
 	return hipGetLastError();
 }

@@ -36,7 +36,15 @@ namespace rdp_mi {
 hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
-hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+
+// Launch shape of the raster stage: the XCD chunk of the tile-to-block
+// mapping and the lowest occupancy class a pass runs in (passes below it are
+// promoted, merging their launches).  Chosen per workload by the autotuner.
+struct LaunchConfig {
+	int32_t xcd_chunk = 64;
+	uint32_t min_class = 0;
+};
 
 static thread_local std::string g_last_error;
 
@@ -151,6 +159,7 @@ struct Context : Lane {
 	bool ok = false;
 	uint8_t *d_lut = nullptr;
 	uint8_t *d_gamma = nullptr;
+	LaunchConfig cfg;  // for immediate submissions (flush, batch flush, serving)
 	std::mutex lock;
 
 	int init()
@@ -252,7 +261,8 @@ static uint64_t framebuffer_bytes(const HostPass &hp)
 	return hp.prims.empty() ? 0 : 2 * tiles * 64 * (color + depth);
 }
 
-static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, hipEvent_t *stage_events = nullptr)
+static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, const LaunchConfig &cfg,
+                            hipEvent_t *stage_events = nullptr)
 {
 	for (size_t r = 0; r < launches.size(); r++) {
 		const Launch &L = launches[r];
@@ -262,8 +272,14 @@ static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, 
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
 		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
-		for (uint32_t c = 0; c < kRasterClasses; c++)
-			HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], s));
+		// Class segments are contiguous in class order, so promoting the
+		// classes below min_class merges them into one launch.
+		const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
+		uint32_t merged = 0;
+		for (uint32_t c = 0; c <= lo; c++) merged += L.raster_n[c];
+		HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
+		for (uint32_t c = lo + 1; c < kRasterClasses; c++)
+			HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
 	}
 	return 0;
@@ -456,7 +472,7 @@ static int submit_rounds(Lane &c, const std::vector<std::vector<PassRef>> &round
 	std::vector<Launch> launches;
 	int err = prepare_rounds(c, rounds, c.input, c.scratch, launches);
 	if (err) return err;
-	return execute_launches(launches, c.stream);
+	return execute_launches(launches, c.stream, ctx().cfg);
 }
 
 static void build_round_list(Processor *const *procs, unsigned count, bool batched,
@@ -523,6 +539,7 @@ static int copy_sync(void *dst, const void *src, size_t size, hipMemcpyKind kind
 struct Recording {
 	std::vector<Launch> launches;
 	DeviceBuffer input, scratch;
+	LaunchConfig cfg;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	uint64_t raster_bytes = 0;
@@ -1158,6 +1175,7 @@ int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, r
 		h->r.raster_bytes += L.raster_bytes;
 		h->r.passes += L.passes;
 	}
+	h->r.cfg = c.cfg;
 	*out = h;
 	return 0;
 }
@@ -1166,7 +1184,7 @@ static int ensure_graph(Context &c, Recording &r)
 {
 	if (r.exec) return 0;
 	HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-	int err = execute_launches(r.launches, c.stream);
+	int err = execute_launches(r.launches, c.stream, r.cfg);
 	hipGraph_t graph = nullptr;
 	hipError_t e = hipStreamEndCapture(c.stream, &graph);
 	if (err) return err;
@@ -1195,7 +1213,7 @@ int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_gr
 			hipError_t e = hipGraphLaunch(r.exec, c.stream);
 			if (e != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
 		} else
-			err = execute_launches(r.launches, c.stream);
+			err = execute_launches(r.launches, c.stream, r.cfg);
 		if (!err && !ev.empty()) HIP_CHECK(hipEventRecord(ev[i + 1], c.stream));
 	}
 	hipError_t se = hipStreamSynchronize(c.stream);
@@ -1204,6 +1222,60 @@ int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_gr
 		for (unsigned i = 0; i < iterations; i++) (void)hipEventElapsedTime(&iteration_ms[i], ev[i], ev[i + 1]);
 	for (auto &e : ev) (void)hipEventDestroy(e);
 	return err;
+}
+
+int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t *xcd_chunk, uint32_t *min_class,
+                              float *best_ms)
+{
+	if (!h || iterations == 0) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Recording &r = h->r;
+	static const int32_t chunks[] = { -1, 0, 32, 64, 128 };
+	std::vector<LaunchConfig> cands;
+	for (uint32_t m = 0; m < kRasterClasses; m++)
+		for (int32_t ch : chunks) cands.push_back(LaunchConfig{ ch, m });
+	std::vector<float> best(cands.size(), 1e30f);
+	hipEvent_t e0, e1;
+	HIP_CHECK(hipEventCreate(&e0));
+	HIP_CHECK(hipEventCreate(&e1));
+	int err = execute_launches(r.launches, c.stream, r.cfg);  // warm caches and code
+	// Two sweeps over the candidates, best of each candidate's timed runs.
+	for (int sweep = 0; sweep < 2 && !err; sweep++)
+		for (size_t i = 0; i < cands.size() && !err; i++) {
+			if ((err = execute_launches(r.launches, c.stream, cands[i]))) break;
+			HIP_CHECK(hipEventRecord(e0, c.stream));
+			for (unsigned it = 0; it < iterations && !err; it++) err = execute_launches(r.launches, c.stream, cands[i]);
+			HIP_CHECK(hipEventRecord(e1, c.stream));
+			HIP_CHECK(hipEventSynchronize(e1));
+			float ms = 0;
+			(void)hipEventElapsedTime(&ms, e0, e1);
+			best[i] = std::min(best[i], ms / (float)iterations);
+		}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (err) return err;
+	size_t pick = 0;
+	for (size_t i = 1; i < cands.size(); i++)
+		if (best[i] < best[pick]) pick = i;
+	r.cfg = cands[pick];
+	// A captured graph holds the old launch shape: capture again on next use.
+	if (r.exec) (void)hipGraphExecDestroy(r.exec);
+	if (r.graph) (void)hipGraphDestroy(r.graph);
+	r.exec = nullptr;
+	r.graph = nullptr;
+	if (xcd_chunk) *xcd_chunk = r.cfg.xcd_chunk;
+	if (min_class) *min_class = r.cfg.min_class;
+	if (best_ms) *best_ms = best[pick];
+	return 0;
+}
+
+void rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class)
+{
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	c.cfg.xcd_chunk = xcd_chunk;
+	c.cfg.min_class = std::min(min_class, kRasterClasses - 1);
 }
 
 int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, float *round_stage_ms)
@@ -1219,7 +1291,7 @@ int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, fl
 	int err = 0;
 	for (unsigned it = 0; it < iterations && !err; it++) {
 		HIP_CHECK(hipEventRecord(ev[0], c.stream));
-		err = execute_launches(r.launches, c.stream, ev.data() + 1);
+		err = execute_launches(r.launches, c.stream, r.cfg, ev.data() + 1);
 		HIP_CHECK(hipStreamSynchronize(c.stream));
 		for (size_t k = 0; k < nev && !err; k++) {
 			float ms = 0;
@@ -1245,7 +1317,7 @@ int rdp_mi_recording_profile(rdp_mi_recording *h, unsigned iterations, float sta
 	int err = 0;
 	for (unsigned it = 0; it < iterations && !err; it++) {
 		HIP_CHECK(hipEventRecord(ev[0], c.stream));
-		err = execute_launches(r.launches, c.stream, ev.data() + 1);
+		err = execute_launches(r.launches, c.stream, r.cfg, ev.data() + 1);
 		HIP_CHECK(hipStreamSynchronize(c.stream));
 		for (size_t k = 0; k < nev && !err; k++) {
 			float ms = 0;

@@ -247,6 +247,13 @@ class CommandProcessor:
         return int(self._lib.rdp_mi_primitive_count(self._h))
 
 
+def set_launch_config(xcd_chunk: int = 64, min_class: int = 0):
+    """Raster launch shape for immediate submissions (flush, batch flush, serving)."""
+    lib = load_library()
+    lib.rdp_mi_set_launch_config.argtypes = [ctypes.c_int32, ctypes.c_uint32]
+    lib.rdp_mi_set_launch_config(xcd_chunk, min_class)
+
+
 def batch_flush(processors: Sequence[CommandProcessor]):
     lib = load_library()
     arr = (ctypes.c_void_p * len(processors))(*[p.handle.value for p in processors])
@@ -283,6 +290,19 @@ class Recording:
         ms = np.zeros(iterations, dtype=np.float32)
         _check(self._lib.rdp_mi_recording_replay(self._h, iterations, int(use_graph), ms.ctypes.data), "replay")
         return ms
+
+    def autotune(self, iterations: int = 3) -> dict:
+        """Times every raster launch shape on this submission and keeps the
+        fastest; returns it with its time per replay."""
+        self._lib.rdp_mi_recording_autotune.restype = ctypes.c_int
+        self._lib.rdp_mi_recording_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint,
+                                                        ctypes.POINTER(ctypes.c_int32),
+                                                        ctypes.POINTER(ctypes.c_uint32),
+                                                        ctypes.POINTER(ctypes.c_float)]
+        chunk, cls, ms = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_float()
+        _check(self._lib.rdp_mi_recording_autotune(self._h, iterations, ctypes.byref(chunk), ctypes.byref(cls),
+                                                   ctypes.byref(ms)), "autotune")
+        return {"xcd_chunk": chunk.value, "min_class": cls.value, "ms": ms.value}
 
     def profile(self, iterations: int = 1) -> np.ndarray:
         ms = np.zeros(4, dtype=np.float32)

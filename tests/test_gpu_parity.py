@@ -137,3 +137,39 @@ def test_batch_flush_many_processors(rdp_mi):
         p.idle()
         assert_same(_state(p), harness.run_oracle(s), s.name)
         p.close()
+
+
+@pytest.mark.parametrize("xcd_chunk,min_class", [(-1, 0), (0, 1), (32, 2), (128, 0), (64, 1)])
+def test_launch_shapes_bit_exact(rdp_mi, xcd_chunk, min_class):
+    """Every raster launch shape the autotuner can pick gives the oracle's bytes:
+    batch streams of all three workloads plus random-state streams, submitted
+    together."""
+    streams = [rs.batch_stream(i, max_tris=800) for i in range(6)] + [rs.fuzz_stream(2000 + i) for i in range(6)]
+    say(f"[case] launch shape xcd_chunk={xcd_chunk} min_class={min_class}")
+    rdp_mi.set_launch_config(xcd_chunk, min_class)
+    try:
+        procs = _procs_for(rdp_mi, streams)
+        rdp_mi.batch_flush(procs)
+        for p, s in zip(procs, streams):
+            p.idle()
+            assert_same(_state(p), harness.run_oracle(s), f"{s.name} ({xcd_chunk}, {min_class})")
+            p.close()
+    finally:
+        rdp_mi.set_launch_config(64, 0)
+
+
+def test_autotuned_recording_bit_exact(rdp_mi):
+    streams = [rs.batch_stream(i) for i in (2, 5, 11, 23)]
+    refs = [harness.run_oracle(s) for s in streams]
+    say("[case] autotuned recording")
+    procs = _procs_for(rdp_mi, streams)
+    rec = rdp_mi.Recording(procs, batched=True)
+    tuned = rec.autotune(1)
+    say(f"[case] autotune picked {tuned}")
+    assert tuned["min_class"] in (0, 1, 2) and tuned["xcd_chunk"] in (-1, 0, 32, 64, 128)
+    rec.replay(2, use_graph=True)
+    for p, s, r in zip(procs, streams, refs):
+        assert_same(_state(p), r, f"{s.name} after autotune")
+    rec.close()
+    for p in procs:
+        p.close()
