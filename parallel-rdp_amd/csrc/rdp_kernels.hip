@@ -435,8 +435,18 @@ DEV int div_floor(int num, int den)
 	return -((-num + den - 1) / den);
 }
 
-DEV bool tmem_update_32(const Mem &m, const UploadInfo &u, int idx, bool upper, bool yuv, uint32_t &value)
+// Where the halfword a TMEM upload writes comes from: the planning half of
+// the gather (no memory access), so a thread can plan several uploads and
+// then issue their RDRAM reads together.
+enum : uint32_t { TMEM_LOAD_16 = 0, TMEM_LOAD_32 = 1 };
+struct TmemLoad {
+	uint32_t addr;
+	uint32_t kind;
+};
+
+DEV bool tmem_plan_32(const UploadInfo &u, int idx, TmemLoad &ld)
 {
+	const bool yuv = u.tmem_fmt == TF_YUV;
 	const int off16 = (u.tmem_offset & 0x7ff) >> 1;
 	const int stride = u.tmem_stride_words;
 	int pixel_offset = (idx - off16) & 0x3ff;
@@ -519,26 +529,12 @@ DEV bool tmem_update_32(const Mem &m, const UploadInfo &u, int idx, bool upper, 
 	}
 	if (ux >= u.width) return false;
 	int line_addr = u.vram_addr + ((uy * u.vram_width) << (u.vram_size - 1));
-	uint32_t addr = (uint32_t)(line_addr + iter_off + 4 * (ux & 1));
-	uint32_t word;
-	if ((addr & 3) == 0)
-		word = m.rd32i(addr >> 2);
-	else
-		word = (m.rd8(addr) << 24) | (m.rd8(addr + 1) << 16) | (m.rd8(addr + 2) << 8) | m.rd8(addr + 3);
-	if (yuv) {
-		if (upper)
-			word = (((word >> 16) & 0xff) << 8) | (word & 0xff);
-		else
-			word = (((word >> 24) & 0xff) << 8) | ((word >> 8) & 0xff);
-	} else {
-		word >>= 16 - 16 * (uint32_t)upper;
-		word &= 0xffff;
-	}
-	value = word;
+	ld.addr = (uint32_t)(line_addr + iter_off + 4 * (ux & 1));
+	ld.kind = TMEM_LOAD_32;
 	return true;
 }
 
-DEV bool tmem_update_16(const Mem &m, const UploadInfo &u, int idx, uint32_t &value)
+DEV bool tmem_plan_16(const UploadInfo &u, int idx, TmemLoad &ld)
 {
 	const int off16 = (u.tmem_offset & 0xfff) >> 1;
 	const int stride = u.tmem_stride_words;
@@ -601,11 +597,12 @@ DEV bool tmem_update_16(const Mem &m, const UploadInfo &u, int idx, uint32_t &va
 	if (ux >= u.width) return false;
 	int line_addr = u.vram_addr + ((uy * u.vram_width) << (u.vram_size - 1));
 	ux ^= ((uy & 1) << 1) | ux_xor;
-	value = m.rd16a((uint32_t)(line_addr + iter_off + 2 * (ux & 3)));
+	ld.addr = (uint32_t)(line_addr + iter_off + 2 * (ux & 3));
+	ld.kind = TMEM_LOAD_16;
 	return true;
 }
 
-DEV bool tmem_update_lut(const Mem &m, const UploadInfo &u, int idx, uint32_t &value)
+DEV bool tmem_plan_lut(const UploadInfo &u, int idx, TmemLoad &ld)
 {
 	const int off16 = (u.tmem_offset & 0xfff) >> 1;
 	const int p = (idx - off16) & 0x7ff;
@@ -634,8 +631,40 @@ DEV bool tmem_update_lut(const Mem &m, const UploadInfo &u, int idx, uint32_t &v
 	}
 	uint32_t addr = (uint32_t)(u.vram_addr + (splat << (u.vram_size - 1)));
 	addr += 2 * (addr & 1) * (uint32_t)(p & 3);
-	value = m.rd16a(addr);
+	ld.addr = addr;
+	ld.kind = TMEM_LOAD_16;
 	return true;
+}
+
+// The read half: the halfword value from RDRAM.  32-bit sources keep the
+// high halfword for the lower TMEM half and the low one for the upper half;
+// YUV splits luma and chroma bytes the same way.
+DEV uint32_t tmem_fetch(const Mem &m, const TmemLoad &ld, bool upper, bool yuv)
+{
+	if (ld.kind == TMEM_LOAD_16) return m.rd16a(ld.addr);
+	const uint32_t addr = ld.addr;
+	uint32_t word;
+	if ((addr & 3) == 0)
+		word = m.rd32i(addr >> 2);
+	else
+		word = (m.rd8(addr) << 24) | (m.rd8(addr + 1) << 16) | (m.rd8(addr + 2) << 8) | m.rd8(addr + 3);
+	if (yuv) {
+		if (upper)
+			word = (((word >> 16) & 0xff) << 8) | (word & 0xff);
+		else
+			word = (((word >> 24) & 0xff) << 8) | ((word >> 8) & 0xff);
+	} else {
+		word >>= 16 - 16 * (uint32_t)upper;
+		word &= 0xffff;
+	}
+	return word;
+}
+
+DEV bool tmem_plan(const UploadInfo &u, int idx, bool upper, TmemLoad &ld)
+{
+	if (u.mode == UP_TLUT) return tmem_plan_lut(u, idx, ld);
+	if (u.tmem_size == 3 || u.tmem_fmt == TF_YUV) return tmem_plan_32(u, idx & 0x3ff, ld);
+	return tmem_plan_16(u, idx, ld);
 }
 
 __global__ __launch_bounds__(256) void k_tmem(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
@@ -649,21 +678,34 @@ __global__ __launch_bounds__(256) void k_tmem(const PassDesc *__restrict__ passe
 	const bool upper = idx >= 0x400;
 	P.tmem_instances[g] = (uint16_t)cur;
 	const int n = (int)P.num_uploads;
-	for (int i = 0; i < n; i++) {
-		const UploadInfo u = P.uploads[i];
-		uint32_t v;
-		bool written;
-		if (u.mode == UP_TLUT)
-			written = tmem_update_lut(m, u, idx, v);
-		else {
-			bool yuv = u.tmem_fmt == TF_YUV;
-			if (u.tmem_size == 3 || yuv)
-				written = tmem_update_32(m, u, idx & 0x3ff, upper, yuv, v);
-			else
-				written = tmem_update_16(m, u, idx, v);
+	// Uploads in groups of four: plan all four, issue their RDRAM reads
+	// together, then apply them in order, so the chain of uploads waits for
+	// memory once per group rather than once per upload.
+	constexpr int K = 4;
+	for (int i0 = 0; i0 < n; i0 += K) {
+		uint32_t v[K];
+		bool written[K];
+		bool yuv[K];
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			written[k] = false;
+			yuv[k] = false;
+			v[k] = 0;
+			if (i0 + k < n) {
+				const UploadInfo u = P.uploads[i0 + k];
+				TmemLoad ld;
+				yuv[k] = u.mode != UP_TLUT && u.tmem_fmt == TF_YUV;
+				written[k] = tmem_plan(u, idx, upper, ld);
+				if (written[k]) v[k] = tmem_fetch(m, ld, upper, yuv[k]);
+			}
 		}
-		if (written) cur = v;
-		P.tmem_instances[(i + 1) * 2048 + g] = (uint16_t)cur;
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			if (i0 + k < n) {
+				if (written[k]) cur = v[k];
+				P.tmem_instances[(i0 + k + 1) * 2048 + g] = (uint16_t)cur;
+			}
+		}
 	}
 	P.tmem[g] = (uint16_t)cur;
 }

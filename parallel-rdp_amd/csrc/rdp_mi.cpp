@@ -131,6 +131,10 @@ struct PinnedBuffer {
 // staging buffer, which waits for the previous upload).
 struct Lane {
 	hipStream_t stream = nullptr;
+	// TMEM replay of a round runs on `side`, beside that round's setup and
+	// binning on `stream` (fork/join events per round).
+	hipStream_t side = nullptr;
+	std::vector<hipEvent_t> fork, join;
 	DeviceBuffer input, scratch;
 	PinnedBuffer staging;
 	size_t last_arena_bytes = 0;  // packed inputs + scratch of the last submission
@@ -140,16 +144,32 @@ struct Lane {
 	int create()
 	{
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+		HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
 		HIP_CHECK(hipEventCreateWithFlags(&staging_done, hipEventDisableTiming));
+		return 0;
+	}
+	int ensure_round_events(size_t rounds)
+	{
+		while (fork.size() < rounds) {
+			hipEvent_t a, b;
+			HIP_CHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+			fork.push_back(a);
+			join.push_back(b);
+		}
 		return 0;
 	}
 	void release()
 	{
 		if (stream) (void)hipStreamSynchronize(stream);
+		if (side) (void)hipStreamSynchronize(side);
 		if (input.ptr) (void)hipFree(input.ptr);
 		if (scratch.ptr) (void)hipFree(scratch.ptr);
 		if (staging.ptr) (void)hipHostFree(staging.ptr);
 		if (staging_done) (void)hipEventDestroy(staging_done);
+		for (auto e : fork) (void)hipEventDestroy(e);
+		for (auto e : join) (void)hipEventDestroy(e);
+		if (side) (void)hipStreamDestroy(side);
 		if (stream) (void)hipStreamDestroy(stream);
 		*this = Lane();
 	}
@@ -261,17 +281,38 @@ static uint64_t framebuffer_bytes(const HostPass &hp)
 	return hp.prims.empty() ? 0 : 2 * tiles * 64 * (color + depth);
 }
 
-static int execute_launches(const std::vector<Launch> &launches, hipStream_t s, const LaunchConfig &cfg,
+// Launches every round on the lane's stream.  The TMEM replay of a round
+// depends only on RDRAM as the previous round left it and writes only its own
+// snapshots, so in rounds of many passes it forks to the side stream at the
+// round's start and joins before the raster launch.  Rounds of a few passes
+// (the plain path, small serving batches) stay in series: there the fork and
+// join cost more than they hide.  Stage timing (stage_events) needs the
+// stages in series too.
+constexpr uint32_t kOverlapMinPasses = 8;
+
+static int execute_launches(const std::vector<Launch> &launches, Lane &lane, const LaunchConfig &cfg,
                             hipEvent_t *stage_events = nullptr)
 {
+	hipStream_t s = lane.stream;
+	const bool overlap = !stage_events && lane.side;
+	if (overlap)
+		if (int err = lane.ensure_round_events(launches.size())) return err;
 	for (size_t r = 0; r < launches.size(); r++) {
 		const Launch &L = launches[r];
+		const bool fork = overlap && L.n[1] && L.passes >= kOverlapMinPasses;
+		if (fork) {
+			HIP_CHECK(hipEventRecord(lane.fork[r], s));
+			HIP_CHECK(hipStreamWaitEvent(lane.side, lane.fork[r], 0));
+			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], lane.side));
+			HIP_CHECK(hipEventRecord(lane.join[r], lane.side));
+		}
 		HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
-		HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], s));
+		if (!fork) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
 		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
+		if (fork) HIP_CHECK(hipStreamWaitEvent(s, lane.join[r], 0));
 		// Class segments are contiguous in class order, so promoting the
 		// classes below min_class merges them into one launch.
 		const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
@@ -472,7 +513,7 @@ static int submit_rounds(Lane &c, const std::vector<std::vector<PassRef>> &round
 	std::vector<Launch> launches;
 	int err = prepare_rounds(c, rounds, c.input, c.scratch, launches);
 	if (err) return err;
-	return execute_launches(launches, c.stream, ctx().cfg);
+	return execute_launches(launches, c, ctx().cfg);
 }
 
 static void build_round_list(Processor *const *procs, unsigned count, bool batched,
@@ -1184,7 +1225,7 @@ static int ensure_graph(Context &c, Recording &r)
 {
 	if (r.exec) return 0;
 	HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-	int err = execute_launches(r.launches, c.stream, r.cfg);
+	int err = execute_launches(r.launches, c, r.cfg);
 	hipGraph_t graph = nullptr;
 	hipError_t e = hipStreamEndCapture(c.stream, &graph);
 	if (err) return err;
@@ -1213,7 +1254,7 @@ int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_gr
 			hipError_t e = hipGraphLaunch(r.exec, c.stream);
 			if (e != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
 		} else
-			err = execute_launches(r.launches, c.stream, r.cfg);
+			err = execute_launches(r.launches, c, r.cfg);
 		if (!err && !ev.empty()) HIP_CHECK(hipEventRecord(ev[i + 1], c.stream));
 	}
 	hipError_t se = hipStreamSynchronize(c.stream);
@@ -1239,13 +1280,13 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	hipEvent_t e0, e1;
 	HIP_CHECK(hipEventCreate(&e0));
 	HIP_CHECK(hipEventCreate(&e1));
-	int err = execute_launches(r.launches, c.stream, r.cfg);  // warm caches and code
+	int err = execute_launches(r.launches, c, r.cfg);  // warm caches and code
 	// Two sweeps over the candidates, best of each candidate's timed runs.
 	for (int sweep = 0; sweep < 2 && !err; sweep++)
 		for (size_t i = 0; i < cands.size() && !err; i++) {
-			if ((err = execute_launches(r.launches, c.stream, cands[i]))) break;
+			if ((err = execute_launches(r.launches, c, cands[i]))) break;
 			HIP_CHECK(hipEventRecord(e0, c.stream));
-			for (unsigned it = 0; it < iterations && !err; it++) err = execute_launches(r.launches, c.stream, cands[i]);
+			for (unsigned it = 0; it < iterations && !err; it++) err = execute_launches(r.launches, c, cands[i]);
 			HIP_CHECK(hipEventRecord(e1, c.stream));
 			HIP_CHECK(hipEventSynchronize(e1));
 			float ms = 0;
@@ -1291,7 +1332,7 @@ int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, fl
 	int err = 0;
 	for (unsigned it = 0; it < iterations && !err; it++) {
 		HIP_CHECK(hipEventRecord(ev[0], c.stream));
-		err = execute_launches(r.launches, c.stream, r.cfg, ev.data() + 1);
+		err = execute_launches(r.launches, c, r.cfg, ev.data() + 1);
 		HIP_CHECK(hipStreamSynchronize(c.stream));
 		for (size_t k = 0; k < nev && !err; k++) {
 			float ms = 0;
@@ -1317,7 +1358,7 @@ int rdp_mi_recording_profile(rdp_mi_recording *h, unsigned iterations, float sta
 	int err = 0;
 	for (unsigned it = 0; it < iterations && !err; it++) {
 		HIP_CHECK(hipEventRecord(ev[0], c.stream));
-		err = execute_launches(r.launches, c.stream, r.cfg, ev.data() + 1);
+		err = execute_launches(r.launches, c, r.cfg, ev.data() + 1);
 		HIP_CHECK(hipStreamSynchronize(c.stream));
 		for (size_t k = 0; k < nev && !err; k++) {
 			float ms = 0;
