@@ -310,7 +310,7 @@ void Frontend::split_at_last_mark()
 	head.fb_height = m.fb_height;
 	head.num_spans = m.spans;
 	head.num_tilerows = m.tilerows;
-	for (const PrimRecord &r : head.prims) head.raster_class = std::max(head.raster_class, r.raster_class);
+	for (const PrimRecord &r : head.prims) head.raster_class = std::max<uint32_t>(head.raster_class, r.raster_class);
 	if (!head.empty()) done.push_back(std::move(head));
 
 	HostPass tail;
@@ -328,11 +328,11 @@ void Frontend::split_at_last_mark()
 		color_write_pending = true;
 		r.state_index = (uint16_t)remap[r.state_index];
 		r.word_offset -= m.words;
-		r.tmem_instance = (uint8_t)(r.tmem_instance - m.uploads);
+		r.tmem_instance = (uint16_t)(r.tmem_instance - m.uploads);
 		r.span_offset -= m.spans;
 		r.tilerow_offset -= m.tilerows;
 		tail.fb_height = std::max(tail.fb_height, (uint32_t)(r.yhi + 1));
-		tail.raster_class = std::max(tail.raster_class, r.raster_class);
+		tail.raster_class = std::max<uint32_t>(tail.raster_class, r.raster_class);
 		tail.prims.push_back(r);
 	}
 	tail.num_spans = cur.num_spans - m.spans;
@@ -547,14 +547,14 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 		memset(&r, 0, sizeof(r));
 		r.word_offset = (uint32_t)cur.words.size();
 		r.state_index = (uint16_t)(cur.states.size() - 1);
-		r.tmem_instance = (uint8_t)cur.uploads.size();
+		r.tmem_instance = (uint16_t)cur.uploads.size();
 		r.opcode = (uint8_t)op;
 		r.ylo = ylo;
 		r.yhi = yhi;
 		r.span_offset = cur.num_spans;
 		r.tilerow_offset = cur.num_tilerows;
 		r.prim_index = prim_index;
-		r.raster_class = class_value;
+		r.raster_class = (uint8_t)class_value;
 		cur.raster_class = std::max(cur.raster_class, class_value);
 		cur.words.insert(cur.words.end(), w, w + len);
 		cur.prims.push_back(r);

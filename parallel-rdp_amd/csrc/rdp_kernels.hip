@@ -337,8 +337,8 @@ DEV void compute_span(const PrimSetup &s, const uint32_t sc[4], int y, SpanSetup
 __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
-	const PassDesc &P = passes[item >> 12];
-	const uint32_t prim = item & 0xfff;
+	const PassDesc &P = passes[item >> 13];
+	const uint32_t prim = item & 0x1fff;
 	const PrimRecord rec = P.prims[prim];
 	const DrawState &st = P.states[rec.state_index];
 	const int lane = threadIdx.x;
@@ -716,8 +716,8 @@ __global__ __launch_bounds__(256) void k_tmem(const PassDesc *__restrict__ passe
 __global__ __launch_bounds__(64) void k_bin(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
-	const PassDesc &P = passes[item >> 13];
-	const uint32_t ty = (item >> 6) & 0x7f, group = item & 0x3f;
+	const PassDesc &P = passes[item >> 14];
+	const uint32_t ty = (item >> 7) & 0x7f, group = item & 0x7f;
 	const int lane = threadIdx.x;
 	const uint32_t prim = group * 64 + lane;
 	uint32_t lo = 0xffff, hi = 0;
@@ -2150,46 +2150,49 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 
 	const uint32_t G = P.num_groups;
 	const uint32_t tile = ty * P.tiles_x + tx;
-	uint64_t my_mask = (uint32_t)lane < G ? gptr(P.bin_fine)[(size_t)tile * G + lane] : 0;
-	uint64_t coarse = __ballot(my_mask != 0);
 	uint32_t staged = 0xffffffffu;
 	const uint32_t needs_tmem = RAS_USES_TEXEL0 | RAS_USES_TEXEL1 | RAS_USES_PIPELINED_TEXEL1 | RAS_COPY;
-	while (coarse) {
-		const int g = __ffsll((unsigned long long)coarse) - 1;
-		coarse &= coarse - 1;
-		const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)my_mask, g);
-		const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(my_mask >> 32), g);
-		uint64_t mask = ((uint64_t)hi << 32) | lo;
-		while (mask) {
-			const int bit = __ffsll((unsigned long long)mask) - 1;
-			mask &= mask - 1;
-			const AS_CONST RasterPrim *ps = rprims + (g * 64 + bit);
+	// Lane l holds the mask of group gbase + l; up to two rounds of 64 groups.
+	for (uint32_t gbase = 0; gbase < G; gbase += 64) {
+		uint64_t my_mask = gbase + (uint32_t)lane < G ? gptr(P.bin_fine)[(size_t)tile * G + gbase + lane] : 0;
+		uint64_t coarse = __ballot(my_mask != 0);
+		while (coarse) {
+			const int g = __ffsll((unsigned long long)coarse) - 1;
+			coarse &= coarse - 1;
+			const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)my_mask, g);
+			const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(my_mask >> 32), g);
+			uint64_t mask = ((uint64_t)hi << 32) | lo;
+			while (mask) {
+				const int bit = __ffsll((unsigned long long)mask) - 1;
+				mask &= mask - 1;
+				const AS_CONST RasterPrim *ps = rprims + ((gbase + (uint32_t)g) * 64 + (uint32_t)bit);
 
-			// Stage the primitive's TMEM snapshot in LDS (wave-uniform).
-			if ((ps->ras_flags & needs_tmem) && ps->tmem_instance != staged) {
-				__syncthreads();
-				const AS_GLOBAL uint32_t *src = tmem_src + (size_t)ps->tmem_instance * 1024;
-				uint32_t *dst = reinterpret_cast<uint32_t *>(s_tmem);
+				// Stage the primitive's TMEM snapshot in LDS (wave-uniform).
+				if ((ps->ras_flags & needs_tmem) && ps->tmem_instance != staged) {
+					__syncthreads();
+					const AS_GLOBAL uint32_t *src = tmem_src + (size_t)ps->tmem_instance * 1024;
+					uint32_t *dst = reinterpret_cast<uint32_t *>(s_tmem);
 #pragma unroll
-				for (int k = 0; k < 16; k++) dst[lane * 4 + (k & 3) + 256 * (k >> 2)] = src[lane * 4 + (k & 3) + 256 * (k >> 2)];
-				__syncthreads();
-				staged = ps->tmem_instance;
-			}
+					for (int k = 0; k < 16; k++) dst[lane * 4 + (k & 3) + 256 * (k >> 2)] = src[lane * 4 + (k & 3) + 256 * (k >> 2)];
+					__syncthreads();
+					staged = ps->tmem_instance;
+				}
 
-			if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
-			switch (ps->variant) {
+				if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
+				switch (ps->variant) {
 #define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt, cls)                                                 \
-	case id:                                                                                                    \
-		if constexpr ((cls) <= CLS)                                                                             \
-			raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt)>>(ps, states, spans, tex, lut, fmt, alias, ci, \
-			                                                                    x, y, px);                        \
-		break;
-				RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
+		case id:                                                                                                    \
+			if constexpr ((cls) <= CLS)                                                                             \
+				raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt)>>(ps, states, spans, tex, lut, fmt, alias, ci, \
+				                                                                    x, y, px);                        \
+			break;
+					RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
 #undef RDP_DISPATCH_VARIANT
-			default:
-				if constexpr (CLS == kRasterClasses - 1)
-					raster_prim<GenericVariant>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px);
-				break;
+				default:
+					if constexpr (CLS == kRasterClasses - 1)
+						raster_prim<GenericVariant>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px);
+					break;
+				}
 			}
 		}
 	}

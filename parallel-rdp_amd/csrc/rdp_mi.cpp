@@ -487,11 +487,11 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.num_groups = ((uint32_t)hp.prims.size() + 63) / 64;
 
 			const uint32_t pidx = (uint32_t)i;
-			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 12) | p;
+			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 13) | p;
 			for (uint32_t k = 0; k < 8; k++) it_tmem[nt++] = (pidx << 12) | k;
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)
-					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 13) | (y << 6) | g;
+					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 14) | (y << 7) | g;
 				const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
 				uint32_t *out = it_raster + rbase[cls];
 				for (uint32_t y = 0; y < D.tiles_y; y++)

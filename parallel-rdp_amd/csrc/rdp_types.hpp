@@ -79,9 +79,9 @@ enum : uint32_t { UP_TILE = 0, UP_TLUT = 1, UP_BLOCK = 2 };
 enum : uint32_t { TILE_CLAMP_S = 1, TILE_MIRROR_S = 2, TILE_CLAMP_T = 4, TILE_MIRROR_T = 8 };
 
 constexpr int kTileSize = 8;                 // 8x8 pixel screen tile = one wave64
-constexpr int kMaxPrimsPerPass = 4096;       // 64 bin groups x 64 primitives
+constexpr int kMaxPrimsPerPass = 8192;       // 128 bin groups x 64 primitives
 constexpr int kBinGroups = kMaxPrimsPerPass / 64;
-constexpr int kMaxUploadsPerPass = 255;      // TMEM snapshots per pass (u8 index)
+constexpr int kMaxUploadsPerPass = 1023;     // TMEM snapshots per pass (4 KiB each)
 constexpr int kMaxWidth = 1024;
 constexpr int kMaxHeight = 1024;
 constexpr int kMaxTilesX = kMaxWidth / kTileSize;
@@ -236,13 +236,14 @@ struct DrawState {
 struct PrimRecord {
 	uint32_t word_offset;   // into the pass word buffer
 	uint16_t state_index;   // into the pass draw-state table
-	uint8_t tmem_instance;  // TMEM snapshot index (0 = state at pass start)
-	uint8_t opcode;
+	uint16_t tmem_instance; // TMEM snapshot index (0 = state at pass start)
 	int32_t ylo, yhi;       // active scanline range (yhi < ylo: nothing to draw)
 	uint32_t span_offset;   // first SpanSetup record (lines ylo .. yhi+1)
 	uint32_t tilerow_offset;// first tile-row range record (rows ylo>>3 .. yhi>>3)
 	uint32_t prim_index;    // stream-global primitive counter (noise seed)
-	uint32_t raster_class;  // occupancy class of the primitive's raster variant
+	uint8_t opcode;
+	uint8_t raster_class;   // occupancy class of the primitive's raster variant
+	uint16_t pad;
 };
 static_assert(sizeof(PrimRecord) == 32, "PrimRecord layout");
 
