@@ -142,6 +142,10 @@ constexpr uint32_t RAS_TEX0_BILINEAR = RAS_USES_TEXEL0 | RAS_PERSPECTIVE | RAS_S
 constexpr uint32_t RAS_2CYC_LOD_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_USES_TEXEL1 | RAS_USES_LOD |
                                        RAS_PERSPECTIVE | RAS_SAMPLE_QUAD | RAS_BILERP0 | RAS_BILERP1 | RAS_TLUT |
                                        RAS_TEX_LOD;
+// 2 cycles, texel 0 only, bilinear with an RGBA16 TLUT, no perspective or LOD
+// (sprites drawn as texture rectangles).
+constexpr uint32_t RAS_2CYC_SPRITE_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_SAMPLE_QUAD | RAS_BILERP0 |
+                                          RAS_BILERP1 | RAS_TLUT;
 
 // X(id, ras_known, ras_value, combiner, texture format, occupancy class).
 // The raster kernel is built once per occupancy class with only the variants
@@ -160,6 +164,8 @@ constexpr uint32_t RAS_2CYC_LOD_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_U
 	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY, 1)                                                \
 	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI8, 1)                               \
 	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI4, 1)                              \
+	X(12, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_ANY, TEXFMT_CI8, 1)                           \
+	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_ANY, TEXFMT_CI4, 1)                           \
 	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, 2)
 constexpr uint32_t kRasterClasses = 3;  // the generic variant (id 0) is in the top class
 
