@@ -226,6 +226,7 @@ def main():
     stage_ms = rec.profile(3)
     raster_bytes = rec.raster_bytes
     rounds = rec.rounds
+    raster_launches = rec.raster_launches
     rec.close()
     for p in procs:
         p.close()
@@ -252,8 +253,10 @@ def main():
     value = exact / (ms_per_step / 1e3)
     plain_value = plain_exact / (float(plain_ms.mean()) / 1e3) if plain_ms is not None else 0.0
 
-    raster_ms_per_launch = float(stage_ms[3]) / max(rounds, 1)
-    bytes_per_launch = raster_bytes / max(rounds, 1)
+    # k_raster is launched once per occupancy class present in a round; the
+    # per-launch figures average over those launches.
+    raster_ms_per_launch = float(stage_ms[3]) / max(raster_launches, 1)
+    bytes_per_launch = raster_bytes / max(raster_launches, 1)
     achieved = bytes_per_launch / (raster_ms_per_launch / 1e3) / 1e9 if raster_ms_per_launch > 0 else 0.0
 
     # HBM bytes per k_raster launch from the committed rocprof counter runs
@@ -310,7 +313,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_raster", "bytes_per_launch": int(bytes_per_launch),
-                     "ms_per_launch": round(raster_ms_per_launch, 4), "valu_issue": valu},
+                     "ms_per_launch": round(raster_ms_per_launch, 4), "launches_per_step": raster_launches,
+                     "valu_issue": valu},
         "cpu_baseline": cpu,
         "serving": serve,
     }

@@ -187,6 +187,9 @@ void rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class);
 uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_rounds(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *r);
+/* Raster kernel launches per replay (one per occupancy class present in a
+ * round, after the launch shape's class floor). */
+unsigned rdp_mi_recording_raster_launches(const rdp_mi_recording *r);
 void rdp_mi_recording_destroy(rdp_mi_recording *r);
 
 /* Serving (no reference counterpart: the reference serves one emulator per

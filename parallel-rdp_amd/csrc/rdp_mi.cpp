@@ -125,16 +125,19 @@ struct PinnedBuffer {
 	}
 };
 
+struct RoundEvents {
+	hipEvent_t fork = nullptr, tmem_done = nullptr, bin_done = nullptr, join = nullptr;
+};
+
 // A submission lane: a stream with its own pinned staging buffer and device
 // input/scratch arenas.  Work on one lane is stream ordered, so the arenas
 // are reused by consecutive submissions without host waits (except for the
 // staging buffer, which waits for the previous upload).
 struct Lane {
 	hipStream_t stream = nullptr;
-	// TMEM replay of a round runs on `side`, beside that round's setup and
-	// binning on `stream` (fork/join events per round).
+	// Parts of a round run on `side` beside `stream` (events per round).
 	hipStream_t side = nullptr;
-	std::vector<hipEvent_t> fork, join;
+	std::vector<RoundEvents> events;
 	DeviceBuffer input, scratch;
 	PinnedBuffer staging;
 	size_t last_arena_bytes = 0;  // packed inputs + scratch of the last submission
@@ -150,12 +153,13 @@ struct Lane {
 	}
 	int ensure_round_events(size_t rounds)
 	{
-		while (fork.size() < rounds) {
-			hipEvent_t a, b;
-			HIP_CHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-			HIP_CHECK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
-			fork.push_back(a);
-			join.push_back(b);
+		while (events.size() < rounds) {
+			RoundEvents e;
+			HIP_CHECK(hipEventCreateWithFlags(&e.fork, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&e.tmem_done, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&e.bin_done, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&e.join, hipEventDisableTiming));
+			events.push_back(e);
 		}
 		return 0;
 	}
@@ -167,8 +171,8 @@ struct Lane {
 		if (scratch.ptr) (void)hipFree(scratch.ptr);
 		if (staging.ptr) (void)hipHostFree(staging.ptr);
 		if (staging_done) (void)hipEventDestroy(staging_done);
-		for (auto e : fork) (void)hipEventDestroy(e);
-		for (auto e : join) (void)hipEventDestroy(e);
+		for (auto &e : events)
+			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join }) (void)hipEventDestroy(ev);
 		if (side) (void)hipStreamDestroy(side);
 		if (stream) (void)hipStreamDestroy(stream);
 		*this = Lane();
@@ -281,46 +285,72 @@ static uint64_t framebuffer_bytes(const HostPass &hp)
 	return hp.prims.empty() ? 0 : 2 * tiles * 64 * (color + depth);
 }
 
-// Launches every round on the lane's stream.  The TMEM replay of a round
-// depends only on RDRAM as the previous round left it and writes only its own
-// snapshots, so in rounds of many passes it forks to the side stream at the
-// round's start and joins before the raster launch.  Rounds of a few passes
-// (the plain path, small serving batches) stay in series: there the fork and
-// join cost more than they hide.  Stage timing (stage_events) needs the
-// stages in series too.
+// Launches every round on the lane's stream.  In rounds of many passes two
+// parts run on the lane's side stream:
+//   - the TMEM replay, which depends only on RDRAM as the previous round left
+//     it and writes only its own snapshots, forks at the round's start and
+//     runs beside setup and binning;
+//   - the raster launches of the heavier occupancy classes, which touch other
+//     passes than the light class's launch, run beside it once binning is
+//     done, so the light launch fills the GPU while the heavy one's long
+//     per-tile chains finish.
+// The round joins both back before the next round reuses the scratch arena.
+// Rounds of a few passes (the plain path, small serving batches) stay in
+// series, where the events cost more than they hide; stage timing
+// (stage_events) needs the stages in series too.
 constexpr uint32_t kOverlapMinPasses = 8;
 
 static int execute_launches(const std::vector<Launch> &launches, Lane &lane, const LaunchConfig &cfg,
                             hipEvent_t *stage_events = nullptr)
 {
-	hipStream_t s = lane.stream;
-	const bool overlap = !stage_events && lane.side;
+	hipStream_t s = lane.stream, side = lane.side;
+	const bool overlap = !stage_events && side;
 	if (overlap)
 		if (int err = lane.ensure_round_events(launches.size())) return err;
 	for (size_t r = 0; r < launches.size(); r++) {
 		const Launch &L = launches[r];
-		const bool fork = overlap && L.n[1] && L.passes >= kOverlapMinPasses;
-		if (fork) {
-			HIP_CHECK(hipEventRecord(lane.fork[r], s));
-			HIP_CHECK(hipStreamWaitEvent(lane.side, lane.fork[r], 0));
-			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], lane.side));
-			HIP_CHECK(hipEventRecord(lane.join[r], lane.side));
+		const bool big = overlap && L.passes >= kOverlapMinPasses;
+		const bool fork_tmem = big && L.n[1];
+		// Raster launches after the class floor: the merged light launch and
+		// the heavier classes.
+		const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
+		uint32_t merged = 0, heavy = 0;
+		for (uint32_t c = 0; c <= lo; c++) merged += L.raster_n[c];
+		for (uint32_t c = lo + 1; c < kRasterClasses; c++) heavy += L.raster_n[c];
+		const bool fork_raster = big && merged && heavy;
+		const RoundEvents E = overlap ? lane.events[r] : RoundEvents();
+
+		if (fork_tmem) {
+			HIP_CHECK(hipEventRecord(E.fork, s));
+			HIP_CHECK(hipStreamWaitEvent(side, E.fork, 0));
+			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], side));
+			HIP_CHECK(hipEventRecord(E.tmem_done, side));
 		}
 		HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
-		if (!fork) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], s));
+		if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
 		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
-		if (fork) HIP_CHECK(hipStreamWaitEvent(s, lane.join[r], 0));
-		// Class segments are contiguous in class order, so promoting the
-		// classes below min_class merges them into one launch.
-		const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
-		uint32_t merged = 0;
-		for (uint32_t c = 0; c <= lo; c++) merged += L.raster_n[c];
-		HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
-		for (uint32_t c = lo + 1; c < kRasterClasses; c++)
-			HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, s));
+		if (fork_raster) {
+			// The side stream already holds the TMEM replay when it forked;
+			// otherwise TMEM ran on the main stream and bin_done covers it.
+			HIP_CHECK(hipEventRecord(E.bin_done, s));
+			HIP_CHECK(hipStreamWaitEvent(side, E.bin_done, 0));
+			for (uint32_t c = lo + 1; c < kRasterClasses; c++)
+				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, side));
+			HIP_CHECK(hipEventRecord(E.join, side));
+			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
+			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
+			HIP_CHECK(hipStreamWaitEvent(s, E.join, 0));
+		} else {
+			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
+			// Class segments are contiguous in class order, so promoting the
+			// classes below min_class merges them into one launch.
+			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
+			for (uint32_t c = lo + 1; c < kRasterClasses; c++)
+				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, s));
+		}
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
 	}
 	return 0;
@@ -1376,6 +1406,21 @@ uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *h) { return h ? h
 unsigned rdp_mi_recording_rounds(const rdp_mi_recording *h) { return h ? (unsigned)h->r.launches.size() : 0; }
 
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *h) { return h ? h->r.passes : 0; }
+
+unsigned rdp_mi_recording_raster_launches(const rdp_mi_recording *h)
+{
+	if (!h) return 0;
+	const LaunchConfig &cfg = h->r.cfg;
+	const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
+	unsigned n = 0;
+	for (const Launch &L : h->r.launches) {
+		uint32_t merged = 0;
+		for (uint32_t c = 0; c <= lo; c++) merged += L.raster_n[c];
+		n += merged != 0;
+		for (uint32_t c = lo + 1; c < kRasterClasses; c++) n += L.raster_n[c] != 0;
+	}
+	return n;
+}
 
 int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t hbm_budget_bytes, unsigned max_batch,
                          unsigned decode_threads, unsigned streams, rdp_mi_server **out)

@@ -325,6 +325,12 @@ class Recording:
         return int(self._lib.rdp_mi_recording_rounds(self._h))
 
     @property
+    def raster_launches(self) -> int:
+        self._lib.rdp_mi_recording_raster_launches.restype = ctypes.c_uint
+        self._lib.rdp_mi_recording_raster_launches.argtypes = [ctypes.c_void_p]
+        return int(self._lib.rdp_mi_recording_raster_launches(self._h))
+
+    @property
     def passes(self) -> int:
         return int(self._lib.rdp_mi_recording_passes(self._h))
 
