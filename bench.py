@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison (profiling runs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape (profiling runs: the tuner's trial launches stay out)")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--peak-frames-per-instance", type=int, default=8)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
@@ -212,8 +214,11 @@ def main():
     enqueue_all(procs, streams)
     rec = rdp_mi.Recording(procs, batched=True)
     log(f"[bench] batched recording: {rec.rounds} rounds, {rec.passes} passes")
-    tuned = rec.autotune(3)
-    log(f"[bench] autotuned launch shape: {tuned}")
+    if args.no_autotune:
+        tuned = {"xcd_chunk": 64, "min_class": 0, "ms": None}
+    else:
+        tuned = rec.autotune(3)
+    log(f"[bench] launch shape: {tuned}")
     # Immediate submissions of the same workload (plain path, serving) use it too.
     rdp_mi.set_launch_config(tuned["xcd_chunk"], tuned["min_class"])
     rec.replay(max(args.warmup, 1), use_graph=True)

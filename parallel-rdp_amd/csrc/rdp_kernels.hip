@@ -2068,7 +2068,7 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *st
 #define RDP_RASTER_WAVES_CLASS0 8
 #endif
 #ifndef RDP_RASTER_WAVES_CLASS1
-#define RDP_RASTER_WAVES_CLASS1 5
+#define RDP_RASTER_WAVES_CLASS1 8
 #endif
 #ifndef RDP_RASTER_WAVES_CLASS2
 #define RDP_RASTER_WAVES_CLASS2 6

@@ -4,7 +4,9 @@
     python parallel-rdp_amd/rocprof_summary.py pmc    <pmc .db> [kernel-substring]
 
 `stats` prints, per kernel, the call count and total/average duration, then
-for kernels matching the substring a breakdown by launch grid size.  `pmc`
+for kernels matching the substring a breakdown by launch grid size and the
+averages over launches that ran alone (no other kernel overlapping in time),
+which are the ones comparable with per-stage event timing.  `pmc`
 prints, per kernel and counter, the number of dispatches, the sum over
 dispatches and the per-dispatch average (counter values are first summed over
 the per-instance rows of one dispatch).  FETCH_SIZE and WRITE_SIZE are in KiB.
@@ -31,6 +33,22 @@ def stats(path: str, kernel_filter: str = ""):
     for short, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
         print(f"{short[:48]:48s} {len(d):7d} {sum(d) / 1e6:10.3f} {sum(d) / len(d) / 1e3:10.2f} "
               f"{100.0 * sum(d) / total_all:6.2f}")
+    if kernel_filter:
+        # Launches that overlap no other kernel in time.
+        iv = sorted((start, end, name.split("(")[0]) for name, start, end, _ in rows)
+        alone = collections.defaultdict(list)
+        reach = -1
+        for i, (start, end, short) in enumerate(iv):
+            nxt = iv[i + 1][0] if i + 1 < len(iv) else None
+            if start >= reach and (nxt is None or nxt >= end) and kernel_filter in short:
+                alone[short].append(end - start)
+            reach = max(reach, end)
+        if alone:
+            print(f"\n{'kernel (launches running alone)':48s} {'calls':>7s} {'avg_us':>10s}")
+            for short, d in sorted(alone.items()):
+                print(f"{short[:48]:48s} {len(d):7d} {sum(d) / len(d) / 1e3:10.2f}")
+            allv = [x for d in alone.values() for x in d]
+            print(f"{'all matching kernels':48s} {len(allv):7d} {sum(allv) / len(allv) / 1e3:10.2f}")
     if grids:
         print(f"\n{'kernel':32s} {'grid_x (threads)':>16s} {'calls':>7s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s}")
         for (short, grid), d in sorted(grids.items(), key=lambda kv: (kv[0][0], -kv[0][1])):
