@@ -522,11 +522,27 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)
 					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 14) | (y << 7) | g;
-				const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
-				uint32_t *out = it_raster + rbase[cls];
-				for (uint32_t y = 0; y < D.tiles_y; y++)
-					for (uint32_t x = 0; x < D.tiles_x; x++) out[nr[cls]++] = (pidx << 14) | (x << 7) | y;
 			}
+		}
+		// Raster items, per class, passes with the most primitives per tile
+		// first: their tiles have the longest chains, so they start early
+		// and the short tiles of light passes fill the end of the launch.
+		std::vector<uint32_t> order(rounds[r].size());
+		for (uint32_t i = 0; i < (uint32_t)order.size(); i++) order[i] = i;
+		auto density = [&](uint32_t i) {
+			const HostPass &hp = *rounds[r][i].pass;
+			const uint64_t tiles = (uint64_t)((hp.fb_width + 7) / 8) * ((hp.fb_height + 7) / 8);
+			return tiles ? (double)hp.prims.size() / (double)tiles : 0.0;
+		};
+		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return density(a) > density(b); });
+		for (uint32_t pidx : order) {
+			const HostPass &hp = *rounds[r][pidx].pass;
+			if (hp.prims.empty()) continue;
+			const uint32_t tiles_x = (hp.fb_width + 7) / 8, tiles_y = (hp.fb_height + 7) / 8;
+			const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
+			uint32_t *out = it_raster + rbase[cls];
+			for (uint32_t y = 0; y < tiles_y; y++)
+				for (uint32_t x = 0; x < tiles_x; x++) out[nr[cls]++] = (pidx << 14) | (x << 7) | y;
 		}
 	}
 
