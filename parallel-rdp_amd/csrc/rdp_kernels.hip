@@ -6,8 +6,10 @@
 //             pass draw state, writes PrimSetup, walks the edges for every
 //             covered scanline (SpanSetup) and reduces the per-8-line x extent
 //             into tile-row ranges.
-//   k_tmem    one lane per TMEM halfword: replays the pass's uploads in order
-//             and writes a TMEM snapshot after each one.
+//   k_tmem_gather / k_tmem_scan
+//             one lane per (upload, TMEM halfword) gathers what each upload
+//             writes; then one lane per halfword walks the uploads in order
+//             and completes the TMEM snapshot after each one.
 //   k_bin     one wave64 per (tile row, group of 64 primitives): a ballot per
 //             tile column produces the ordered 64-bit primitive masks.
 //   k_raster  one wave64 per 8x8 screen tile: colour/depth/coverage of the 64
@@ -667,43 +669,67 @@ DEV bool tmem_plan(const UploadInfo &u, int idx, bool upper, TmemLoad &ld)
 	return tmem_plan_16(u, idx, ld);
 }
 
-__global__ __launch_bounds__(256) void k_tmem(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+// Stage 2 runs in two kernels so a pass's uploads are processed in parallel.
+// k_tmem_gather: one thread per (upload, TMEM halfword); each upload's
+// written halfwords are stored in its snapshot and flagged in a bit mask.
+// The item is pass << 12 | block of 8 uploads << 3 | 256-halfword chunk.
+__global__ __launch_bounds__(256) void k_tmem_gather(const PassDesc *__restrict__ passes,
+                                                     const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 12];
-	const int g = (int)((item & 0xfff) * 256 + threadIdx.x);  // 0..2047
+	const int g = (int)((item & 7) * 256 + threadIdx.x);  // 0..2047
+	const int i0 = (int)((item >> 3) & 0x1ff) * 8;
 	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
-	uint32_t cur = P.tmem[g];
 	const int idx = g ^ 1;
 	const bool upper = idx >= 0x400;
+	const int n = (int)P.num_uploads;
+	for (int i = i0; i < i0 + 8 && i < n; i++) {
+		const UploadInfo u = P.uploads[i];
+		TmemLoad ld;
+		const bool written = tmem_plan(u, idx, upper, ld);
+		if (written)
+			P.tmem_instances[(i + 1) * 2048 + g] =
+			    (uint16_t)tmem_fetch(m, ld, upper, u.mode != UP_TLUT && u.tmem_fmt == TF_YUV);
+		const uint64_t mask = __ballot(written);
+		if ((threadIdx.x & 63) == 0) P.tmem_written[i * 32 + (g >> 6)] = mask;
+	}
+}
+
+// k_tmem_scan: one thread per TMEM halfword walks the uploads in order and
+// fills every snapshot slot its halfword keeps from the previous one (the
+// last writer wins), then writes the final TMEM back to the stream.  The
+// item is pass << 12 | 256-halfword chunk.
+__global__ __launch_bounds__(256) void k_tmem_scan(const PassDesc *__restrict__ passes,
+                                                   const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 12];
+	const int g = (int)((item & 7) * 256 + threadIdx.x);
+	const int w = g >> 6, bit = g & 63;
+	uint32_t cur = P.tmem[g];
 	P.tmem_instances[g] = (uint16_t)cur;
 	const int n = (int)P.num_uploads;
-	// Uploads in groups of four: plan all four, issue their RDRAM reads
-	// together, then apply them in order, so the chain of uploads waits for
-	// memory once per group rather than once per upload.
-	constexpr int K = 4;
+	constexpr int K = 8;
 	for (int i0 = 0; i0 < n; i0 += K) {
-		uint32_t v[K];
 		bool written[K];
-		bool yuv[K];
+		uint32_t v[K];
 #pragma unroll
 		for (int k = 0; k < K; k++) {
 			written[k] = false;
-			yuv[k] = false;
 			v[k] = 0;
 			if (i0 + k < n) {
-				const UploadInfo u = P.uploads[i0 + k];
-				TmemLoad ld;
-				yuv[k] = u.mode != UP_TLUT && u.tmem_fmt == TF_YUV;
-				written[k] = tmem_plan(u, idx, upper, ld);
-				if (written[k]) v[k] = tmem_fetch(m, ld, upper, yuv[k]);
+				written[k] = ((P.tmem_written[(i0 + k) * 32 + w] >> bit) & 1) != 0;
+				if (written[k]) v[k] = P.tmem_instances[(i0 + k + 1) * 2048 + g];
 			}
 		}
 #pragma unroll
 		for (int k = 0; k < K; k++) {
 			if (i0 + k < n) {
-				if (written[k]) cur = v[k];
-				P.tmem_instances[(i0 + k + 1) * 2048 + g] = (uint16_t)cur;
+				if (written[k])
+					cur = v[k];
+				else
+					P.tmem_instances[(i0 + k + 1) * 2048 + g] = (uint16_t)cur;
 			}
 		}
 	}
@@ -2214,10 +2240,11 @@ hipError_t launch_setup(const PassDesc *passes, const uint32_t *items, uint32_t 
 	return hipGetLastError();
 }
 
-hipError_t launch_tmem(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+hipError_t launch_tmem(const PassDesc *passes, const uint32_t *gather_items, uint32_t n_gather,
+                       const uint32_t *scan_items, uint32_t n_scan, hipStream_t s)
 {
-	if (!n) return hipSuccess;
-	hipLaunchKernelGGL(k_tmem, dim3(n), dim3(256), 0, s, passes, items);
+	if (n_gather) hipLaunchKernelGGL(k_tmem_gather, dim3(n_gather), dim3(256), 0, s, passes, gather_items);
+	if (n_scan) hipLaunchKernelGGL(k_tmem_scan, dim3(n_scan), dim3(256), 0, s, passes, scan_items);
 	return hipGetLastError();
 }
 

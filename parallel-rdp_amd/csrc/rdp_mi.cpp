@@ -34,7 +34,7 @@
 namespace rdp_mi {
 
 hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
-hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 
@@ -241,7 +241,7 @@ struct PassRef {
 
 // Byte sizes of the per-pass device scratch.
 struct ScratchLayout {
-	size_t setups, spans, tilerows, bins, tmem, total;
+	size_t setups, spans, tilerows, bins, tmem, tmem_written, total;
 };
 
 static ScratchLayout scratch_layout(const HostPass &hp)
@@ -254,7 +254,8 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 	l.tilerows = align_up((size_t)hp.num_tilerows * 4, 256);
 	l.bins = align_up((size_t)tx * ty * groups * 8, 256);
 	l.tmem = align_up((hp.uploads.size() + 1) * 4096, 256);
-	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem;
+	l.tmem_written = align_up(std::max<size_t>(hp.uploads.size(), 1) * 32 * 8, 256);
+	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem + l.tmem_written;
 	return l;
 }
 
@@ -262,8 +263,10 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 // raster stage as one launch per occupancy class present).
 struct Launch {
 	const PassDesc *descs;
-	const uint32_t *items[4];
+	const uint32_t *items[4];  // setup, TMEM gather, bin, raster
 	uint32_t n[4];
+	const uint32_t *tmem_scan_items;
+	uint32_t tmem_scan_n;
 	const uint32_t *raster_items[kRasterClasses];
 	uint32_t raster_n[kRasterClasses];
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
@@ -310,7 +313,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 	for (size_t r = 0; r < launches.size(); r++) {
 		const Launch &L = launches[r];
 		const bool big = overlap && L.passes >= kOverlapMinPasses;
-		const bool fork_tmem = big && L.n[1];
+		const bool fork_tmem = big && L.tmem_scan_n;
 		// Raster launches after the class floor: the merged light launch and
 		// the heavier classes.
 		const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
@@ -323,12 +326,12 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		if (fork_tmem) {
 			HIP_CHECK(hipEventRecord(E.fork, s));
 			HIP_CHECK(hipStreamWaitEvent(side, E.fork, 0));
-			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], side));
+			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, side));
 			HIP_CHECK(hipEventRecord(E.tmem_done, side));
 		}
 		HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
-		if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], s));
+		if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
 		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
@@ -371,7 +374,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	// Input layout: per round [PassDesc x n][items: setup|tmem|bin|raster], then per pass data.
 	struct RoundLayout {
 		size_t descs, items_setup, items_tmem, items_bin, items_raster;
-		uint32_t n_setup, n_tmem, n_bin, n_raster;
+		uint32_t n_setup, n_tmem, n_tmem_gather, n_bin, n_raster;
 		uint32_t n_raster_class[kRasterClasses];
 		size_t scratch;
 	};
@@ -386,7 +389,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	size_t off = 0, max_scratch = 0;
 	for (size_t r = 0; r < rounds.size(); r++) {
 		RoundLayout &L = rl[r];
-		L.n_setup = L.n_tmem = L.n_bin = L.n_raster = 0;
+		L.n_setup = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = 0;
 		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = 0;
 		size_t scr = 0;
 		pl[r].resize(rounds[r].size());
@@ -396,6 +399,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
 			L.n_setup += (uint32_t)hp.prims.size();
 			L.n_tmem += 8;
+			L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 7) / 8);
 			if (!hp.prims.empty()) {
 				L.n_bin += ty * groups;
 				L.n_raster += tx * ty;
@@ -413,7 +417,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		L.items_setup = off;
 		off = align_up(off + L.n_setup * 4ull, 256);
 		L.items_tmem = off;
-		off = align_up(off + L.n_tmem * 4ull, 256);
+		off = align_up(off + (L.n_tmem + L.n_tmem_gather) * 4ull, 256);
 		L.items_bin = off;
 		off = align_up(off + L.n_bin * 4ull, 256);
 		L.items_raster = off;
@@ -451,7 +455,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		uint32_t *it_tmem = reinterpret_cast<uint32_t *>(h + L.items_tmem);
 		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
-		uint32_t ns = 0, nt = 0, nb = 0;
+		uint32_t ns = 0, nt = 0, ntg = 0, nb = 0;
 		uint32_t nr[kRasterClasses], rbase[kRasterClasses];
 		for (uint32_t c = 0, acc = 0; c < kRasterClasses; c++) {
 			nr[c] = 0;
@@ -462,10 +466,12 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		LA.descs = reinterpret_cast<const PassDesc *>(d + L.descs);
 		LA.items[0] = reinterpret_cast<const uint32_t *>(d + L.items_setup);
 		LA.items[1] = reinterpret_cast<const uint32_t *>(d + L.items_tmem);
+		LA.tmem_scan_items = LA.items[1] + L.n_tmem_gather;
+		LA.tmem_scan_n = L.n_tmem;
 		LA.items[2] = reinterpret_cast<const uint32_t *>(d + L.items_bin);
 		LA.items[3] = reinterpret_cast<const uint32_t *>(d + L.items_raster);
 		LA.n[0] = L.n_setup;
-		LA.n[1] = L.n_tmem;
+		LA.n[1] = L.n_tmem_gather;
 		LA.n[2] = L.n_bin;
 		LA.n[3] = L.n_raster;
 		for (uint32_t c = 0; c < kRasterClasses; c++) {
@@ -496,7 +502,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.spans = reinterpret_cast<SpanSetup *>(sb + pi.sl.setups);
 			D.tilerows = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans);
 			D.bin_fine = reinterpret_cast<uint64_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows);
-			D.bin_coarse = nullptr;
+			D.tmem_written = reinterpret_cast<uint64_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins +
+			                                              pi.sl.tmem);
 			D.tmem_instances = reinterpret_cast<uint16_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins);
 			D.rdram = pr.d_rdram;
 			D.hidden = pr.d_hidden;
@@ -518,7 +525,9 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 
 			const uint32_t pidx = (uint32_t)i;
 			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 13) | p;
-			for (uint32_t k = 0; k < 8; k++) it_tmem[nt++] = (pidx << 12) | k;
+			for (uint32_t k = 0; k < 8; k++) it_tmem[L.n_tmem_gather + nt++] = (pidx << 12) | k;
+			for (uint32_t b = 0; b < (uint32_t)(hp.uploads.size() + 7) / 8; b++)
+				for (uint32_t k = 0; k < 8; k++) it_tmem[ntg++] = (pidx << 12) | (b << 3) | k;
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)
 					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 14) | (y << 7) | g;

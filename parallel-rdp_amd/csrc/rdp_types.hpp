@@ -341,7 +341,7 @@ struct PassDesc {
 	SpanSetup *spans;
 	uint32_t *tilerows;        // per (prim, tile row): lo tile x | hi tile x << 16, 0xffff lo = empty
 	uint64_t *bin_fine;        // [tile][group] primitive masks
-	uint64_t *bin_coarse;      // [tile] group masks
+	uint64_t *tmem_written;    // [upload][32]: halfwords the upload writes, 64 per mask
 	uint16_t *tmem_instances;  // (num_uploads + 1) x 2048 halfwords
 	// stream memory
 	uint8_t *rdram;
