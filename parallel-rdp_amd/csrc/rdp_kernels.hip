@@ -382,7 +382,7 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 		r.z_mode = st.z_mode;
 		r.fill_color = st.fill_color;
 		r.min_lod = st.min_lod;
-		r.variant = select_raster_variant(st.ras_flags, st.comb[1], st.tiles, s.tile);
+		r.variant = select_raster_variant(st.ras_flags, st.comb, st.tiles, s.tile);
 		r.pad0[0] = r.pad0[1] = 0;
 		for (int c = 0; c < 2; c++) {
 			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
@@ -1389,21 +1389,9 @@ struct RuntimeSel {
 	MDEV int operator[](int i) const { return p[i]; }
 };
 
-template <int C>
+template <int C, int CYCLE>
 struct KnownSel {
-	MDEV int operator[](int i) const
-	{
-		constexpr uint8_t v[4][8] = {
-			{ 0, 0, 0, 0, 0, 0, 0, 0 },
-			{ kKnownCombiners[1][0], kKnownCombiners[1][1], kKnownCombiners[1][2], kKnownCombiners[1][3],
-			  kKnownCombiners[1][4], kKnownCombiners[1][5], kKnownCombiners[1][6], kKnownCombiners[1][7] },
-			{ kKnownCombiners[2][0], kKnownCombiners[2][1], kKnownCombiners[2][2], kKnownCombiners[2][3],
-			  kKnownCombiners[2][4], kKnownCombiners[2][5], kKnownCombiners[2][6], kKnownCombiners[2][7] },
-			{ kKnownCombiners[3][0], kKnownCombiners[3][1], kKnownCombiners[3][2], kKnownCombiners[3][3],
-			  kKnownCombiners[3][4], kKnownCombiners[3][5], kKnownCombiners[3][6], kKnownCombiners[3][7] },
-		};
-		return v[C][i];
-	}
+	MDEV int operator[](int i) const { return kKnownCombiners[C][CYCLE][i]; }
 };
 
 template <typename Sel>
@@ -1840,7 +1828,10 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	int32_t alpha_ref;
 	if (multi) {
 		int32_t c0[4];
-		combine(ci, RuntimeSel{ ps->comb[0] }, ps->constants[0], c0);
+		if constexpr (V::comb != COMB_ANY)
+			combine(ci, KnownSel<V::comb, 0>{}, ps->constants[0], c0);
+		else
+			combine(ci, RuntimeSel{ ps->comb[0] }, ps->constants[0], c0);
 		if (alpha_test) {
 			int32_t cl = clamp_9bit(c0[3]);
 			int32_t ex = cl + ((cl + 1) >> 8);
@@ -1862,9 +1853,12 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), ps->prim_index + 11);
 			ci.noise = (int32_t)(((n2 & 7) << 6) | 0x20);
 		}
-		combine(ci, RuntimeSel{ ps->comb[1] }, ps->constants[1], combined);
-	} else if (V::comb != COMB_ANY) {
-		combine(ci, KnownSel<V::comb>{}, ps->constants[1], combined);
+		if constexpr (V::comb != COMB_ANY)
+			combine(ci, KnownSel<V::comb, 1>{}, ps->constants[1], combined);
+		else
+			combine(ci, RuntimeSel{ ps->comb[1] }, ps->constants[1], combined);
+	} else if constexpr (V::comb != COMB_ANY) {
+		combine(ci, KnownSel<V::comb, 1>{}, ps->constants[1], combined);
 	} else {
 		combine(ci, RuntimeSel{ ps->comb[1] }, ps->constants[1], combined);
 	}

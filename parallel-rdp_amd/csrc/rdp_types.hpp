@@ -115,16 +115,23 @@ enum : uint32_t {
 	COMB_SHADE = 1,      // 1 cycle: rgb = SHADE, alpha = SHADE
 	COMB_TEX0_SHADE = 2, // 1 cycle: TEXEL0 * SHADE (rgb and alpha)
 	COMB_TEX0 = 3,       // 1 cycle: TEXEL0 (decal)
+	COMB_LOD_SHADE = 4,  // 2 cycles: lerp(TEXEL0, TEXEL1, LOD_FRAC), then COMBINED * SHADE
+	COMB_TEX0_ENV = 5,   // 2 cycles: TEXEL0, then rgb COMBINED * ENV, alpha COMBINED
+	COMB_COUNT = 6,
 };
 
-// Raw SetCombine selectors of cycle 1 for the known combiners
-// (rgb muladd, mulsub, mul, add, alpha muladd, mulsub, mul, add).
-constexpr uint8_t kKnownCombiners[4][8] = {
-	{ 0, 0, 0, 0, 0, 0, 0, 0 },
-	{ 8, 8, 16, 4, 7, 7, 7, 4 },
-	{ 1, 8, 4, 7, 1, 7, 4, 7 },
-	{ 8, 8, 16, 1, 7, 7, 7, 1 },
+// Raw SetCombine selectors of the known combiners, per cycle (rgb muladd,
+// mulsub, mul, add, alpha muladd, mulsub, mul, add).  One-cycle combiners
+// use cycle 1 only; for two-cycle ones both cycles must match.
+constexpr uint8_t kKnownCombiners[COMB_COUNT][2][8] = {
+	{ { 0, 0, 0, 0, 0, 0, 0, 0 }, { 0, 0, 0, 0, 0, 0, 0, 0 } },
+	{ { 0, 0, 0, 0, 0, 0, 0, 0 }, { 8, 8, 16, 4, 7, 7, 7, 4 } },
+	{ { 0, 0, 0, 0, 0, 0, 0, 0 }, { 1, 8, 4, 7, 1, 7, 4, 7 } },
+	{ { 0, 0, 0, 0, 0, 0, 0, 0 }, { 8, 8, 16, 1, 7, 7, 7, 1 } },
+	{ { 2, 1, 13, 1, 2, 1, 0, 1 }, { 0, 8, 4, 7, 0, 7, 4, 7 } },
+	{ { 8, 8, 16, 1, 7, 7, 7, 1 }, { 0, 8, 5, 7, 7, 7, 7, 0 } },
 };
+constexpr bool kKnownCombinerTwoCycle[COMB_COUNT] = { false, false, false, false, true, true };
 
 constexpr int TEXFMT_ANY = -1;
 constexpr int TEXFMT_RGBA16 = 0 * 4 + 2;
@@ -162,10 +169,10 @@ constexpr uint32_t RAS_2CYC_SPRITE_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RA
 	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY, 1)                                         \
 	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY, 1)                                               \
 	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY, 1)                                                \
-	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI8, 1)                               \
-	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_ANY, TEXFMT_CI4, 1)                              \
-	X(12, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_ANY, TEXFMT_CI8, 1)                           \
-	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_ANY, TEXFMT_CI4, 1)                           \
+	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI8, 1)                               \
+	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI4, 1)                              \
+	X(12, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI8, 1)                           \
+	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI4, 1)                           \
 	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, 2)
 constexpr uint32_t kRasterClasses = 3;  // the generic variant (id 0) is in the top class
 
@@ -187,15 +194,18 @@ RDP_HD inline int static_texture_format(uint32_t ras_flags, const TileDesc *tile
 	return (int)tiles[base].fmt * 4 + (int)tiles[base].size;
 }
 
-RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t comb_cycle1[8], const TileDesc *tiles,
+RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t comb[2][8], const TileDesc *tiles,
                                              uint32_t setup_tile)
 {
 	const int tf = static_texture_format(ras_flags, tiles, setup_tile);
-#define RDP_MATCH_VARIANT(id, known, value, comb, texfmt, cls)                                             \
+#define RDP_MATCH_VARIANT(id, known, value, kcomb, texfmt, cls)                                            \
 	if ((ras_flags & (known)) == (value) && ((texfmt) == TEXFMT_ANY || (texfmt) == tf)) {                 \
 		bool ok = true;                                                                                     \
-		if ((comb) != COMB_ANY)                                                                             \
-			for (int i = 0; i < 8; i++) ok = ok && comb_cycle1[i] == kKnownCombiners[(comb)][i];            \
+		if ((kcomb) != COMB_ANY)                                                                            \
+			for (int i = 0; i < 8; i++) {                                                                   \
+				ok = ok && comb[1][i] == kKnownCombiners[(kcomb)][1][i];                                     \
+				if (kKnownCombinerTwoCycle[(kcomb)]) ok = ok && comb[0][i] == kKnownCombiners[(kcomb)][0][i]; \
+			}                                                                                               \
 		if (ok) return (id);                                                                                \
 	}
 	RDP_RASTER_VARIANTS(RDP_MATCH_VARIANT)
