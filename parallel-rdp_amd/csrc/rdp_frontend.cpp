@@ -538,7 +538,8 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 		const uint32_t setup_tile = (op >= 0x08 && op <= 0x0f) ? (w[0] >> 16) & 63 : (op == 0x36 ? 0 : (w[1] >> 24) & 7);
 		if (class_state_serial != state_serial || class_setup_tile != setup_tile) {
 			const DrawState &st = cur.states.back();
-			class_value = raster_variant_class(select_raster_variant(st.ras_flags, st.comb, st.tiles, setup_tile));
+			class_value = raster_variant_class(select_raster_variant(st.ras_flags, st.comb, st.tiles, setup_tile, st.db_flags,
+			                                                         st.blend, st.z_mode, st.coverage_mode));
 			class_state_serial = state_serial;
 			class_setup_tile = setup_tile;
 		}

@@ -382,7 +382,8 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 		r.z_mode = st.z_mode;
 		r.fill_color = st.fill_color;
 		r.min_lod = st.min_lod;
-		r.variant = select_raster_variant(st.ras_flags, st.comb, st.tiles, s.tile);
+		r.variant = select_raster_variant(st.ras_flags, st.comb, st.tiles, s.tile, st.db_flags, st.blend, st.z_mode,
+		                                  st.coverage_mode);
 		r.pad0[0] = r.pad0[1] = 0;
 		for (int c = 0; c < 2; c++) {
 			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
@@ -1500,8 +1501,9 @@ DEV void alias_depth_to_color(Pixel &p, uint32_t fmt)
 	p.color_dirty = true;
 }
 
+template <typename Modes>
 DEV void blender(const AS_GLOBAL uint8_t *lut, const int32_t pixel[4], const int32_t memory[4], const AS_CONST uint8_t *fog,
-                 const AS_CONST uint8_t *blendc, int32_t shade_alpha, const AS_CONST uint8_t *modes, bool force_blend, bool blend_en,
+                 const AS_CONST uint8_t *blendc, int32_t shade_alpha, const Modes &modes, bool force_blend, bool blend_en,
                  bool color_on_cvg, bool cvg_wrap, int shift0, int shift1, bool final_cycle, int32_t out[3])
 {
 	int32_t src1[3], src0[3];
@@ -1656,13 +1658,32 @@ struct Shaded {
 // shading.h:38-359 for one pixel of one primitive (no upscaling).
 // A raster code variant: flag bits in KNOWN are the compile-time constants in
 // VALUE, everything else comes from the primitive (see RDP_RASTER_VARIANTS).
-template <uint32_t KNOWN, uint32_t VALUE, int COMB, int TF>
+template <uint32_t KNOWN, uint32_t VALUE, int COMB, int TF, int DBP>
 struct RasterVariant {
 	static constexpr int comb = COMB;
 	static constexpr int texfmt = TF;
+	static constexpr int dbp = DBP;
 	static MDEV bool ras(uint32_t f, uint32_t bit) { return (KNOWN & bit) ? ((VALUE & bit) != 0) : ((f & bit) != 0); }
+	// Depth/blend state: the profile's constants, or the primitive's values.
+	static MDEV bool db(uint32_t f, uint32_t bit)
+	{
+		return DBP != DBP_ANY ? (kDepthBlendProfiles[DBP].db_flags & bit) != 0 : (f & bit) != 0;
+	}
+	static MDEV uint32_t z_mode(uint32_t v) { return DBP != DBP_ANY ? kDepthBlendProfiles[DBP].z_mode : v; }
+	static MDEV uint32_t coverage_mode(uint32_t v) { return DBP != DBP_ANY ? kDepthBlendProfiles[DBP].coverage_mode : v; }
 };
-using GenericVariant = RasterVariant<0u, 0u, (int)COMB_ANY, TEXFMT_ANY>;
+using GenericVariant = RasterVariant<0u, 0u, (int)COMB_ANY, TEXFMT_ANY, (int)DBP_ANY>;
+
+// Blender selector sources, like the combiner's: the primitive's, or a
+// profile's constants.
+struct RuntimeBlend {
+	const AS_CONST uint8_t *p;
+	MDEV int operator[](int i) const { return p[i]; }
+};
+template <int DBP, int CYCLE>
+struct KnownBlend {
+	MDEV int operator[](int i) const { return kDepthBlendProfiles[DBP].blend[CYCLE][i]; }
+};
 
 template <class V>
 DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *tiles, const Tex &tx, uint32_t fb_fmt,
@@ -1892,6 +1913,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 }
 
 // memory_interfacing.h:488-571
+template <class V>
 DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps, uint32_t fmt, bool alias, Pixel &p,
                      const Shaded &sh)
 {
@@ -1899,9 +1921,10 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 	const int dith = sh.z_dith & 0x1ff;
 	int cvg = sh.coverage;
 	const uint32_t f = ps->db_flags;
-	const bool force_blend = f & DB_FORCE_BLEND, z_compare = f & DB_DEPTH_TEST, z_update = f & DB_DEPTH_UPDATE;
-	const bool image_read = f & DB_IMAGE_READ, color_on_cvg = f & DB_COLOR_ON_CVG, multi = f & DB_MULTI_CYCLE;
-	const bool aa = f & DB_AA, dither_en = f & DB_DITHER;
+	const bool force_blend = V::db(f, DB_FORCE_BLEND), z_compare = V::db(f, DB_DEPTH_TEST);
+	const bool z_update = V::db(f, DB_DEPTH_UPDATE), image_read = V::db(f, DB_IMAGE_READ);
+	const bool color_on_cvg = V::db(f, DB_COLOR_ON_CVG), multi = V::db(f, DB_MULTI_CYCLE);
+	const bool aa = V::db(f, DB_AA), dither_en = V::db(f, DB_DITHER);
 
 	int32_t memory[4];
 	int32_t mem_cov = image_read ? (p.c[3] & 0xe0) : 0xe0;
@@ -1950,7 +1973,7 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 		const bool max_z = memory_z == 0x3ffff;
 		const bool front = z < memory_z;
 		const bool nearer = coplanar || ((z - comb) <= memory_z);
-		switch (ps->z_mode) {
+		switch (V::z_mode(ps->z_mode)) {
 		case 0: pass = max_z || (overflow ? front : nearer); break;
 		case 1:
 			if (!front || !farther || !overflow)
@@ -1976,18 +1999,30 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 
 	if (pass && (!aa || cvg != 0)) {
 		int32_t pixel[4] = { sh.combined[0], sh.combined[1], sh.combined[2], sh.combined[3] };
-		const AS_CONST uint8_t *modes = ps->blend[0];
 		int32_t rgb[3];
 		if (multi) {
-			blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, modes, force_blend, blend_en,
-			        color_on_cvg, cvg_wrap, shift0, shift1, false, rgb);
+			if constexpr (V::dbp != DBP_ANY)
+				blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, KnownBlend<V::dbp, 0>{}, force_blend,
+				        blend_en, color_on_cvg, cvg_wrap, shift0, shift1, false, rgb);
+			else
+				blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, RuntimeBlend{ ps->blend[0] }, force_blend,
+				        blend_en, color_on_cvg, cvg_wrap, shift0, shift1, false, rgb);
 			pixel[0] = rgb[0];
 			pixel[1] = rgb[1];
 			pixel[2] = rgb[2];
-			modes = ps->blend[1];
 		}
-		blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, modes, force_blend, blend_en,
-		        color_on_cvg, cvg_wrap, shift0, shift1, true, rgb);
+		// The final cycle uses cycle 1's selectors in two-cycle mode and
+		// cycle 0's in one-cycle mode.
+		if constexpr (V::dbp != DBP_ANY) {
+			if (multi)
+				blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, KnownBlend<V::dbp, 1>{}, force_blend,
+				        blend_en, color_on_cvg, cvg_wrap, shift0, shift1, true, rgb);
+			else
+				blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, KnownBlend<V::dbp, 0>{}, force_blend,
+				        blend_en, color_on_cvg, cvg_wrap, shift0, shift1, true, rgb);
+		} else
+			blender(lut, pixel, memory, ps->fog, ps->blendc, sh.shade_alpha, RuntimeBlend{ ps->blend[multi ? 1 : 0] },
+			        force_blend, blend_en, color_on_cvg, cvg_wrap, shift0, shift1, true, rgb);
 		if (dither_en) {
 #pragma unroll
 			for (int i = 0; i < 3; i++) {
@@ -1999,7 +2034,7 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 			}
 		}
 		int new_cov;
-		switch (ps->coverage_mode) {
+		switch (V::coverage_mode(ps->coverage_mode)) {
 		case 0: new_cov = blend_en ? imin(7, memory_coverage + cvg) : ((cvg - 1) & 7); break;
 		case 1: new_cov = (cvg + memory_coverage) & 7; break;
 		case 2: new_cov = 7; break;
@@ -2079,7 +2114,7 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *st
 	else if (sh.coverage & COVERAGE_COPY)
 		copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
 	else
-		depth_blend(lut, ps, fmt, alias, px, sh);
+		depth_blend<V>(lut, ps, fmt, alias, px, sh);
 }
 
 // Waves per SIMD each occupancy class is compiled for (the register budget
@@ -2200,10 +2235,10 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 
 				if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
 				switch (ps->variant) {
-#define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt, cls)                                                 \
+#define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt, dbp, cls)                                            \
 		case id:                                                                                                    \
 			if constexpr ((cls) <= CLS)                                                                             \
-				raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt)>>(ps, states, spans, tex, lut, fmt, alias, ci, \
+				raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt), (int)(dbp)>>(ps, states, spans, tex, lut, fmt, alias, ci, \
 				                                                                    x, y, px);                        \
 			break;
 					RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)

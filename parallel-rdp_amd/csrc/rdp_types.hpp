@@ -133,6 +133,42 @@ constexpr uint8_t kKnownCombiners[COMB_COUNT][2][8] = {
 };
 constexpr bool kKnownCombinerTwoCycle[COMB_COUNT] = { false, false, false, false, true, true };
 
+// Depth/blend profiles a raster variant can fix at compile time: all
+// depth/blend flags, both cycles' blender selectors (1a, 1b, 2a, 2b), the Z
+// mode and the coverage mode.
+struct DepthBlendProfile {
+	uint32_t db_flags;
+	uint8_t blend[2][4];
+	uint32_t z_mode, coverage_mode;
+};
+enum : uint32_t {
+	DBP_ANY = 0,            // read at run time
+	DBP_Z_OPAQUE = 1,       // Z test + update, no blending, 1 cycle
+	DBP_Z_FORCE_BLEND = 2,  // Z test + update, forced pixel/memory blend by pixel alpha, 1 cycle
+	DBP_Z_AA_FOG = 3,       // Z test + update, coverage AA, 2 cycles: fog by shade alpha, then pixel/memory
+	DBP_AA_SPRITE = 4,      // coverage AA without Z, 2 cycles: pixel, then pixel/memory
+	DBP_COUNT = 5,
+};
+constexpr DepthBlendProfile kDepthBlendProfiles[DBP_COUNT] = {
+	{ 0, { { 0, 0, 0, 0 }, { 0, 0, 0, 0 } }, 0, 0 },
+	{ DB_DEPTH_TEST | DB_DEPTH_UPDATE, { { 0, 0, 1, 0 }, { 0, 0, 1, 0 } }, 0, 0 },
+	{ DB_DEPTH_TEST | DB_DEPTH_UPDATE | DB_FORCE_BLEND, { { 0, 0, 1, 0 }, { 0, 0, 1, 0 } }, 0, 0 },
+	{ DB_DEPTH_TEST | DB_DEPTH_UPDATE | DB_IMAGE_READ | DB_MULTI_CYCLE | DB_AA, { { 3, 2, 0, 0 }, { 0, 0, 1, 1 } }, 0, 0 },
+	{ DB_IMAGE_READ | DB_MULTI_CYCLE | DB_AA, { { 0, 0, 0, 0 }, { 0, 0, 1, 1 } }, 0, 0 },
+};
+
+RDP_HD inline bool depth_blend_profile_matches(uint32_t dbp, uint32_t db_flags, const uint8_t blend[2][4], uint32_t z_mode,
+                                               uint32_t coverage_mode)
+{
+	if (dbp == DBP_ANY) return true;
+	const DepthBlendProfile &p = kDepthBlendProfiles[dbp];
+	if (p.db_flags != db_flags || p.z_mode != z_mode || p.coverage_mode != coverage_mode) return false;
+	for (int c = 0; c < 2; c++)
+		for (int i = 0; i < 4; i++)
+			if (p.blend[c][i] != blend[c][i]) return false;
+	return true;
+}
+
 constexpr int TEXFMT_ANY = -1;
 constexpr int TEXFMT_RGBA16 = 0 * 4 + 2;
 constexpr int TEXFMT_CI4 = 2 * 4 + 0;
@@ -154,26 +190,28 @@ constexpr uint32_t RAS_2CYC_LOD_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_U
 constexpr uint32_t RAS_2CYC_SPRITE_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RAS_SAMPLE_QUAD | RAS_BILERP0 |
                                           RAS_BILERP1 | RAS_TLUT;
 
-// X(id, ras_known, ras_value, combiner, texture format, occupancy class).
+// X(id, ras_known, ras_value, combiner, texture format, depth/blend profile,
+// occupancy class).
 // The raster kernel is built once per occupancy class with only the variants
 // of that class and below (the top class adds the generic code), and a pass
 // runs in the lowest class that holds all its primitives' variants: light
 // passes then get the register budget, and so the waves per SIMD, of their
 // own code rather than of the largest variant.
-#define RDP_RASTER_VARIANTS(X)                                                                                     \
-	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY, 0)                                                   \
-	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY, 0)                                                   \
-	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY, 0)                                                           \
-	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY, 0)                                                             \
-	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, 0)                     \
-	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY, 1)                                         \
-	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY, 1)                                               \
-	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY, 1)                                                \
-	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI8, 1)                               \
-	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI4, 1)                              \
-	X(12, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI8, 1)                           \
-	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI4, 1)                           \
-	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, 2)
+#define RDP_RASTER_VARIANTS(X)                                                                                        \
+	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                \
+	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                \
+	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY, DBP_Z_OPAQUE, 0)                                                   \
+	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                          \
+	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_Z_FORCE_BLEND, 0)        \
+	X(14, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_ANY, 0)                 \
+	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY, DBP_ANY, 1)                                      \
+	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY, DBP_ANY, 1)                                            \
+	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY, DBP_ANY, 1)                                             \
+	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI8, DBP_Z_AA_FOG, 1)                 \
+	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI4, DBP_Z_AA_FOG, 1)                \
+	X(12, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI8, DBP_AA_SPRITE, 1)             \
+	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI4, DBP_AA_SPRITE, 1)             \
+	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, DBP_ANY, 2)
 constexpr uint32_t kRasterClasses = 3;  // the generic variant (id 0) is in the top class
 
 // The format every tile a primitive can sample shares, or TEXFMT_ANY.
@@ -195,11 +233,13 @@ RDP_HD inline int static_texture_format(uint32_t ras_flags, const TileDesc *tile
 }
 
 RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t comb[2][8], const TileDesc *tiles,
-                                             uint32_t setup_tile)
+                                             uint32_t setup_tile, uint32_t db_flags, const uint8_t blend[2][4],
+                                             uint32_t z_mode, uint32_t coverage_mode)
 {
 	const int tf = static_texture_format(ras_flags, tiles, setup_tile);
-#define RDP_MATCH_VARIANT(id, known, value, kcomb, texfmt, cls)                                            \
-	if ((ras_flags & (known)) == (value) && ((texfmt) == TEXFMT_ANY || (texfmt) == tf)) {                 \
+#define RDP_MATCH_VARIANT(id, known, value, kcomb, texfmt, dbp, cls)                                       \
+	if ((ras_flags & (known)) == (value) && ((texfmt) == TEXFMT_ANY || (texfmt) == tf) &&                \
+	    depth_blend_profile_matches((dbp), db_flags, blend, z_mode, coverage_mode)) {                      \
 		bool ok = true;                                                                                     \
 		if ((kcomb) != COMB_ANY)                                                                            \
 			for (int i = 0; i < 8; i++) {                                                                   \
@@ -215,7 +255,7 @@ RDP_HD inline uint32_t select_raster_variant(uint32_t ras_flags, const uint8_t c
 
 RDP_HD inline uint32_t raster_variant_class(uint32_t variant)
 {
-#define RDP_VARIANT_CLASS(id, known, value, comb, texfmt, cls) \
+#define RDP_VARIANT_CLASS(id, known, value, comb, texfmt, dbp, cls) \
 	if (variant == (id)) return (cls);
 	RDP_RASTER_VARIANTS(RDP_VARIANT_CLASS)
 #undef RDP_VARIANT_CLASS
