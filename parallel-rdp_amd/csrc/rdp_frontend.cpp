@@ -533,13 +533,14 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 			state_dirty = false;
 			state_serial++;
 		}
-		// The kernel's variant choice (k_setup) from the same state and setup
-		// tile, reduced to its occupancy class.
+		// The raster variant for this state and setup tile (k_raster
+		// dispatches on it) and its occupancy class.
 		const uint32_t setup_tile = (op >= 0x08 && op <= 0x0f) ? (w[0] >> 16) & 63 : (op == 0x36 ? 0 : (w[1] >> 24) & 7);
 		if (class_state_serial != state_serial || class_setup_tile != setup_tile) {
 			const DrawState &st = cur.states.back();
-			class_value = raster_variant_class(select_raster_variant(st.ras_flags, st.comb, st.tiles, setup_tile, st.db_flags,
-			                                                         st.blend, st.z_mode, st.coverage_mode));
+			variant_value = select_raster_variant(st.ras_flags, st.comb, st.tiles, setup_tile, st.db_flags, st.blend,
+			                                      st.z_mode, st.coverage_mode);
+			class_value = raster_variant_class(variant_value);
 			class_state_serial = state_serial;
 			class_setup_tile = setup_tile;
 		}
@@ -556,6 +557,7 @@ void Frontend::draw(unsigned op, const uint32_t *w, unsigned len)
 		r.tilerow_offset = cur.num_tilerows;
 		r.prim_index = prim_index;
 		r.raster_class = (uint8_t)class_value;
+		r.variant = (uint16_t)variant_value;
 		cur.raster_class = std::max(cur.raster_class, class_value);
 		cur.words.insert(cur.words.end(), w, w + len);
 		cur.prims.push_back(r);

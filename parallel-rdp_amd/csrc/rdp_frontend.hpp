@@ -131,7 +131,7 @@ private:
 	uint64_t state_gen = 0;  // bumped by every command that is not a draw
 	// Raster class of the last (draw state, setup tile) pair.
 	uint64_t class_state_serial = 0, state_serial = 1;
-	uint32_t class_setup_tile = 0, class_value = 0;
+	uint32_t class_setup_tile = 0, class_value = 0, variant_value = 0;
 	bool color_write_pending = false, depth_write_pending = false;
 
 	HostPass cur;

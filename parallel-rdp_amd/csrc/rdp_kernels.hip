@@ -7,9 +7,10 @@
 //             covered scanline (SpanSetup) and reduces the per-8-line x extent
 //             into tile-row ranges.
 //   k_tmem_gather / k_tmem_scan
-//             one lane per (upload, TMEM halfword) gathers what each upload
-//             writes; then one lane per halfword walks the uploads in order
-//             and completes the TMEM snapshot after each one.
+//             one lane per (block of 32 uploads, TMEM halfword) applies the
+//             block's uploads in order; then one lane per halfword carries
+//             values across blocks, completing the TMEM snapshot after every
+//             upload.
 //   k_bin     one wave64 per (tile row, group of 64 primitives): a ballot per
 //             tile column produces the ordered 64-bit primitive masks.
 //   k_raster  one wave64 per 8x8 screen tile: colour/depth/coverage of the 64
@@ -382,8 +383,7 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 		r.z_mode = st.z_mode;
 		r.fill_color = st.fill_color;
 		r.min_lod = st.min_lod;
-		r.variant = select_raster_variant(st.ras_flags, st.comb, st.tiles, s.tile, st.db_flags, st.blend, st.z_mode,
-		                                  st.coverage_mode);
+		r.variant = rec.variant;
 		r.pad0[0] = r.pad0[1] = 0;
 		for (int c = 0; c < 2; c++) {
 			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
@@ -671,70 +671,64 @@ DEV bool tmem_plan(const UploadInfo &u, int idx, bool upper, TmemLoad &ld)
 }
 
 // Stage 2 runs in two kernels so a pass's uploads are processed in parallel.
-// k_tmem_gather: one thread per (upload, TMEM halfword); each upload's
-// written halfwords are stored in its snapshot and flagged in a bit mask.
-// The item is pass << 12 | block of 8 uploads << 3 | 256-halfword chunk.
+// k_tmem_gather: one thread per (block of 32 uploads, TMEM halfword) applies
+// the block's uploads in order: from the block's first write to its halfword
+// on, every snapshot slot of the block is complete.  It records where that
+// first write is and the halfword's value after the block.  The item is
+// pass << 12 | block << 3 | 256-halfword chunk.
+constexpr int kTmemBlock = 32;
+
 __global__ __launch_bounds__(256) void k_tmem_gather(const PassDesc *__restrict__ passes,
                                                      const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 12];
 	const int g = (int)((item & 7) * 256 + threadIdx.x);  // 0..2047
-	const int i0 = (int)((item >> 3) & 0x1ff) * 8;
+	const int block = (int)((item >> 3) & 0x1ff);
+	const int i0 = block * kTmemBlock;
 	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
 	const int idx = g ^ 1;
 	const bool upper = idx >= 0x400;
 	const int n = (int)P.num_uploads;
-	for (int i = i0; i < i0 + 8 && i < n; i++) {
+	const int end = imin(i0 + kTmemBlock, n);
+	uint32_t cur = 0;
+	int first = kTmemBlock;
+	for (int i = i0; i < end; i++) {
 		const UploadInfo u = P.uploads[i];
 		TmemLoad ld;
-		const bool written = tmem_plan(u, idx, upper, ld);
-		if (written)
-			P.tmem_instances[(i + 1) * 2048 + g] =
-			    (uint16_t)tmem_fetch(m, ld, upper, u.mode != UP_TLUT && u.tmem_fmt == TF_YUV);
-		const uint64_t mask = __ballot(written);
-		if ((threadIdx.x & 63) == 0) P.tmem_written[i * 32 + (g >> 6)] = mask;
+		if (tmem_plan(u, idx, upper, ld)) {
+			cur = tmem_fetch(m, ld, upper, u.mode != UP_TLUT && u.tmem_fmt == TF_YUV);
+			if (first == kTmemBlock) first = i - i0;
+		}
+		if (first != kTmemBlock) P.tmem_instances[(i + 1) * 2048 + g] = (uint16_t)cur;
 	}
+	P.tmem_blocks[block * 2048 + g] = ((uint32_t)first << 16) | cur;
 }
 
-// k_tmem_scan: one thread per TMEM halfword walks the uploads in order and
-// fills every snapshot slot its halfword keeps from the previous one (the
-// last writer wins), then writes the final TMEM back to the stream.  The
-// item is pass << 12 | 256-halfword chunk.
+// k_tmem_scan: one thread per TMEM halfword walks the blocks in order,
+// filling each block's slots before its first write with the value carried
+// from the blocks before (the last writer wins), then writes the final TMEM
+// back to the stream.  The item is pass << 12 | 256-halfword chunk.
 __global__ __launch_bounds__(256) void k_tmem_scan(const PassDesc *__restrict__ passes,
                                                    const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 12];
 	const int g = (int)((item & 7) * 256 + threadIdx.x);
-	const int w = g >> 6, bit = g & 63;
-	uint32_t cur = P.tmem[g];
-	P.tmem_instances[g] = (uint16_t)cur;
+	uint32_t carry = P.tmem[g];
+	P.tmem_instances[g] = (uint16_t)carry;
 	const int n = (int)P.num_uploads;
-	constexpr int K = 8;
-	for (int i0 = 0; i0 < n; i0 += K) {
-		bool written[K];
-		uint32_t v[K];
-#pragma unroll
-		for (int k = 0; k < K; k++) {
-			written[k] = false;
-			v[k] = 0;
-			if (i0 + k < n) {
-				written[k] = ((P.tmem_written[(i0 + k) * 32 + w] >> bit) & 1) != 0;
-				if (written[k]) v[k] = P.tmem_instances[(i0 + k + 1) * 2048 + g];
-			}
-		}
-#pragma unroll
-		for (int k = 0; k < K; k++) {
-			if (i0 + k < n) {
-				if (written[k])
-					cur = v[k];
-				else
-					P.tmem_instances[(i0 + k + 1) * 2048 + g] = (uint16_t)cur;
-			}
-		}
+	const int blocks = (n + kTmemBlock - 1) / kTmemBlock;
+	for (int b = 0; b < blocks; b++) {
+		const uint32_t info = P.tmem_blocks[b * 2048 + g];
+		const int i0 = b * kTmemBlock;
+		const int count = imin(kTmemBlock, n - i0);
+		const int first = (int)(info >> 16);
+		const int gap = imin(first, count);
+		for (int k = 0; k < gap; k++) P.tmem_instances[(i0 + k + 1) * 2048 + g] = (uint16_t)carry;
+		if (first < count) carry = info & 0xffff;
 	}
-	P.tmem[g] = (uint16_t)cur;
+	P.tmem[g] = (uint16_t)carry;
 }
 
 // ==========================================================================

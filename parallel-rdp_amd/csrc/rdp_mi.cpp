@@ -126,7 +126,7 @@ struct PinnedBuffer {
 };
 
 struct RoundEvents {
-	hipEvent_t fork = nullptr, tmem_done = nullptr, bin_done = nullptr, join = nullptr;
+	hipEvent_t fork = nullptr, tmem_done = nullptr, bin_done = nullptr, join = nullptr, prep_done = nullptr;
 };
 
 // A submission lane: a stream with its own pinned staging buffer and device
@@ -135,8 +135,10 @@ struct RoundEvents {
 // staging buffer, which waits for the previous upload).
 struct Lane {
 	hipStream_t stream = nullptr;
-	// Parts of a round run on `side` beside `stream` (events per round).
-	hipStream_t side = nullptr;
+	// Parts of a round run on `side` beside `stream`, and setup/binning of
+	// all rounds on `prep` ahead of them (events per round).
+	hipStream_t side = nullptr, prep = nullptr;
+	hipEvent_t start = nullptr;
 	std::vector<RoundEvents> events;
 	DeviceBuffer input, scratch;
 	PinnedBuffer staging;
@@ -148,6 +150,8 @@ struct Lane {
 	{
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 		HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+		HIP_CHECK(hipStreamCreateWithFlags(&prep, hipStreamNonBlocking));
+		HIP_CHECK(hipEventCreateWithFlags(&start, hipEventDisableTiming));
 		HIP_CHECK(hipEventCreateWithFlags(&staging_done, hipEventDisableTiming));
 		return 0;
 	}
@@ -159,6 +163,7 @@ struct Lane {
 			HIP_CHECK(hipEventCreateWithFlags(&e.tmem_done, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.bin_done, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.join, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&e.prep_done, hipEventDisableTiming));
 			events.push_back(e);
 		}
 		return 0;
@@ -167,13 +172,16 @@ struct Lane {
 	{
 		if (stream) (void)hipStreamSynchronize(stream);
 		if (side) (void)hipStreamSynchronize(side);
+		if (prep) (void)hipStreamSynchronize(prep);
 		if (input.ptr) (void)hipFree(input.ptr);
 		if (scratch.ptr) (void)hipFree(scratch.ptr);
 		if (staging.ptr) (void)hipHostFree(staging.ptr);
 		if (staging_done) (void)hipEventDestroy(staging_done);
 		for (auto &e : events)
-			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join }) (void)hipEventDestroy(ev);
+			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join, e.prep_done }) (void)hipEventDestroy(ev);
+		if (start) (void)hipEventDestroy(start);
 		if (side) (void)hipStreamDestroy(side);
+		if (prep) (void)hipStreamDestroy(prep);
 		if (stream) (void)hipStreamDestroy(stream);
 		*this = Lane();
 	}
@@ -241,7 +249,7 @@ struct PassRef {
 
 // Byte sizes of the per-pass device scratch.
 struct ScratchLayout {
-	size_t setups, spans, tilerows, bins, tmem, tmem_written, total;
+	size_t setups, spans, tilerows, bins, tmem, tmem_blocks, total;
 };
 
 static ScratchLayout scratch_layout(const HostPass &hp)
@@ -254,8 +262,8 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 	l.tilerows = align_up((size_t)hp.num_tilerows * 4, 256);
 	l.bins = align_up((size_t)tx * ty * groups * 8, 256);
 	l.tmem = align_up((hp.uploads.size() + 1) * 4096, 256);
-	l.tmem_written = align_up(std::max<size_t>(hp.uploads.size(), 1) * 32 * 8, 256);
-	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem + l.tmem_written;
+	l.tmem_blocks = align_up(std::max<size_t>((hp.uploads.size() + 31) / 32, 1) * 2048 * 4, 256);
+	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem + l.tmem_blocks;
 	return l;
 }
 
@@ -269,6 +277,7 @@ struct Launch {
 	uint32_t tmem_scan_n;
 	const uint32_t *raster_items[kRasterClasses];
 	uint32_t raster_n[kRasterClasses];
+	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
 	uint32_t passes;
 };
@@ -288,16 +297,19 @@ static uint64_t framebuffer_bytes(const HostPass &hp)
 	return hp.prims.empty() ? 0 : 2 * tiles * 64 * (color + depth);
 }
 
-// Launches every round on the lane's stream.  In rounds of many passes two
-// parts run on the lane's side stream:
-//   - the TMEM replay, which depends only on RDRAM as the previous round left
-//     it and writes only its own snapshots, forks at the round's start and
-//     runs beside setup and binning;
-//   - the raster launches of the heavier occupancy classes, which touch other
-//     passes than the light class's launch, run beside it once binning is
-//     done, so the light launch fills the GPU while the heavy one's long
-//     per-tile chains finish.
-// The round joins both back before the next round reuses the scratch arena.
+// Launches every round on the lane's stream.  In submissions with a round of
+// many passes, three streams share the work:
+//   - `prep` runs the setup and binning of every round ahead, starting with
+//     the submission (they read only the packed inputs, and each round has
+//     its own scratch region), so round r+1's setup overlaps round r's
+//     raster;
+//   - `side` runs a round's TMEM replay, which depends on RDRAM as the
+//     previous round left it and writes only its own snapshots, beside the
+//     rest of the round, and then the raster launches of the heavier
+//     occupancy classes beside the light class's launch on the main stream,
+//     so the light launch fills the GPU while the heavy one's long per-tile
+//     chains finish;
+//   - the main stream joins everything back before the next round's TMEM.
 // Rounds of a few passes (the plain path, small serving batches) stay in
 // series, where the events cost more than they hide; stage timing
 // (stage_events) needs the stages in series too.
@@ -306,10 +318,21 @@ constexpr uint32_t kOverlapMinPasses = 8;
 static int execute_launches(const std::vector<Launch> &launches, Lane &lane, const LaunchConfig &cfg,
                             hipEvent_t *stage_events = nullptr)
 {
-	hipStream_t s = lane.stream, side = lane.side;
-	const bool overlap = !stage_events && side;
+	hipStream_t s = lane.stream, side = lane.side, prep = lane.prep;
+	const bool overlap = !stage_events && side && prep;
+	const bool ahead = overlap && !launches.empty() && launches[0].prep_ahead;
 	if (overlap)
 		if (int err = lane.ensure_round_events(launches.size())) return err;
+	if (ahead) {
+		HIP_CHECK(hipEventRecord(lane.start, s));
+		HIP_CHECK(hipStreamWaitEvent(prep, lane.start, 0));
+		for (size_t r = 0; r < launches.size(); r++) {
+			const Launch &L = launches[r];
+			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], prep));
+			HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], prep));
+			HIP_CHECK(hipEventRecord(lane.events[r].prep_done, prep));
+		}
+	}
 	for (size_t r = 0; r < launches.size(); r++) {
 		const Launch &L = launches[r];
 		const bool big = overlap && L.passes >= kOverlapMinPasses;
@@ -329,17 +352,28 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, side));
 			HIP_CHECK(hipEventRecord(E.tmem_done, side));
 		}
-		HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
-		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
-		if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
-		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
-		HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
-		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
+		if (ahead) {
+			if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
+			HIP_CHECK(hipStreamWaitEvent(s, E.prep_done, 0));
+		} else {
+			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
+			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
+			if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
+			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
+			HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
+			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
+		}
 		if (fork_raster) {
-			// The side stream already holds the TMEM replay when it forked;
-			// otherwise TMEM ran on the main stream and bin_done covers it.
-			HIP_CHECK(hipEventRecord(E.bin_done, s));
-			HIP_CHECK(hipStreamWaitEvent(side, E.bin_done, 0));
+			// The heavy launch needs setup, binning and the TMEM replay.
+			// With both forked, prep_done covers the first two and the side
+			// stream's own order the third; otherwise the main stream has
+			// reached all of them here.
+			hipEvent_t ready = E.prep_done;
+			if (!(ahead && fork_tmem)) {
+				HIP_CHECK(hipEventRecord(E.bin_done, s));
+				ready = E.bin_done;
+			}
+			HIP_CHECK(hipStreamWaitEvent(side, ready, 0));
 			for (uint32_t c = lo + 1; c < kRasterClasses; c++)
 				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, side));
 			HIP_CHECK(hipEventRecord(E.join, side));
@@ -386,12 +420,18 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	};
 	std::vector<std::vector<PassInputs>> pl(rounds.size());
 
-	size_t off = 0, max_scratch = 0;
+	// Submissions with a round of many passes run every round's setup and
+	// binning ahead on the lane's prep stream (execute_launches), so each
+	// round keeps its own scratch region; otherwise rounds run in series and
+	// share one region.
+	bool prep_ahead = false;
+	for (const auto &round : rounds) prep_ahead = prep_ahead || round.size() >= kOverlapMinPasses;
+	size_t off = 0, max_scratch = 0, scratch_base = 0;
 	for (size_t r = 0; r < rounds.size(); r++) {
 		RoundLayout &L = rl[r];
 		L.n_setup = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = 0;
 		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = 0;
-		size_t scr = 0;
+		size_t scr = scratch_base;
 		pl[r].resize(rounds[r].size());
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
@@ -399,7 +439,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
 			L.n_setup += (uint32_t)hp.prims.size();
 			L.n_tmem += 8;
-			L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 7) / 8);
+			L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 31) / 32);
 			if (!hp.prims.empty()) {
 				L.n_bin += ty * groups;
 				L.n_raster += tx * ty;
@@ -411,6 +451,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			scr += pi.sl.total;
 		}
 		max_scratch = std::max(max_scratch, scr);
+		if (prep_ahead) scratch_base = align_up(scr, 256);
 		L.scratch = scr;
 		L.descs = off;
 		off = align_up(off + rounds[r].size() * sizeof(PassDesc), 256);
@@ -474,6 +515,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		LA.n[1] = L.n_tmem_gather;
 		LA.n[2] = L.n_bin;
 		LA.n[3] = L.n_raster;
+		LA.prep_ahead = prep_ahead;
 		for (uint32_t c = 0; c < kRasterClasses; c++) {
 			LA.raster_items[c] = LA.items[3] + rbase[c];
 			LA.raster_n[c] = L.n_raster_class[c];
@@ -502,8 +544,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.spans = reinterpret_cast<SpanSetup *>(sb + pi.sl.setups);
 			D.tilerows = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans);
 			D.bin_fine = reinterpret_cast<uint64_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows);
-			D.tmem_written = reinterpret_cast<uint64_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins +
-			                                              pi.sl.tmem);
+			D.tmem_blocks = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins +
+			                                             pi.sl.tmem);
 			D.tmem_instances = reinterpret_cast<uint16_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins);
 			D.rdram = pr.d_rdram;
 			D.hidden = pr.d_hidden;
@@ -526,7 +568,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			const uint32_t pidx = (uint32_t)i;
 			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 13) | p;
 			for (uint32_t k = 0; k < 8; k++) it_tmem[L.n_tmem_gather + nt++] = (pidx << 12) | k;
-			for (uint32_t b = 0; b < (uint32_t)(hp.uploads.size() + 7) / 8; b++)
+			for (uint32_t b = 0; b < (uint32_t)(hp.uploads.size() + 31) / 32; b++)
 				for (uint32_t k = 0; k < 8; k++) it_tmem[ntg++] = (pidx << 12) | (b << 3) | k;
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)

@@ -299,7 +299,7 @@ struct PrimRecord {
 	uint32_t prim_index;    // stream-global primitive counter (noise seed)
 	uint8_t opcode;
 	uint8_t raster_class;   // occupancy class of the primitive's raster variant
-	uint16_t pad;
+	uint16_t variant;       // raster variant (select_raster_variant)
 };
 static_assert(sizeof(PrimRecord) == 32, "PrimRecord layout");
 
@@ -391,7 +391,7 @@ struct PassDesc {
 	SpanSetup *spans;
 	uint32_t *tilerows;        // per (prim, tile row): lo tile x | hi tile x << 16, 0xffff lo = empty
 	uint64_t *bin_fine;        // [tile][group] primitive masks
-	uint64_t *tmem_written;    // [upload][32]: halfwords the upload writes, 64 per mask
+	uint32_t *tmem_blocks;     // [block of 32 uploads][2048]: first write in block << 16 | value after it
 	uint16_t *tmem_instances;  // (num_uploads + 1) x 2048 halfwords
 	// stream memory
 	uint8_t *rdram;
