@@ -135,7 +135,7 @@ DEV int normalize_dzpix(int dz)
 // CommandProcessor decoders, rdp_device.cpp:268-344 and 713-821, plus
 // Renderer::fixup_triangle_setup (rdp_renderer.cpp:1402-1423) and the
 // primitive-depth override (rdp_renderer.cpp:1489-1497).
-DEV void decode_primitive(const uint32_t *w, uint32_t op, const DrawState &st, PrimSetup &s)
+DEV void decode_primitive(const AS_CONST uint32_t *w, uint32_t op, const AS_CONST DrawState &st, PrimSetup &s)
 {
 	const uint32_t raw = st.raw_ras_flags;
 	for (int i = 0; i < 4; i++) {
@@ -162,7 +162,7 @@ DEV void decode_primitive(const uint32_t *w, uint32_t op, const DrawState &st, P
 		s.dxldy = sext((int32_t)(w[3] >> 2), 28) >> 1;
 		s.dxhdy = sext((int32_t)(w[5] >> 2), 28) >> 1;
 		s.dxmdy = sext((int32_t)(w[7] >> 2), 28) >> 1;
-		const uint32_t *p = w + 8;
+		const AS_CONST uint32_t *p = w + 8;
 		if (op & 4) {
 			for (int i = 0; i < 2; i++) {
 				s.rgba[2 * i + 0] = (int32_t)((p[i] & 0xffff0000u) | ((p[4 + i] >> 16) & 0xffff));
@@ -262,7 +262,7 @@ DEV void decode_primitive(const uint32_t *w, uint32_t op, const DrawState &st, P
 DEV int32_t quantize_x(int32_t x) { return (x >> 12) | ((x & 0xfff) != 0); }
 
 // span_setup.comp:87-227 for one scanline (no upscaling).
-DEV void compute_span(const PrimSetup &s, const uint32_t sc[4], int y, SpanSetup &out)
+DEV void compute_span(const PrimSetup &s, const AS_CONST uint32_t *sc, int y, SpanSetup &out)
 {
 	const bool flip = (s.flags & TRI_FLIP) != 0;
 	{
@@ -342,12 +342,14 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 13];
 	const uint32_t prim = item & 0x1fff;
-	const PrimRecord rec = P.prims[prim];
-	const DrawState &st = P.states[rec.state_index];
+	// Everything per primitive is wave-uniform: read it through the constant
+	// address space so it lives in scalar registers.
+	const AS_CONST PrimRecord &rec = cptr(P.prims)[prim];
+	const AS_CONST DrawState &st = cptr(P.states)[rec.state_index];
 	const int lane = threadIdx.x;
 
 	PrimSetup s;
-	decode_primitive(P.words + rec.word_offset, rec.opcode, st, s);
+	decode_primitive(cptr(P.words) + rec.word_offset, rec.opcode, st, s);
 	s.state_index = rec.state_index;
 	s.tmem_instance = rec.tmem_instance;
 	s.prim_index = rec.prim_index;
