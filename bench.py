@@ -112,7 +112,8 @@ def serving(rdp_mi, streams, refs, args):
     budget = args.hbm_budget_mib << 20
 
     def run(arrivals, instances):
-        with rdp_mi.Server(procs, hbm_budget_bytes=budget, max_batch=args.max_batch, streams=args.serve_streams) as srv:
+        with rdp_mi.Server(procs, hbm_budget_bytes=budget, max_batch=args.max_batch, streams=args.serve_streams,
+                           decode_threads=args.serve_decode_threads) as srv:
             for a, i in zip(arrivals, instances):
                 srv.submit(int(i), words[tmpl[int(i)]], arrival_us=float(a))
             srv.start()
@@ -127,6 +128,7 @@ def serving(rdp_mi, streams, refs, args):
         f"(largest {st['max_batch_frames']} frames, {st['max_arena_bytes'] / 2**20:.1f} MiB arena)")
     out = {"instances": n, "distinct_streams": len(streams), "hbm_budget_bytes": budget,
            "submission_streams": args.serve_streams, "max_batch": args.max_batch,
+           "decode_threads": args.serve_decode_threads,
            "resident_bytes_per_instance": rs.RDRAM_SIZE + rs.RDRAM_SIZE // 2 + 4096,
            "peak_frames_per_s": round(peak, 1), "peak_stats": st, "loads": []}
     for frac in (0.25, 0.5, 0.9):
@@ -176,7 +178,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-max-frames", type=int, default=4096)
     ap.add_argument("--max-batch", type=int, default=64, help="frames per serving batch at most")
-    ap.add_argument("--serve-streams", type=int, default=2, help="submission streams of the server")
+    ap.add_argument("--serve-streams", type=int, default=3, help="submission streams of the server")
+    ap.add_argument("--serve-decode-threads", type=int, default=0,
+                    help="host decode threads of the server (0: half the host threads, at most 8)")
     ap.add_argument("--serve-dump", default="", help="write per-request serving times to PREFIX_<load>.npz")
     ap.add_argument("--hbm-budget-mib", type=int, default=2048,
                     help="device bytes one serving batch may use for packed inputs and scratch")

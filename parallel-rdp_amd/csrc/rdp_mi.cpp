@@ -1506,9 +1506,11 @@ int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t 
 	}
 	s.budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)1 << 32);
 	s.max_batch = max_batch ? max_batch : count;
+	// Defaults: half the host threads (at most 8) decode, leaving the rest to
+	// the scheduler, submitter and completion threads; three streams.
 	s.decode_threads = decode_threads ? decode_threads
-	                                  : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-	for (unsigned i = 0; i < (streams ? std::min(streams, 8u) : 2u); i++)
+	                                  : std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+	for (unsigned i = 0; i < (streams ? std::min(streams, 8u) : 3u); i++)
 		s.lanes.emplace_back(new Server::LaneState());
 	s.queues.resize(count);
 	s.busy.assign(count, false);
