@@ -49,6 +49,17 @@ def stats(path: str, kernel_filter: str = ""):
                 print(f"{short[:48]:48s} {len(d):7d} {sum(d) / len(d) / 1e3:10.2f}")
             allv = [x for d in alone.values() for x in d]
             print(f"{'all matching kernels':48s} {len(allv):7d} {sum(allv) / len(allv) / 1e3:10.2f}")
+            grid_of = {(start, end): g for name, start, end, g in rows}
+            by_grid = collections.defaultdict(list)
+            reach = -1
+            for i, (start, end, short) in enumerate(iv):
+                nxt = iv[i + 1][0] if i + 1 < len(iv) else None
+                if start >= reach and (nxt is None or nxt >= end) and kernel_filter in short:
+                    by_grid[(short, grid_of[(start, end)])].append(end - start)
+                reach = max(reach, end)
+            print(f"\n{'kernel (alone)':32s} {'grid_x (threads)':>16s} {'calls':>7s} {'avg_us':>10s}")
+            for (short, grid), d in sorted(by_grid.items(), key=lambda kv: (kv[0][0], -kv[0][1])):
+                print(f"{short[:32]:32s} {grid:16d} {len(d):7d} {sum(d) / len(d) / 1e3:10.2f}")
     if grids:
         print(f"\n{'kernel':32s} {'grid_x (threads)':>16s} {'calls':>7s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s}")
         for (short, grid), d in sorted(grids.items(), key=lambda kv: (kv[0][0], -kv[0][1])):
