@@ -1461,6 +1461,54 @@ struct Pixel {
 	bool color_dirty, depth_dirty;
 };
 
+// The Z comparison terms of depth_test.h:41-144 for a fragment at depth z
+// with primitive dz against the pixel's stored depth and dz code; shared by
+// the depth test and the early rejection in shade_pixel.
+struct DepthCompare {
+	int32_t memory_z, comb_ip, comb;
+	bool farther, nearer, front, max_z;
+};
+
+DEV DepthCompare depth_compare(uint32_t depth, uint32_t dz_code, int32_t z, int32_t prim_dz)
+{
+	DepthCompare d;
+	d.memory_z = z_decompress(depth);
+	int32_t memory_dz = 1 << dz_code;
+	const int32_t precision = ((int32_t)depth >> 11) & 0xf;
+	bool coplanar = false;
+	if (precision < 3) {
+		if (memory_dz != 0x8000)
+			memory_dz = imax(memory_dz << 1, 16 >> precision);
+		else {
+			coplanar = true;
+			memory_dz = 0xffff;
+		}
+	}
+	int32_t comb = prim_dz | memory_dz;
+	if (comb != 0) comb = 1 << find_msb(comb);
+	d.comb_ip = comb;
+	d.comb = comb << 3;
+	d.farther = coplanar || ((z + d.comb) >= d.memory_z);
+	d.max_z = d.memory_z == 0x3ffff;
+	d.front = z < d.memory_z;
+	d.nearer = coplanar || ((z - d.comb) <= d.memory_z);
+	return d;
+}
+
+// True when the depth test fails whatever the fragment's coverage (which
+// alpha may still change through coverage-times-alpha or alpha-coverage
+// select): with every Z mode the failing terms do not involve coverage.  A
+// rejected fragment changes nothing, so its shading can be skipped.
+DEV bool depth_rejects(const DepthCompare &d, uint32_t z_mode)
+{
+	switch (z_mode) {
+	case 0:
+	case 1: return !d.max_z && !d.front && !d.nearer;
+	case 2: return !d.front && !d.max_z;
+	default: return !(d.farther && d.nearer && !d.max_z);
+	}
+}
+
 DEV void write_color(Pixel &p, uint32_t fmt, int32_t r, int32_t g, int32_t b, int32_t a)
 {
 	p.c[0] = r;
@@ -1683,7 +1731,8 @@ struct KnownBlend {
 
 template <class V>
 DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *tiles, const Tex &tx, uint32_t fb_fmt,
-                     const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, Shaded &out)
+                     const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, const Pixel &px,
+                     Shaded &out)
 {
 	const uint32_t f = ps->ras_flags;
 	const bool tlut = V::ras(f, RAS_TLUT), tlut_type = V::ras(f, RAS_TLUT_TYPE), sample_quad = V::ras(f, RAS_SAMPLE_QUAD);
@@ -1747,6 +1796,18 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	const int first = find_lsb((uint32_t)coverage);
 	const int yoff = first >> 1, xoff = ((first & 1) << 1) + (yoff & 1);
 
+	int32_t z;
+	{
+		int32_t zz = span.stzw[2] + ps->dstzw_dx[2] * dx;
+		int32_t sn = (zz >> 10) << 2;
+		sn += xoff * (ps->dstzw_dx[2] >> 10) + yoff * (ps->dstzw_dy[2] >> 10);
+		z = clamp_z(sn >> 5);
+	}
+	// Early depth rejection: shading, texturing, LOD and the combiner only
+	// feed a fragment the depth test is known to discard.
+	if (V::db(ps->db_flags, DB_DEPTH_TEST) && depth_rejects(depth_compare(px.depth, px.dz, z, ps->dz), V::z_mode(ps->z_mode)))
+		return false;
+
 	int32_t shade[4];
 #pragma unroll
 	for (int i = 0; i < 4; i++) {
@@ -1770,13 +1831,6 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 			stw_to_st(s + (ps->dstzw_dy[0] & ~0x7fff), t + (ps->dstzw_dy[1] & ~0x7fff), w + (ps->dstzw_dy[3] & ~0x7fff),
 			          perspective, stdy[0], stdy[1], povf);
 		}
-	}
-	int32_t z;
-	{
-		int32_t zz = span.stzw[2] + ps->dstzw_dx[2] * dx;
-		int32_t sn = (zz >> 10) << 2;
-		sn += xoff * (ps->dstzw_dx[2] >> 10) + yoff * (ps->dstzw_dy[2] >> 10);
-		z = clamp_z(sn >> 5);
 	}
 
 	uint32_t tile0 = ps->tile & 7, tile1 = (tile0 + 1) & 7;
@@ -1945,44 +1999,26 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 	bool blend_en, cvg_wrap, pass;
 	int shift0, shift1;
 	if (z_compare) {
-		int32_t memory_z = z_decompress(p.depth);
-		int32_t memory_dz = 1 << p.dz;
-		int32_t precision = ((int32_t)p.depth >> 11) & 0xf;
-		bool coplanar = false;
+		const DepthCompare d = depth_compare(p.depth, p.dz, z, ps->dz);
 		shift0 = iclamp(ps->dz_compressed - (int32_t)p.dz, 0, 4);
 		shift1 = iclamp((int32_t)p.dz - ps->dz_compressed, 0, 4);
-		if (precision < 3) {
-			if (memory_dz != 0x8000) memory_dz = imax(memory_dz << 1, 16 >> precision);
-			else {
-				coplanar = true;
-				memory_dz = 0xffff;
-			}
-		}
-		int32_t comb = ps->dz | memory_dz;
-		if (comb != 0) comb = 1 << find_msb(comb);
-		const int32_t comb_ip = comb;
-		comb <<= 3;
-		const bool farther = coplanar || ((z + comb) >= memory_z);
 		const bool overflow = (cvg + memory_coverage) >= 8;
-		blend_en = force_blend || (!overflow && aa && farther);
+		blend_en = force_blend || (!overflow && aa && d.farther);
 		cvg_wrap = overflow;
-		const bool max_z = memory_z == 0x3ffff;
-		const bool front = z < memory_z;
-		const bool nearer = coplanar || ((z - comb) <= memory_z);
 		switch (V::z_mode(ps->z_mode)) {
-		case 0: pass = max_z || (overflow ? front : nearer); break;
+		case 0: pass = d.max_z || (overflow ? d.front : d.nearer); break;
 		case 1:
-			if (!front || !farther || !overflow)
-				pass = max_z || (overflow ? front : nearer);
+			if (!d.front || !d.farther || !overflow)
+				pass = d.max_z || (overflow ? d.front : d.nearer);
 			else {
-				int32_t c = imax(find_msb(comb_ip & 0xffff), 0);
-				int32_t coeff = ((memory_z >> c) - (z >> c)) & 0xf;
+				int32_t c = imax(find_msb(d.comb_ip & 0xffff), 0);
+				int32_t coeff = ((d.memory_z >> c) - (z >> c)) & 0xf;
 				cvg = imin((coeff * cvg) >> 3, 8);
 				pass = true;
 			}
 			break;
-		case 2: pass = front || max_z; break;
-		default: pass = farther && nearer && !max_z; break;
+		case 2: pass = d.front || d.max_z; break;
+		default: pass = d.farther && d.nearer && !d.max_z; break;
 		}
 	} else {
 		shift0 = 0;
@@ -2104,7 +2140,7 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *st
 	const SpanSetup span = ldg16(sp);
 	if (!span.valid) return;
 	Shaded sh;
-	if (!shade_pixel<V>(ps, states[ps->state_index].tiles, tex, fmt, span, sp + 1, x, y, sh)) return;
+	if (!shade_pixel<V>(ps, states[ps->state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh)) return;
 	if (sh.coverage & COVERAGE_FILL)
 		fill_pixel(px, fmt, alias, ci, ps->fill_color);
 	else if (sh.coverage & COVERAGE_COPY)
