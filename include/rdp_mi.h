@@ -218,6 +218,11 @@ int rdp_mi_server_submit(rdp_mi_server *s, unsigned instance, const uint32_t *wo
 int rdp_mi_server_start(rdp_mi_server *s);
 /* Block until every submitted request has completed. */
 int rdp_mi_server_drain(rdp_mi_server *s);
+/* After a drain: forget completed requests and counters and start a new
+ * time base start_in_us microseconds from now (arrival times of later
+ * requests count from it).  The server keeps its threads, streams and
+ * arenas, so measurements after a reset see it warm. */
+int rdp_mi_server_reset(rdp_mi_server *s, double start_in_us);
 /* Completed requests in completion order: ticket, processor, arrival,
  * admission, submission (decoded and queued to the GPU) and completion times
  * (microseconds since the start).  Returns the number completed so far;
@@ -226,8 +231,9 @@ size_t rdp_mi_server_results(rdp_mi_server *s, uint64_t *tickets, uint32_t *inst
                              double *admit_us, double *submit_us, double *complete_us, size_t max);
 /* Counters so far: out[0] batches submitted, out[1] frames, out[2] frames in
  * the largest batch, out[3] largest estimated batch bytes (the admission
- * figure), out[4] largest actual packed-input plus scratch bytes of a batch. */
-void rdp_mi_server_stats(rdp_mi_server *s, uint64_t out[5]);
+ * figure), out[4] largest actual packed-input plus scratch bytes of a batch,
+ * out[5] total and out[6] longest host time spent submitting a batch (us). */
+void rdp_mi_server_stats(rdp_mi_server *s, uint64_t out[7]);
 void rdp_mi_server_destroy(rdp_mi_server *s);
 
 /* Library diagnostics. */

@@ -696,10 +696,11 @@ struct Recording {
 //     once;
 //   - decode workers run the host front end of one frame each (front ends
 //     are per processor, so frames decode in parallel) and mark it ready;
-//   - the submitter takes every ready frame (up to max_batch, within the
-//     budget) whenever a submission lane is free, and submits their passes
-//     as one batched launch set on that lane, so a batch can start on one
-//     lane while a long one still runs on the other;
+//   - each submission lane's submitter thread takes every ready frame (up
+//     to max_batch, within the budget) whenever its lane is free, and
+//     submits their passes as one batched launch set on that lane, so a
+//     batch can start on one lane while a long one still runs on another
+//     (launching kernels costs host time, so lanes submit in parallel);
 //   - one completion thread per lane waits on its batches' events in order,
 //     stamps completion times and frees the processors and the lane.
 // Results are identical to rendering each frame alone: a processor has at
@@ -797,17 +798,19 @@ struct Server {
 	};
 	struct LaneState {
 		Lane lane;
+		hipEvent_t done_ev = nullptr;  // completion of the lane's batch in flight
 		bool busy = false;
 		std::deque<Batch> inflight;
 		std::condition_variable cv;
-		std::thread completer;
+		std::thread completer, submitter;
 	};
 	std::vector<std::unique_ptr<LaneState>> lanes;
-	std::thread scheduler, submitter;
+	std::thread scheduler;
 	unsigned rr = 0;
 	// Statistics: batches, frames, largest batch, largest estimated and
 	// actual device bytes of one batch.
 	uint64_t n_batches = 0, n_frames = 0, max_frames = 0, max_est_bytes = 0, max_arena_bytes = 0;
+	double submit_time_us = 0, max_submit_us = 0;  // host time inside batched submissions
 
 	double now_us() const { return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count(); }
 
@@ -819,7 +822,7 @@ struct Server {
 	bool done() const { return stopping && outstanding == 0; }
 	void decode(ServeRequest req, size_t bytes, double admit);
 	void scheduler_loop();
-	void submitter_loop();
+	void submitter_loop(LaneState *ls);
 	void completion_loop(LaneState &ls);
 };
 
@@ -841,7 +844,7 @@ void Server::decode(ServeRequest req, size_t bytes, double admit)
 	p->fe.flush();
 	std::lock_guard<std::mutex> g(m);
 	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0 }, bytes });
-	submit_cv.notify_one();
+	submit_cv.notify_all();  // any lane's submitter may be the free one
 }
 
 void Server::scheduler_loop()
@@ -877,25 +880,15 @@ void Server::scheduler_loop()
 	}
 }
 
-void Server::submitter_loop()
+// One submitter per lane: kernel launches cost host time, so lanes submit in
+// parallel.
+void Server::submitter_loop(LaneState *ls)
 {
 	std::unique_lock<std::mutex> g(m);
 	for (;;) {
-		LaneState *ls = nullptr;
-		submit_cv.wait(g, [&] {
-			if (done()) return true;
-			if (ready.empty()) return false;
-			for (auto &l : lanes)
-				if (!l->busy) return true;
-			return false;
-		});
+		submit_cv.wait(g, [&] { return done() || (!ready.empty() && !ls->busy); });
 		if (ready.empty() && done()) break;
-		for (auto &l : lanes)
-			if (!l->busy) {
-				ls = l.get();
-				break;
-			}
-		if (!ls) continue;
+		if (ready.empty() || ls->busy) continue;
 
 		// Take ready frames in order, within max_batch and the budget.
 		std::vector<Ready> batch;
@@ -907,15 +900,17 @@ void Server::submitter_loop()
 			ready.pop_front();
 		}
 		ls->busy = true;
+		if (!ready.empty()) submit_cv.notify_all();  // frames left for the other lanes
 		g.unlock();
 
 		std::vector<Processor *> ps;
 		ps.reserve(batch.size());
 		for (auto &r : batch) ps.push_back(procs[r.r.instance]);
 		Batch b;
+		const double t_sub = now_us();
 		int err = submit_processors(ps.data(), (unsigned)ps.size(), true, &ls->lane);
-		if (!err && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess)
-			err = set_error(RDP_MI_ERROR_DEVICE, "event creation failed");
+		// A lane holds one batch at a time, so its completion event is reused.
+		b.ev = ls->done_ev;
 		if (!err && hipEventRecord(b.ev, ls->lane.stream) != hipSuccess)
 			err = set_error(RDP_MI_ERROR_DEVICE, "event record failed");
 		const double submitted = now_us();
@@ -930,6 +925,8 @@ void Server::submitter_loop()
 			error_msg = rdp_mi_last_error();
 		}
 		n_batches++;
+		submit_time_us += submitted - t_sub;
+		max_submit_us = std::max(max_submit_us, submitted - t_sub);
 		n_frames += batch.size();
 		max_frames = std::max<uint64_t>(max_frames, batch.size());
 		max_est_bytes = std::max<uint64_t>(max_est_bytes, bytes);
@@ -954,7 +951,6 @@ void Server::completion_loop(LaneState &ls)
 		g.lock();
 		Batch b = std::move(ls.inflight.front());
 		ls.inflight.pop_front();
-		if (b.ev) (void)hipEventDestroy(b.ev);
 		for (auto &r : b.reqs) {
 			r.complete_us = t;
 			results.push_back(r);
@@ -1566,14 +1562,22 @@ int rdp_mi_server_start(rdp_mi_server *h)
 		if (ln.staging.reserve(input_bytes) || ln.input.reserve(input_bytes, ln.stream) ||
 		    ln.scratch.reserve(scratch_bytes, ln.stream))
 			return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "lane arenas of %zu bytes failed", input_bytes * 2 + scratch_bytes);
+		// First use of a stream or event costs far more than later ones:
+		// create the per-round events and touch every stream now.
+		if (int err = ln.ensure_round_events(4)) return err;
+		HIP_CHECK(hipEventCreateWithFlags(&l->done_ev, hipEventDisableTiming));
+		for (hipStream_t st : { ln.stream, ln.side, ln.prep }) {
+			HIP_CHECK(hipEventRecord(l->done_ev, st));
+			HIP_CHECK(hipEventSynchronize(l->done_ev));
+		}
 	}
 	s.pool.reset(new ThreadPool(s.decode_threads));
 	s.t0 = std::chrono::steady_clock::now();
 	s.started = true;
 	s.scheduler = std::thread([&s] { s.scheduler_loop(); });
-	s.submitter = std::thread([&s] { s.submitter_loop(); });
 	for (auto &l : s.lanes) {
 		Server::LaneState *ls = l.get();
+		ls->submitter = std::thread([&s, ls] { s.submitter_loop(ls); });
 		ls->completer = std::thread([&s, ls] { s.completion_loop(*ls); });
 	}
 	return 0;
@@ -1589,6 +1593,21 @@ int rdp_mi_server_drain(rdp_mi_server *h)
 		s.cv.wait(g, [&s] { return s.outstanding == 0; });
 		if (s.error) return set_error(s.error, "%s", s.error_msg.c_str());
 	}
+	return 0;
+}
+
+int rdp_mi_server_reset(rdp_mi_server *h, double start_in_us)
+{
+	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null server");
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	if (!s.started) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "server not started");
+	if (s.outstanding != 0) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "server has requests in progress");
+	s.results.clear();
+	s.n_batches = s.n_frames = s.max_frames = s.max_est_bytes = s.max_arena_bytes = 0;
+	s.submit_time_us = s.max_submit_us = 0;
+	s.t0 = std::chrono::steady_clock::now() + std::chrono::microseconds((int64_t)std::max(0.0, start_in_us));
+	s.cv.notify_all();
 	return 0;
 }
 
@@ -1611,7 +1630,7 @@ size_t rdp_mi_server_results(rdp_mi_server *h, uint64_t *tickets, uint32_t *inst
 	return n;
 }
 
-void rdp_mi_server_stats(rdp_mi_server *h, uint64_t out[5])
+void rdp_mi_server_stats(rdp_mi_server *h, uint64_t out[7])
 {
 	if (!h || !out) return;
 	Server &s = h->s;
@@ -1621,6 +1640,8 @@ void rdp_mi_server_stats(rdp_mi_server *h, uint64_t out[5])
 	out[2] = s.max_frames;
 	out[3] = s.max_est_bytes;
 	out[4] = s.max_arena_bytes;
+	out[5] = (uint64_t)s.submit_time_us;
+	out[6] = (uint64_t)s.max_submit_us;
 }
 
 void rdp_mi_server_destroy(rdp_mi_server *h)
@@ -1636,11 +1657,15 @@ void rdp_mi_server_destroy(rdp_mi_server *h)
 		for (auto &l : s.lanes) l->cv.notify_all();
 	}
 	if (s.scheduler.joinable()) s.scheduler.join();
-	if (s.submitter.joinable()) s.submitter.join();
-	for (auto &l : s.lanes)
+	for (auto &l : s.lanes) {
+		if (l->submitter.joinable()) l->submitter.join();
 		if (l->completer.joinable()) l->completer.join();
+	}
 	s.pool.reset();
-	for (auto &l : s.lanes) l->lane.release();
+	for (auto &l : s.lanes) {
+		if (l->done_ev) (void)hipEventDestroy(l->done_ev);
+		l->lane.release();
+	}
 	delete h;
 }
 

@@ -375,6 +375,8 @@ class Server:
         lib.rdp_mi_server_results.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
                                               dp, dp, dp, dp, sz]
         lib.rdp_mi_server_stats.argtypes = [vp, vp]
+        lib.rdp_mi_server_reset.restype = ctypes.c_int
+        lib.rdp_mi_server_reset.argtypes = [vp, ctypes.c_double]
         lib.rdp_mi_server_destroy.argtypes = [vp]
         self._lib = lib
         self.processors = list(processors)
@@ -397,6 +399,11 @@ class Server:
     def drain(self):
         _check(self._lib.rdp_mi_server_drain(self._h), "rdp_mi_server_drain")
 
+    def reset(self, start_in_us: float = 0.0):
+        """After drain(): clear results and counters; arrival times of later
+        requests count from start_in_us microseconds from now."""
+        _check(self._lib.rdp_mi_server_reset(self._h, float(start_in_us)), "rdp_mi_server_reset")
+
     def results(self) -> dict:
         """Completed requests in completion order: arrays ticket, instance,
         arrival_us, admit_us, submit_us, complete_us."""
@@ -418,10 +425,10 @@ class Server:
         return {k: v[:min(n, got)] for k, v in out.items()}
 
     def stats(self) -> dict:
-        v = np.zeros(5, np.uint64)
+        v = np.zeros(7, np.uint64)
         self._lib.rdp_mi_server_stats(self._h, v.ctypes.data)
-        return dict(zip(("batches", "frames", "max_batch_frames", "max_estimated_bytes", "max_arena_bytes"),
-                        (int(x) for x in v)))
+        return dict(zip(("batches", "frames", "max_batch_frames", "max_estimated_bytes", "max_arena_bytes",
+                         "submit_us_total", "submit_us_max"), (int(x) for x in v)))
 
     def close(self):
         if self._h:
