@@ -60,3 +60,30 @@ def test_create_rejects_bad_arguments_without_touching_the_device(lib):
     assert b"power of two" in lib.rdp_mi_last_error()
     assert lib.rdp_mi_create(ctypes.byref(h), None, 0, 8 << 20, 1 << 20, 0) != 0
     assert not h.value
+
+
+def test_server_rejects_bad_arguments_without_touching_the_device(lib):
+    vp = ctypes.c_void_p
+    lib.rdp_mi_server_create.restype = ctypes.c_int
+    lib.rdp_mi_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_uint, ctypes.c_size_t, ctypes.c_uint,
+                                         ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(vp)]
+    lib.rdp_mi_server_start.restype = ctypes.c_int
+    lib.rdp_mi_server_start.argtypes = [vp]
+    lib.rdp_mi_server_reset.restype = ctypes.c_int
+    lib.rdp_mi_server_reset.argtypes = [vp, ctypes.c_double]
+    lib.rdp_mi_last_error.restype = ctypes.c_char_p
+    out = vp()
+    assert lib.rdp_mi_server_create(None, 0, 0, 0, 0, 0, ctypes.byref(out)) != 0
+    assert not out.value
+    procs = (vp * 2)(None, None)
+    assert lib.rdp_mi_server_create(procs, 2, 0, 0, 0, 0, ctypes.byref(out)) != 0
+    assert b"null processor" in lib.rdp_mi_last_error()
+    assert lib.rdp_mi_server_start(None) != 0
+    assert lib.rdp_mi_server_reset(None, 0.0) != 0
+
+
+def test_launch_config_accepts_any_shape_without_a_device(lib):
+    lib.rdp_mi_set_launch_config.argtypes = [ctypes.c_int32, ctypes.c_uint32]
+    for chunk, cls in ((-1, 0), (0, 1), (128, 2), (64, 9)):
+        lib.rdp_mi_set_launch_config(chunk, cls)
+    lib.rdp_mi_set_launch_config(64, 0)

@@ -113,3 +113,33 @@ def test_server_rejects_bad_instance():
         srv.drain()
         assert srv.results()["ticket"].size == 0
     p.close()
+
+
+def test_server_reset_keeps_serving_exact():
+    """A server reset between two halves of every instance's stream leaves
+    the results identical to the oracle's run of the whole stream."""
+    import rdp_mi
+    streams = _instances(n_fuzz=6, n_batch=3)
+    print(f"serving case reset: {len(streams)} instances", flush=True)
+    procs = [rdp_mi.CommandProcessor(rs.RDRAM_SIZE) for _ in streams]
+    for p, s in zip(procs, streams):
+        _init(p, s)
+    parts = [[] for _ in streams]
+    for inst, words in _requests(streams):
+        parts[inst].append(words)
+    with rdp_mi.Server(procs) as srv:
+        srv.start()
+        for half in (0, 1):
+            for inst, frames in enumerate(parts):
+                k = len(frames) // 2
+                for j, words in enumerate(frames[:k] if half == 0 else frames[k:]):
+                    srv.submit(inst, words, arrival_us=j * 20.0)
+            srv.drain()
+            res = srv.results()
+            assert res["ticket"].size == sum(len(f) // 2 if half == 0 else len(f) - len(f) // 2 for f in parts)
+            srv.reset(0.0)
+            assert srv.results()["ticket"].size == 0
+    for p, s in zip(procs, streams):
+        n, lines = compare((p.read_rdram(), p.read_hidden_rdram(), p.read_tmem()), run_oracle(s))
+        assert n == 0, f"{s.name}: " + "; ".join(lines)
+        p.close()
