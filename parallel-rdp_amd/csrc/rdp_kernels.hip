@@ -2152,10 +2152,10 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *st
 // Waves per SIMD each occupancy class is compiled for (the register budget
 // follows: 512 / waves, in steps of 8).
 #ifndef RDP_RASTER_WAVES_CLASS0
-#define RDP_RASTER_WAVES_CLASS0 8
+#define RDP_RASTER_WAVES_CLASS0 7
 #endif
 #ifndef RDP_RASTER_WAVES_CLASS1
-#define RDP_RASTER_WAVES_CLASS1 8
+#define RDP_RASTER_WAVES_CLASS1 7
 #endif
 #ifndef RDP_RASTER_WAVES_CLASS2
 #define RDP_RASTER_WAVES_CLASS2 6
