@@ -1848,11 +1848,19 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	}
 
 	if (V::ras(f, RAS_USES_PIPELINED_TEXEL1)) {
-		const SpanSetup next = ldg16(next_span);
+		// The next span's record is read only for the last pixel of a long
+		// span, the one case whose LOD texel comes from it.
 		const bool long_span = span.lodlength >= 8;
 		const bool end_span = x == (flip ? span.end_x : span.start_x);
-		if (end_span && long_span && next.valid) {
-			int32_t s = next.stzw[0] >> 16, t = next.stzw[1] >> 16, w = next.stzw[3] >> 16;
+		bool from_next = false;
+		int32_t s = 0, t = 0, w = 0;
+		if (end_span && long_span && next_span->valid) {
+			s = next_span->stzw[0] >> 16;
+			t = next_span->stzw[1] >> 16;
+			w = next_span->stzw[3] >> 16;
+			from_next = true;
+		}
+		if (from_next) {
 			if (perspective) {
 				bool o2 = false;
 				perspective_divide(s, t, w, st0[0], st0[1], o2);
