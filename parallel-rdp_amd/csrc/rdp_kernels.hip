@@ -734,33 +734,88 @@ __global__ __launch_bounds__(256) void k_tmem_scan(const PassDesc *__restrict__ 
 }
 
 // ==========================================================================
-// Stage 3: binning with wave64 ballots.
+// Stage 3: binning.
 // ==========================================================================
+// One wave per (pass, tile row, block of 8 bin groups).  Primitive p of a
+// group covers the tile columns [lo_p, hi_p] of the row, so its bit is on in
+// the mask of column x iff lo_p <= x < hi_p + 1: XOR the bit into a
+// difference array at lo_p and hi_p + 1, and the inclusive XOR prefix over the
+// columns gives every column's mask.  Each lane then holds one column's masks
+// of the 8 groups and writes them as one 64-byte record (the group count is
+// padded to a multiple of 8, so the records are line aligned).
+DEV uint64_t xor_prefix64(uint64_t v, int lane)
+{
+	uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t tl = __shfl_up(lo, o, 64), th = __shfl_up(hi, o, 64);
+		if (lane >= o) {
+			lo ^= tl;
+			hi ^= th;
+		}
+	}
+	return ((uint64_t)hi << 32) | lo;
+}
+
 __global__ __launch_bounds__(64) void k_bin(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
+	__shared__ unsigned long long s_diff[65];
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 14];
-	const uint32_t ty = (item >> 7) & 0x7f, group = item & 0x7f;
+	const uint32_t ty = (item >> 7) & 0x7f, gb = (item & 0x7f) * 8;
 	const int lane = threadIdx.x;
-	const uint32_t prim = group * 64 + lane;
-	uint32_t lo = 0xffff, hi = 0;
-	if (prim < P.num_prims) {
-		const PrimRecord &r = P.prims[prim];
-		if (r.yhi >= r.ylo) {
-			int row0 = r.ylo >> 3, row1 = r.yhi >> 3;
-			if ((int)ty >= row0 && (int)ty <= row1) {
-				uint32_t v = P.tilerows[r.tilerow_offset + (ty - row0)];
-				lo = v & 0xffff;
-				hi = v >> 16;
+	const uint32_t tiles_x = P.tiles_x, G = P.num_groups;
+	const uint32_t real_groups = (P.num_prims + 63) / 64;
+	const uint32_t ng = real_groups > gb ? min(real_groups - gb, 8u) : 0u;
+
+	// Column range of each of the lane's 8 primitives in this tile row
+	// (lo > hi when the primitive misses the row).
+	uint32_t lo[8], hi[8];
+#pragma unroll
+	for (uint32_t k = 0; k < 8; k++) {
+		lo[k] = 0xffff;
+		hi[k] = 0;
+		const uint32_t prim = (gb + k) * 64 + (uint32_t)lane;
+		if (k < ng && prim < P.num_prims) {
+			const PrimRecord &r = P.prims[prim];
+			if (r.yhi >= r.ylo) {
+				const int row0 = r.ylo >> 3, row1 = r.yhi >> 3;
+				if ((int)ty >= row0 && (int)ty <= row1) {
+					const uint32_t v = P.tilerows[r.tilerow_offset + (ty - row0)];
+					lo[k] = v & 0xffff;
+					hi[k] = v >> 16;
+				}
 			}
 		}
 	}
-	const uint32_t tiles_x = P.tiles_x, G = P.num_groups;
-	uint64_t *out = P.bin_fine + (size_t)ty * tiles_x * G + group;
-	for (uint32_t tx = 0; tx < tiles_x; tx++) {
-		uint64_t mask = __ballot(lo <= tx && tx <= hi);
-		if (lane == 0)
-			out[(size_t)tx * G] = mask;
+
+	uint64_t *out = P.bin_fine + (size_t)ty * tiles_x * G + gb;
+	for (uint32_t x0 = 0; x0 < tiles_x; x0 += 64) {
+		uint64_t m[8];
+#pragma unroll
+		for (uint32_t k = 0; k < 8; k++) {
+			m[k] = 0;
+			if (k >= ng) continue;
+			s_diff[lane] = 0;
+			if (lane == 0) s_diff[64] = 0;
+			__syncthreads();
+			// The part of [lo, hi] inside columns x0 .. x0 + 63.
+			const uint32_t a = max(lo[k], x0), b = min(hi[k], x0 + 63);
+			if (lo[k] <= hi[k] && a <= b) {
+				const unsigned long long bit = 1ull << lane;
+				atomicXor(&s_diff[a - x0], bit);
+				atomicXor(&s_diff[b + 1 - x0], bit);
+			}
+			__syncthreads();
+			m[k] = xor_prefix64(s_diff[lane], lane);
+			__syncthreads();
+		}
+		const uint32_t x = x0 + (uint32_t)lane;
+		if (x < tiles_x) {
+			uint64_t *o = out + (size_t)x * G;
+#pragma unroll
+			for (uint32_t k = 0; k < 8; k++) o[k] = m[k];
+		}
 	}
 }
 

@@ -252,11 +252,18 @@ struct ScratchLayout {
 	size_t setups, spans, tilerows, bins, tmem, tmem_blocks, total;
 };
 
+// Bin groups of a pass: one per 64 primitives, padded to a multiple of 8 so
+// the binning stage writes each tile's masks as whole 64-byte records.
+static uint32_t bin_groups(size_t num_prims)
+{
+	return (((uint32_t)num_prims + 63) / 64 + 7) & ~7u;
+}
+
 static ScratchLayout scratch_layout(const HostPass &hp)
 {
 	ScratchLayout l;
 	const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
-	const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
+	const uint32_t groups = bin_groups(hp.prims.size());
 	l.setups = align_up(hp.prims.size() * sizeof(RasterPrim), 256);
 	l.spans = align_up((size_t)hp.num_spans * sizeof(SpanSetup), 256);
 	l.tilerows = align_up((size_t)hp.num_tilerows * 4, 256);
@@ -436,12 +443,12 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
 			const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
-			const uint32_t groups = ((uint32_t)hp.prims.size() + 63) / 64;
+			const uint32_t groups = bin_groups(hp.prims.size());
 			L.n_setup += (uint32_t)hp.prims.size();
 			L.n_tmem += 8;
 			L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 31) / 32);
 			if (!hp.prims.empty()) {
-				L.n_bin += ty * groups;
+				L.n_bin += ty * (groups / 8);
 				L.n_raster += tx * ty;
 				L.n_raster_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
 			}
@@ -563,7 +570,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.tiles_y = (hp.fb_height + 7) / 8;
 			D.num_spans = hp.num_spans;
 			D.num_tilerows = hp.num_tilerows;
-			D.num_groups = ((uint32_t)hp.prims.size() + 63) / 64;
+			D.num_groups = bin_groups(hp.prims.size());
 
 			const uint32_t pidx = (uint32_t)i;
 			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 13) | p;
@@ -572,7 +579,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 				for (uint32_t k = 0; k < 8; k++) it_tmem[ntg++] = (pidx << 12) | (b << 3) | k;
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)
-					for (uint32_t g = 0; g < D.num_groups; g++) it_bin[nb++] = (pidx << 14) | (y << 7) | g;
+					for (uint32_t g = 0; g < D.num_groups / 8; g++) it_bin[nb++] = (pidx << 14) | (y << 7) | g;
 			}
 		}
 		// Raster items, per class, passes with the most primitives per tile

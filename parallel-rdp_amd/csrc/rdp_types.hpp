@@ -405,7 +405,7 @@ struct PassDesc {
 	uint32_t tiles_x, tiles_y;
 	uint32_t num_spans;
 	uint32_t num_tilerows;
-	uint32_t num_groups;       // ceil(num_prims / 64)
+	uint32_t num_groups;       // ceil(num_prims / 64) rounded up to a multiple of 8
 	uint32_t pad[2];
 };
 
