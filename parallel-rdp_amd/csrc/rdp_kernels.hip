@@ -135,7 +135,8 @@ DEV int normalize_dzpix(int dz)
 // CommandProcessor decoders, rdp_device.cpp:268-344 and 713-821, plus
 // Renderer::fixup_triangle_setup (rdp_renderer.cpp:1402-1423) and the
 // primitive-depth override (rdp_renderer.cpp:1489-1497).
-DEV void decode_primitive(const AS_CONST uint32_t *w, uint32_t op, const AS_CONST DrawState &st, PrimSetup &s)
+template <typename WordPtr, typename State>
+DEV void decode_primitive(WordPtr w, uint32_t op, const State &st, PrimSetup &s)
 {
 	const uint32_t raw = st.raw_ras_flags;
 	for (int i = 0; i < 4; i++) {
@@ -162,7 +163,7 @@ DEV void decode_primitive(const AS_CONST uint32_t *w, uint32_t op, const AS_CONS
 		s.dxldy = sext((int32_t)(w[3] >> 2), 28) >> 1;
 		s.dxhdy = sext((int32_t)(w[5] >> 2), 28) >> 1;
 		s.dxmdy = sext((int32_t)(w[7] >> 2), 28) >> 1;
-		const AS_CONST uint32_t *p = w + 8;
+		WordPtr p = w + 8;
 		if (op & 4) {
 			for (int i = 0; i < 2; i++) {
 				s.rgba[2 * i + 0] = (int32_t)((p[i] & 0xffff0000u) | ((p[4 + i] >> 16) & 0xffff));
@@ -262,7 +263,8 @@ DEV void decode_primitive(const AS_CONST uint32_t *w, uint32_t op, const AS_CONS
 DEV int32_t quantize_x(int32_t x) { return (x >> 12) | ((x & 0xfff) != 0); }
 
 // span_setup.comp:87-227 for one scanline (no upscaling).
-DEV void compute_span(const PrimSetup &s, const AS_CONST uint32_t *sc, int y, SpanSetup &out)
+template <typename Scissor>
+DEV void compute_span(const PrimSetup &s, Scissor sc, int y, SpanSetup &out)
 {
 	const bool flip = (s.flags & TRI_FLIP) != 0;
 	{
@@ -337,97 +339,114 @@ DEV void compute_span(const PrimSetup &s, const AS_CONST uint32_t *sc, int y, Sp
 	out.lodlength = (int16_t)(flip ? (out.end_x - out.base_x) : (out.base_x - out.start_x));
 }
 
+// Setup runs in two launches.  k_setup: one lane per primitive decodes its
+// command words and draw state, writes the RasterPrim record and the decoded
+// PrimSetup, and marks which primitive owns each of its tile-row entries.
+// k_span: one wave per 8 tile-row entries, one lane per span row: the rows
+// of small triangles pack densely into waves and a tall triangle spreads over
+// many waves.
 __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
-	const PassDesc &P = passes[item >> 13];
-	const uint32_t prim = item & 0x1fff;
-	// Everything per primitive is wave-uniform: read it through the constant
-	// address space so it lives in scalar registers.
-	const AS_CONST PrimRecord &rec = cptr(P.prims)[prim];
-	const AS_CONST DrawState &st = cptr(P.states)[rec.state_index];
-	const int lane = threadIdx.x;
+	const PassDesc &P = passes[item >> 7];
+	const uint32_t prim = (item & 0x7f) * 64 + threadIdx.x;
+	if (prim >= P.num_prims) return;
 
+	const PrimRecord &rec = P.prims[prim];
+	const DrawState &st = P.states[rec.state_index];
 	PrimSetup s;
-	decode_primitive(cptr(P.words) + rec.word_offset, rec.opcode, st, s);
+	decode_primitive(P.words + rec.word_offset, rec.opcode, st, s);
 	s.state_index = rec.state_index;
 	s.tmem_instance = rec.tmem_instance;
 	s.prim_index = rec.prim_index;
 	s.ylo = rec.ylo;
 	s.yhi = rec.yhi;
 	s.span_offset = rec.span_offset;
-	s.pad = 0;
-	if (lane == 0) {
-		RasterPrim r;
-		for (int i = 0; i < 4; i++) {
-			r.drgba_dx[i] = s.drgba_dx[i];
-			r.drgba_dy[i] = s.drgba_dy[i];
-			r.dstzw_dx[i] = s.dstzw_dx[i];
-			r.dstzw_dy[i] = s.dstzw_dy[i];
-			r.factors[i] = st.factors[i];
-			r.fog[i] = st.fog[i];
-			r.blendc[i] = st.blendc[i];
-		}
-		r.flags = s.flags;
-		r.tile = s.tile;
-		r.ylo = s.ylo;
-		r.yhi = s.yhi;
-		r.span_offset = s.span_offset;
-		r.tmem_instance = s.tmem_instance;
-		r.prim_index = s.prim_index;
-		r.state_index = s.state_index;
-		r.dz = s.dz;
-		r.dz_compressed = s.dz_compressed;
-		r.ras_flags = st.ras_flags;
-		r.dither = st.dither;
-		r.db_flags = st.db_flags;
-		r.coverage_mode = st.coverage_mode;
-		r.z_mode = st.z_mode;
-		r.fill_color = st.fill_color;
-		r.min_lod = st.min_lod;
-		r.variant = rec.variant;
-		r.pad0[0] = r.pad0[1] = 0;
-		for (int c = 0; c < 2; c++) {
-			for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
-			for (int i = 0; i < 4; i++) {
-				r.blend[c][i] = st.blend[c][i];
-				for (int k = 0; k < 4; k++) r.constants[c][i][k] = st.constants[c][i][k];
-			}
-		}
-		for (int i = 0; i < 8; i++) r.pad1[i] = 0;
-		P.rprims[prim] = r;
-	}
+	s.tilerow_offset = rec.tilerow_offset;
 
-	const int ylo = rec.ylo, yhi = rec.yhi;
-	const int row0 = ylo >> 3, row1 = yhi >> 3;
-	const int fbw = (int)P.fb_width;
-	for (int yb = ylo & ~7; yb <= yhi + 1; yb += 64) {
-		const int y = yb + lane;
-		int tlo = 0x7fffffff, thi = -1;
-		if (y >= ylo && y <= yhi + 1) {
+	RasterPrim r;
+	for (int i = 0; i < 4; i++) {
+		r.drgba_dx[i] = s.drgba_dx[i];
+		r.drgba_dy[i] = s.drgba_dy[i];
+		r.dstzw_dx[i] = s.dstzw_dx[i];
+		r.dstzw_dy[i] = s.dstzw_dy[i];
+		r.factors[i] = st.factors[i];
+		r.fog[i] = st.fog[i];
+		r.blendc[i] = st.blendc[i];
+	}
+	r.flags = s.flags;
+	r.tile = s.tile;
+	r.ylo = s.ylo;
+	r.yhi = s.yhi;
+	r.span_offset = s.span_offset;
+	r.tmem_instance = s.tmem_instance;
+	r.prim_index = s.prim_index;
+	r.state_index = s.state_index;
+	r.dz = s.dz;
+	r.dz_compressed = s.dz_compressed;
+	r.ras_flags = st.ras_flags;
+	r.dither = st.dither;
+	r.db_flags = st.db_flags;
+	r.coverage_mode = st.coverage_mode;
+	r.z_mode = st.z_mode;
+	r.fill_color = st.fill_color;
+	r.min_lod = st.min_lod;
+	r.variant = rec.variant;
+	r.pad0[0] = r.pad0[1] = 0;
+	for (int c = 0; c < 2; c++) {
+		for (int i = 0; i < 8; i++) r.comb[c][i] = st.comb[c][i];
+		for (int i = 0; i < 4; i++) {
+			r.blend[c][i] = st.blend[c][i];
+			for (int k = 0; k < 4; k++) r.constants[c][i][k] = st.constants[c][i][k];
+		}
+	}
+	for (int i = 0; i < 8; i++) r.pad1[i] = 0;
+	P.rprims[prim] = r;
+	P.psetups[prim] = s;
+
+	// Spans cover rows ylo .. yhi + 1 and k_span covers the tile rows
+	// ylo / 8 .. yhi / 8; row yhi + 1 lies beyond them when it starts a new
+	// tile row, and is done here.
+	const int rows = (s.yhi >> 3) - (s.ylo >> 3) + 1;
+	for (int k = 0; k < rows; k++) P.tilerow_prim[s.tilerow_offset + (uint32_t)k] = (uint16_t)prim;
+	if (((s.yhi + 1) & 7) == 0) {
+		SpanSetup sp;
+		compute_span(s, st.scissor, s.yhi + 1, sp);
+		P.spans[s.span_offset + (uint32_t)(s.yhi + 1 - s.ylo)] = sp;
+	}
+}
+
+__global__ __launch_bounds__(64) void k_span(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 17];
+	const int lane = threadIdx.x;
+	const uint32_t e = (item & 0x1ffff) * 8 + (uint32_t)(lane >> 3);
+	const bool live = e < P.num_tilerows;
+	int tlo = 0x7fffffff, thi = -1;
+	if (live) {
+		const PrimSetup &s = P.psetups[P.tilerow_prim[e]];
+		const int y = ((s.ylo >> 3) + (int)(e - s.tilerow_offset)) * 8 + (lane & 7);
+		if (y >= s.ylo && y <= s.yhi + 1) {
 			SpanSetup sp;
-			compute_span(s, st.scissor, y, sp);
-			P.spans[rec.span_offset + (y - ylo)] = sp;
-			if (y <= yhi && sp.valid) {
-				int lo = imax(sp.start_x, 0), hi = imin(sp.end_x, fbw - 1);
+			compute_span(s, P.states[s.state_index].scissor, y, sp);
+			P.spans[s.span_offset + (uint32_t)(y - s.ylo)] = sp;
+			if (y <= s.yhi && sp.valid) {
+				const int lo = imax(sp.start_x, 0), hi = imin(sp.end_x, (int)P.fb_width - 1);
 				if (lo <= hi) {
 					tlo = lo >> 3;
 					thi = hi >> 3;
 				}
 			}
 		}
-		// Reduce over the 8 lanes of one tile row.
-#pragma unroll
-		for (int m = 1; m < 8; m <<= 1) {
-			tlo = imin(tlo, __shfl_xor(tlo, m, 64));
-			thi = imax(thi, __shfl_xor(thi, m, 64));
-		}
-		const int row = (yb >> 3) + (lane >> 3);
-		if ((lane & 7) == 0 && row >= row0 && row <= row1) {
-			uint32_t v = thi >= tlo ? ((uint32_t)tlo | ((uint32_t)thi << 16)) : 0xffffu;
-			P.tilerows[rec.tilerow_offset + (row - row0)] = v;
-		}
 	}
+	// Reduce over the 8 lanes of one tile row.
+#pragma unroll
+	for (int m = 1; m < 8; m <<= 1) {
+		tlo = imin(tlo, __shfl_xor(tlo, m, 64));
+		thi = imax(thi, __shfl_xor(thi, m, 64));
+	}
+	if (live && (lane & 7) == 0) P.tilerows[e] = thi >= tlo ? ((uint32_t)tlo | ((uint32_t)thi << 16)) : 0xffffu;
 }
 
 // ==========================================================================
@@ -2354,13 +2373,89 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 	}
 }
 
+// Passes whose every primitive is a FILL-mode primitive (variant 1, the
+// first row of the variant table).  A fill writes every channel of a pixel
+// it covers and reads neither colour nor depth, so the tile's framebuffer is
+// not loaded; a wave takes kFillTiles horizontally adjacent tiles, reads
+// their bin masks up front and runs them back to back.
+constexpr uint32_t kFillTiles = kFillTilesPerItem;
+using FillVariant = RasterVariant<RAS_FILL | RAS_COPY, RAS_FILL, (int)COMB_ANY, TEXFMT_ANY, (int)DBP_ANY>;
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
+{
+	__shared__ uint16_t s_unused[2];
+	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)];
+	const PassDesc &P = passes[item >> 14];
+	const uint32_t tx0 = ((item >> 7) & 0x7f) * kFillTiles, ty = item & 0x7f;
+	const int lane = threadIdx.x;
+	const uint32_t fmt = P.fb_fmt;
+	const bool alias = P.fb_addr == P.depth_addr;
+	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
+	const AS_CONST RasterPrim *rprims = cptr(P.rprims);
+	const AS_CONST DrawState *states = cptr(P.states);
+	const AS_GLOBAL SpanSetup *spans = gptr(P.spans);
+	const AS_GLOBAL uint8_t *lut = gptr(P.blender_lut);
+	const AS_GLOBAL uint64_t *bins = gptr(P.bin_fine);
+	const Tex tex{ (const AS_LDS uint16_t *)(s_unused) };
+	const uint32_t G = P.num_groups, tiles_x = P.tiles_x;
+	const uint32_t y = ty * 8 + ((uint32_t)lane >> 3);
+
+	uint64_t first[kFillTiles];
+#pragma unroll
+	for (uint32_t t = 0; t < kFillTiles; t++)
+		first[t] = tx0 + t < tiles_x && (uint32_t)lane < G ? bins[(size_t)(ty * tiles_x + tx0 + t) * G + lane] : 0;
+
+#pragma unroll
+	for (uint32_t t = 0; t < kFillTiles; t++) {
+		const uint32_t tx = tx0 + t;
+		if (tx >= tiles_x) break;
+		const uint32_t x = tx * 8 + ((uint32_t)lane & 7);
+		const bool in_fb = x < P.fb_width && y < P.fb_height;
+		uint32_t addr_index;
+		switch (fmt) {
+		case FB_I4:
+		case FB_I8: addr_index = P.fb_addr; break;
+		case FB_RGBA5551:
+		case FB_IA88: addr_index = P.fb_addr >> 1; break;
+		default: addr_index = P.fb_addr >> 2; break;
+		}
+		const uint32_t ci = addr_index + P.fb_width * y + x;
+		Pixel px;
+		px.c[0] = px.c[1] = px.c[2] = px.c[3] = 0;
+		px.depth = px.dz = 0;
+		px.color_dirty = px.depth_dirty = false;
+		for (uint32_t gbase = 0; gbase < G; gbase += 64) {
+			const uint64_t my_mask =
+			    gbase == 0 ? first[t] : (gbase + (uint32_t)lane < G ? bins[(size_t)(ty * tiles_x + tx) * G + gbase + lane] : 0);
+			uint64_t coarse = __ballot(my_mask != 0);
+			while (coarse) {
+				const int g = __ffsll((unsigned long long)coarse) - 1;
+				coarse &= coarse - 1;
+				const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)my_mask, g);
+				const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(my_mask >> 32), g);
+				uint64_t mask = ((uint64_t)hi << 32) | lo;
+				while (mask) {
+					const int bit = __ffsll((unsigned long long)mask) - 1;
+					mask &= mask - 1;
+					const AS_CONST RasterPrim *ps = rprims + ((gbase + (uint32_t)g) * 64 + (uint32_t)bit);
+					if (!in_fb || (int)y < ps->ylo || (int)y > ps->yhi) continue;
+					raster_prim<FillVariant>(ps, states, spans, tex, lut, fmt, alias, ci, (int)x, (int)y, px);
+				}
+			}
+		}
+		if (in_fb && px.color_dirty) store_color(m, fmt, ci, px);
+	}
+}
+
 // ==========================================================================
 // Launchers (host side).
 // ==========================================================================
-hipError_t launch_setup(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+hipError_t launch_setup(const PassDesc *passes, const uint32_t *items, uint32_t n, const uint32_t *span_items,
+                        uint32_t n_span, hipStream_t s)
 {
-	if (!n) return hipSuccess;
-	hipLaunchKernelGGL(k_setup, dim3(n), dim3(64), 0, s, passes, items);
+	if (n) hipLaunchKernelGGL(k_setup, dim3(n), dim3(64), 0, s, passes, items);
+	if (n_span) hipLaunchKernelGGL(k_span, dim3(n_span), dim3(64), 0, s, passes, span_items);
 	return hipGetLastError();
 }
 
@@ -2376,6 +2471,13 @@ hipError_t launch_bin(const PassDesc *passes, const uint32_t *items, uint32_t n,
 {
 	if (!n) return hipSuccess;
 	hipLaunchKernelGGL(k_bin, dim3(n), dim3(64), 0, s, passes, items);
+	return hipGetLastError();
+}
+
+hipError_t launch_raster_fill(const PassDesc *passes, const uint32_t *items, uint32_t n, int32_t xcd_chunk, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_raster_fill, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk);
 	return hipGetLastError();
 }
 

@@ -33,10 +33,11 @@
 
 namespace rdp_mi {
 
-hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
+hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+hipError_t launch_raster_fill(const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 
 // Launch shape of the raster stage: the XCD chunk of the tile-to-block
 // mapping and the lowest occupancy class a pass runs in (passes below it are
@@ -249,7 +250,7 @@ struct PassRef {
 
 // Byte sizes of the per-pass device scratch.
 struct ScratchLayout {
-	size_t setups, spans, tilerows, bins, tmem, tmem_blocks, total;
+	size_t setups, spans, tilerows, bins, tmem, tmem_blocks, psetups, tilerow_prim, total;
 };
 
 // Bin groups of a pass: one per 64 primitives, padded to a multiple of 8 so
@@ -270,7 +271,9 @@ static ScratchLayout scratch_layout(const HostPass &hp)
 	l.bins = align_up((size_t)tx * ty * groups * 8, 256);
 	l.tmem = align_up((hp.uploads.size() + 1) * 4096, 256);
 	l.tmem_blocks = align_up(std::max<size_t>((hp.uploads.size() + 31) / 32, 1) * 2048 * 4, 256);
-	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem + l.tmem_blocks;
+	l.psetups = align_up(hp.prims.size() * sizeof(PrimSetup), 256);
+	l.tilerow_prim = align_up((size_t)hp.num_tilerows * 2, 256);
+	l.total = l.setups + l.spans + l.tilerows + l.bins + l.tmem + l.tmem_blocks + l.psetups + l.tilerow_prim;
 	return l;
 }
 
@@ -282,12 +285,24 @@ struct Launch {
 	uint32_t n[4];
 	const uint32_t *tmem_scan_items;
 	uint32_t tmem_scan_n;
+	const uint32_t *span_items;  // second setup launch: one item per 8 tile-row entries
+	uint32_t span_n;
 	const uint32_t *raster_items[kRasterClasses];
 	uint32_t raster_n[kRasterClasses];
+	const uint32_t *fill_items;  // passes of FILL primitives only: k_raster_fill, 4 tiles per item
+	uint32_t fill_n;
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
 	uint32_t passes;
 };
+
+// A pass whose every primitive is a FILL-mode primitive.
+static bool fill_only(const HostPass &hp)
+{
+	for (const PrimRecord &p : hp.prims)
+		if (p.variant != kFillVariant) return false;
+	return !hp.prims.empty();
+}
 
 static uint64_t framebuffer_bytes(const HostPass &hp)
 {
@@ -301,7 +316,10 @@ static uint64_t framebuffer_bytes(const HostPass &hp)
 	}
 	const uint64_t depth = hp.fb_addr == hp.depth_addr ? 0 : 3;
 	const uint64_t tiles = (uint64_t)((hp.fb_width + 7) / 8) * ((hp.fb_height + 7) / 8);
-	return hp.prims.empty() ? 0 : 2 * tiles * 64 * (color + depth);
+	if (hp.prims.empty()) return 0;
+	// A fill-only pass writes colour and reads nothing.
+	if (fill_only(hp)) return tiles * 64 * color;
+	return 2 * tiles * 64 * (color + depth);
 }
 
 // Launches every round on the lane's stream.  In submissions with a round of
@@ -335,7 +353,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		HIP_CHECK(hipStreamWaitEvent(prep, lane.start, 0));
 		for (size_t r = 0; r < launches.size(); r++) {
 			const Launch &L = launches[r];
-			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], prep));
+			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], L.span_items, L.span_n, prep));
 			HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], prep));
 			HIP_CHECK(hipEventRecord(lane.events[r].prep_done, prep));
 		}
@@ -363,7 +381,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
 			HIP_CHECK(hipStreamWaitEvent(s, E.prep_done, 0));
 		} else {
-			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], s));
+			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], L.span_items, L.span_n, s));
 			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 0], s));
 			if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
 			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 1], s));
@@ -385,10 +403,12 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, side));
 			HIP_CHECK(hipEventRecord(E.join, side));
 			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
+			HIP_CHECK(launch_raster_fill(L.descs, L.fill_items, L.fill_n, cfg.xcd_chunk, s));
 			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
 			HIP_CHECK(hipStreamWaitEvent(s, E.join, 0));
 		} else {
 			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
+			HIP_CHECK(launch_raster_fill(L.descs, L.fill_items, L.fill_n, cfg.xcd_chunk, s));
 			// Class segments are contiguous in class order, so promoting the
 			// classes below min_class merges them into one launch.
 			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
@@ -415,7 +435,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	// Input layout: per round [PassDesc x n][items: setup|tmem|bin|raster], then per pass data.
 	struct RoundLayout {
 		size_t descs, items_setup, items_tmem, items_bin, items_raster;
-		uint32_t n_setup, n_tmem, n_tmem_gather, n_bin, n_raster;
+		uint32_t n_setup, n_span, n_tmem, n_tmem_gather, n_bin, n_raster, n_fill;
 		uint32_t n_raster_class[kRasterClasses];
 		size_t scratch;
 	};
@@ -424,6 +444,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		size_t words, states, prims, uploads;
 		ScratchLayout sl;
 		size_t scratch_base;
+		bool fill;  // FILL primitives only (k_raster_fill)
 	};
 	std::vector<std::vector<PassInputs>> pl(rounds.size());
 
@@ -435,8 +456,11 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	for (const auto &round : rounds) prep_ahead = prep_ahead || round.size() >= kOverlapMinPasses;
 	size_t off = 0, max_scratch = 0, scratch_base = 0;
 	for (size_t r = 0; r < rounds.size(); r++) {
+		// Work items carry the pass index above 17 bits of per-pass index.
+		if (rounds[r].size() > (1u << 15))
+			return set_error(RDP_MI_ERROR_UNSUPPORTED, "%zu passes in one round (at most 32768)", rounds[r].size());
 		RoundLayout &L = rl[r];
-		L.n_setup = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = 0;
+		L.n_setup = L.n_span = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = L.n_fill = 0;
 		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = 0;
 		size_t scr = scratch_base;
 		pl[r].resize(rounds[r].size());
@@ -444,15 +468,21 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			const HostPass &hp = *rounds[r][i].pass;
 			const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
 			const uint32_t groups = bin_groups(hp.prims.size());
-			L.n_setup += (uint32_t)hp.prims.size();
+			L.n_setup += (uint32_t)(hp.prims.size() + 63) / 64;
+			L.n_span += (hp.num_tilerows + 7) / 8;
 			L.n_tmem += 8;
 			L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 31) / 32);
+			PassInputs &pi = pl[r][i];
+			pi.fill = fill_only(hp);
 			if (!hp.prims.empty()) {
 				L.n_bin += ty * (groups / 8);
-				L.n_raster += tx * ty;
-				L.n_raster_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
+				if (pi.fill) {
+					L.n_fill += ty * ((tx + kFillTilesPerItem - 1) / kFillTilesPerItem);
+				} else {
+					L.n_raster += tx * ty;
+					L.n_raster_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
+				}
 			}
-			PassInputs &pi = pl[r][i];
 			pi.sl = scratch_layout(hp);
 			pi.scratch_base = scr;
 			scr += pi.sl.total;
@@ -463,13 +493,13 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		L.descs = off;
 		off = align_up(off + rounds[r].size() * sizeof(PassDesc), 256);
 		L.items_setup = off;
-		off = align_up(off + L.n_setup * 4ull, 256);
+		off = align_up(off + (L.n_setup + L.n_span) * 4ull, 256);
 		L.items_tmem = off;
 		off = align_up(off + (L.n_tmem + L.n_tmem_gather) * 4ull, 256);
 		L.items_bin = off;
 		off = align_up(off + L.n_bin * 4ull, 256);
 		L.items_raster = off;
-		off = align_up(off + L.n_raster * 4ull, 256);
+		off = align_up(off + (L.n_raster + L.n_fill) * 4ull, 256);
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
 			PassInputs &pi = pl[r][i];
@@ -503,7 +533,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		uint32_t *it_tmem = reinterpret_cast<uint32_t *>(h + L.items_tmem);
 		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
-		uint32_t ns = 0, nt = 0, ntg = 0, nb = 0;
+		uint32_t ns = 0, nsp = 0, nt = 0, ntg = 0, nb = 0;
 		uint32_t nr[kRasterClasses], rbase[kRasterClasses];
 		for (uint32_t c = 0, acc = 0; c < kRasterClasses; c++) {
 			nr[c] = 0;
@@ -516,12 +546,16 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		LA.items[1] = reinterpret_cast<const uint32_t *>(d + L.items_tmem);
 		LA.tmem_scan_items = LA.items[1] + L.n_tmem_gather;
 		LA.tmem_scan_n = L.n_tmem;
+		LA.span_items = LA.items[0] + L.n_setup;
+		LA.span_n = L.n_span;
 		LA.items[2] = reinterpret_cast<const uint32_t *>(d + L.items_bin);
 		LA.items[3] = reinterpret_cast<const uint32_t *>(d + L.items_raster);
 		LA.n[0] = L.n_setup;
 		LA.n[1] = L.n_tmem_gather;
 		LA.n[2] = L.n_bin;
 		LA.n[3] = L.n_raster;
+		LA.fill_items = LA.items[3] + L.n_raster;
+		LA.fill_n = L.n_fill;
 		LA.prep_ahead = prep_ahead;
 		for (uint32_t c = 0; c < kRasterClasses; c++) {
 			LA.raster_items[c] = LA.items[3] + rbase[c];
@@ -554,6 +588,11 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.tmem_blocks = reinterpret_cast<uint32_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins +
 			                                             pi.sl.tmem);
 			D.tmem_instances = reinterpret_cast<uint16_t *>(sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins);
+			{
+				uint8_t *tail = sb + pi.sl.setups + pi.sl.spans + pi.sl.tilerows + pi.sl.bins + pi.sl.tmem + pi.sl.tmem_blocks;
+				D.psetups = reinterpret_cast<PrimSetup *>(tail);
+				D.tilerow_prim = reinterpret_cast<uint16_t *>(tail + pi.sl.psetups);
+			}
 			D.rdram = pr.d_rdram;
 			D.hidden = pr.d_hidden;
 			D.tmem = pr.d_tmem;
@@ -573,7 +612,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.num_groups = bin_groups(hp.prims.size());
 
 			const uint32_t pidx = (uint32_t)i;
-			for (uint32_t p = 0; p < D.num_prims; p++) it_setup[ns++] = (pidx << 13) | p;
+			for (uint32_t g = 0; g < (D.num_prims + 63) / 64; g++) it_setup[ns++] = (pidx << 7) | g;
+			for (uint32_t b = 0; b < (D.num_tilerows + 7) / 8; b++) it_setup[L.n_setup + nsp++] = (pidx << 17) | b;
 			for (uint32_t k = 0; k < 8; k++) it_tmem[L.n_tmem_gather + nt++] = (pidx << 12) | k;
 			for (uint32_t b = 0; b < (uint32_t)(hp.uploads.size() + 31) / 32; b++)
 				for (uint32_t k = 0; k < 8; k++) it_tmem[ntg++] = (pidx << 12) | (b << 3) | k;
@@ -593,10 +633,18 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			return tiles ? (double)hp.prims.size() / (double)tiles : 0.0;
 		};
 		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return density(a) > density(b); });
+		uint32_t nf = 0;
 		for (uint32_t pidx : order) {
 			const HostPass &hp = *rounds[r][pidx].pass;
 			if (hp.prims.empty()) continue;
 			const uint32_t tiles_x = (hp.fb_width + 7) / 8, tiles_y = (hp.fb_height + 7) / 8;
+			if (pl[r][pidx].fill) {
+				uint32_t *out = it_raster + L.n_raster;
+				for (uint32_t y = 0; y < tiles_y; y++)
+					for (uint32_t x = 0; x < (tiles_x + kFillTilesPerItem - 1) / kFillTilesPerItem; x++)
+						out[nf++] = (pidx << 14) | (x << 7) | y;
+				continue;
+			}
 			const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
 			uint32_t *out = it_raster + rbase[cls];
 			for (uint32_t y = 0; y < tiles_y; y++)
@@ -1487,6 +1535,7 @@ unsigned rdp_mi_recording_raster_launches(const rdp_mi_recording *h)
 		uint32_t merged = 0;
 		for (uint32_t c = 0; c <= lo; c++) merged += L.raster_n[c];
 		n += merged != 0;
+		n += L.fill_n != 0;
 		for (uint32_t c = lo + 1; c < kRasterClasses; c++) n += L.raster_n[c] != 0;
 	}
 	return n;

@@ -213,6 +213,10 @@ constexpr uint32_t RAS_2CYC_SPRITE_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RA
 	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI4, DBP_AA_SPRITE, 1)             \
 	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, DBP_ANY, 2)
 constexpr uint32_t kRasterClasses = 3;  // the generic variant (id 0) is in the top class
+// Variant 1 is the FILL-mode row.  A pass whose every primitive uses it reads
+// nothing from the framebuffer and is rasterized by its own light kernel.
+constexpr uint32_t kFillVariant = 1;
+constexpr uint32_t kFillTilesPerItem = 4;  // horizontally adjacent tiles per k_raster_fill wave
 
 // The format every tile a primitive can sample shares, or TEXFMT_ANY.
 // Without texture LOD the sampled tiles are base (texel 0, pipelined texel 1)
@@ -318,8 +322,9 @@ struct PrimSetup {
 	uint32_t prim_index;
 	int32_t ylo, yhi;
 	uint32_t span_offset;
-	uint32_t pad;
+	uint32_t tilerow_offset;
 };
+static_assert(sizeof(PrimSetup) % 16 == 0, "PrimSetup layout");
 
 // Everything the per-tile kernel needs about one primitive, 256 bytes, read
 // with scalar loads: the per-pixel attribute gradients, the span range and
@@ -393,6 +398,8 @@ struct PassDesc {
 	uint64_t *bin_fine;        // [tile][group] primitive masks
 	uint32_t *tmem_blocks;     // [block of 32 uploads][2048]: first write in block << 16 | value after it
 	uint16_t *tmem_instances;  // (num_uploads + 1) x 2048 halfwords
+	PrimSetup *psetups;        // decoded edge and attribute setup per primitive
+	uint16_t *tilerow_prim;    // per tile-row entry: the primitive it belongs to
 	// stream memory
 	uint8_t *rdram;
 	uint8_t *hidden;
@@ -406,7 +413,6 @@ struct PassDesc {
 	uint32_t num_spans;
 	uint32_t num_tilerows;
 	uint32_t num_groups;       // ceil(num_prims / 64) rounded up to a multiple of 8
-	uint32_t pad[2];
 };
 
 }  // namespace rdp_mi
