@@ -363,6 +363,7 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 	s.yhi = rec.yhi;
 	s.span_offset = rec.span_offset;
 	s.tilerow_offset = rec.tilerow_offset;
+	for (int i = 0; i < 4; i++) s.scissor[i] = st.scissor[i];
 
 	RasterPrim r;
 	for (int i = 0; i < 4; i++) {
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(64) void k_span(const PassDesc *__restrict__ passes
 		const int y = ((s.ylo >> 3) + (int)(e - s.tilerow_offset)) * 8 + (lane & 7);
 		if (y >= s.ylo && y <= s.yhi + 1) {
 			SpanSetup sp;
-			compute_span(s, P.states[s.state_index].scissor, y, sp);
+			compute_span(s, s.scissor, y, sp);
 			P.spans[s.span_offset + (uint32_t)(y - s.ylo)] = sp;
 			if (y <= s.yhi && sp.valid) {
 				const int lo = imax(sp.start_x, 0), hi = imin(sp.end_x, (int)P.fb_width - 1);

@@ -323,8 +323,9 @@ struct PrimSetup {
 	int32_t ylo, yhi;
 	uint32_t span_offset;
 	uint32_t tilerow_offset;
+	uint32_t scissor[4];    // the draw state's scissor (xlo, ylo, xhi, yhi in 10.2)
 };
-static_assert(sizeof(PrimSetup) % 16 == 0, "PrimSetup layout");
+static_assert(sizeof(PrimSetup) == 224, "PrimSetup layout");
 
 // Everything the per-tile kernel needs about one primitive, 256 bytes, read
 // with scalar loads: the per-pixel attribute gradients, the span range and
