@@ -128,6 +128,7 @@ struct PinnedBuffer {
 
 struct RoundEvents {
 	hipEvent_t fork = nullptr, tmem_done = nullptr, bin_done = nullptr, join = nullptr, prep_done = nullptr;
+	hipEvent_t pre_raster = nullptr;  // the round's stream just before its raster launches
 };
 
 // A submission lane: a stream with its own pinned staging buffer and device
@@ -165,6 +166,7 @@ struct Lane {
 			HIP_CHECK(hipEventCreateWithFlags(&e.bin_done, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.join, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.prep_done, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&e.pre_raster, hipEventDisableTiming));
 			events.push_back(e);
 		}
 		return 0;
@@ -179,7 +181,8 @@ struct Lane {
 		if (staging.ptr) (void)hipHostFree(staging.ptr);
 		if (staging_done) (void)hipEventDestroy(staging_done);
 		for (auto &e : events)
-			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join, e.prep_done }) (void)hipEventDestroy(ev);
+			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join, e.prep_done, e.pre_raster })
+				(void)hipEventDestroy(ev);
 		if (start) (void)hipEventDestroy(start);
 		if (side) (void)hipStreamDestroy(side);
 		if (prep) (void)hipStreamDestroy(prep);
@@ -292,9 +295,95 @@ struct Launch {
 	const uint32_t *fill_items;  // passes of FILL primitives only: k_raster_fill, 4 tiles per item
 	uint32_t fill_n;
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
+	bool tmem_early;  // no upload reads what the previous round's raster writes
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
 	uint32_t passes;
 };
+
+// RDRAM byte range [lo, hi) before the wrap at the RDRAM size.
+struct ByteRange {
+	uint64_t lo, hi;
+};
+
+// Every RDRAM byte the raster stage of a pass can write: colour image and
+// depth image rows 0 .. fb_height-1 of fb_width pixels, word aligned.
+static void pass_write_ranges(const HostPass &hp, std::vector<ByteRange> &out)
+{
+	if (hp.prims.empty()) return;
+	const uint64_t px = (uint64_t)hp.fb_width * hp.fb_height;
+	uint64_t base, bytes;
+	switch (hp.fb_fmt) {
+	case FB_I4:
+	case FB_I8: base = hp.fb_addr; bytes = px; break;
+	case FB_RGBA5551:
+	case FB_IA88: base = hp.fb_addr & ~1u; bytes = 2 * px; break;
+	default: base = hp.fb_addr & ~3u; bytes = 4 * px; break;
+	}
+	out.push_back({ base & ~3ull, (base + bytes + 3) & ~3ull });
+	if (hp.depth_addr != hp.fb_addr) {
+		const uint64_t d = hp.depth_addr & ~1u;
+		out.push_back({ d & ~3ull, (d + 2 * px + 3) & ~3ull });
+	}
+}
+
+// A bound on the RDRAM bytes a TMEM upload reads: rows 0 .. height-1 of the
+// source image (vram_width texels apart, at most 4 bytes each) and, for
+// TLUT loads, vram_effective_width entries from vram_addr.  False when the
+// record does not allow a bound.
+static bool upload_read_range(const UploadInfo &u, ByteRange &r)
+{
+	if (u.vram_addr < 0 || u.width < 0 || u.height < 0 || u.vram_width < 0 || u.vram_effective_width < 0) return false;
+	const uint64_t w = (uint64_t)std::max(std::max(u.vram_width, u.width), u.vram_effective_width);
+	r.lo = (uint64_t)u.vram_addr & ~3ull;
+	r.hi = (uint64_t)u.vram_addr + (((uint64_t)u.height + 1) * w + w + 16) * 4 + 64;
+	return true;
+}
+
+static bool ranges_overlap(const ByteRange &a, const ByteRange &b, uint64_t size)
+{
+	if (a.hi - a.lo >= size || b.hi - b.lo >= size) return true;
+	ByteRange pa[2], pb[2];
+	auto split = [size](const ByteRange &x, ByteRange *out) {
+		const uint64_t lo = x.lo % size, hi = lo + (x.hi - x.lo);
+		if (hi <= size) {
+			out[0] = { lo, hi };
+			return 1;
+		}
+		out[0] = { lo, size };
+		out[1] = { 0, hi - size };
+		return 2;
+	};
+	const int na = split(a, pa), nb = split(b, pb);
+	for (int i = 0; i < na; i++)
+		for (int j = 0; j < nb; j++)
+			if (pa[i].lo < pb[j].hi && pb[j].lo < pa[i].hi) return true;
+	return false;
+}
+
+// Whether round r's TMEM replay may start before round r-1's raster: no
+// upload of a pass reads RDRAM that the same processor's pass in round r-1
+// can write.
+static bool tmem_may_start_early(const std::vector<std::vector<PassRef>> &rounds, size_t r)
+{
+	if (r == 0) return false;
+	std::vector<ByteRange> writes;
+	for (const PassRef &cur : rounds[r]) {
+		if (cur.pass->uploads.empty()) continue;
+		const HostPass *prev = nullptr;
+		for (const PassRef &p : rounds[r - 1])
+			if (p.proc == cur.proc) prev = p.pass;
+		if (!prev) continue;
+		writes.clear();
+		pass_write_ranges(*prev, writes);
+		for (const UploadInfo &u : cur.pass->uploads) {
+			ByteRange rd;
+			if (!upload_read_range(u, rd)) return false;
+			for (const ByteRange &w : writes)
+				if (ranges_overlap(rd, w, cur.proc->rdram_size)) return false;
+		}
+	}
+	return true;
+}
 
 // A pass whose every primitive is a FILL-mode primitive.
 static bool fill_only(const HostPass &hp)
@@ -371,9 +460,14 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		const bool fork_raster = big && merged && heavy;
 		const RoundEvents E = overlap ? lane.events[r] : RoundEvents();
 
+		const bool early = fork_tmem && L.tmem_early && r > 0;
 		if (fork_tmem) {
+			// The replay reads RDRAM as the previous round left it, unless
+			// none of its uploads reads what that round's raster writes:
+			// then it starts before that raster (`early`).  E.fork marks the
+			// end of the previous round's raster launches on the main stream.
 			HIP_CHECK(hipEventRecord(E.fork, s));
-			HIP_CHECK(hipStreamWaitEvent(side, E.fork, 0));
+			HIP_CHECK(hipStreamWaitEvent(side, early ? lane.events[r - 1].pre_raster : E.fork, 0));
 			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, side));
 			HIP_CHECK(hipEventRecord(E.tmem_done, side));
 		}
@@ -388,6 +482,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], s));
 			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
 		}
+		if (overlap) HIP_CHECK(hipEventRecord(E.pre_raster, s));
 		if (fork_raster) {
 			// The heavy launch needs setup, binning and the TMEM replay.
 			// With both forked, prep_done covers the first two and the side
@@ -399,6 +494,9 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 				ready = E.bin_done;
 			}
 			HIP_CHECK(hipStreamWaitEvent(side, ready, 0));
+			// After an early replay, side-stream order no longer puts the
+			// previous round's raster first.
+			if (early) HIP_CHECK(hipStreamWaitEvent(side, E.fork, 0));
 			for (uint32_t c = lo + 1; c < kRasterClasses; c++)
 				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, side));
 			HIP_CHECK(hipEventRecord(E.join, side));
@@ -557,6 +655,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		LA.fill_items = LA.items[3] + L.n_raster;
 		LA.fill_n = L.n_fill;
 		LA.prep_ahead = prep_ahead;
+		LA.tmem_early = tmem_may_start_early(rounds, r);
 		for (uint32_t c = 0; c < kRasterClasses; c++) {
 			LA.raster_items[c] = LA.items[3] + rbase[c];
 			LA.raster_n[c] = L.n_raster_class[c];
