@@ -216,7 +216,10 @@ constexpr uint32_t kRasterClasses = 3;  // the generic variant (id 0) is in the 
 // Variant 1 is the FILL-mode row.  A pass whose every primitive uses it reads
 // nothing from the framebuffer and is rasterized by its own light kernel.
 constexpr uint32_t kFillVariant = 1;
-constexpr uint32_t kFillTilesPerItem = 4;  // horizontally adjacent tiles per k_raster_fill wave
+#ifndef RDP_FILL_TILES
+#define RDP_FILL_TILES 4
+#endif
+constexpr uint32_t kFillTilesPerItem = RDP_FILL_TILES;  // horizontally adjacent tiles per k_raster_fill wave
 
 // The format every tile a primitive can sample shares, or TEXFMT_ANY.
 // Without texture LOD the sampled tiles are base (texel 0, pipelined texel 1)
