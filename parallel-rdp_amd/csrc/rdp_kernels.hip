@@ -2272,6 +2272,18 @@ DEV uint32_t xcd_local_item(uint32_t b, uint32_t n, int32_t chunk)
 	return ch * c + k % c;
 }
 
+// The RasterPrim fields ylo .. variant, which the tile loop tests for every
+// primitive before it shades: read together with one scalar load instead of
+// one dependent load per field.
+struct RasterPrimHead {
+	int32_t ylo, yhi;
+	uint32_t span_offset, tmem_instance, prim_index, state_index;
+	int32_t dz, dz_compressed;
+	uint32_t ras_flags, dither, db_flags, coverage_mode, z_mode, fill_color, min_lod, variant;
+};
+static_assert(offsetof(RasterPrim, variant) - offsetof(RasterPrim, ylo) == offsetof(RasterPrimHead, variant),
+              "RasterPrimHead mirrors RasterPrim");
+
 // CLS: occupancy class; the kernel holds the variants of classes <= CLS (and
 // the generic code in the top class).  The host launches each pass in the
 // class of its heaviest primitive.
@@ -2336,20 +2348,21 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 				const int bit = __ffsll((unsigned long long)mask) - 1;
 				mask &= mask - 1;
 				const AS_CONST RasterPrim *ps = rprims + ((gbase + (uint32_t)g) * 64 + (uint32_t)bit);
+				const RasterPrimHead h = ldc(reinterpret_cast<const AS_CONST RasterPrimHead *>(&ps->ylo));
 
 				// Stage the primitive's TMEM snapshot in LDS (wave-uniform).
-				if ((ps->ras_flags & needs_tmem) && ps->tmem_instance != staged) {
+				if ((h.ras_flags & needs_tmem) && h.tmem_instance != staged) {
 					__syncthreads();
-					const AS_GLOBAL uint32_t *src = tmem_src + (size_t)ps->tmem_instance * 1024;
+					const AS_GLOBAL uint32_t *src = tmem_src + (size_t)h.tmem_instance * 1024;
 					uint32_t *dst = reinterpret_cast<uint32_t *>(s_tmem);
 #pragma unroll
 					for (int k = 0; k < 16; k++) dst[lane * 4 + (k & 3) + 256 * (k >> 2)] = src[lane * 4 + (k & 3) + 256 * (k >> 2)];
 					__syncthreads();
-					staged = ps->tmem_instance;
+					staged = h.tmem_instance;
 				}
 
-				if (!in_fb || y < ps->ylo || y > ps->yhi) continue;
-				switch (ps->variant) {
+				if (!in_fb || y < h.ylo || y > h.yhi) continue;
+				switch (h.variant) {
 #define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt, dbp, cls)                                            \
 		case id:                                                                                                    \
 			if constexpr ((cls) <= CLS)                                                                             \
