@@ -1530,6 +1530,18 @@ DEV uint32_t z_compress(int32_t z)
 	return (uint32_t)((e << 11) + man) & 0xffff;
 }
 
+// The RasterPrim fields ylo .. variant, which the tile loop tests for every
+// primitive before it shades: read together with one scalar load instead of
+// one dependent load per field.
+struct RasterPrimHead {
+	int32_t ylo, yhi;
+	uint32_t span_offset, tmem_instance, prim_index, state_index;
+	int32_t dz, dz_compressed;
+	uint32_t ras_flags, dither, db_flags, coverage_mode, z_mode, fill_color, min_lod, variant;
+};
+static_assert(offsetof(RasterPrim, variant) - offsetof(RasterPrim, ylo) == offsetof(RasterPrimHead, variant),
+              "RasterPrimHead mirrors RasterPrim");
+
 struct Pixel {
 	int32_t c[4];
 	uint32_t depth, dz;
@@ -1805,11 +1817,11 @@ struct KnownBlend {
 };
 
 template <class V>
-DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *tiles, const Tex &tx, uint32_t fb_fmt,
-                     const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, const Pixel &px,
+DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST TileDesc *tiles,
+                     const Tex &tx, uint32_t fb_fmt, const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, const Pixel &px,
                      Shaded &out)
 {
-	const uint32_t f = ps->ras_flags;
+	const uint32_t f = h.ras_flags;
 	const bool tlut = V::ras(f, RAS_TLUT), tlut_type = V::ras(f, RAS_TLUT_TYPE), sample_quad = V::ras(f, RAS_SAMPLE_QUAD);
 	const bool cvg_times_alpha = V::ras(f, RAS_CVG_TIMES_ALPHA), alpha_cvg_select = V::ras(f, RAS_ALPHA_CVG_SELECT);
 	const bool perspective = V::ras(f, RAS_PERSPECTIVE), aa = V::ras(f, RAS_AA), multi = V::ras(f, RAS_MULTI_CYCLE);
@@ -1820,13 +1832,13 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 
 	uint32_t noise = 0;
 	if (V::ras(f, RAS_NEED_NOISE))
-		noise = reseed_noise((uint32_t)x, (uint32_t)y, ps->prim_index);
+		noise = reseed_noise((uint32_t)x, (uint32_t)y, h.prim_index);
 
 	const bool flip = (ps->flags & TRI_FLIP) != 0;
 	const int32_t factors[4] = { ps->factors[0], ps->factors[1], ps->factors[2], ps->factors[3] };
 
 	if (V::ras(f, RAS_COPY)) {
-		if (x < span.start_x || x > span.end_x) return false;
+		if (!span.valid || x < span.start_x || x > span.end_x) return false;
 		int dxp = flip ? (x - span.start_x) : (span.end_x - x);
 		uint32_t dx_mask, dx_shift, fb_size;
 		switch (fb_fmt) {
@@ -1858,11 +1870,11 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 		return true;
 	} else if (V::ras(f, RAS_FILL)) {
 		out.coverage = COVERAGE_FILL;
-		return x >= span.start_x && x <= span.end_x;
+		return span.valid && x >= span.start_x && x <= span.end_x;
 	}
 
 	const int coverage = compute_coverage(span, x);
-	if (coverage == 0) return false;
+	if (!span.valid || coverage == 0) return false;
 	int cvg_count = __popc(coverage);
 	if (!aa && (coverage & 1) == 0) return false;
 
@@ -1880,7 +1892,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	}
 	// Early depth rejection: shading, texturing, LOD and the combiner only
 	// feed a fragment the depth test is known to discard.
-	if (V::db(ps->db_flags, DB_DEPTH_TEST) && depth_rejects(depth_compare(px.depth, px.dz, z, ps->dz), V::z_mode(ps->z_mode)))
+	if (V::db(h.db_flags, DB_DEPTH_TEST) && depth_rejects(depth_compare(px.depth, px.dz, z, h.dz), V::z_mode(h.z_mode)))
 		return false;
 
 	int32_t shade[4];
@@ -1911,7 +1923,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	uint32_t tile0 = ps->tile & 7, tile1 = (tile0 + 1) & 7;
 	int32_t lod_frac = 0;
 	if (uses_lod)
-		compute_lod(tile0, tile1, lod_frac, ps->tile >> 3, (int)ps->min_lod, st0, stdx, stdy, povf, V::ras(f, RAS_TEX_LOD),
+		compute_lod(tile0, tile1, lod_frac, ps->tile >> 3, (int)h.min_lod, st0, stdx, stdy, povf, V::ras(f, RAS_TEX_LOD),
 		            V::ras(f, RAS_SHARPEN_LOD), V::ras(f, RAS_DETAIL_LOD));
 
 	int32_t texel0[4] = { 0, 0, 0, 0 }, texel1[4] = { 0, 0, 0, 0 };
@@ -1964,7 +1976,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 	}
 
 	int rgb_dith, alpha_dith;
-	dither_coefficients(x, y >> (interlace ? 1 : 0), (int)(ps->dither >> 2), (int)(ps->dither & 3), noise, rgb_dith,
+	dither_coefficients(x, y >> (interlace ? 1 : 0), (int)(h.dither >> 2), (int)(h.dither & 3), noise, rgb_dith,
 	                    alpha_dith);
 
 	CombIn ci;
@@ -2004,7 +2016,7 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 			ci.texel1[i] = tmp;
 		}
 		if (V::ras(f, RAS_NEED_NOISE_DUAL)) {
-			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), ps->prim_index + 11);
+			uint32_t n2 = reseed_noise((uint32_t)(x + 1023), (uint32_t)(y + 7), h.prim_index + 11);
 			ci.noise = (int32_t)(((n2 & 7) << 6) | 0x20);
 		}
 		if constexpr (V::comb != COMB_ANY)
@@ -2047,13 +2059,13 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const AS_CONST TileDesc *til
 
 // memory_interfacing.h:488-571
 template <class V>
-DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps, uint32_t fmt, bool alias, Pixel &p,
+DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps, const RasterPrimHead &h, uint32_t fmt, bool alias, Pixel &p,
                      const Shaded &sh)
 {
 	const int32_t z = sh.z_dith >> 9;
 	const int dith = sh.z_dith & 0x1ff;
 	int cvg = sh.coverage;
-	const uint32_t f = ps->db_flags;
+	const uint32_t f = h.db_flags;
 	const bool force_blend = V::db(f, DB_FORCE_BLEND), z_compare = V::db(f, DB_DEPTH_TEST);
 	const bool z_update = V::db(f, DB_DEPTH_UPDATE), image_read = V::db(f, DB_IMAGE_READ);
 	const bool color_on_cvg = V::db(f, DB_COLOR_ON_CVG), multi = V::db(f, DB_MULTI_CYCLE);
@@ -2082,13 +2094,13 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 	bool blend_en, cvg_wrap, pass;
 	int shift0, shift1;
 	if (z_compare) {
-		const DepthCompare d = depth_compare(p.depth, p.dz, z, ps->dz);
-		shift0 = iclamp(ps->dz_compressed - (int32_t)p.dz, 0, 4);
-		shift1 = iclamp((int32_t)p.dz - ps->dz_compressed, 0, 4);
+		const DepthCompare d = depth_compare(p.depth, p.dz, z, h.dz);
+		shift0 = iclamp(h.dz_compressed - (int32_t)p.dz, 0, 4);
+		shift1 = iclamp((int32_t)p.dz - h.dz_compressed, 0, 4);
 		const bool overflow = (cvg + memory_coverage) >= 8;
 		blend_en = force_blend || (!overflow && aa && d.farther);
 		cvg_wrap = overflow;
-		switch (V::z_mode(ps->z_mode)) {
+		switch (V::z_mode(h.z_mode)) {
 		case 0: pass = d.max_z || (overflow ? d.front : d.nearer); break;
 		case 1:
 			if (!d.front || !d.farther || !overflow)
@@ -2105,7 +2117,7 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 		}
 	} else {
 		shift0 = 0;
-		shift1 = imin(0xf - ps->dz_compressed, 4);
+		shift1 = imin(0xf - h.dz_compressed, 4);
 		const bool overflow = (cvg + memory_coverage) >= 8;
 		blend_en = force_blend || (!overflow && aa);
 		cvg_wrap = overflow;
@@ -2149,7 +2161,7 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 			}
 		}
 		int new_cov;
-		switch (V::coverage_mode(ps->coverage_mode)) {
+		switch (V::coverage_mode(h.coverage_mode)) {
 		case 0: new_cov = blend_en ? imin(7, memory_coverage + cvg) : ((cvg - 1) & 7); break;
 		case 1: new_cov = (cvg + memory_coverage) & 7; break;
 		case 2: new_cov = 7; break;
@@ -2158,7 +2170,7 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 		write_color(p, fmt, rgb[0], rgb[1], rgb[2], new_cov << 5);
 		if (z_update) {
 			p.depth = z_compress(z);
-			p.dz = (uint32_t)ps->dz_compressed;
+			p.dz = (uint32_t)h.dz_compressed;
 			p.depth_dirty = true;
 			if (alias) alias_depth_to_color(p, fmt);
 		} else if (alias)
@@ -2215,21 +2227,22 @@ DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
 // One primitive at one pixel: span lookup, shading, then fill / copy /
 // depth test + blend into the pixel's registers.
 template <class V>
-DEV void raster_prim(const AS_CONST RasterPrim *ps, const AS_CONST DrawState *states, const AS_GLOBAL SpanSetup *spans,
+DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST DrawState *states, const AS_GLOBAL SpanSetup *spans,
                      const Tex &tex, const AS_GLOBAL uint8_t *lut, uint32_t fmt, bool alias, uint32_t ci, int x, int y,
                      Pixel &px)
 {
-	const AS_GLOBAL SpanSetup *sp = spans + (ps->span_offset + (uint32_t)(y - ps->ylo));
+	const AS_GLOBAL SpanSetup *sp = spans + (h.span_offset + (uint32_t)(y - h.ylo));
+	// The span's valid flag is tested with the first coverage test in
+	// shade_pixel, so every field the test needs is fetched at once.
 	const SpanSetup span = ldg16(sp);
-	if (!span.valid) return;
 	Shaded sh;
-	if (!shade_pixel<V>(ps, states[ps->state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh)) return;
+	if (!shade_pixel<V>(ps, h, states[h.state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh)) return;
 	if (sh.coverage & COVERAGE_FILL)
-		fill_pixel(px, fmt, alias, ci, ps->fill_color);
+		fill_pixel(px, fmt, alias, ci, h.fill_color);
 	else if (sh.coverage & COVERAGE_COPY)
 		copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
 	else
-		depth_blend<V>(lut, ps, fmt, alias, px, sh);
+		depth_blend<V>(lut, ps, h, fmt, alias, px, sh);
 }
 
 // Waves per SIMD each occupancy class is compiled for (the register budget
@@ -2271,18 +2284,6 @@ DEV uint32_t xcd_local_item(uint32_t b, uint32_t n, int32_t chunk)
 	const uint32_t ch = (k / c) * RDP_NUM_XCDS + xcd;
 	return ch * c + k % c;
 }
-
-// The RasterPrim fields ylo .. variant, which the tile loop tests for every
-// primitive before it shades: read together with one scalar load instead of
-// one dependent load per field.
-struct RasterPrimHead {
-	int32_t ylo, yhi;
-	uint32_t span_offset, tmem_instance, prim_index, state_index;
-	int32_t dz, dz_compressed;
-	uint32_t ras_flags, dither, db_flags, coverage_mode, z_mode, fill_color, min_lod, variant;
-};
-static_assert(offsetof(RasterPrim, variant) - offsetof(RasterPrim, ylo) == offsetof(RasterPrimHead, variant),
-              "RasterPrimHead mirrors RasterPrim");
 
 // CLS: occupancy class; the kernel holds the variants of classes <= CLS (and
 // the generic code in the top class).  The host launches each pass in the
@@ -2366,14 +2367,14 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 #define RDP_DISPATCH_VARIANT(id, known, value, comb, texfmt, dbp, cls)                                            \
 		case id:                                                                                                    \
 			if constexpr ((cls) <= CLS)                                                                             \
-				raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt), (int)(dbp)>>(ps, states, spans, tex, lut, fmt, alias, ci, \
+				raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt), (int)(dbp)>>(ps, h, states, spans, tex, lut, fmt, alias, ci, \
 				                                                                    x, y, px);                        \
 			break;
 					RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
 #undef RDP_DISPATCH_VARIANT
 				default:
 					if constexpr (CLS == kRasterClasses - 1)
-						raster_prim<GenericVariant>(ps, states, spans, tex, lut, fmt, alias, ci, x, y, px);
+						raster_prim<GenericVariant>(ps, h, states, spans, tex, lut, fmt, alias, ci, x, y, px);
 					break;
 				}
 			}
@@ -2453,8 +2454,9 @@ void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restri
 					const int bit = __ffsll((unsigned long long)mask) - 1;
 					mask &= mask - 1;
 					const AS_CONST RasterPrim *ps = rprims + ((gbase + (uint32_t)g) * 64 + (uint32_t)bit);
-					if (!in_fb || (int)y < ps->ylo || (int)y > ps->yhi) continue;
-					raster_prim<FillVariant>(ps, states, spans, tex, lut, fmt, alias, ci, (int)x, (int)y, px);
+					const RasterPrimHead h = ldc(reinterpret_cast<const AS_CONST RasterPrimHead *>(&ps->ylo));
+					if (!in_fb || (int)y < h.ylo || (int)y > h.yhi) continue;
+					raster_prim<FillVariant>(ps, h, states, spans, tex, lut, fmt, alias, ci, (int)x, (int)y, px);
 				}
 			}
 		}
