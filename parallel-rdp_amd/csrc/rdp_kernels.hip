@@ -417,7 +417,10 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 	}
 }
 
-__global__ __launch_bounds__(64) void k_span(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+#ifndef RDP_SPAN_WAVES
+#define RDP_SPAN_WAVES 6
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_SPAN_WAVES))) void k_span(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 17];
