@@ -331,7 +331,7 @@ def main():
                               "bin": round(float(stage_ms[2]), 4), "raster": round(float(stage_ms[3]), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_raster", "bytes_per_launch": int(bytes_per_launch),
+                     "kernel": "k_raster (the k_raster<class> and k_raster_fill launches of a step)", "bytes_per_launch": int(bytes_per_launch),
                      "ms_per_launch": round(raster_ms_per_launch, 4), "launches_per_step": raster_launches,
                      "valu_issue": valu},
         "cpu_baseline": cpu,
