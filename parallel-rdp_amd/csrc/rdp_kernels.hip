@@ -2288,6 +2288,16 @@ DEV uint32_t xcd_local_item(uint32_t b, uint32_t n, int32_t chunk)
 	return ch * c + k % c;
 }
 
+// The blender divider table (one byte per 15-bit index), uploaded once per
+// device by upload_blender_lut.  As a symbol its address is a constant the
+// compiler rematerializes instead of keeping it in registers.
+__device__ uint8_t g_blender_lut[0x8000];
+
+hipError_t upload_blender_lut(const uint8_t *lut)
+{
+	return hipMemcpyToSymbol(HIP_SYMBOL(g_blender_lut), lut, sizeof(g_blender_lut), 0, hipMemcpyHostToDevice);
+}
+
 // CLS: occupancy class; the kernel holds the variants of classes <= CLS (and
 // the generic code in the top class).  The host launches each pass in the
 // class of its heaviest primitive.
@@ -2310,8 +2320,7 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 	const AS_CONST RasterPrim *rprims = cptr(P.rprims);
 	const AS_CONST DrawState *states = cptr(P.states);
 	const AS_GLOBAL SpanSetup *spans = gptr(P.spans);
-	const AS_GLOBAL uint8_t *lut = gptr(P.blender_lut);
-	const AS_GLOBAL uint32_t *tmem_src = (const AS_GLOBAL uint32_t *)gptr(P.tmem_instances);
+	const AS_GLOBAL uint8_t *lut = (const AS_GLOBAL uint8_t *)g_blender_lut;
 	const Tex tex{ (const AS_LDS uint16_t *)(s_tmem) };
 
 	uint32_t addr_index;
@@ -2357,7 +2366,12 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 				// Stage the primitive's TMEM snapshot in LDS (wave-uniform).
 				if ((h.ras_flags & needs_tmem) && h.tmem_instance != staged) {
 					__syncthreads();
-					const AS_GLOBAL uint32_t *src = tmem_src + (size_t)h.tmem_instance * 1024;
+					// The snapshot base is read from the descriptor here, not kept
+					// live through the loop.
+					const PassDesc *pt = &P;
+					asm volatile("" : "+s"(pt));
+					const AS_GLOBAL uint32_t *src =
+					    (const AS_GLOBAL uint32_t *)gptr(pt->tmem_instances) + (size_t)h.tmem_instance * 1024;
 					uint32_t *dst = reinterpret_cast<uint32_t *>(s_tmem);
 #pragma unroll
 					for (int k = 0; k < 16; k++) dst[lane * 4 + (k & 3) + 256 * (k >> 2)] = src[lane * 4 + (k & 3) + 256 * (k >> 2)];
@@ -2384,10 +2398,16 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 		}
 	}
 
-	if (in_fb && px.color_dirty) store_color(m, fmt, ci, px);
+	// The stream memory pointers and the depth index are read again here
+	// rather than kept live through the primitive loop.
+	const PassDesc *pe = &P;
+	asm volatile("" : "+s"(pe));
+	const Mem me{ gptr(pe->rdram), gptr(pe->hidden), pe->rdram_mask };
+	if (in_fb && px.color_dirty) store_color(me, fmt, ci, px);
 	if (in_fb && !alias && px.depth_dirty) {
-		m.wr16i(di, (px.depth << 2) | (px.dz >> 2));
-		m.set_hid(di, px.dz & 3);
+		const uint32_t de = (pe->depth_addr >> 1) + pe->fb_width * (uint32_t)y + (uint32_t)x;
+		me.wr16i(de, (px.depth << 2) | (px.dz >> 2));
+		me.set_hid(de, px.dz & 3);
 	}
 }
 
@@ -2413,7 +2433,7 @@ void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restri
 	const AS_CONST RasterPrim *rprims = cptr(P.rprims);
 	const AS_CONST DrawState *states = cptr(P.states);
 	const AS_GLOBAL SpanSetup *spans = gptr(P.spans);
-	const AS_GLOBAL uint8_t *lut = gptr(P.blender_lut);
+	const AS_GLOBAL uint8_t *lut = (const AS_GLOBAL uint8_t *)g_blender_lut;
 	const AS_GLOBAL uint64_t *bins = gptr(P.bin_fine);
 	const Tex tex{ (const AS_LDS uint16_t *)(s_unused) };
 	const uint32_t G = P.num_groups, tiles_x = P.tiles_x;

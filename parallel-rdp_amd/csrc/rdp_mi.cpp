@@ -38,6 +38,7 @@ hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, const uint3
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 hipError_t launch_raster_fill(const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+hipError_t upload_blender_lut(const uint8_t *lut);
 
 // Launch shape of the raster stage: the XCD chunk of the tile-to-block
 // mapping and the lowest occupancy class a pass runs in (passes below it are
@@ -193,7 +194,6 @@ struct Lane {
 
 struct Context : Lane {
 	bool ok = false;
-	uint8_t *d_lut = nullptr;
 	uint8_t *d_gamma = nullptr;
 	LaunchConfig cfg;  // for immediate submissions (flush, batch flush, serving)
 	std::mutex lock;
@@ -206,8 +206,7 @@ struct Context : Lane {
 		if (int err = create()) return err;
 		std::vector<uint8_t> lut(0x8000);
 		blender_divider_table(lut.data());
-		HIP_CHECK(hipMalloc(&d_lut, 0x8000));
-		HIP_CHECK(hipMemcpy(d_lut, lut.data(), 0x8000, hipMemcpyHostToDevice));
+		HIP_CHECK(upload_blender_lut(lut.data()));
 		std::vector<uint8_t> gamma(256 + 256 * 64);
 		vi_gamma_table(gamma.data());
 		HIP_CHECK(hipMalloc(&d_gamma, gamma.size()));
@@ -695,7 +694,6 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.rdram = pr.d_rdram;
 			D.hidden = pr.d_hidden;
 			D.tmem = pr.d_tmem;
-			D.blender_lut = ctx().d_lut;
 			D.rdram_mask = pr.rdram_size - 1;
 			D.num_prims = (uint32_t)hp.prims.size();
 			D.num_uploads = (uint32_t)hp.uploads.size();

@@ -408,7 +408,6 @@ struct PassDesc {
 	uint8_t *rdram;
 	uint8_t *hidden;
 	uint16_t *tmem;            // persistent TMEM of the stream (2048 halfwords)
-	const uint8_t *blender_lut;
 	uint32_t rdram_mask;       // rdram_size - 1
 	uint32_t num_prims;
 	uint32_t num_uploads;
