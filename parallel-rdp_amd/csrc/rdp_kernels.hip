@@ -1798,6 +1798,9 @@ struct RasterVariant {
 	static constexpr int texfmt = TF;
 	static constexpr int dbp = DBP;
 	static MDEV bool ras(uint32_t f, uint32_t bit) { return (KNOWN & bit) ? ((VALUE & bit) != 0) : ((f & bit) != 0); }
+	// Known to be a fill or copy primitive, or to sample texel 0.
+	static constexpr bool fill_or_copy = (KNOWN & VALUE & (RAS_FILL | RAS_COPY)) != 0;
+	static constexpr bool textured = (KNOWN & VALUE & RAS_USES_TEXEL0) != 0;
 	// Depth/blend state: the profile's constants, or the primitive's values.
 	static MDEV bool db(uint32_t f, uint32_t bit)
 	{
@@ -2239,6 +2242,26 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 	// shade_pixel, so every field the test needs is fetched at once.
 	const SpanSetup span = ldg16(sp);
 	Shaded sh;
+	if constexpr (!V::fill_or_copy) {
+		// Fetch the shade and depth gradients (and the texture ones of a
+		// textured variant) now, beside the span load.
+		int32_t gr[10];
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			gr[i] = ps->drgba_dx[i];
+			gr[4 + i] = ps->drgba_dy[i];
+		}
+		gr[8] = ps->dstzw_dx[2];
+		gr[9] = ps->dstzw_dy[2];
+#pragma unroll
+		for (int i = 0; i < 10; i++) asm volatile("" ::"s"(gr[i]));
+		if constexpr (V::textured) {
+			int32_t gt[6] = { ps->dstzw_dx[0], ps->dstzw_dx[1], ps->dstzw_dx[3],
+			                  ps->dstzw_dy[0], ps->dstzw_dy[1], ps->dstzw_dy[3] };
+#pragma unroll
+			for (int i = 0; i < 6; i++) asm volatile("" ::"s"(gt[i]));
+		}
+	}
 	if (!shade_pixel<V>(ps, h, states[h.state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh)) return;
 	if (sh.coverage & COVERAGE_FILL)
 		fill_pixel(px, fmt, alias, ci, h.fill_color);
