@@ -2391,7 +2391,7 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 					__syncthreads();
 					// The snapshot base is read from the descriptor here, not kept
 					// live through the loop.
-					const PassDesc *pt = &P;
+					const AS_CONST PassDesc *pt = cptr(&P);
 					asm volatile("" : "+s"(pt));
 					const AS_GLOBAL uint32_t *src =
 					    (const AS_GLOBAL uint32_t *)gptr(pt->tmem_instances) + (size_t)h.tmem_instance * 1024;
@@ -2423,7 +2423,7 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 
 	// The stream memory pointers and the depth index are read again here
 	// rather than kept live through the primitive loop.
-	const PassDesc *pe = &P;
+	const AS_CONST PassDesc *pe = cptr(&P);
 	asm volatile("" : "+s"(pe));
 	const Mem me{ gptr(pe->rdram), gptr(pe->hidden), pe->rdram_mask };
 	if (in_fb && px.color_dirty) store_color(me, fmt, ci, px);
