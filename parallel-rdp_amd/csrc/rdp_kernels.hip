@@ -420,7 +420,8 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 #ifndef RDP_SPAN_WAVES
 #define RDP_SPAN_WAVES 6
 #endif
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_SPAN_WAVES))) void k_span(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_SPAN_WAVES)))
+void k_span(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items)
 {
 	const uint32_t item = items[blockIdx.x];
 	const PassDesc &P = passes[item >> 17];
@@ -1824,7 +1825,8 @@ struct KnownBlend {
 
 template <class V>
 DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST TileDesc *tiles,
-                     const Tex &tx, uint32_t fb_fmt, const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x, int y, const Pixel &px,
+                     const Tex &tx, uint32_t fb_fmt, const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x,
+                     int y, const Pixel &px,
                      Shaded &out)
 {
 	const uint32_t f = h.ras_flags;
@@ -2065,7 +2067,8 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 
 // memory_interfacing.h:488-571
 template <class V>
-DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps, const RasterPrimHead &h, uint32_t fmt, bool alias, Pixel &p,
+DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps, const RasterPrimHead &h, uint32_t fmt,
+                     bool alias, Pixel &p,
                      const Shaded &sh)
 {
 	const int32_t z = sh.z_dith >> 9;
@@ -2233,7 +2236,8 @@ DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
 // One primitive at one pixel: span lookup, shading, then fill / copy /
 // depth test + blend into the pixel's registers.
 template <class V>
-DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST DrawState *states, const AS_GLOBAL SpanSetup *spans,
+DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST DrawState *states,
+                     const AS_GLOBAL SpanSetup *spans,
                      const Tex &tex, const AS_GLOBAL uint8_t *lut, uint32_t fmt, bool alias, uint32_t ci, int x, int y,
                      Pixel &px)
 {
