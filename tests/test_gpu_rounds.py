@@ -77,3 +77,27 @@ def test_tmem_replay_order_across_rounds(sources):
     for out, s in zip(outs, streams):
         total, lines = harness.compare(out, harness.run_oracle(s))
         assert total == 0, f"{s.name}: " + "; ".join(lines)
+
+
+def test_tmem_replay_order_through_server():
+    """The served form batches the same frames: 12 instances, requests queued
+    together so one batch carries every instance's two passes."""
+    import rdp_mi
+    streams = [texture_stream(200 + i, i % 2 == 0) for i in range(12)]
+    print(f"[case] server batch of {len(streams)} render-to-texture frames", flush=True)
+    procs = [rdp_mi.CommandProcessor(rs.RDRAM_SIZE) for _ in streams]
+    for p, s in zip(procs, streams):
+        p.write_rdram(s.rdram_init)
+        p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
+    with rdp_mi.Server(procs, streams=1) as srv:
+        for inst, s in enumerate(streams):
+            srv.submit(inst, s.frames[0].words, arrival_us=0.0)
+        srv.start()
+        srv.drain()
+        assert srv.results()["ticket"].size == len(streams)
+    outs = [(p.read_rdram(), p.read_hidden_rdram(), p.read_tmem()) for p in procs]
+    for p in procs:
+        p.close()
+    for out, s in zip(outs, streams):
+        total, lines = harness.compare(out, harness.run_oracle(s))
+        assert total == 0, f"{s.name}: " + "; ".join(lines)
