@@ -34,6 +34,8 @@ enum {
 	RDP_MI_ERROR_OUT_OF_MEMORY = -2,
 	RDP_MI_ERROR_DEVICE = -3,
 	RDP_MI_ERROR_UNSUPPORTED = -4,
+	RDP_MI_ERROR_TIMEOUT = -5,
+	RDP_MI_ERROR_LIMIT = -6,
 };
 
 /* Flags accepted by rdp_mi_create (rdp_device.hpp:43-51).  Upscaling flags
@@ -67,25 +69,48 @@ void rdp_mi_destroy(rdp_mi_processor *p);
 /* CommandProcessor::device_is_supported, rdp_device.cpp:225-228. */
 int rdp_mi_is_supported(const rdp_mi_processor *p);
 
-/* CommandProcessor::enqueue_command / enqueue_command_direct,
- * rdp_device.cpp:874-959.  words holds one complete command. */
+/* CommandProcessor::enqueue_command, rdp_device.cpp:874-897, through the
+ * CommandRing (command_ring.cpp:71-134): the words are copied into the
+ * processor's command ring and decoded on its own thread, so the call returns
+ * without decoding (it blocks only while the ring is full).  words holds one
+ * complete command.  Opcodes 1-3 are private meta commands and are dropped. */
 void rdp_mi_enqueue_command(rdp_mi_processor *p, unsigned num_words, const uint32_t *words);
+
+/* CommandProcessor::enqueue_command_direct, rdp_device.cpp:899-959: decodes
+ * on the calling thread (after the ring has drained, so the order of
+ * commands is kept). */
 void rdp_mi_enqueue_command_direct(rdp_mi_processor *p, unsigned num_words, const uint32_t *words);
 
-/* Enqueue a packed run of complete commands (no reference counterpart: the
- * reference's callers loop over enqueue_command, which this performs in one
- * call).  Returns the number of words consumed; a truncated trailing command
- * is left unconsumed, a word with an undefined opcode is skipped. */
+/* Enqueue a packed run of complete commands through the command ring (no
+ * reference counterpart: the reference's callers loop over enqueue_command,
+ * which this performs with one ring hand-off per 64 Ki words).  Returns the
+ * number of words consumed; a truncated trailing command is left unconsumed,
+ * a word with an undefined or private opcode is skipped. */
 size_t rdp_mi_enqueue_command_list(rdp_mi_processor *p, size_t num_words, const uint32_t *words);
 
 /* CommandProcessor::set_quirks, rdp_device.cpp:961-968 (in-band). */
 void rdp_mi_set_quirks(rdp_mi_processor *p, uint32_t quirks);
 
 /* CommandProcessor::flush / signal_timeline / wait_for_timeline / idle /
- * begin_frame_context, rdp_device.cpp:169-174, 1039-1080. */
+ * begin_frame_context, rdp_device.cpp:169-174, 1039-1080.
+ * flush and signal_timeline are in-band: they travel through the command
+ * ring behind the commands enqueued before them.  signal_timeline returns the
+ * next value of the processor's monotonic timeline; a completion thread
+ * (worker_thread.hpp:39-126) publishes it once the device has finished every
+ * command enqueued before the signal.  wait_for_timeline blocks the calling
+ * thread until the value is published, also for a value not yet signalled
+ * (worker_thread.hpp:69-73): it returns once another thread's signal_timeline
+ * reaches it.  It returns RDP_MI_ERROR_DEVICE if the device reported an error
+ * for the processor's work. */
 void rdp_mi_flush(rdp_mi_processor *p);
 uint64_t rdp_mi_signal_timeline(rdp_mi_processor *p);
 int rdp_mi_wait_for_timeline(rdp_mi_processor *p, uint64_t value);
+/* wait_for_timeline with a time limit (no reference counterpart): returns
+ * RDP_MI_ERROR_TIMEOUT when the value is not published within timeout_us. */
+int rdp_mi_wait_for_timeline_timeout(rdp_mi_processor *p, uint64_t value, uint64_t timeout_us);
+/* The last timeline value whose work is complete (no reference counterpart;
+ * the reference keeps it in thread_timeline_value, rdp_device.cpp:1184-1188). */
+uint64_t rdp_mi_completed_timeline(rdp_mi_processor *p);
 int rdp_mi_idle(rdp_mi_processor *p);
 void rdp_mi_begin_frame_context(rdp_mi_processor *p);
 
@@ -130,6 +155,25 @@ void rdp_mi_scanout_options_default(rdp_mi_scanout_options *opts);
  * rdp_device.cpp:1159-1182, with explicit options. */
 int rdp_mi_scanout_sync_ex(rdp_mi_processor *p, const rdp_mi_scanout_options *opts, uint32_t *out, unsigned *width,
                            unsigned *height);
+
+/* VIScanoutBuffer (rdp_device.hpp:53-58): a host-readable image of one
+ * scanout with a fence.  One buffer holds up to 640 x 576 RGBA8 words. */
+typedef struct rdp_mi_scanout_buffer rdp_mi_scanout_buffer;
+int rdp_mi_scanout_buffer_create(rdp_mi_scanout_buffer **out);
+void rdp_mi_scanout_buffer_destroy(rdp_mi_scanout_buffer *b);
+
+/* CommandProcessor::scanout_async_buffer(VIScanoutBuffer &, opts),
+ * rdp_device.cpp:1130-1157: drains the command ring, submits the pending
+ * rendering and queues the VI scanout and its copy into the buffer behind
+ * it, then returns without waiting.  The buffer's width and height are known
+ * at once (0 x 0 for a blank or invalid frame); its pixels once
+ * rdp_mi_scanout_buffer_wait returns. */
+int rdp_mi_scanout_async_buffer(rdp_mi_processor *p, const rdp_mi_scanout_options *opts, rdp_mi_scanout_buffer *b);
+
+/* Waits for the buffer's fence (VIScanoutBuffer::fence->wait(),
+ * rdp_device.cpp:1176) and returns its pixels (valid until the buffer is used
+ * again or destroyed) and size. */
+int rdp_mi_scanout_buffer_wait(rdp_mi_scanout_buffer *b, const uint32_t **pixels, unsigned *width, unsigned *height);
 
 /* Per-scanline HStart / XScale registers: begin_vi_register_per_scanline,
  * set_vi_register_for_scanline, latch_vi_register_for_scanline,

@@ -21,6 +21,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -223,6 +224,24 @@ static Context &ctx()
 	return c;
 }
 
+// Command ring of one processor (CommandRing, command_ring.cpp:31-134):
+// rdp_mi_enqueue_command copies complete commands in, and the processor's
+// ring thread takes everything available at once and decodes it.  Private
+// meta commands ride in the same ring so they stay ordered with the RDP
+// commands (Op::MetaSignalTimeline and Op::MetaFlush, rdp_common.hpp:41-42).
+constexpr uint32_t kMetaSignal = 1, kMetaFlush = 2;
+constexpr size_t kRingWords = 1u << 16;
+
+struct CommandRing {
+	std::vector<uint32_t> words = std::vector<uint32_t>(kRingWords);
+	uint64_t write_count = 0, read_count = 0, completed_count = 0;
+	std::mutex lock;
+	std::condition_variable cv;  // data for the thread, space for producers, completion for drains
+	std::thread thr;
+	bool stop = false;
+	bool pending = false;  // the processor held undelivered passes after the last batch
+};
+
 struct Processor {
 	Frontend fe;
 	uint32_t rdram_size = 0;
@@ -234,11 +253,24 @@ struct Processor {
 	uint16_t tmem_mirror[2048];
 	uint32_t vi_regs[RDP_MI_VI_COUNT] = {};
 	ViState vi;
-	uint64_t timeline = 0;
-	std::map<uint64_t, hipEvent_t> timeline_events;
 	rdp_mi_validation_fn validation_fn = nullptr;
 	void *validation_user = nullptr;
 	bool supported = false;
+
+	// Guards the front end and the VI state: held by the ring thread while
+	// it decodes and by every entry point that touches them directly.
+	std::mutex lock;
+	std::unique_ptr<CommandRing> ring;  // created by the first rdp_mi_enqueue_command
+	std::mutex ring_create;
+
+	// Timeline (rdp_device.cpp:1053-1080, 1184-1188): `timeline` counts the
+	// values handed out by signal_timeline; `timeline_done` is the last value
+	// whose work the completion thread has seen finish.
+	std::atomic<uint64_t> timeline{ 0 };
+	std::mutex tl_lock;
+	std::condition_variable tl_cv;
+	uint64_t timeline_done = 0;
+	int timeline_error = 0;
 
 	explicit Processor(uint32_t size) : fe(size), rdram_size(size) { memset(tmem_mirror, 0, sizeof(tmem_mirror)); }
 	uint8_t *host_rdram() { return user_rdram ? user_rdram : rdram_mirror.data(); }
@@ -765,6 +797,282 @@ static int copy_sync(void *dst, const void *src, size_t size, hipMemcpyKind kind
 	return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Timeline completion (the reference's FenceExecutor on a WorkerThread,
+// rdp_device.cpp:1184-1263, worker_thread.hpp:39-126).  A signal records an
+// event on the context stream behind the processor's work and queues it here;
+// one thread waits for the events in order and publishes each value.
+// ---------------------------------------------------------------------------
+static void publish_timeline(Processor &p, uint64_t value, int err)
+{
+	std::lock_guard<std::mutex> g(p.tl_lock);
+	if (err && !p.timeline_error) p.timeline_error = err;
+	if (value > p.timeline_done) p.timeline_done = value;
+	p.tl_cv.notify_all();
+}
+
+class TimelineWorker {
+public:
+	struct Item {
+		Processor *proc;
+		uint64_t value;
+		hipEvent_t ev;
+		int err;
+	};
+
+	// An event from the pool, or a new one (null if creation fails).
+	hipEvent_t acquire()
+	{
+		{
+			std::lock_guard<std::mutex> g(m);
+			if (!pool.empty()) {
+				hipEvent_t ev = pool.back();
+				pool.pop_back();
+				return ev;
+			}
+		}
+		hipEvent_t ev = nullptr;
+		if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return nullptr;
+		return ev;
+	}
+	void release(hipEvent_t ev)
+	{
+		std::lock_guard<std::mutex> g(m);
+		pool.push_back(ev);
+	}
+	void push(const Item &it)
+	{
+		std::lock_guard<std::mutex> g(m);
+		if (!started) {
+			started = true;
+			std::thread([this] { loop(); }).detach();
+		}
+		queue.push_back(it);
+		cv.notify_all();
+	}
+
+private:
+	std::mutex m;
+	std::condition_variable cv;
+	std::deque<Item> queue;
+	std::vector<hipEvent_t> pool;
+	bool started = false;
+
+	void loop()
+	{
+		for (;;) {
+			Item it;
+			{
+				std::unique_lock<std::mutex> g(m);
+				cv.wait(g, [this] { return !queue.empty(); });
+				it = queue.front();
+				queue.pop_front();
+			}
+			const hipError_t e = hipEventSynchronize(it.ev);
+			release(it.ev);
+			publish_timeline(*it.proc, it.value, it.err ? it.err : (e != hipSuccess ? RDP_MI_ERROR_DEVICE : 0));
+		}
+	}
+};
+
+static TimelineWorker &timeline_worker()
+{
+	// Never destroyed: its thread may still be parked on the queue at exit.
+	static TimelineWorker *w = new TimelineWorker();
+	return *w;
+}
+
+// Hand the processor's completed passes to the device (caller holds p.lock).
+static int submit_pending(Processor &p)
+{
+	if (p.fe.completed().empty()) return 0;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Processor *pp = &p;
+	return submit_processors(&pp, 1, false);
+}
+
+// MetaFlush: end the open pass and submit (caller holds p.lock).
+static int flush_now(Processor &p)
+{
+	p.fe.flush();
+	return submit_pending(p);
+}
+
+// MetaSignalTimeline (rdp_device.cpp:926-934): flush, then publish `value`
+// once the device has finished everything submitted so far.  Without an event
+// the stream is waited for here, so a value is never published early.
+static void signal_now(Processor &p, uint64_t value)
+{
+	const int err = flush_now(p);
+	Context &c = ctx();
+	TimelineWorker &w = timeline_worker();
+	hipEvent_t ev = w.acquire();
+	if (ev) {
+		std::lock_guard<std::mutex> g(c.lock);
+		if (hipEventRecord(ev, c.stream) != hipSuccess) {
+			w.release(ev);
+			ev = nullptr;
+		}
+	}
+	if (ev) {
+		w.push({ &p, value, ev, err });
+		return;
+	}
+	hipError_t e;
+	{
+		std::lock_guard<std::mutex> g(c.lock);
+		e = hipStreamSynchronize(c.stream);
+	}
+	publish_timeline(p, value, err ? err : (e != hipSuccess ? RDP_MI_ERROR_DEVICE : 0));
+}
+
+// Number of words of a command in the ring, meta commands included.
+static unsigned ring_command_length(uint32_t w0)
+{
+	const uint32_t op = (w0 >> 24) & 63;
+	if (op == kMetaSignal) return 3;
+	if (op == kMetaFlush) return 1;
+	return Frontend::command_length(w0);
+}
+
+// Decode complete commands in order (caller holds p.lock).  Completed passes
+// are handed to the device once enough queue up, bounding host memory.
+static void run_commands(Processor &p, const uint32_t *w, size_t n)
+{
+	size_t i = 0;
+	while (i < n) {
+		const uint32_t op = (w[i] >> 24) & 63;
+		const unsigned len = ring_command_length(w[i]);
+		if (len == 0) {
+			i++;
+			continue;
+		}
+		if (i + len > n) break;
+		if (op == kMetaSignal)
+			signal_now(p, (uint64_t)w[i + 1] | ((uint64_t)w[i + 2] << 32));
+		else if (op == kMetaFlush)
+			(void)flush_now(p);
+		else {
+			p.fe.command(w + i);
+			if (p.fe.completed().size() >= 32) (void)submit_pending(p);
+		}
+		i += len;
+	}
+}
+
+// The ring thread (CommandRing::thread_loop, command_ring.cpp:86-134).  When
+// no command arrives for 500 us while passes are waiting, it hands them to
+// the device (the reference's MetaIdle).
+static void ring_loop(Processor *p)
+{
+	CommandRing &r = *p->ring;
+	std::vector<uint32_t> batch;
+	for (;;) {
+		bool idle = false;
+		{
+			std::unique_lock<std::mutex> g(r.lock);
+			auto has_work = [&r] { return r.write_count > r.read_count || r.stop; };
+			if (r.pending)
+				idle = !r.cv.wait_for(g, std::chrono::microseconds(500), has_work);
+			else
+				r.cv.wait(g, has_work);
+			if (!idle) {
+				if (r.write_count == r.read_count) break;  // stop requested, ring empty
+				const size_t n = (size_t)(r.write_count - r.read_count);
+				batch.resize(n);
+				for (size_t k = 0; k < n; k++) batch[k] = r.words[(r.read_count + k) & (kRingWords - 1)];
+				r.read_count += n;
+				r.cv.notify_all();  // space for producers
+			}
+		}
+		bool pending;
+		{
+			std::lock_guard<std::mutex> g(p->lock);
+			if (idle)
+				(void)submit_pending(*p);
+			else
+				run_commands(*p, batch.data(), batch.size());
+			pending = !p->fe.completed().empty();
+		}
+		std::lock_guard<std::mutex> g(r.lock);
+		r.pending = pending;
+		if (!idle) {
+			r.completed_count = r.read_count;
+			r.cv.notify_all();  // drains
+		}
+	}
+}
+
+static CommandRing &ensure_ring(Processor &p)
+{
+	std::lock_guard<std::mutex> g(p.ring_create);
+	if (!p.ring) {
+		p.ring.reset(new CommandRing());
+		p.ring->thr = std::thread(ring_loop, &p);
+	}
+	return *p.ring;
+}
+
+// Copy complete commands (n <= kRingWords words) into the ring.
+static void ring_push(Processor &p, const uint32_t *w, size_t n)
+{
+	CommandRing &r = ensure_ring(p);
+	std::unique_lock<std::mutex> g(r.lock);
+	r.cv.wait(g, [&] { return r.write_count + n <= r.read_count + kRingWords; });
+	for (size_t k = 0; k < n; k++) r.words[(r.write_count + k) & (kRingWords - 1)] = w[k];
+	r.write_count += n;
+	r.cv.notify_all();
+}
+
+// Wait until the ring thread has decoded everything enqueued so far
+// (CommandRing::drain, command_ring.cpp:63-69).
+static void ring_drain(Processor &p)
+{
+	CommandRing *r;
+	{
+		std::lock_guard<std::mutex> g(p.ring_create);
+		r = p.ring.get();
+	}
+	if (!r) return;
+	std::unique_lock<std::mutex> g(r->lock);
+	r->cv.wait(g, [r] { return r->completed_count == r->write_count; });
+}
+
+static bool has_ring(Processor &p)
+{
+	std::lock_guard<std::mutex> g(p.ring_create);
+	return p.ring != nullptr;
+}
+
+// Send a meta command in-band when the processor has a ring, else run it
+// on the calling thread.
+static void meta_command(Processor &p, const uint32_t *w, size_t n)
+{
+	if (has_ring(p)) {
+		ring_push(p, w, n);
+		return;
+	}
+	std::lock_guard<std::mutex> g(p.lock);
+	run_commands(p, w, n);
+}
+
+static void stop_ring(Processor &p)
+{
+	CommandRing *r;
+	{
+		std::lock_guard<std::mutex> g(p.ring_create);
+		r = p.ring.get();
+	}
+	if (!r) return;
+	{
+		std::lock_guard<std::mutex> g(r->lock);
+		r->stop = true;
+		r->cv.notify_all();
+	}
+	if (r->thr.joinable()) r->thr.join();
+}
+
 // A recorded submission: device-resident inputs and scratch for a fixed set
 // of passes, replayable as direct launches or as one captured hipGraph.
 struct Recording {
@@ -923,18 +1231,21 @@ void Server::decode(ServeRequest req, size_t bytes, double admit)
 {
 	Processor *p = procs[req.instance];
 	const std::vector<uint32_t> &w = req.words;
-	size_t i = 0;
-	while (i < w.size()) {
-		const unsigned len = Frontend::command_length(w[i]);
-		if (len == 0) {
-			i++;
-			continue;
+	{
+		std::lock_guard<std::mutex> pg(p->lock);
+		size_t i = 0;
+		while (i < w.size()) {
+			const unsigned len = Frontend::command_length(w[i]);
+			if (len == 0) {
+				i++;
+				continue;
+			}
+			if (i + len > w.size()) break;
+			p->fe.command(w.data() + i);
+			i += len;
 		}
-		if (i + len > w.size()) break;
-		p->fe.command(w.data() + i);
-		i += len;
+		p->fe.flush();
 	}
-	p->fe.flush();
 	std::lock_guard<std::mutex> g(m);
 	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0 }, bytes });
 	submit_cv.notify_all();  // any lane's submitter may be the free one
@@ -1077,6 +1388,15 @@ struct rdp_mi_processor {
 	explicit rdp_mi_processor(uint32_t size) : p(size) {}
 };
 
+// VIScanoutBuffer (rdp_device.hpp:53-58): pinned host pixels and a fence.
+static constexpr size_t kScanoutBufferWords = 640 * 576;
+struct rdp_mi_scanout_buffer {
+	uint32_t *pixels = nullptr;
+	hipEvent_t fence = nullptr;
+	unsigned width = 0, height = 0;
+	bool in_flight = false;
+};
+
 extern "C" {
 
 const char *rdp_mi_last_error(void) { return g_last_error.c_str(); }
@@ -1135,7 +1455,7 @@ void rdp_mi_destroy(rdp_mi_processor *h)
 	if (!h) return;
 	rdp_mi_idle(h);
 	Processor &p = h->p;
-	for (auto &kv : p.timeline_events) (void)hipEventDestroy(kv.second);
+	stop_ring(p);
 	p.vi.release();
 	(void)hipFree(p.d_rdram);
 	(void)hipFree(p.d_hidden);
@@ -1150,35 +1470,38 @@ void rdp_mi_enqueue_command_direct(rdp_mi_processor *h, unsigned num_words, cons
 	if (!h || !words || num_words == 0) return;
 	unsigned len = Frontend::command_length(words[0]);
 	if (len == 0 || len > num_words) return;
-	h->p.fe.command(words);
-	// Bound host memory: hand completed passes to the GPU once enough queue up.
-	if (h->p.fe.completed().size() >= 32) {
-		Context &c = ctx();
-		std::lock_guard<std::mutex> g(c.lock);
-		Processor *pp = &h->p;
-		(void)submit_processors(&pp, 1, false);
-	}
+	Processor &p = h->p;
+	ring_drain(p);
+	std::lock_guard<std::mutex> g(p.lock);
+	run_commands(p, words, len);
 }
 
 void rdp_mi_enqueue_command(rdp_mi_processor *h, unsigned num_words, const uint32_t *words)
 {
-	rdp_mi_enqueue_command_direct(h, num_words, words);
+	if (!h || !words || num_words == 0) return;
+	unsigned len = Frontend::command_length(words[0]);
+	if (len == 0 || len > num_words) return;
+	ring_push(h->p, words, len);
 }
 
 size_t rdp_mi_enqueue_command_list(rdp_mi_processor *h, size_t num_words, const uint32_t *words)
 {
 	if (!h || !words) return 0;
-	size_t i = 0;
+	size_t i = 0, start = 0;
+	// Hand runs of complete commands to the ring, at most kRingWords at once;
+	// words with an undefined or private opcode end a run and are skipped.
 	while (i < num_words) {
 		unsigned len = Frontend::command_length(words[i]);
-		if (len == 0) {
-			i++;
+		if (len == 0 || i + len > num_words || i + len - start > kRingWords) {
+			if (i > start) ring_push(h->p, words + start, i - start);
+			if (len != 0 && i + len > num_words) return i;
+			if (len == 0) i++;
+			start = i;
 			continue;
 		}
-		if (i + len > num_words) break;
-		rdp_mi_enqueue_command_direct(h, len, words + i);
 		i += len;
 	}
+	if (i > start) ring_push(h->p, words + start, i - start);
 	return i;
 }
 
@@ -1186,66 +1509,67 @@ void rdp_mi_set_quirks(rdp_mi_processor *h, uint32_t quirks)
 {
 	if (!h) return;
 	const uint32_t w[2] = { 4u << 24, quirks };
-	h->p.fe.command(w);
+	meta_command(h->p, w, 2);
 }
 
 void rdp_mi_flush(rdp_mi_processor *h)
 {
 	if (!h) return;
-	h->p.fe.flush();
-	Context &c = ctx();
-	std::lock_guard<std::mutex> g(c.lock);
-	Processor *pp = &h->p;
-	(void)submit_processors(&pp, 1, false);
+	const uint32_t w[1] = { kMetaFlush << 24 };
+	meta_command(h->p, w, 1);
 }
 
 uint64_t rdp_mi_signal_timeline(rdp_mi_processor *h)
 {
 	if (!h) return 0;
-	rdp_mi_flush(h);
-	Context &c = ctx();
-	std::lock_guard<std::mutex> g(c.lock);
 	Processor &p = h->p;
-	uint64_t v = ++p.timeline;
-	hipEvent_t ev;
-	if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return v;
-	(void)hipEventRecord(ev, c.stream);
-	p.timeline_events[v] = ev;
+	const uint64_t v = ++p.timeline;
+	const uint32_t w[3] = { kMetaSignal << 24, (uint32_t)v, (uint32_t)(v >> 32) };
+	meta_command(p, w, 3);
 	return v;
+}
+
+int rdp_mi_wait_for_timeline_timeout(rdp_mi_processor *h, uint64_t value, uint64_t timeout_us)
+{
+	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor");
+	Processor &p = h->p;
+	std::unique_lock<std::mutex> g(p.tl_lock);
+	auto ready = [&] { return p.timeline_done >= value || p.timeline_error != 0; };
+	if (timeout_us == UINT64_MAX)
+		p.tl_cv.wait(g, ready);
+	else if (!p.tl_cv.wait_for(g, std::chrono::microseconds((int64_t)std::min<uint64_t>(timeout_us, 1ull << 50)), ready))
+		return set_error(RDP_MI_ERROR_TIMEOUT, "timeline value %llu not reached within %llu us",
+		                 (unsigned long long)value, (unsigned long long)timeout_us);
+	if (p.timeline_error) return set_error(p.timeline_error, "device error in the processor's work");
+	return 0;
 }
 
 int rdp_mi_wait_for_timeline(rdp_mi_processor *h, uint64_t value)
 {
-	if (!h) return RDP_MI_ERROR_INVALID_ARGUMENT;
-	Processor &p = h->p;
-	hipEvent_t ev = nullptr;
-	{
-		Context &c = ctx();
-		std::lock_guard<std::mutex> g(c.lock);
-		auto it = p.timeline_events.find(value);
-		if (it == p.timeline_events.end()) return 0;
-		ev = it->second;
-	}
-	HIP_CHECK(hipEventSynchronize(ev));
-	Context &c = ctx();
-	std::lock_guard<std::mutex> g(c.lock);
-	for (auto it = p.timeline_events.begin(); it != p.timeline_events.end();) {
-		if (it->first <= value) {
-			(void)hipEventDestroy(it->second);
-			it = p.timeline_events.erase(it);
-		} else
-			++it;
-	}
-	return 0;
+	return rdp_mi_wait_for_timeline_timeout(h, value, UINT64_MAX);
+}
+
+uint64_t rdp_mi_completed_timeline(rdp_mi_processor *h)
+{
+	if (!h) return 0;
+	std::lock_guard<std::mutex> g(h->p.tl_lock);
+	return h->p.timeline_done;
 }
 
 int rdp_mi_idle(rdp_mi_processor *h)
 {
-	if (!h) return RDP_MI_ERROR_INVALID_ARGUMENT;
+	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor");
 	return rdp_mi_wait_for_timeline(h, rdp_mi_signal_timeline(h));
 }
 
-void rdp_mi_begin_frame_context(rdp_mi_processor *h) { rdp_mi_flush(h); }
+// CommandProcessor::begin_frame_context (rdp_device.cpp:169-174): flush and
+// drain the ring.
+void rdp_mi_begin_frame_context(rdp_mi_processor *h)
+{
+	if (!h) return;
+	rdp_mi_flush(h);
+	ring_drain(h->p);
+}
 
 void *rdp_mi_begin_read_rdram(rdp_mi_processor *h)
 {
@@ -1304,20 +1628,31 @@ void rdp_mi_scanout_options_default(rdp_mi_scanout_options *o)
 	o->vi_aa = o->vi_scale = o->vi_serrate = o->vi_dither_filter = o->vi_divot_filter = o->vi_gamma_dither = 1;
 }
 
+// CommandProcessor::scanout (rdp_device.cpp:1082-1115): drain the ring, end
+// the open pass and submit the processor's rendering, then queue the VI
+// scanout behind it on the same stream, copying the image to `out` (host
+// memory).  With `done` the copy is left in flight and `done` is recorded
+// behind it; otherwise the call waits for it.
+static int scanout_queue(Processor &p, const rdp_mi_scanout_options &opts, uint32_t *out, unsigned *width,
+                         unsigned *height, hipEvent_t done)
+{
+	ring_drain(p);
+	std::lock_guard<std::mutex> g(p.lock);
+	if (int err = flush_now(p)) return err;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lg(c.lock);
+	int err = vi_scanout(p.vi, p.vi_regs, p.d_rdram, p.d_hidden, p.rdram_size, c.d_gamma, c.stream, opts, out, width,
+	                     height, done == nullptr);
+	if (err) return set_error(err, "VI scanout failed");
+	if (done) HIP_CHECK(hipEventRecord(done, c.stream));
+	return 0;
+}
+
 int rdp_mi_scanout_sync_ex(rdp_mi_processor *h, const rdp_mi_scanout_options *opts, uint32_t *out, unsigned *width,
                            unsigned *height)
 {
 	if (!h || !opts || !out || !width || !height) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
-	// CommandProcessor::scanout drains and flushes rendering first (rdp_device.cpp:1082-1106).
-	int err = rdp_mi_idle(h);
-	if (err) return err;
-	Context &c = ctx();
-	std::lock_guard<std::mutex> g(c.lock);
-	Processor &p = h->p;
-	err = vi_scanout(p.vi, p.vi_regs, p.d_rdram, p.d_hidden, p.rdram_size, c.d_gamma, c.stream, *opts, out, width,
-	                 height);
-	if (err) return set_error(err, "VI scanout failed");
-	return 0;
+	return scanout_queue(h->p, *opts, out, width, height, nullptr);
 }
 
 int rdp_mi_scanout_sync(rdp_mi_processor *h, uint32_t *out, unsigned *width, unsigned *height)
@@ -1327,29 +1662,93 @@ int rdp_mi_scanout_sync(rdp_mi_processor *h, uint32_t *out, unsigned *width, uns
 	return rdp_mi_scanout_sync_ex(h, &opts, out, width, height);
 }
 
+int rdp_mi_scanout_buffer_create(rdp_mi_scanout_buffer **out)
+{
+	if (!out) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	*out = nullptr;
+	rdp_mi_scanout_buffer *b = new (std::nothrow) rdp_mi_scanout_buffer();
+	if (!b) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	if (hipHostMalloc((void **)&b->pixels, kScanoutBufferWords * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+		delete b;
+		return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "scanout buffer allocation failed");
+	}
+	if (hipEventCreateWithFlags(&b->fence, hipEventDisableTiming) != hipSuccess) {
+		(void)hipHostFree(b->pixels);
+		delete b;
+		return set_error(RDP_MI_ERROR_DEVICE, "scanout fence creation failed");
+	}
+	*out = b;
+	return 0;
+}
+
+void rdp_mi_scanout_buffer_destroy(rdp_mi_scanout_buffer *b)
+{
+	if (!b) return;
+	if (b->in_flight) (void)hipEventSynchronize(b->fence);
+	(void)hipEventDestroy(b->fence);
+	(void)hipHostFree(b->pixels);
+	delete b;
+}
+
+int rdp_mi_scanout_async_buffer(rdp_mi_processor *h, const rdp_mi_scanout_options *opts, rdp_mi_scanout_buffer *b)
+{
+	if (!h || !opts || !b) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	// The buffer's previous copy must land before the buffer is reused.
+	if (b->in_flight) HIP_CHECK(hipEventSynchronize(b->fence));
+	b->in_flight = false;
+	b->width = b->height = 0;
+	int err = scanout_queue(h->p, *opts, b->pixels, &b->width, &b->height, b->fence);
+	if (err) return err;
+	b->in_flight = true;
+	return 0;
+}
+
+int rdp_mi_scanout_buffer_wait(rdp_mi_scanout_buffer *b, const uint32_t **pixels, unsigned *width, unsigned *height)
+{
+	if (!b) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null buffer");
+	if (b->in_flight) {
+		const hipError_t e = hipEventSynchronize(b->fence);
+		if (e != hipSuccess) return set_error(RDP_MI_ERROR_DEVICE, "scanout fence: %s", hipGetErrorString(e));
+		b->in_flight = false;
+	}
+	if (pixels) *pixels = b->pixels;
+	if (width) *width = b->width;
+	if (height) *height = b->height;
+	return 0;
+}
+
 void rdp_mi_begin_vi_register_per_scanline(rdp_mi_processor *h, uint32_t flags)
 {
-	if (h) h->p.vi.begin_per_scanline(h->p.vi_regs, flags);
+	if (!h) return;
+	std::lock_guard<std::mutex> g(h->p.lock);
+	h->p.vi.begin_per_scanline(h->p.vi_regs, flags);
 }
 
 void rdp_mi_set_vi_register_for_scanline(rdp_mi_processor *h, uint32_t reg_bit, uint32_t value)
 {
-	if (h) h->p.vi.set_for_scanline(reg_bit, value);
+	if (!h) return;
+	std::lock_guard<std::mutex> g(h->p.lock);
+	h->p.vi.set_for_scanline(reg_bit, value);
 }
 
 void rdp_mi_latch_vi_register_for_scanline(rdp_mi_processor *h, unsigned vi_line)
 {
-	if (h) h->p.vi.latch_for_scanline(vi_line);
+	if (!h) return;
+	std::lock_guard<std::mutex> g(h->p.lock);
+	h->p.vi.latch_for_scanline(vi_line);
 }
 
 void rdp_mi_end_vi_register_per_scanline(rdp_mi_processor *h)
 {
-	if (h) h->p.vi.end_per_scanline();
+	if (!h) return;
+	std::lock_guard<std::mutex> g(h->p.lock);
+	h->p.vi.end_per_scanline();
 }
 
 void rdp_mi_set_validation_callback(rdp_mi_processor *h, rdp_mi_validation_fn fn, void *user)
 {
 	if (!h) return;
+	std::lock_guard<std::mutex> g(h->p.lock);
 	h->p.validation_fn = fn;
 	h->p.validation_user = user;
 	if (fn)
@@ -1358,39 +1757,51 @@ void rdp_mi_set_validation_callback(rdp_mi_processor *h, rdp_mi_validation_fn fn
 		h->p.fe.set_validation_callback(nullptr);
 }
 
+// The distinct processors of a list, their rings drained and their locks
+// taken in address order (the order every multi-processor entry point uses).
+struct ProcessorSet {
+	std::vector<Processor *> ps;
+	std::vector<std::unique_lock<std::mutex>> locks;
+	int init(rdp_mi_processor *const *procs, unsigned count)
+	{
+		if (!procs && count) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor list");
+		for (unsigned i = 0; i < count; i++) {
+			if (!procs[i]) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
+			if (std::find(ps.begin(), ps.end(), &procs[i]->p) == ps.end()) ps.push_back(&procs[i]->p);
+		}
+		for (Processor *p : ps) ring_drain(*p);
+		std::vector<Processor *> order(ps);
+		std::sort(order.begin(), order.end());
+		for (Processor *p : order) locks.emplace_back(p->lock);
+		for (Processor *p : ps) p->fe.flush();
+		return 0;
+	}
+};
+
 int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count)
 {
-	if (!procs && count) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor list");
-	std::vector<Processor *> ps;
-	ps.reserve(count);
-	for (unsigned i = 0; i < count; i++) {
-		if (!procs[i]) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
-		procs[i]->p.fe.flush();
-		ps.push_back(&procs[i]->p);
-	}
+	ProcessorSet set;
+	if (int err = set.init(procs, count)) return err;
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
-	return submit_processors(ps.data(), count, true);
+	return submit_processors(set.ps.data(), (unsigned)set.ps.size(), true);
 }
 
 uint64_t rdp_mi_primitive_count(const rdp_mi_processor *h) { return h ? h->p.fe.primitive_counter() : 0; }
 
 int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, rdp_mi_recording **out)
 {
-	if (!out || (!procs && count)) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	if (!out) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
 	*out = nullptr;
-	std::vector<Processor *> ps;
-	for (unsigned i = 0; i < count; i++) {
-		if (!procs[i]) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
-		procs[i]->p.fe.flush();
-		ps.push_back(&procs[i]->p);
-	}
+	ProcessorSet set;
+	if (int err = set.init(procs, count)) return err;
+	std::vector<Processor *> &ps = set.ps;
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	rdp_mi_recording *h = new (std::nothrow) rdp_mi_recording();
 	if (!h) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
 	std::vector<std::vector<PassRef>> rounds;
-	build_round_list(ps.data(), count, batched != 0, rounds);
+	build_round_list(ps.data(), (unsigned)ps.size(), batched != 0, rounds);
 	int err = prepare_rounds(c, rounds, h->r.input, h->r.scratch, h->r.launches);
 	for (auto *p : ps) p->fe.completed().clear();
 	if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "upload failed");

@@ -107,6 +107,12 @@ void ViState::end_per_scanline()
 
 void ViState::release()
 {
+	if (params_done) {
+		if (params_busy) (void)hipEventSynchronize(params_done);
+		(void)hipEventDestroy(params_done);
+		params_done = nullptr;
+		params_busy = false;
+	}
 	for (auto &p : d_images)
 		if (p) (void)hipFree(p), p = nullptr;
 	if (d_filter) (void)hipFree(d_filter), d_filter = nullptr;
@@ -478,12 +484,18 @@ __global__ __launch_bounds__(256) void k_vi_scale(const ViParams *__restrict__ p
 // no downscale or deinterlace pass).
 int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const uint8_t *d_hidden, uint32_t rdram_size,
                const uint8_t *d_gamma, hipStream_t stream, const rdp_mi_scanout_options &opts, uint32_t *out,
-               unsigned *width, unsigned *height)
+               unsigned *width, unsigned *height, bool sync)
 {
 	if (!s.h_params) {
 		VI_CHECK(hipHostMalloc((void **)&s.h_params, sizeof(ViParams), hipHostMallocDefault));
 		VI_CHECK(hipMalloc((void **)&s.d_params, sizeof(ViParams)));
 		for (auto &img : s.d_images) VI_CHECK(hipMalloc((void **)&img, 640 * 576 * sizeof(uint32_t)));
+		VI_CHECK(hipEventCreateWithFlags(&s.params_done, hipEventDisableTiming));
+	}
+	// The previous scanout's parameter upload reads the pinned block.
+	if (s.params_busy) {
+		VI_CHECK(hipEventSynchronize(s.params_done));
+		s.params_busy = false;
 	}
 	ViParams &P = *s.h_params;
 	memset(&P, 0, sizeof(P));
@@ -524,7 +536,7 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 			if (s.prev_valid) {
 				VI_CHECK(hipMemcpyAsync(out, s.d_images[s.cur ^ 1], (size_t)s.prev_w * s.prev_h * 4, hipMemcpyDeviceToHost,
 				                        stream));
-				VI_CHECK(hipStreamSynchronize(stream));
+				if (sync) VI_CHECK(hipStreamSynchronize(stream));
 				*width = s.prev_w;
 				*height = s.prev_h;
 			}
@@ -629,6 +641,8 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 		}
 	}
 	VI_CHECK(hipMemcpyAsync(s.d_params, s.h_params, sizeof(ViParams), hipMemcpyHostToDevice, stream));
+	VI_CHECK(hipEventRecord(s.params_done, stream));
+	s.params_busy = true;
 	if (!degenerate) {
 		dim3 grid((unsigned)((P.aw + 15) / 16), (unsigned)((P.ah + 15) / 16));
 		hipLaunchKernelGGL(k_vi_filter, grid, dim3(256), 0, stream, s.d_params, d_rdram, d_hidden, s.d_filter);
@@ -642,7 +656,7 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 		VI_CHECK(hipGetLastError());
 	}
 	VI_CHECK(hipMemcpyAsync(out, dst, (size_t)P.ow * (size_t)P.oh * 4, hipMemcpyDeviceToHost, stream));
-	VI_CHECK(hipStreamSynchronize(stream));
+	if (sync) VI_CHECK(hipStreamSynchronize(stream));
 	*width = (unsigned)P.ow;
 	*height = (unsigned)P.oh;
 	s.prev_w = (unsigned)P.ow;

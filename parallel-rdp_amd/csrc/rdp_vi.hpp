@@ -48,6 +48,8 @@ struct ViState {
 	size_t filter_words = 0;
 	ViParams *h_params = nullptr;                   // pinned
 	ViParams *d_params = nullptr;
+	hipEvent_t params_done = nullptr;               // the upload of h_params has been read
+	bool params_busy = false;
 
 	uint32_t per_line_flags = 0;
 	uint32_t h_start_latched = 0, x_scale_latched = 0;
@@ -63,11 +65,13 @@ struct ViState {
 };
 
 // Scans out the processor's current RDRAM through the VI into out
-// (width x height RGBA8 words, byte order R, G, B, A).  Caller has drained
-// rendering on `stream`.  Returns 0 or an RDP_MI_ERROR_* code.
+// (width x height RGBA8 words, byte order R, G, B, A), queued on `stream`
+// behind the rendering.  With sync the call waits for the copy to `out`;
+// otherwise the copy is still in flight on `stream` when it returns.
+// Returns 0 or an RDP_MI_ERROR_* code.
 int vi_scanout(ViState &vi, const uint32_t *regs, const uint8_t *d_rdram, const uint8_t *d_hidden, uint32_t rdram_size,
                const uint8_t *d_gamma, hipStream_t stream, const rdp_mi_scanout_options &opts, uint32_t *out,
-               unsigned *width, unsigned *height);
+               unsigned *width, unsigned *height, bool sync);
 
 // Gamma table (luts.hpp:1034-1555) from its closed form, 256 + 16384 bytes.
 void vi_gamma_table(uint8_t *out);

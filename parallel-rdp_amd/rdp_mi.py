@@ -27,6 +27,10 @@ class RDPError(RuntimeError):
     pass
 
 
+ERROR_TIMEOUT = -5
+ERROR_LIMIT = -6
+
+
 class ScanoutOptions(ctypes.Structure):
     """rdp_mi_scanout_options (the reference's ScanoutOptions without upscaling)."""
     _fields_ = [(n, ctypes.c_uint) for n in ("crop_overscan_pixels", "crop_left", "crop_right", "crop_top",
@@ -67,6 +71,10 @@ def load_library():
     lib.rdp_mi_signal_timeline.argtypes = [vp]
     lib.rdp_mi_wait_for_timeline.restype = ctypes.c_int
     lib.rdp_mi_wait_for_timeline.argtypes = [vp, u64]
+    lib.rdp_mi_wait_for_timeline_timeout.restype = ctypes.c_int
+    lib.rdp_mi_wait_for_timeline_timeout.argtypes = [vp, u64, u64]
+    lib.rdp_mi_completed_timeline.restype = u64
+    lib.rdp_mi_completed_timeline.argtypes = [vp]
     lib.rdp_mi_idle.restype = ctypes.c_int
     lib.rdp_mi_idle.argtypes = [vp]
     lib.rdp_mi_begin_frame_context.argtypes = [vp]
@@ -86,6 +94,14 @@ def load_library():
     lib.rdp_mi_scanout_sync_ex.restype = ctypes.c_int
     lib.rdp_mi_scanout_sync_ex.argtypes = [vp, ctypes.POINTER(ScanoutOptions), vp, ctypes.POINTER(cu),
                                            ctypes.POINTER(cu)]
+    lib.rdp_mi_scanout_buffer_create.restype = ctypes.c_int
+    lib.rdp_mi_scanout_buffer_create.argtypes = [ctypes.POINTER(vp)]
+    lib.rdp_mi_scanout_buffer_destroy.argtypes = [vp]
+    lib.rdp_mi_scanout_async_buffer.restype = ctypes.c_int
+    lib.rdp_mi_scanout_async_buffer.argtypes = [vp, ctypes.POINTER(ScanoutOptions), vp]
+    lib.rdp_mi_scanout_buffer_wait.restype = ctypes.c_int
+    lib.rdp_mi_scanout_buffer_wait.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32)),
+                                               ctypes.POINTER(cu), ctypes.POINTER(cu)]
     lib.rdp_mi_begin_vi_register_per_scanline.argtypes = [vp, u32]
     lib.rdp_mi_set_vi_register_for_scanline.argtypes = [vp, u32, u32]
     lib.rdp_mi_latch_vi_register_for_scanline.argtypes = [vp, cu]
@@ -159,8 +175,21 @@ class CommandProcessor:
     def signal_timeline(self) -> int:
         return int(self._lib.rdp_mi_signal_timeline(self._h))
 
-    def wait_for_timeline(self, value: int):
-        _check(self._lib.rdp_mi_wait_for_timeline(self._h, value), "wait_for_timeline")
+    def wait_for_timeline(self, value: int, timeout_us: Optional[int] = None):
+        """Blocks until `value` is published (also for a value not yet
+        signalled).  With timeout_us, raises TimeoutError when it is not
+        published in time."""
+        if timeout_us is None:
+            _check(self._lib.rdp_mi_wait_for_timeline(self._h, value), "wait_for_timeline")
+            return
+        ret = self._lib.rdp_mi_wait_for_timeline_timeout(self._h, value, int(timeout_us))
+        if ret == ERROR_TIMEOUT:
+            raise TimeoutError(last_error())
+        _check(ret, "wait_for_timeline")
+
+    @property
+    def completed_timeline(self) -> int:
+        return int(self._lib.rdp_mi_completed_timeline(self._h))
 
     def idle(self):
         _check(self._lib.rdp_mi_idle(self._h), "idle")
@@ -223,6 +252,12 @@ class CommandProcessor:
             return out[:0].reshape(0, 0)
         return out[: w.value * h.value].reshape(h.value, w.value)
 
+    def scanout_async(self, buffer: "ScanoutBuffer", options: Optional[ScanoutOptions] = None):
+        """Queue a scanout into `buffer` and return at once
+        (CommandProcessor::scanout_async_buffer)."""
+        opts = options if options is not None else ScanoutOptions.default()
+        _check(self._lib.rdp_mi_scanout_async_buffer(self._h, ctypes.byref(opts), buffer.handle), "scanout_async")
+
     def begin_vi_register_per_scanline(self, flags: int = 3):
         self._lib.rdp_mi_begin_vi_register_per_scanline(self._h, flags)
 
@@ -245,6 +280,42 @@ class CommandProcessor:
     @property
     def primitive_count(self) -> int:
         return int(self._lib.rdp_mi_primitive_count(self._h))
+
+
+class ScanoutBuffer:
+    """VIScanoutBuffer: a host image of one scanout plus its fence."""
+
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.rdp_mi_scanout_buffer_create(ctypes.byref(h)), "scanout_buffer_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def wait(self) -> np.ndarray:
+        """Waits for the fence; the image as a (height, width) array."""
+        px = ctypes.POINTER(ctypes.c_uint32)()
+        w = ctypes.c_uint(0)
+        h = ctypes.c_uint(0)
+        _check(self._lib.rdp_mi_scanout_buffer_wait(self._h, ctypes.byref(px), ctypes.byref(w), ctypes.byref(h)),
+               "scanout_buffer_wait")
+        if not w.value or not h.value:
+            return np.zeros((0, 0), np.uint32)
+        return np.ctypeslib.as_array(px, shape=(h.value * w.value,)).copy().reshape(h.value, w.value)
+
+    def close(self):
+        if self._h:
+            self._lib.rdp_mi_scanout_buffer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def set_launch_config(xcd_chunk: int = 64, min_class: int = 0):

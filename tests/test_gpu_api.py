@@ -113,3 +113,134 @@ def test_small_rdram_wraps_like_the_oracle():
     assert np.array_equal(p.read_hidden_rdram(), o.hidden)
     o.close()
     p.close()
+
+
+def _prepare(proc, s):
+    proc.write_rdram(s.rdram_init if s.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
+    proc.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
+
+
+def test_wait_blocks_until_a_later_signal(proc):
+    """wait_for_timeline on a value not yet signalled blocks until another
+    thread's signal_timeline reaches it (worker_thread.hpp:69-73)."""
+    import threading
+    s = rs.config0_stream(num_triangles=64)
+    _prepare(proc, s)
+    target = proc.signal_timeline() + 1
+    proc.wait_for_timeline(target - 1)
+    done = threading.Event()
+    err = []
+
+    def waiter():
+        try:
+            proc.wait_for_timeline(target, timeout_us=30_000_000)
+        except Exception as e:  # noqa: BLE001
+            err.append(e)
+        done.set()
+
+    t = threading.Thread(target=waiter)
+    t.start()
+    assert not done.wait(0.3), "wait_for_timeline returned before its value was signalled"
+    proc.enqueue_commands(s.frames[0].words)
+    assert proc.signal_timeline() == target
+    assert done.wait(30), "wait_for_timeline did not return after the signal"
+    t.join()
+    assert not err, err
+    assert proc.completed_timeline >= target
+    total, lines = harness.compare((proc.read_rdram(), proc.read_hidden_rdram(), proc.read_tmem()),
+                                   harness.run_oracle(s))
+    assert total == 0, lines
+
+
+def test_wait_with_timeout_reports_an_unsignalled_value(proc):
+    import rdp_mi  # noqa: F401
+    v = proc.signal_timeline()
+    proc.wait_for_timeline(v, timeout_us=10_000_000)
+    with pytest.raises(TimeoutError):
+        proc.wait_for_timeline(v + 1, timeout_us=20_000)
+
+
+def test_direct_commands_follow_the_ring_in_order(proc):
+    """enqueue_command_direct decodes after everything already in the ring."""
+    s = rs.batch_stream(5)
+    _prepare(proc, s)
+    words = s.frames[0].words
+    i, half = 0, len(words) // 2
+    while i < half:
+        i += int(rs.COMMAND_LENGTHS[(int(words[i]) >> 24) & 63])
+    proc.enqueue_commands(words[:i])
+    j = i
+    import ctypes
+    lib = proc._lib
+    while j < len(words):
+        n = int(rs.COMMAND_LENGTHS[(int(words[j]) >> 24) & 63])
+        w = np.ascontiguousarray(words[j:j + n], dtype=np.uint32)
+        lib.rdp_mi_enqueue_command_direct(proc.handle, ctypes.c_uint(n), w.ctypes.data)
+        j += n
+    proc.idle()
+    total, lines = harness.compare((proc.read_rdram(), proc.read_hidden_rdram(), proc.read_tmem()),
+                                   harness.run_oracle(s))
+    assert total == 0, lines
+
+
+def test_processors_on_separate_threads():
+    """Each processor has its own lock and ring: four threads drive four
+    processors at once and every result equals the oracle's."""
+    import threading
+    import rdp_mi
+    streams = [rs.batch_stream(k) for k in (1, 4, 9, 12)]
+    procs = [rdp_mi.CommandProcessor(rs.RDRAM_SIZE) for _ in streams]
+    for p, s in zip(procs, streams):
+        _prepare(p, s)
+    errs = []
+
+    def drive(p, s):
+        try:
+            for f in s.frames:
+                w = f.words
+                k = 0
+                while k < len(w):  # in several pieces, with signals between
+                    e = min(len(w), k + 4096)
+                    while e < len(w) and rs.COMMAND_LENGTHS[(int(w[e]) >> 24) & 63] == 0:
+                        e += 1
+                    k += p.enqueue_commands(w[k:e])
+                    p.signal_timeline()
+            p.idle()
+        except Exception as ex:  # noqa: BLE001
+            errs.append(ex)
+
+    threads = [threading.Thread(target=drive, args=(p, s)) for p, s in zip(procs, streams)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errs, errs
+    for p, s in zip(procs, streams):
+        total, lines = harness.compare((p.read_rdram(), p.read_hidden_rdram(), p.read_tmem()), harness.run_oracle(s))
+        assert total == 0, lines
+        p.close()
+
+
+def test_async_scanout_buffer_matches_scanout_sync():
+    """scanout_async_buffer returns before the image is ready; after the
+    fence it holds the same image as scanout_sync of the same frame."""
+    import rdp_mi
+    s = rs.config2_stream(frames=2)
+    a = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+    b = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+    buf = rdp_mi.ScanoutBuffer()
+    for p in (a, b):
+        _prepare(p, s)
+    for f in s.frames:
+        for p in (a, b):
+            p.enqueue_commands(f.words)
+            for r, v in f.vi.items():
+                p.set_vi_register(r, v)
+        want = a.scanout_sync()
+        b.scanout_async(buf)
+        got = buf.wait()
+        assert want.shape == got.shape and want.size > 0
+        assert np.array_equal(want, got)
+    buf.close()
+    a.close()
+    b.close()
