@@ -18,6 +18,15 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* A build with -DRDPO_PLANT=n carries exactly one deliberate mistake, the
+ * one marked PLANTED(n) below.  tests/test_oracle_pins.py builds such
+ * variants to show that the pin of each function fails on a mistake in that
+ * function.  The normal build (RDPO_PLANT 0) has none of them. */
+#ifndef RDPO_PLANT
+#define RDPO_PLANT 0
+#endif
+#define PLANTED(n) (RDPO_PLANT == (n))
+
 /* ------------------------------------------------------------------------ */
 /* Flag bits.  Values follow parallel-rdp/rdp_data_structures.hpp:33-90 so   */
 /* that the state words read the same as the reference's.                   */
@@ -429,6 +438,7 @@ uint64_t rdpo_primitive_count(const rdpo *o) { return o->prim_counter; }
 static int32_t quantize_x(int32_t x)
 {
 	int32_t sticky = (x & 0xfff) != 0;
+	if (PLANTED(1)) sticky = 0;
 	return (x >> 12) | sticky;
 }
 
@@ -459,7 +469,7 @@ static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t sc
 			}
 		}
 		int32_t base_x = xh >> 15;
-		int32_t xfrac = skip_xfrac ? 0 : ((xh >> 7) & 0xff);
+		int32_t xfrac = skip_xfrac ? 0 : ((xh >> (PLANTED(3) ? 8 : 7)) & 0xff);
 		for (int i = 0; i < 4; i++) {
 			int32_t v = attr->rgba[i] + dy * attr->drgba_de[i];
 			s->rgba[i] = ((v & ~0x1ff) + drgba_diff[i] - xfrac * ((attr->drgba_dx[i] >> 8) & ~1)) & ~0x3ff;
@@ -470,7 +480,7 @@ static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t sc
 	}
 
 	/* X extents at the four sub-scanlines (span_setup.comp:151-225). */
-	int yh_base = setup->yh & ~3;
+	int yh_base = PLANTED(2) ? setup->yh : (setup->yh & ~3);
 	int ym_base = setup->ym;
 	int ylo = imax(setup->yh, (int)scissor[1]);
 	int yhi = imin(setup->yl, (int)scissor[3]);
@@ -555,7 +565,7 @@ static void interpolate_rgba(const span_t *s, const attr_t *a, int dx, int cover
 {
 	int first = find_lsb((uint32_t)coverage);
 	int yoff = first >> 1;
-	int xoff = ((first & 1) << 1) + (yoff & 1);
+	int xoff = ((first & 1) << 1) + (PLANTED(4) ? 0 : (yoff & 1));
 	for (int i = 0; i < 4; i++) {
 		int32_t v = s->rgba[i] + (a->drgba_dx[i] & ~0x1f) * dx;
 		int32_t snapped = v >> 14;
@@ -657,7 +667,7 @@ static void interpolate_stz(const span_t *s, const attr_t *a, int dx, int covera
 	int xoff = ((first & 1) << 1) + (yoff & 1);
 	snapped <<= 2;
 	snapped += xoff * (a->dstzw_dx[2] >> 10) + yoff * (a->dstzw_dy[2] >> 10);
-	snapped >>= 5;
+	snapped >>= PLANTED(5) ? 4 : 5;
 	*z = clamp_z(snapped);
 }
 
@@ -697,7 +707,7 @@ static int texel_mask_s(const tile_t *t, int s)
 	if (t->mask_s) {
 		int mask = 1 << t->mask_s;
 		if (t->flags & TILE_MIRROR_S)
-			s ^= imax((s & mask) - 1, 0);
+			s ^= PLANTED(9) ? (s & mask) : imax((s & mask) - 1, 0);
 		s &= mask - 1;
 	}
 	return s;
@@ -718,7 +728,7 @@ static void convert_rgba16(uint32_t w, int32_t out[4])
 {
 	for (int i = 0; i < 3; i++) {
 		uint32_t c = (w >> (11 - 5 * i)) & 31;
-		out[i] = (int32_t)((c << 3) | (c >> 2));
+		out[i] = (int32_t)((c << 3) | (PLANTED(6) ? 0 : (c >> 2)));
 	}
 	out[3] = (w & 1) ? 0xff : 0;
 }
@@ -746,7 +756,7 @@ static uint32_t texel_index16(const tile_t *t, uint32_t s, uint32_t v, uint32_t 
 	uint32_t byte_offset = t->offset + t->stride * v + s * 2;
 	byte_offset &= wrap;
 	uint32_t index = byte_offset >> 1;
-	index ^= (v & 1) << 1;
+	if (!PLANTED(8)) index ^= (v & 1) << 1;
 	return index ^ 1;
 }
 
@@ -771,7 +781,7 @@ static void fetch_texel(const rdpo *o, const tile_t *t, uint32_t s, uint32_t v, 
 			word = tmem16(o, idx);
 			lut_entry = ((word >> 6) & ~3u) + lut_offset;
 		}
-		lut_entry ^= addr_xor;
+		if (!PLANTED(20)) lut_entry ^= addr_xor;
 		word = tmem16(o, 0x400 | lut_entry);
 		if (tlut_type)
 			convert_ia16(word, out);
@@ -814,7 +824,7 @@ static void fetch_texel(const rdpo *o, const tile_t *t, uint32_t s, uint32_t v, 
 			uint32_t idx = texel_index8(t, s, v, 0xfff, s >> 1);
 			uint32_t w = (tmem8(o, idx) >> ((~s & 1) * 4)) & 0xf;
 			uint32_t in = w & 0xe;
-			in = (in << 4) | (in << 1) | (in >> 2);
+			in = (in << 4) | (in << 1) | (PLANTED(7) ? 0 : (in >> 2));
 			out[0] = out[1] = out[2] = (int32_t)in;
 			out[3] = (w & 1) ? 0xff : 0;
 			return;
@@ -891,7 +901,7 @@ static int clamp_and_shift_coord(int clamp_bit, int coord, int lo, int hi, int s
 		coord >>= 16;
 	}
 	if (clamp_bit) {
-		int clamp_hi = (coord >> 3) >= hi;
+		int clamp_hi = PLANTED(10) ? (coord >> 3) > hi : (coord >> 3) >= hi;
 		if (clamp_hi)
 			coord = (((hi >> 2) - (lo >> 2)) & 0x3ff) << 5;
 		else
@@ -1090,7 +1100,7 @@ static void sample_texture(const rdpo *o, const tile_t *tile, const int32_t st_i
 		for (int i = 0; i < 4; i++) {
 			int32_t a = (t10[i] - tb[i]) * ffx;
 			a += (t01[i] - tb[i]) * ffy;
-			a += 0x10;
+			a += PLANTED(11) ? 0xf : 0x10;
 			a >>= 5;
 			acc[i] = a + tb[i];
 		}
@@ -1133,7 +1143,7 @@ static void compute_lod_2cycle(uint32_t *tile0, uint32_t *tile1, int32_t *lod_fr
 			else
 				*lod_frac = (imax(min_lod, max_d) << 3) + (sharpen ? -0x100 : 0);
 		} else {
-			int mip_base = imax(find_msb(max_d >> 5), 0);
+			int mip_base = imax(find_msb(max_d >> (PLANTED(12) ? 4 : 5)), 0);
 			distant = (uint32_t)mip_base >= max_level;
 			if (distant && !sharpen && !detail)
 				*lod_frac = 0xff;
@@ -1295,7 +1305,7 @@ static void select4(const comb_in_t *in, int which, int sel_rgb, int sel_a, int3
 		case 4: memcpy(out, in->shade, 12); r = 1; break;
 		case 8: out[0] = out[1] = out[2] = in->texel0[3]; r = 1; break;
 		case 9: out[0] = out[1] = out[2] = in->texel1[3]; r = 1; break;
-		case 11: out[0] = out[1] = out[2] = in->shade[3]; r = 1; break;
+		case 11: out[0] = out[1] = out[2] = PLANTED(13) ? in->texel0[3] : in->shade[3]; r = 1; break;
 		case 13: out[0] = out[1] = out[2] = in->lod_frac; r = 1; break;
 		case 15: out[0] = out[1] = out[2] = (k[1] << 8) | k[2]; r = 1; break;
 		default: break;
@@ -1491,7 +1501,7 @@ static int depth_test(int32_t z, int32_t dz, int32_t dz_compressed, uint32_t cur
 		*cvg_wrap = overflow;
 		int max_z = memory_z == 0x3ffff;
 		int front = z < memory_z;
-		int nearer = coplanar || ((z - combined) <= memory_z);
+		int nearer = coplanar || (PLANTED(14) ? (z - combined) < memory_z : (z - combined) <= memory_z);
 		switch (z_mode) {
 		case 0:
 			pass = max_z || (overflow ? front : nearer);
@@ -1577,7 +1587,7 @@ static void blender(const rdpo *o, const blend_in_t *in, const uint8_t modes[4],
 	}
 	int32_t blended[3];
 	for (int i = 0; i < 3; i++)
-		blended[i] = src0[i] * a0 + src1[i] * (a1 + 1);
+		blended[i] = src0[i] * a0 + src1[i] * (a1 + (PLANTED(15) ? 0 : 1));
 	if (!final_cycle || force_blend) {
 		for (int i = 0; i < 3; i++)
 			out[i] = (blended[i] >> 5) & 0xff;
@@ -1595,7 +1605,7 @@ static int blend_coverage(int coverage, int memory_coverage, int blend_en, int m
 {
 	switch (mode) {
 	case 0: return blend_en ? imin(7, memory_coverage + coverage) : ((coverage - 1) & 7);
-	case 1: return (coverage + memory_coverage) & 7;
+	case 1: return (coverage + memory_coverage) & (PLANTED(19) ? 15 : 7);
 	case 2: return 7;
 	default: return memory_coverage;
 	}
@@ -1672,7 +1682,7 @@ static void store_color(rdpo *o, uint32_t fmt, uint32_t index, const int32_t c[4
 		index &= hmask;
 		uint32_t r = (uint32_t)c[0] & 0xf8, g = (uint32_t)c[1] & 0xf8, b = (uint32_t)c[2] & 0xf8;
 		uint32_t cov = ((uint32_t)c[3] & 0xff) >> 5;
-		uint32_t w = (r << 8) | (g << 3) | (b >> 2) | (cov >> 2);
+		uint32_t w = (r << 8) | (g << 3) | (b >> 2) | (PLANTED(16) ? (cov & 1) : (cov >> 2));
 		wr16_index(o, index, (uint16_t)w);
 		o->hidden[index] = (uint8_t)(cov & 3);
 		break;
@@ -2057,7 +2067,7 @@ static void fill_color(pixel_state_t *p, uint32_t fmt, int alias, uint32_t col)
 		write_color(p, fmt, c);
 		break;
 	case FB_RGBA5551:
-		col >>= ((p->color_index & 1) ^ 1) * 16;
+		col >>= ((p->color_index & 1) ^ (PLANTED(17) ? 0 : 1)) * 16;
 		c[0] = (int32_t)((col >> 8) & 0xf8);
 		c[1] = (int32_t)((col >> 3) & 0xf8);
 		c[2] = (int32_t)((col << 2) & 0xf8);
@@ -2101,7 +2111,7 @@ static void copy_pipeline(pixel_state_t *p, uint32_t fmt, int alias, uint32_t wo
 		c[0] = (int32_t)((word >> 8) & 0xf8);
 		c[1] = (int32_t)((word >> 3) & 0xf8);
 		c[2] = (int32_t)((word << 2) & 0xf8);
-		c[3] = (int32_t)((word & 1) * 0xe0);
+		c[3] = (int32_t)(((PLANTED(18) ? (word >> 1) : word) & 1) * 0xe0);
 		write_color(p, fmt, c);
 		break;
 	default:
@@ -3637,3 +3647,249 @@ int rdpo_test_coverage(const uint16_t xleft[4], const uint16_t xright[4], int x)
 	return compute_coverage(&s, x);
 }
 void rdpo_test_rgb_dither(int32_t rgb[3], int dith) { rgb_dither(rgb, dith); }
+
+/* Hooks for the function pins of tests/test_oracle_pins.py.  Records are
+ * passed as flat integer arrays:
+ *   triangle: xh, xm, xl, yh, ym, yl, dxhdy, dxmdy, dxldy, flags;
+ *   attributes: rgba, drgba_dx, drgba_de, drgba_dy, stzw, dstzw_dx,
+ *               dstzw_de, dstzw_dy (4 each);
+ *   tile: slo, shi, tlo, thi, offset, stride, fmt, size, palette, mask_s,
+ *         shift_s, mask_t, shift_t, flags. */
+static void unpack_tri(const int32_t v[10], tri_t *t)
+{
+	memset(t, 0, sizeof(*t));
+	t->xh = v[0];
+	t->xm = v[1];
+	t->xl = v[2];
+	t->yh = v[3];
+	t->ym = v[4];
+	t->yl = v[5];
+	t->dxhdy = v[6];
+	t->dxmdy = v[7];
+	t->dxldy = v[8];
+	t->flags = (uint32_t)v[9];
+}
+
+static void unpack_attr(const int32_t v[32], attr_t *a)
+{
+	for (int i = 0; i < 4; i++) {
+		a->rgba[i] = v[i];
+		a->drgba_dx[i] = v[4 + i];
+		a->drgba_de[i] = v[8 + i];
+		a->drgba_dy[i] = v[12 + i];
+		a->stzw[i] = v[16 + i];
+		a->dstzw_dx[i] = v[20 + i];
+		a->dstzw_de[i] = v[24 + i];
+		a->dstzw_dy[i] = v[28 + i];
+	}
+}
+
+static void unpack_tile(const uint32_t v[14], tile_t *t)
+{
+	t->slo = v[0];
+	t->shi = v[1];
+	t->tlo = v[2];
+	t->thi = v[3];
+	t->offset = v[4];
+	t->stride = v[5];
+	t->fmt = v[6];
+	t->size = v[7];
+	t->palette = v[8];
+	t->mask_s = v[9];
+	t->shift_s = v[10];
+	t->mask_t = v[11];
+	t->shift_t = v[12];
+	t->flags = v[13];
+}
+
+/* out: rgba[4], stzw[4], xleft[4], xright[4], base_x, start_x, end_x,
+ * lodlength, valid. */
+void rdpo_test_span_setup(const int32_t tri[10], const int32_t attr[32], const uint32_t scissor[4], int y, int32_t out[21])
+{
+	tri_t t;
+	attr_t a;
+	span_t s;
+	unpack_tri(tri, &t);
+	unpack_attr(attr, &a);
+	memset(&s, 0, sizeof(s));
+	span_setup(&t, &a, scissor, y, &s);
+	for (int i = 0; i < 4; i++) {
+		out[i] = s.rgba[i];
+		out[4 + i] = s.stzw[i];
+		out[8 + i] = s.xleft[i];
+		out[12 + i] = s.xright[i];
+	}
+	out[16] = s.base_x;
+	out[17] = s.start_x;
+	out[18] = s.end_x;
+	out[19] = s.lodlength;
+	out[20] = s.valid;
+}
+
+void rdpo_test_interpolate_rgba(const int32_t span_rgba[4], const int32_t attr[32], int dx, int coverage, int32_t out[4])
+{
+	span_t s;
+	attr_t a;
+	memset(&s, 0, sizeof(s));
+	unpack_attr(attr, &a);
+	memcpy(s.rgba, span_rgba, sizeof(s.rgba));
+	interpolate_rgba(&s, &a, dx, coverage, out);
+}
+
+/* out: st[2], st_dx[2], st_dy[2], z, overflow. */
+void rdpo_test_interpolate_stz(const int32_t span_stzw[4], const int32_t attr[32], int dx, int coverage, int perspective,
+                               int uses_lod, int flip_dir, int32_t out[8])
+{
+	span_t s;
+	attr_t a;
+	memset(&s, 0, sizeof(s));
+	unpack_attr(attr, &a);
+	memcpy(s.stzw, span_stzw, sizeof(s.stzw));
+	int32_t st[2], st_dx[2] = { 0, 0 }, st_dy[2] = { 0, 0 }, z;
+	int ovf = 0;
+	interpolate_stz(&s, &a, dx, coverage, perspective, uses_lod, flip_dir, st, st_dx, st_dy, &z, &ovf);
+	out[0] = st[0];
+	out[1] = st[1];
+	out[2] = st_dx[0];
+	out[3] = st_dx[1];
+	out[4] = st_dy[0];
+	out[5] = st_dy[1];
+	out[6] = z;
+	out[7] = ovf;
+}
+
+void rdpo_test_fetch_texel(const rdpo *o, const uint32_t tile[14], uint32_t s, uint32_t t, int tlut, int tlut_type,
+                           uint32_t lut_offset, uint32_t addr_xor, int32_t out[4])
+{
+	tile_t tl;
+	unpack_tile(tile, &tl);
+	fetch_texel(o, &tl, s, t, tlut, tlut_type, lut_offset, addr_xor, out);
+}
+
+void rdpo_test_fetch_yuv(const rdpo *o, const uint32_t tile[14], uint32_t s, uint32_t t, uint32_t chroma_x, int32_t out[4])
+{
+	tile_t tl;
+	unpack_tile(tile, &tl);
+	fetch_yuv16(o, &tl, s, t, chroma_x, out);
+}
+
+int rdpo_test_texel_mask(const uint32_t tile[14], int coord, int axis_t)
+{
+	tile_t tl;
+	unpack_tile(tile, &tl);
+	return axis_t ? texel_mask_t(&tl, coord) : texel_mask_s(&tl, coord);
+}
+
+int rdpo_test_clamp_and_shift(int clamp_bit, int coord, int lo, int hi, int shift)
+{
+	return clamp_and_shift_coord(clamp_bit, coord, lo, hi, shift);
+}
+
+void rdpo_test_sample_texture(const rdpo *o, const uint32_t tile[14], const int32_t st[2], int tlut, int tlut_type,
+                              int sample_quad, int mid_texel, int convert_one, int bilerp, const int32_t factors[4],
+                              const int32_t prev[4], int32_t out[4])
+{
+	tile_t tl;
+	unpack_tile(tile, &tl);
+	sample_texture(o, &tl, st, tlut, tlut_type, sample_quad, mid_texel, convert_one, bilerp, factors, prev, out);
+}
+
+/* stq: st, st_dx, st_dy; out: tile0, tile1, lod_frac. */
+void rdpo_test_compute_lod(uint32_t tile, uint32_t max_level, int min_lod, const int32_t stq[6], int ovf, int tex_lod_en,
+                           int sharpen, int detail, int32_t out[3])
+{
+	uint32_t t0 = tile & 7, t1 = (tile + 1) & 7;
+	int32_t frac = 0;
+	compute_lod_2cycle(&t0, &t1, &frac, max_level, min_lod, stq, stq + 2, stq + 4, ovf, tex_lod_en, sharpen, detail);
+	out[0] = (int32_t)t0;
+	out[1] = (int32_t)t1;
+	out[2] = frac;
+}
+
+/* in: shade[4], combined[4], texel0[4], texel1[4], lod_frac, noise;
+ * k: the muladd, mulsub, mul and add constant slots (4 each). */
+void rdpo_test_select(const int32_t in[18], const int32_t k[16], int which, int sel_rgb, int sel_a, int32_t out[4])
+{
+	comb_in_t ci;
+	memcpy(ci.shade, in, 16);
+	memcpy(ci.combined, in + 4, 16);
+	memcpy(ci.texel0, in + 8, 16);
+	memcpy(ci.texel1, in + 12, 16);
+	ci.lod_frac = in[16];
+	ci.noise = in[17];
+	ci.const_muladd = k;
+	ci.const_mulsub = k + 4;
+	ci.const_mul = k + 8;
+	ci.const_add = k + 12;
+	select4(&ci, which, sel_rgb, sel_a, out);
+}
+
+/* Returns pass; out: coverage after the test, blend_en, cvg_wrap, shift0, shift1. */
+int rdpo_test_depth_test(int32_t z, int32_t dz, int32_t dz_compressed, uint32_t cur_depth, uint32_t cur_dz, int coverage,
+                         int cur_cvg, int z_compare, int z_mode, int force_blend, int aa, int32_t out[5])
+{
+	int cvg = coverage, blend_en = 0, wrap = 0, shift[2] = { 0, 0 };
+	int pass = depth_test(z, dz, dz_compressed, cur_depth, cur_dz, &cvg, cur_cvg, z_compare, z_mode, force_blend, aa,
+	                      &blend_en, &wrap, shift);
+	out[0] = cvg;
+	out[1] = blend_en;
+	out[2] = wrap;
+	out[3] = shift[0];
+	out[4] = shift[1];
+	return pass;
+}
+
+/* in: pixel[4], memory[4], fog[4], blendc[4], shade_alpha; modes: 1a, 1b, 2a, 2b. */
+void rdpo_test_blender(const rdpo *o, const int32_t in[17], const int32_t modes[4], int force_blend, int blend_en,
+                       int color_on_cvg, int cvg_wrap, int shift0, int shift1, int final_cycle, int32_t out[3])
+{
+	blend_in_t b;
+	memcpy(b.pixel, in, 16);
+	memcpy(b.memory, in + 4, 16);
+	memcpy(b.fog, in + 8, 16);
+	memcpy(b.blendc, in + 12, 16);
+	b.shade_alpha = in[16];
+	const uint8_t m[4] = { (uint8_t)modes[0], (uint8_t)modes[1], (uint8_t)modes[2], (uint8_t)modes[3] };
+	const int shift[2] = { shift0, shift1 };
+	blender(o, &b, m, force_blend, blend_en, color_on_cvg, cvg_wrap, shift, final_cycle, out);
+}
+
+int rdpo_test_blend_coverage(int coverage, int memory_coverage, int blend_en, int mode)
+{
+	return blend_coverage(coverage, memory_coverage, blend_en, mode);
+}
+
+void rdpo_test_load_color(const rdpo *o, uint32_t fmt, uint32_t index, int32_t out[4]) { load_color(o, fmt, index, out); }
+void rdpo_test_store_color(rdpo *o, uint32_t fmt, uint32_t index, const int32_t c[4]) { store_color(o, fmt, index, c); }
+
+void rdpo_test_load_depth(const rdpo *o, uint32_t index, uint32_t out[2]) { load_depth(o, index, &out[0], &out[1]); }
+void rdpo_test_store_depth(rdpo *o, uint32_t index, uint32_t depth, uint32_t dz) { store_depth(o, index, depth, dz); }
+
+/* A pixel through the fill or copy pipe from the given colour and depth
+ * state; io: color[4], depth, dz (in and out). */
+void rdpo_test_fill_color(uint32_t fmt, uint32_t color_index, int alias, uint32_t col, int32_t io[6])
+{
+	pixel_state_t p;
+	memset(&p, 0, sizeof(p));
+	memcpy(p.color, io, 16);
+	p.depth = (uint32_t)io[4];
+	p.dz = (uint32_t)io[5];
+	p.color_index = color_index;
+	fill_color(&p, fmt, alias, col);
+	memcpy(io, p.color, 16);
+	io[4] = (int32_t)p.depth;
+	io[5] = (int32_t)p.dz;
+}
+
+void rdpo_test_copy_pipeline(uint32_t fmt, int alias, uint32_t word, int32_t io[6])
+{
+	pixel_state_t p;
+	memset(&p, 0, sizeof(p));
+	memcpy(p.color, io, 16);
+	p.depth = (uint32_t)io[4];
+	p.dz = (uint32_t)io[5];
+	copy_pipeline(&p, fmt, alias, word);
+	memcpy(io, p.color, 16);
+	io[4] = (int32_t)p.depth;
+	io[5] = (int32_t)p.dz;
+}
