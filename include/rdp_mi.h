@@ -273,6 +273,29 @@ int rdp_mi_server_reset(rdp_mi_server *s, double start_in_us);
  * copies at most max.  Any output pointer may be null. */
 size_t rdp_mi_server_results(rdp_mi_server *s, uint64_t *tickets, uint32_t *instances, double *arrival_us,
                              double *admit_us, double *submit_us, double *complete_us, size_t max);
+/* Per-frame state digests (no reference counterpart; the reference's
+ * conformance driver compares all of RDRAM after each test,
+ * conformance_utils.hpp:282-304).  Enabled before rdp_mi_server_start: after
+ * every batch the server computes, on the device and behind the batch's
+ * launches, the digest of each served processor's RDRAM, hidden RDRAM and
+ * TMEM (the definition of rdp_mi_state_digest), so each frame can be checked
+ * on its own. */
+int rdp_mi_server_set_digests(rdp_mi_server *s, int enable);
+/* Completed requests in the order of rdp_mi_server_results: ticket, the
+ * digest after the frame (0 when digests are off) and a status (0, or the
+ * negative RDP_MI_ERROR_* code of a batch that failed). */
+size_t rdp_mi_server_digests(rdp_mi_server *s, uint64_t *tickets, uint64_t *digests, int32_t *status, size_t max);
+
+/* Digest of a processor's state (no reference counterpart), computed on the
+ * device after the work already submitted for it: the sum modulo 2^64 over
+ * the 32-bit words w_i of RDRAM (host order), of hidden RDRAM (bytes packed
+ * four to a little-endian word) and of TMEM (halfwords packed two to a
+ * little-endian word) of SplitMix64(key_r ^ (i << 32 | w_i)), i counting
+ * from 0 in each region, key_r = 0x243f6a8885a308d3 (RDRAM),
+ * 0x13198a2e03707344 (hidden), 0xa4093822299f31d0 (TMEM).  Call it after
+ * rdp_mi_idle for the state the reference's readback would show. */
+int rdp_mi_state_digest(rdp_mi_processor *p, uint64_t *digest);
+
 /* Counters so far: out[0] batches submitted, out[1] frames, out[2] frames in
  * the largest batch, out[3] largest estimated batch bytes (the admission
  * figure), out[4] largest actual packed-input plus scratch bytes of a batch,

@@ -77,6 +77,8 @@ def _load():
     lib.rdpo_fragment_count.argtypes = [vp]
     lib.rdpo_primitive_count.restype = u64
     lib.rdpo_primitive_count.argtypes = [vp]
+    lib.rdpo_state_digest.restype = u64
+    lib.rdpo_state_digest.argtypes = [vp]
     lib.rdpo_command_length.restype = ctypes.c_uint
     lib.rdpo_command_length.argtypes = [u32]
     lib.rdpo_blender_divider_table.argtypes = [vp]
@@ -155,6 +157,10 @@ class Oracle:
         out = np.zeros(n, dtype=np.uint64)
         self._lib.rdpo_pass_ends(self._h, out.ctypes.data, n)
         return out
+
+    def digest(self) -> int:
+        """Digest of RDRAM, hidden RDRAM and TMEM (the library's state digest)."""
+        return int(self._lib.rdpo_state_digest(self._h))
 
     @property
     def fragments(self) -> int:

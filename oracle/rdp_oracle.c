@@ -3893,3 +3893,32 @@ void rdpo_test_copy_pipeline(uint32_t fmt, int alias, uint32_t word, int32_t io[
 	io[4] = (int32_t)p.depth;
 	io[5] = (int32_t)p.dz;
 }
+
+/* State digest with the definition of the device digest (rdp_mi_state_digest
+ * of the library under test): the sum modulo 2^64 of SplitMix64(key ^ (i << 32
+ * | w_i)) over RDRAM words, hidden RDRAM bytes packed four to a little-endian
+ * word and TMEM halfwords packed two to a little-endian word. */
+static uint64_t digest_mix64(uint64_t z)
+{
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+static uint64_t digest_region(const uint8_t *p, size_t words, uint64_t key)
+{
+	uint64_t acc = 0;
+	for (size_t i = 0; i < words; i++) {
+		uint32_t w;
+		memcpy(&w, p + 4 * i, 4);
+		acc += digest_mix64(key ^ (((uint64_t)i << 32) | w));
+	}
+	return acc;
+}
+
+uint64_t rdpo_state_digest(const rdpo *o)
+{
+	return digest_region(o->rdram, o->rdram_size / 4, 0x243f6a8885a308d3ull) +
+	       digest_region(o->hidden, o->rdram_size / 8, 0x13198a2e03707344ull) +
+	       digest_region((const uint8_t *)o->tmem, 1024, 0xa4093822299f31d0ull);
+}

@@ -100,6 +100,10 @@ void rdpo_end_vi_register_per_scanline(rdpo *o);
 uint64_t rdpo_fragment_count(const rdpo *o);
 uint64_t rdpo_primitive_count(const rdpo *o);
 
+/* Digest of RDRAM, hidden RDRAM and TMEM with the definition the library's
+ * rdp_mi_state_digest documents (rdp_mi.h), for per-frame checks. */
+uint64_t rdpo_state_digest(const rdpo *o);
+
 /* Generated lookup tables, exposed so tests can pin them. */
 void rdpo_blender_divider_table(uint8_t out[0x8000]);
 void rdpo_gamma_table(uint8_t out[256 + 256 * 64]);

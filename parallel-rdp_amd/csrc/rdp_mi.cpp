@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/rdp_mi.h"
+#include "rdp_digest.hpp"
 #include "rdp_frontend.hpp"
 #include "rdp_hazard.hpp"
 #include "rdp_types.hpp"
@@ -199,6 +200,7 @@ struct Context : Lane {
 	uint8_t *d_gamma = nullptr;
 	LaunchConfig cfg;  // for immediate submissions (flush, batch flush, serving)
 	std::mutex lock;
+	uint8_t *d_digest = nullptr;                // rdp_mi_state_digest: one target and its result
 
 	int init()
 	{
@@ -1164,6 +1166,8 @@ struct ServeResult {
 	uint64_t ticket;
 	uint32_t instance;
 	double arrival_us, admit_us, submit_us, complete_us;
+	uint64_t digest;  // state digest after the frame (when enabled)
+	int32_t status;   // 0, or the error of the frame's batch
 };
 
 struct Server {
@@ -1201,6 +1205,10 @@ struct Server {
 		Lane lane;
 		hipEvent_t done_ev = nullptr;  // completion of the lane's batch in flight
 		bool busy = false;
+		// Per-frame digests: targets staged in pinned memory, results copied
+		// back behind the batch (one batch in flight per lane).
+		DigestTarget *h_targets = nullptr, *d_targets = nullptr;
+		uint64_t *h_digests = nullptr, *d_digests = nullptr;
 		std::deque<Batch> inflight;
 		std::condition_variable cv;
 		std::thread completer, submitter;
@@ -1208,6 +1216,7 @@ struct Server {
 	std::vector<std::unique_ptr<LaneState>> lanes;
 	std::thread scheduler;
 	unsigned rr = 0;
+	bool digests = false;
 	// Statistics: batches, frames, largest batch, largest estimated and
 	// actual device bytes of one batch.
 	uint64_t n_batches = 0, n_frames = 0, max_frames = 0, max_est_bytes = 0, max_arena_bytes = 0;
@@ -1221,11 +1230,36 @@ struct Server {
 	static size_t estimate_bytes(const ServeRequest &r) { return r.words.size() * 4 * 24 + (1u << 20); }
 
 	bool done() const { return stopping && outstanding == 0; }
+	int queue_digests(LaneState &ls, const std::vector<Processor *> &ps);
 	void decode(ServeRequest req, size_t bytes, double admit);
 	void scheduler_loop();
 	void submitter_loop(LaneState *ls);
 	void completion_loop(LaneState &ls);
 };
+
+static DigestTarget digest_target(const Processor &p)
+{
+	return DigestTarget{ reinterpret_cast<const uint32_t *>(p.d_rdram), p.d_hidden, p.d_tmem, p.rdram_size / 4,
+		                 p.rdram_size / 2 };
+}
+
+static uint32_t digest_words(const Processor &p) { return p.rdram_size / 4 + p.rdram_size / 8 + 1024; }
+
+// Queue the digest of every processor of the lane's batch behind its
+// launches, and the copy of the digests to the lane's pinned results.
+int Server::queue_digests(LaneState &ls, const std::vector<Processor *> &ps)
+{
+	uint32_t max_words = 0;
+	for (size_t k = 0; k < ps.size(); k++) {
+		ls.h_targets[k] = digest_target(*ps[k]);
+		max_words = std::max(max_words, digest_words(*ps[k]));
+	}
+	hipStream_t st = ls.lane.stream;
+	HIP_CHECK(hipMemcpyAsync(ls.d_targets, ls.h_targets, ps.size() * sizeof(DigestTarget), hipMemcpyHostToDevice, st));
+	HIP_CHECK(launch_digest(ls.d_targets, (uint32_t)ps.size(), max_words, ls.d_digests, st));
+	HIP_CHECK(hipMemcpyAsync(ls.h_digests, ls.d_digests, ps.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	return 0;
+}
 
 void Server::decode(ServeRequest req, size_t bytes, double admit)
 {
@@ -1247,7 +1281,7 @@ void Server::decode(ServeRequest req, size_t bytes, double admit)
 		p->fe.flush();
 	}
 	std::lock_guard<std::mutex> g(m);
-	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0 }, bytes });
+	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0, 0, 0 }, bytes });
 	submit_cv.notify_all();  // any lane's submitter may be the free one
 }
 
@@ -1313,13 +1347,25 @@ void Server::submitter_loop(LaneState *ls)
 		Batch b;
 		const double t_sub = now_us();
 		int err = submit_processors(ps.data(), (unsigned)ps.size(), true, &ls->lane);
+		if (!err && digests) err = queue_digests(*ls, ps);
 		// A lane holds one batch at a time, so its completion event is reused.
 		b.ev = ls->done_ev;
-		if (!err && hipEventRecord(b.ev, ls->lane.stream) != hipSuccess)
-			err = set_error(RDP_MI_ERROR_DEVICE, "event record failed");
+		if (hipEventRecord(b.ev, ls->lane.stream) != hipSuccess) {
+			if (!err) err = set_error(RDP_MI_ERROR_DEVICE, "event record failed");
+			b.ev = nullptr;
+		}
+		if (err) {
+			// Launches of the failed batch may be queued on any of the lane's
+			// streams: wait for all of them before its processors are freed,
+			// so no later frame of theirs overlaps them.
+			for (hipStream_t st : { ls->lane.stream, ls->lane.side, ls->lane.prep })
+				if (st) (void)hipStreamSynchronize(st);
+			b.ev = nullptr;
+		}
 		const double submitted = now_us();
 		for (auto &r : batch) {
 			r.r.submit_us = submitted;
+			r.r.status = err;
 			b.reqs.push_back(r.r);
 		}
 
@@ -1355,7 +1401,9 @@ void Server::completion_loop(LaneState &ls)
 		g.lock();
 		Batch b = std::move(ls.inflight.front());
 		ls.inflight.pop_front();
-		for (auto &r : b.reqs) {
+		for (size_t k = 0; k < b.reqs.size(); k++) {
+			ServeResult &r = b.reqs[k];
+			if (digests && r.status == 0) r.digest = ls.h_digests[k];
 			r.complete_us = t;
 			results.push_back(r);
 			busy[r.instance] = false;
@@ -2071,6 +2119,13 @@ int rdp_mi_server_start(rdp_mi_server *h)
 		// create the per-round events and touch every stream now.
 		if (int err = ln.ensure_round_events(4)) return err;
 		HIP_CHECK(hipEventCreateWithFlags(&l->done_ev, hipEventDisableTiming));
+		if (s.digests) {
+			const size_t n = s.max_batch;
+			HIP_CHECK(hipHostMalloc((void **)&l->h_targets, n * sizeof(DigestTarget), hipHostMallocDefault));
+			HIP_CHECK(hipMalloc((void **)&l->d_targets, n * sizeof(DigestTarget)));
+			HIP_CHECK(hipHostMalloc((void **)&l->h_digests, n * sizeof(uint64_t), hipHostMallocDefault));
+			HIP_CHECK(hipMalloc((void **)&l->d_digests, n * sizeof(uint64_t)));
+		}
 		for (hipStream_t st : { ln.stream, ln.side, ln.prep }) {
 			HIP_CHECK(hipEventRecord(l->done_ev, st));
 			HIP_CHECK(hipEventSynchronize(l->done_ev));
@@ -2135,6 +2190,50 @@ size_t rdp_mi_server_results(rdp_mi_server *h, uint64_t *tickets, uint32_t *inst
 	return n;
 }
 
+int rdp_mi_server_set_digests(rdp_mi_server *h, int enable)
+{
+	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null server");
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	if (s.started) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "server already started");
+	s.digests = enable != 0;
+	return 0;
+}
+
+size_t rdp_mi_server_digests(rdp_mi_server *h, uint64_t *tickets, uint64_t *digests, int32_t *status, size_t max)
+{
+	if (!h) return 0;
+	Server &s = h->s;
+	std::lock_guard<std::mutex> g(s.m);
+	const size_t n = s.results.size();
+	for (size_t i = 0; i < n && i < max; i++) {
+		const ServeResult &r = s.results[i];
+		if (tickets) tickets[i] = r.ticket;
+		if (digests) digests[i] = r.digest;
+		if (status) status[i] = r.status;
+	}
+	return n;
+}
+
+int rdp_mi_state_digest(rdp_mi_processor *h, uint64_t *digest)
+{
+	if (!h || !digest) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	Processor &p = h->p;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	if (!c.d_digest) {
+		HIP_CHECK(hipMalloc((void **)&c.d_digest, sizeof(DigestTarget) + 256));
+	}
+	const DigestTarget t = digest_target(p);
+	DigestTarget *d_t = reinterpret_cast<DigestTarget *>(c.d_digest + 256);
+	uint64_t *d_out = reinterpret_cast<uint64_t *>(c.d_digest);
+	HIP_CHECK(hipMemcpyAsync(d_t, &t, sizeof(t), hipMemcpyHostToDevice, c.stream));
+	HIP_CHECK(launch_digest(d_t, 1, digest_words(p), d_out, c.stream));
+	HIP_CHECK(hipMemcpyAsync(digest, d_out, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
+	HIP_CHECK(hipStreamSynchronize(c.stream));
+	return 0;
+}
+
 void rdp_mi_server_stats(rdp_mi_server *h, uint64_t out[7])
 {
 	if (!h || !out) return;
@@ -2170,6 +2269,10 @@ void rdp_mi_server_destroy(rdp_mi_server *h)
 	for (auto &l : s.lanes) {
 		if (l->done_ev) (void)hipEventDestroy(l->done_ev);
 		l->lane.release();
+		if (l->h_targets) (void)hipHostFree(l->h_targets);
+		if (l->h_digests) (void)hipHostFree(l->h_digests);
+		if (l->d_targets) (void)hipFree(l->d_targets);
+		if (l->d_digests) (void)hipFree(l->d_digests);
 	}
 	delete h;
 }

@@ -30,6 +30,14 @@ class RDPError(RuntimeError):
 ERROR_TIMEOUT = -5
 ERROR_LIMIT = -6
 
+# Words per command by opcode (the RDP's command lengths).
+COMMAND_LENGTHS = np.array([
+    2, 0, 0, 0, 2, 0, 0, 0, 8, 12, 24, 28, 24, 28, 40, 44,
+    0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+    0, 0, 0, 0, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
+    2, 0, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
+], dtype=np.int64)
+
 
 class ScanoutOptions(ctypes.Structure):
     """rdp_mi_scanout_options (the reference's ScanoutOptions without upscaling)."""
@@ -113,6 +121,8 @@ def load_library():
     lib.rdp_mi_last_error.argtypes = []
     lib.rdp_mi_primitive_count.restype = u64
     lib.rdp_mi_primitive_count.argtypes = [vp]
+    lib.rdp_mi_state_digest.restype = ctypes.c_int
+    lib.rdp_mi_state_digest.argtypes = [vp, ctypes.POINTER(u64)]
     _lib = lib
     return lib
 
@@ -171,6 +181,22 @@ class CommandProcessor:
     # -- synchronisation --------------------------------------------------
     def flush(self):
         self._lib.rdp_mi_flush(self._h)
+
+    def enqueue_commands_direct(self, words: np.ndarray) -> int:
+        """enqueue_command_direct for each complete command (decoded on this
+        thread, no ring).  Returns the number of words consumed."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        i = 0
+        while i < w.size:
+            n = int(COMMAND_LENGTHS[(int(w[i]) >> 24) & 63])
+            if n == 0:
+                i += 1
+                continue
+            if i + n > w.size:
+                break
+            self._lib.rdp_mi_enqueue_command_direct(self._h, n, w[i:].ctypes.data)
+            i += n
+        return i
 
     def signal_timeline(self) -> int:
         return int(self._lib.rdp_mi_signal_timeline(self._h))
@@ -280,6 +306,13 @@ class CommandProcessor:
     @property
     def primitive_count(self) -> int:
         return int(self._lib.rdp_mi_primitive_count(self._h))
+
+    def state_digest(self) -> int:
+        """Device-side digest of RDRAM, hidden RDRAM and TMEM after the work
+        submitted so far (call after idle())."""
+        d = ctypes.c_uint64()
+        _check(self._lib.rdp_mi_state_digest(self._h, ctypes.byref(d)), "state_digest")
+        return int(d.value)
 
 
 class ScanoutBuffer:
@@ -449,6 +482,11 @@ class Server:
         lib.rdp_mi_server_reset.restype = ctypes.c_int
         lib.rdp_mi_server_reset.argtypes = [vp, ctypes.c_double]
         lib.rdp_mi_server_destroy.argtypes = [vp]
+        lib.rdp_mi_server_set_digests.restype = ctypes.c_int
+        lib.rdp_mi_server_set_digests.argtypes = [vp, ctypes.c_int]
+        lib.rdp_mi_server_digests.restype = sz
+        lib.rdp_mi_server_digests.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_int32), sz]
         self._lib = lib
         self.processors = list(processors)
         arr = (vp * len(processors))(*[p.handle.value for p in processors])
@@ -494,6 +532,23 @@ class Server:
             out["arrival_us"].ctypes.data_as(dp), out["admit_us"].ctypes.data_as(dp),
             out["submit_us"].ctypes.data_as(dp), out["complete_us"].ctypes.data_as(dp), n))
         return {k: v[:min(n, got)] for k, v in out.items()}
+
+    def set_digests(self, enable: bool = True):
+        """Digest every served processor's state after each frame (before start())."""
+        _check(self._lib.rdp_mi_server_set_digests(self._h, int(enable)), "rdp_mi_server_set_digests")
+
+    def digests(self) -> dict:
+        """Per completed request, in the order of results(): ticket, the state
+        digest after the frame and the batch status (0 or an error code)."""
+        n = int(self._lib.rdp_mi_server_digests(self._h, None, None, None, 0))
+        t = np.zeros(n, np.uint64)
+        d = np.zeros(n, np.uint64)
+        st = np.zeros(n, np.int32)
+        got = int(self._lib.rdp_mi_server_digests(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                  d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                  st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n))
+        m = min(n, got)
+        return {"ticket": t[:m], "digest": d[:m], "status": st[:m]}
 
     def stats(self) -> dict:
         v = np.zeros(7, np.uint64)

@@ -1033,3 +1033,52 @@ def fuzz_stream(seed: int, segments: int = 12, width: Optional[int] = None, heig
     return Stream(name=f"fuzz{seed:#x}:{width}x{height}:{fb_fmt}/{fb_size}", seed=seed, frames=[Frame(cl.words())],
                   rdram_init=rdram, hidden_init=int(rng.integers(0, 4)), color_addr=COLOR_ADDR,
                   depth_addr=DEPTH_ADDR, width=width, height=height, color_size=fb_size, num_triangles=total)
+
+
+def alias_stream(seed: int, fb_fmt: int = FMT_RGBA, cycle: int = 0, partial: bool = False, segments: int = 8) -> Stream:
+    """Colour and depth images that share memory, drawn with the depth test
+    and update on (16-bit images: RGBA5551 or IA88).
+
+    partial=False: the depth image is the colour image, the reference's
+    colour/depth alias (memory_interfacing.h:100-151, 484-571).
+    partial=True: the depth image starts half the image height into the
+    colour image.  Each segment draws inside the top or the bottom half
+    (scissor) and ends with a SyncFull, so within one render pass the colour
+    and depth addresses written never meet (two tiles storing to one address
+    in a pass would race in the reference too), while later passes read
+    colour memory that holds depth values and the other way round."""
+    rng = np.random.default_rng(seed)
+    width = int(rng.choice([32, 64, 96]))
+    height = int(rng.choice([32, 48, 64]))
+    rdram = rng.integers(0, 1 << 32, size=RDRAM_SIZE // 4, dtype=np.uint64).astype(np.uint32)
+    depth_addr = COLOR_ADDR + ((height // 2) * width * 2 if partial else 0)
+    cl = CommandList()
+    cl.set_color_image(COLOR_ADDR, width, SIZ_16, fb_fmt)
+    cl.set_mask_image(depth_addr)
+    total = 0
+    for seg in range(segments):
+        if partial:
+            half = height // 2
+            y0 = 0 if seg % 2 == 0 else half
+            cl.set_scissor(0, y0 << 2, width << 2, (y0 + half) << 2)
+        else:
+            cl.set_scissor(0, 0, width << 2, height << 2)
+        modes = _random_modes(rng, cycle)
+        modes.depth_test = True
+        modes.depth_write = bool(rng.random() < 0.8)
+        cl.set_other_modes(modes)
+        _random_combine(rng, cl)
+        _random_colors(rng, cl)
+        _random_tiles(rng, cl)
+        if rng.random() < 0.5:
+            _random_load(rng, cl)
+        total += _random_triangles(rng, cl, int(rng.integers(4, 24)), width, height)
+        if rng.random() < 0.3:
+            _random_rects(rng, cl, int(rng.integers(1, 3)), width, height)
+        if partial or rng.random() < 0.4:
+            cl.sync_full()
+    cl.sync_full()
+    kind = "partial" if partial else "alias"
+    return Stream(name=f"{kind}{seed:#x}:{width}x{height}:{fb_fmt}:c{cycle}", seed=seed, frames=[Frame(cl.words())],
+                  rdram_init=rdram, hidden_init=int(rng.integers(0, 4)), color_addr=COLOR_ADDR, depth_addr=depth_addr,
+                  width=width, height=height, color_size=SIZ_16, num_triangles=total)

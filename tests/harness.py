@@ -73,3 +73,35 @@ def compare(gpu, ref):
         total += n
         lines.append(line)
     return total, lines
+
+
+def _oracle_state(stream):
+    rd, hid, tm, _ = run_oracle(stream)
+    return rd, hid, tm
+
+
+def run_oracle_many(streams, workers=8):
+    """run_oracle over many streams in worker processes (spawned: they load
+    only the oracle, never the device); results in stream order, without the
+    fragment counts."""
+    import concurrent.futures
+    import multiprocessing
+    ctx = multiprocessing.get_context("spawn")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as pool:
+        return list(pool.map(_oracle_state, streams))
+
+
+def oracle_frame_digests(stream):
+    """The oracle's state digest after each frame of the stream."""
+    import oracle
+    o = oracle.Oracle(rs.RDRAM_SIZE)
+    if stream.rdram_init is not None:
+        o.rdram[:] = stream.rdram_init
+    o.hidden[:] = stream.hidden_init
+    out = []
+    for f in stream.frames:
+        o.process(f.words)
+        o.flush()
+        out.append(o.digest())
+    o.close()
+    return out
