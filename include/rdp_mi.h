@@ -207,7 +207,11 @@ int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count);
  * the recorded passes on the processors' current RDRAM and TMEM, either as
  * direct launches or as one captured hipGraph; iteration_ms (optional)
  * receives each replay's GPU time.  rdp_mi_recording_profile times the four
- * stages (setup, TMEM, bin, raster) with events between direct launches. */
+ * stages (setup, TMEM, bin, raster) with events between direct launches.
+ * The profile and autotune calls execute the recording many times; they copy
+ * the recorded processors' RDRAM, hidden RDRAM and TMEM aside on the device
+ * first and put them back afterwards, so they leave the state as they found
+ * it (a replay changes it).  The processors must outlive the recording. */
 typedef struct rdp_mi_recording rdp_mi_recording;
 int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, rdp_mi_recording **out);
 int rdp_mi_recording_replay(rdp_mi_recording *r, unsigned iterations, int use_graph, float *iteration_ms);
@@ -273,6 +277,12 @@ int rdp_mi_server_reset(rdp_mi_server *s, double start_in_us);
  * copies at most max.  Any output pointer may be null. */
 size_t rdp_mi_server_results(rdp_mi_server *s, uint64_t *tickets, uint32_t *instances, double *arrival_us,
                              double *admit_us, double *submit_us, double *complete_us, size_t max);
+/* Batching window (no reference counterpart): a free submission stream
+ * takes the decoded frames only once max_batch of them are ready, no
+ * admitted frame is still decoding, or the oldest ready frame has waited
+ * window_us microseconds.  0 (the default) submits whatever is ready as soon
+ * as a stream is free.  Trades latency for larger batches. */
+int rdp_mi_server_set_batch_window(rdp_mi_server *s, double window_us);
 /* Per-frame state digests (no reference counterpart; the reference's
  * conformance driver compares all of RDRAM after each test,
  * conformance_utils.hpp:282-304).  Enabled before rdp_mi_server_start: after
@@ -302,6 +312,80 @@ int rdp_mi_state_digest(rdp_mi_processor *p, uint64_t *digest);
  * out[5] total and out[6] longest host time spent submitting a batch (us). */
 void rdp_mi_server_stats(rdp_mi_server *s, uint64_t out[7]);
 void rdp_mi_server_destroy(rdp_mi_server *s);
+
+/* ---- RDPDUMP2 captures ----
+ * The reference's capture format (writer rdp_dump_write.hpp:31-57 /
+ * rdp_dump_write.cpp:37-145, reader rdp_dump.hpp:49-68 / rdp_dump.cpp:43-200),
+ * byte for byte: "RDPDUMP2", u32 RDRAM size, u32 hidden RDRAM size, then
+ * records of host-order u32 words (kinds below).
+ *
+ * Capture from a processor: rdp_mi_create starts one when the environment
+ * variable PARALLEL_RDP_DUMP_PATH names a file (rdp_device.cpp:60-73), or
+ * rdp_mi_begin_dump starts one later.  Then, as in rdp_device.cpp:874-897,
+ * 970-975 and 1087-1092: the first rdp_mi_enqueue_command of a command list
+ * waits for the processor to go idle and records the 4 KiB blocks of RDRAM
+ * and hidden RDRAM that changed since the last record; each command is
+ * recorded, except SyncFull, which is recorded as SignalComplete and ends the
+ * list; VI register writes are recorded; a scanout records the changed
+ * blocks and EndFrame.  rdp_mi_enqueue_command_direct is not captured (as in
+ * the reference).  rdp_mi_end_dump (or rdp_mi_destroy) writes EndOfFile. */
+enum {
+	RDP_MI_DUMP_EOF = 0,  /* reported by rdp_mi_dump_player_next at the end */
+	RDP_MI_DUMP_UPDATE_DRAM = 1,
+	RDP_MI_DUMP_RDP_COMMAND = 2,
+	RDP_MI_DUMP_SET_VI_REGISTER = 3,
+	RDP_MI_DUMP_END_FRAME = 4,
+	RDP_MI_DUMP_SIGNAL_COMPLETE = 5,
+	RDP_MI_DUMP_END_OF_FILE = 6,
+	RDP_MI_DUMP_UPDATE_DRAM_FLUSH = 7,
+	RDP_MI_DUMP_UPDATE_HIDDEN_DRAM = 8,
+	RDP_MI_DUMP_UPDATE_HIDDEN_DRAM_FLUSH = 9
+};
+int rdp_mi_begin_dump(rdp_mi_processor *p, const char *path);
+void rdp_mi_end_dump(rdp_mi_processor *p);
+
+/* RDPDumpWriter (rdp_dump_write.hpp:31-57) on its own, for front ends that
+ * write captures themselves. */
+typedef struct rdp_mi_dump_writer rdp_mi_dump_writer;
+int rdp_mi_dump_writer_open(const char *path, uint32_t rdram_size, uint32_t hidden_rdram_size, rdp_mi_dump_writer **out);
+void rdp_mi_dump_writer_flush_rdram(rdp_mi_dump_writer *w, const void *rdram, uint32_t size);
+void rdp_mi_dump_writer_flush_hidden_rdram(rdp_mi_dump_writer *w, const void *hidden, uint32_t size);
+void rdp_mi_dump_writer_command(rdp_mi_dump_writer *w, uint32_t op, const uint32_t *words, uint32_t num_words);
+void rdp_mi_dump_writer_set_vi_register(rdp_mi_dump_writer *w, uint32_t reg, uint32_t value);
+void rdp_mi_dump_writer_end_frame(rdp_mi_dump_writer *w);
+void rdp_mi_dump_writer_signal_complete(rdp_mi_dump_writer *w);
+/* Writes EndOfFile, closes the file and frees the writer. */
+void rdp_mi_dump_writer_close(rdp_mi_dump_writer *w);
+
+/* DumpPlayer (rdp_dump.hpp:49-68): rdp_mi_dump_player_next reads one record
+ * (returns 1; 0 with kind RDP_MI_DUMP_EOF at the end; a negative error for a
+ * malformed record).  Block records update the player's RDRAM and hidden
+ * RDRAM shadows (rdp_mi_dump_player_rdram / _hidden_rdram); `words` stays
+ * valid until the next call.  rdp_mi_dump_player_apply drives a processor
+ * with one record as the reference's replay driver does
+ * (replayer_driver_parallel.cpp:124-205): commands are enqueued, VI
+ * registers set, SignalComplete flushes, and each UpdateDramFlush /
+ * UpdateHiddenDramFlush waits for idle and copies the shadow into the
+ * processor's RDRAM / hidden RDRAM; EndFrame is left to the caller (the
+ * reference's driver scans out there with blend_previous_frame = 1 and
+ * upscale_deinterlacing = 0). */
+typedef struct rdp_mi_dump_player rdp_mi_dump_player;
+typedef struct rdp_mi_dump_event {
+	uint32_t kind;
+	uint32_t op, num_words;
+	const uint32_t *words;
+	uint32_t vi_register, vi_value;
+	uint32_t offset, size;
+} rdp_mi_dump_event;
+int rdp_mi_dump_player_open(const char *path, rdp_mi_dump_player **out);
+uint32_t rdp_mi_dump_player_rdram_size(const rdp_mi_dump_player *pl);
+uint32_t rdp_mi_dump_player_hidden_rdram_size(const rdp_mi_dump_player *pl);
+int rdp_mi_dump_player_rewind(rdp_mi_dump_player *pl);
+int rdp_mi_dump_player_next(rdp_mi_dump_player *pl, rdp_mi_dump_event *ev);
+const uint8_t *rdp_mi_dump_player_rdram(const rdp_mi_dump_player *pl);
+const uint8_t *rdp_mi_dump_player_hidden_rdram(const rdp_mi_dump_player *pl);
+int rdp_mi_dump_player_apply(rdp_mi_dump_player *pl, const rdp_mi_dump_event *ev, rdp_mi_processor *p);
+void rdp_mi_dump_player_destroy(rdp_mi_dump_player *pl);
 
 /* Library diagnostics. */
 const char *rdp_mi_last_error(void);

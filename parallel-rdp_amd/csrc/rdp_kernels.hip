@@ -1394,14 +1394,18 @@ DEV void write_color(Pixel &p, uint32_t fmt, int32_t r, int32_t g, int32_t b, in
 
 DEV void alias_color_to_depth(Pixel &p, uint32_t fmt)
 {
-	if (fmt == FB_RGBA5551) {
-		p.dz = (((uint32_t)p.c[3] >> 3) | ((uint32_t)p.c[2] & 8)) & 0xff;
-		p.depth = ((((uint32_t)p.c[0] & 0xf8) << 6) | (((uint32_t)p.c[1] & 0xf8) << 1) | (((uint32_t)p.c[2] & 0xf8) >> 4)) & 0xffff;
-	} else if (fmt == FB_IA88) {
-		uint32_t w = (((uint32_t)p.c[0] & 0xff) << 8) | ((uint32_t)p.c[3] & 0xff);
-		p.depth = (w >> 2) & 0xffff;
-		p.dz = (((w & 3) << 2) | ((w & 1) * 3)) & 0xff;
-	}
+	if (fmt != FB_RGBA5551 && fmt != FB_IA88) return;
+	// Both forms are computed and selected, then stored once each: stores of
+	// depth and dz from two branches get merged into one store at a variable
+	// field offset, which keeps the whole Pixel in scratch memory.
+	const uint32_t dz5551 = (((uint32_t)p.c[3] >> 3) | ((uint32_t)p.c[2] & 8)) & 0xff;
+	const uint32_t depth5551 =
+	    ((((uint32_t)p.c[0] & 0xf8) << 6) | (((uint32_t)p.c[1] & 0xf8) << 1) | (((uint32_t)p.c[2] & 0xf8) >> 4)) & 0xffff;
+	const uint32_t w = (((uint32_t)p.c[0] & 0xff) << 8) | ((uint32_t)p.c[3] & 0xff);
+	const uint32_t depth88 = (w >> 2) & 0xffff, dz88 = (((w & 3) << 2) | ((w & 1) * 3)) & 0xff;
+	const bool is5551 = fmt == FB_RGBA5551;
+	p.depth = is5551 ? depth5551 : depth88;
+	p.dz = is5551 ? dz5551 : dz88;
 }
 
 DEV void alias_depth_to_color(Pixel &p, uint32_t fmt)

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -29,6 +30,7 @@
 
 #include "../../include/rdp_mi.h"
 #include "rdp_digest.hpp"
+#include "rdp_dump.hpp"
 #include "rdp_frontend.hpp"
 #include "rdp_hazard.hpp"
 #include "rdp_types.hpp"
@@ -273,6 +275,14 @@ struct Processor {
 	std::condition_variable tl_cv;
 	uint64_t timeline_done = 0;
 	int timeline_error = 0;
+
+	// RDPDUMP2 capture of everything sent through rdp_mi_enqueue_command,
+	// the VI registers and scanouts (rdp_device.cpp:60-73, 874-897, 970-975,
+	// 1087-1092).  dump_in_list: the RDRAM state at the start of the current
+	// command list is already in the capture.
+	std::unique_ptr<DumpWriter> dump;
+	bool dump_in_list = false;
+	std::mutex dump_lock;
 
 	explicit Processor(uint32_t size) : fe(size), rdram_size(size) { memset(tmem_mirror, 0, sizeof(tmem_mirror)); }
 	uint8_t *host_rdram() { return user_rdram ? user_rdram : rdram_mirror.data(); }
@@ -1085,6 +1095,34 @@ struct Recording {
 	hipGraphExec_t exec = nullptr;
 	uint64_t raster_bytes = 0;
 	uint32_t passes = 0;
+	// The recorded processors' device state, for trial runs (autotune,
+	// profile) that must leave it as they found it.
+	std::vector<Processor *> procs;
+	DeviceBuffer saved;
+
+	size_t state_bytes() const
+	{
+		size_t n = 0;
+		for (const Processor *p : procs) n += (size_t)p->rdram_size + p->rdram_size / 2 + 4096;
+		return n;
+	}
+	// Copies every processor's RDRAM, hidden RDRAM and TMEM aside (restore
+	// = false) or back (restore = true), in stream order.
+	int copy_state(hipStream_t st, bool restore)
+	{
+		if (!restore && saved.reserve(state_bytes(), st)) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "state save buffer");
+		size_t off = 0;
+		for (Processor *p : procs) {
+			void *dev[3] = { p->d_rdram, p->d_hidden, p->d_tmem };
+			const size_t sz[3] = { p->rdram_size, p->rdram_size / 2, 4096 };
+			for (int k = 0; k < 3; k++) {
+				void *dst = restore ? dev[k] : saved.ptr + off, *src = restore ? saved.ptr + off : dev[k];
+				HIP_CHECK(hipMemcpyAsync(dst, src, sz[k], hipMemcpyDeviceToDevice, st));
+				off += sz[k];
+			}
+		}
+		return 0;
+	}
 };
 
 // ---------------------------------------------------------------------------
@@ -1185,6 +1223,8 @@ struct Server {
 	uint64_t next_ticket = 1;
 	size_t outstanding = 0;    // submitted by the caller, not completed
 	size_t pending_bytes = 0;  // admitted, not yet submitted to the GPU
+	size_t decoding = 0;       // admitted, not yet decoded
+	double batch_window_us = 0;  // how long ready frames may wait for more (0: submit at once)
 	std::vector<ServeResult> results;
 	bool started = false, stopping = false;
 	int error = 0;
@@ -1194,6 +1234,7 @@ struct Server {
 	struct Ready {
 		ServeResult r;
 		size_t bytes;
+		double ready_us;
 	};
 	std::deque<Ready> ready;
 
@@ -1230,6 +1271,14 @@ struct Server {
 	static size_t estimate_bytes(const ServeRequest &r) { return r.words.size() * 4 * 24 + (1u << 20); }
 
 	bool done() const { return stopping && outstanding == 0; }
+	// With a batching window, a free lane takes the ready frames only once
+	// max_batch are ready, nothing admitted is still decoding, or the oldest
+	// has waited the window; returns the microseconds left otherwise.
+	double window_left(double now) const
+	{
+		if (batch_window_us <= 0 || ready.size() >= max_batch || decoding == 0) return 0;
+		return std::max(0.0, ready.front().ready_us + batch_window_us - now);
+	}
 	int queue_digests(LaneState &ls, const std::vector<Processor *> &ps);
 	void decode(ServeRequest req, size_t bytes, double admit);
 	void scheduler_loop();
@@ -1281,7 +1330,9 @@ void Server::decode(ServeRequest req, size_t bytes, double admit)
 		p->fe.flush();
 	}
 	std::lock_guard<std::mutex> g(m);
-	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0, 0, 0 }, bytes });
+	ready.push_back(Ready{ ServeResult{ req.ticket, req.instance, req.arrival_us, admit, 0.0, 0.0, 0, 0 }, bytes,
+	                       now_us() });
+	decoding--;
 	submit_cv.notify_all();  // any lane's submitter may be the free one
 }
 
@@ -1304,6 +1355,7 @@ void Server::scheduler_loop()
 			const size_t est = estimate_bytes(r);
 			if (pending_bytes != 0 && pending_bytes + est > budget) break;
 			pending_bytes += est;
+			decoding++;
 			busy[i] = true;
 			ServeRequest req = std::move(queues[i].front());
 			queues[i].pop_front();
@@ -1327,6 +1379,10 @@ void Server::submitter_loop(LaneState *ls)
 		submit_cv.wait(g, [&] { return done() || (!ready.empty() && !ls->busy); });
 		if (ready.empty() && done()) break;
 		if (ready.empty() || ls->busy) continue;
+		if (const double left = window_left(now_us())) {
+			submit_cv.wait_for(g, std::chrono::microseconds((int64_t)std::max(1.0, left)));
+			continue;
+		}
 
 		// Take ready frames in order, within max_batch and the budget.
 		std::vector<Ready> batch;
@@ -1494,6 +1550,15 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 	if (hipMemsetAsync(p.d_tmem, 0, 4096, c.stream) != hipSuccess) return fail(RDP_MI_ERROR_DEVICE, "TMEM clear failed");
 	if (hipStreamSynchronize(c.stream) != hipSuccess) return fail(RDP_MI_ERROR_DEVICE, "device synchronisation failed");
 	p.supported = true;
+	// The reference's capture switch (rdp_device.cpp:60-73): failing to open
+	// the file is reported and leaves capture off.
+	if (const char *env = getenv("PARALLEL_RDP_DUMP_PATH")) {
+		std::unique_ptr<DumpWriter> w(new DumpWriter());
+		if (w->open(env, (uint32_t)rdram_size, (uint32_t)hidden_rdram_size))
+			p.dump = std::move(w);
+		else
+			fprintf(stderr, "rdp_mi: cannot open the RDP dump %s\n", env);
+	}
 	*out = h;
 	return 0;
 }
@@ -1503,6 +1568,7 @@ void rdp_mi_destroy(rdp_mi_processor *h)
 	if (!h) return;
 	rdp_mi_idle(h);
 	Processor &p = h->p;
+	if (p.dump) p.dump->close();
 	stop_ring(p);
 	p.vi.release();
 	(void)hipFree(p.d_rdram);
@@ -1524,17 +1590,60 @@ void rdp_mi_enqueue_command_direct(rdp_mi_processor *h, unsigned num_words, cons
 	run_commands(p, words, len);
 }
 
+// Capture around one command (CommandProcessor::enqueue_command,
+// rdp_device.cpp:874-897): the first command of a command list writes the
+// RDRAM and hidden RDRAM blocks that changed since the last capture; a
+// SyncFull ends the list and is recorded as SignalComplete.
+static void dump_command(rdp_mi_processor *h, const uint32_t *words, unsigned len, bool before)
+{
+	Processor &p = h->p;
+	std::lock_guard<std::mutex> g(p.dump_lock);
+	if (!p.dump) return;
+	if (before) {
+		if (p.dump_in_list) return;
+		rdp_mi_idle(h);
+		if (const void *r = rdp_mi_begin_read_rdram(h)) p.dump->memory((const uint8_t *)r, p.rdram_size, false);
+		if (const void *hd = rdp_mi_begin_read_hidden_rdram(h))
+			p.dump->memory((const uint8_t *)hd, p.rdram_size / 2, true);
+		p.dump_in_list = true;
+		return;
+	}
+	const uint32_t op = (words[0] >> 24) & 63;
+	if (op == 0x29) {
+		p.dump->signal_complete();
+		p.dump_in_list = false;
+	} else
+		p.dump->command(op, words, len);
+}
+
 void rdp_mi_enqueue_command(rdp_mi_processor *h, unsigned num_words, const uint32_t *words)
 {
 	if (!h || !words || num_words == 0) return;
 	unsigned len = Frontend::command_length(words[0]);
 	if (len == 0 || len > num_words) return;
+	if (h->p.dump) dump_command(h, words, len, true);
 	ring_push(h->p, words, len);
+	if (h->p.dump) dump_command(h, words, len, false);
 }
 
 size_t rdp_mi_enqueue_command_list(rdp_mi_processor *h, size_t num_words, const uint32_t *words)
 {
 	if (!h || !words) return 0;
+	if (h->p.dump) {
+		// Capturing: one command at a time, each through the capture.
+		size_t i = 0;
+		while (i < num_words) {
+			const unsigned len = Frontend::command_length(words[i]);
+			if (len == 0) {
+				i++;
+				continue;
+			}
+			if (i + len > num_words) return i;
+			rdp_mi_enqueue_command(h, len, words + i);
+			i += len;
+		}
+		return i;
+	}
 	size_t i = 0, start = 0;
 	// Hand runs of complete commands to the ring, at most kRingWords at once;
 	// words with an undefined or private opcode end a run and are skipped.
@@ -1666,6 +1775,23 @@ void rdp_mi_set_vi_register(rdp_mi_processor *h, unsigned reg, uint32_t value)
 {
 	if (!h || reg >= RDP_MI_VI_COUNT) return;
 	h->p.vi_regs[reg] = value;
+	if (h->p.dump) {
+		std::lock_guard<std::mutex> g(h->p.dump_lock);
+		if (h->p.dump) h->p.dump->vi_register(reg, value);
+	}
+}
+
+// Scanout with capture (rdp_device.cpp:1087-1092): the frame's RDRAM and
+// hidden RDRAM blocks, then EndFrame.
+static void dump_scanout(rdp_mi_processor *h)
+{
+	Processor &p = h->p;
+	std::lock_guard<std::mutex> g(p.dump_lock);
+	if (!p.dump) return;
+	rdp_mi_idle(h);
+	if (const void *r = rdp_mi_begin_read_rdram(h)) p.dump->memory((const uint8_t *)r, p.rdram_size, false);
+	if (const void *hd = rdp_mi_begin_read_hidden_rdram(h)) p.dump->memory((const uint8_t *)hd, p.rdram_size / 2, true);
+	p.dump->end_frame();
 }
 
 void rdp_mi_scanout_options_default(rdp_mi_scanout_options *o)
@@ -1700,6 +1826,7 @@ int rdp_mi_scanout_sync_ex(rdp_mi_processor *h, const rdp_mi_scanout_options *op
                            unsigned *height)
 {
 	if (!h || !opts || !out || !width || !height) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	if (h->p.dump) dump_scanout(h);
 	return scanout_queue(h->p, *opts, out, width, height, nullptr);
 }
 
@@ -1745,6 +1872,7 @@ int rdp_mi_scanout_async_buffer(rdp_mi_processor *h, const rdp_mi_scanout_option
 	if (b->in_flight) HIP_CHECK(hipEventSynchronize(b->fence));
 	b->in_flight = false;
 	b->width = b->height = 0;
+	if (h->p.dump) dump_scanout(h);
 	int err = scanout_queue(h->p, *opts, b->pixels, &b->width, &b->height, b->fence);
 	if (err) return err;
 	b->in_flight = true;
@@ -1862,6 +1990,7 @@ int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, r
 		h->r.passes += L.passes;
 	}
 	h->r.cfg = c.cfg;
+	h->r.procs = ps;
 	*out = h;
 	return 0;
 }
@@ -1925,7 +2054,8 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	hipEvent_t e0, e1;
 	HIP_CHECK(hipEventCreate(&e0));
 	HIP_CHECK(hipEventCreate(&e1));
-	int err = execute_launches(r.launches, c, r.cfg);  // warm caches and code
+	int err = r.copy_state(c.stream, false);
+	if (!err) err = execute_launches(r.launches, c, r.cfg);  // warm caches and code
 	// Two sweeps over the candidates, best of each candidate's timed runs.
 	for (int sweep = 0; sweep < 2 && !err; sweep++)
 		for (size_t i = 0; i < cands.size() && !err; i++) {
@@ -1940,6 +2070,8 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 		}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
+	if (!err) err = r.copy_state(c.stream, true);
+	if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "autotune failed");
 	if (err) return err;
 	size_t pick = 0;
 	for (size_t i = 1; i < cands.size(); i++)
@@ -1974,7 +2106,7 @@ int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, fl
 	std::vector<hipEvent_t> ev(nev + 1);
 	for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
 	std::vector<double> acc(nev, 0.0);
-	int err = 0;
+	int err = r.copy_state(c.stream, false);
 	for (unsigned it = 0; it < iterations && !err; it++) {
 		HIP_CHECK(hipEventRecord(ev[0], c.stream));
 		err = execute_launches(r.launches, c, r.cfg, ev.data() + 1);
@@ -1985,6 +2117,8 @@ int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, fl
 			acc[k] += ms;
 		}
 	}
+	if (!err) err = r.copy_state(c.stream, true);
+	if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "profile failed");
 	for (auto &e : ev) (void)hipEventDestroy(e);
 	for (size_t k = 0; k < nev; k++) round_stage_ms[k] = (float)(acc[k] / iterations);
 	return err;
@@ -2000,7 +2134,7 @@ int rdp_mi_recording_profile(rdp_mi_recording *h, unsigned iterations, float sta
 	std::vector<hipEvent_t> ev(nev + 1);
 	for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
 	double acc[4] = { 0, 0, 0, 0 };
-	int err = 0;
+	int err = r.copy_state(c.stream, false);
 	for (unsigned it = 0; it < iterations && !err; it++) {
 		HIP_CHECK(hipEventRecord(ev[0], c.stream));
 		err = execute_launches(r.launches, c, r.cfg, ev.data() + 1);
@@ -2011,6 +2145,8 @@ int rdp_mi_recording_profile(rdp_mi_recording *h, unsigned iterations, float sta
 			acc[k % 4] += ms;
 		}
 	}
+	if (!err) err = r.copy_state(c.stream, true);
+	if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "profile failed");
 	for (auto &e : ev) (void)hipEventDestroy(e);
 	for (int k = 0; k < 4; k++) stage_ms[k] = (float)(acc[k] / iterations);
 	return err;
@@ -2190,6 +2326,15 @@ size_t rdp_mi_server_results(rdp_mi_server *h, uint64_t *tickets, uint32_t *inst
 	return n;
 }
 
+int rdp_mi_server_set_batch_window(rdp_mi_server *h, double window_us)
+{
+	if (!h || !(window_us >= 0)) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	std::lock_guard<std::mutex> g(h->s.m);
+	h->s.batch_window_us = window_us;
+	h->s.submit_cv.notify_all();
+	return 0;
+}
+
 int rdp_mi_server_set_digests(rdp_mi_server *h, int enable)
 {
 	if (!h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null server");
@@ -2287,8 +2432,181 @@ void rdp_mi_recording_destroy(rdp_mi_recording *h)
 	if (h->r.graph) (void)hipGraphDestroy(h->r.graph);
 	if (h->r.input.ptr) (void)hipFree(h->r.input.ptr);
 	if (h->r.scratch.ptr) (void)hipFree(h->r.scratch.ptr);
+	if (h->r.saved.ptr) (void)hipFree(h->r.saved.ptr);
 	delete h;
 }
+
+// ---- RDPDUMP2 capture and replay ----
+
+struct rdp_mi_dump_writer {
+	DumpWriter w;
+};
+
+int rdp_mi_dump_writer_open(const char *path, uint32_t rdram_size, uint32_t hidden_rdram_size, rdp_mi_dump_writer **out)
+{
+	if (!out || !path) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	*out = nullptr;
+	rdp_mi_dump_writer *w = new (std::nothrow) rdp_mi_dump_writer();
+	if (!w) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	if (!w->w.open(path, rdram_size, hidden_rdram_size)) {
+		delete w;
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "cannot create %s", path);
+	}
+	*out = w;
+	return 0;
+}
+
+void rdp_mi_dump_writer_flush_rdram(rdp_mi_dump_writer *w, const void *rdram, uint32_t size)
+{
+	if (w && rdram) w->w.memory((const uint8_t *)rdram, size, false);
+}
+
+void rdp_mi_dump_writer_flush_hidden_rdram(rdp_mi_dump_writer *w, const void *hidden, uint32_t size)
+{
+	if (w && hidden) w->w.memory((const uint8_t *)hidden, size, true);
+}
+
+void rdp_mi_dump_writer_command(rdp_mi_dump_writer *w, uint32_t op, const uint32_t *words, uint32_t num_words)
+{
+	if (w && (words || !num_words)) w->w.command(op, words, num_words);
+}
+
+void rdp_mi_dump_writer_set_vi_register(rdp_mi_dump_writer *w, uint32_t reg, uint32_t value)
+{
+	if (w) w->w.vi_register(reg, value);
+}
+
+void rdp_mi_dump_writer_end_frame(rdp_mi_dump_writer *w)
+{
+	if (w) w->w.end_frame();
+}
+
+void rdp_mi_dump_writer_signal_complete(rdp_mi_dump_writer *w)
+{
+	if (w) w->w.signal_complete();
+}
+
+void rdp_mi_dump_writer_close(rdp_mi_dump_writer *w)
+{
+	if (!w) return;
+	w->w.close();
+	delete w;
+}
+
+int rdp_mi_begin_dump(rdp_mi_processor *h, const char *path)
+{
+	if (!h || !path) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	Processor &p = h->p;
+	ring_drain(p);
+	std::lock_guard<std::mutex> g(p.dump_lock);
+	if (p.dump) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "the processor is already capturing");
+	std::unique_ptr<DumpWriter> w(new (std::nothrow) DumpWriter());
+	if (!w) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	if (!w->open(path, p.rdram_size, p.rdram_size / 2)) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "cannot create %s", path);
+	p.dump = std::move(w);
+	p.dump_in_list = false;
+	return 0;
+}
+
+void rdp_mi_end_dump(rdp_mi_processor *h)
+{
+	if (!h) return;
+	Processor &p = h->p;
+	ring_drain(p);
+	std::lock_guard<std::mutex> g(p.dump_lock);
+	if (p.dump) p.dump->close();
+	p.dump.reset();
+	p.dump_in_list = false;
+}
+
+struct rdp_mi_dump_player {
+	DumpReader r;
+};
+
+int rdp_mi_dump_player_open(const char *path, rdp_mi_dump_player **out)
+{
+	if (!out || !path) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	*out = nullptr;
+	rdp_mi_dump_player *pl = new (std::nothrow) rdp_mi_dump_player();
+	if (!pl) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
+	if (!pl->r.open(path)) {
+		delete pl;
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "%s is not an RDPDUMP2 capture", path);
+	}
+	*out = pl;
+	return 0;
+}
+
+uint32_t rdp_mi_dump_player_rdram_size(const rdp_mi_dump_player *pl) { return pl ? pl->r.rdram_size() : 0; }
+uint32_t rdp_mi_dump_player_hidden_rdram_size(const rdp_mi_dump_player *pl) { return pl ? pl->r.hidden_size() : 0; }
+
+int rdp_mi_dump_player_rewind(rdp_mi_dump_player *pl)
+{
+	if (!pl) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null player");
+	return pl->r.rewind() ? 0 : set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "rewind failed");
+}
+
+int rdp_mi_dump_player_next(rdp_mi_dump_player *pl, rdp_mi_dump_event *ev)
+{
+	if (!pl || !ev) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	DumpEvent e;
+	const int r = pl->r.next(e);
+	memset(ev, 0, sizeof(*ev));
+	if (r < 0) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "malformed capture record");
+	if (r == 0) {
+		ev->kind = RDP_MI_DUMP_EOF;
+		return 0;
+	}
+	ev->kind = e.kind;
+	ev->op = e.op;
+	ev->num_words = e.num_words;
+	ev->words = e.words;
+	ev->vi_register = e.vi_register;
+	ev->vi_value = e.vi_value;
+	ev->offset = e.offset;
+	ev->size = e.size;
+	return 1;
+}
+
+const uint8_t *rdp_mi_dump_player_rdram(const rdp_mi_dump_player *pl) { return pl ? pl->r.rdram() : nullptr; }
+const uint8_t *rdp_mi_dump_player_hidden_rdram(const rdp_mi_dump_player *pl) { return pl ? pl->r.hidden() : nullptr; }
+
+// The reference's replay driver (replayer_driver_parallel.cpp:124-205).
+int rdp_mi_dump_player_apply(rdp_mi_dump_player *pl, const rdp_mi_dump_event *ev, rdp_mi_processor *h)
+{
+	if (!pl || !ev || !h) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	switch (ev->kind) {
+	case RDP_MI_DUMP_RDP_COMMAND:
+		if (ev->num_words) rdp_mi_enqueue_command(h, ev->num_words, ev->words);
+		return 0;
+	case RDP_MI_DUMP_SET_VI_REGISTER:
+		if (ev->vi_register >= RDP_MI_VI_COUNT)
+			return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "VI register %u", ev->vi_register);
+		rdp_mi_set_vi_register(h, ev->vi_register, ev->vi_value);
+		return 0;
+	case RDP_MI_DUMP_SIGNAL_COMPLETE:
+		rdp_mi_flush(h);
+		return 0;
+	case RDP_MI_DUMP_UPDATE_DRAM_FLUSH:
+	case RDP_MI_DUMP_UPDATE_HIDDEN_DRAM_FLUSH: {
+		const bool hidden = ev->kind == RDP_MI_DUMP_UPDATE_HIDDEN_DRAM_FLUSH;
+		if (int err = rdp_mi_idle(h)) return err;
+		void *dst = hidden ? rdp_mi_begin_read_hidden_rdram(h) : rdp_mi_begin_read_rdram(h);
+		if (!dst) return set_error(RDP_MI_ERROR_DEVICE, "RDRAM read-back failed");
+		const size_t n = hidden ? std::min<size_t>(pl->r.hidden_size(), rdp_mi_get_hidden_rdram_size(h))
+		                        : std::min<size_t>(pl->r.rdram_size(), rdp_mi_get_rdram_size(h));
+		memcpy(dst, hidden ? pl->r.hidden() : pl->r.rdram(), n);
+		if (hidden)
+			rdp_mi_end_write_hidden_rdram(h);
+		else
+			rdp_mi_end_write_rdram(h);
+		return 0;
+	}
+	default: return 0;  // block updates land in the shadow; EndFrame and EOF are the caller's
+	}
+}
+
+void rdp_mi_dump_player_destroy(rdp_mi_dump_player *pl) { delete pl; }
 
 }  // extern "C"
 

@@ -56,6 +56,22 @@ class ScanoutOptions(ctypes.Structure):
         return o
 
 
+class DumpEvent(ctypes.Structure):
+    """rdp_mi_dump_event: one record of an RDPDUMP2 capture."""
+    _fields_ = [("kind", ctypes.c_uint32), ("op", ctypes.c_uint32), ("num_words", ctypes.c_uint32),
+                ("words", ctypes.POINTER(ctypes.c_uint32)), ("vi_register", ctypes.c_uint32),
+                ("vi_value", ctypes.c_uint32), ("offset", ctypes.c_uint32), ("size", ctypes.c_uint32)]
+
+
+# Record kinds of RDPDUMP2 (rdp_dump.cpp:28-40); EOF is the player's end marker.
+DUMP_EOF, DUMP_UPDATE_DRAM, DUMP_RDP_COMMAND, DUMP_SET_VI_REGISTER, DUMP_END_FRAME = 0, 1, 2, 3, 4
+DUMP_SIGNAL_COMPLETE, DUMP_END_OF_FILE, DUMP_UPDATE_DRAM_FLUSH = 5, 6, 7
+DUMP_UPDATE_HIDDEN_DRAM, DUMP_UPDATE_HIDDEN_DRAM_FLUSH = 8, 9
+
+# Draw commands (replayer_driver.hpp:92-111, command_is_draw_call).
+DRAW_OPS = frozenset((0x08, 0x09, 0x0a, 0x0b, 0x0c, 0x0d, 0x0e, 0x0f, 0x24, 0x25, 0x36))
+
+
 def load_library():
     global _lib
     if _lib is not None:
@@ -123,6 +139,36 @@ def load_library():
     lib.rdp_mi_primitive_count.argtypes = [vp]
     lib.rdp_mi_state_digest.restype = ctypes.c_int
     lib.rdp_mi_state_digest.argtypes = [vp, ctypes.POINTER(u64)]
+    # RDPDUMP2 captures
+    lib.rdp_mi_begin_dump.restype = ctypes.c_int
+    lib.rdp_mi_begin_dump.argtypes = [vp, ctypes.c_char_p]
+    lib.rdp_mi_end_dump.argtypes = [vp]
+    lib.rdp_mi_dump_writer_open.restype = ctypes.c_int
+    lib.rdp_mi_dump_writer_open.argtypes = [ctypes.c_char_p, u32, u32, ctypes.POINTER(vp)]
+    lib.rdp_mi_dump_writer_flush_rdram.argtypes = [vp, vp, u32]
+    lib.rdp_mi_dump_writer_flush_hidden_rdram.argtypes = [vp, vp, u32]
+    lib.rdp_mi_dump_writer_command.argtypes = [vp, u32, vp, u32]
+    lib.rdp_mi_dump_writer_set_vi_register.argtypes = [vp, u32, u32]
+    lib.rdp_mi_dump_writer_end_frame.argtypes = [vp]
+    lib.rdp_mi_dump_writer_signal_complete.argtypes = [vp]
+    lib.rdp_mi_dump_writer_close.argtypes = [vp]
+    lib.rdp_mi_dump_player_open.restype = ctypes.c_int
+    lib.rdp_mi_dump_player_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    lib.rdp_mi_dump_player_rdram_size.restype = u32
+    lib.rdp_mi_dump_player_rdram_size.argtypes = [vp]
+    lib.rdp_mi_dump_player_hidden_rdram_size.restype = u32
+    lib.rdp_mi_dump_player_hidden_rdram_size.argtypes = [vp]
+    lib.rdp_mi_dump_player_rewind.restype = ctypes.c_int
+    lib.rdp_mi_dump_player_rewind.argtypes = [vp]
+    lib.rdp_mi_dump_player_next.restype = ctypes.c_int
+    lib.rdp_mi_dump_player_next.argtypes = [vp, ctypes.POINTER(DumpEvent)]
+    lib.rdp_mi_dump_player_rdram.restype = vp
+    lib.rdp_mi_dump_player_rdram.argtypes = [vp]
+    lib.rdp_mi_dump_player_hidden_rdram.restype = vp
+    lib.rdp_mi_dump_player_hidden_rdram.argtypes = [vp]
+    lib.rdp_mi_dump_player_apply.restype = ctypes.c_int
+    lib.rdp_mi_dump_player_apply.argtypes = [vp, ctypes.POINTER(DumpEvent), vp]
+    lib.rdp_mi_dump_player_destroy.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -307,6 +353,14 @@ class CommandProcessor:
     def primitive_count(self) -> int:
         return int(self._lib.rdp_mi_primitive_count(self._h))
 
+    def begin_dump(self, path: str):
+        """Capture this processor's commands, VI registers and scanouts to an
+        RDPDUMP2 file (the reference's PARALLEL_RDP_DUMP_PATH capture)."""
+        _check(self._lib.rdp_mi_begin_dump(self._h, os.fsencode(path)), "begin_dump")
+
+    def end_dump(self):
+        self._lib.rdp_mi_end_dump(self._h)
+
     def state_digest(self) -> int:
         """Device-side digest of RDRAM, hidden RDRAM and TMEM after the work
         submitted so far (call after idle())."""
@@ -483,6 +537,8 @@ class Server:
         lib.rdp_mi_server_reset.argtypes = [vp, ctypes.c_double]
         lib.rdp_mi_server_destroy.argtypes = [vp]
         lib.rdp_mi_server_set_digests.restype = ctypes.c_int
+        lib.rdp_mi_server_set_batch_window.restype = ctypes.c_int
+        lib.rdp_mi_server_set_batch_window.argtypes = [vp, ctypes.c_double]
         lib.rdp_mi_server_set_digests.argtypes = [vp, ctypes.c_int]
         lib.rdp_mi_server_digests.restype = sz
         lib.rdp_mi_server_digests.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
@@ -533,6 +589,11 @@ class Server:
             out["submit_us"].ctypes.data_as(dp), out["complete_us"].ctypes.data_as(dp), n))
         return {k: v[:min(n, got)] for k, v in out.items()}
 
+    def set_batch_window(self, window_us: float):
+        """A free submission stream waits until max_batch frames are ready, no
+        admitted frame is still decoding, or the oldest has waited window_us."""
+        _check(self._lib.rdp_mi_server_set_batch_window(self._h, float(window_us)), "rdp_mi_server_set_batch_window")
+
     def set_digests(self, enable: bool = True):
         """Digest every served processor's state after each frame (before start())."""
         _check(self._lib.rdp_mi_server_set_digests(self._h, int(enable)), "rdp_mi_server_set_digests")
@@ -572,3 +633,96 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class DumpWriter:
+    """RDPDumpWriter: writes an RDPDUMP2 capture (rdp_dump_write.hpp:31-57)."""
+
+    def __init__(self, path: str, rdram_size: int = 8 << 20, hidden_rdram_size: int = 4 << 20):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.rdp_mi_dump_writer_open(os.fsencode(path), rdram_size, hidden_rdram_size, ctypes.byref(h)),
+               "dump_writer_open")
+        self._h = h
+
+    def flush_rdram(self, rdram: np.ndarray):
+        a = np.ascontiguousarray(rdram)
+        self._lib.rdp_mi_dump_writer_flush_rdram(self._h, a.ctypes.data, a.nbytes)
+
+    def flush_hidden_rdram(self, hidden: np.ndarray):
+        a = np.ascontiguousarray(hidden, dtype=np.uint8)
+        self._lib.rdp_mi_dump_writer_flush_hidden_rdram(self._h, a.ctypes.data, a.nbytes)
+
+    def command(self, words: np.ndarray):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        self._lib.rdp_mi_dump_writer_command(self._h, (int(w[0]) >> 24) & 63, w.ctypes.data, w.size)
+
+    def set_vi_register(self, reg: int, value: int):
+        self._lib.rdp_mi_dump_writer_set_vi_register(self._h, reg, value & 0xFFFFFFFF)
+
+    def end_frame(self):
+        self._lib.rdp_mi_dump_writer_end_frame(self._h)
+
+    def signal_complete(self):
+        self._lib.rdp_mi_dump_writer_signal_complete(self._h)
+
+    def close(self):
+        if self._h:
+            self._lib.rdp_mi_dump_writer_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class DumpPlayer:
+    """DumpPlayer (rdp_dump.hpp:49-68) over the library's reader."""
+
+    def __init__(self, path: str):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.rdp_mi_dump_player_open(os.fsencode(path), ctypes.byref(h)), "dump_player_open")
+        self._h = h
+        self.rdram_size = int(self._lib.rdp_mi_dump_player_rdram_size(h))
+        self.hidden_rdram_size = int(self._lib.rdp_mi_dump_player_hidden_rdram_size(h))
+        self.event = DumpEvent()
+
+    def next(self) -> Optional[DumpEvent]:
+        """The next record, or None at the end of the capture."""
+        r = self._lib.rdp_mi_dump_player_next(self._h, ctypes.byref(self.event))
+        if r < 0:
+            raise RDPError("malformed capture: " + last_error())
+        return None if r == 0 else self.event
+
+    def words(self) -> np.ndarray:
+        e = self.event
+        return np.ctypeslib.as_array(e.words, shape=(e.num_words,)).copy() if e.num_words else np.zeros(0, np.uint32)
+
+    def apply(self, proc: CommandProcessor):
+        """Drive `proc` with the current record as the reference's replay driver does."""
+        _check(self._lib.rdp_mi_dump_player_apply(self._h, ctypes.byref(self.event), proc.handle), "dump_player_apply")
+
+    def rdram(self) -> np.ndarray:
+        p = self._lib.rdp_mi_dump_player_rdram(self._h)
+        return np.frombuffer((ctypes.c_uint8 * self.rdram_size).from_address(p), dtype=np.uint32).copy()
+
+    def hidden_rdram(self) -> np.ndarray:
+        p = self._lib.rdp_mi_dump_player_hidden_rdram(self._h)
+        return np.frombuffer((ctypes.c_uint8 * self.hidden_rdram_size).from_address(p), dtype=np.uint8).copy()
+
+    def rewind(self):
+        _check(self._lib.rdp_mi_dump_player_rewind(self._h), "dump_player_rewind")
+
+    def close(self):
+        if self._h:
+            self._lib.rdp_mi_dump_player_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -173,3 +173,29 @@ def test_autotuned_recording_bit_exact(rdp_mi):
     rec.close()
     for p in procs:
         p.close()
+
+
+def test_trial_runs_leave_state_unchanged(rdp_mi):
+    """Autotune and profile execute the recording many times; random-state
+    streams are not idempotent, so only a saved and restored state lets one
+    replay afterwards give the oracle's result."""
+    streams = [rs.fuzz_stream(3000 + i) for i in range(6)]
+    refs = [harness.run_oracle(s) for s in streams]
+    say("[case] autotune and profile on non-idempotent streams")
+    procs = _procs_for(rdp_mi, [rs.Stream(s.name, s.seed, [], s.rdram_init, s.hidden_init) for s in streams])
+    before = [_state(p) for p in procs]
+    for p, s in zip(procs, streams):
+        for f in s.frames:
+            p.enqueue_commands(f.words)
+    rec = rdp_mi.Recording(procs, batched=True)
+    rec.autotune(1)
+    rec.profile(2)
+    rec.profile_rounds(1)
+    for p, b, s in zip(procs, before, streams):
+        assert_same(_state(p), b, f"{s.name} state after trial runs")
+    rec.replay(1, use_graph=True)
+    for p, s, r in zip(procs, streams, refs):
+        assert_same(_state(p), r, f"{s.name} after one replay")
+    rec.close()
+    for p in procs:
+        p.close()

@@ -81,7 +81,9 @@ def test_tmem_replay_order_across_rounds(sources):
 
 def test_tmem_replay_order_through_server():
     """The served form batches the same frames: 12 instances, requests queued
-    together so one batch carries every instance's two passes."""
+    together and a batching window that holds the first batch until every
+    frame has decoded, so one batch carries every instance's two passes
+    (rounds of 12 passes: the side-stream TMEM replay runs)."""
     import rdp_mi
     streams = [texture_stream(200 + i, i % 2 == 0) for i in range(12)]
     print(f"[case] server batch of {len(streams)} render-to-texture frames", flush=True)
@@ -90,11 +92,14 @@ def test_tmem_replay_order_through_server():
         p.write_rdram(s.rdram_init)
         p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
     with rdp_mi.Server(procs, streams=1) as srv:
+        srv.set_batch_window(10e6)
         for inst, s in enumerate(streams):
             srv.submit(inst, s.frames[0].words, arrival_us=0.0)
         srv.start()
         srv.drain()
         assert srv.results()["ticket"].size == len(streams)
+        st = srv.stats()
+        assert st["batches"] == 1 and st["max_batch_frames"] == len(streams), st
     outs = [(p.read_rdram(), p.read_hidden_rdram(), p.read_tmem()) for p in procs]
     for p in procs:
         p.close()
