@@ -64,23 +64,73 @@ def enqueue_all(procs, streams):
             p.enqueue_commands(f.words)
 
 
-def oracle_results(streams):
-    """CPU oracle over every stream (also the CPU baseline sample)."""
+def _oracle_module():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    results = []
+    return oracle
+
+
+def oracle_stream(s, digests=False, scanout_frames=False):
+    """CPU oracle over one stream: final (RDRAM, hidden RDRAM, TMEM), the
+    state digest after every frame (digests) and the VI image of every frame
+    that carries VI registers (scanout_frames), and the seconds it took."""
+    oracle = _oracle_module()
     t0 = time.perf_counter()
-    for s in streams:
-        o = oracle.Oracle(rs.RDRAM_SIZE)
-        if s.rdram_init is not None:
-            o.rdram[:] = s.rdram_init
-        o.hidden[:] = s.hidden_init
-        for f in s.frames:
-            o.process(f.words)
-        o.flush()
-        results.append((o.rdram.copy(), o.hidden.copy(), o.tmem.copy()))
-        o.close()
-    return results, time.perf_counter() - t0
+    o = oracle.Oracle(rs.RDRAM_SIZE)
+    if s.rdram_init is not None:
+        o.rdram[:] = s.rdram_init
+    o.hidden[:] = s.hidden_init
+    per_frame, images = [], []
+    for f in s.frames:
+        o.process(f.words)
+        if scanout_frames and f.vi:
+            for r, v in f.vi.items():
+                o.set_vi_register(r, v)
+            images.append(o.scanout())
+        if digests:
+            o.flush()
+            per_frame.append(o.digest())
+    o.flush()
+    out = (o.rdram.copy(), o.hidden.copy(), o.tmem.copy())
+    o.close()
+    return out, per_frame, images, time.perf_counter() - t0
+
+
+def cpu_threads():
+    """Host threads the CPU baseline may use: this process's CPU affinity."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_results(streams, threads):
+    """CPU oracle over every batch stream on `threads` host threads (the
+    ctypes calls release the GIL; each stream is independent), which is the
+    all-core CPU baseline and the reference the GPU results are checked
+    against; then a one-core sample: the first 16 streams in sequence."""
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        outs = list(ex.map(oracle_stream, streams))
+    wall = time.perf_counter() - t0
+    sample = streams[:16]
+    t0 = time.perf_counter()
+    for s in sample:
+        oracle_stream(s)
+    one_core = time.perf_counter() - t0
+    return [o[0] for o in outs], wall, len(sample), one_core
 
 
 def exact_frames(procs, refs):
@@ -172,6 +222,66 @@ def serving(rdp_mi, streams, refs, args):
     return out
 
 
+def config_rows(rdp_mi, pending, args):
+    """Configs 0-2 on their own: each one stream (one processor), its frames
+    recorded and replayed from device memory as one hipGraph per replay; the
+    VI scanout of config 2 is timed on the rendered frame (scanout_sync).  A
+    frame counts as bit-exact when the state digest after it (RDRAM, hidden
+    RDRAM and TMEM, computed on the device) equals the oracle's, frame by
+    frame, and the final state and, for config 2, the last scanout image
+    match byte for byte.  `pending` maps each config to the future of its
+    oracle run."""
+    rows = {}
+    for name, stream in (("c0", rs.config0_stream()), ("c1", rs.config1_stream()), ("c2", rs.config2_stream())):
+        frames = len(stream.frames)
+        # Per-frame digests: the frames submitted one at a time.
+        p = setup_processors(rdp_mi, [stream])[0]
+        digests = []
+        last_image = None
+        for f in stream.frames:
+            p.enqueue_commands(f.words)
+            if f.vi:
+                for r, v in f.vi.items():
+                    p.set_vi_register(r, v)
+                last_image = p.scanout_sync()
+            p.idle()
+            digests.append(p.state_digest())
+        p.close()
+        # Timed: the whole stream recorded once, replayed as a graph.
+        p = setup_processors(rdp_mi, [stream])[0]
+        p.enqueue_commands(stream.all_words())
+        rec = rdp_mi.Recording([p], batched=True)
+        rec.replay(1, use_graph=True)
+        reps = max(2, min(args.steps, 10))
+        ms = float(rec.replay(reps, use_graph=True).mean())
+        vi_ms = 0.0
+        if name == "c2":
+            for r, v in stream.frames[-1].vi.items():
+                p.set_vi_register(r, v)
+            p.scanout_sync()
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                img = p.scanout_sync()
+            vi_ms = (time.perf_counter() - t0) * 1e3 / frames
+        state = (p.read_rdram(), p.read_hidden_rdram(), p.read_tmem())
+        rec.close()
+        p.close()
+        ref, ref_digests, ref_images, cpu_s = pending[name].result()
+        final_ok = all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(state, ref))
+        image_ok = name != "c2" or (ref_images and last_image is not None and
+                                    np.array_equal(last_image, ref_images[-1]) and np.array_equal(img, ref_images[-1]))
+        exact = sum(int(a == b) for a, b in zip(digests, ref_digests)) if final_ok and image_ok else 0
+        frame_ms = ms / frames + vi_ms
+        rows[name] = {"frames": frames, "triangles": stream.num_triangles,
+                      "framebuffer": f"{stream.width}x{stream.height}",
+                      "gpu_frames_per_s": round(exact / (frame_ms / 1e3), 1), "ms_per_frame": round(frame_ms, 4),
+                      "render_ms_per_frame": round(ms / frames, 4), "vi_ms_per_frame": round(vi_ms, 4) or None,
+                      "bit_exact_frames": f"{exact}/{frames}",
+                      "cpu_1core_frames_per_s": round(frames / cpu_s, 3)}
+        log(f"[config] {name}: {rows[name]}")
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +291,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison (profiling runs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the per-config rows (configs 0-2 on their own)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (profiling runs: the tuner's trial launches stay out)")
     ap.add_argument("--instances", type=int, default=256)
@@ -220,8 +331,19 @@ def main():
     total_tris = sum(s.num_triangles for s in streams)
     log(f"[bench] {len(streams)} streams, {total_tris} triangles per step")
 
-    refs, cpu_seconds = oracle_results(streams)
-    log(f"[bench] oracle: {cpu_seconds:.2f}s for {len(streams)} frames")
+    threads = cpu_threads()
+    refs, cpu_wall, sample_n, sample_s = oracle_results(streams, threads)
+    log(f"[bench] oracle: {cpu_wall:.2f}s for {len(streams)} frames on {threads} threads; "
+        f"{sample_s:.2f}s for {sample_n} frames on one")
+    # The per-config oracle runs (long sequential streams) go on in the
+    # background while the GPU is measured.
+    from concurrent.futures import ThreadPoolExecutor
+    bg = ThreadPoolExecutor(max_workers=3)
+    pending = {}
+    if not args.no_configs:
+        pending = {"c0": bg.submit(oracle_stream, rs.config0_stream(), True),
+                   "c1": bg.submit(oracle_stream, rs.config1_stream(), True),
+                   "c2": bg.submit(oracle_stream, rs.config2_stream(), True, True)}
 
     # ---- batched path: one launch per stage per pass round, hipGraph replay
     procs = setup_processors(rdp_mi, streams)
@@ -243,6 +365,7 @@ def main():
     log(f"[bench] batched: {step_ms.mean():.3f} ms/step (gpu), wall {wall / args.steps * 1e3:.3f} ms/step, "
         f"bit-exact {exact}/{len(streams)} ({mism} mismatched bytes)")
     stage_ms = rec.profile(3)
+    launches = rec.profile_raster(3)
     raster_bytes = rec.raster_bytes
     rounds = rec.rounds
     raster_launches = rec.raster_launches
@@ -266,39 +389,49 @@ def main():
             p.close()
 
     serve = None if args.no_serve else serving(rdp_mi, streams, refs, args)
+    configs = None if args.no_configs else config_rows(rdp_mi, pending, args)
+    bg.shutdown()
 
     frames_per_step = len(streams)
     ms_per_step = float(step_ms.mean())
     value = exact / (ms_per_step / 1e3)
     plain_value = plain_exact / (float(plain_ms.mean()) / 1e3) if plain_ms is not None else 0.0
 
-    # k_raster is launched once per occupancy class present in a round; the
-    # per-launch figures average over those launches.
-    raster_ms_per_launch = float(stage_ms[3]) / max(raster_launches, 1)
-    bytes_per_launch = raster_bytes / max(raster_launches, 1)
-    achieved = bytes_per_launch / (raster_ms_per_launch / 1e3) / 1e9 if raster_ms_per_launch > 0 else 0.0
+    # The dominant kernel: the raster launch with the longest duration when
+    # each launch runs alone (events around it, stages in series, after the
+    # timed loop).  Its algorithmic bytes are the framebuffer colour, depth
+    # and hidden bits each tile reads and writes once.
+    dom = max(launches, key=lambda l: l["ms"])
+    achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9 if dom["ms"] > 0 else 0.0
 
-    # HBM bytes per k_raster launch from the committed rocprof counter runs
-    # (FETCH_SIZE and WRITE_SIZE, scaled by a plain-copy calibration).
-    traffic, valu = None, None
+    # HBM bytes and SQ instruction counts per launch of the dominant kernel
+    # come from the committed rocprofv3 counter runs of this workload
+    # (profiles/), not from this run: counters need their own profiled runs.
+    traffic, valu, traffic_src = None, None, None
     traffic_path = os.path.join(ROOT, "profiles", "k_raster_batch64_traffic.json")
-    if os.path.exists(traffic_path):
+    if os.path.exists(traffic_path) and args.streams == 64:
         with open(traffic_path) as fh:
-            prof = json.load(fh)
+            prof = json.load(fh).get(dom["kernel"], {})
         traffic = prof.get("traffic_bytes_per_launch")
-        # VALU issue rate: the committed SQ_INSTS_VALU count per launch of this
-        # deterministic workload over the live launch duration.
-        if prof.get("valu_insts_per_launch") and raster_ms_per_launch > 0 and args.streams == 64:
-            ops = prof["valu_insts_per_launch"] * 64 / (raster_ms_per_launch / 1e3)
+        traffic_src = "committed rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs (profiles/k_raster_batch64_traffic.json)"
+        # VALU issue rate: the committed SQ_INSTS_VALU count per launch over
+        # this run's duration of the launch.
+        if prof.get("valu_insts_per_launch") and dom["ms"] > 0:
+            ops = prof["valu_insts_per_launch"] * 64 / (dom["ms"] / 1e3)
             valu = {"achieved": round(ops / 1e12, 3), "peak": VALU_PEAK_TOPS, "unit": "T lane-ops/s",
                     "frac": round(ops / 1e12 / VALU_PEAK_TOPS, 4),
                     "insts_per_launch": prof["valu_insts_per_launch"],
-                    "salu_insts_per_launch": prof.get("salu_insts_per_launch")}
+                    "salu_insts_per_launch": prof.get("salu_insts_per_launch"),
+                    "source": "committed SQ_INSTS_VALU count per launch (profiles/k_raster_batch64_traffic.json)"}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = {"value": round(frames_per_step / cpu_seconds, 3), "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"all {frames_per_step} batch streams, one frame each, through the C oracle on 1 core"}
+        cpu = {"value": round(frames_per_step / cpu_wall, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"all {frames_per_step} batch streams, one frame each, through the C oracle, one stream "
+                         f"per thread on {threads} host threads",
+               "one_core_value": round(sample_n / sample_s, 3),
+               "one_core_sample": f"the first {sample_n} batch streams in sequence on one thread",
+               "host_cpu": cpu_model(), "host_threads_available": os.cpu_count()}
 
     out = {
         "metric": "bit-exact RDP frames/s, 64-stream batch vs plain path; p99 frame latency",
@@ -315,7 +448,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched form: {frames_per_step} seeded streams (8 MB RDRAM each), "
                                f"{total_tris} triangles per step, 320x240/640x480 at 16/32 bpp, "
-                               "modes drawn from the shaded, textured and 2-cycle CI/LOD/fog/AA workloads",
+                               "modes drawn from the shaded, textured and 2-cycle CI/LOD/fog/AA workloads; "
+                               "shaded streams of more than 512 triangles use 2-200 px triangles rather than "
+                               "full-screen ones",
                    "streams": frames_per_step, "triangles_per_step": total_tris},
         "bit_exact_frames": f"{exact}/{frames_per_step}",
         "mismatched_bytes": mism,
@@ -330,11 +465,14 @@ def main():
         "stage_ms_per_step": {"setup": round(float(stage_ms[0]), 4), "tmem": round(float(stage_ms[1]), 4),
                               "bin": round(float(stage_ms[2]), 4), "raster": round(float(stage_ms[3]), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_raster (the k_raster<class> and k_raster_fill launches of a step)", "bytes_per_launch": int(bytes_per_launch),
-                     "ms_per_launch": round(raster_ms_per_launch, 4), "launches_per_step": raster_launches,
-                     "valu_issue": valu},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": dom["kernel"], "bytes_per_launch": dom["bytes"], "tiles_per_launch": dom["tiles"],
+                     "ms_per_launch": round(dom["ms"], 4), "timing": "the launch alone, HIP events on its stream",
+                     "valu_issue": valu,
+                     "launches": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in l.items()}
+                                  for l in launches]},
         "cpu_baseline": cpu,
+        "configs": configs,
         "serving": serve,
     }
     print(json.dumps(out), flush=True)

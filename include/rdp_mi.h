@@ -218,6 +218,14 @@ int rdp_mi_recording_replay(rdp_mi_recording *r, unsigned iterations, int use_gr
 int rdp_mi_recording_profile(rdp_mi_recording *r, unsigned iterations, float stage_ms[4]);
 /* Per-round variant: round_stage_ms holds rounds * 4 entries. */
 int rdp_mi_recording_profile_rounds(rdp_mi_recording *r, unsigned iterations, float *round_stage_ms);
+/* Every raster launch of the recording timed alone (stages in series, events
+ * around each launch), averaged over `iterations` replays.  For launch k, in
+ * launch order: ms[k], its algorithmic framebuffer bytes[k] (colour, depth
+ * and hidden bits read and written once per tile), kind[k] (0-2: the
+ * k_raster occupancy class, 3: k_raster_fill) and tiles[k].  *count receives
+ * the number of raster launches; at most max entries are written. */
+int rdp_mi_recording_profile_raster(rdp_mi_recording *r, unsigned iterations, unsigned max, float *ms,
+                                    uint64_t *bytes, uint32_t *kind, uint32_t *tiles, unsigned *count);
 /* Launch-shape autotuner (no reference counterpart; the reference picks its
  * shader variants and workgroup layout statically, rdp_renderer.cpp:1790-1822).
  * Times the recorded submission under every candidate raster launch shape:

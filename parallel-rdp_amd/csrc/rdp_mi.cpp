@@ -341,8 +341,14 @@ struct Launch {
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
 	bool tmem_early;  // no upload reads what the previous round's raster writes
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
+	uint64_t class_bytes[kRasterClasses], fill_bytes;  // the same per raster launch
+	uint32_t fill_tiles;
 	uint32_t passes;
 };
+
+// Events around each raster launch of a round when the stages run in series
+// (profiling): begin and end of k_raster_fill, then of each class launch.
+constexpr size_t kRasterEventsPerRound = 2 * (1 + kRasterClasses);
 
 // Whether round r's TMEM replay may start before round r-1's raster: no
 // upload of a pass reads RDRAM that the same processor's pass in round r-1
@@ -414,7 +420,7 @@ static uint64_t framebuffer_bytes(const HostPass &hp)
 constexpr uint32_t kOverlapMinPasses = 8;
 
 static int execute_launches(const std::vector<Launch> &launches, Lane &lane, const LaunchConfig &cfg,
-                            hipEvent_t *stage_events = nullptr)
+                            hipEvent_t *stage_events = nullptr, hipEvent_t *raster_events = nullptr)
 {
 	hipStream_t s = lane.stream, side = lane.side, prep = lane.prep;
 	const bool overlap = !stage_events && side && prep;
@@ -490,12 +496,20 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			HIP_CHECK(hipStreamWaitEvent(s, E.join, 0));
 		} else {
 			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
+			hipEvent_t *re = raster_events ? raster_events + r * kRasterEventsPerRound : nullptr;
+			if (re) HIP_CHECK(hipEventRecord(re[0], s));
 			HIP_CHECK(launch_raster_fill(L.descs, L.fill_items, L.fill_n, cfg.xcd_chunk, s));
+			if (re) HIP_CHECK(hipEventRecord(re[1], s));
 			// Class segments are contiguous in class order, so promoting the
 			// classes below min_class merges them into one launch.
+			if (re) HIP_CHECK(hipEventRecord(re[2 + 2 * lo], s));
 			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
-			for (uint32_t c = lo + 1; c < kRasterClasses; c++)
+			if (re) HIP_CHECK(hipEventRecord(re[3 + 2 * lo], s));
+			for (uint32_t c = lo + 1; c < kRasterClasses; c++) {
+				if (re) HIP_CHECK(hipEventRecord(re[2 + 2 * c], s));
 				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, s));
+				if (re) HIP_CHECK(hipEventRecord(re[3 + 2 * c], s));
+			}
 		}
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
 	}
@@ -645,9 +659,19 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			LA.raster_n[c] = L.n_raster_class[c];
 		}
 		LA.raster_bytes = 0;
+		LA.fill_bytes = 0;
+		LA.fill_tiles = 0;
+		for (uint32_t c = 0; c < kRasterClasses; c++) LA.class_bytes[c] = 0;
 		LA.passes = (uint32_t)rounds[r].size();
 		for (size_t i = 0; i < rounds[r].size(); i++) {
-			LA.raster_bytes += framebuffer_bytes(*rounds[r][i].pass);
+			const uint64_t fb_bytes = framebuffer_bytes(*rounds[r][i].pass);
+			LA.raster_bytes += fb_bytes;
+			if (pl[r][i].fill) {
+				LA.fill_bytes += fb_bytes;
+				if (!rounds[r][i].pass->prims.empty())
+					LA.fill_tiles += ((rounds[r][i].pass->fb_width + 7) / 8) * ((rounds[r][i].pass->fb_height + 7) / 8);
+			} else
+				LA.class_bytes[std::min(rounds[r][i].pass->raster_class, kRasterClasses - 1)] += fb_bytes;
 			const HostPass &hp = *rounds[r][i].pass;
 			Processor &pr = *rounds[r][i].proc;
 			PassInputs &pi = pl[r][i];
@@ -2122,6 +2146,64 @@ int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, fl
 	for (auto &e : ev) (void)hipEventDestroy(e);
 	for (size_t k = 0; k < nev; k++) round_stage_ms[k] = (float)(acc[k] / iterations);
 	return err;
+}
+
+int rdp_mi_recording_profile_raster(rdp_mi_recording *h, unsigned iterations, unsigned max, float *ms,
+                                    uint64_t *bytes, uint32_t *kind, uint32_t *tiles, unsigned *count)
+{
+	if (!h || iterations == 0 || !count) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	Recording &r = h->r;
+	const size_t nr = r.launches.size();
+	std::vector<hipEvent_t> sev(nr * 4), rev(nr * kRasterEventsPerRound);
+	for (auto &e : sev) HIP_CHECK(hipEventCreate(&e));
+	for (auto &e : rev) HIP_CHECK(hipEventCreate(&e));
+	std::vector<double> acc(rev.size() / 2, 0.0);
+	int err = r.copy_state(c.stream, false);
+	for (unsigned it = 0; it < iterations && !err; it++) {
+		// Every event is recorded on each replay (launches with no work
+		// still get their pair), so all pairs are valid.
+		err = execute_launches(r.launches, c, r.cfg, sev.data(), rev.data());
+		if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "profile failed");
+		for (size_t k = 0; k < acc.size() && !err; k++) {
+			float t = 0;
+			if (hipEventElapsedTime(&t, rev[2 * k], rev[2 * k + 1]) == hipSuccess) acc[k] += t;
+		}
+	}
+	if (!err) err = r.copy_state(c.stream, true);
+	if (!err && hipStreamSynchronize(c.stream) != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "profile failed");
+	for (auto &e : sev) (void)hipEventDestroy(e);
+	for (auto &e : rev) (void)hipEventDestroy(e);
+	if (err) return err;
+	// One entry per launch that ran: the fill launch, the merged light
+	// class launch and each heavier class launch, in launch order.
+	const uint32_t lo = std::min(r.cfg.min_class, kRasterClasses - 1);
+	unsigned n = 0;
+	for (size_t rd = 0; rd < nr; rd++) {
+		const Launch &L = r.launches[rd];
+		auto add = [&](size_t slot, uint64_t b, uint32_t k, uint32_t t) {
+			if (t == 0) return;
+			if (n < max) {
+				if (ms) ms[n] = (float)(acc[rd * (kRasterEventsPerRound / 2) + slot] / iterations);
+				if (bytes) bytes[n] = b;
+				if (kind) kind[n] = k;
+				if (tiles) tiles[n] = t;
+			}
+			n++;
+		};
+		add(0, L.fill_bytes, kRasterClasses, L.fill_tiles);
+		uint64_t mb = 0;
+		uint32_t mt = 0;
+		for (uint32_t cl = 0; cl <= lo; cl++) {
+			mb += L.class_bytes[cl];
+			mt += L.raster_n[cl];
+		}
+		add(1 + lo, mb, lo, mt);
+		for (uint32_t cl = lo + 1; cl < kRasterClasses; cl++) add(1 + cl, L.class_bytes[cl], cl, L.raster_n[cl]);
+	}
+	*count = n;
+	return 0;
 }
 
 int rdp_mi_recording_profile(rdp_mi_recording *h, unsigned iterations, float stage_ms[4])

@@ -467,6 +467,25 @@ class Recording:
         _check(self._lib.rdp_mi_recording_profile(self._h, iterations, ms.ctypes.data), "profile")
         return ms
 
+    def profile_raster(self, iterations: int = 3) -> list:
+        """Each raster launch timed alone: dicts of kind ('k_raster<c>' or
+        'k_raster_fill'), ms, algorithmic bytes and tiles, in launch order."""
+        lib = self._lib
+        lib.rdp_mi_recording_profile_raster.restype = ctypes.c_int
+        lib.rdp_mi_recording_profile_raster.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint] + \
+            [ctypes.c_void_p] * 4 + [ctypes.POINTER(ctypes.c_uint)]
+        cap = 64
+        ms = np.zeros(cap, np.float32)
+        by = np.zeros(cap, np.uint64)
+        kind = np.zeros(cap, np.uint32)
+        tiles = np.zeros(cap, np.uint32)
+        n = ctypes.c_uint()
+        _check(lib.rdp_mi_recording_profile_raster(self._h, iterations, cap, ms.ctypes.data, by.ctypes.data,
+                                                   kind.ctypes.data, tiles.ctypes.data, ctypes.byref(n)),
+               "profile_raster")
+        return [{"kernel": "k_raster_fill" if int(kind[i]) == 3 else f"k_raster<{int(kind[i])}>",
+                 "ms": float(ms[i]), "bytes": int(by[i]), "tiles": int(tiles[i])} for i in range(min(n.value, cap))]
+
     def profile_rounds(self, iterations: int = 1) -> np.ndarray:
         self._lib.rdp_mi_recording_profile_rounds.restype = ctypes.c_int
         self._lib.rdp_mi_recording_profile_rounds.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]

@@ -2,6 +2,14 @@
 
     python parallel-rdp_amd/rocprof_summary.py stats  <kernel-trace .db> [kernel-substring]
     python parallel-rdp_amd/rocprof_summary.py pmc    <pmc .db> [kernel-substring]
+    python parallel-rdp_amd/rocprof_summary.py traffic <fetch .db> <write .db> <copy-fetch .db> <copy-write .db>
+                                                       <sq .db> <copy bytes> <out .json> [kernel-substring]
+
+`traffic` writes, per kernel matching the substring (default k_raster), the
+HBM bytes per dispatch (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by the
+factor that makes the same counter read the known byte count of a plain copy
+kernel: the copy runs' largest kernel) and the SQ instruction counts per
+dispatch, as JSON keyed by the kernel's short name (k_raster<0>, ...).
 
 `stats` prints, per kernel, the call count and total/average duration, then
 for kernels matching the substring a breakdown by launch grid size and the
@@ -86,7 +94,52 @@ def pmc(path: str, kernel_filter: str = ""):
         print(f"{short[:48]:48s} {counter:14s} {n:10d} {total:16.1f} {total / n:16.1f}")
 
 
+def _per_dispatch(path: str, counter: str, kernel_filter: str = ""):
+    """{short kernel name: average of `counter` per dispatch}."""
+    db = sqlite3.connect(path)
+    per = collections.defaultdict(float)
+    for disp, kname, cname, value in db.execute(
+            "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+        if cname == counter and (not kernel_filter or kernel_filter in kname):
+            per[(short_name(kname), disp)] += value
+    agg = collections.defaultdict(list)
+    for (k, _), v in per.items():
+        agg[k].append(v)
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def short_name(kname: str) -> str:
+    """'void rdp_mi::k_raster<0u>(...)' -> 'k_raster<0>'."""
+    n = kname.split("(")[0].replace("void ", "").replace("rdp_mi::", "")
+    return n.replace("u>", ">")
+
+
+def traffic(fetch_db, write_db, copyf_db, copyw_db, sq_db, copy_bytes, out, kernel_filter="k_raster"):
+    import json
+    fetch = _per_dispatch(fetch_db, "FETCH_SIZE", kernel_filter)
+    write = _per_dispatch(write_db, "WRITE_SIZE", kernel_filter)
+    cf = max(_per_dispatch(copyf_db, "FETCH_SIZE").values())
+    cw = max(_per_dispatch(copyw_db, "WRITE_SIZE").values())
+    ff, wf = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    res = {}
+    for k in sorted(fetch):
+        res[k] = {"fetch_kib_per_dispatch": round(fetch[k], 1), "fetch_factor": round(ff, 3),
+                  "write_kib_per_dispatch": round(write.get(k, 0.0), 1), "write_factor": round(wf, 3),
+                  "traffic_bytes_per_launch": int(fetch[k] * 1024 * ff + write.get(k, 0.0) * 1024 * wf)}
+    for counter, key in (("SQ_INSTS_VALU", "valu_insts_per_launch"), ("SQ_INSTS_SALU", "salu_insts_per_launch"),
+                         ("SQ_INSTS_LDS", "lds_insts_per_launch"), ("SQ_LDS_BANK_CONFLICT", "lds_bank_conflict_cycles")):
+        for k, v in _per_dispatch(sq_db, counter, kernel_filter).items():
+            res.setdefault(k, {})[key] = int(v)
+    with open(out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res, indent=1))
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "traffic":
+        a = sys.argv[2:]
+        traffic(a[0], a[1], a[2], a[3], a[4], float(a[5]), a[6], *(a[7:8]))
+        sys.exit(0)
     if len(sys.argv) < 3:
         print(__doc__)
         sys.exit(1)
