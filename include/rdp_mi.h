@@ -58,8 +58,25 @@ enum {
 /* CommandProcessor::CommandProcessor(device, rdram_ptr, rdram_offset,
  * rdram_size, hidden_rdram_size, flags), rdp_device.cpp:44-162.
  * rdram_size must be a power of two; hidden_rdram_size must be
- * rdram_size / 2.  RDRAM starts from *rdram_ptr (or zero), hidden RDRAM at
- * 0x03 per byte, TMEM at zero. */
+ * rdram_size / 2.  Hidden RDRAM starts at 0x03 per byte, TMEM at zero.
+ *
+ * Without rdram_ptr, RDRAM lives on the device (zeroed) and the host sees it
+ * through rdp_mi_begin_read_rdram (a copy, after idle or a wait) and
+ * rdp_mi_end_write_rdram (uploads the host copy).
+ *
+ * With rdram_ptr, the caller's memory at rdram_ptr + rdram_offset is RDRAM,
+ * as with the reference's host-memory RDRAM when the device cannot import it
+ * (rdp_device.cpp:75-112, rdp_renderer.cpp:2523-2907): the device keeps a
+ * copy of it; each submission first copies in the 4 KiB pages its render
+ * passes read (colour and depth image rows, TMEM upload sources), and a
+ * scanout the pages the VI fetches (video_interface.cpp:369-396); the raster
+ * marks every byte it writes, and once the work is complete (a wait,
+ * idle) only those bytes are written into the caller's memory
+ * (masked_rdram_resolve.comp:47-70), so caller writes elsewhere in the same
+ * pages are kept.  The caller may write RDRAM at any time between
+ * submissions without rdp_mi_end_write_rdram (a no-op in this mode);
+ * rdp_mi_begin_read_rdram returns rdram_ptr + rdram_offset.  Recordings and
+ * servers need device-resident RDRAM and reject such processors. */
 int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, size_t rdram_size,
                   size_t hidden_rdram_size, uint32_t flags);
 

@@ -111,6 +111,24 @@ struct Mem {
 	MDEV void set_hid(uint32_t index, uint32_t v) const { hidden[index & (mask8 >> 1)] = (uint8_t)v; }
 };
 
+// Write mask of a host-coherent stream: the bytes store_color / the depth
+// store write, in the same layout as RDRAM, set to 0xff.
+DEV void mark_color(AS_GLOBAL uint8_t *wm, uint32_t mask8, uint32_t fmt, uint32_t index)
+{
+	switch (fmt) {
+	case FB_I4:
+	case FB_I8: wm[((index & mask8) ^ 3)] = 0xff; break;
+	case FB_RGBA5551:
+	case FB_IA88: reinterpret_cast<AS_GLOBAL uint16_t *>(wm)[(index & (mask8 >> 1)) ^ 1] = 0xffff; break;
+	default: reinterpret_cast<AS_GLOBAL uint32_t *>(wm)[index & (mask8 >> 2)] = 0xffffffffu; break;
+	}
+}
+
+DEV void mark_depth(AS_GLOBAL uint8_t *wm, uint32_t mask8, uint32_t index)
+{
+	reinterpret_cast<AS_GLOBAL uint16_t *>(wm)[(index & (mask8 >> 1)) ^ 1] = 0xffff;
+}
+
 // ==========================================================================
 // Stage 1: primitive decode + span setup.
 // ==========================================================================
@@ -2217,11 +2235,16 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 	const AS_CONST PassDesc *pe = cptr(&P);
 	asm volatile("" : "+s"(pe));
 	const Mem me{ gptr(pe->rdram), gptr(pe->hidden), pe->rdram_mask };
-	if (in_fb && px.color_dirty) store_color(me, fmt, ci, px);
+	AS_GLOBAL uint8_t *wm = gptr(pe->wmask);
+	if (in_fb && px.color_dirty) {
+		store_color(me, fmt, ci, px);
+		if (wm) mark_color(wm, pe->rdram_mask, fmt, ci);
+	}
 	if (in_fb && !alias && px.depth_dirty) {
 		const uint32_t de = (pe->depth_addr >> 1) + pe->fb_width * (uint32_t)y + (uint32_t)x;
 		me.wr16i(de, (px.depth << 2) | (px.dz >> 2));
 		me.set_hid(de, px.dz & 3);
+		if (wm) mark_depth(wm, pe->rdram_mask, de);
 	}
 }
 
@@ -2297,7 +2320,10 @@ void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restri
 				}
 			}
 		}
-		if (in_fb && px.color_dirty) store_color(m, fmt, ci, px);
+		if (in_fb && px.color_dirty) {
+			store_color(m, fmt, ci, px);
+			if (P.wmask) mark_color(gptr(P.wmask), P.rdram_mask, fmt, ci);
+		}
 	}
 }
 

@@ -284,6 +284,23 @@ struct Processor {
 	bool dump_in_list = false;
 	std::mutex dump_lock;
 
+	// Host-coherent RDRAM (created with the caller's RDRAM pointer; the
+	// reference's incoherent host-memory path, rdp_renderer.cpp:2523-2907,
+	// masked_rdram_resolve.comp:47-70): the caller's memory is RDRAM and the
+	// device copy a cache.  Before a submission the pages its passes read
+	// (framebuffer rows, TMEM upload sources) are copied in from the caller;
+	// the raster marks every byte it writes in d_wmask; after the submission
+	// the written pages and their mask are copied out and, once the work is
+	// complete, only the marked bytes are merged into the caller's memory, so
+	// bytes the caller wrote meanwhile elsewhere in those pages survive.
+	bool coherent = false;
+	uint8_t *d_wmask = nullptr;
+	PinnedBuffer wb_data, wb_mask;
+	std::vector<ByteRange> wb_runs;  // page runs of the write-back in flight
+	hipEvent_t wb_event = nullptr;
+	bool wb_pending = false;
+	std::mutex wb_lock;
+
 	explicit Processor(uint32_t size) : fe(size), rdram_size(size) { memset(tmem_mirror, 0, sizeof(tmem_mirror)); }
 	uint8_t *host_rdram() { return user_rdram ? user_rdram : rdram_mirror.data(); }
 };
@@ -703,6 +720,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.rdram = pr.d_rdram;
 			D.hidden = pr.d_hidden;
 			D.tmem = pr.d_tmem;
+			D.wmask = pr.coherent ? pr.d_wmask : nullptr;
 			D.rdram_mask = pr.rdram_size - 1;
 			D.num_prims = (uint32_t)hp.prims.size();
 			D.num_uploads = (uint32_t)hp.uploads.size();
@@ -791,6 +809,99 @@ static void build_round_list(Processor *const *procs, unsigned count, bool batch
 	}
 }
 
+// ---- host-coherent RDRAM ----
+constexpr uint32_t kCoherentPage = 4096;
+
+// Page-aligned runs [lo, hi) inside [0, size) covering the byte ranges
+// (which may wrap at the RDRAM size).
+static void page_runs(const std::vector<ByteRange> &ranges, uint32_t size, std::vector<ByteRange> &runs)
+{
+	runs.clear();
+	const uint32_t pages = size / kCoherentPage;
+	std::vector<uint8_t> mark(pages, 0);
+	for (const ByteRange &r : ranges) {
+		if (r.hi <= r.lo) continue;
+		if (r.hi - r.lo >= size) {
+			std::fill(mark.begin(), mark.end(), 1);
+			break;
+		}
+		const uint64_t first = r.lo / kCoherentPage, last = (r.hi - 1) / kCoherentPage;
+		for (uint64_t pg = first; pg <= last; pg++) mark[pg % pages] = 1;
+	}
+	for (uint32_t pg = 0; pg < pages;) {
+		if (!mark[pg]) {
+			pg++;
+			continue;
+		}
+		uint32_t e = pg;
+		while (e < pages && mark[e]) e++;
+		runs.push_back({ (uint64_t)pg * kCoherentPage, (uint64_t)e * kCoherentPage });
+		pg = e;
+	}
+}
+
+// Merge the write-back in flight into the caller's memory: bytes the raster
+// marked take the device value.
+static int coherent_complete(Processor &p)
+{
+	std::lock_guard<std::mutex> g(p.wb_lock);
+	if (!p.wb_pending) return 0;
+	HIP_CHECK(hipEventSynchronize(p.wb_event));
+	uint8_t *host = p.user_rdram;
+	for (const ByteRange &r : p.wb_runs) {
+		const uint64_t *d = reinterpret_cast<const uint64_t *>(p.wb_data.ptr + r.lo);
+		const uint64_t *m = reinterpret_cast<const uint64_t *>(p.wb_mask.ptr + r.lo);
+		uint64_t *h = reinterpret_cast<uint64_t *>(host + r.lo);
+		for (uint64_t i = 0; i < (r.hi - r.lo) / 8; i++) {
+			uint64_t hv;
+			memcpy(&hv, h + i, 8);  // the caller's pointer may be unaligned
+			hv = (hv & ~m[i]) | (d[i] & m[i]);
+			memcpy(h + i, &hv, 8);
+		}
+	}
+	p.wb_pending = false;
+	return 0;
+}
+
+// Copy in the pages the passes read, before their launches.
+static int coherent_pull(Processor &p, const std::vector<HostPass> &passes, hipStream_t st)
+{
+	if (int err = coherent_complete(p)) return err;
+	std::vector<ByteRange> ranges, runs;
+	for (const HostPass &hp : passes) {
+		pass_write_ranges(hp, ranges);
+		for (const UploadInfo &u : hp.uploads) {
+			ByteRange b;
+			if (!upload_read_range(u, b)) b = { 0, p.rdram_size };
+			ranges.push_back(b);
+		}
+	}
+	page_runs(ranges, p.rdram_size, runs);
+	for (const ByteRange &r : runs)
+		HIP_CHECK(hipMemcpyAsync(p.d_rdram + r.lo, p.user_rdram + r.lo, r.hi - r.lo, hipMemcpyHostToDevice, st));
+	return 0;
+}
+
+// After the launches: copy out the written pages and their mask, clear the
+// mask, and leave the merge to coherent_complete.
+static int coherent_push(Processor &p, const std::vector<HostPass> &passes, hipStream_t st)
+{
+	std::vector<ByteRange> ranges;
+	for (const HostPass &hp : passes) pass_write_ranges(hp, ranges);
+	std::lock_guard<std::mutex> g(p.wb_lock);
+	page_runs(ranges, p.rdram_size, p.wb_runs);
+	if (p.wb_runs.empty()) return 0;
+	for (const ByteRange &r : p.wb_runs) {
+		const size_t n = r.hi - r.lo;
+		HIP_CHECK(hipMemcpyAsync(p.wb_data.ptr + r.lo, p.d_rdram + r.lo, n, hipMemcpyDeviceToHost, st));
+		HIP_CHECK(hipMemcpyAsync(p.wb_mask.ptr + r.lo, p.d_wmask + r.lo, n, hipMemcpyDeviceToHost, st));
+		HIP_CHECK(hipMemsetAsync(p.d_wmask + r.lo, 0, n, st));
+	}
+	HIP_CHECK(hipEventRecord(p.wb_event, st));
+	p.wb_pending = true;
+	return 0;
+}
+
 // Submit every completed pass of the given processors (plain path: one
 // processor, one pass per round) on a lane (default: the context's).
 static int submit_processors(Processor *const *procs, unsigned count, bool batched, Lane *lane = nullptr)
@@ -809,7 +920,10 @@ static int submit_processors(Processor *const *procs, unsigned count, bool batch
 		}
 	}
 	int ret = 0;
-	if (batched) {
+	for (unsigned i = 0; i < count && !ret; i++)
+		if (procs[i]->coherent) ret = coherent_pull(*procs[i], procs[i]->fe.completed(), c.stream);
+	if (ret) {
+	} else if (batched) {
 		// Keep each launch's pass count within the work-item encoding.
 		ret = submit_rounds(c, rounds);
 	} else {
@@ -818,6 +932,8 @@ static int submit_processors(Processor *const *procs, unsigned count, bool batch
 			if ((ret = submit_rounds(c, one))) break;
 		}
 	}
+	for (unsigned i = 0; i < count && !ret; i++)
+		if (procs[i]->coherent) ret = coherent_push(*procs[i], procs[i]->fe.completed(), c.stream);
 	for (unsigned i = 0; i < count; i++) procs[i]->fe.completed().clear();
 	return ret;
 }
@@ -1552,6 +1668,10 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 		if (p.d_rdram) (void)hipFree(p.d_rdram);
 		if (p.d_hidden) (void)hipFree(p.d_hidden);
 		if (p.d_tmem) (void)hipFree(p.d_tmem);
+		if (p.d_wmask) (void)hipFree(p.d_wmask);
+		if (p.wb_data.ptr) (void)hipHostFree(p.wb_data.ptr);
+		if (p.wb_mask.ptr) (void)hipHostFree(p.wb_mask.ptr);
+		if (p.wb_event) (void)hipEventDestroy(p.wb_event);
 		delete h;
 		return set_error(code, "%s", msg);
 	};
@@ -1561,6 +1681,15 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 	if (hipMalloc(&p.d_tmem, 4096) != hipSuccess) return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "TMEM allocation failed");
 	if (rdram_ptr) {
 		p.user_rdram = static_cast<uint8_t *>(rdram_ptr) + rdram_offset;
+		p.coherent = true;
+		if (hipMalloc(&p.d_wmask, rdram_size) != hipSuccess)
+			return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "RDRAM write mask allocation failed");
+		if (hipMemsetAsync(p.d_wmask, 0, rdram_size, c.stream) != hipSuccess)
+			return fail(RDP_MI_ERROR_DEVICE, "write mask clear failed");
+		if (p.wb_data.reserve(rdram_size) || p.wb_mask.reserve(rdram_size))
+			return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "write-back buffers failed");
+		if (hipEventCreateWithFlags(&p.wb_event, hipEventDisableTiming) != hipSuccess)
+			return fail(RDP_MI_ERROR_DEVICE, "write-back event creation failed");
 		if (hipMemcpyAsync(p.d_rdram, p.user_rdram, rdram_size, hipMemcpyHostToDevice, c.stream) != hipSuccess)
 			return fail(RDP_MI_ERROR_DEVICE, "RDRAM upload failed");
 	} else {
@@ -1595,9 +1724,16 @@ void rdp_mi_destroy(rdp_mi_processor *h)
 	if (p.dump) p.dump->close();
 	stop_ring(p);
 	p.vi.release();
+	(void)coherent_complete(p);
 	(void)hipFree(p.d_rdram);
 	(void)hipFree(p.d_hidden);
 	(void)hipFree(p.d_tmem);
+	if (p.coherent) {
+		(void)hipFree(p.d_wmask);
+		(void)hipHostFree(p.wb_data.ptr);
+		(void)hipHostFree(p.wb_mask.ptr);
+		(void)hipEventDestroy(p.wb_event);
+	}
 	delete h;
 }
 
@@ -1722,6 +1858,9 @@ int rdp_mi_wait_for_timeline_timeout(rdp_mi_processor *h, uint64_t value, uint64
 		return set_error(RDP_MI_ERROR_TIMEOUT, "timeline value %llu not reached within %llu us",
 		                 (unsigned long long)value, (unsigned long long)timeout_us);
 	if (p.timeline_error) return set_error(p.timeline_error, "device error in the processor's work");
+	g.unlock();
+	// The caller's RDRAM gets the bytes the finished work wrote.
+	if (p.coherent) return coherent_complete(p);
 	return 0;
 }
 
@@ -1756,6 +1895,8 @@ void *rdp_mi_begin_read_rdram(rdp_mi_processor *h)
 {
 	if (!h) return nullptr;
 	Processor &p = h->p;
+	// Host-coherent: the caller's memory is RDRAM (complete after a wait).
+	if (p.coherent) return coherent_complete(p) ? nullptr : p.user_rdram;
 	if (copy_sync(p.host_rdram(), p.d_rdram, p.rdram_size, hipMemcpyDeviceToHost)) return nullptr;
 	return p.host_rdram();
 }
@@ -1763,8 +1904,10 @@ void *rdp_mi_begin_read_rdram(rdp_mi_processor *h)
 void rdp_mi_end_write_rdram(rdp_mi_processor *h)
 {
 	if (!h) return;
-	rdp_mi_idle(h);
 	Processor &p = h->p;
+	// Host-coherent: pages are read from the caller's memory when used.
+	if (p.coherent) return;
+	rdp_mi_idle(h);
 	(void)copy_sync(p.d_rdram, p.host_rdram(), p.rdram_size, hipMemcpyHostToDevice);
 }
 
@@ -1839,6 +1982,17 @@ static int scanout_queue(Processor &p, const rdp_mi_scanout_options &opts, uint3
 	if (int err = flush_now(p)) return err;
 	Context &c = ctx();
 	std::lock_guard<std::mutex> lg(c.lock);
+	if (p.coherent) {
+		// The caller may have written the frame: copy the pages the VI
+		// fetches in first (resolve_coherency_external, rdp_device.cpp:1101-1106).
+		if (int err = coherent_complete(p)) return err;
+		uint32_t off = 0, len = 0;
+		vi_memory_range(p.vi, p.vi_regs, off, len);
+		std::vector<ByteRange> runs;
+		page_runs({ { off, (uint64_t)off + len } }, p.rdram_size, runs);
+		for (const ByteRange &r : runs)
+			HIP_CHECK(hipMemcpyAsync(p.d_rdram + r.lo, p.user_rdram + r.lo, r.hi - r.lo, hipMemcpyHostToDevice, c.stream));
+	}
 	int err = vi_scanout(p.vi, p.vi_regs, p.d_rdram, p.d_hidden, p.rdram_size, c.d_gamma, c.stream, opts, out, width,
 	                     height, done == nullptr);
 	if (err) return set_error(err, "VI scanout failed");
@@ -1996,6 +2150,9 @@ int rdp_mi_record(rdp_mi_processor *const *procs, unsigned count, int batched, r
 	ProcessorSet set;
 	if (int err = set.init(procs, count)) return err;
 	std::vector<Processor *> &ps = set.ps;
+	for (Processor *p : ps)
+		if (p->coherent)
+			return set_error(RDP_MI_ERROR_UNSUPPORTED, "recordings need device-resident RDRAM (no caller RDRAM pointer)");
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	rdp_mi_recording *h = new (std::nothrow) rdp_mi_recording();
@@ -2268,6 +2425,10 @@ int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t 
 		if (!procs[i]) {
 			delete h;
 			return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null processor %u", i);
+		}
+		if (procs[i]->p.coherent) {
+			delete h;
+			return set_error(RDP_MI_ERROR_UNSUPPORTED, "serving needs device-resident RDRAM (processor %u has a caller RDRAM pointer)", i);
 		}
 		s.procs.push_back(&procs[i]->p);
 	}

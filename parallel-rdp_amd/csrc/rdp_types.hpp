@@ -408,6 +408,7 @@ struct PassDesc {
 	uint8_t *rdram;
 	uint8_t *hidden;
 	uint16_t *tmem;            // persistent TMEM of the stream (2048 halfwords)
+	uint8_t *wmask;            // host-coherent streams: 0xff over every RDRAM byte the raster writes (else null)
 	uint32_t rdram_mask;       // rdram_size - 1
 	uint32_t num_prims;
 	uint32_t num_uploads;

@@ -14,6 +14,8 @@
 // so both kernels are bound by launch and L2 latency, not HBM.
 #include "rdp_vi.hpp"
 
+#include <vector>
+
 #include <math.h>
 #include <string.h>
 
@@ -482,6 +484,25 @@ __global__ __launch_bounds__(256) void k_vi_scale(const ViParams *__restrict__ p
 
 // VideoInterface::scanout, video_interface.cpp:1215-1426 (scaling factor 1,
 // no downscale or deinterlace pass).
+// VideoInterface::scanout_memory_range (video_interface.cpp:369-396): the
+// RDRAM bytes a scanout fetches, a 2-pixel border on each side for the AA
+// and divot filters included; length 0 for a frame that fetches nothing.
+void vi_memory_range(const ViState &s, const uint32_t *regs, uint32_t &offset, uint32_t &length)
+{
+	Decoded r;
+	std::vector<ViLine> lines(kViMaxLines);
+	decode(s, regs, r, lines.data());
+	offset = length = 0;
+	if (r.vi_offset == 0 || r.h_res <= 0 || r.h_start >= kScanoutWidth) return;
+	const int divot = (r.status & VI_DIVOT) ? 1 : 0;
+	const int aa_width = r.max_x + 2 + 4 + divot * 2, aa_height = r.max_y + 1 + 4;
+	const int x_off = divot ? -3 : -2, y_off = -2;
+	const int pixel = ((r.status & VI_TYPE_MASK) | VI_TYPE_RGBA5551) == VI_TYPE_RGBA8888 ? 4 : 2;
+	const int64_t off = (int64_t)(r.vi_offset & ~(pixel - 1)) + (int64_t)(x_off + y_off * r.vi_width) * pixel;
+	offset = (uint32_t)off;  // wraps like the fetch, modulo the RDRAM size
+	length = (uint32_t)((int64_t)(aa_height * r.vi_width + aa_width) * pixel);
+}
+
 int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const uint8_t *d_hidden, uint32_t rdram_size,
                const uint8_t *d_gamma, hipStream_t stream, const rdp_mi_scanout_options &opts, uint32_t *out,
                unsigned *width, unsigned *height, bool sync)

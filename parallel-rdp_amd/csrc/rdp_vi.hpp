@@ -69,6 +69,9 @@ struct ViState {
 // behind the rendering.  With sync the call waits for the copy to `out`;
 // otherwise the copy is still in flight on `stream` when it returns.
 // Returns 0 or an RDP_MI_ERROR_* code.
+// RDRAM bytes [offset, offset + length) (modulo the RDRAM size) a scanout
+// with these registers fetches.
+void vi_memory_range(const ViState &vi, const uint32_t *regs, uint32_t &offset, uint32_t &length);
 int vi_scanout(ViState &vi, const uint32_t *regs, const uint8_t *d_rdram, const uint8_t *d_hidden, uint32_t rdram_size,
                const uint8_t *d_gamma, hipStream_t stream, const rdp_mi_scanout_options &opts, uint32_t *out,
                unsigned *width, unsigned *height, bool sync);

@@ -185,11 +185,20 @@ def _check(ret: int, what: str):
 class CommandProcessor:
     """One emulated RDP with device-resident RDRAM, hidden RDRAM and TMEM."""
 
-    def __init__(self, rdram_size: int = 8 << 20, flags: int = 0):
+    def __init__(self, rdram_size: int = 8 << 20, flags: int = 0, host_rdram: Optional[np.ndarray] = None):
+        """host_rdram: a writable uint32 array of rdram_size bytes that becomes
+        the processor's RDRAM (host-coherent mode); it must outlive the
+        processor."""
         self._lib = load_library()
         h = ctypes.c_void_p()
-        _check(self._lib.rdp_mi_create(ctypes.byref(h), None, 0, rdram_size, rdram_size // 2, flags), "rdp_mi_create")
+        ptr = None
+        if host_rdram is not None:
+            if host_rdram.nbytes != rdram_size or not host_rdram.flags.c_contiguous or not host_rdram.flags.writeable:
+                raise ValueError("host_rdram must be a writable contiguous array of rdram_size bytes")
+            ptr = host_rdram.ctypes.data
+        _check(self._lib.rdp_mi_create(ctypes.byref(h), ptr, 0, rdram_size, rdram_size // 2, flags), "rdp_mi_create")
         self._h = h
+        self.host_rdram = host_rdram
         self.rdram_size = rdram_size
         self._validation_cb = None
         self.validation_errors = []
