@@ -338,6 +338,23 @@ int rdp_mi_state_digest(rdp_mi_processor *p, uint64_t *digest);
 void rdp_mi_server_stats(rdp_mi_server *s, uint64_t out[7]);
 void rdp_mi_server_destroy(rdp_mi_server *s);
 
+/* Framebuffer views for capture inspection (the colour / depth / coverage
+ * views of the reference's capture replayer, rdp_replayer.cpp:246-393):
+ * width x height RGBA8 pixels (R in the low byte) decoded from the
+ * processor's RDRAM and hidden RDRAM after the work enqueued so far.
+ *  - RDP_MI_VIEW_COLOR: the image at fb_addr; fb_size is SetColorImage's
+ *    size field (2: RGBA5551 with 5-bit channels replicated to 8 and alpha
+ *    from bit 0; 3: RGBA8888; 1: 8-bit as grey, which the reference omits).
+ *  - RDP_MI_VIEW_DEPTH: the 16-bit words at depth_addr with the two low bits
+ *    cleared, shown as RGBA5551, and the 4-bit dz (two hidden bits below the
+ *    word's two low bits) in alpha.
+ *  - RDP_MI_VIEW_COVERAGE: for 16-bit colour images, the 3-bit coverage
+ *    (bit 0 over the two hidden bits) as grey; black otherwise.
+ * height 0 takes width * 3 / 4, as the reference assumes 4:3. */
+enum { RDP_MI_VIEW_COLOR = 0, RDP_MI_VIEW_DEPTH = 1, RDP_MI_VIEW_COVERAGE = 2 };
+int rdp_mi_framebuffer_view(rdp_mi_processor *p, uint32_t mode, uint32_t fb_addr, uint32_t fb_size,
+                            uint32_t depth_addr, uint32_t width, uint32_t height, uint32_t *out);
+
 /* ---- RDPDUMP2 captures ----
  * The reference's capture format (writer rdp_dump_write.hpp:31-57 /
  * rdp_dump_write.cpp:37-145, reader rdp_dump.hpp:49-68 / rdp_dump.cpp:43-200),

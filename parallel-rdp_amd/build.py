@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librdp_mi.so")
 BUILD_DIR = os.path.join(HERE, "build")
 
-SOURCES = ["rdp_frontend.cpp", "rdp_dump.cpp", "rdp_mi.cpp", "rdp_kernels.hip", "rdp_vi.hip", "rdp_digest.hip"]
+SOURCES = ["rdp_frontend.cpp", "rdp_dump.cpp", "rdp_mi.cpp", "rdp_kernels.hip", "rdp_vi.hip", "rdp_digest.hip", "rdp_view.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -35,6 +35,7 @@
 #include "rdp_hazard.hpp"
 #include "rdp_types.hpp"
 #include "rdp_vi.hpp"
+#include "rdp_view.hpp"
 
 namespace rdp_mi {
 
@@ -203,6 +204,7 @@ struct Context : Lane {
 	LaunchConfig cfg;  // for immediate submissions (flush, batch flush, serving)
 	std::mutex lock;
 	uint8_t *d_digest = nullptr;                // rdp_mi_state_digest: one target and its result
+	DeviceBuffer view;                          // rdp_mi_framebuffer_view output
 
 	int init()
 	{
@@ -2677,6 +2679,33 @@ void rdp_mi_recording_destroy(rdp_mi_recording *h)
 	if (h->r.scratch.ptr) (void)hipFree(h->r.scratch.ptr);
 	if (h->r.saved.ptr) (void)hipFree(h->r.saved.ptr);
 	delete h;
+}
+
+int rdp_mi_framebuffer_view(rdp_mi_processor *h, uint32_t mode, uint32_t fb_addr, uint32_t fb_size,
+                            uint32_t depth_addr, uint32_t width, uint32_t height, uint32_t *out)
+{
+	if (!h || !out || mode > RDP_MI_VIEW_COVERAGE || width == 0 || width > 4096 || height > 4096)
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
+	if (height == 0) height = std::max(1u, width * 3 / 4);
+	Processor &p = h->p;
+	ring_drain(p);
+	std::lock_guard<std::mutex> g(p.lock);
+	if (int err = flush_now(p)) return err;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> lg(c.lock);
+	const size_t bytes = (size_t)width * height * 4;
+	DeviceBuffer &buf = c.view;
+	if (buf.reserve(bytes, c.stream)) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "view buffer");
+	if (p.coherent) {
+		// The views read the device copy: bring in the caller's pages.
+		if (int err = coherent_complete(p)) return err;
+		HIP_CHECK(hipMemcpyAsync(p.d_rdram, p.user_rdram, p.rdram_size, hipMemcpyHostToDevice, c.stream));
+	}
+	ViewArgs a{ p.d_rdram, p.d_hidden, p.rdram_size, mode, fb_addr, fb_size, depth_addr, width, height };
+	HIP_CHECK(launch_view(a, reinterpret_cast<uint32_t *>(buf.ptr), c.stream));
+	HIP_CHECK(hipMemcpyAsync(out, buf.ptr, bytes, hipMemcpyDeviceToHost, c.stream));
+	HIP_CHECK(hipStreamSynchronize(c.stream));
+	return 0;
 }
 
 // ---- RDPDUMP2 capture and replay ----

@@ -169,6 +169,8 @@ def load_library():
     lib.rdp_mi_dump_player_apply.restype = ctypes.c_int
     lib.rdp_mi_dump_player_apply.argtypes = [vp, ctypes.POINTER(DumpEvent), vp]
     lib.rdp_mi_dump_player_destroy.argtypes = [vp]
+    lib.rdp_mi_framebuffer_view.restype = ctypes.c_int
+    lib.rdp_mi_framebuffer_view.argtypes = [vp, u32, u32, u32, u32, u32, u32, vp]
     _lib = lib
     return lib
 
@@ -361,6 +363,18 @@ class CommandProcessor:
     @property
     def primitive_count(self) -> int:
         return int(self._lib.rdp_mi_primitive_count(self._h))
+
+    VIEW_COLOR, VIEW_DEPTH, VIEW_COVERAGE = 0, 1, 2
+
+    def framebuffer_view(self, mode: int, fb_addr: int, fb_size: int, depth_addr: int, width: int,
+                         height: int = 0) -> np.ndarray:
+        """(height, width) RGBA8 words (R in the low byte) of the colour,
+        depth or coverage view of a framebuffer (rdp_mi_framebuffer_view)."""
+        h = height or max(1, width * 3 // 4)
+        out = np.zeros(width * h, np.uint32)
+        _check(self._lib.rdp_mi_framebuffer_view(self._h, mode, fb_addr, fb_size, depth_addr, width, h,
+                                                 out.ctypes.data), "framebuffer_view")
+        return out.reshape(h, width)
 
     def begin_dump(self, path: str):
         """Capture this processor's commands, VI registers and scanouts to an
