@@ -52,6 +52,9 @@ def setup_processors(rdp_mi, streams):
     procs = []
     for s in streams:
         p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+        # Every enqueued pass stays pending for the recording (no idle-time
+        # submission behind the recording's back).
+        p.hold_submissions()
         p.write_rdram(s.rdram_init if s.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
         p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
         procs.append(p)
@@ -251,6 +254,8 @@ def config_rows(rdp_mi, pending, args):
         p = setup_processors(rdp_mi, [stream])[0]
         p.enqueue_commands(stream.all_words())
         rec = rdp_mi.Recording([p], batched=True)
+        if rec.passes < frames:
+            raise RuntimeError(f"{name}: the recording holds {rec.passes} passes for {frames} frames")
         rec.replay(1, use_graph=True)
         reps = max(2, min(args.steps, 10))
         ms = float(rec.replay(reps, use_graph=True).mean())
@@ -274,7 +279,8 @@ def config_rows(rdp_mi, pending, args):
         frame_ms = ms / frames + vi_ms
         rows[name] = {"frames": frames, "triangles": stream.num_triangles,
                       "framebuffer": f"{stream.width}x{stream.height}",
-                      "gpu_frames_per_s": round(exact / (frame_ms / 1e3), 1), "ms_per_frame": round(frame_ms, 4),
+                      "gpu_frames_per_s": round((exact / frames) / (frame_ms / 1e3), 1),
+                      "ms_per_frame": round(frame_ms, 4),
                       "render_ms_per_frame": round(ms / frames, 4), "vi_ms_per_frame": round(vi_ms, 4) or None,
                       "bit_exact_frames": f"{exact}/{frames}",
                       "cpu_1core_frames_per_s": round(frames / cpu_s, 3)}
@@ -350,6 +356,8 @@ def main():
     enqueue_all(procs, streams)
     rec = rdp_mi.Recording(procs, batched=True)
     log(f"[bench] batched recording: {rec.rounds} rounds, {rec.passes} passes")
+    if rec.passes < len(streams):
+        raise RuntimeError(f"the batched recording holds {rec.passes} passes for {len(streams)} streams")
     if args.no_autotune:
         tuned = {"xcd_chunk": 64, "min_class": 0, "ms": None}
     else:
@@ -368,6 +376,7 @@ def main():
     launches = rec.profile_raster(3)
     raster_bytes = rec.raster_bytes
     rounds = rec.rounds
+    rec_passes = rec.passes
     raster_launches = rec.raster_launches
     rec.close()
     for p in procs:
@@ -379,6 +388,8 @@ def main():
         procs = setup_processors(rdp_mi, streams)
         enqueue_all(procs, streams)
         plain = rdp_mi.Recording(procs, batched=False)
+        if plain.passes != rec_passes:
+            raise RuntimeError(f"the plain recording holds {plain.passes} passes, the batched one {rec_passes}")
         plain.replay(max(args.warmup, 1), use_graph=False)
         plain_steps = max(3, args.steps // 5)
         plain_ms = plain.replay(plain_steps, use_graph=False)

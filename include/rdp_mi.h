@@ -105,6 +105,16 @@ void rdp_mi_enqueue_command_direct(rdp_mi_processor *p, unsigned num_words, cons
  * a word with an undefined or private opcode is skipped. */
 size_t rdp_mi_enqueue_command_list(rdp_mi_processor *p, size_t num_words, const uint32_t *words);
 
+/* No reference counterpart (a switch for the batched and recorded
+ * submissions below).  With hold != 0 the processor's completed render
+ * passes stay on the host: the submissions the reference makes on its own
+ * (after 500 us without commands, command_ring.cpp:103-121, and when many
+ * passes queue up) are suspended, so rdp_mi_record and rdp_mi_batch_flush
+ * take every pass enqueued since the last submission.  Explicit flush,
+ * signal_timeline, idle and scanout still submit.  Host memory grows with
+ * the held passes.  Default: 0 (the reference's behaviour). */
+void rdp_mi_set_hold_submissions(rdp_mi_processor *p, int hold);
+
 /* CommandProcessor::set_quirks, rdp_device.cpp:961-968 (in-band). */
 void rdp_mi_set_quirks(rdp_mi_processor *p, uint32_t quirks);
 

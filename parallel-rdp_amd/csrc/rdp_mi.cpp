@@ -262,6 +262,9 @@ struct Processor {
 	rdp_mi_validation_fn validation_fn = nullptr;
 	void *validation_user = nullptr;
 	bool supported = false;
+	// rdp_mi_set_hold_submissions: completed passes stay on the host until an
+	// explicit flush, a recording or a batched submission takes them.
+	std::atomic<bool> hold{ false };
 
 	// Guards the front end and the VI state: held by the ring thread while
 	// it decodes and by every entry point that touches them directly.
@@ -1109,7 +1112,7 @@ static void run_commands(Processor &p, const uint32_t *w, size_t n)
 			(void)flush_now(p);
 		else {
 			p.fe.command(w + i);
-			if (p.fe.completed().size() >= 32) (void)submit_pending(p);
+			if (p.fe.completed().size() >= 32 && !p.hold.load()) (void)submit_pending(p);
 		}
 		i += len;
 	}
@@ -1143,11 +1146,11 @@ static void ring_loop(Processor *p)
 		bool pending;
 		{
 			std::lock_guard<std::mutex> g(p->lock);
-			if (idle)
-				(void)submit_pending(*p);
-			else
+			if (idle) {
+				if (!p->hold.load()) (void)submit_pending(*p);
+			} else
 				run_commands(*p, batch.data(), batch.size());
-			pending = !p->fe.completed().empty();
+			pending = !p->fe.completed().empty() && !p->hold.load();
 		}
 		std::lock_guard<std::mutex> g(r.lock);
 		r.pending = pending;
@@ -1822,6 +1825,12 @@ size_t rdp_mi_enqueue_command_list(rdp_mi_processor *h, size_t num_words, const 
 	}
 	if (i > start) ring_push(h->p, words + start, i - start);
 	return i;
+}
+
+void rdp_mi_set_hold_submissions(rdp_mi_processor *h, int hold)
+{
+	if (!h) return;
+	h->p.hold.store(hold != 0);
 }
 
 void rdp_mi_set_quirks(rdp_mi_processor *h, uint32_t quirks)

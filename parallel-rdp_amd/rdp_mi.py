@@ -90,6 +90,7 @@ def load_library():
     lib.rdp_mi_enqueue_command_list.restype = sz
     lib.rdp_mi_enqueue_command_list.argtypes = [vp, sz, vp]
     lib.rdp_mi_set_quirks.argtypes = [vp, u32]
+    lib.rdp_mi_set_hold_submissions.argtypes = [vp, ctypes.c_int]
     lib.rdp_mi_flush.argtypes = [vp]
     lib.rdp_mi_signal_timeline.restype = u64
     lib.rdp_mi_signal_timeline.argtypes = [vp]
@@ -234,6 +235,11 @@ class CommandProcessor:
 
     def set_quirks(self, quirks: int):
         self._lib.rdp_mi_set_quirks(self._h, quirks)
+
+    def hold_submissions(self, hold: bool = True):
+        """rdp_mi_set_hold_submissions: keep completed passes on the host
+        until a recording, a batched submission or an explicit flush."""
+        self._lib.rdp_mi_set_hold_submissions(self._h, 1 if hold else 0)
 
     # -- synchronisation --------------------------------------------------
     def flush(self):

@@ -244,3 +244,36 @@ def test_async_scanout_buffer_matches_scanout_sync():
     buf.close()
     a.close()
     b.close()
+
+
+def test_held_passes_wait_for_the_recording():
+    """With submissions held, passes enqueued long before the recording (well
+    past the ring's 500 us idle submission) are all still pending for it, and
+    its replay gives the oracle's bytes."""
+    import time
+
+    import rdp_mi
+    streams = [rs.batch_stream(i, max_tris=400) for i in (1, 4)]
+    procs = []
+    for s in streams:
+        p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+        p.hold_submissions()
+        p.write_rdram(s.rdram_init if s.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
+        p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
+        for f in s.frames:
+            p.enqueue_commands(f.words)
+        procs.append(p)
+    time.sleep(0.02)
+    # Nothing reached the device: RDRAM is still the initial image.
+    for p, s in zip(procs, streams):
+        init = s.rdram_init if s.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32)
+        assert np.array_equal(p.read_rdram(), init), s.name
+    rec = rdp_mi.Recording(procs, batched=True)
+    assert rec.passes >= 2 * len(streams)
+    rec.replay(1, use_graph=True)
+    for p, s in zip(procs, streams):
+        total, lines = harness.compare((p.read_rdram(), p.read_hidden_rdram(), p.read_tmem()), harness.run_oracle(s))
+        assert total == 0, f"{s.name}: " + "; ".join(lines)
+    rec.close()
+    for p in procs:
+        p.close()

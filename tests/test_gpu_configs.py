@@ -77,10 +77,12 @@ def test_batch64_graph_replay(rdp_mi):
     refs = harness.run_oracle_many(streams)
     procs = [_new_proc(rdp_mi, s) for s in streams]
     for p, s in zip(procs, streams):
+        p.hold_submissions()
         for f in s.frames:
             p.enqueue_commands(f.words)
     rec = rdp_mi.Recording(procs, batched=True)
     say(f"[case] recording: {rec.rounds} rounds, {rec.passes} passes")
+    assert rec.passes >= 2 * len(streams)
     for it in range(2):
         rec.replay(1, use_graph=True)
         for p, s, r in zip(procs, streams, refs):

@@ -74,10 +74,12 @@ def test_fill_passes_batched_recording():
     print(f"[case] batched recording of {len(streams)} fill streams", flush=True)
     procs = [rdp_mi.CommandProcessor(rs.RDRAM_SIZE) for _ in streams]
     for p, s in zip(procs, streams):
+        p.hold_submissions()
         p.write_rdram(s.rdram_init)
         p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
         p.enqueue_commands(s.frames[0].words)
     rec = rdp_mi.Recording(procs, batched=True)
+    assert rec.passes >= len(streams)
     rec.replay(1)
     outs = [(p.read_rdram(), p.read_hidden_rdram(), p.read_tmem()) for p in procs]
     rec.close()

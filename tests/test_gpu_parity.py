@@ -68,6 +68,7 @@ def _procs_for(rdp_mi, streams):
     procs = []
     for s in streams:
         p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+        p.hold_submissions()  # every pass stays pending for the recording or batched flush
         p.write_rdram(s.rdram_init if s.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
         p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
         for f in s.frames:
@@ -126,6 +127,7 @@ def test_batch_flush_many_processors(rdp_mi):
     procs = []
     for s in streams:
         p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+        p.hold_submissions()
         p.write_rdram(s.rdram_init if s.rdram_init is not None else np.zeros(rs.RDRAM_SIZE // 4, np.uint32))
         p.write_hidden_rdram(np.full(rs.RDRAM_SIZE // 2, s.hidden_init, np.uint8))
         procs.append(p)
