@@ -226,6 +226,22 @@ void rdp_mi_set_validation_callback(rdp_mi_processor *p, rdp_mi_validation_fn fn
  * to flushing each processor on its own. */
 int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count);
 
+/* Allocation-free rdp_mi_batch_flush.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * launches as rdp_mi_batch_flush, with the packed inputs and per-pass
+ * scratch in a static device array of the library (16 MiB of packed inputs
+ * plus 112 MiB of scratch; each GPU the library runs on holds it until the
+ * process ends) and the host side of the upload in a static host array: the
+ * call allocates no device or pinned host memory.  It returns when the
+ * device has finished the submitted work.  A submission beyond either limit
+ * returns RDP_MI_ERROR_LIMIT before any copy or launch and leaves the passes
+ * pending (rdp_mi_batch_flush or rdp_mi_flush then submits them).  Every
+ * call on one GPU shares the array; calls are serialised by the library's
+ * context lock, and the kernels write every scratch byte before reading it,
+ * so nothing needs clearing between calls. */
+int rdp_mi_batch_flush_noalloc(rdp_mi_processor *const *procs, unsigned count);
+
 /* Recorded submissions (no reference counterpart; the analogue of a
  * pre-recorded command buffer).  rdp_mi_record takes the pending render
  * passes of the processors (batched != 0: one launch per stage per pass

@@ -93,12 +93,17 @@ static void blender_divider_table(uint8_t *out)
 	}
 }
 
+// A device arena that grows on demand.  A fixed arena (memory the library
+// does not own, such as part of the static `scratch` array) never grows: a
+// request beyond its size fails with RDP_MI_ERROR_LIMIT.
 struct DeviceBuffer {
 	uint8_t *ptr = nullptr;
 	size_t size = 0;
+	bool fixed = false;
 	int reserve(size_t want, hipStream_t stream)
 	{
 		if (want <= size) return 0;
+		if (fixed) return RDP_MI_ERROR_LIMIT;
 		size_t ns = std::max(want, size * 2);
 		ns = (ns + 0xfffff) & ~(size_t)0xfffff;
 		if (ptr) {
@@ -117,9 +122,11 @@ struct DeviceBuffer {
 struct PinnedBuffer {
 	uint8_t *ptr = nullptr;
 	size_t size = 0;
+	bool fixed = false;
 	int reserve(size_t want)
 	{
 		if (want <= size) return 0;
+		if (fixed) return RDP_MI_ERROR_LIMIT;
 		size_t ns = std::max(want, size * 2);
 		ns = (ns + 0xfffff) & ~(size_t)0xfffff;
 		if (ptr) {
@@ -198,8 +205,38 @@ struct Lane {
 	}
 };
 
+// ---- allocation-free submission (rdp_mi_batch_flush_noalloc) ----
+// Packed inputs and per-pass scratch of that entry point live in `scratch`,
+// a static device array every GPU the library runs on holds until the
+// process ends: the first kNoAllocInputBytes hold the packed inputs, the rest
+// the per-pass scratch.  In host code the array's name is a host-side
+// stand-in, not a device address: set-up looks the device address up once
+// per device (noalloc_init) and every later use goes through that address.
+// The host side of the upload is a static (pageable) array, so the entry
+// point allocates neither device nor pinned host memory.
+constexpr size_t kNoAllocInputBytes = 16u << 20;
+constexpr size_t kNoAllocScratchBytes = 112u << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocInputBytes + kNoAllocScratchBytes];
+static uint8_t *g_scratch_addr[kNoAllocMaxDevices];  // device address of `scratch`, per device
+alignas(4096) static uint8_t g_noalloc_staging[kNoAllocInputBytes];
+
+// Once per device, with that device current.
+static int noalloc_init(int device)
+{
+	if (device < 0 || device >= kNoAllocMaxDevices)
+		return set_error(RDP_MI_ERROR_UNSUPPORTED, "device %d beyond the %d the library tracks", device, kNoAllocMaxDevices);
+	if (g_scratch_addr[device]) return 0;
+	void *addr = nullptr;
+	HIP_CHECK(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+	if (!addr) return set_error(RDP_MI_ERROR_DEVICE, "no device address for the static scratch array");
+	g_scratch_addr[device] = static_cast<uint8_t *>(addr);
+	return 0;
+}
+
 struct Context : Lane {
 	bool ok = false;
+	int device = 0;  // the device current at init; every launch goes there
 	uint8_t *d_gamma = nullptr;
 	LaunchConfig cfg;  // for immediate submissions (flush, batch flush, serving)
 	std::mutex lock;
@@ -211,6 +248,8 @@ struct Context : Lane {
 		if (ok) return 0;
 		int n = 0;
 		if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(RDP_MI_ERROR_DEVICE, "no HIP device");
+		HIP_CHECK(hipGetDevice(&device));
+		if (int err = noalloc_init(device)) return err;
 		if (int err = create()) return err;
 		std::vector<uint8_t> lut(0x8000);
 		blender_divider_table(lut.data());
@@ -541,8 +580,9 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 // Pack the inputs of all rounds into `input` (one async copy on the context
 // stream) and describe the launches.  Caller holds the context lock.
 static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &rounds, DeviceBuffer &input,
-                          DeviceBuffer &scratch, std::vector<Launch> &launches)
+                          DeviceBuffer &scratch, std::vector<Launch> &launches, PinnedBuffer *staging_override = nullptr)
 {
+	PinnedBuffer &staging = staging_override ? *staging_override : c.staging;
 	launches.clear();
 	if (rounds.empty()) return 0;
 	if (c.staging_busy) {
@@ -634,14 +674,20 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	const size_t total_input = off;
 	c.last_arena_bytes = total_input + max_scratch;
 
+	if (input.fixed && total_input > input.size)
+		return set_error(RDP_MI_ERROR_LIMIT, "the submission needs %zu bytes of packed inputs, more than the %zu the static "
+		                 "array holds", total_input, input.size);
+	if (scratch.fixed && max_scratch > scratch.size)
+		return set_error(RDP_MI_ERROR_LIMIT, "the submission needs %zu bytes of scratch, more than the %zu the static array "
+		                 "holds", max_scratch, scratch.size);
 	int err;
-	if ((err = c.staging.reserve(total_input))) return set_error(err, "staging allocation of %zu bytes failed", total_input);
+	if ((err = staging.reserve(total_input))) return set_error(err, "staging allocation of %zu bytes failed", total_input);
 	if ((err = input.reserve(total_input, c.stream)))
 		return set_error(err, "input allocation of %zu bytes failed", total_input);
 	if ((err = scratch.reserve(std::max<size_t>(max_scratch, 256), c.stream)))
 		return set_error(err, "scratch allocation of %zu bytes failed", max_scratch);
 
-	uint8_t *h = c.staging.ptr;
+	uint8_t *h = staging.ptr;
 	uint8_t *d = input.ptr;
 	launches.resize(rounds.size());
 	for (size_t r = 0; r < rounds.size(); r++) {
@@ -789,10 +835,17 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 
 // Submit rounds of passes through a lane's arenas.  The caller owns the lane
 // (for the context's own lane: holds the context lock).
-static int submit_rounds(Lane &c, const std::vector<std::vector<PassRef>> &rounds)
+// Arenas a submission uses in place of its lane's own.
+struct Arenas {
+	DeviceBuffer *input, *scratch;
+	PinnedBuffer *staging;
+};
+
+static int submit_rounds(Lane &c, const std::vector<std::vector<PassRef>> &rounds, const Arenas *arenas = nullptr)
 {
 	std::vector<Launch> launches;
-	int err = prepare_rounds(c, rounds, c.input, c.scratch, launches);
+	int err = arenas ? prepare_rounds(c, rounds, *arenas->input, *arenas->scratch, launches, arenas->staging)
+	                 : prepare_rounds(c, rounds, c.input, c.scratch, launches);
 	if (err) return err;
 	return execute_launches(launches, c, ctx().cfg);
 }
@@ -909,7 +962,8 @@ static int coherent_push(Processor &p, const std::vector<HostPass> &passes, hipS
 
 // Submit every completed pass of the given processors (plain path: one
 // processor, one pass per round) on a lane (default: the context's).
-static int submit_processors(Processor *const *procs, unsigned count, bool batched, Lane *lane = nullptr)
+static int submit_processors(Processor *const *procs, unsigned count, bool batched, Lane *lane = nullptr,
+                             const Arenas *arenas = nullptr)
 {
 	Lane &c = lane ? *lane : ctx();
 	std::vector<std::vector<PassRef>> rounds;
@@ -930,13 +984,16 @@ static int submit_processors(Processor *const *procs, unsigned count, bool batch
 	if (ret) {
 	} else if (batched) {
 		// Keep each launch's pass count within the work-item encoding.
-		ret = submit_rounds(c, rounds);
+		ret = submit_rounds(c, rounds, arenas);
 	} else {
 		for (auto &r : rounds) {
 			std::vector<std::vector<PassRef>> one{ r };
-			if ((ret = submit_rounds(c, one))) break;
+			if ((ret = submit_rounds(c, one, arenas))) break;
 		}
 	}
+	// A batched submission that does not fit fixed arenas fails before its
+	// first copy or launch: its passes stay pending for another entry point.
+	if (ret == RDP_MI_ERROR_LIMIT && batched && arenas) return ret;
 	for (unsigned i = 0; i < count && !ret; i++)
 		if (procs[i]->coherent) ret = coherent_push(*procs[i], procs[i]->fe.completed(), c.stream);
 	for (unsigned i = 0; i < count; i++) procs[i]->fe.completed().clear();
@@ -2150,6 +2207,33 @@ int rdp_mi_batch_flush(rdp_mi_processor *const *procs, unsigned count)
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	return submit_processors(set.ps.data(), (unsigned)set.ps.size(), true);
+}
+
+int rdp_mi_batch_flush_noalloc(rdp_mi_processor *const *procs, unsigned count)
+{
+	ProcessorSet set;
+	if (int err = set.init(procs, count)) return err;
+	Context &c = ctx();
+	std::lock_guard<std::mutex> g(c.lock);
+	uint8_t *base = g_scratch_addr[c.device];
+	if (!base) return set_error(RDP_MI_ERROR_DEVICE, "the static scratch array has no device address");
+	DeviceBuffer input, scr;
+	input.ptr = base;
+	input.size = kNoAllocInputBytes;
+	input.fixed = true;
+	scr.ptr = base + kNoAllocInputBytes;
+	scr.size = kNoAllocScratchBytes;
+	scr.fixed = true;
+	PinnedBuffer staging;
+	staging.ptr = g_noalloc_staging;
+	staging.size = sizeof(g_noalloc_staging);
+	staging.fixed = true;
+	const Arenas arenas{ &input, &scr, &staging };
+	int err = submit_processors(set.ps.data(), (unsigned)set.ps.size(), true, nullptr, &arenas);
+	// The next call reuses the array and the host staging: finish here.
+	const hipError_t e = hipStreamSynchronize(c.stream);
+	if (!err && e != hipSuccess) err = set_error(RDP_MI_ERROR_DEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+	return err;
 }
 
 uint64_t rdp_mi_primitive_count(const rdp_mi_processor *h) { return h ? h->p.fe.primitive_counter() : 0; }

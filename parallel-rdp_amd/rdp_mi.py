@@ -134,6 +134,8 @@ def load_library():
     lib.rdp_mi_set_validation_callback.argtypes = [vp, VALIDATION_FN, vp]
     lib.rdp_mi_batch_flush.restype = ctypes.c_int
     lib.rdp_mi_batch_flush.argtypes = [ctypes.POINTER(vp), cu]
+    lib.rdp_mi_batch_flush_noalloc.restype = ctypes.c_int
+    lib.rdp_mi_batch_flush_noalloc.argtypes = [ctypes.POINTER(vp), cu]
     lib.rdp_mi_last_error.restype = ctypes.c_char_p
     lib.rdp_mi_last_error.argtypes = []
     lib.rdp_mi_primitive_count.restype = u64
@@ -445,6 +447,19 @@ def batch_flush(processors: Sequence[CommandProcessor]):
     lib = load_library()
     arr = (ctypes.c_void_p * len(processors))(*[p.handle.value for p in processors])
     _check(lib.rdp_mi_batch_flush(arr, len(processors)), "batch_flush")
+
+
+def batch_flush_noalloc(processors: Sequence[CommandProcessor]) -> int:
+    """rdp_mi_batch_flush_noalloc: the batched submission through the
+    library's static device scratch array (no allocation); returns when the
+    work is done.  Returns 0, or ERROR_LIMIT when the submission does not fit
+    the array (its passes then stay pending); raises on other errors."""
+    lib = load_library()
+    arr = (ctypes.c_void_p * len(processors))(*[p.handle.value for p in processors])
+    ret = lib.rdp_mi_batch_flush_noalloc(arr, len(processors))
+    if ret != ERROR_LIMIT:
+        _check(ret, "batch_flush_noalloc")
+    return ret
 
 
 class Recording:
