@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--max-batch", type=int, default=64, help="frames per serving batch at most")
     ap.add_argument("--serve-streams", type=int, default=3, help="submission streams of the server")
     ap.add_argument("--serve-decode-threads", type=int, default=0,
-                    help="host decode threads of the server (0: half the host threads, at most 8)")
+                    help="host decode threads of the server (0: a quarter of the host threads, at most 8)")
     ap.add_argument("--serve-dump", default="", help="write per-request serving times to PREFIX_<load>.npz")
     ap.add_argument("--hbm-budget-mib", type=int, default=2048,
                     help="device bytes one serving batch may use for packed inputs and scratch")

@@ -297,7 +297,7 @@ void rdp_mi_recording_destroy(rdp_mi_recording *r);
  *  - a request that has arrived is admitted as soon as its processor is idle
  *    and the estimated device bytes of admitted, not yet submitted frames
  *    stay within hbm_budget_bytes (0 = 4 GiB); it is decoded at once on one
- *    of decode_threads host threads (0 = half the host threads, at most 8);
+ *    of decode_threads host threads (0 = a quarter of the host threads, at most 8);
  *  - decoded frames are submitted together (at most max_batch per batch,
  *    0 = count, within the budget) whenever one of `streams` submission
  *    streams (0 = 3, at most 8) is free, so new work starts while earlier

@@ -2529,10 +2529,12 @@ int rdp_mi_server_create(rdp_mi_processor *const *procs, unsigned count, size_t 
 	}
 	s.budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)1 << 32);
 	s.max_batch = max_batch ? max_batch : count;
-	// Defaults: half the host threads (at most 8) decode, leaving the rest to
-	// the scheduler, submitter and completion threads; three streams.
+	// Defaults: a quarter of the host threads (at most 8) decode, leaving the
+	// rest to the scheduler, submitter and completion threads and the
+	// caller (more decode threads lengthened the latency tail on a 16-thread
+	// host); three streams.
 	s.decode_threads = decode_threads ? decode_threads
-	                                  : std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+	                                  : std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 4));
 	for (unsigned i = 0; i < (streams ? std::min(streams, 8u) : 3u); i++)
 		s.lanes.emplace_back(new Server::LaneState());
 	s.queues.resize(count);
