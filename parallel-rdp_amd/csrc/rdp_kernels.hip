@@ -2130,19 +2130,50 @@ hipError_t upload_blender_lut(const uint8_t *lut)
 	return hipMemcpyToSymbol(HIP_SYMBOL(g_blender_lut), lut, sizeof(g_blender_lut), 0, hipMemcpyHostToDevice);
 }
 
+// Orders a wave's LDS staging writes against its own later reads (and its
+// earlier reads against the next staging): a workgroup barrier when the
+// workgroup is the one wave, a wave-scope fence and barrier when each wave of
+// the workgroup stages into its own LDS slice and runs its own primitive list.
+template <bool QUAD>
+DEV void staging_sync()
+{
+	if constexpr (QUAD) {
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	} else
+		__syncthreads();
+}
+
 // CLS: occupancy class; the kernel holds the variants of classes <= CLS (and
 // the generic code in the top class).  The host launches each pass in the
 // class of its heaviest primitive.
-template <uint32_t CLS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
+// QUAD: a work item is a 16x16 tile and the workgroup its four 8x8 quadrants,
+// one wave each (quadrant q = wave q: x + (q & 1), y + (q >> 1)), so the
+// quadrants of a 16x16 tile share one CU and its L1 for the primitive and
+// span records they have in common.  Each wave walks its own quadrant's bin
+// masks and stages TMEM into its own LDS slice; waves never wait for each
+// other.  Otherwise a work item is one 8x8 tile and the workgroup one wave.
+template <uint32_t CLS, bool QUAD>
+__global__ __launch_bounds__(QUAD ? 256 : 64) __attribute__((amdgpu_waves_per_eu(
 	CLS == 0 ? RDP_RASTER_WAVES_CLASS0 : (CLS == 1 ? RDP_RASTER_WAVES_CLASS1 : RDP_RASTER_WAVES_CLASS2))))
 void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
 {
-	__shared__ uint16_t s_tmem[2048];
+	__shared__ uint16_t s_tmem_all[(QUAD ? 4 : 1) * 2048];
 	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)];
 	const PassDesc &P = passes[item >> 14];
-	const uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
-	const int lane = threadIdx.x;
+	uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
+	const int lane = threadIdx.x & 63;
+	uint16_t *s_tmem = s_tmem_all;
+	if constexpr (QUAD) {
+		const uint32_t q = uniform_u32(threadIdx.x >> 6);
+		tx = tx * 2 + (q & 1);
+		ty = ty * 2 + (q >> 1);
+		// A quadrant past the framebuffer's last tile column or row: the wave
+		// has nothing to do (no other wave waits for it).
+		if (tx >= P.tiles_x || ty >= P.tiles_y) return;
+		s_tmem += q * 2048;
+	}
 	const int x = (int)(tx * 8) + (lane & 7);
 	const int y = (int)(ty * 8) + (lane >> 3);
 	const uint32_t fmt = P.fb_fmt;
@@ -2197,7 +2228,7 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 
 				// Stage the primitive's TMEM snapshot in LDS (wave-uniform).
 				if ((h.ras_flags & needs_tmem) && h.tmem_instance != staged) {
-					__syncthreads();
+					staging_sync<QUAD>();
 					// The snapshot base is read from the descriptor here, not kept
 					// live through the loop.
 					const AS_CONST PassDesc *pt = cptr(&P);
@@ -2207,7 +2238,7 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 					uint32_t *dst = reinterpret_cast<uint32_t *>(s_tmem);
 #pragma unroll
 					for (int k = 0; k < 16; k++) dst[lane * 4 + (k & 3) + 256 * (k >> 2)] = src[lane * 4 + (k & 3) + 256 * (k >> 2)];
-					__syncthreads();
+					staging_sync<QUAD>();
 					staged = h.tmem_instance;
 				}
 
@@ -2360,14 +2391,23 @@ hipError_t launch_raster_fill(const PassDesc *passes, const uint32_t *items, uin
 	return hipGetLastError();
 }
 
+// quad: the items are 16x16 tiles, four waves per workgroup.
 hipError_t launch_raster(uint32_t cls, const PassDesc *passes, const uint32_t *items, uint32_t n, int32_t xcd_chunk,
-                         hipStream_t s)
+                         bool quad, hipStream_t s)
 {
 	if (!n) return hipSuccess;
-	switch (cls) {
-	case 0: hipLaunchKernelGGL(k_raster<0>, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
-	case 1: hipLaunchKernelGGL(k_raster<1>, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
-	default: hipLaunchKernelGGL(k_raster<2>, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+	if (quad) {
+		switch (cls) {
+		case 0: hipLaunchKernelGGL((k_raster<0, true>), dim3(n), dim3(256), 0, s, passes, items, xcd_chunk); break;
+		case 1: hipLaunchKernelGGL((k_raster<1, true>), dim3(n), dim3(256), 0, s, passes, items, xcd_chunk); break;
+		default: hipLaunchKernelGGL((k_raster<2, true>), dim3(n), dim3(256), 0, s, passes, items, xcd_chunk); break;
+		}
+	} else {
+		switch (cls) {
+		case 0: hipLaunchKernelGGL((k_raster<0, false>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+		case 1: hipLaunchKernelGGL((k_raster<1, false>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+		default: hipLaunchKernelGGL((k_raster<2, false>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+		}
 	}
 	return hipGetLastError();
 }

@@ -42,16 +42,19 @@ namespace rdp_mi {
 hipError_t launch_setup(const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
-hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, bool, hipStream_t);
 hipError_t launch_raster_fill(const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 hipError_t upload_blender_lut(const uint8_t *lut);
 
 // Launch shape of the raster stage: the XCD chunk of the tile-to-block
-// mapping and the lowest occupancy class a pass runs in (passes below it are
-// promoted, merging their launches).  Chosen per workload by the autotuner.
+// mapping, the lowest occupancy class a pass runs in (passes below it are
+// promoted, merging their launches) and the tile a workgroup takes (8x8 with
+// one wave, or 16x16 with one wave per 8x8 quadrant).  Chosen per workload by
+// the autotuner.
 struct LaunchConfig {
 	int32_t xcd_chunk = 64;
 	uint32_t min_class = 0;
+	bool quad = false;  // 16x16 tiles
 };
 
 static thread_local std::string g_last_error;
@@ -395,8 +398,10 @@ struct Launch {
 	uint32_t tmem_scan_n;
 	const uint32_t *span_items;  // second setup launch: one item per 8 tile-row entries
 	uint32_t span_n;
-	const uint32_t *raster_items[kRasterClasses];
+	const uint32_t *raster_items[kRasterClasses];  // 8x8 tiles
 	uint32_t raster_n[kRasterClasses];
+	const uint32_t *quad_items[kRasterClasses];  // the same passes as 16x16 tiles
+	uint32_t quad_n[kRasterClasses];
 	const uint32_t *fill_items;  // passes of FILL primitives only: k_raster_fill, 4 tiles per item
 	uint32_t fill_n;
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
@@ -405,6 +410,9 @@ struct Launch {
 	uint64_t class_bytes[kRasterClasses], fill_bytes;  // the same per raster launch
 	uint32_t fill_tiles;
 	uint32_t passes;
+	// Raster work items of class c in the configured tile size.
+	const uint32_t *ritems(uint32_t c, bool quad) const { return quad ? quad_items[c] : raster_items[c]; }
+	uint32_t rn(uint32_t c, bool quad) const { return quad ? quad_n[c] : raster_n[c]; }
 };
 
 // Events around each raster launch of a round when the stages run in series
@@ -506,8 +514,8 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		// the heavier classes.
 		const uint32_t lo = std::min(cfg.min_class, kRasterClasses - 1);
 		uint32_t merged = 0, heavy = 0;
-		for (uint32_t c = 0; c <= lo; c++) merged += L.raster_n[c];
-		for (uint32_t c = lo + 1; c < kRasterClasses; c++) heavy += L.raster_n[c];
+		for (uint32_t c = 0; c <= lo; c++) merged += L.rn(c, cfg.quad);
+		for (uint32_t c = lo + 1; c < kRasterClasses; c++) heavy += L.rn(c, cfg.quad);
 		const bool fork_raster = big && merged && heavy;
 		const RoundEvents E = overlap ? lane.events[r] : RoundEvents();
 
@@ -549,11 +557,11 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			// previous round's raster first.
 			if (early) HIP_CHECK(hipStreamWaitEvent(side, E.fork, 0));
 			for (uint32_t c = lo + 1; c < kRasterClasses; c++)
-				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, side));
+				HIP_CHECK(launch_raster(c, L.descs, L.ritems(c, cfg.quad), L.rn(c, cfg.quad), cfg.xcd_chunk, cfg.quad, side));
 			HIP_CHECK(hipEventRecord(E.join, side));
 			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
 			HIP_CHECK(launch_raster_fill(L.descs, L.fill_items, L.fill_n, cfg.xcd_chunk, s));
-			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
+			HIP_CHECK(launch_raster(lo, L.descs, L.ritems(0, cfg.quad), merged, cfg.xcd_chunk, cfg.quad, s));
 			HIP_CHECK(hipStreamWaitEvent(s, E.join, 0));
 		} else {
 			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
@@ -564,11 +572,11 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			// Class segments are contiguous in class order, so promoting the
 			// classes below min_class merges them into one launch.
 			if (re) HIP_CHECK(hipEventRecord(re[2 + 2 * lo], s));
-			HIP_CHECK(launch_raster(lo, L.descs, L.raster_items[0], merged, cfg.xcd_chunk, s));
+			HIP_CHECK(launch_raster(lo, L.descs, L.ritems(0, cfg.quad), merged, cfg.xcd_chunk, cfg.quad, s));
 			if (re) HIP_CHECK(hipEventRecord(re[3 + 2 * lo], s));
 			for (uint32_t c = lo + 1; c < kRasterClasses; c++) {
 				if (re) HIP_CHECK(hipEventRecord(re[2 + 2 * c], s));
-				HIP_CHECK(launch_raster(c, L.descs, L.raster_items[c], L.raster_n[c], cfg.xcd_chunk, s));
+				HIP_CHECK(launch_raster(c, L.descs, L.ritems(c, cfg.quad), L.rn(c, cfg.quad), cfg.xcd_chunk, cfg.quad, s));
 				if (re) HIP_CHECK(hipEventRecord(re[3 + 2 * c], s));
 			}
 		}
@@ -593,8 +601,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	// Input layout: per round [PassDesc x n][items: setup|tmem|bin|raster], then per pass data.
 	struct RoundLayout {
 		size_t descs, items_setup, items_tmem, items_bin, items_raster;
-		uint32_t n_setup, n_span, n_tmem, n_tmem_gather, n_bin, n_raster, n_fill;
-		uint32_t n_raster_class[kRasterClasses];
+		uint32_t n_setup, n_span, n_tmem, n_tmem_gather, n_bin, n_raster, n_fill, n_quad;
+		uint32_t n_raster_class[kRasterClasses], n_quad_class[kRasterClasses];
 		size_t scratch;
 	};
 	std::vector<RoundLayout> rl(rounds.size());
@@ -618,8 +626,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		if (rounds[r].size() > (1u << 15))
 			return set_error(RDP_MI_ERROR_UNSUPPORTED, "%zu passes in one round (at most 32768)", rounds[r].size());
 		RoundLayout &L = rl[r];
-		L.n_setup = L.n_span = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = L.n_fill = 0;
-		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = 0;
+		L.n_setup = L.n_span = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = L.n_fill = L.n_quad = 0;
+		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = L.n_quad_class[c] = 0;
 		size_t scr = scratch_base;
 		pl[r].resize(rounds[r].size());
 		for (size_t i = 0; i < rounds[r].size(); i++) {
@@ -639,6 +647,9 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 				} else {
 					L.n_raster += tx * ty;
 					L.n_raster_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
+					const uint32_t quads = ((tx + 1) / 2) * ((ty + 1) / 2);
+					L.n_quad += quads;
+					L.n_quad_class[std::min(hp.raster_class, kRasterClasses - 1)] += quads;
 				}
 			}
 			pi.sl = scratch_layout(hp);
@@ -657,7 +668,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		L.items_bin = off;
 		off = align_up(off + L.n_bin * 4ull, 256);
 		L.items_raster = off;
-		off = align_up(off + (L.n_raster + L.n_fill) * 4ull, 256);
+		off = align_up(off + (L.n_raster + L.n_fill + L.n_quad) * 4ull, 256);
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
 			PassInputs &pi = pl[r][i];
@@ -698,11 +709,15 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
 		uint32_t ns = 0, nsp = 0, nt = 0, ntg = 0, nb = 0;
-		uint32_t nr[kRasterClasses], rbase[kRasterClasses];
-		for (uint32_t c = 0, acc = 0; c < kRasterClasses; c++) {
-			nr[c] = 0;
+		// Raster items: the 8x8-tile list by class, the fill items, then the
+		// 16x16-tile list by class.
+		uint32_t nr[kRasterClasses], rbase[kRasterClasses], nq[kRasterClasses], qbase[kRasterClasses];
+		for (uint32_t c = 0, acc = 0, qacc = L.n_raster + L.n_fill; c < kRasterClasses; c++) {
+			nr[c] = nq[c] = 0;
 			rbase[c] = acc;
 			acc += L.n_raster_class[c];
+			qbase[c] = qacc;
+			qacc += L.n_quad_class[c];
 		}
 		Launch &LA = launches[r];
 		LA.descs = reinterpret_cast<const PassDesc *>(d + L.descs);
@@ -725,6 +740,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		for (uint32_t c = 0; c < kRasterClasses; c++) {
 			LA.raster_items[c] = LA.items[3] + rbase[c];
 			LA.raster_n[c] = L.n_raster_class[c];
+			LA.quad_items[c] = LA.items[3] + qbase[c];
+			LA.quad_n[c] = L.n_quad_class[c];
 		}
 		LA.raster_bytes = 0;
 		LA.fill_bytes = 0;
@@ -824,6 +841,9 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			uint32_t *out = it_raster + rbase[cls];
 			for (uint32_t y = 0; y < tiles_y; y++)
 				for (uint32_t x = 0; x < tiles_x; x++) out[nr[cls]++] = (pidx << 14) | (x << 7) | y;
+			uint32_t *qout = it_raster + qbase[cls];
+			for (uint32_t y = 0; y < (tiles_y + 1) / 2; y++)
+				for (uint32_t x = 0; x < (tiles_x + 1) / 2; x++) qout[nq[cls]++] = (pidx << 14) | (x << 7) | y;
 		}
 	}
 
@@ -1293,6 +1313,7 @@ struct Recording {
 	std::vector<Launch> launches;
 	DeviceBuffer input, scratch;
 	LaunchConfig cfg;
+	std::vector<std::pair<LaunchConfig, float>> tuned;  // every autotune candidate and its time
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	uint64_t raster_bytes = 0;
@@ -2316,7 +2337,7 @@ int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_gr
 }
 
 int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t *xcd_chunk, uint32_t *min_class,
-                              float *best_ms)
+                              uint32_t *tile_size, float *best_ms)
 {
 	if (!h || iterations == 0) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
 	Context &c = ctx();
@@ -2324,8 +2345,9 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	Recording &r = h->r;
 	static const int32_t chunks[] = { -1, 0, 32, 64, 128 };
 	std::vector<LaunchConfig> cands;
-	for (uint32_t m = 0; m < kRasterClasses; m++)
-		for (int32_t ch : chunks) cands.push_back(LaunchConfig{ ch, m });
+	for (bool quad : { false, true })
+		for (uint32_t m = 0; m < kRasterClasses; m++)
+			for (int32_t ch : chunks) cands.push_back(LaunchConfig{ ch, m, quad });
 	std::vector<float> best(cands.size(), 1e30f);
 	hipEvent_t e0, e1;
 	HIP_CHECK(hipEventCreate(&e0));
@@ -2353,6 +2375,8 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	for (size_t i = 1; i < cands.size(); i++)
 		if (best[i] < best[pick]) pick = i;
 	r.cfg = cands[pick];
+	r.tuned.clear();
+	for (size_t i = 0; i < cands.size(); i++) r.tuned.emplace_back(cands[i], best[i]);
 	// A captured graph holds the old launch shape: capture again on next use.
 	if (r.exec) (void)hipGraphExecDestroy(r.exec);
 	if (r.graph) (void)hipGraphDestroy(r.graph);
@@ -2360,16 +2384,34 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	r.graph = nullptr;
 	if (xcd_chunk) *xcd_chunk = r.cfg.xcd_chunk;
 	if (min_class) *min_class = r.cfg.min_class;
+	if (tile_size) *tile_size = r.cfg.quad ? 16 : 8;
 	if (best_ms) *best_ms = best[pick];
 	return 0;
 }
 
-void rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class)
+unsigned rdp_mi_recording_autotune_results(const rdp_mi_recording *h, unsigned max, int32_t *xcd_chunk,
+                                           uint32_t *min_class, uint32_t *tile_size, float *ms)
 {
+	if (!h) return 0;
+	const auto &t = h->r.tuned;
+	for (size_t i = 0; i < t.size() && i < max; i++) {
+		if (xcd_chunk) xcd_chunk[i] = t[i].first.xcd_chunk;
+		if (min_class) min_class[i] = t[i].first.min_class;
+		if (tile_size) tile_size[i] = t[i].first.quad ? 16 : 8;
+		if (ms) ms[i] = t[i].second;
+	}
+	return (unsigned)t.size();
+}
+
+int rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class, uint32_t tile_size)
+{
+	if (tile_size != 8 && tile_size != 16) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "tile size %u (8 or 16)", tile_size);
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	c.cfg.xcd_chunk = xcd_chunk;
 	c.cfg.min_class = std::min(min_class, kRasterClasses - 1);
+	c.cfg.quad = tile_size == 16;
+	return 0;
 }
 
 int rdp_mi_recording_profile_rounds(rdp_mi_recording *h, unsigned iterations, float *round_stage_ms)

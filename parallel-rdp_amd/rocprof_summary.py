@@ -2,6 +2,7 @@
 
     python parallel-rdp_amd/rocprof_summary.py stats  <kernel-trace .db> [kernel-substring]
     python parallel-rdp_amd/rocprof_summary.py pmc    <pmc .db> [kernel-substring]
+    python parallel-rdp_amd/rocprof_summary.py gaps   <kernel-trace .db> <step-marker kernel> <marker grid_x>
     python parallel-rdp_amd/rocprof_summary.py traffic <fetch .db> <write .db> <copy-fetch .db> <copy-write .db>
                                                        <sq .db> <copy bytes> <out .json> [kernel-substring]
 
@@ -10,6 +11,12 @@ HBM bytes per dispatch (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by the
 factor that makes the same counter read the known byte count of a plain copy
 kernel: the copy runs' largest kernel) and the SQ instruction counts per
 dispatch, as JSON keyed by the kernel's short name (k_raster<0>, ...).
+
+`gaps` splits the trace into steps at each launch of the marker kernel with
+the given grid size (one per step) and prints, over the steps, the median of
+the step's wall time, of the time at least one kernel runs (the union of the
+kernels' intervals) and of the idle time between kernels: the most a single
+persistent launch per step could take off the step.
 
 `stats` prints, per kernel, the call count and total/average duration, then
 for kernels matching the substring a breakdown by launch grid size and the
@@ -73,6 +80,39 @@ def stats(path: str, kernel_filter: str = ""):
         for (short, grid), d in sorted(grids.items(), key=lambda kv: (kv[0][0], -kv[0][1])):
             print(f"{short[:32]:32s} {grid:16d} {len(d):7d} {sum(d) / len(d) / 1e3:10.2f} {min(d) / 1e3:10.2f} "
                   f"{max(d) / 1e3:10.2f}")
+
+
+def gaps(path: str, marker: str, marker_grid: int):
+    db = sqlite3.connect(path)
+    rows = sorted(db.execute("select start, end, name, grid_x from kernels").fetchall())
+    starts = [s for s, e, n, g in rows if marker in n and g == marker_grid]
+    steps = []
+    for a, b in zip(starts, starts[1:]):
+        iv = [(s, e) for s, e, n, g in rows if a <= s < b]
+        if not iv:
+            continue
+        end = max(e for s, e in iv)
+        busy, cur_s, cur_e = 0, None, None
+        for s, e in iv:
+            if cur_e is None or s > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = s, e
+            else:
+                cur_e = max(cur_e, e)
+        busy += cur_e - cur_s
+        steps.append((end - a, busy, len(iv)))
+    if not steps:
+        print("no steps found")
+        return
+    med = lambda xs: sorted(xs)[len(xs) // 2]
+    wall = med([w for w, b, k in steps])
+    busy = med([b for w, b, k in steps])
+    idle = med([w - b for w, b, k in steps])
+    kernels = med([k for w, b, k in steps])
+    print(f"steps {len(steps)}  kernels per step {kernels}")
+    print(f"median step wall {wall / 1e3:.1f} us, kernels running {busy / 1e3:.1f} us, "
+          f"idle between kernels {idle / 1e3:.1f} us ({100.0 * idle / wall:.1f} %)")
 
 
 def pmc(path: str, kernel_filter: str = ""):
@@ -144,7 +184,9 @@ if __name__ == "__main__":
         print(__doc__)
         sys.exit(1)
     flt = sys.argv[3] if len(sys.argv) > 3 else ""
-    if sys.argv[1] == "stats":
+    if sys.argv[1] == "gaps":
+        gaps(sys.argv[2], sys.argv[3], int(sys.argv[4]))
+    elif sys.argv[1] == "stats":
         stats(sys.argv[2], flt)
     else:
         pmc(sys.argv[2], flt)

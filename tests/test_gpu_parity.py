@@ -141,23 +141,25 @@ def test_batch_flush_many_processors(rdp_mi):
         p.close()
 
 
-@pytest.mark.parametrize("xcd_chunk,min_class", [(-1, 0), (0, 1), (32, 2), (128, 0), (64, 1)])
-def test_launch_shapes_bit_exact(rdp_mi, xcd_chunk, min_class):
+@pytest.mark.parametrize("xcd_chunk,min_class,tile_size", [(-1, 0, 8), (0, 1, 8), (32, 2, 8), (128, 0, 8), (64, 1, 8),
+                                                            (-1, 0, 16), (0, 1, 16), (32, 2, 16), (128, 0, 16),
+                                                            (64, 1, 16), (64, 2, 16)])
+def test_launch_shapes_bit_exact(rdp_mi, xcd_chunk, min_class, tile_size):
     """Every raster launch shape the autotuner can pick gives the oracle's bytes:
     batch streams of all three workloads plus random-state streams, submitted
     together."""
     streams = [rs.batch_stream(i, max_tris=800) for i in range(6)] + [rs.fuzz_stream(2000 + i) for i in range(6)]
-    say(f"[case] launch shape xcd_chunk={xcd_chunk} min_class={min_class}")
-    rdp_mi.set_launch_config(xcd_chunk, min_class)
+    say(f"[case] launch shape xcd_chunk={xcd_chunk} min_class={min_class} tile_size={tile_size}")
+    rdp_mi.set_launch_config(xcd_chunk, min_class, tile_size)
     try:
         procs = _procs_for(rdp_mi, streams)
         rdp_mi.batch_flush(procs)
         for p, s in zip(procs, streams):
             p.idle()
-            assert_same(_state(p), harness.run_oracle(s), f"{s.name} ({xcd_chunk}, {min_class})")
+            assert_same(_state(p), harness.run_oracle(s), f"{s.name} ({xcd_chunk}, {min_class}, {tile_size})")
             p.close()
     finally:
-        rdp_mi.set_launch_config(64, 0)
+        rdp_mi.set_launch_config(64, 0, 8)
 
 
 def test_autotuned_recording_bit_exact(rdp_mi):
@@ -169,6 +171,7 @@ def test_autotuned_recording_bit_exact(rdp_mi):
     tuned = rec.autotune(1)
     say(f"[case] autotune picked {tuned}")
     assert tuned["min_class"] in (0, 1, 2) and tuned["xcd_chunk"] in (-1, 0, 32, 64, 128)
+    assert tuned["tile_size"] in (8, 16)
     rec.replay(2, use_graph=True)
     for p, s, r in zip(procs, streams, refs):
         assert_same(_state(p), r, f"{s.name} after autotune")
