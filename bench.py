@@ -76,7 +76,8 @@ def _oracle_module():
 def oracle_stream(s, digests=False, scanout_frames=False):
     """CPU oracle over one stream: final (RDRAM, hidden RDRAM, TMEM), the
     state digest after every frame (digests) and the VI image of every frame
-    that carries VI registers (scanout_frames), and the seconds it took."""
+    that carries VI registers (scanout_frames), the seconds it took and its
+    fragments by render mode."""
     oracle = _oracle_module()
     t0 = time.perf_counter()
     o = oracle.Oracle(rs.RDRAM_SIZE)
@@ -95,8 +96,10 @@ def oracle_stream(s, digests=False, scanout_frames=False):
             per_frame.append(o.digest())
     o.flush()
     out = (o.rdram.copy(), o.hidden.copy(), o.tmem.copy())
+    elapsed = time.perf_counter() - t0
+    frags = o.fragments_by_mode()
     o.close()
-    return out, per_frame, images, time.perf_counter() - t0
+    return out, per_frame, images, elapsed, frags
 
 
 def cpu_threads():
@@ -133,7 +136,11 @@ def oracle_results(streams, threads):
     for s in sample:
         oracle_stream(s)
     one_core = time.perf_counter() - t0
-    return [o[0] for o in outs], wall, len(sample), one_core
+    frags = {}
+    for o in outs:
+        for k, v in o[4].items():
+            frags[k] = frags.get(k, 0) + v
+    return [o[0] for o in outs], wall, len(sample), one_core, frags
 
 
 def exact_frames(procs, refs):
@@ -271,7 +278,7 @@ def config_rows(rdp_mi, pending, args):
         state = (p.read_rdram(), p.read_hidden_rdram(), p.read_tmem())
         rec.close()
         p.close()
-        ref, ref_digests, ref_images, cpu_s = pending[name].result()
+        ref, ref_digests, ref_images, cpu_s, _ = pending[name].result()
         final_ok = all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(state, ref))
         image_ok = name != "c2" or (ref_images and last_image is not None and
                                     np.array_equal(last_image, ref_images[-1]) and np.array_equal(img, ref_images[-1]))
@@ -338,7 +345,7 @@ def main():
     log(f"[bench] {len(streams)} streams, {total_tris} triangles per step")
 
     threads = cpu_threads()
-    refs, cpu_wall, sample_n, sample_s = oracle_results(streams, threads)
+    refs, cpu_wall, sample_n, sample_s, step_fragments = oracle_results(streams, threads)
     log(f"[bench] oracle: {cpu_wall:.2f}s for {len(streams)} frames on {threads} threads; "
         f"{sample_s:.2f}s for {sample_n} frames on one")
     # The per-config oracle runs (long sequential streams) go on in the
@@ -439,6 +446,27 @@ def main():
                     "salu_insts_per_launch": prof.get("salu_insts_per_launch"),
                     "source": "committed SQ_INSTS_VALU count per launch (profiles/k_raster_batch64_traffic.json)"}
 
+    # Algorithmic VALU work of a step: O = sum over render modes of the step's
+    # fragments (the oracle's count) x K_mode, the lane-ops per covered
+    # fragment of the mode measured on fully covered tiles (committed
+    # rocprofv3 calibration, profiles/valu_per_fragment.json).  O over the
+    # step time against the VALU peak is the share of the peak that covered
+    # fragments account for; valu_issue above counts every issued lane.
+    valu_alg = None
+    k_path = os.path.join(ROOT, "profiles", "valu_per_fragment.json")
+    if os.path.exists(k_path):
+        with open(k_path) as fh:
+            ktab = json.load(fh)
+        ops = sum(step_fragments.get(m, 0) * v["k_lane_ops_per_fragment"] for m, v in ktab.items())
+        unpriced = {m: n for m, n in step_fragments.items() if n and m not in ktab}
+        rate = ops / (ms_per_step / 1e3)
+        valu_alg = {"ops_per_step": round(ops), "achieved": round(rate / 1e12, 3), "peak": VALU_PEAK_TOPS,
+                    "unit": "T lane-ops/s", "frac": round(rate / 1e12 / VALU_PEAK_TOPS, 4),
+                    "fragments_per_step": step_fragments, "unpriced_fragments": unpriced or None,
+                    "k_lane_ops_per_fragment": {m: v["k_lane_ops_per_fragment"] for m, v in ktab.items()},
+                    "source": "fragments: the oracle's per-mode count over the step's streams; K: committed "
+                              "SQ_INSTS_VALU calibration on fully covered tiles (profiles/valu_per_fragment.json)"}
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = {"value": round(frames_per_step / cpu_wall, 3), "unit": "frames/s", "cores": threads, "kind": "port",
@@ -484,6 +512,7 @@ def main():
                      "kernel": dom["kernel"], "bytes_per_launch": dom["bytes"], "tiles_per_launch": dom["tiles"],
                      "ms_per_launch": round(dom["ms"], 4), "timing": "the launch alone, HIP events on its stream",
                      "valu_issue": valu,
+                     "valu_algorithmic": valu_alg,
                      "launches": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in l.items()}
                                   for l in launches]},
         "cpu_baseline": cpu,

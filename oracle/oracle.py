@@ -75,6 +75,7 @@ def _load():
     lib.rdpo_scanout.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
     lib.rdpo_fragment_count.restype = u64
     lib.rdpo_fragment_count.argtypes = [vp]
+    lib.rdpo_fragment_counts.argtypes = [vp, vp]
     lib.rdpo_primitive_count.restype = u64
     lib.rdpo_primitive_count.argtypes = [vp]
     lib.rdpo_state_digest.restype = u64
@@ -165,6 +166,14 @@ class Oracle:
     @property
     def fragments(self) -> int:
         return int(self._lib.rdpo_fragment_count(self._h))
+
+    FRAGMENT_MODES = ("fill", "copy", "1cycle", "1cycle_tex", "2cycle")
+
+    def fragments_by_mode(self) -> dict:
+        """Fragments split by the draw's mode (FRAGMENT_MODES)."""
+        out = np.zeros(len(self.FRAGMENT_MODES), dtype=np.uint64)
+        self._lib.rdpo_fragment_counts(self._h, out.ctypes.data)
+        return {m: int(v) for m, v in zip(self.FRAGMENT_MODES, out)}
 
     @property
     def primitives(self) -> int:

@@ -201,6 +201,7 @@ struct rdpo {
 
 	uint64_t prim_counter;
 	uint64_t fragments;
+	uint64_t fragments_mode[RDPO_FRAGMENT_MODES]; /* by rdpo_fragment_counts' modes */
 
 	uint32_t vi_regs[14];
 
@@ -430,6 +431,10 @@ uint8_t *rdpo_hidden_rdram(rdpo *o) { return o->hidden; }
 uint16_t *rdpo_tmem(rdpo *o) { return o->tmem; }
 uint32_t rdpo_rdram_size(const rdpo *o) { return o->rdram_size; }
 uint64_t rdpo_fragment_count(const rdpo *o) { return o->fragments; }
+void rdpo_fragment_counts(const rdpo *o, uint64_t out[RDPO_FRAGMENT_MODES])
+{
+	memcpy(out, o->fragments_mode, sizeof(o->fragments_mode));
+}
 uint64_t rdpo_primitive_count(const rdpo *o) { return o->prim_counter; }
 
 /* ------------------------------------------------------------------------ */
@@ -2624,6 +2629,15 @@ static pixel_state_t *pass_pixel(rdpo *o, uint32_t fmt, int x, int y, uint32_t c
 	return p;
 }
 
+/* The mode a fragment is counted under (rdpo_fragment_counts). */
+static int fragment_mode(const draw_t *d)
+{
+	if (d->ras_flags & RAS_FILL) return 0;
+	if (d->ras_flags & RAS_COPY) return 1;
+	if (d->ras_flags & RAS_MULTI_CYCLE) return 4;
+	return (d->uses_texel0 || d->uses_texel1) ? 3 : 2;
+}
+
 static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 {
 	uint32_t prim_index = (uint32_t)o->prim_counter++;
@@ -2707,6 +2721,7 @@ static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 			if (!shade_pixel(o, &pc, x, y, &sh))
 				continue;
 			o->fragments++;
+			o->fragments_mode[fragment_mode(&d)]++;
 			uint32_t ci = addr_index + o->fb_width * (uint32_t)y + (uint32_t)x;
 			uint32_t di = depth_index + o->fb_width * (uint32_t)y + (uint32_t)x;
 			pixel_state_t *p = pass_pixel(o, fmt, x, y, ci, di);

@@ -98,6 +98,10 @@ void rdpo_end_vi_register_per_scanline(rdpo *o);
 /* Instrumentation: number of (pixel, primitive) fragments that reached the
  * depth/blend stage, and number of draw commands seen. */
 uint64_t rdpo_fragment_count(const rdpo *o);
+/* The same count split by the draw's mode: 0 fill, 1 copy, 2 one-cycle
+ * without texels, 3 one-cycle with texels, 4 two-cycle. */
+#define RDPO_FRAGMENT_MODES 5
+void rdpo_fragment_counts(const rdpo *o, uint64_t out[RDPO_FRAGMENT_MODES]);
 uint64_t rdpo_primitive_count(const rdpo *o);
 
 /* Digest of RDRAM, hidden RDRAM and TMEM with the definition the library's

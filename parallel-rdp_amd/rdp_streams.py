@@ -847,6 +847,84 @@ def batch_stream(index: int, seed_base: int = 0x52449000, modes: Sequence[str] =
                   height=height, color_size=size, num_triangles=tris)
 
 
+DENSE_MODES = ("fill", "1cycle", "1cycle_tex", "2cycle")
+
+
+def dense_stream(mode: str, width: int = 640, height: int = 480, layers: int = 16, seed: int = 0x5244D000) -> Stream:
+    """Per-fragment cost calibration: frame 0 clears Z (and colour); frame 1
+    draws `layers` primitives that each cover every pixel of the framebuffer,
+    nearer one after the other, in one of the batch workloads' render modes
+    (fill rectangles; the c0 shaded, c1 textured or c2 two-cycle states with
+    one triangle per layer whose vertices lie outside the screen).  Every
+    8x8 tile of frame 1 is fully covered by every layer, so the raster
+    launch's instruction count over its fragments is the mode's cost per
+    covered fragment with no idle lanes."""
+    rng = np.random.default_rng(seed)
+    size = SIZ_32 if mode == "1cycle_tex" else SIZ_16
+    rdram = None
+    cl = CommandList()
+    _clear_passes(cl, width, height, size, clear_color=True)
+    cl.sync_full()
+    frames = [Frame(cl.words())]
+    cl = CommandList()
+    xy = np.tile(np.array([[0, 0], [width * 8, 0], [0, height * 8]], dtype=np.int64), (layers, 1, 1))
+    z = np.repeat((0x30000 - 0x800 * np.arange(layers, dtype=np.float64))[:, None], 3, axis=1)
+    rgba = rng.integers(32, 256, size=(layers, 3, 4)).astype(np.float64)
+    if mode == "fill":
+        cl.set_other_modes(OtherModes(cycle_type=3))
+        for k in range(layers):
+            cl.set_fill_color(int(rng.integers(0, 1 << 32, dtype=np.int64)))
+            cl.fill_rectangle(0, 0, (width - 1) << 2, (height - 1) << 2)
+        n = layers
+    elif mode == "1cycle":
+        cl.set_other_modes(OtherModes(cycle_type=0, depth_test=True, depth_write=True,
+                                      blend=((0, 0, 1, 0), (0, 0, 1, 0))))
+        cl.set_combine(*SHADE_COMBINE)
+        tris = triangle_commands(xy, rgba=rgba, z=z)
+        cl.extend_array(tris)
+        n = int(tris.shape[0])
+    elif mode == "1cycle_tex":
+        rdram = np.zeros(RDRAM_SIZE // 4, dtype=np.uint32)
+        rdram_write16(rdram, TEXTURE_ADDR, _random_texture_rgba16(rng, 32, 32))
+        cl.set_other_modes(OtherModes(cycle_type=0, perspective=True, sample_quad=True, bilerp0=True, bilerp1=True,
+                                      depth_test=True, depth_write=True, force_blend=True,
+                                      blend=((0, 0, 1, 0), (0, 0, 1, 0))))
+        cl.set_combine(*TEX_SHADE_COMBINE)
+        cl.set_texture_image(TEXTURE_ADDR, 32, SIZ_16, FMT_RGBA)
+        cl.set_tile(7, FMT_RGBA, SIZ_16, 0, 64)
+        cl.sync_load()
+        cl.load_tile(7, 0, 0, 31 << 2, 31 << 2)
+        cl.sync_tile()
+        cl.set_tile(0, FMT_RGBA, SIZ_16, 0, 64, mask_s=5, mask_t=5)
+        cl.set_tile_size(0, 0, 0, 31 << 2, 31 << 2)
+        uv = np.tile(np.array([[0.0, 0.0], [160.0, 0.0], [0.0, 120.0]]), (layers, 1, 1))
+        w = np.tile(np.array([1.0, 0.6, 0.8]), (layers, 1))
+        tris = triangle_commands(xy, rgba=rgba, z=z, uv=uv, w=w, perspective=True)
+        cl.extend_array(tris)
+        n = int(tris.shape[0])
+    elif mode == "2cycle":
+        rdram = np.zeros(RDRAM_SIZE // 4, dtype=np.uint32)
+        _write_ci_texture(rdram, rng, 0, SIZ_8)
+        cl.set_fog_color(0x80604020)
+        cl.set_env_color(0x80808080)
+        _emit_ci_texture(cl, 0, SIZ_8, palette=0)
+        cl.sync_pipe()
+        cl.set_other_modes(C2_MODES)
+        cl.set_combine(C2_COMBINE[0][0], C2_COMBINE[0][1], C2_COMBINE[1][0], C2_COMBINE[1][1])
+        uv = np.tile(np.array([[0.0, 0.0], [400.0, 0.0], [0.0, 300.0]]), (layers, 1, 1))
+        w = np.tile(np.array([1.0, 0.6, 0.8]), (layers, 1))
+        tris = triangle_commands(xy, rgba=rgba, z=z, uv=uv, w=w, perspective=True, tile=0, max_level=3)
+        cl.extend_array(tris)
+        n = int(tris.shape[0])
+    else:
+        raise ValueError(mode)
+    cl.sync_full()
+    frames.append(Frame(cl.words()))
+    return Stream(name=f"dense:{mode}:{width}x{height}", seed=seed, frames=frames, rdram_init=rdram, hidden_init=0,
+                  color_addr=COLOR_ADDR, depth_addr=DEPTH_ADDR, width=width, height=height, color_size=size,
+                  num_triangles=n)
+
+
 def batch_streams(count: int = 64, **kw) -> List[Stream]:
     return [batch_stream(i, **kw) for i in range(count)]
 

@@ -149,9 +149,12 @@ def _per_dispatch(path: str, counter: str, kernel_filter: str = ""):
 
 
 def short_name(kname: str) -> str:
-    """'void rdp_mi::k_raster<0u>(...)' -> 'k_raster<0>'."""
+    """'void rdp_mi::k_raster<0u, false>(...)' -> 'k_raster<0>' (8x8 tiles);
+    'void rdp_mi::k_raster<0u, true>(...)' -> 'k_raster<0> 16x16'."""
     n = kname.split("(")[0].replace("void ", "").replace("rdp_mi::", "")
-    return n.replace("u>", ">")
+    quad = ", true>" in n
+    n = n.replace(", false>", ">").replace(", true>", ">").replace("u>", ">")
+    return n + " 16x16" if quad else n
 
 
 def traffic(fetch_db, write_db, copyf_db, copyw_db, sq_db, copy_bytes, out, kernel_filter="k_raster"):

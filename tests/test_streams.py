@@ -1,5 +1,6 @@
 """Host-side stream generation: deterministic, well-formed command words."""
 import numpy as np
+import pytest
 
 import rdp_streams as rs
 
@@ -60,3 +61,24 @@ def test_triangle_setup_matches_reference_edge_rules():
 def test_zero_area_triangles_are_dropped():
     xy = np.array([[[0, 0], [10, 10], [20, 20]], [[0, 0], [40, 0], [0, 40]]])
     assert rs.triangle_commands(xy).shape[0] == 1
+
+
+@pytest.mark.parametrize("mode", rs.DENSE_MODES)
+def test_dense_streams_cover_every_pixel_once_per_layer(oracle_mod, mode):
+    """The per-fragment cost calibration counts frame 1 of a dense stream as
+    layers x width x height fragments, all in the stream's mode."""
+    s = rs.dense_stream(mode, width=160, height=120, layers=4)
+    o = oracle_mod.Oracle(rs.RDRAM_SIZE)
+    if s.rdram_init is not None:
+        o.rdram[:] = s.rdram_init
+    o.hidden[:] = s.hidden_init
+    o.process(s.frames[0].words)
+    o.flush()
+    before = o.fragments_by_mode()
+    o.process(s.frames[1].words)
+    o.flush()
+    after = o.fragments_by_mode()
+    o.close()
+    delta = {k: after[k] - before[k] for k in after}
+    assert delta[mode] == s.num_triangles * s.width * s.height
+    assert sum(delta.values()) == delta[mode]
