@@ -39,7 +39,7 @@ enum {
 };
 
 /* Flags accepted by rdp_mi_create (rdp_device.hpp:43-51).  Upscaling flags
- * are rejected with RDP_MI_ERROR_UNSUPPORTED. */
+ * are rejected with RDP_MI_ERROR_UNSUPPORTED (upscaled rendering is not built). */
 enum {
 	RDP_MI_FLAG_HOST_VISIBLE_HIDDEN_RDRAM = 1 << 0,
 	RDP_MI_FLAG_HOST_VISIBLE_TMEM = 1 << 1,
