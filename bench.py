@@ -366,16 +366,17 @@ def main():
     if rec.passes < len(streams):
         raise RuntimeError(f"the batched recording holds {rec.passes} passes for {len(streams)} streams")
     if args.no_autotune:
-        tuned = {"xcd_chunk": 64, "min_class": 0, "tile_size": 8, "ms": None}
+        tuned = {"xcd_chunk": 64, "min_class": 0, "tile_size": 8, "persistent": False, "ms": None}
     else:
         tuned = rec.autotune(3)
-        # The best time of each tile size among the candidates.
+        # The best time of each tile size and of the persistent round launch
+        # among the candidates.
         for cand in rec.autotune_results():
-            key = f"best_ms_tile{cand['tile_size']}"
+            key = "best_ms_persistent" if cand["persistent"] else f"best_ms_tile{cand['tile_size']}"
             tuned[key] = round(min(tuned.get(key, 1e9), cand["ms"]), 4)
     log(f"[bench] launch shape: {tuned}")
     # Immediate submissions of the same workload (plain path, serving) use it too.
-    rdp_mi.set_launch_config(tuned["xcd_chunk"], tuned["min_class"], tuned["tile_size"])
+    rdp_mi.set_launch_config(tuned["xcd_chunk"], tuned["min_class"], tuned["tile_size"], tuned["persistent"])
     rec.replay(max(args.warmup, 1), use_graph=True)
     t0 = time.perf_counter()
     step_ms = rec.replay(args.steps, use_graph=True)

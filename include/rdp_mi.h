@@ -274,24 +274,27 @@ int rdp_mi_recording_profile_raster(rdp_mi_recording *r, unsigned iterations, un
  * Times the recorded submission under every candidate raster launch shape:
  * the XCD chunk of the tile-to-workgroup mapping (-1 block order, 0 one
  * contiguous eighth per XCD, 32/64/128 tiles per chunk), the lowest
- * occupancy class passes run in (0-2) and the tile a workgroup takes
+ * occupancy class passes run in (0-2), the tile a workgroup takes
  * (tile_size 8: one 8x8 tile, one wave; 16: a 16x16 tile, one wave per 8x8
  * quadrant; the reference's binner anticipates both block sizes,
- * tile_binning_combined.comp:23); keeps the fastest for later replays and
- * reports it.  Each candidate is timed over `iterations` replays, best of two
- * sweeps. */
+ * tile_binning_combined.comp:23) and whether a round's raster work runs as
+ * one persistent launch (persistent = 1: resident one-wave workgroups stride
+ * over the round's 8x8 tiles, heaviest class first; rounds with top-class
+ * passes keep the per-class launches); keeps the fastest for later
+ * replays and reports it.  Each candidate is timed over `iterations`
+ * replays, best of two sweeps. */
 int rdp_mi_recording_autotune(rdp_mi_recording *r, unsigned iterations, int32_t *xcd_chunk, uint32_t *min_class,
-                              uint32_t *tile_size, float *best_ms);
+                              uint32_t *tile_size, int32_t *persistent, float *best_ms);
 /* The last autotune's candidates and their milliseconds per replay (no
  * reference counterpart): writes up to max entries, returns how many
  * candidates there are (0 before an autotune). */
 unsigned rdp_mi_recording_autotune_results(const rdp_mi_recording *r, unsigned max, int32_t *xcd_chunk,
-                                           uint32_t *min_class, uint32_t *tile_size, float *ms);
+                                           uint32_t *min_class, uint32_t *tile_size, int32_t *persistent, float *ms);
 /* Launch shape for immediate submissions (flush, batch flush, serving), for
  * example the one an autotuned recording of the same workload picked.
- * tile_size is 8 or 16 (RDP_MI_ERROR_INVALID_ARGUMENT otherwise).
- * Default: xcd_chunk 64, min_class 0, tile_size 8. */
-int rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class, uint32_t tile_size);
+ * tile_size is 8 or 16, and persistent needs 8 (RDP_MI_ERROR_INVALID_ARGUMENT
+ * otherwise).  Default: xcd_chunk 64, min_class 0, tile_size 8, persistent 0. */
+int rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class, uint32_t tile_size, int persistent);
 uint64_t rdp_mi_recording_raster_bytes(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_rounds(const rdp_mi_recording *r);
 unsigned rdp_mi_recording_passes(const rdp_mi_recording *r);

@@ -2154,13 +2154,11 @@ DEV void staging_sync()
 // span records they have in common.  Each wave walks its own quadrant's bin
 // masks and stages TMEM into its own LDS slice; waves never wait for each
 // other.  Otherwise a work item is one 8x8 tile and the workgroup one wave.
+// One work item: the whole tile (or, with QUAD, this wave's quadrant of it).
+// s_tmem_all: the workgroup's LDS staging area, 2048 halfwords per wave.
 template <uint32_t CLS, bool QUAD>
-__global__ __launch_bounds__(QUAD ? 256 : 64) __attribute__((amdgpu_waves_per_eu(
-	CLS == 0 ? RDP_RASTER_WAVES_CLASS0 : (CLS == 1 ? RDP_RASTER_WAVES_CLASS1 : RDP_RASTER_WAVES_CLASS2))))
-void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
+DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_t *s_tmem_all)
 {
-	__shared__ uint16_t s_tmem_all[(QUAD ? 4 : 1) * 2048];
-	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)];
 	const PassDesc &P = passes[item >> 14];
 	uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
 	const int lane = threadIdx.x & 63;
@@ -2279,6 +2277,17 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 	}
 }
 
+#define RDP_RASTER_WAVES(CLS) \
+	(CLS == 0 ? RDP_RASTER_WAVES_CLASS0 : (CLS == 1 ? RDP_RASTER_WAVES_CLASS1 : RDP_RASTER_WAVES_CLASS2))
+
+template <uint32_t CLS, bool QUAD>
+__global__ __launch_bounds__(QUAD ? 256 : 64) __attribute__((amdgpu_waves_per_eu(RDP_RASTER_WAVES(CLS))))
+void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
+{
+	__shared__ uint16_t s_tmem_all[(QUAD ? 4 : 1) * 2048];
+	raster_tile<CLS, QUAD>(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_tmem_all);
+}
+
 // Passes whose every primitive is a FILL-mode primitive (variant 1, the
 // first row of the variant table).  A fill writes every channel of a pixel
 // it covers and reads neither colour nor depth, so the tile's framebuffer is
@@ -2287,14 +2296,11 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 constexpr uint32_t kFillTiles = kFillTilesPerItem;
 using FillVariant = RasterVariant<RAS_FILL | RAS_COPY, RAS_FILL, (int)COMB_ANY, TEXFMT_ANY, (int)DBP_ANY>;
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
-void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
+DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t *s_unused)
 {
-	__shared__ uint16_t s_unused[2];
-	const uint32_t item = items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)];
 	const PassDesc &P = passes[item >> 14];
 	const uint32_t tx0 = ((item >> 7) & 0x7f) * kFillTiles, ty = item & 0x7f;
-	const int lane = threadIdx.x;
+	const int lane = threadIdx.x & 63;
 	const uint32_t fmt = P.fb_fmt;
 	const bool alias = P.fb_addr == P.depth_addr;
 	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
@@ -2358,6 +2364,40 @@ void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restri
 	}
 }
 
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
+{
+	__shared__ uint16_t s_unused[2];
+	fill_item(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_unused);
+}
+
+// One launch for every raster work item of a round (persistent): the grid
+// holds at most as many one-wave workgroups as the GPU keeps resident, and
+// workgroup b takes items b, b + grid, b + 2 grid, ... of the round's list:
+// first the heavier class's tiles (hi, the densest passes first), then the
+// lighter class's (lo), then the fill items.  (Taking items from one shared
+// counter instead serialised on the counter's atomics across the XCDs: three
+// times slower.)
+template <uint32_t CLS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RDP_RASTER_WAVES(CLS))))
+void k_raster_round(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items_hi, uint32_t n_hi,
+                    const uint32_t *__restrict__ items_lo, uint32_t n_lo, const uint32_t *__restrict__ fill_items,
+                    uint32_t n_fill)
+{
+	__shared__ uint16_t s_tmem[2048];
+	const uint32_t total = n_hi + n_lo + n_fill;
+	for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
+		if (k < n_hi)
+			raster_tile<CLS, false>(passes, items_hi[k], s_tmem);
+		else if (k < n_hi + n_lo)
+			raster_tile<CLS, false>(passes, items_lo[k - n_hi], s_tmem);
+		else
+			fill_item(passes, fill_items[k - n_hi - n_lo], s_tmem);
+		// The next item's staging must not overwrite texels this item's
+		// lanes still read.
+		__syncthreads();
+	}
+}
 // ==========================================================================
 // Launchers (host side).
 // ==========================================================================
@@ -2388,6 +2428,22 @@ hipError_t launch_raster_fill(const PassDesc *passes, const uint32_t *items, uin
 {
 	if (!n) return hipSuccess;
 	hipLaunchKernelGGL(k_raster_fill, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk);
+	return hipGetLastError();
+}
+
+// The persistent raster launch of a round (k_raster_round): cls is the
+// heavier class present (0 or 1), grid the number of one-wave workgroups.
+hipError_t launch_raster_round(uint32_t cls, const PassDesc *passes, const uint32_t *items_hi, uint32_t n_hi,
+                               const uint32_t *items_lo, uint32_t n_lo, const uint32_t *fill_items, uint32_t n_fill,
+                               uint32_t grid, hipStream_t s)
+{
+	if (!(n_hi + n_lo + n_fill)) return hipSuccess;
+	if (cls == 0)
+		hipLaunchKernelGGL(k_raster_round<0>, dim3(grid), dim3(64), 0, s, passes, items_hi, n_hi, items_lo, n_lo, fill_items,
+		                   n_fill);
+	else
+		hipLaunchKernelGGL(k_raster_round<1>, dim3(grid), dim3(64), 0, s, passes, items_hi, n_hi, items_lo, n_lo, fill_items,
+		                   n_fill);
 	return hipGetLastError();
 }
 

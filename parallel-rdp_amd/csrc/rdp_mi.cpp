@@ -44,6 +44,8 @@ hipError_t launch_tmem(const PassDesc *, const uint32_t *, uint32_t, const uint3
 hipError_t launch_bin(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, bool, hipStream_t);
 hipError_t launch_raster_fill(const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+hipError_t launch_raster_round(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t,
+                               const uint32_t *, uint32_t, uint32_t, hipStream_t);
 hipError_t upload_blender_lut(const uint8_t *lut);
 
 // Launch shape of the raster stage: the XCD chunk of the tile-to-block
@@ -54,8 +56,14 @@ hipError_t upload_blender_lut(const uint8_t *lut);
 struct LaunchConfig {
 	int32_t xcd_chunk = 64;
 	uint32_t min_class = 0;
-	bool quad = false;  // 16x16 tiles
+	bool quad = false;        // 16x16 tiles
+	bool persistent = false;  // one k_raster_round launch per round (8x8 tiles, classes 0-1)
 };
+
+// Workgroups of a persistent raster launch: one-wave workgroups the GPU
+// keeps resident at the raster's occupancy (7 waves per SIMD, 4 SIMDs per
+// CU); set from the device's CU count at context init.
+static uint32_t g_persistent_grid = 256 * 4 * 7;
 
 static thread_local std::string g_last_error;
 
@@ -252,6 +260,9 @@ struct Context : Lane {
 		int n = 0;
 		if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(RDP_MI_ERROR_DEVICE, "no HIP device");
 		HIP_CHECK(hipGetDevice(&device));
+		int cus = 0;
+		if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+			g_persistent_grid = (uint32_t)cus * 4 * 7;
 		if (int err = noalloc_init(device)) return err;
 		if (int err = create()) return err;
 		std::vector<uint8_t> lut(0x8000);
@@ -516,7 +527,12 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		uint32_t merged = 0, heavy = 0;
 		for (uint32_t c = 0; c <= lo; c++) merged += L.rn(c, cfg.quad);
 		for (uint32_t c = lo + 1; c < kRasterClasses; c++) heavy += L.rn(c, cfg.quad);
-		const bool fork_raster = big && merged && heavy;
+		// One persistent launch for the round's raster work (8x8 tiles), unless
+		// the round has top-class passes (the generic code keeps its own
+		// register budget) or the launches are being timed one by one.
+		const bool persist = cfg.persistent && !raster_events && L.raster_n[kRasterClasses - 1] == 0 &&
+		                     L.raster_n[0] + L.raster_n[1] + L.fill_n > 0;
+		const bool fork_raster = big && merged && heavy && !persist;
 		const RoundEvents E = overlap ? lane.events[r] : RoundEvents();
 
 		const bool early = fork_tmem && L.tmem_early && r > 0;
@@ -542,7 +558,14 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
 		}
 		if (overlap) HIP_CHECK(hipEventRecord(E.pre_raster, s));
-		if (fork_raster) {
+		if (persist) {
+			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
+			// Class 1's tiles first (the longest chains), class 0's, then the fills.
+			const uint32_t cls = (L.raster_n[1] || cfg.min_class >= 1) ? 1 : 0;
+			const uint32_t total = L.raster_n[0] + L.raster_n[1] + L.fill_n;
+			HIP_CHECK(launch_raster_round(cls, L.descs, L.raster_items[1], L.raster_n[1], L.raster_items[0], L.raster_n[0],
+			                              L.fill_items, L.fill_n, std::min(total, g_persistent_grid), s));
+		} else if (fork_raster) {
 			// The heavy launch needs setup, binning and the TMEM replay.
 			// With both forked, prep_done covers the first two and the side
 			// stream's own order the third; otherwise the main stream has
@@ -2337,7 +2360,7 @@ int rdp_mi_recording_replay(rdp_mi_recording *h, unsigned iterations, int use_gr
 }
 
 int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t *xcd_chunk, uint32_t *min_class,
-                              uint32_t *tile_size, float *best_ms)
+                              uint32_t *tile_size, int32_t *persistent, float *best_ms)
 {
 	if (!h || iterations == 0) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "bad argument");
 	Context &c = ctx();
@@ -2347,7 +2370,10 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	std::vector<LaunchConfig> cands;
 	for (bool quad : { false, true })
 		for (uint32_t m = 0; m < kRasterClasses; m++)
-			for (int32_t ch : chunks) cands.push_back(LaunchConfig{ ch, m, quad });
+			for (int32_t ch : chunks) cands.push_back(LaunchConfig{ ch, m, quad, false });
+	// The persistent round launch takes its items in order (no XCD chunks);
+	// the chunk still shapes the other launches.
+	for (uint32_t m = 0; m < 2; m++) cands.push_back(LaunchConfig{ 64, m, false, true });
 	std::vector<float> best(cands.size(), 1e30f);
 	hipEvent_t e0, e1;
 	HIP_CHECK(hipEventCreate(&e0));
@@ -2385,12 +2411,13 @@ int rdp_mi_recording_autotune(rdp_mi_recording *h, unsigned iterations, int32_t 
 	if (xcd_chunk) *xcd_chunk = r.cfg.xcd_chunk;
 	if (min_class) *min_class = r.cfg.min_class;
 	if (tile_size) *tile_size = r.cfg.quad ? 16 : 8;
+	if (persistent) *persistent = r.cfg.persistent ? 1 : 0;
 	if (best_ms) *best_ms = best[pick];
 	return 0;
 }
 
 unsigned rdp_mi_recording_autotune_results(const rdp_mi_recording *h, unsigned max, int32_t *xcd_chunk,
-                                           uint32_t *min_class, uint32_t *tile_size, float *ms)
+                                           uint32_t *min_class, uint32_t *tile_size, int32_t *persistent, float *ms)
 {
 	if (!h) return 0;
 	const auto &t = h->r.tuned;
@@ -2398,19 +2425,23 @@ unsigned rdp_mi_recording_autotune_results(const rdp_mi_recording *h, unsigned m
 		if (xcd_chunk) xcd_chunk[i] = t[i].first.xcd_chunk;
 		if (min_class) min_class[i] = t[i].first.min_class;
 		if (tile_size) tile_size[i] = t[i].first.quad ? 16 : 8;
+		if (persistent) persistent[i] = t[i].first.persistent ? 1 : 0;
 		if (ms) ms[i] = t[i].second;
 	}
 	return (unsigned)t.size();
 }
 
-int rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class, uint32_t tile_size)
+int rdp_mi_set_launch_config(int32_t xcd_chunk, uint32_t min_class, uint32_t tile_size, int persistent)
 {
 	if (tile_size != 8 && tile_size != 16) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "tile size %u (8 or 16)", tile_size);
+	if (persistent && tile_size != 8)
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "the persistent raster launch takes 8x8 tiles");
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	c.cfg.xcd_chunk = xcd_chunk;
 	c.cfg.min_class = std::min(min_class, kRasterClasses - 1);
 	c.cfg.quad = tile_size == 16;
+	c.cfg.persistent = persistent != 0;
 	return 0;
 }
 

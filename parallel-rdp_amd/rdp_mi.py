@@ -436,12 +436,12 @@ class ScanoutBuffer:
             pass
 
 
-def set_launch_config(xcd_chunk: int = 64, min_class: int = 0, tile_size: int = 8):
+def set_launch_config(xcd_chunk: int = 64, min_class: int = 0, tile_size: int = 8, persistent: bool = False):
     """Raster launch shape for immediate submissions (flush, batch flush, serving)."""
     lib = load_library()
     lib.rdp_mi_set_launch_config.restype = ctypes.c_int
-    lib.rdp_mi_set_launch_config.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
-    _check(lib.rdp_mi_set_launch_config(xcd_chunk, min_class, tile_size), "set_launch_config")
+    lib.rdp_mi_set_launch_config.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    _check(lib.rdp_mi_set_launch_config(xcd_chunk, min_class, tile_size, int(persistent)), "set_launch_config")
 
 
 def batch_flush(processors: Sequence[CommandProcessor]):
@@ -502,23 +502,27 @@ class Recording:
                                                         ctypes.POINTER(ctypes.c_int32),
                                                         ctypes.POINTER(ctypes.c_uint32),
                                                         ctypes.POINTER(ctypes.c_uint32),
+                                                        ctypes.POINTER(ctypes.c_int32),
                                                         ctypes.POINTER(ctypes.c_float)]
-        chunk, cls, tile, ms = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
+        chunk, cls, tile, pers, ms = (ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32(),
+                                      ctypes.c_float())
         _check(self._lib.rdp_mi_recording_autotune(self._h, iterations, ctypes.byref(chunk), ctypes.byref(cls),
-                                                   ctypes.byref(tile), ctypes.byref(ms)), "autotune")
-        return {"xcd_chunk": chunk.value, "min_class": cls.value, "tile_size": tile.value, "ms": ms.value}
+                                                   ctypes.byref(tile), ctypes.byref(pers), ctypes.byref(ms)),
+               "autotune")
+        return {"xcd_chunk": chunk.value, "min_class": cls.value, "tile_size": tile.value,
+                "persistent": bool(pers.value), "ms": ms.value}
 
     def autotune_results(self) -> list:
         """Every candidate of the last autotune with its ms per replay."""
         f = self._lib.rdp_mi_recording_autotune_results
         f.restype = ctypes.c_uint
-        f.argtypes = [ctypes.c_void_p, ctypes.c_uint] + [ctypes.c_void_p] * 4
-        n = int(f(self._h, 0, None, None, None, None))
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint] + [ctypes.c_void_p] * 5
+        n = int(f(self._h, 0, None, None, None, None, None))
         chunk, cls, tile = (np.zeros(n, np.int32), np.zeros(n, np.uint32), np.zeros(n, np.uint32))
-        ms = np.zeros(n, np.float32)
-        f(self._h, n, chunk.ctypes.data, cls.ctypes.data, tile.ctypes.data, ms.ctypes.data)
-        return [{"xcd_chunk": int(a), "min_class": int(b), "tile_size": int(c), "ms": float(d)}
-                for a, b, c, d in zip(chunk, cls, tile, ms)]
+        pers, ms = np.zeros(n, np.int32), np.zeros(n, np.float32)
+        f(self._h, n, chunk.ctypes.data, cls.ctypes.data, tile.ctypes.data, pers.ctypes.data, ms.ctypes.data)
+        return [{"xcd_chunk": int(a), "min_class": int(b), "tile_size": int(c), "persistent": bool(p),
+                 "ms": float(d)} for a, b, c, p, d in zip(chunk, cls, tile, pers, ms)]
 
     def profile(self, iterations: int = 1) -> np.ndarray:
         ms = np.zeros(4, dtype=np.float32)

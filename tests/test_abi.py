@@ -84,8 +84,9 @@ def test_server_rejects_bad_arguments_without_touching_the_device(lib):
 
 def test_launch_config_accepts_any_shape_without_a_device(lib):
     lib.rdp_mi_set_launch_config.restype = ctypes.c_int
-    lib.rdp_mi_set_launch_config.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
-    for chunk, cls, tile in ((-1, 0, 8), (0, 1, 16), (128, 2, 8), (64, 9, 16)):
-        assert lib.rdp_mi_set_launch_config(chunk, cls, tile) == 0
-    assert lib.rdp_mi_set_launch_config(64, 0, 12) != 0
-    assert lib.rdp_mi_set_launch_config(64, 0, 8) == 0
+    lib.rdp_mi_set_launch_config.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    for chunk, cls, tile, pers in ((-1, 0, 8, 0), (0, 1, 16, 0), (128, 2, 8, 1), (64, 9, 16, 0)):
+        assert lib.rdp_mi_set_launch_config(chunk, cls, tile, pers) == 0
+    assert lib.rdp_mi_set_launch_config(64, 0, 12, 0) != 0
+    assert lib.rdp_mi_set_launch_config(64, 0, 16, 1) != 0
+    assert lib.rdp_mi_set_launch_config(64, 0, 8, 0) == 0

@@ -141,22 +141,23 @@ def test_batch_flush_many_processors(rdp_mi):
         p.close()
 
 
-@pytest.mark.parametrize("xcd_chunk,min_class,tile_size", [(-1, 0, 8), (0, 1, 8), (32, 2, 8), (128, 0, 8), (64, 1, 8),
-                                                            (-1, 0, 16), (0, 1, 16), (32, 2, 16), (128, 0, 16),
-                                                            (64, 1, 16), (64, 2, 16)])
-def test_launch_shapes_bit_exact(rdp_mi, xcd_chunk, min_class, tile_size):
+@pytest.mark.parametrize("xcd_chunk,min_class,tile_size,persistent",
+                         [(-1, 0, 8, 0), (0, 1, 8, 0), (32, 2, 8, 0), (128, 0, 8, 0), (64, 1, 8, 0),
+                          (-1, 0, 16, 0), (0, 1, 16, 0), (32, 2, 16, 0), (128, 0, 16, 0), (64, 1, 16, 0), (64, 2, 16, 0),
+                          (64, 0, 8, 1), (64, 1, 8, 1), (64, 2, 8, 1)])
+def test_launch_shapes_bit_exact(rdp_mi, xcd_chunk, min_class, tile_size, persistent):
     """Every raster launch shape the autotuner can pick gives the oracle's bytes:
     batch streams of all three workloads plus random-state streams, submitted
     together."""
     streams = [rs.batch_stream(i, max_tris=800) for i in range(6)] + [rs.fuzz_stream(2000 + i) for i in range(6)]
-    say(f"[case] launch shape xcd_chunk={xcd_chunk} min_class={min_class} tile_size={tile_size}")
-    rdp_mi.set_launch_config(xcd_chunk, min_class, tile_size)
+    say(f"[case] launch shape xcd_chunk={xcd_chunk} min_class={min_class} tile_size={tile_size} persistent={persistent}")
+    rdp_mi.set_launch_config(xcd_chunk, min_class, tile_size, bool(persistent))
     try:
         procs = _procs_for(rdp_mi, streams)
         rdp_mi.batch_flush(procs)
         for p, s in zip(procs, streams):
             p.idle()
-            assert_same(_state(p), harness.run_oracle(s), f"{s.name} ({xcd_chunk}, {min_class}, {tile_size})")
+            assert_same(_state(p), harness.run_oracle(s), f"{s.name} ({xcd_chunk}, {min_class}, {tile_size}, {persistent})")
             p.close()
     finally:
         rdp_mi.set_launch_config(64, 0, 8)
@@ -201,6 +202,28 @@ def test_trial_runs_leave_state_unchanged(rdp_mi):
     rec.replay(1, use_graph=True)
     for p, s, r in zip(procs, streams, refs):
         assert_same(_state(p), r, f"{s.name} after one replay")
+    rec.close()
+    for p in procs:
+        p.close()
+
+
+def test_persistent_round_launch_graph_replays(rdp_mi):
+    """A recording made under the persistent round launch, replayed three
+    times as one captured graph: every replay leaves the oracle's bytes (the
+    batch frames clear first, so replays are idempotent)."""
+    streams = [rs.batch_stream(i, max_tris=1500) for i in range(10)]
+    refs = [harness.run_oracle(s) for s in streams]
+    say("[case] persistent round launch, graph replays")
+    rdp_mi.set_launch_config(64, 0, 8, True)
+    try:
+        procs = _procs_for(rdp_mi, streams)
+        rec = rdp_mi.Recording(procs, batched=True)
+    finally:
+        rdp_mi.set_launch_config(64, 0, 8)
+    for it in range(3):
+        rec.replay(1, use_graph=True)
+        for p, s, r in zip(procs, streams, refs):
+            assert_same(_state(p), r, f"{s.name} persistent graph replay {it}")
     rec.close()
     for p in procs:
         p.close()
