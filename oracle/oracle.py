@@ -41,12 +41,25 @@ class ScanoutOptions(ctypes.Structure):
         return o
 
 
-def _load():
+_other_libs = {}
+
+
+def _load(path: "str | None" = None):
+    """The built oracle library, or with path another build of the same
+    sources (the planted-mistake builds of the pin tests)."""
     global _lib
+    if path is not None:
+        if path not in _other_libs:
+            _other_libs[path] = _bind(ctypes.CDLL(path))
+        return _other_libs[path]
     if _lib is not None:
         return _lib
     build()
-    lib = ctypes.CDLL(_LIB_PATH)
+    _lib = _bind(ctypes.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _bind(lib):
     vp, u32, u64, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t
     lib.rdpo_create.restype = vp
     lib.rdpo_create.argtypes = [u32]
@@ -82,18 +95,23 @@ def _load():
     lib.rdpo_state_digest.argtypes = [vp]
     lib.rdpo_command_length.restype = ctypes.c_uint
     lib.rdpo_command_length.argtypes = [u32]
+    lib.rdpo_enable_upscaling.restype = ctypes.c_int
+    lib.rdpo_enable_upscaling.argtypes = [vp, ctypes.c_uint]
+    lib.rdpo_upscaled_rdram.restype = vp
+    lib.rdpo_upscaled_rdram.argtypes = [vp]
+    lib.rdpo_upscaled_hidden_rdram.restype = vp
+    lib.rdpo_upscaled_hidden_rdram.argtypes = [vp]
     lib.rdpo_blender_divider_table.argtypes = [vp]
     lib.rdpo_gamma_table.argtypes = [vp]
     lib.rdpo_perspective_table.argtypes = [vp]
-    _lib = lib
     return lib
 
 
 class Oracle:
     """One emulated RDP: RDRAM, hidden RDRAM, TMEM and register state."""
 
-    def __init__(self, rdram_size: int = 8 << 20):
-        self._lib = _load()
+    def __init__(self, rdram_size: int = 8 << 20, lib_path: "str | None" = None):
+        self._lib = _load(lib_path)
         self._h = self._lib.rdpo_create(rdram_size)
         if not self._h:
             raise RuntimeError("rdpo_create failed")
@@ -105,11 +123,28 @@ class Oracle:
         tbuf = (ctypes.c_uint16 * 2048).from_address(self._lib.rdpo_tmem(self._h))
         self.tmem = np.frombuffer(tbuf, dtype=np.uint16)
 
+    def enable_upscaling(self, factor: int):
+        """Render qualifying passes a second time at factor (2, 4 or 8) times
+        the resolution (before the first command).  upscaled_rdram /
+        upscaled_hidden then view the factor^2 sample planes: (planes,
+        rdram_size // 4) words and (planes, rdram_size // 2) bytes, plane
+        sy * factor + sx holding sample (sx, sy) of each pixel."""
+        if self._lib.rdpo_enable_upscaling(self._h, factor) != 0:
+            raise ValueError(f"upscaling factor {factor} refused")
+        planes = factor * factor
+        n = self.rdram_size
+        buf = (ctypes.c_uint8 * (planes * n)).from_address(self._lib.rdpo_upscaled_rdram(self._h))
+        self.upscaled_rdram = np.frombuffer(buf, dtype=np.uint32).reshape(planes, n // 4)
+        hbuf = (ctypes.c_uint8 * (planes * (n // 2))).from_address(self._lib.rdpo_upscaled_hidden_rdram(self._h))
+        self.upscaled_hidden = np.frombuffer(hbuf, dtype=np.uint8).reshape(planes, n // 2)
+        self.upscaling = factor
+
     def close(self):
         if self._h:
             self._lib.rdpo_destroy(self._h)
             self._h = None
             self.rdram = self.hidden = self.tmem = None
+            self.upscaled_rdram = self.upscaled_hidden = None
 
     def __del__(self):
         try:
@@ -134,7 +169,7 @@ class Oracle:
     def scanout(self, options: "ScanoutOptions | None" = None):
         """VI scanout: (height, width) uint32 RGBA8 words (R in the low byte)."""
         opts = options if options is not None else ScanoutOptions.default()
-        out = np.zeros(640 * 576, dtype=np.uint32)
+        out = np.zeros(640 * 576 * getattr(self, "upscaling", 1) ** 2, dtype=np.uint32)
         w = ctypes.c_uint(0)
         h = ctypes.c_uint(0)
         self._lib.rdpo_scanout_ex(self._h, ctypes.byref(opts), out.ctypes.data, ctypes.byref(w), ctypes.byref(h))

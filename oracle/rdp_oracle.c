@@ -19,9 +19,10 @@
 #include <string.h>
 
 /* A build with -DRDPO_PLANT=n carries exactly one deliberate mistake, the
- * one marked PLANTED(n) below.  tests/test_oracle_pins.py builds such
- * variants to show that the pin of each function fails on a mistake in that
- * function.  The normal build (RDPO_PLANT 0) has none of them. */
+ * one marked PLANTED(n) below (or in vi_oracle.c).  tests/test_oracle_pins.py
+ * and tests/test_oracle_upscale.py build such variants to show that the pin
+ * of each function fails on a mistake in that function.  The normal build
+ * (RDPO_PLANT 0) has none of them. */
 #ifndef RDPO_PLANT
 #define RDPO_PLANT 0
 #endif
@@ -90,6 +91,9 @@ enum { FB_I4 = 0, FB_I8 = 1, FB_RGBA5551 = 2, FB_IA88 = 3, FB_RGBA8888 = 4 };
 enum { TF_RGBA = 0, TF_YUV = 1, TF_CI = 2, TF_IA = 3, TF_I = 4 };
 enum { UP_TILE = 0, UP_TLUT = 1, UP_BLOCK = 2 };
 enum { TILE_CLAMP_S = 1, TILE_MIRROR_S = 2, TILE_CLAMP_T = 4, TILE_MIRROR_T = 8 };
+
+#define MAX_FB_LINES 1024
+#define MAX_FB_WIDTH 1024
 
 #define COVERAGE_FILL 0x40
 #define COVERAGE_COPY 0x20
@@ -216,6 +220,26 @@ struct rdpo {
 	uint8_t *px_loaded;
 	uint32_t *px_touched;
 	size_t num_touched;
+
+	/* Upscaled domain (rdpo_enable_upscaling; the reference's
+	 * Renderer::init_internal_upscaling_factor, rdp_renderer.cpp:545-601):
+	 * up_log2 > 0 renders every pass whose framebuffer is RGBA5551 or
+	 * RGBA8888 and at least 256 pixels wide a second time at 2^up_log2 times
+	 * the resolution.  Sample (sx, sy) of a native pixel lives in plane
+	 * sy * S + sx of up_rdram (planes of rdram_size bytes) and up_hidden
+	 * (planes of rdram_size / 2); up_ref is the native RDRAM the domain last
+	 * agreed with.  upx[native key] holds the open pass's S x S sample states
+	 * of a native pixel, allocated on first touch. */
+	uint32_t up_log2;
+	uint8_t *up_rdram, *up_hidden, *up_ref;
+	struct pixel_state **upx;
+	uint32_t *upx_touched;
+	size_t upx_num;
+	/* Draws of the open pass, kept with the state they were drawn with
+	 * (tile descriptors and TMEM included) when the pass qualifies for the
+	 * upscaled render; replayed at scale when the pass ends. */
+	struct up_draw *up_draws;
+	unsigned up_num;
 
 	/* Video interface state (frame counter, previous image, per-scanline
 	 * register latches). */
@@ -412,16 +436,53 @@ rdpo *rdpo_create(uint32_t rdram_size)
 	return o;
 }
 
+static void upx_release(rdpo *o);
+
+int rdpo_enable_upscaling(rdpo *o, unsigned factor)
+{
+	unsigned l = factor == 2 ? 1 : factor == 4 ? 2 : factor == 8 ? 3 : 0;
+	if (!l || o->up_log2 || o->prim_counter)
+		return -1;
+	size_t planes = (size_t)1 << (2 * l);
+	o->up_rdram = (uint8_t *)calloc(planes, o->rdram_size);
+	o->up_hidden = (uint8_t *)malloc(planes * (o->rdram_size / 2));
+	o->up_ref = (uint8_t *)calloc(o->rdram_size, 1);
+	o->upx = (struct pixel_state **)calloc((size_t)MAX_FB_LINES * MAX_FB_WIDTH, sizeof(*o->upx));
+	o->upx_touched = (uint32_t *)malloc(sizeof(uint32_t) * MAX_FB_LINES * MAX_FB_WIDTH);
+	if (!o->up_rdram || !o->up_hidden || !o->up_ref || !o->upx || !o->upx_touched)
+		abort();
+	/* The upscaled hidden RDRAM starts at 0x03 like the native one
+	 * (rdp_renderer.cpp:588-593); the domain and its reference start equal
+	 * to the (zeroed) native RDRAM. */
+	memset(o->up_hidden, 0x03, planes * (o->rdram_size / 2));
+	o->up_log2 = l;
+	return 0;
+}
+
+uint8_t *rdpo_upscaled_rdram(rdpo *o) { return o->up_rdram; }
+uint8_t *rdpo_upscaled_hidden_rdram(rdpo *o) { return o->up_hidden; }
+unsigned rdpo_upscaling_factor(const rdpo *o) { return 1u << o->up_log2; }
+
 void rdpo_destroy(rdpo *o)
 {
 	if (!o)
 		return;
+	if (o->upx)
+		upx_release(o);
+	free(o->up_draws);
+	free(o->upx);
+	free(o->upx_touched);
+	free(o->up_rdram);
+	free(o->up_hidden);
+	free(o->up_ref);
 	free(o->rdram);
 	free(o->hidden);
 	free(o->px);
 	free(o->px_loaded);
 	free(o->px_touched);
 	free(o->pass_log);
+	if (o->vi)
+		vi_state_free(o->vi);
 	free(o->vi);
 	free(o);
 }
@@ -438,7 +499,9 @@ void rdpo_fragment_counts(const rdpo *o, uint64_t out[RDPO_FRAGMENT_MODES])
 uint64_t rdpo_primitive_count(const rdpo *o) { return o->prim_counter; }
 
 /* ------------------------------------------------------------------------ */
-/* Span setup: span_setup.comp:87-227 (SCALING_FACTOR = 1).                  */
+/* Span setup: span_setup.comp:87-227.  L is SCALING_LOG2: 0 for the native */
+/* pass, log2 of the upscaling factor for the upscaled pass, where y counts */
+/* upscaled lines.                                                          */
 /* ------------------------------------------------------------------------ */
 static int32_t quantize_x(int32_t x)
 {
@@ -447,8 +510,16 @@ static int32_t quantize_x(int32_t x)
 	return (x >> 12) | sticky;
 }
 
-static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t scissor[4], int y, span_t *s)
+/* interpolate_snapped (span_setup.comp:80-85): whole native lines step by
+ * dvalue, the remaining sub-lines by dvalue >> L. */
+static inline int32_t interpolate_snapped(int32_t dvalue, int dy, int L)
 {
+	return (dy >> L) * dvalue + (dy & ((1 << L) - 1)) * (dvalue >> L);
+}
+
+static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t scissor[4], int y, span_t *s, int L)
+{
+	const int S = 1 << L;
 	/* scissor: xlo, ylo, xhi, yhi */
 	int flip = (setup->flags & TRI_FLIP) != 0;
 	int interlace_en = (setup->flags & TRI_INTERLACE_FIELD) != 0;
@@ -458,12 +529,12 @@ static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t sc
 	{
 		int do_offset = (setup->flags & TRI_DO_OFFSET) != 0;
 		int skip_xfrac = (setup->flags & TRI_SKIP_XFRAC) != 0;
-		int y_base = setup->yh >> 2;
+		int y_base = (setup->yh >> 2) * S;
 		int dy = y - y_base;
-		int32_t xh = setup->xh + dy * (setup->dxhdy << 2);
+		int32_t xh = setup->xh * S + dy * (setup->dxhdy << 2);
 		int32_t drgba_diff[4] = { 0, 0, 0, 0 }, dstzw_diff[4] = { 0, 0, 0, 0 };
 		if (do_offset) {
-			xh += 3 * setup->dxhdy;
+			xh += (S * 3) * setup->dxhdy;
 			for (int i = 0; i < 4; i++) {
 				int32_t deh = attr->drgba_de[i] & ~0x1ff;
 				int32_t dyh = attr->drgba_dy[i] & ~0x1ff;
@@ -476,32 +547,34 @@ static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t sc
 		int32_t base_x = xh >> 15;
 		int32_t xfrac = skip_xfrac ? 0 : ((xh >> (PLANTED(3) ? 8 : 7)) & 0xff);
 		for (int i = 0; i < 4; i++) {
-			int32_t v = attr->rgba[i] + dy * attr->drgba_de[i];
-			s->rgba[i] = ((v & ~0x1ff) + drgba_diff[i] - xfrac * ((attr->drgba_dx[i] >> 8) & ~1)) & ~0x3ff;
-			v = attr->stzw[i] + dy * attr->dstzw_de[i];
-			s->stzw[i] = ((v & ~0x1ff) + dstzw_diff[i] - xfrac * ((attr->dstzw_dx[i] >> 8) & ~1)) & ~0x3ff;
+			int32_t v = attr->rgba[i] + interpolate_snapped(attr->drgba_de[i], dy, L);
+			s->rgba[i] = ((v & ~0x1ff) + drgba_diff[i] - interpolate_snapped((attr->drgba_dx[i] >> 8) & ~1, xfrac, L)) &
+			             ~0x3ff;
+			v = attr->stzw[i] + interpolate_snapped(attr->dstzw_de[i], dy, L);
+			s->stzw[i] = ((v & ~0x1ff) + dstzw_diff[i] - interpolate_snapped((attr->dstzw_dx[i] >> 8) & ~1, xfrac, L)) &
+			             ~0x3ff;
 		}
 		s->base_x = base_x;
 	}
 
 	/* X extents at the four sub-scanlines (span_setup.comp:151-225). */
-	int yh_base = PLANTED(2) ? setup->yh : (setup->yh & ~3);
-	int ym_base = setup->ym;
-	int ylo = imax(setup->yh, (int)scissor[1]);
-	int yhi = imin(setup->yl, (int)scissor[3]);
+	int yh_base = (PLANTED(2) ? setup->yh : (setup->yh & ~3)) * S;
+	int ym_base = setup->ym * S;
+	int ylo = imax(setup->yh, (int)scissor[1]) * S;
+	int yhi = imin(setup->yl, (int)scissor[3]) * S;
 	int32_t xleft[4], xright[4], invalid[4];
-	int32_t lo_sc = (int32_t)(scissor[0] << 1), hi_sc = (int32_t)(scissor[2] << 1);
+	int32_t lo_sc = S * (int32_t)(scissor[0] << 1), hi_sc = S * (int32_t)(scissor[2] << 1);
 	int all_over = 1, all_under = 1, all_invalid = 1;
 	for (int i = 0; i < 4; i++) {
 		int ys = y * 4 + i;
 		int clip_y = ys < ylo || ys >= yhi;
-		int32_t xh = setup->xh + (ys - yh_base) * setup->dxhdy;
-		int32_t xm = setup->xm + (ys - yh_base) * setup->dxmdy;
-		int32_t xl = setup->xl + (ys - ym_base) * setup->dxldy;
-		if (ys < setup->ym)
+		int32_t xh = setup->xh * (PLANTED(31) ? 1 : S) + (ys - yh_base) * setup->dxhdy;
+		int32_t xm = setup->xm * S + (ys - yh_base) * setup->dxmdy;
+		int32_t xl = setup->xl * S + (ys - ym_base) * setup->dxldy;
+		if (ys < S * setup->ym)
 			xl = xm;
-		xl = sext(xl, 27);
-		xh = sext(xh, 27);
+		xl = sext(xl, 27 + L);
+		xh = sext(xh, 27 + L);
 		int32_t xh_s = quantize_x(xh), xl_s = quantize_x(xl);
 		int32_t l = flip ? xh_s : xl_s;
 		int32_t r = flip ? xl_s : xh_s;
@@ -531,7 +604,7 @@ static void span_setup(const tri_t *setup, const attr_t *attr, const uint32_t sc
 	s->start_x = start_x;
 	s->end_x = end_x;
 	s->valid = !all_invalid && !all_over && !all_under;
-	if (interlace_en && ((y & 1) != keep_odd))
+	if (interlace_en && (((y >> L) & 1) != keep_odd))
 		s->valid = 0;
 	s->lodlength = (int32_t)(int16_t)(flip ? (end_x - s->base_x) : (s->base_x - start_x));
 }
@@ -566,17 +639,20 @@ static inline int32_t clamp_z(int32_t z)
 	return iclamp(z, 0, 0x3ffff);
 }
 
-static void interpolate_rgba(const span_t *s, const attr_t *a, int dx, int coverage, int32_t out[4])
+/* interpolation.h:30-53; L is SCALING_LOG2.  The reference keeps the
+ * snapped value in 16 bits; only the low 9 bits of the result reach
+ * clamp_9bit, and they are the same in 32-bit arithmetic. */
+static void interpolate_rgba(const span_t *s, const attr_t *a, int dx, int coverage, int32_t out[4], int L)
 {
 	int first = find_lsb((uint32_t)coverage);
 	int yoff = first >> 1;
 	int xoff = ((first & 1) << 1) + (PLANTED(4) ? 0 : (yoff & 1));
 	for (int i = 0; i < 4; i++) {
-		int32_t v = s->rgba[i] + (a->drgba_dx[i] & ~0x1f) * dx;
+		int32_t v = s->rgba[i] + ((a->drgba_dx[i] & ~0x1f) >> (PLANTED(30) ? 0 : L)) * dx;
 		int32_t snapped = v >> 14;
-		snapped <<= 2;
+		snapped <<= 2 + L;
 		snapped += xoff * (a->drgba_dx[i] >> 14) + yoff * (a->drgba_dy[i] >> 14);
-		snapped >>= 4;
+		snapped >>= 4 + L;
 		out[i] = clamp_9bit(snapped);
 	}
 }
@@ -645,18 +721,21 @@ static void stw_to_st(const int32_t stw[3], int perspective, int32_t st[2], int 
 	}
 }
 
-/* interpolation.h:101-142 */
+/* interpolation.h:93-142; L is SCALING_LOG2. */
 static void interpolate_stz(const span_t *s, const attr_t *a, int dx, int coverage, int perspective, int uses_lod,
-                            int flip_dir, int32_t st[2], int32_t st_dx[2], int32_t st_dy[2], int32_t *z, int *ovf)
+                            int flip_dir, int32_t st[2], int32_t st_dx[2], int32_t st_dy[2], int32_t *z, int *ovf, int L)
 {
 	int32_t stw[3], stw_dx[3], stw_dy[3];
 	static const int comp[3] = { 0, 1, 3 };
 	for (int i = 0; i < 3; i++) {
 		int c = comp[i];
-		stw[i] = s->stzw[c] + (a->dstzw_dx[c] & ~0x1f) * dx;
+		stw[i] = s->stzw[c] + ((a->dstzw_dx[c] & ~0x1f) >> (PLANTED(37) ? 0 : L)) * dx;
 		if (uses_lod) {
-			stw_dx[i] = stw[i] + flip_dir * (a->dstzw_dx[c] & ~0x1f);
-			stw_dy[i] = stw[i] + (a->dstzw_dy[c] & ~0x7fff);
+			stw_dx[i] = stw[i] + flip_dir * ((a->dstzw_dx[c] & ~0x1f) >> L);
+			if (L)
+				stw_dy[i] = stw[i] + (flip_dir < 0 ? -flip_dir : flip_dir) * ((a->dstzw_dy[c] & ~0x7fff) >> L);
+			else
+				stw_dy[i] = stw[i] + (a->dstzw_dy[c] & ~0x7fff);
 		}
 	}
 	stw_to_st(stw, perspective, st, ovf);
@@ -665,23 +744,25 @@ static void interpolate_stz(const span_t *s, const attr_t *a, int dx, int covera
 		stw_to_st(stw_dy, perspective, st_dy, ovf);
 	}
 
-	int32_t zz = s->stzw[2] + a->dstzw_dx[2] * dx;
+	/* Whole native pixels step by the full gradient, sub-pixels by the
+	 * gradient >> L. */
+	int32_t zz = s->stzw[2] + a->dstzw_dx[2] * (dx >> L) + (a->dstzw_dx[2] >> L) * (dx & ((1 << L) - 1));
 	int32_t snapped = zz >> 10;
 	int first = find_lsb((uint32_t)coverage);
 	int yoff = first >> 1;
 	int xoff = ((first & 1) << 1) + (yoff & 1);
-	snapped <<= 2;
+	snapped <<= 2 + L;
 	snapped += xoff * (a->dstzw_dx[2] >> 10) + yoff * (a->dstzw_dy[2] >> 10);
-	snapped >>= PLANTED(5) ? 4 : 5;
+	snapped >>= (PLANTED(5) ? 4 : 5) + L;
 	*z = clamp_z(snapped);
 }
 
-static void interpolate_st_single(const span_t *s, const attr_t *a, int dx, int perspective, int32_t st[2])
+static void interpolate_st_single(const span_t *s, const attr_t *a, int dx, int perspective, int32_t st[2], int L)
 {
 	int32_t stw[3];
 	static const int comp[3] = { 0, 1, 3 };
 	for (int i = 0; i < 3; i++)
-		stw[i] = s->stzw[comp[i]] + (a->dstzw_dx[comp[i]] & ~0x1f) * dx;
+		stw[i] = s->stzw[comp[i]] + ((a->dstzw_dx[comp[i]] & ~0x1f) >> L) * dx;
 	int ovf = 0;
 	stw_to_st(stw, perspective, st, &ovf);
 }
@@ -1624,6 +1705,7 @@ typedef struct pixel_state {
 	uint32_t depth, dz;
 	int color_dirty, depth_dirty;
 	uint32_t color_index, depth_index;
+	int loaded; /* upscaled samples: loaded from their plane */
 } pixel_state_t;
 
 static void load_color(const rdpo *o, uint32_t fmt, uint32_t index, int32_t c[4])
@@ -1784,8 +1866,9 @@ typedef struct {
 	const attr_t *attr;
 	const draw_t *d;
 	const tile_t *tiles; /* snapshot at draw time */
-	const span_t *spans; /* lines ylo .. yhi+1 */
-	int ylo, yhi;
+	const span_t *spans; /* lines ylo .. yhi+1 (upscaled: S * ylo .. S * (yhi + 2) - 1) */
+	int ylo, yhi;        /* native lines */
+	int L;               /* SCALING_LOG2: 0 native, > 0 the upscaled pass */
 	uint32_t fb_size; /* 0,1,2,4 */
 	uint32_t dx_mask, dx_shift;
 } prim_ctx_t;
@@ -1794,9 +1877,15 @@ static int shade_pixel(const rdpo *o, const prim_ctx_t *pc, int x, int y, shaded
 {
 	const draw_t *d = pc->d;
 	const attr_t *attr = pc->attr;
-	if (y < pc->ylo || y > pc->yhi)
+	const int L = pc->L, S = 1 << L;
+	/* shading.h:38-56 */
+	if (y < S * pc->ylo || y > pc->yhi * S + (S - 1))
 		return 0;
-	const span_t *span = &pc->spans[y - pc->ylo];
+	if (L && (pc->tri->flags & TRI_DISABLE_UPSCALING)) {
+		x &= ~(S - 1);
+		y &= ~(S - 1);
+	}
+	const span_t *span = &pc->spans[y - S * pc->ylo];
 	if (!span->valid)
 		return 0;
 
@@ -1835,6 +1924,8 @@ static int shade_pixel(const rdpo *o, const prim_ctx_t *pc, int x, int y, shaded
 		if (!(x >= span->start_x && x <= span->end_x))
 			return 0;
 		int dxp = flip ? (x - span->start_x) : (span->end_x - x);
+		/* interpolate_st_copy: sub-pixels repeat their native pixel. */
+		dxp >>= PLANTED(35) ? 0 : L;
 		int snapped_dx = dxp & (int)pc->dx_mask;
 		int s_offset = dxp - snapped_dx;
 		int lerp_dx = (dxp >> pc->dx_shift) * (flip ? 1 : -1);
@@ -1869,11 +1960,16 @@ static int shade_pixel(const rdpo *o, const prim_ctx_t *pc, int x, int y, shaded
 	int interp_dir = flip ? 1 : -1;
 
 	int32_t shade[4];
-	interpolate_rgba(span, attr, dx, coverage, shade);
+	interpolate_rgba(span, attr, dx, coverage, shade, L);
 
+	/* shading.h:192-198: with the native-LOD quirk the LOD neighbours are a
+	 * native pixel away. */
+	int tex_dir = interp_dir;
+	if (L && uses_lod && (pc->tri->flags & TRI_NATIVE_LOD))
+		tex_dir *= S;
 	int32_t st[2], st_dx[2] = { 0, 0 }, st_dy[2] = { 0, 0 }, z;
 	int povf = 0;
-	interpolate_stz(span, attr, dx, coverage, perspective, uses_lod, interp_dir, st, st_dx, st_dy, &z, &povf);
+	interpolate_stz(span, attr, dx, coverage, perspective, uses_lod, tex_dir, st, st_dx, st_dy, &z, &povf, L);
 
 	uint32_t tile0 = setup_tile & 7;
 	uint32_t tile1 = (tile0 + 1) & 7;
@@ -1890,7 +1986,7 @@ static int shade_pixel(const rdpo *o, const prim_ctx_t *pc, int x, int y, shaded
 		               texel0);
 
 	if (uses_pipelined_texel1) {
-		const span_t *next = &pc->spans[y - pc->ylo + 1];
+		const span_t *next = &pc->spans[y - S * pc->ylo + 1];
 		int long_span = span->lodlength >= 8;
 		int end_span = x == (flip ? span->end_x : span->start_x);
 		if (end_span && long_span && next->valid) {
@@ -1903,7 +1999,7 @@ static int shade_pixel(const rdpo *o, const prim_ctx_t *pc, int x, int y, shaded
 				st[1] = stw[1];
 			}
 		} else
-			interpolate_st_single(span, attr, dx + interp_dir, perspective, st);
+			interpolate_st_single(span, attr, dx + interp_dir * S, perspective, st, L);
 		tile1 = tile0;
 		uses_texel1 = 1;
 	}
@@ -2330,8 +2426,6 @@ static void resolve_draw_state(rdpo *o, const attr_t *attr, draw_t *d)
 /* (maintain_queues_idle, rdp_renderer.cpp:2403-2411) depend on timing and are */
 /* not modelled: the oracle behaves as enqueue_command_direct.                */
 /* ------------------------------------------------------------------------ */
-#define MAX_FB_LINES 1024
-#define MAX_FB_WIDTH 1024
 
 static void pass_write_back(rdpo *o)
 {
@@ -2349,10 +2443,15 @@ static void pass_write_back(rdpo *o)
 	o->num_touched = 0;
 }
 
+static void up_pass_end(rdpo *o, void (*native_write_back)(rdpo *));
+
 /* Renderer::flush_queues + reset_context. */
 static void pass_flush(rdpo *o)
 {
-	pass_write_back(o);
+	if (o->up_log2)
+		up_pass_end(o, pass_write_back);
+	else
+		pass_write_back(o);
 	if (o->pass.num_prims) {
 		if (o->pass_log_len == o->pass_log_cap) {
 			o->pass_log_cap = o->pass_log_cap ? 2 * o->pass_log_cap : 256;
@@ -2479,15 +2578,16 @@ static unsigned cache_add(void *keys, unsigned *count, const void *key, size_t s
 	return (*count)++;
 }
 
-static void interpolate_x_64(const tri_t *s, int y, int flip, int64_t *left, int64_t *right)
+/* interpolate_x, rdp_renderer.cpp:962-992: edges at sub-scanline y of the
+ * S-times domain (S = 1 natively). */
+static void interpolate_x_64(const tri_t *s, int y, int flip, int64_t S, int64_t *left, int64_t *right)
 {
-	/* interpolate_x, rdp_renderer.cpp:962-992 (no upscaling). */
-	int64_t yh_base = s->yh & ~3;
-	int64_t ym_base = s->ym;
-	int64_t xh = (int64_t)s->xh + ((int64_t)y - yh_base) * s->dxhdy;
-	int64_t xm = (int64_t)s->xm + ((int64_t)y - yh_base) * s->dxmdy;
-	int64_t xl = (int64_t)s->xl + ((int64_t)y - ym_base) * s->dxldy;
-	if (y < s->ym)
+	int64_t yh_base = (int64_t)(s->yh & ~3) * S;
+	int64_t ym_base = (int64_t)s->ym * S;
+	int64_t xh = S * s->xh + ((int64_t)y - yh_base) * s->dxhdy;
+	int64_t xm = S * s->xm + ((int64_t)y - yh_base) * s->dxmdy;
+	int64_t xl = S * s->xl + ((int64_t)y - ym_base) * s->dxldy;
+	if (y < S * s->ym)
 		xl = xm;
 	int64_t hs = xh >> 15, ls = xl >> 15;
 	int64_t l = flip ? hs : ls, r = flip ? ls : hs;
@@ -2499,24 +2599,27 @@ static void interpolate_x_64(const tri_t *s, int y, int flip, int64_t *left, int
 	*right = r;
 }
 
-/* compute_conservative_max_num_tiles, rdp_renderer.cpp:994-1060. */
+/* compute_conservative_max_num_tiles, rdp_renderer.cpp:994-1060: 8x8 tiles
+ * of the S-times domain, S the upscaling factor (1 natively). */
 static uint32_t conservative_tile_count(const rdpo *o, const tri_t *s)
 {
 	if (s->yl <= s->yh)
 		return 0;
-	int start_y = imax(s->yh & ~3, (int)o->scissor.ylo);
-	int end_y = imin((s->yl - 1) | 3, (int)o->scissor.yhi - 1);
+	const int S = 1 << o->up_log2;
+	int start_y = imax(s->yh & ~3, (int)o->scissor.ylo) * S;
+	int end_y = imin((s->yl - 1) | 3, (int)o->scissor.yhi - 1) * S;
 	if (end_y < start_y)
 		return 0;
 	int flip = (s->flags & TRI_FLIP) != 0;
 	int64_t l[4], r[4];
-	interpolate_x_64(s, start_y, flip, &l[0], &r[0]);
-	interpolate_x_64(s, end_y, flip, &l[1], &r[1]);
+	interpolate_x_64(s, start_y, flip, S, &l[0], &r[0]);
+	interpolate_x_64(s, end_y, flip, S, &l[1], &r[1]);
 	l[2] = l[3] = l[0];
 	r[2] = r[3] = r[0];
-	if (s->ym > start_y && s->ym < end_y) {
-		interpolate_x_64(s, s->ym, flip, &l[2], &r[2]);
-		interpolate_x_64(s, s->ym - 1, flip, &l[3], &r[3]);
+	int ym = S * s->ym;
+	if (ym > start_y && ym < end_y) {
+		interpolate_x_64(s, ym, flip, S, &l[2], &r[2]);
+		interpolate_x_64(s, ym - 1, flip, S, &l[3], &r[3]);
 	}
 	int64_t sx = l[0], ex = r[0];
 	for (int i = 1; i < 4; i++) {
@@ -2524,14 +2627,14 @@ static uint32_t conservative_tile_count(const rdpo *o, const tri_t *s)
 		if (r[i] > ex) ex = r[i];
 	}
 	int64_t range = (sx < 0 ? -sx : sx) > (ex < 0 ? -ex : ex) ? (sx < 0 ? -sx : sx) : (ex < 0 ? -ex : ex);
-	if (range > 2047) {
+	if (range > 2047 * S) {
 		sx = 0;
 		ex = 0x7fffffff;
 	}
-	if (sx < (int64_t)(o->scissor.xlo >> 2))
-		sx = o->scissor.xlo >> 2;
-	if (ex > (int64_t)((o->scissor.xhi + 3) >> 2) - 1)
-		ex = (int64_t)((o->scissor.xhi + 3) >> 2) - 1;
+	if (sx < (int64_t)S * (o->scissor.xlo >> 2))
+		sx = (int64_t)S * (o->scissor.xlo >> 2);
+	if (ex > (int64_t)S * ((o->scissor.xhi + 3) >> 2) - 1)
+		ex = (int64_t)S * ((o->scissor.xhi + 3) >> 2) - 1;
 	if (ex < sx)
 		return 0;
 	return (uint32_t)((ex / 8 - sx / 8 + 1) * (end_y / 32 - start_y / 32 + 1));
@@ -2577,7 +2680,10 @@ static void pass_account_draw(rdpo *o, const tri_t *setup)
 	int cache_full = t->num_static == 64 || t->num_depth_blend == 64 || t->num_tiles + 8 > 256;
 	int prims_full = t->num_prims == 256;
 	int spans_full = t->num_span_jobs * 64 + 1024 > 32 * 1024;
-	int tiles_full = t->max_shaded_tiles + 128 * 128 > 0x8000;
+	/* Tile-instance capacity scales with the factor squared
+	 * (rdp_renderer.cpp:63-67). */
+	const uint32_t S = 1u << o->up_log2;
+	int tiles_full = t->max_shaded_tiles + (128 * S) * (128 * S) > S * S * 0x8000;
 	if (cache_full || prims_full || spans_full || tiles_full)
 		pass_flush(o);
 }
@@ -2629,6 +2735,219 @@ static pixel_state_t *pass_pixel(rdpo *o, uint32_t fmt, int x, int y, uint32_t c
 	return p;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Upscaled domain.  Renderer::should_render_upscaled and                    */
+/* render_pass_is_upscaled (rdp_renderer.cpp:2988-3010), the domain updates  */
+/* (submit_update_upscaled_domain, rdp_renderer.cpp:1943-2033;               */
+/* update_upscaled_domain_pre.comp / _post.comp) and the upscaled pass.      */
+/* ------------------------------------------------------------------------ */
+typedef struct up_draw {
+	tri_t tri;
+	attr_t attr;
+	draw_t d;
+	tile_t tiles[8];
+	uint32_t scissor[4];
+	int ylo, yhi;
+	uint16_t tmem[2048];
+} up_draw_t;
+
+#define UP_MAX_DRAWS 256 /* a pass ends at 256 primitives */
+
+/* Only 16- and 32-bit colour framebuffers of at least 256 pixels are
+ * rendered upscaled. */
+static int up_pass_eligible(const rdpo *o)
+{
+	return (o->fb_fmt == FB_RGBA5551 || o->fb_fmt == FB_RGBA8888) && o->fb_width >= 256;
+}
+
+static void up_record(rdpo *o, const tri_t *setup, const attr_t *attr, const draw_t *d, const uint32_t scissor[4],
+                      int ylo, int yhi)
+{
+	if (!o->up_draws) {
+		o->up_draws = (up_draw_t *)malloc(sizeof(up_draw_t) * UP_MAX_DRAWS);
+		if (!o->up_draws)
+			abort();
+	}
+	if (o->up_num >= UP_MAX_DRAWS)
+		abort();
+	up_draw_t *r = &o->up_draws[o->up_num++];
+	r->tri = *setup;
+	r->attr = *attr;
+	r->d = *d;
+	memcpy(r->tiles, o->tiles, sizeof(r->tiles));
+	memcpy(r->scissor, scissor, sizeof(r->scissor));
+	r->ylo = ylo;
+	r->yhi = yhi;
+	memcpy(r->tmem, o->tmem, sizeof(r->tmem));
+}
+
+/* Point the RDRAM / hidden RDRAM views at sample plane `slice` of the
+ * upscaled domain (plane < 0: back to the native memory).  Plane addressing
+ * is the native addressing plus slice * size (memory_interfacing.h:55-157),
+ * so the native load/store functions serve the planes unchanged. */
+static void up_select_plane(rdpo *o, int slice, uint8_t *native_rdram, uint8_t *native_hidden)
+{
+	if (slice < 0) {
+		o->rdram = native_rdram;
+		o->hidden = native_hidden;
+	} else {
+		o->rdram = o->up_rdram + (size_t)slice * o->rdram_size;
+		o->hidden = o->up_hidden + (size_t)slice * (o->rdram_size / 2);
+	}
+}
+
+static pixel_state_t *up_pixel(rdpo *o, uint32_t fmt, int x, int y, uint32_t ci, uint32_t di)
+{
+	const int L = (int)o->up_log2, S = 1 << L;
+	uint32_t key = ((uint32_t)y >> L) * MAX_FB_WIDTH + ((uint32_t)x >> L);
+	pixel_state_t *blk = o->upx[key];
+	if (!blk) {
+		blk = (pixel_state_t *)calloc((size_t)S * S, sizeof(pixel_state_t));
+		if (!blk)
+			abort();
+		o->upx[key] = blk;
+		o->upx_touched[o->upx_num++] = key;
+	}
+	int slice = PLANTED(32) ? (x & (S - 1)) * S + (y & (S - 1)) : (y & (S - 1)) * S + (x & (S - 1));
+	pixel_state_t *p = &blk[slice];
+	if (!p->loaded) {
+		uint8_t *nr = o->rdram, *nh = o->hidden;
+		p->color_index = ci;
+		p->depth_index = di;
+		up_select_plane(o, slice, nr, nh);
+		load_color(o, fmt, ci, p->color);
+		load_depth(o, di, &p->depth, &p->dz);
+		up_select_plane(o, -1, nr, nh);
+		p->loaded = 1;
+	}
+	return p;
+}
+
+static void upx_release(rdpo *o)
+{
+	for (size_t i = 0; i < o->upx_num; i++) {
+		free(o->upx[o->upx_touched[i]]);
+		o->upx[o->upx_touched[i]] = NULL;
+	}
+	o->upx_num = 0;
+}
+
+static void up_write_back(rdpo *o)
+{
+	const int S = 1 << o->up_log2;
+	uint32_t fmt = o->fb_fmt;
+	int alias = o->fb_addr == o->depth_addr;
+	uint8_t *nr = o->rdram, *nh = o->hidden;
+	for (size_t i = 0; i < o->upx_num; i++) {
+		pixel_state_t *blk = o->upx[o->upx_touched[i]];
+		for (int slice = 0; slice < S * S; slice++) {
+			pixel_state_t *p = &blk[slice];
+			if (!p->loaded)
+				continue;
+			up_select_plane(o, slice, nr, nh);
+			if (p->color_dirty)
+				store_color(o, fmt, p->color_index, p->color);
+			if (!alias && p->depth_dirty)
+				store_depth(o, p->depth_index, p->depth, p->dz);
+		}
+	}
+	up_select_plane(o, -1, nr, nh);
+	upx_release(o);
+}
+
+/* One element of update_upscaled_domain_pre.comp (post = 0: where native
+ * RDRAM differs from the reference copy, broadcast the native element and
+ * its hidden bits to every plane, then update the reference) or
+ * update_upscaled_domain_post.comp (post = 1: reference := native).  index
+ * counts host-order elements of 1 << size_log2 bytes. */
+static void up_sync_element(rdpo *o, int post, unsigned size_log2, uint32_t index)
+{
+	const unsigned bytes = 1u << size_log2;
+	const uint32_t planes = 1u << (2 * o->up_log2);
+	const size_t hplane = o->rdram_size >> 1;
+	index &= (o->rdram_size >> size_log2) - 1;
+	size_t off = (size_t)index << size_log2;
+	if (post) {
+		memcpy(o->up_ref + off, o->rdram + off, bytes);
+		return;
+	}
+	if (!memcmp(o->rdram + off, o->up_ref + off, bytes))
+		return;
+	for (uint32_t i = 0; i < planes; i++) {
+		memcpy(o->up_rdram + (size_t)i * o->rdram_size + off, o->rdram + off, bytes);
+		uint8_t *uh = o->up_hidden + (size_t)i * hplane;
+		if (PLANTED(34)) {
+		} else if (size_log2 == 0) {
+			uint32_t m = index ^ 3u;
+			if (m & 1u)
+				uh[m >> 1] = o->hidden[m >> 1];
+		} else if (size_log2 == 1) {
+			uh[index ^ 1u] = o->hidden[index ^ 1u];
+		} else {
+			uh[2 * index] = o->hidden[2 * index];
+			uh[2 * index + 1] = o->hidden[2 * index + 1];
+		}
+	}
+	memcpy(o->up_ref + off, o->rdram + off, bytes);
+}
+
+/* submit_update_upscaled_domain (rdp_renderer.cpp:1943-2033) for the pre and
+ * post stages: whole words only, colour elements of 1 << size_log2 bytes
+ * from addr, depth halfwords from depth_addr unless the two coincide. */
+static void up_domain_update(rdpo *o, int post, uint32_t addr, uint32_t depth_addr, uint32_t num_pixels,
+                             unsigned size_log2)
+{
+	addr &= ~3u;
+	depth_addr &= ~3u;
+	uint32_t align = 4u >> size_log2;
+	num_pixels = (num_pixels + align - 1u) & ~(align - 1u);
+	int alias = addr == depth_addr;
+	uint32_t color_index = addr >> size_log2, depth_index = depth_addr >> 1;
+	for (uint32_t i = 0; i < num_pixels; i++) {
+		up_sync_element(o, post, size_log2, color_index + i);
+		if (!alias)
+			up_sync_element(o, post, 1, depth_index + i);
+	}
+}
+
+static unsigned fb_pixel_size_log2(uint32_t fmt)
+{
+	return fmt == FB_RGBA8888 ? 2 : (fmt == FB_RGBA5551 || fmt == FB_IA88) ? 1 : 0;
+}
+
+static void render_draw(rdpo *o, const tri_t *setup, const attr_t *attr, const draw_t *d, const tile_t *tiles,
+                        const uint32_t scissor[4], int ylo, int yhi, int L);
+
+/* The render-pass end of the upscaled domain (submit_render_pass and
+ * submit_render_pass_upscaled, rdp_renderer.cpp:2223-2342, in the order
+ * Renderer::flush_queues issues them, :2955-2985): before the native
+ * raster, the pre update over the framebuffer region; after it, for a pass
+ * that qualifies, the post update and the same draws rendered at scale into
+ * the sample planes.  native_write_back runs the native raster's merge. */
+static void up_pass_end(rdpo *o, void (*native_write_back)(rdpo *))
+{
+	int need_render_pass = o->fb_width && o->pass.deduced_height && o->pass.num_span_jobs;
+	uint32_t num_pixels = o->fb_width * o->pass.deduced_height;
+	unsigned size_log2 = fb_pixel_size_log2(o->fb_fmt);
+	if (need_render_pass)
+		up_domain_update(o, 0, o->fb_addr, o->depth_addr, num_pixels, size_log2);
+	native_write_back(o);
+	if (need_render_pass && up_pass_eligible(o)) {
+		if (!PLANTED(33))
+			up_domain_update(o, 1, o->fb_addr, o->depth_addr, num_pixels, size_log2);
+		uint16_t saved[2048];
+		memcpy(saved, o->tmem, sizeof(saved));
+		for (unsigned i = 0; i < o->up_num; i++) {
+			up_draw_t *r = &o->up_draws[i];
+			memcpy(o->tmem, r->tmem, sizeof(saved));
+			render_draw(o, &r->tri, &r->attr, &r->d, r->tiles, r->scissor, r->ylo, r->yhi, (int)o->up_log2);
+		}
+		memcpy(o->tmem, saved, sizeof(saved));
+		up_write_back(o);
+	}
+	o->up_num = 0;
+}
+
 /* The mode a fragment is counted under (rdpo_fragment_counts). */
 static int fragment_mode(const draw_t *d)
 {
@@ -2636,6 +2955,87 @@ static int fragment_mode(const draw_t *d)
 	if (d->ras_flags & RAS_COPY) return 1;
 	if (d->ras_flags & RAS_MULTI_CYCLE) return 4;
 	return (d->uses_texel0 || d->uses_texel1) ? 3 : 2;
+}
+
+/* Rasterise, shade and merge one draw into the open pass's pixel states.
+ * L = 0 is the native pass: the span walker over native lines ylo .. yhi
+ * (plus the line after, read by the pipelined texel-1 path) and the pixel
+ * cache of the native framebuffer.  L > 0 is the upscaled pass
+ * (submit_render_pass_upscaled, rdp_renderer.cpp:2287-2342): span lines
+ * S * ylo .. S * (yhi + 2) - 1 (span_setup.comp:89-95), pixels of the
+ * S-times framebuffer, each merged into its own sample of the native pixel
+ * (init_tile/finish_tile, memory_interfacing.h:286-362). */
+static void render_draw(rdpo *o, const tri_t *setup, const attr_t *attr, const draw_t *d, const tile_t *tiles,
+                        const uint32_t scissor[4], int ylo, int yhi, int L)
+{
+	const int S = 1 << L;
+	int nlines = S * (yhi - ylo + 2);
+	span_t *spans = (span_t *)malloc(sizeof(span_t) * (size_t)nlines);
+	if (!spans)
+		abort();
+	for (int i = 0; i < nlines; i++)
+		span_setup(setup, attr, scissor, S * ylo + i, &spans[i], L);
+
+	prim_ctx_t pc;
+	pc.tri = setup;
+	pc.attr = attr;
+	pc.d = d;
+	pc.tiles = tiles;
+	pc.spans = spans;
+	pc.ylo = ylo;
+	pc.yhi = yhi;
+	pc.L = L;
+	switch (o->fb_fmt) {
+	case FB_I4: pc.fb_size = 0; pc.dx_mask = 0; pc.dx_shift = 0; break;
+	case FB_I8: pc.fb_size = 1; pc.dx_mask = ~7u; pc.dx_shift = 3; break;
+	case FB_RGBA5551:
+	case FB_IA88: pc.fb_size = 2; pc.dx_mask = ~3u; pc.dx_shift = 2; break;
+	default: pc.fb_size = 4; pc.dx_mask = ~1u; pc.dx_shift = 1; break;
+	}
+
+	uint32_t fmt = o->fb_fmt;
+	uint32_t addr_index;
+	switch (fmt) {
+	case FB_I4:
+	case FB_I8: addr_index = o->fb_addr; break;
+	case FB_RGBA5551:
+	case FB_IA88: addr_index = o->fb_addr >> 1; break;
+	default: addr_index = o->fb_addr >> 2; break;
+	}
+	uint32_t depth_index = o->depth_addr >> 1;
+	int alias = o->fb_addr == o->depth_addr;
+	/* Native-resolution draws shade every sample as the pixel's first one,
+	 * so a covered pixel's samples reach up to S - 1 past the span end. */
+	int snap = L && (setup->flags & TRI_DISABLE_UPSCALING);
+
+	for (int y = S * ylo; y <= S * yhi + (S - 1); y++) {
+		const span_t *sp = &spans[(snap ? (y & ~(S - 1)) : y) - S * ylo];
+		if (!sp->valid)
+			continue;
+		int x0 = imax(sp->start_x, 0);
+		int x1 = imin(sp->end_x + (snap ? S - 1 : 0), S * (int)o->fb_width - 1);
+		for (int x = x0; x <= x1; x++) {
+			shaded_t sh;
+			memset(&sh, 0, sizeof(sh));
+			if (!shade_pixel(o, &pc, x, y, &sh))
+				continue;
+			if (!L) {
+				o->fragments++;
+				o->fragments_mode[fragment_mode(d)]++;
+			}
+			uint32_t nx = (uint32_t)x >> L, ny = (uint32_t)y >> L;
+			uint32_t ci = addr_index + o->fb_width * ny + nx;
+			uint32_t di = depth_index + o->fb_width * ny + nx;
+			pixel_state_t *p = L ? up_pixel(o, fmt, x, y, ci, di) : pass_pixel(o, fmt, x, y, ci, di);
+			if (sh.coverage_count & COVERAGE_FILL)
+				fill_color(p, fmt, alias, d->fill_color);
+			else if (sh.coverage_count & COVERAGE_COPY)
+				copy_pipeline(p, fmt, alias, (uint32_t)sh.z_dith);
+			else
+				depth_blend(o, d, fmt, alias, p, &sh);
+		}
+	}
+	free(spans);
 }
 
 static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
@@ -2674,66 +3074,9 @@ static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 		return;
 	}
 
-	int nlines = yhi - ylo + 2;
-	span_t *spans = (span_t *)malloc(sizeof(span_t) * (size_t)nlines);
-	if (!spans)
-		abort();
-	for (int i = 0; i < nlines; i++)
-		span_setup(setup, &attr, scissor, ylo + i, &spans[i]);
-
-	prim_ctx_t pc;
-	pc.tri = setup;
-	pc.attr = &attr;
-	pc.d = &d;
-	pc.tiles = o->tiles;
-	pc.spans = spans;
-	pc.ylo = ylo;
-	pc.yhi = yhi;
-	switch (o->fb_fmt) {
-	case FB_I4: pc.fb_size = 0; pc.dx_mask = 0; pc.dx_shift = 0; break;
-	case FB_I8: pc.fb_size = 1; pc.dx_mask = ~7u; pc.dx_shift = 3; break;
-	case FB_RGBA5551:
-	case FB_IA88: pc.fb_size = 2; pc.dx_mask = ~3u; pc.dx_shift = 2; break;
-	default: pc.fb_size = 4; pc.dx_mask = ~1u; pc.dx_shift = 1; break;
-	}
-
-	uint32_t fmt = o->fb_fmt;
-	uint32_t addr_index;
-	switch (fmt) {
-	case FB_I4:
-	case FB_I8: addr_index = o->fb_addr; break;
-	case FB_RGBA5551:
-	case FB_IA88: addr_index = o->fb_addr >> 1; break;
-	default: addr_index = o->fb_addr >> 2; break;
-	}
-	uint32_t depth_index = o->depth_addr >> 1;
-	int alias = o->fb_addr == o->depth_addr;
-
-	for (int y = ylo; y <= yhi; y++) {
-		const span_t *sp = &spans[y - ylo];
-		if (!sp->valid)
-			continue;
-		int x0 = imax(sp->start_x, 0);
-		int x1 = imin(sp->end_x, (int)o->fb_width - 1);
-		for (int x = x0; x <= x1; x++) {
-			shaded_t sh;
-			memset(&sh, 0, sizeof(sh));
-			if (!shade_pixel(o, &pc, x, y, &sh))
-				continue;
-			o->fragments++;
-			o->fragments_mode[fragment_mode(&d)]++;
-			uint32_t ci = addr_index + o->fb_width * (uint32_t)y + (uint32_t)x;
-			uint32_t di = depth_index + o->fb_width * (uint32_t)y + (uint32_t)x;
-			pixel_state_t *p = pass_pixel(o, fmt, x, y, ci, di);
-			if (sh.coverage_count & COVERAGE_FILL)
-				fill_color(p, fmt, alias, d.fill_color);
-			else if (sh.coverage_count & COVERAGE_COPY)
-				copy_pipeline(p, fmt, alias, (uint32_t)sh.z_dith);
-			else
-				depth_blend(o, &d, fmt, alias, p, &sh);
-		}
-	}
-	free(spans);
+	render_draw(o, setup, &attr, &d, o->tiles, scissor, ylo, yhi, 0);
+	if (o->up_log2 && up_pass_eligible(o))
+		up_record(o, setup, &attr, &d, scissor, ylo, yhi);
 	pass_account_draw(o, setup);
 }
 
@@ -3598,11 +3941,32 @@ size_t rdpo_pass_ends(const rdpo *o, uint64_t *out, size_t cap)
 	return o->pass_log_len;
 }
 
+/* submit_update_upscaled_domain_external (rdp_renderer.cpp:1936-1941): the
+ * pre update over the range the VI reads, colour only, length counted in
+ * pixels of 1 << size_log2 bytes. */
+static void up_vi_domain_pre(void *ctx, uint32_t offset, uint32_t length, unsigned size_log2)
+{
+	up_domain_update((rdpo *)ctx, 0, offset, offset, length, size_log2);
+}
+
 int rdpo_scanout_ex(rdpo *o, const rdpo_scanout_options *opts, uint32_t *out, unsigned *width, unsigned *height)
 {
 	/* CommandProcessor::scanout flushes the renderer first (rdp_device.cpp:1095-1106). */
 	pass_flush(o);
-	return vi_scanout(o->vi, o->vi_regs, o->rdram, o->hidden, o->rdram_size, opts, out, width, height);
+	vi_source_t src;
+	memset(&src, 0, sizeof(src));
+	src.rdram_size = o->rdram_size;
+	if (o->up_log2) {
+		src.rdram = o->up_rdram;
+		src.hidden = o->up_hidden;
+		src.scale_log2 = o->up_log2;
+		src.domain_pre = up_vi_domain_pre;
+		src.ctx = o;
+	} else {
+		src.rdram = o->rdram;
+		src.hidden = o->hidden;
+	}
+	return vi_scanout(o->vi, o->vi_regs, &src, opts, out, width, height);
 }
 
 int rdpo_scanout(rdpo *o, uint32_t *out, unsigned *width, unsigned *height)
@@ -3727,7 +4091,7 @@ void rdpo_test_span_setup(const int32_t tri[10], const int32_t attr[32], const u
 	unpack_tri(tri, &t);
 	unpack_attr(attr, &a);
 	memset(&s, 0, sizeof(s));
-	span_setup(&t, &a, scissor, y, &s);
+	span_setup(&t, &a, scissor, y, &s, 0);
 	for (int i = 0; i < 4; i++) {
 		out[i] = s.rgba[i];
 		out[4 + i] = s.stzw[i];
@@ -3748,7 +4112,7 @@ void rdpo_test_interpolate_rgba(const int32_t span_rgba[4], const int32_t attr[3
 	memset(&s, 0, sizeof(s));
 	unpack_attr(attr, &a);
 	memcpy(s.rgba, span_rgba, sizeof(s.rgba));
-	interpolate_rgba(&s, &a, dx, coverage, out);
+	interpolate_rgba(&s, &a, dx, coverage, out, 0);
 }
 
 /* out: st[2], st_dx[2], st_dy[2], z, overflow. */
@@ -3762,7 +4126,7 @@ void rdpo_test_interpolate_stz(const int32_t span_stzw[4], const int32_t attr[32
 	memcpy(s.stzw, span_stzw, sizeof(s.stzw));
 	int32_t st[2], st_dx[2] = { 0, 0 }, st_dy[2] = { 0, 0 }, z;
 	int ovf = 0;
-	interpolate_stz(&s, &a, dx, coverage, perspective, uses_lod, flip_dir, st, st_dx, st_dy, &z, &ovf);
+	interpolate_stz(&s, &a, dx, coverage, perspective, uses_lod, flip_dir, st, st_dx, st_dy, &z, &ovf, 0);
 	out[0] = st[0];
 	out[1] = st[1];
 	out[2] = st_dx[0];

@@ -39,6 +39,17 @@ typedef struct rdpo rdpo;
 rdpo *rdpo_create(uint32_t rdram_size);
 void rdpo_destroy(rdpo *o);
 
+/* Upscaled rendering (the reference's COMMAND_PROCESSOR_FLAG_UPSCALING_*,
+ * rdp_device.cpp:188-203): factor 2, 4 or 8, before the first command.
+ * Returns 0, or -1 for another factor or a second call.  The upscaled
+ * domain: factor^2 sample planes of RDRAM (rdram_size bytes each) and of
+ * hidden RDRAM (rdram_size / 2 each); sample (sx, sy) of a pixel lives in
+ * plane sy * factor + sx at the pixel's native address. */
+int rdpo_enable_upscaling(rdpo *o, unsigned factor);
+uint8_t *rdpo_upscaled_rdram(rdpo *o);
+uint8_t *rdpo_upscaled_hidden_rdram(rdpo *o);
+unsigned rdpo_upscaling_factor(const rdpo *o);
+
 uint8_t *rdpo_rdram(rdpo *o);
 uint8_t *rdpo_hidden_rdram(rdpo *o);
 uint16_t *rdpo_tmem(rdpo *o);
@@ -82,7 +93,9 @@ typedef struct {
 void rdpo_scanout_options_default(rdpo_scanout_options *opts);
 
 /* Scan out the current frame through the VI filters (ends the open render
- * pass first).  out must hold 640 x 576 RGBA8 words (byte order R, G, B, A);
+ * pass first).  out must hold 640 x 576 RGBA8 words (byte order R, G, B, A),
+ * times factor^2 with upscaling, where the image is read from the sample
+ * planes and comes out factor times larger in each direction;
  * width/height receive the output size, 0x0 for a blank or invalid frame.
  * rdpo_scanout uses the default options.  Returns 0. */
 int rdpo_scanout(rdpo *o, uint32_t *out, unsigned *width, unsigned *height);

@@ -19,6 +19,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted-mistake builds, as in rdp_oracle.c (-DRDPO_PLANT=n). */
+#ifndef RDPO_PLANT
+#define RDPO_PLANT 0
+#endif
+#define PLANTED(n) (RDPO_PLANT == (n))
+
 enum {
 	VI_TYPE_MASK = 3,
 	VI_TYPE_RGBA5551 = 2,
@@ -65,6 +71,13 @@ static inline int imin(int a, int b) { return a < b ? a : b; }
 static inline int imax(int a, int b) { return a > b ? a : b; }
 
 void vi_state_init(vi_state_t *s) { memset(s, 0, sizeof(*s)); }
+
+void vi_state_free(vi_state_t *s)
+{
+	free(s->prev);
+	s->prev = NULL;
+	s->prev_cap = 0;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Per-scanline registers, video_interface.cpp:1126-1213.                   */
@@ -262,11 +275,18 @@ static texel_t img_get(const image_t *im, int x, int y)
 	return im->t[(size_t)y * (size_t)im->w + (size_t)x];
 }
 
-/* extract_vram.comp:61-106 */
-static texel_t fetch_color(const uint8_t *rdram, const uint8_t *hidden, uint32_t rdram_size, uint32_t status,
-                           int fb_offset, int fb_width, int x, int y)
+/* extract_vram.comp:61-106; (x, y) is a pixel of the 2^L-times image,
+ * sample (x mod 2^L, y mod 2^L) of native pixel (x >> L, y >> L). */
+static texel_t fetch_color(const vi_source_t *src, uint32_t status, int fb_offset, int fb_width, int x, int y)
 {
 	texel_t t;
+	const int L = (int)src->scale_log2, S = 1 << L;
+	const uint32_t rdram_size = src->rdram_size;
+	int slice = PLANTED(36) ? (x & (S - 1)) * S + (y & (S - 1)) : (y & (S - 1)) * S + (x & (S - 1));
+	x >>= L;
+	y >>= L;
+	const uint8_t *rdram = src->rdram + (size_t)slice * rdram_size;
+	const uint8_t *hidden = src->hidden + (size_t)slice * (rdram_size >> 1);
 	int type = (int)(status & VI_TYPE_MASK);
 	int linear = y * fb_width + x + fb_offset;
 	if (type == VI_TYPE_RGBA8888) {
@@ -469,12 +489,35 @@ static void shift_rect(int *ox, int *oy, int *w, int *h, int dx, int dy)
 		*h = 0;
 }
 
-/* VideoInterface::scanout, video_interface.cpp:1215-1426 (scaling factor 1,
- * no downscale), with scale_stage (:796-998). */
-int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, const uint8_t *hidden,
-               uint32_t rdram_size, const rdpo_scanout_options *opts, uint32_t *out, unsigned *width,
-               unsigned *height)
+/* VideoInterface::scanout_memory_range, video_interface.cpp:369-396, on
+ * the registers as they decode after the scanout has cleared the
+ * per-scanline state. */
+static void scanout_memory_range(const vi_state_t *s, const uint32_t regs[14], uint32_t *offset, uint32_t *length)
 {
+	vi_regs_t r;
+	decode_vi_registers(s, regs, &r, NULL);
+	int divot = (r.status & VI_DIVOT) != 0;
+	int aa_width = r.max_x + 2 + 4 + divot * 2;
+	int aa_height = r.max_y + 1 + 4;
+	int x_off = divot ? -3 : -2, y_off = -2;
+	if (r.vi_offset == 0 || r.h_res <= 0 || r.h_start >= SCANOUT_WIDTH) {
+		*offset = *length = 0;
+		return;
+	}
+	int pixel_size = (((r.status & VI_TYPE_MASK) | VI_TYPE_RGBA5551) == VI_TYPE_RGBA8888) ? 4 : 2;
+	uint32_t vi_offset = (uint32_t)r.vi_offset & ~(uint32_t)(pixel_size - 1);
+	vi_offset += (uint32_t)((x_off + y_off * r.vi_width) * pixel_size);
+	*offset = vi_offset;
+	*length = (uint32_t)((aa_height * r.vi_width + aa_width) * pixel_size);
+}
+
+/* VideoInterface::scanout, video_interface.cpp:1215-1426 (no downscale
+ * steps), with vram_fetch_stage (:405-510) and scale_stage (:719-998);
+ * src->scale_log2 is the renderer's upscaling factor. */
+int vi_scanout(vi_state_t *s, const uint32_t regs[14], const vi_source_t *src, const rdpo_scanout_options *opts,
+               uint32_t *out, unsigned *width, unsigned *height)
+{
+	const int L = (int)src->scale_log2, SF = 1 << L;
 	vi_line_t lines[MAX_OUTPUT_SCANLINES];
 	vi_regs_t r;
 	decode_vi_registers(s, regs, &r, lines);
@@ -528,12 +571,35 @@ int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, con
 	}
 	s->last_valid_frame_count = s->frame_count;
 	int degenerate = r.h_res <= 0 || r.v_res <= 0;
-	int fetch_bug = r.init_y_add < 1024;
+	int fetch_bug = r.init_y_add < 1024 && SF == 1;
+
+	/* video_interface.cpp:1299-1318 */
+	r.h_start *= SF;
+	r.h_start_clamp *= SF;
+	r.v_start *= SF;
+	r.h_res *= SF;
+	r.h_res_clamp *= SF;
+	r.v_res *= SF;
+	r.max_x = r.max_x * SF + (SF - 1);
+	r.max_y = r.max_y * SF + (SF - 1);
+	for (int l = 0; l < MAX_OUTPUT_SCANLINES; l++) {
+		lines[l].h_start *= SF;
+		lines[l].h_start_clamp *= SF;
+		lines[l].h_end_clamp *= SF;
+		lines[l].x_start *= SF;
+		lines[l].y_start *= SF;
+		lines[l].y_base *= SF;
+	}
 
 	/* Stages 1-3: extract, AA/dither fetch filter, divot. */
 	image_t E = { 0, 0, NULL }, A[2] = { { 0, 0, NULL }, { 0, 0, NULL } }, D[2] = { { 0, 0, NULL }, { 0, 0, NULL } };
 	const image_t *S[2] = { NULL, NULL };
 	if (!degenerate) {
+		if (L && src->domain_pre) {
+			uint32_t offset, length;
+			scanout_memory_range(s, regs, &offset, &length);
+			src->domain_pre(src->ctx, offset, length, (r.status & VI_TYPE_MASK) == VI_TYPE_RGBA8888 ? 2 : 1);
+		}
 		E.w = r.max_x + 2 + 4 + divot * 2;
 		E.h = r.max_y + 1 + 4;
 		E.t = (texel_t *)malloc(sizeof(texel_t) * (size_t)E.w * (size_t)E.h);
@@ -541,8 +607,7 @@ int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, con
 		int x_off = divot ? -3 : -2, y_off = -2;
 		for (int y = 0; y < E.h; y++)
 			for (int x = 0; x < E.w; x++)
-				E.t[(size_t)y * E.w + x] =
-				    fetch_color(rdram, hidden, rdram_size, r.status, fb_offset, r.vi_width, x + x_off, y + y_off);
+				E.t[(size_t)y * E.w + x] = fetch_color(src, r.status, fb_offset, r.vi_width, x + x_off, y + y_off);
 
 		for (int l = 0; l < 2; l++) {
 			A[l].w = r.max_x + 3 + divot * 2;
@@ -572,8 +637,8 @@ int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, con
 
 	/* Stage 4: scale_stage. */
 	int serrate = (r.status & VI_SERRATE) != 0 && !opts->upscale_deinterlacing;
-	int W = SCANOUT_WIDTH;
-	int H = (r.is_pal ? V_RES_PAL : V_RES_NTSC) >> (serrate ? 0 : 1);
+	int W = SCANOUT_WIDTH * SF;
+	int H = ((r.is_pal ? V_RES_PAL : V_RES_NTSC) >> (serrate ? 0 : 1)) * SF;
 	int crop_l, crop_r, crop_t, crop_b;
 	if (opts->crop_enable) {
 		crop_l = (int)opts->crop_left;
@@ -590,6 +655,10 @@ int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, con
 		crop_l = crop_r = cx;
 		crop_t = crop_b = cy;
 	}
+	crop_l *= SF;
+	crop_r *= SF;
+	crop_t *= SF;
+	crop_b *= SF;
 	int OW = W, OH = H;
 	if (crop_l + crop_r < W && crop_t + crop_b < H) {
 		OW = W - (crop_l + crop_r);
@@ -597,14 +666,14 @@ int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, con
 	}
 
 	int v_start = r.v_start, v_res = r.v_res;
-	int serrate_shift = 0, serrate_mask = 0, serrate_select = 0, info_y_shift = 0;
+	int serrate_shift = 0, serrate_mask = 0, serrate_select = 0, info_y_shift = L;
 	if (serrate) {
 		v_start *= 2;
 		v_res *= 2;
 		serrate_shift = 1;
 		serrate_mask = 1;
 		serrate_select = r.v_current_line == 0;
-		info_y_shift = 1;
+		info_y_shift++;
 	}
 
 	memset(out, 0, sizeof(uint32_t) * (size_t)OW * (size_t)OH);
@@ -707,6 +776,13 @@ int vi_scanout(vi_state_t *s, const uint32_t regs[14], const uint8_t *rdram, con
 
 	*width = (unsigned)OW;
 	*height = (unsigned)OH;
+	if ((size_t)OW * (size_t)OH > s->prev_cap) {
+		free(s->prev);
+		s->prev_cap = (size_t)OW * (size_t)OH;
+		s->prev = (uint32_t *)malloc(sizeof(uint32_t) * s->prev_cap);
+		if (!s->prev)
+			abort();
+	}
 	memcpy(s->prev, out, sizeof(uint32_t) * (size_t)OW * (size_t)OH);
 	s->prev_width = (unsigned)OW;
 	s->prev_height = (unsigned)OH;

@@ -234,6 +234,11 @@ class CommandList:
             m |= (v & 0x1FF) << (45 - 9 * i)
         self.raw([OP_SET_CONVERT << 24 | (m >> 32) & 0xFFFFFF, m & 0xFFFFFFFF])
 
+    def set_quirks(self, native_resolution_tex_rect: bool = True, native_texture_lod: bool = False):
+        """MetaSetQuirks (rdp_device.cpp:948-968): byte 0 and byte 1 of the
+        Quirks word.  Both matter only to upscaled rendering."""
+        self.raw([0x04 << 24, int(native_resolution_tex_rect) | int(native_texture_lod) << 8])
+
     def sync_full(self):
         self.raw([OP_SYNC_FULL << 24, 0])
 

@@ -41,35 +41,46 @@ def suite_options(**kw):
     return oracle.ScanoutOptions.default(blend_previous_frame=1, upscale_deinterlacing=0, **kw)
 
 
-def per_scanline_reference(crop=None):
-    """The expected image of run_per_scanline_xh_vi (red channel only)."""
-    ref = np.zeros((240, 640), dtype=np.int64)
+def per_scanline_reference(crop=None, sf=1):
+    """The expected image of run_per_scanline_xh_vi (red channel only); sf
+    is its scale_factor (2 for the upscaled variant): the image is sf times
+    larger and a sample at x_lo of the larger image reads native pixel
+    x_lo / sf."""
+    ref = np.zeros((240 * sf, 640 * sf), dtype=np.int64)
 
-    def region(lines, h_start, h_end, left_clamp, right_clamp, x_add, x_start):
+    def region(y0, y1, h_start, h_end, left_clamp, right_clamp, x_add, x_start):
         x_base = h_start
         if not left_clamp:
             h_start += 8
         if not right_clamp:
             h_end -= 7
-        for x in range(max(h_start, 0), min(h_end, 640)):
-            sx = (x - x_base) * x_add + x_start
+        x_begin, x_end = max(h_start, 0) * sf, min(h_end, 640) * sf
+        for x in range(x_begin, x_end):
+            sx = (x - x_base * sf) * x_add + x_start * sf
             frac = (sx >> 5) & 31
             lo = sx >> 10
-            ref[lines, x] = (lo * (32 - frac) + (lo + 1) * frac + 16) >> 5
+            hi = lo + 1
+            lo, hi = lo // sf, hi // sf
+            ref[y0 * sf:y1 * sf, x] = (lo * (32 - frac) + hi * frac + 16) >> 5
 
-    region(slice(20, 50), 0, 320, False, False, 256, 0)
-    region(slice(50, 100), 0, 640, False, False, 240, 200)
-    region(slice(100, 150), -8, 648, True, True, 220, 400)
-    region(slice(150, 200), 8, 640, False, True, 210, 600)
+    region(20, 50, 0, 320, False, False, 256, 0)
+    region(50, 100, 0, 640, False, False, 240, 200)
+    region(100, 150, -8, 648, True, True, 220, 400)
+    region(150, 200, 8, 640, False, True, 210, 600)
     if crop:
-        l, r, t, b = crop
-        ref = ref[t:240 - b, l:640 - r]
+        l, r, t, b = (c * sf for c in crop)
+        ref = ref[t:240 * sf - b, l:640 * sf - r]
     return ref
 
 
-@pytest.mark.parametrize("crop", [None, (9, 11, 10, 12)])
-def test_per_scanline_xh_known_answer(oracle_mod, crop):
+@pytest.mark.parametrize("crop,factor", [(None, 1), ((9, 11, 10, 12), 1), (None, 2), ((9, 11, 10, 12), 2)])
+def test_per_scanline_xh_known_answer(oracle_mod, crop, factor):
+    """factor 2: the reference's upscaled variant (state.gpu_scaled); the
+    framebuffer is written to native RDRAM only and reaches the sample
+    planes through the scanout's domain update."""
     o = oracle_mod.Oracle(rs.RDRAM_SIZE)
+    if factor > 1:
+        o.enable_upscaling(factor)
     default_vi(o, fmt=RGBA8888, aa=AA_RESAMP_ONLY)
     fb = np.zeros(240 * 200, dtype=np.uint32)
     fb[:] = (np.tile(np.arange(200, dtype=np.uint32), 240) << 24)
@@ -90,7 +101,7 @@ def test_per_scanline_xh_known_answer(oracle_mod, crop):
     if crop:
         kw = dict(crop_enable=1, crop_left=crop[0], crop_right=crop[1], crop_top=crop[2], crop_bottom=crop[3])
     img = o.scanout(suite_options(**kw))
-    ref = per_scanline_reference(crop)
+    ref = per_scanline_reference(crop, factor)
     assert img.shape == ref.shape
     rgb = np.stack([img & 0xFF, (img >> 8) & 0xFF, (img >> 16) & 0xFF], axis=-1).astype(np.int64)
     assert np.array_equal(rgb[:, :, 0], ref)
