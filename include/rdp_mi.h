@@ -38,8 +38,15 @@ enum {
 	RDP_MI_ERROR_LIMIT = -6,
 };
 
-/* Flags accepted by rdp_mi_create (rdp_device.hpp:43-51).  Upscaling flags
- * are rejected with RDP_MI_ERROR_UNSUPPORTED (upscaled rendering is not built). */
+/* Flags accepted by rdp_mi_create (rdp_device.hpp:43-51).  One upscaling
+ * flag renders every render pass whose colour image is RGBA5551 or RGBA8888
+ * and at least 256 pixels wide a second time at 2, 4 or 8 times the
+ * resolution into factor^2 sample planes, and the VI scans out from those
+ * planes at factor times the size (rdp_device.cpp:188-203, rdp_renderer.cpp:
+ * 545-601, 2223-2342, video_interface.cpp:1215-1318).  Native RDRAM is
+ * rendered exactly as without the flag.  Not with host-coherent RDRAM
+ * (RDP_MI_ERROR_UNSUPPORTED).  Super-sampled readback into native RDRAM
+ * and the VI's downscale / upscale-deinterlace blits are not offered. */
 enum {
 	RDP_MI_FLAG_HOST_VISIBLE_HIDDEN_RDRAM = 1 << 0,
 	RDP_MI_FLAG_HOST_VISIBLE_TMEM = 1 << 1,
@@ -150,13 +157,31 @@ size_t rdp_mi_get_rdram_size(const rdp_mi_processor *p);
 size_t rdp_mi_get_hidden_rdram_size(const rdp_mi_processor *p);
 void *rdp_mi_get_tmem(rdp_mi_processor *p);
 
+/* Renderer::get_scaling_factor (rdp_renderer.hpp): 1, or the factor of the
+ * processor's upscaling flag. */
+unsigned rdp_mi_get_upscaling_factor(const rdp_mi_processor *p);
+
+/* Copies of the upscaled domain (after idle or a wait, as with
+ * rdp_mi_begin_read_rdram).  The reference keeps these buffers internal to
+ * its renderer (upscaling_multisampled_rdram / _hidden_rdram,
+ * rdp_renderer.cpp:566-593); they are exported here for inspection and
+ * tests.  The RDRAM planes are factor^2 arrays of rdram_size bytes laid out
+ * like RDRAM, plane sy * factor + sx holding sample (sx, sy) of each pixel;
+ * the hidden planes are factor^2 arrays of rdram_size / 2 bytes.  size bytes
+ * from byte offset into the concatenated planes.  Returns 0,
+ * RDP_MI_ERROR_UNSUPPORTED for a processor without upscaling or
+ * RDP_MI_ERROR_INVALID_ARGUMENT for a range outside the planes. */
+int rdp_mi_read_upscaled_rdram(rdp_mi_processor *p, size_t offset, void *dst, size_t size);
+int rdp_mi_read_upscaled_hidden_rdram(rdp_mi_processor *p, size_t offset, void *dst, size_t size);
+
 /* CommandProcessor::set_vi_register, rdp_device.cpp:970-974. */
 void rdp_mi_set_vi_register(rdp_mi_processor *p, unsigned reg, uint32_t value);
 
 /* CommandProcessor::scanout_sync, rdp_device.cpp:1159-1182: drains work,
  * scans out through the VI and writes width*height RGBA8 words (alpha 255)
- * into out, which must hold 640 * 576 words.  A blank or invalid frame
- * gives 0x0. */
+ * into out, which must hold 640 * 576 words (times factor^2 for an
+ * upscaling processor, whose image is factor times larger in each
+ * direction).  A blank or invalid frame gives 0x0. */
 int rdp_mi_scanout_sync(rdp_mi_processor *p, uint32_t *out, unsigned *width, unsigned *height);
 
 /* ScanoutOptions (video_interface.hpp:31-80) without upscaling, downscaling
@@ -184,7 +209,8 @@ int rdp_mi_scanout_sync_ex(rdp_mi_processor *p, const rdp_mi_scanout_options *op
                            unsigned *height);
 
 /* VIScanoutBuffer (rdp_device.hpp:53-58): a host-readable image of one
- * scanout with a fence.  One buffer holds up to 640 x 576 RGBA8 words. */
+ * scanout with a fence.  One buffer holds up to 640 x 576 RGBA8 words, and
+ * grows on first use for an upscaling processor's larger image. */
 typedef struct rdp_mi_scanout_buffer rdp_mi_scanout_buffer;
 int rdp_mi_scanout_buffer_create(rdp_mi_scanout_buffer **out);
 void rdp_mi_scanout_buffer_destroy(rdp_mi_scanout_buffer *b);

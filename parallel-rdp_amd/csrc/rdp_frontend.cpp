@@ -34,7 +34,20 @@ void Frontend::flush()
 		cur.fb_fmt = fb_fmt;
 		cur.depth_addr = depth_addr;
 		cur.marks.clear();
-		done.push_back(std::move(cur));
+		if (scale_log2) {
+			// render_pass_is_upscaled / should_render_upscaled
+			// (rdp_renderer.cpp:2988-3010): a pass that renders runs the
+			// domain's pre update over its region; a 16/32-bit colour pass at
+			// least 256 pixels wide is then rendered again at scale.
+			const bool renders = fb_width && ref.deduced_height && ref.span_jobs;
+			const bool upscaled = renders && (fb_fmt == FB_RGBA5551 || fb_fmt == FB_RGBA8888) && fb_width >= 256;
+			cur.domain_height = renders ? ref.deduced_height : 0;
+			cur.domain_pre = renders;
+			cur.domain_post = upscaled;
+			done.push_back(std::move(cur));
+			if (upscaled) done.push_back(upscaled_copy(done.back()));
+		} else
+			done.push_back(std::move(cur));
 	}
 	cur = HostPass();
 	state_dirty = true;
@@ -42,6 +55,41 @@ void Frontend::flush()
 	depth_write_pending = false;
 	if (pass_end_log && ref.prims) pass_end_log->push_back(prim_counter);
 	ref.reset();
+}
+
+// The same draws at S = 2^scale_log2 times the resolution
+// (submit_render_pass_upscaled, rdp_renderer.cpp:2287-2342): lines
+// S * ylo .. S * yhi + S - 1 per primitive (span_setup.comp:89-95), an
+// S-times framebuffer, no TMEM uploads of its own (it reads the native
+// pass's TMEM snapshots).
+HostPass Frontend::upscaled_copy(const HostPass &n) const
+{
+	const int32_t S = 1 << scale_log2;
+	HostPass u;
+	u.words = n.words;
+	u.states = n.states;
+	u.prims = n.prims;
+	u.fb_addr = n.fb_addr;
+	u.fb_width = n.fb_width * (uint32_t)S;
+	u.fb_fmt = n.fb_fmt;
+	u.depth_addr = n.depth_addr;
+	u.raster_class = n.raster_class;
+	u.scale_log2 = scale_log2;
+	u.native_uploads = (uint32_t)n.uploads.size();
+	uint32_t spans = 0, rows = 0, height = 0;
+	for (PrimRecord &r : u.prims) {
+		r.ylo *= S;
+		r.yhi = r.yhi * S + (S - 1);
+		r.span_offset = spans;
+		r.tilerow_offset = rows;
+		spans += (uint32_t)(r.yhi - r.ylo + 2);
+		rows += (uint32_t)((r.yhi >> 3) - (r.ylo >> 3) + 1);
+		height = std::max(height, (uint32_t)(r.yhi + 1));
+	}
+	u.num_spans = spans;
+	u.num_tilerows = rows;
+	u.fb_height = height;
+	return u;
 }
 
 // ---------------------------------------------------------------------------
@@ -88,43 +136,46 @@ struct EdgeSetup {
 	bool flip;
 };
 
-static void interpolate_x_64(const EdgeSetup &s, int y, int64_t &left, int64_t &right)
+// interpolate_x, rdp_renderer.cpp:962-992: the edges at sub-scanline y of
+// the S-times domain (S = 1 without upscaling).
+static void interpolate_x_64(const EdgeSetup &s, int y, int64_t S, int64_t &left, int64_t &right)
 {
-	// interpolate_x, rdp_renderer.cpp:962-992 (no upscaling).
-	const int64_t yh_base = s.yh & ~3, ym_base = s.ym;
-	const int64_t xh = (int64_t)s.xh + ((int64_t)y - yh_base) * s.dxhdy;
-	const int64_t xm = (int64_t)s.xm + ((int64_t)y - yh_base) * s.dxmdy;
-	int64_t xl = (int64_t)s.xl + ((int64_t)y - ym_base) * s.dxldy;
-	if (y < s.ym) xl = xm;
+	const int64_t yh_base = (int64_t)(s.yh & ~3) * S, ym_base = (int64_t)s.ym * S;
+	const int64_t xh = S * s.xh + ((int64_t)y - yh_base) * s.dxhdy;
+	const int64_t xm = S * s.xm + ((int64_t)y - yh_base) * s.dxmdy;
+	int64_t xl = S * s.xl + ((int64_t)y - ym_base) * s.dxldy;
+	if (y < S * s.ym) xl = xm;
 	const int64_t hs = xh >> 15, ls = xl >> 15;
 	left = std::min<int64_t>(std::max<int64_t>(s.flip ? hs : ls, INT32_MIN), INT32_MAX);
 	right = std::min<int64_t>(std::max<int64_t>(s.flip ? ls : hs, INT32_MIN), INT32_MAX);
 }
 
-// Renderer::compute_conservative_max_num_tiles, rdp_renderer.cpp:994-1060.
-static uint32_t conservative_tile_count(const EdgeSetup &s, const uint32_t scissor[4])
+// Renderer::compute_conservative_max_num_tiles, rdp_renderer.cpp:994-1060:
+// 8x8 tiles of the S-times domain, S the upscaling factor.
+static uint32_t conservative_tile_count(const EdgeSetup &s, const uint32_t scissor[4], int S)
 {
 	if (s.yl <= s.yh) return 0;
-	const int start_y = std::max(s.yh & ~3, (int)scissor[1]);
-	const int end_y = std::min((s.yl - 1) | 3, (int)scissor[3] - 1);
+	const int start_y = std::max(s.yh & ~3, (int)scissor[1]) * S;
+	const int end_y = std::min((s.yl - 1) | 3, (int)scissor[3] - 1) * S;
 	if (end_y < start_y) return 0;
 	int64_t l[4], r[4];
-	interpolate_x_64(s, start_y, l[0], r[0]);
-	interpolate_x_64(s, end_y, l[1], r[1]);
+	interpolate_x_64(s, start_y, S, l[0], r[0]);
+	interpolate_x_64(s, end_y, S, l[1], r[1]);
 	l[2] = l[3] = l[0];
 	r[2] = r[3] = r[0];
-	if (s.ym > start_y && s.ym < end_y) {
-		interpolate_x_64(s, s.ym, l[2], r[2]);
-		interpolate_x_64(s, s.ym - 1, l[3], r[3]);
+	const int ym = S * s.ym;
+	if (ym > start_y && ym < end_y) {
+		interpolate_x_64(s, ym, S, l[2], r[2]);
+		interpolate_x_64(s, ym - 1, S, l[3], r[3]);
 	}
 	int64_t sx = std::min(std::min(l[0], l[1]), std::min(l[2], l[3]));
 	int64_t ex = std::max(std::max(r[0], r[1]), std::max(r[2], r[3]));
-	if (std::max(sx < 0 ? -sx : sx, ex < 0 ? -ex : ex) > 2047) {
+	if (std::max(sx < 0 ? -sx : sx, ex < 0 ? -ex : ex) > 2047 * (int64_t)S) {
 		sx = 0;
 		ex = 0x7fffffff;
 	}
-	sx = std::max<int64_t>(sx, scissor[0] >> 2);
-	ex = std::min<int64_t>(ex, (int64_t)((scissor[2] + 3) >> 2) - 1);
+	sx = std::max<int64_t>(sx, (int64_t)S * (scissor[0] >> 2));
+	ex = std::min<int64_t>(ex, (int64_t)S * ((scissor[2] + 3) >> 2) - 1);
 	if (ex < sx) return 0;
 	return (uint32_t)((ex / 8 - sx / 8 + 1) * (end_y / 32 - start_y / 32 + 1));
 }
@@ -160,7 +211,8 @@ bool Frontend::ref_account_draw(unsigned op, const uint32_t *w)
 	}
 	if (e.ym < (e.yh & ~3)) e.ym = 0x7fff;
 
-	ref.max_shaded_tiles += conservative_tile_count(e, scissor);
+	const int S = 1 << scale_log2;
+	ref.max_shaded_tiles += conservative_tile_count(e, scissor, S);
 	const int max_sub = std::min(e.yl - 1, (int)scissor[3] - 1);
 	ref.deduced_height = std::max<uint32_t>(ref.deduced_height, (uint32_t)std::max((max_sub >> 2) + 1, 0));
 	const int min_line = std::max(e.yh, (int)scissor[1]) >> 2, max_line = max_sub >> 2;
@@ -177,8 +229,11 @@ bool Frontend::ref_account_draw(unsigned op, const uint32_t *w)
 	if (db_flags & DB_DEPTH_UPDATE) ref.depth_write_pending = true;
 	ref.prims++;
 
+	// The tile-instance capacity grows with the factor squared (rdp_renderer.cpp:63-67).
+	const uint32_t tiles_side = 128u * (uint32_t)S;
 	return ref.statics.size() == 64 || ref.depth_blends.size() == 64 || ref.tiles.size() + 8 > 256 ||
-	       ref.prims == 256 || ref.span_jobs * 64 + 1024 > 32 * 1024 || ref.max_shaded_tiles + 128 * 128 > 0x8000;
+	       ref.prims == 256 || ref.span_jobs * 64 + 1024 > 32 * 1024 ||
+	       ref.max_shaded_tiles + tiles_side * tiles_side > (uint32_t)(S * S) * 0x8000u;
 }
 
 // Adds the draw's static, depth/blend and tile keys to the pass caches:
@@ -272,7 +327,9 @@ void Frontend::add_ref_keys(unsigned setup_tile)
 // The reference ends a render pass here (flush_queues).
 void Frontend::ref_pass_end()
 {
-	if (fb_fmt == FB_I8 || fb_addr == depth_addr) {
+	// With upscaling, every reference pass is a pass of its own: the domain
+	// updates run over each pass's own framebuffer region.
+	if (fb_fmt == FB_I8 || fb_addr == depth_addr || scale_log2) {
 		flush();
 		return;
 	}

@@ -46,6 +46,15 @@ struct HostPass {
 	uint32_t fb_addr = 0, fb_width = 0, fb_fmt = 0, depth_addr = 0, fb_height = 0;
 	uint32_t num_spans = 0, num_tilerows = 0;
 	uint32_t raster_class = 0;  // highest raster occupancy class of its primitives
+	// Upscaled rendering.  A native pass that renders: domain_pre runs the
+	// domain's pre update over fb_width x domain_height before its raster,
+	// domain_post the post update after it.  scale_log2 > 0: the pass is the
+	// upscaled copy of the native pass before it (fb_width, fb_height and
+	// the primitives' lines already scaled), rendering into the sample planes
+	// with that pass's native_uploads + 1 TMEM snapshots.
+	bool domain_pre = false, domain_post = false;
+	uint32_t domain_height = 0;
+	uint32_t scale_log2 = 0, native_uploads = 0;
 	bool empty() const { return prims.empty() && uploads.empty(); }
 };
 
@@ -104,6 +113,10 @@ public:
 	// pass of the reference's renderer that held a draw (for tests).
 	std::vector<uint64_t> *pass_end_log = nullptr;
 
+	// log2 of the upscaling factor (0: native only); set before the first
+	// command.
+	uint32_t scale_log2 = 0;
+
 private:
 	uint32_t rdram_size;
 
@@ -151,6 +164,7 @@ private:
 	void ref_pass_end();
 	void split_at_last_mark();
 	void split_or_flush();
+	HostPass upscaled_copy(const HostPass &native) const;
 	void validate_draw() const;
 	void report(uint32_t err, const char *msg) const
 	{

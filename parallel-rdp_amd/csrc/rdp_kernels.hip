@@ -23,6 +23,7 @@
 // complement (compiled with -fwrapv).  Semantics follow the reference shaders
 // cited at each function.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 #include "rdp_types.hpp"
@@ -281,19 +282,28 @@ DEV void decode_primitive(WordPtr w, uint32_t op, const State &st, PrimSetup &s)
 
 DEV int32_t quantize_x(int32_t x) { return (x >> 12) | ((x & 0xfff) != 0); }
 
-// span_setup.comp:87-227 for one scanline (no upscaling).
-template <typename Scissor>
-DEV void compute_span(const PrimSetup &s, Scissor sc, int y, SpanSetup &out)
+// interpolate_snapped (span_setup.comp:80-85): whole native lines step by
+// the full gradient, the lines between them by the gradient >> L.
+DEV int32_t interpolate_snapped(int32_t dvalue, int dy, int L)
 {
+	return (dy >> L) * dvalue + (dy & ((1 << L) - 1)) * (dvalue >> L);
+}
+
+// span_setup.comp:87-227 for one scanline; L is SCALING_LOG2 (0 natively;
+// otherwise y counts lines of the 2^L-times domain).
+template <typename Scissor>
+DEV void compute_span(const PrimSetup &s, Scissor sc, int y, SpanSetup &out, int L)
+{
+	const int S = 1 << L;
 	const bool flip = (s.flags & TRI_FLIP) != 0;
 	{
 		const bool do_offset = (s.flags & TRI_DO_OFFSET) != 0;
 		const bool skip_xfrac = (s.flags & TRI_SKIP_XFRAC) != 0;
-		int dy = y - (s.yh >> 2);
-		int32_t xh = s.xh + dy * (s.dxhdy << 2);
+		int dy = y - (s.yh >> 2) * S;
+		int32_t xh = s.xh * S + dy * (s.dxhdy << 2);
 		int32_t diff_c[4] = { 0, 0, 0, 0 }, diff_t[4] = { 0, 0, 0, 0 };
 		if (do_offset) {
-			xh += 3 * s.dxhdy;
+			xh += (3 * S) * s.dxhdy;
 #pragma unroll
 			for (int i = 0; i < 4; i++) {
 				int32_t deh = s.drgba_de[i] & ~0x1ff, dyh = s.drgba_dy[i] & ~0x1ff;
@@ -307,29 +317,32 @@ DEV void compute_span(const PrimSetup &s, Scissor sc, int y, SpanSetup &out)
 		int32_t xfrac = skip_xfrac ? 0 : ((xh >> 7) & 0xff);
 #pragma unroll
 		for (int i = 0; i < 4; i++) {
-			int32_t v = s.rgba[i] + dy * s.drgba_de[i];
-			out.rgba[i] = ((v & ~0x1ff) + diff_c[i] - xfrac * ((s.drgba_dx[i] >> 8) & ~1)) & ~0x3ff;
-			v = s.stzw[i] + dy * s.dstzw_de[i];
-			out.stzw[i] = ((v & ~0x1ff) + diff_t[i] - xfrac * ((s.dstzw_dx[i] >> 8) & ~1)) & ~0x3ff;
+			int32_t v = s.rgba[i] + interpolate_snapped(s.drgba_de[i], dy, L);
+			out.rgba[i] =
+			    ((v & ~0x1ff) + diff_c[i] - interpolate_snapped((s.drgba_dx[i] >> 8) & ~1, xfrac, L)) & ~0x3ff;
+			v = s.stzw[i] + interpolate_snapped(s.dstzw_de[i], dy, L);
+			out.stzw[i] =
+			    ((v & ~0x1ff) + diff_t[i] - interpolate_snapped((s.dstzw_dx[i] >> 8) & ~1, xfrac, L)) & ~0x3ff;
 		}
 	}
 
-	const int yh_base = s.yh & ~3;
-	const int ylo = imax(s.yh, (int)sc[1]);
-	const int yhi = imin(s.yl, (int)sc[3]);
-	const int32_t lo_sc = (int32_t)(sc[0] << 1), hi_sc = (int32_t)(sc[2] << 1);
+	const int yh_base = (s.yh & ~3) * S;
+	const int ym_base = s.ym * S;
+	const int ylo = imax(s.yh, (int)sc[1]) * S;
+	const int yhi = imin(s.yl, (int)sc[3]) * S;
+	const int32_t lo_sc = S * (int32_t)(sc[0] << 1), hi_sc = S * (int32_t)(sc[2] << 1);
 	bool all_over = true, all_under = true, all_invalid = true;
 	int32_t start_x = 0x7fffffff, end_x = (int32_t)0x80000000;
 #pragma unroll
 	for (int i = 0; i < 4; i++) {
 		int ys = y * 4 + i;
 		bool clip_y = ys < ylo || ys >= yhi;
-		int32_t xh = s.xh + (ys - yh_base) * s.dxhdy;
-		int32_t xm = s.xm + (ys - yh_base) * s.dxmdy;
-		int32_t xl = s.xl + (ys - s.ym) * s.dxldy;
-		if (ys < s.ym) xl = xm;
-		xl = sext(xl, 27);
-		xh = sext(xh, 27);
+		int32_t xh = s.xh * S + (ys - yh_base) * s.dxhdy;
+		int32_t xm = s.xm * S + (ys - yh_base) * s.dxmdy;
+		int32_t xl = s.xl * S + (ys - ym_base) * s.dxldy;
+		if (ys < ym_base) xl = xm;
+		xl = sext(xl, 27 + L);
+		xh = sext(xh, 27 + L);
 		int32_t xh_s = quantize_x(xh), xl_s = quantize_x(xl);
 		int32_t l = flip ? xh_s : xl_s;
 		int32_t r = flip ? xl_s : xh_s;
@@ -352,7 +365,7 @@ DEV void compute_span(const PrimSetup &s, Scissor sc, int y, SpanSetup &out)
 	out.start_x = start_x >> 3;
 	out.end_x = end_x >> 3;
 	bool valid = !all_invalid && !all_over && !all_under;
-	if ((s.flags & TRI_INTERLACE_FIELD) && ((y & 1) != (int)((s.flags & TRI_INTERLACE_KEEP_ODD) != 0)))
+	if ((s.flags & TRI_INTERLACE_FIELD) && (((y >> L) & 1) != (int)((s.flags & TRI_INTERLACE_KEEP_ODD) != 0)))
 		valid = false;
 	out.valid = valid ? 1 : 0;
 	out.lodlength = (int16_t)(flip ? (out.end_x - out.base_x) : (out.base_x - out.start_x));
@@ -431,7 +444,7 @@ __global__ __launch_bounds__(64) void k_setup(const PassDesc *__restrict__ passe
 	for (int k = 0; k < rows; k++) P.tilerow_prim[s.tilerow_offset + (uint32_t)k] = (uint16_t)prim;
 	if (((s.yhi + 1) & 7) == 0) {
 		SpanSetup sp;
-		compute_span(s, st.scissor, s.yhi + 1, sp);
+		compute_span(s, st.scissor, s.yhi + 1, sp, (int)P.scale_log2);
 		P.spans[s.span_offset + (uint32_t)(s.yhi + 1 - s.ylo)] = sp;
 	}
 }
@@ -453,7 +466,7 @@ void k_span(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ it
 		const int y = ((s.ylo >> 3) + (int)(e - s.tilerow_offset)) * 8 + (lane & 7);
 		if (y >= s.ylo && y <= s.yhi + 1) {
 			SpanSetup sp;
-			compute_span(s, s.scissor, y, sp);
+			compute_span(s, s.scissor, y, sp, (int)P.scale_log2);
 			P.spans[s.span_offset + (uint32_t)(y - s.ylo)] = sp;
 			if (y <= s.yhi && sp.valid) {
 				const int lo = imax(sp.start_x, 0), hi = imin(sp.end_x, (int)P.fb_width - 1);
@@ -586,8 +599,8 @@ __global__ __launch_bounds__(64) void k_bin(const PassDesc *__restrict__ passes,
 {
 	__shared__ unsigned long long s_diff[65];
 	const uint32_t item = items[blockIdx.x];
-	const PassDesc &P = passes[item >> 14];
-	const uint32_t ty = (item >> 7) & 0x7f, gb = (item & 0x7f) * 8;
+	const PassDesc &P = passes[item >> 17];
+	const uint32_t ty = (item >> 4) & 0x1fff, gb = (item & 0xf) * 8;
 	const int lane = threadIdx.x;
 	const uint32_t tiles_x = P.tiles_x, G = P.num_groups;
 	const uint32_t real_groups = (P.num_prims + 63) / 64;
@@ -1629,10 +1642,12 @@ struct KnownBlend {
 };
 
 template <class V>
+// L: SCALING_LOG2 of the pass (0 natively, where every term below that
+// carries it reduces to the native form at compile time).
 DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST TileDesc *tiles,
                      const Tex &tx, uint32_t fb_fmt, const SpanSetup &span, const AS_GLOBAL SpanSetup *next_span, int x,
                      int y, const Pixel &px,
-                     Shaded &out)
+                     Shaded &out, int L)
 {
 	const uint32_t f = h.ras_flags;
 	const bool tlut = V::ras(f, RAS_TLUT), tlut_type = V::ras(f, RAS_TLUT_TYPE), sample_quad = V::ras(f, RAS_SAMPLE_QUAD);
@@ -1652,7 +1667,8 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 
 	if (V::ras(f, RAS_COPY)) {
 		if (!span.valid || x < span.start_x || x > span.end_x) return false;
-		int dxp = flip ? (x - span.start_x) : (span.end_x - x);
+		// interpolate_st_copy: the samples of a pixel repeat its texel.
+		int dxp = (flip ? (x - span.start_x) : (span.end_x - x)) >> L;
 		uint32_t dx_mask, dx_shift, fb_size;
 		switch (fb_fmt) {
 		case FB_I4: fb_size = 0; dx_mask = 0; dx_shift = 0; break;
@@ -1698,10 +1714,12 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 
 	int32_t z;
 	{
-		int32_t zz = span.stzw[2] + ps->dstzw_dx[2] * dx;
-		int32_t sn = (zz >> 10) << 2;
+		// Whole native pixels step by the full gradient, samples by the
+		// gradient >> L (interpolation.h:135).
+		int32_t zz = span.stzw[2] + ps->dstzw_dx[2] * (dx >> L) + (ps->dstzw_dx[2] >> L) * (dx & ((1 << L) - 1));
+		int32_t sn = (zz >> 10) << (2 + L);
 		sn += xoff * (ps->dstzw_dx[2] >> 10) + yoff * (ps->dstzw_dy[2] >> 10);
-		z = clamp_z(sn >> 5);
+		z = clamp_z(sn >> (5 + L));
 	}
 	// Early depth rejection: shading, texturing, LOD and the combiner only
 	// feed a fragment the depth test is known to discard.
@@ -1711,25 +1729,29 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 	int32_t shade[4];
 #pragma unroll
 	for (int i = 0; i < 4; i++) {
-		int32_t v = span.rgba[i] + (ps->drgba_dx[i] & ~0x1f) * dx;
-		int32_t sn = (v >> 14) << 2;
+		int32_t v = span.rgba[i] + ((ps->drgba_dx[i] & ~0x1f) >> L) * dx;
+		int32_t sn = (v >> 14) << (2 + L);
 		sn += xoff * (ps->drgba_dx[i] >> 14) + yoff * (ps->drgba_dy[i] >> 14);
-		shade[i] = clamp_9bit(sn >> 4);
+		shade[i] = clamp_9bit(sn >> (4 + L));
 	}
 
 	const bool uses_lod = V::ras(f, RAS_USES_LOD);
 	int32_t st0[2], stdx[2] = { 0, 0 }, stdy[2] = { 0, 0 };
 	bool povf = false;
 	{
-		int32_t s = span.stzw[0] + (ps->dstzw_dx[0] & ~0x1f) * dx;
-		int32_t t = span.stzw[1] + (ps->dstzw_dx[1] & ~0x1f) * dx;
-		int32_t w = span.stzw[3] + (ps->dstzw_dx[3] & ~0x1f) * dx;
+		int32_t s = span.stzw[0] + ((ps->dstzw_dx[0] & ~0x1f) >> L) * dx;
+		int32_t t = span.stzw[1] + ((ps->dstzw_dx[1] & ~0x1f) >> L) * dx;
+		int32_t w = span.stzw[3] + ((ps->dstzw_dx[3] & ~0x1f) >> L) * dx;
 		stw_to_st(s, t, w, perspective, st0[0], st0[1], povf);
 		if (uses_lod) {
-			stw_to_st(s + dir * (ps->dstzw_dx[0] & ~0x1f), t + dir * (ps->dstzw_dx[1] & ~0x1f),
-			          w + dir * (ps->dstzw_dx[3] & ~0x1f), perspective, stdx[0], stdx[1], povf);
-			stw_to_st(s + (ps->dstzw_dy[0] & ~0x7fff), t + (ps->dstzw_dy[1] & ~0x7fff), w + (ps->dstzw_dy[3] & ~0x7fff),
-			          perspective, stdy[0], stdy[1], povf);
+			// shading.h:192-198: with the native-LOD quirk the LOD neighbours
+			// are a native pixel away; interpolation.h:99-105.
+			const int ydir = (L && (ps->flags & TRI_NATIVE_LOD)) ? (1 << L) : 1;
+			const int tdir = dir * ydir;
+			stw_to_st(s + tdir * ((ps->dstzw_dx[0] & ~0x1f) >> L), t + tdir * ((ps->dstzw_dx[1] & ~0x1f) >> L),
+			          w + tdir * ((ps->dstzw_dx[3] & ~0x1f) >> L), perspective, stdx[0], stdx[1], povf);
+			stw_to_st(s + ydir * ((ps->dstzw_dy[0] & ~0x7fff) >> L), t + ydir * ((ps->dstzw_dy[1] & ~0x7fff) >> L),
+			          w + ydir * ((ps->dstzw_dy[3] & ~0x7fff) >> L), perspective, stdy[0], stdy[1], povf);
 		}
 	}
 
@@ -1770,9 +1792,10 @@ DEV bool shade_pixel(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 			}
 		} else {
 			bool o2 = false;
-			const int d2 = dx + dir;
-			stw_to_st(span.stzw[0] + (ps->dstzw_dx[0] & ~0x1f) * d2, span.stzw[1] + (ps->dstzw_dx[1] & ~0x1f) * d2,
-			          span.stzw[3] + (ps->dstzw_dx[3] & ~0x1f) * d2, perspective, st0[0], st0[1], o2);
+			const int d2 = dx + dir * (1 << L);
+			stw_to_st(span.stzw[0] + ((ps->dstzw_dx[0] & ~0x1f) >> L) * d2,
+			          span.stzw[1] + ((ps->dstzw_dx[1] & ~0x1f) >> L) * d2,
+			          span.stzw[3] + ((ps->dstzw_dx[3] & ~0x1f) >> L) * d2, perspective, st0[0], st0[1], o2);
 		}
 		tile1 = tile0;
 		uses_texel1 = true;
@@ -2044,8 +2067,14 @@ template <class V>
 DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, const AS_CONST DrawState *states,
                      const AS_GLOBAL SpanSetup *spans,
                      const Tex &tex, const AS_GLOBAL uint8_t *lut, uint32_t fmt, bool alias, uint32_t ci, int x, int y,
-                     Pixel &px)
+                     Pixel &px, int L = 0)
 {
+	// shading.h:45-52: a native-resolution primitive shades every sample as
+	// its pixel's first one.
+	if (L && (ps->flags & TRI_DISABLE_UPSCALING)) {
+		x &= ~((1 << L) - 1);
+		y &= ~((1 << L) - 1);
+	}
 	const AS_GLOBAL SpanSetup *sp = spans + (h.span_offset + (uint32_t)(y - h.ylo));
 	// The span's valid flag is tested with the first coverage test in
 	// shade_pixel, so every field the test needs is fetched at once.
@@ -2071,7 +2100,7 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 			for (int i = 0; i < 6; i++) asm volatile("" ::"s"(gt[i]));
 		}
 	}
-	if (!shade_pixel<V>(ps, h, states[h.state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh)) return;
+	if (!shade_pixel<V>(ps, h, states[h.state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh, L)) return;
 	if (sh.coverage & COVERAGE_FILL)
 		fill_pixel(px, fmt, alias, ci, h.fill_color);
 	else if (sh.coverage & COVERAGE_COPY)
@@ -2156,11 +2185,14 @@ DEV void staging_sync()
 // other.  Otherwise a work item is one 8x8 tile and the workgroup one wave.
 // One work item: the whole tile (or, with QUAD, this wave's quadrant of it).
 // s_tmem_all: the workgroup's LDS staging area, 2048 halfwords per wave.
-template <uint32_t CLS, bool QUAD>
+// UP: the pass renders into the sample planes at 2^scale_log2 times the
+// resolution (submit_render_pass_upscaled, rdp_renderer.cpp:2287-2342), and
+// work items are pass << 20 | tile x << 10 | tile y.
+template <uint32_t CLS, bool QUAD, bool UP = false>
 DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_t *s_tmem_all)
 {
-	const PassDesc &P = passes[item >> 14];
-	uint32_t tx = (item >> 7) & 0x7f, ty = item & 0x7f;
+	const PassDesc &P = passes[UP ? (item >> 20) : (item >> 14)];
+	uint32_t tx = UP ? (item >> 10) & 0x3ff : (item >> 7) & 0x7f, ty = UP ? item & 0x3ff : item & 0x7f;
 	const int lane = threadIdx.x & 63;
 	uint16_t *s_tmem = s_tmem_all;
 	if constexpr (QUAD) {
@@ -2177,7 +2209,14 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 	const uint32_t fmt = P.fb_fmt;
 	const bool in_fb = x < (int)P.fb_width && y < (int)P.fb_height;
 	const bool alias = P.fb_addr == P.depth_addr;
-	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
+	// Upscaled: pixel (x, y) is sample (x mod S, y mod S) of native pixel
+	// (x / S, y / S), stored in plane (y mod S) * S + x mod S at the native
+	// pixel's address (memory_interfacing.h:286-362).
+	const int L = UP ? (int)uniform_u32(P.scale_log2) : 0;
+	const uint32_t smask = (1u << L) - 1;
+	const uint32_t slice = UP ? (((uint32_t)y & smask) << L) + ((uint32_t)x & smask) : 0;
+	const size_t plane8 = UP ? (size_t)slice * ((size_t)P.rdram_mask + 1) : 0;
+	const Mem m{ gptr(P.rdram) + plane8, gptr(P.hidden) + (plane8 >> 1), P.rdram_mask };
 	const AS_CONST RasterPrim *rprims = cptr(P.rprims);
 	const AS_CONST DrawState *states = cptr(P.states);
 	const AS_GLOBAL SpanSetup *spans = gptr(P.spans);
@@ -2192,8 +2231,9 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 	case FB_IA88: addr_index = P.fb_addr >> 1; break;
 	default: addr_index = P.fb_addr >> 2; break;
 	}
-	const uint32_t ci = addr_index + P.fb_width * (uint32_t)y + (uint32_t)x;
-	const uint32_t di = (P.depth_addr >> 1) + P.fb_width * (uint32_t)y + (uint32_t)x;
+	const uint32_t nrow = P.fb_width >> L, nx = (uint32_t)x >> L, ny = (uint32_t)y >> L;
+	const uint32_t ci = addr_index + nrow * ny + nx;
+	const uint32_t di = (P.depth_addr >> 1) + nrow * ny + nx;
 
 	Pixel px;
 	px.color_dirty = px.depth_dirty = false;
@@ -2246,13 +2286,13 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 		case id:                                                                                                    \
 			if constexpr ((cls) <= CLS)                                                                             \
 				raster_prim<RasterVariant<(known), (value), (int)(comb), (texfmt), (int)(dbp)>>(ps, h, states, spans, tex, lut, fmt, alias, ci, \
-				                                                                    x, y, px);                        \
+				                                                                    x, y, px, L);                     \
 			break;
 					RDP_RASTER_VARIANTS(RDP_DISPATCH_VARIANT)
 #undef RDP_DISPATCH_VARIANT
 				default:
 					if constexpr (CLS == kRasterClasses - 1)
-						raster_prim<GenericVariant>(ps, h, states, spans, tex, lut, fmt, alias, ci, x, y, px);
+						raster_prim<GenericVariant>(ps, h, states, spans, tex, lut, fmt, alias, ci, x, y, px, L);
 					break;
 				}
 			}
@@ -2263,14 +2303,14 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 	// rather than kept live through the primitive loop.
 	const AS_CONST PassDesc *pe = cptr(&P);
 	asm volatile("" : "+s"(pe));
-	const Mem me{ gptr(pe->rdram), gptr(pe->hidden), pe->rdram_mask };
+	const Mem me{ gptr(pe->rdram) + plane8, gptr(pe->hidden) + (plane8 >> 1), pe->rdram_mask };
 	AS_GLOBAL uint8_t *wm = gptr(pe->wmask);
 	if (in_fb && px.color_dirty) {
 		store_color(me, fmt, ci, px);
 		if (wm) mark_color(wm, pe->rdram_mask, fmt, ci);
 	}
 	if (in_fb && !alias && px.depth_dirty) {
-		const uint32_t de = (pe->depth_addr >> 1) + pe->fb_width * (uint32_t)y + (uint32_t)x;
+		const uint32_t de = (pe->depth_addr >> 1) + (pe->fb_width >> L) * ny + nx;
 		me.wr16i(de, (px.depth << 2) | (px.dz >> 2));
 		me.set_hid(de, px.dz & 3);
 		if (wm) mark_depth(wm, pe->rdram_mask, de);
@@ -2280,12 +2320,12 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 #define RDP_RASTER_WAVES(CLS) \
 	(CLS == 0 ? RDP_RASTER_WAVES_CLASS0 : (CLS == 1 ? RDP_RASTER_WAVES_CLASS1 : RDP_RASTER_WAVES_CLASS2))
 
-template <uint32_t CLS, bool QUAD>
+template <uint32_t CLS, bool QUAD, bool UP = false>
 __global__ __launch_bounds__(QUAD ? 256 : 64) __attribute__((amdgpu_waves_per_eu(RDP_RASTER_WAVES(CLS))))
 void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
 {
 	__shared__ uint16_t s_tmem_all[(QUAD ? 4 : 1) * 2048];
-	raster_tile<CLS, QUAD>(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_tmem_all);
+	raster_tile<CLS, QUAD, UP>(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_tmem_all);
 }
 
 // Passes whose every primitive is a FILL-mode primitive (variant 1, the
@@ -2465,6 +2505,126 @@ hipError_t launch_raster(uint32_t cls, const PassDesc *passes, const uint32_t *i
 		default: hipLaunchKernelGGL((k_raster<2, false>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
 		}
 	}
+	return hipGetLastError();
+}
+
+// Upscaled passes (k_raster<cls, false, true>): 8x8 tiles of the S-times
+// framebuffer.
+hipError_t launch_raster_up(uint32_t cls, const PassDesc *passes, const uint32_t *items, uint32_t n, int32_t xcd_chunk,
+                            hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	switch (cls) {
+	case 0: hipLaunchKernelGGL((k_raster<0, false, true>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+	case 1: hipLaunchKernelGGL((k_raster<1, false, true>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+	default: hipLaunchKernelGGL((k_raster<2, false, true>), dim3(n), dim3(64), 0, s, passes, items, xcd_chunk); break;
+	}
+	return hipGetLastError();
+}
+
+// ==========================================================================
+// Upscaled domain updates: update_upscaled_domain_pre.comp (post = false)
+// and update_upscaled_domain_post.comp (post = true), one lane per element.
+// ==========================================================================
+struct DomainMem {
+	const AS_GLOBAL uint8_t *rdram, *hidden;
+	AS_GLOBAL uint8_t *ref, *planes, *hplanes;
+	uint32_t rdram_size, num_planes;
+};
+
+// index counts host-order elements of 1 << size_log2 bytes.  Pre: where the
+// native element differs from the reference copy, write it and its hidden
+// bits to every plane and update the reference.  Post: reference := native.
+template <int SIZE_LOG2>
+DEV void domain_element(const DomainMem &d, bool post, uint32_t index)
+{
+	using T = typename std::conditional<SIZE_LOG2 == 0, uint8_t, typename std::conditional<SIZE_LOG2 == 1, uint16_t,
+	                                                                                       uint32_t>::type>::type;
+	index &= (d.rdram_size >> SIZE_LOG2) - 1;
+	const T real = ((const AS_GLOBAL T *)d.rdram)[index];
+	AS_GLOBAL T *ref = (AS_GLOBAL T *)d.ref;
+	if (post) {
+		ref[index] = real;
+		return;
+	}
+	if (ref[index] == real) return;
+	const size_t plane_elems = d.rdram_size >> SIZE_LOG2, hplane = d.rdram_size >> 1;
+	for (uint32_t i = 0; i < d.num_planes; i++) {
+		((AS_GLOBAL T *)d.planes)[index + i * plane_elems] = real;
+		AS_GLOBAL uint8_t *uh = d.hplanes + i * hplane;
+		if (SIZE_LOG2 == 0) {
+			const uint32_t m = index ^ 3u;
+			if (m & 1u) uh[m >> 1] = d.hidden[m >> 1];
+		} else if (SIZE_LOG2 == 1) {
+			uh[index ^ 1u] = d.hidden[index ^ 1u];
+		} else {
+			uh[2 * index] = d.hidden[2 * index];
+			uh[2 * index + 1] = d.hidden[2 * index + 1];
+		}
+	}
+	ref[index] = real;
+}
+
+DEV void domain_update(const DomainMem &d, bool post, uint32_t addr, uint32_t depth_addr, uint32_t num_pixels,
+                       uint32_t size_log2, uint32_t i)
+{
+	// submit_update_upscaled_domain (rdp_renderer.cpp:1943-1975): whole words.
+	addr &= ~3u;
+	depth_addr &= ~3u;
+	const uint32_t align = 4u >> size_log2;
+	num_pixels = (num_pixels + align - 1u) & ~(align - 1u);
+	if (i >= num_pixels) return;
+	const uint32_t ci = (addr >> size_log2) + i;
+	switch (size_log2) {
+	case 0: domain_element<0>(d, post, ci); break;
+	case 1: domain_element<1>(d, post, ci); break;
+	default: domain_element<2>(d, post, ci); break;
+	}
+	if (addr != depth_addr) domain_element<1>(d, post, (depth_addr >> 1) + i);
+}
+
+// Per pass: the framebuffer region fb_width x domain_height; work items are
+// pass << 17 | block of 256 elements.
+__global__ __launch_bounds__(256) void k_domain(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items,
+                                                uint32_t post)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 17];
+	const uint32_t i = (item & 0x1ffff) * 256 + threadIdx.x;
+	const uint32_t fmt = P.fb_fmt;
+	const uint32_t size_log2 = fmt == FB_RGBA8888 ? 2 : (fmt == FB_RGBA5551 || fmt == FB_IA88) ? 1 : 0;
+	const DomainMem d{ gptr(P.rdram), gptr(P.hidden), gptr(P.up_ref), gptr(P.up_rdram), gptr(P.up_hidden),
+		               P.rdram_mask + 1, 1u << (2 * P.domain_log2) };
+	domain_update(d, post != 0, P.fb_addr, P.depth_addr, P.fb_width * P.domain_height, size_log2, i);
+}
+
+// One range, colour only (submit_update_upscaled_domain_external,
+// rdp_renderer.cpp:1936-1941: the VI's scanout range before its fetch).
+__global__ __launch_bounds__(256) void k_domain_range(const uint8_t *rdram, const uint8_t *hidden, uint8_t *ref,
+                                                      uint8_t *planes, uint8_t *hplanes, uint32_t rdram_size,
+                                                      uint32_t num_planes, uint32_t addr, uint32_t num_pixels,
+                                                      uint32_t size_log2)
+{
+	const DomainMem d{ gptr(rdram), gptr(hidden), gptr(ref), gptr(planes), gptr(hplanes), rdram_size, num_planes };
+	domain_update(d, false, addr, addr, num_pixels, size_log2, blockIdx.x * 256 + threadIdx.x);
+}
+
+hipError_t launch_domain(const PassDesc *passes, const uint32_t *items, uint32_t n, bool post, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_domain, dim3(n), dim3(256), 0, s, passes, items, post ? 1u : 0u);
+	return hipGetLastError();
+}
+
+hipError_t launch_domain_range(const uint8_t *rdram, const uint8_t *hidden, uint8_t *ref, uint8_t *planes,
+                               uint8_t *hplanes, uint32_t rdram_size, uint32_t num_planes, uint32_t addr,
+                               uint32_t num_pixels, uint32_t size_log2, hipStream_t s)
+{
+	const uint32_t align = 4u >> size_log2;
+	const uint32_t n = (((num_pixels + align - 1u) & ~(align - 1u)) + 255u) / 256u;
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_domain_range, dim3(n), dim3(256), 0, s, rdram, hidden, ref, planes, hplanes, rdram_size,
+	                   num_planes, addr, num_pixels, size_log2);
 	return hipGetLastError();
 }
 

@@ -46,6 +46,11 @@ hipError_t launch_raster(uint32_t cls, const PassDesc *, const uint32_t *, uint3
 hipError_t launch_raster_fill(const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 hipError_t launch_raster_round(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t,
                                const uint32_t *, uint32_t, uint32_t, hipStream_t);
+hipError_t launch_raster_up(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+hipError_t launch_domain(const PassDesc *, const uint32_t *, uint32_t, bool post, hipStream_t);
+hipError_t launch_domain_range(const uint8_t *rdram, const uint8_t *hidden, uint8_t *ref, uint8_t *planes,
+                               uint8_t *hplanes, uint32_t rdram_size, uint32_t num_planes, uint32_t addr,
+                               uint32_t num_pixels, uint32_t size_log2, hipStream_t s);
 hipError_t upload_blender_lut(const uint8_t *lut);
 
 // Launch shape of the raster stage: the XCD chunk of the tile-to-block
@@ -333,6 +338,10 @@ struct Processor {
 	std::condition_variable tl_cv;
 	uint64_t timeline_done = 0;
 	int timeline_error = 0;
+	std::string timeline_error_msg;
+	// The first failed submission of the processor and its message.
+	int submit_error = 0;
+	std::string submit_error_msg;
 
 	// RDPDUMP2 capture of everything sent through rdp_mi_enqueue_command,
 	// the VI registers and scanouts (rdp_device.cpp:60-73, 874-897, 970-975,
@@ -358,6 +367,13 @@ struct Processor {
 	hipEvent_t wb_event = nullptr;
 	bool wb_pending = false;
 	std::mutex wb_lock;
+
+	// Upscaled rendering (RDP_MI_FLAG_UPSCALING_*; the reference's
+	// init_internal_upscaling_factor, rdp_renderer.cpp:545-601): up_log2 > 0
+	// holds 4^up_log2 sample planes of RDRAM and hidden RDRAM and the
+	// reference copy of RDRAM the domain last agreed with.
+	uint32_t up_log2 = 0;
+	uint8_t *d_up_rdram = nullptr, *d_up_hidden = nullptr, *d_up_ref = nullptr;
 
 	explicit Processor(uint32_t size) : fe(size), rdram_size(size) { memset(tmem_mirror, 0, sizeof(tmem_mirror)); }
 	uint8_t *host_rdram() { return user_rdram ? user_rdram : rdram_mirror.data(); }
@@ -415,6 +431,13 @@ struct Launch {
 	uint32_t quad_n[kRasterClasses];
 	const uint32_t *fill_items;  // passes of FILL primitives only: k_raster_fill, 4 tiles per item
 	uint32_t fill_n;
+	// Upscaled rendering: the domain's pre updates (before the raster) and
+	// post updates (after it) of the round's native passes, and the 8x8 tiles
+	// of its upscaled passes by class (after the post updates).
+	const uint32_t *domain_pre_items, *domain_post_items;
+	uint32_t domain_pre_n, domain_post_n;
+	const uint32_t *up_items[kRasterClasses];
+	uint32_t up_n[kRasterClasses];
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
 	bool tmem_early;  // no upload reads what the previous round's raster writes
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
@@ -453,6 +476,16 @@ static bool tmem_may_start_early(const std::vector<std::vector<PassRef>> &rounds
 		}
 	}
 	return true;
+}
+
+// Blocks of 256 elements of a native pass's domain update: the framebuffer
+// region fb_width x domain_height in whole words (rdp_renderer.cpp:1966-1973).
+static uint32_t domain_blocks(const HostPass &hp)
+{
+	const uint32_t size_log2 = hp.fb_fmt == FB_RGBA8888 ? 2 : (hp.fb_fmt == FB_RGBA5551 || hp.fb_fmt == FB_IA88) ? 1 : 0;
+	const uint32_t align = 4u >> size_log2;
+	const uint32_t n = (hp.fb_width * hp.domain_height + align - 1u) & ~(align - 1u);
+	return (n + 255u) / 256u;
 }
 
 // A pass whose every primitive is a FILL-mode primitive.
@@ -558,6 +591,9 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 2], s));
 		}
 		if (overlap) HIP_CHECK(hipEventRecord(E.pre_raster, s));
+		// The domain's pre update reads native RDRAM before this round's
+		// raster writes it (submit_render_pass, rdp_renderer.cpp:2240-2246).
+		HIP_CHECK(launch_domain(L.descs, L.domain_pre_items, L.domain_pre_n, false, s));
 		if (persist) {
 			if (fork_tmem) HIP_CHECK(hipStreamWaitEvent(s, E.tmem_done, 0));
 			// Class 1's tiles first (the longest chains), class 0's, then the fills.
@@ -603,6 +639,11 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 				if (re) HIP_CHECK(hipEventRecord(re[3 + 2 * c], s));
 			}
 		}
+		// After the native raster: the post updates, then the upscaled passes
+		// (rdp_renderer.cpp:2269-2278, 2287-2342, in flush_queues' order).
+		HIP_CHECK(launch_domain(L.descs, L.domain_post_items, L.domain_post_n, true, s));
+		for (uint32_t c = 0; c < kRasterClasses; c++)
+			HIP_CHECK(launch_raster_up(c, L.descs, L.up_items[c], L.up_n[c], cfg.xcd_chunk, s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
 	}
 	return 0;
@@ -623,9 +664,10 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 
 	// Input layout: per round [PassDesc x n][items: setup|tmem|bin|raster], then per pass data.
 	struct RoundLayout {
-		size_t descs, items_setup, items_tmem, items_bin, items_raster;
+		size_t descs, items_setup, items_tmem, items_bin, items_raster, items_domain, items_up;
 		uint32_t n_setup, n_span, n_tmem, n_tmem_gather, n_bin, n_raster, n_fill, n_quad;
 		uint32_t n_raster_class[kRasterClasses], n_quad_class[kRasterClasses];
+		uint32_t n_dom_pre, n_dom_post, n_up, n_up_class[kRasterClasses];
 		size_t scratch;
 	};
 	std::vector<RoundLayout> rl(rounds.size());
@@ -641,8 +683,14 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	// binning ahead on the lane's prep stream (execute_launches), so each
 	// round keeps its own scratch region; otherwise rounds run in series and
 	// share one region.
+	// An upscaled pass reads the TMEM snapshots its native pass (the round
+	// before) left in that round's scratch, so upscaled submissions keep
+	// every round's scratch too.
 	bool prep_ahead = false;
-	for (const auto &round : rounds) prep_ahead = prep_ahead || round.size() >= kOverlapMinPasses;
+	for (const auto &round : rounds) {
+		prep_ahead = prep_ahead || round.size() >= kOverlapMinPasses;
+		for (const PassRef &pr : round) prep_ahead = prep_ahead || pr.pass->scale_log2 != 0;
+	}
 	size_t off = 0, max_scratch = 0, scratch_base = 0;
 	for (size_t r = 0; r < rounds.size(); r++) {
 		// Work items carry the pass index above 17 bits of per-pass index.
@@ -650,22 +698,36 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			return set_error(RDP_MI_ERROR_UNSUPPORTED, "%zu passes in one round (at most 32768)", rounds[r].size());
 		RoundLayout &L = rl[r];
 		L.n_setup = L.n_span = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = L.n_fill = L.n_quad = 0;
-		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = L.n_quad_class[c] = 0;
+		L.n_dom_pre = L.n_dom_post = L.n_up = 0;
+		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = L.n_quad_class[c] = L.n_up_class[c] = 0;
 		size_t scr = scratch_base;
 		pl[r].resize(rounds[r].size());
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
 			const uint32_t tx = (hp.fb_width + 7) / 8, ty = (hp.fb_height + 7) / 8;
 			const uint32_t groups = bin_groups(hp.prims.size());
+			const bool up = hp.scale_log2 != 0;
+			if (up && (i >= (1u << 12) || tx > 1024 || ty > 1024))
+				return set_error(RDP_MI_ERROR_UNSUPPORTED, "upscaled round of %zu passes, %ux%u tiles (at most 4096 "
+				                 "passes of 1024x1024 tiles)", rounds[r].size(), tx, ty);
 			L.n_setup += (uint32_t)(hp.prims.size() + 63) / 64;
 			L.n_span += (hp.num_tilerows + 7) / 8;
-			L.n_tmem += 8;
-			L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 31) / 32);
+			// An upscaled pass replays no uploads: it reads its native pass's
+			// TMEM snapshots.
+			if (!up) {
+				L.n_tmem += 8;
+				L.n_tmem_gather += 8 * (uint32_t)((hp.uploads.size() + 31) / 32);
+			}
+			if (hp.domain_pre) L.n_dom_pre += domain_blocks(hp);
+			if (hp.domain_post) L.n_dom_post += domain_blocks(hp);
 			PassInputs &pi = pl[r][i];
 			pi.fill = fill_only(hp);
 			if (!hp.prims.empty()) {
 				L.n_bin += ty * (groups / 8);
-				if (pi.fill) {
+				if (up) {
+					L.n_up += tx * ty;
+					L.n_up_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
+				} else if (pi.fill) {
 					L.n_fill += ty * ((tx + kFillTilesPerItem - 1) / kFillTilesPerItem);
 				} else {
 					L.n_raster += tx * ty;
@@ -692,6 +754,10 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		off = align_up(off + L.n_bin * 4ull, 256);
 		L.items_raster = off;
 		off = align_up(off + (L.n_raster + L.n_fill + L.n_quad) * 4ull, 256);
+		L.items_domain = off;
+		off = align_up(off + (L.n_dom_pre + L.n_dom_post) * 4ull, 256);
+		L.items_up = off;
+		off = align_up(off + L.n_up * 4ull, 256);
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
 			PassInputs &pi = pl[r][i];
@@ -724,6 +790,9 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 	uint8_t *h = staging.ptr;
 	uint8_t *d = input.ptr;
 	launches.resize(rounds.size());
+	// Per processor: the TMEM snapshots of its pass in the round before, for
+	// an upscaled pass that follows it.
+	std::map<const Processor *, uint16_t *> native_tmem;
 	for (size_t r = 0; r < rounds.size(); r++) {
 		RoundLayout &L = rl[r];
 		PassDesc *descs = reinterpret_cast<PassDesc *>(h + L.descs);
@@ -731,7 +800,16 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		uint32_t *it_tmem = reinterpret_cast<uint32_t *>(h + L.items_tmem);
 		uint32_t *it_bin = reinterpret_cast<uint32_t *>(h + L.items_bin);
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
-		uint32_t ns = 0, nsp = 0, nt = 0, ntg = 0, nb = 0;
+		uint32_t *it_domain = reinterpret_cast<uint32_t *>(h + L.items_domain);
+		uint32_t *it_up = reinterpret_cast<uint32_t *>(h + L.items_up);
+		uint32_t ns = 0, nsp = 0, nt = 0, ntg = 0, nb = 0, ndp = 0, ndq = 0;
+		uint32_t upbase[kRasterClasses], nu[kRasterClasses];
+		for (uint32_t c = 0, acc = 0; c < kRasterClasses; c++) {
+			upbase[c] = acc;
+			nu[c] = 0;
+			acc += L.n_up_class[c];
+		}
+		std::map<const Processor *, uint16_t *> round_tmem;
 		// Raster items: the 8x8-tile list by class, the fill items, then the
 		// 16x16-tile list by class.
 		uint32_t nr[kRasterClasses], rbase[kRasterClasses], nq[kRasterClasses], qbase[kRasterClasses];
@@ -765,7 +843,13 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			LA.raster_n[c] = L.n_raster_class[c];
 			LA.quad_items[c] = LA.items[3] + qbase[c];
 			LA.quad_n[c] = L.n_quad_class[c];
+			LA.up_items[c] = reinterpret_cast<const uint32_t *>(d + L.items_up) + upbase[c];
+			LA.up_n[c] = L.n_up_class[c];
 		}
+		LA.domain_pre_items = reinterpret_cast<const uint32_t *>(d + L.items_domain);
+		LA.domain_pre_n = L.n_dom_pre;
+		LA.domain_post_items = LA.domain_pre_items + L.n_dom_pre;
+		LA.domain_post_n = L.n_dom_post;
 		LA.raster_bytes = 0;
 		LA.fill_bytes = 0;
 		LA.fill_tiles = 0;
@@ -825,17 +909,41 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.num_spans = hp.num_spans;
 			D.num_tilerows = hp.num_tilerows;
 			D.num_groups = bin_groups(hp.prims.size());
+			D.scale_log2 = hp.scale_log2;
+			D.up_rdram = pr.d_up_rdram;
+			D.up_hidden = pr.d_up_hidden;
+			D.up_ref = pr.d_up_ref;
+			D.domain_height = hp.domain_height;
+			D.domain_log2 = pr.up_log2;
+			if (hp.scale_log2) {
+				// The upscaled pass renders into the sample planes with the
+				// TMEM snapshots of its native pass.
+				auto it = native_tmem.find(&pr);
+				if (it == native_tmem.end())
+					return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "upscaled pass without its native pass");
+				D.rdram = pr.d_up_rdram;
+				D.hidden = pr.d_up_hidden;
+				D.wmask = nullptr;
+				D.tmem_instances = it->second;
+			} else
+				round_tmem[&pr] = D.tmem_instances;
 
 			const uint32_t pidx = (uint32_t)i;
 			for (uint32_t g = 0; g < (D.num_prims + 63) / 64; g++) it_setup[ns++] = (pidx << 7) | g;
 			for (uint32_t b = 0; b < (D.num_tilerows + 7) / 8; b++) it_setup[L.n_setup + nsp++] = (pidx << 17) | b;
-			for (uint32_t k = 0; k < 8; k++) it_tmem[L.n_tmem_gather + nt++] = (pidx << 12) | k;
-			for (uint32_t b = 0; b < (uint32_t)(hp.uploads.size() + 31) / 32; b++)
-				for (uint32_t k = 0; k < 8; k++) it_tmem[ntg++] = (pidx << 12) | (b << 3) | k;
+			if (!hp.scale_log2) {
+				for (uint32_t k = 0; k < 8; k++) it_tmem[L.n_tmem_gather + nt++] = (pidx << 12) | k;
+				for (uint32_t b = 0; b < (uint32_t)(hp.uploads.size() + 31) / 32; b++)
+					for (uint32_t k = 0; k < 8; k++) it_tmem[ntg++] = (pidx << 12) | (b << 3) | k;
+			}
 			if (D.num_prims) {
 				for (uint32_t y = 0; y < D.tiles_y; y++)
-					for (uint32_t g = 0; g < D.num_groups / 8; g++) it_bin[nb++] = (pidx << 14) | (y << 7) | g;
+					for (uint32_t g = 0; g < D.num_groups / 8; g++) it_bin[nb++] = (pidx << 17) | (y << 4) | g;
 			}
+			if (hp.domain_pre)
+				for (uint32_t b = 0; b < domain_blocks(hp); b++) it_domain[ndp++] = (pidx << 17) | b;
+			if (hp.domain_post)
+				for (uint32_t b = 0; b < domain_blocks(hp); b++) it_domain[L.n_dom_pre + ndq++] = (pidx << 17) | b;
 		}
 		// Raster items, per class, passes with the most primitives per tile
 		// first: their tiles have the longest chains, so they start early
@@ -853,6 +961,13 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			const HostPass &hp = *rounds[r][pidx].pass;
 			if (hp.prims.empty()) continue;
 			const uint32_t tiles_x = (hp.fb_width + 7) / 8, tiles_y = (hp.fb_height + 7) / 8;
+			if (hp.scale_log2) {
+				const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
+				uint32_t *out = it_up + upbase[cls];
+				for (uint32_t y = 0; y < tiles_y; y++)
+					for (uint32_t x = 0; x < tiles_x; x++) out[nu[cls]++] = (pidx << 20) | (x << 10) | y;
+				continue;
+			}
 			if (pl[r][pidx].fill) {
 				uint32_t *out = it_raster + L.n_raster;
 				for (uint32_t y = 0; y < tiles_y; y++)
@@ -868,6 +983,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			for (uint32_t y = 0; y < (tiles_y + 1) / 2; y++)
 				for (uint32_t x = 0; x < (tiles_x + 1) / 2; x++) qout[nq[cls]++] = (pidx << 14) | (x << 7) | y;
 		}
+		native_tmem = std::move(round_tmem);
 	}
 
 	HIP_CHECK(hipMemcpyAsync(d, h, total_input, hipMemcpyHostToDevice, c.stream));
@@ -1029,9 +1145,12 @@ static int submit_processors(Processor *const *procs, unsigned count, bool batch
 		// Keep each launch's pass count within the work-item encoding.
 		ret = submit_rounds(c, rounds, arenas);
 	} else {
-		for (auto &r : rounds) {
-			std::vector<std::vector<PassRef>> one{ r };
-			if ((ret = submit_rounds(c, one, arenas))) break;
+		// One pass per submission, except that an upscaled pass goes with its
+		// native pass: it reads the TMEM snapshots in that pass's scratch.
+		for (size_t k = 0; k < rounds.size() && !ret;) {
+			std::vector<std::vector<PassRef>> group{ rounds[k++] };
+			while (k < rounds.size() && rounds[k][0].pass->scale_log2) group.push_back(rounds[k++]);
+			ret = submit_rounds(c, group, arenas);
 		}
 	}
 	// A batched submission that does not fit fixed arenas fails before its
@@ -1146,7 +1265,14 @@ static int submit_pending(Processor &p)
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	Processor *pp = &p;
-	return submit_processors(&pp, 1, false);
+	const int err = submit_processors(&pp, 1, false);
+	// A failed submission from the ring thread (idle or pass-count submits)
+	// is reported by the next timeline signal, with its message.
+	if (err && !p.submit_error) {
+		p.submit_error = err;
+		p.submit_error_msg = g_last_error;
+	}
+	return err;
 }
 
 // MetaFlush: end the open pass and submit (caller holds p.lock).
@@ -1162,6 +1288,13 @@ static int flush_now(Processor &p)
 static void signal_now(Processor &p, uint64_t value)
 {
 	const int err = flush_now(p);
+	if (p.submit_error) {
+		std::lock_guard<std::mutex> g(p.tl_lock);
+		if (!p.timeline_error) {
+			p.timeline_error = p.submit_error;
+			p.timeline_error_msg = p.submit_error_msg;
+		}
+	}
 	Context &c = ctx();
 	TimelineWorker &w = timeline_worker();
 	hipEvent_t ev = w.acquire();
@@ -1742,6 +1875,7 @@ struct rdp_mi_processor {
 static constexpr size_t kScanoutBufferWords = 640 * 576;
 struct rdp_mi_scanout_buffer {
 	uint32_t *pixels = nullptr;
+	size_t words = 0;  // capacity; grown for an upscaling processor's larger image
 	hipEvent_t fence = nullptr;
 	unsigned width = 0, height = 0;
 	bool in_flight = false;
@@ -1760,8 +1894,12 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "rdram_size must be a power of two");
 	if (hidden_rdram_size != rdram_size / 2)
 		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "hidden_rdram_size must be rdram_size / 2");
-	if (flags & (RDP_MI_FLAG_UPSCALING_2X | RDP_MI_FLAG_UPSCALING_4X | RDP_MI_FLAG_UPSCALING_8X))
-		return set_error(RDP_MI_ERROR_UNSUPPORTED, "upscaling is not supported");
+	const uint32_t up_flags = flags & (RDP_MI_FLAG_UPSCALING_2X | RDP_MI_FLAG_UPSCALING_4X | RDP_MI_FLAG_UPSCALING_8X);
+	const uint32_t up_log2 = up_flags == RDP_MI_FLAG_UPSCALING_2X ? 1 : up_flags == RDP_MI_FLAG_UPSCALING_4X ? 2
+	                         : up_flags == RDP_MI_FLAG_UPSCALING_8X ? 3 : 0;
+	if (up_flags && !up_log2) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "more than one upscaling flag");
+	if (up_log2 && rdram_ptr)
+		return set_error(RDP_MI_ERROR_UNSUPPORTED, "upscaling with host-coherent RDRAM is not supported");
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	int err = c.init();
@@ -1771,6 +1909,9 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 	if (!h) return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "out of host memory");
 	Processor &p = h->p;
 	auto fail = [&](int code, const char *msg) {
+		if (p.d_up_rdram) (void)hipFree(p.d_up_rdram);
+		if (p.d_up_hidden) (void)hipFree(p.d_up_hidden);
+		if (p.d_up_ref) (void)hipFree(p.d_up_ref);
 		if (p.d_rdram) (void)hipFree(p.d_rdram);
 		if (p.d_hidden) (void)hipFree(p.d_hidden);
 		if (p.d_tmem) (void)hipFree(p.d_tmem);
@@ -1807,6 +1948,21 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 	if (hipMemsetAsync(p.d_hidden, 0x03, hidden_rdram_size, c.stream) != hipSuccess)
 		return fail(RDP_MI_ERROR_DEVICE, "hidden RDRAM clear failed");
 	if (hipMemsetAsync(p.d_tmem, 0, 4096, c.stream) != hipSuccess) return fail(RDP_MI_ERROR_DEVICE, "TMEM clear failed");
+	if (up_log2) {
+		// Sample planes zeroed, hidden planes 0x03, reference zeroed
+		// (rdp_renderer.cpp:566-593).
+		const size_t planes = (size_t)1 << (2 * up_log2);
+		if (hipMalloc(&p.d_up_rdram, planes * rdram_size) != hipSuccess ||
+		    hipMalloc(&p.d_up_hidden, planes * hidden_rdram_size) != hipSuccess ||
+		    hipMalloc(&p.d_up_ref, rdram_size) != hipSuccess)
+			return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "upscaled domain allocation failed");
+		if (hipMemsetAsync(p.d_up_rdram, 0, planes * rdram_size, c.stream) != hipSuccess ||
+		    hipMemsetAsync(p.d_up_hidden, 0x03, planes * hidden_rdram_size, c.stream) != hipSuccess ||
+		    hipMemsetAsync(p.d_up_ref, 0, rdram_size, c.stream) != hipSuccess)
+			return fail(RDP_MI_ERROR_DEVICE, "upscaled domain clear failed");
+		p.up_log2 = up_log2;
+		p.fe.scale_log2 = up_log2;
+	}
 	if (hipStreamSynchronize(c.stream) != hipSuccess) return fail(RDP_MI_ERROR_DEVICE, "device synchronisation failed");
 	p.supported = true;
 	// The reference's capture switch (rdp_device.cpp:60-73): failing to open
@@ -1834,6 +1990,11 @@ void rdp_mi_destroy(rdp_mi_processor *h)
 	(void)hipFree(p.d_rdram);
 	(void)hipFree(p.d_hidden);
 	(void)hipFree(p.d_tmem);
+	if (p.up_log2) {
+		(void)hipFree(p.d_up_rdram);
+		(void)hipFree(p.d_up_hidden);
+		(void)hipFree(p.d_up_ref);
+	}
 	if (p.coherent) {
 		(void)hipFree(p.d_wmask);
 		(void)hipHostFree(p.wb_data.ptr);
@@ -1969,7 +2130,9 @@ int rdp_mi_wait_for_timeline_timeout(rdp_mi_processor *h, uint64_t value, uint64
 	else if (!p.tl_cv.wait_for(g, std::chrono::microseconds((int64_t)std::min<uint64_t>(timeout_us, 1ull << 50)), ready))
 		return set_error(RDP_MI_ERROR_TIMEOUT, "timeline value %llu not reached within %llu us",
 		                 (unsigned long long)value, (unsigned long long)timeout_us);
-	if (p.timeline_error) return set_error(p.timeline_error, "device error in the processor's work");
+	if (p.timeline_error)
+		return set_error(p.timeline_error, "error in the processor's work: %s",
+		                 p.timeline_error_msg.empty() ? "device error" : p.timeline_error_msg.c_str());
 	g.unlock();
 	// The caller's RDRAM gets the bytes the finished work wrote.
 	if (p.coherent) return coherent_complete(p);
@@ -2050,6 +2213,31 @@ void *rdp_mi_get_tmem(rdp_mi_processor *h)
 	return p.tmem_mirror;
 }
 
+unsigned rdp_mi_get_upscaling_factor(const rdp_mi_processor *h) { return h ? 1u << h->p.up_log2 : 0; }
+
+static int read_upscaled(rdp_mi_processor *h, bool hidden, size_t offset, void *dst, size_t size)
+{
+	if (!h || (!dst && size)) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "null argument");
+	Processor &p = h->p;
+	if (!p.up_log2) return set_error(RDP_MI_ERROR_UNSUPPORTED, "the processor does not upscale");
+	const size_t total = ((size_t)p.rdram_size >> (hidden ? 1 : 0)) << (2 * p.up_log2);
+	if (offset > total || size > total - offset)
+		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "range [%zu, %zu) outside the %zu-byte planes", offset,
+		                 offset + size, total);
+	if (!size) return 0;
+	return copy_sync(dst, (hidden ? p.d_up_hidden : p.d_up_rdram) + offset, size, hipMemcpyDeviceToHost);
+}
+
+int rdp_mi_read_upscaled_rdram(rdp_mi_processor *h, size_t offset, void *dst, size_t size)
+{
+	return read_upscaled(h, false, offset, dst, size);
+}
+
+int rdp_mi_read_upscaled_hidden_rdram(rdp_mi_processor *h, size_t offset, void *dst, size_t size)
+{
+	return read_upscaled(h, true, offset, dst, size);
+}
+
 void rdp_mi_set_vi_register(rdp_mi_processor *h, unsigned reg, uint32_t value)
 {
 	if (!h || reg >= RDP_MI_VI_COUNT) return;
@@ -2105,8 +2293,17 @@ static int scanout_queue(Processor &p, const rdp_mi_scanout_options &opts, uint3
 		for (const ByteRange &r : runs)
 			HIP_CHECK(hipMemcpyAsync(p.d_rdram + r.lo, p.user_rdram + r.lo, r.hi - r.lo, hipMemcpyHostToDevice, c.stream));
 	}
-	int err = vi_scanout(p.vi, p.vi_regs, p.d_rdram, p.d_hidden, p.rdram_size, c.d_gamma, c.stream, opts, out, width,
-	                     height, done == nullptr);
+	ViSource src;
+	src.rdram_size = p.rdram_size;
+	src.scale_log2 = p.up_log2;
+	src.native_rdram = p.d_rdram;
+	src.native_hidden = p.d_hidden;
+	src.ref = p.d_up_ref;
+	src.planes = p.d_up_rdram;
+	src.hplanes = p.d_up_hidden;
+	src.rdram = p.up_log2 ? p.d_up_rdram : p.d_rdram;
+	src.hidden = p.up_log2 ? p.d_up_hidden : p.d_hidden;
+	int err = vi_scanout(p.vi, p.vi_regs, src, c.d_gamma, c.stream, opts, out, width, height, done == nullptr);
 	if (err) return set_error(err, "VI scanout failed");
 	if (done) HIP_CHECK(hipEventRecord(done, c.stream));
 	return 0;
@@ -2137,6 +2334,7 @@ int rdp_mi_scanout_buffer_create(rdp_mi_scanout_buffer **out)
 		delete b;
 		return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "scanout buffer allocation failed");
 	}
+	b->words = kScanoutBufferWords;
 	if (hipEventCreateWithFlags(&b->fence, hipEventDisableTiming) != hipSuccess) {
 		(void)hipHostFree(b->pixels);
 		delete b;
@@ -2162,6 +2360,15 @@ int rdp_mi_scanout_async_buffer(rdp_mi_processor *h, const rdp_mi_scanout_option
 	if (b->in_flight) HIP_CHECK(hipEventSynchronize(b->fence));
 	b->in_flight = false;
 	b->width = b->height = 0;
+	const size_t need = kScanoutBufferWords << (2 * h->p.up_log2);
+	if (b->words < need) {
+		(void)hipHostFree(b->pixels);
+		b->pixels = nullptr;
+		b->words = 0;
+		if (hipHostMalloc((void **)&b->pixels, need * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+			return set_error(RDP_MI_ERROR_OUT_OF_MEMORY, "scanout buffer allocation failed");
+		b->words = need;
+	}
 	if (h->p.dump) dump_scanout(h);
 	int err = scanout_queue(h->p, *opts, b->pixels, &b->width, &b->height, b->fence);
 	if (err) return err;

@@ -417,6 +417,14 @@ struct PassDesc {
 	uint32_t num_spans;
 	uint32_t num_tilerows;
 	uint32_t num_groups;       // ceil(num_prims / 64) rounded up to a multiple of 8
+	// Upscaled rendering.  scale_log2 > 0: the pass renders at 2^scale_log2
+	// times the resolution (fb_width, fb_height, tiles_* and the primitives'
+	// lines are of that size) into the sample planes at rdram / hidden.
+	uint32_t scale_log2;
+	// The domain of a native pass (k_domain): planes, the reference copy and
+	// the region height; domain_log2 is the factor of the domain.
+	uint8_t *up_rdram, *up_hidden, *up_ref;
+	uint32_t domain_height, domain_log2;
 };
 
 }  // namespace rdp_mi

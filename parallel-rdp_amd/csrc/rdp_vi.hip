@@ -246,11 +246,19 @@ DEV uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 DEV uint32_t pack(Texel t) { return t.r | (t.g << 8) | (t.b << 16) | (t.a << 24); }
 DEV Texel unpack(uint32_t v) { return Texel{ v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff, v >> 24 }; }
 
-// extract_vram.comp:61-106, for VRAM pixel (x, y) relative to the origin.
+// extract_vram.comp:61-106, for VRAM pixel (x, y) relative to the origin;
+// upscaled, (x, y) is sample (x mod S, y mod S) of native pixel (x / S, y / S)
+// in plane (y mod S) * S + x mod S.
 DEV Texel fetch_vram(const ViParams &P, const uint8_t *__restrict__ rdram, const uint8_t *__restrict__ hidden, int x,
                      int y)
 {
 	Texel t;
+	const int L = P.scale_log2, S = 1 << L;
+	const size_t slice = (size_t)(((y & (S - 1)) << L) + (x & (S - 1)));
+	x >>= L;
+	y >>= L;
+	rdram += slice * ((size_t)P.rdram_mask + 1);
+	hidden += slice * (((size_t)P.rdram_mask + 1) >> 1);
 	int linear = y * P.fb_width + x + P.fb_offset;
 	if ((P.status & VI_TYPE_MASK) == VI_TYPE_RGBA8888) {
 		linear &= (int)(P.rdram_mask >> 2);
@@ -503,15 +511,30 @@ void vi_memory_range(const ViState &s, const uint32_t *regs, uint32_t &offset, u
 	length = (uint32_t)((int64_t)(aa_height * r.vi_width + aa_width) * pixel);
 }
 
-int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const uint8_t *d_hidden, uint32_t rdram_size,
-               const uint8_t *d_gamma, hipStream_t stream, const rdp_mi_scanout_options &opts, uint32_t *out,
-               unsigned *width, unsigned *height, bool sync)
+hipError_t launch_domain_range(const uint8_t *rdram, const uint8_t *hidden, uint8_t *ref, uint8_t *planes,
+                               uint8_t *hplanes, uint32_t rdram_size, uint32_t num_planes, uint32_t addr,
+                               uint32_t num_pixels, uint32_t size_log2, hipStream_t s);
+
+int vi_scanout(ViState &s, const uint32_t *regs, const ViSource &src, const uint8_t *d_gamma, hipStream_t stream,
+               const rdp_mi_scanout_options &opts, uint32_t *out, unsigned *width, unsigned *height, bool sync)
 {
+	const uint8_t *d_rdram = src.rdram, *d_hidden = src.hidden;
+	const uint32_t rdram_size = src.rdram_size;
+	const int L = (int)src.scale_log2, SF = 1 << L;
 	if (!s.h_params) {
 		VI_CHECK(hipHostMalloc((void **)&s.h_params, sizeof(ViParams), hipHostMallocDefault));
 		VI_CHECK(hipMalloc((void **)&s.d_params, sizeof(ViParams)));
-		for (auto &img : s.d_images) VI_CHECK(hipMalloc((void **)&img, 640 * 576 * sizeof(uint32_t)));
 		VI_CHECK(hipEventCreateWithFlags(&s.params_done, hipEventDisableTiming));
+	}
+	const size_t image_words = (size_t)640 * 576 << (2 * L);
+	if (s.image_words < image_words) {
+		VI_CHECK(hipStreamSynchronize(stream));
+		for (auto &img : s.d_images) {
+			if (img) VI_CHECK(hipFree(img));
+			img = nullptr;
+		}
+		for (auto &img : s.d_images) VI_CHECK(hipMalloc((void **)&img, image_words * sizeof(uint32_t)));
+		s.image_words = image_words;
 	}
 	// The previous scanout's parameter upload reads the pinned block.
 	if (s.params_busy) {
@@ -568,6 +591,33 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 	s.last_valid_frame_count = s.frame_count;
 	const bool degenerate = r.h_res <= 0 || r.v_res <= 0;
 
+	// video_interface.cpp:1299-1318: every coordinate scales with the factor.
+	r.h_start *= SF;
+	r.h_start_clamp *= SF;
+	r.v_start *= SF;
+	r.h_res *= SF;
+	r.h_res_clamp *= SF;
+	r.v_res *= SF;
+	r.max_x = r.max_x * SF + (SF - 1);
+	r.max_y = r.max_y * SF + (SF - 1);
+	for (ViLine &li : P.lines) {
+		li.h_start *= SF;
+		li.h_start_clamp *= SF;
+		li.h_end_clamp *= SF;
+		li.x_start *= SF;
+		li.y_start *= SF;
+		li.y_base *= SF;
+	}
+	if (L && !degenerate) {
+		// vram_fetch_stage (video_interface.cpp:412-420): the domain's pre
+		// update over the memory the fetch reads, colour only.
+		uint32_t offset = 0, length = 0;
+		vi_memory_range(s, regs, offset, length);
+		VI_CHECK(launch_domain_range(src.native_rdram, src.native_hidden, src.ref, src.planes, src.hplanes, rdram_size,
+		                             1u << (2 * L), offset, length, (r.status & VI_TYPE_MASK) == VI_TYPE_RGBA8888 ? 2 : 1,
+		                             stream));
+	}
+
 	P.status = r.status;
 	P.fb_offset = (r.status & VI_TYPE_MASK) == VI_TYPE_RGBA8888 ? r.vi_offset >> 2 : r.vi_offset >> 1;
 	P.fb_width = r.vi_width;
@@ -576,14 +626,15 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 	P.aw = r.max_x + 3 + divot * 2;
 	P.ah = r.max_y + 2;
 	P.divot = divot;
-	P.fetch_bug = r.init_y_add < 1024;
+	P.fetch_bug = r.init_y_add < 1024 && SF == 1;
 	P.init_y_add = r.init_y_add;
 	P.frame_count = s.frame_count;
 	P.rdram_mask = rdram_size - 1;
+	P.scale_log2 = L;
 
-	// scale_stage, video_interface.cpp:796-998.
+	// scale_stage, video_interface.cpp:719-998.
 	const bool serrate = (r.status & VI_SERRATE) && !opts.upscale_deinterlacing;
-	const int W = kScanoutWidth, H = (r.is_pal ? kVResPal : kVResNtsc) >> (serrate ? 0 : 1);
+	const int W = kScanoutWidth * SF, H = ((r.is_pal ? kVResPal : kVResNtsc) >> (serrate ? 0 : 1)) * SF;
 	int cl, cr, ct, cb;
 	if (opts.crop_enable) {
 		cl = (int)opts.crop_left;
@@ -600,6 +651,10 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 		cl = cr = cx;
 		ct = cb = cy;
 	}
+	cl *= SF;
+	cr *= SF;
+	ct *= SF;
+	cb *= SF;
 	P.ow = W;
 	P.oh = H;
 	if (cl + cr < W && ct + cb < H) {
@@ -609,13 +664,14 @@ int vi_scanout(ViState &s, const uint32_t *regs, const uint8_t *d_rdram, const u
 	P.crop_l = cl;
 	P.crop_t = ct;
 	int v_start = r.v_start, v_res = r.v_res;
+	P.info_y_shift = L;
 	if (serrate) {
 		v_start *= 2;
 		v_res *= 2;
 		P.serrate_shift = 1;
 		P.serrate_mask = 1;
 		P.serrate_select = r.v_current_line == 0;
-		P.info_y_shift = 1;
+		P.info_y_shift++;
 	}
 	P.v_start = v_start;
 	int dx = r.h_start, dy = v_start, dw = r.h_res, dh = v_res;

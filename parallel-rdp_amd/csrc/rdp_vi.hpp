@@ -31,8 +31,22 @@ struct ViParams {
 	int32_t blend, nrects, prev_w, prev_h;
 	int32_t rects[3][4];
 	uint32_t rdram_mask;
-	uint32_t pad[3];
+	int32_t scale_log2;  // upscaled scanout: the image is read from 4^scale_log2 sample planes
+	uint32_t pad[2];
 	ViLine lines[kViMaxLines];
+};
+
+// Where the scanout reads the frame.  scale_log2 = 0: rdram / hidden are
+// the native memories.  scale_log2 > 0: the sample planes of the upscaled
+// domain (rdram_size and rdram_size / 2 bytes each); before the fetch the
+// domain's pre update runs over the range the VI reads, from the native
+// memories and the reference copy below.
+struct ViSource {
+	const uint8_t *rdram, *hidden;
+	uint32_t rdram_size, scale_log2;
+	const uint8_t *native_rdram, *native_hidden;
+	uint8_t *ref;
+	uint8_t *planes, *hplanes;
 };
 
 // Per-processor VI state that persists between scanouts
@@ -43,6 +57,7 @@ struct ViState {
 	bool prev_valid = false;
 	unsigned prev_w = 0, prev_h = 0;
 	uint32_t *d_images[2] = { nullptr, nullptr };  // output, ping-pong with the previous one
+	size_t image_words = 0;                         // capacity of each
 	int cur = 0;
 	uint32_t *d_filter = nullptr;                   // fetch-filter image, both layers
 	size_t filter_words = 0;
@@ -72,9 +87,9 @@ struct ViState {
 // RDRAM bytes [offset, offset + length) (modulo the RDRAM size) a scanout
 // with these registers fetches.
 void vi_memory_range(const ViState &vi, const uint32_t *regs, uint32_t &offset, uint32_t &length);
-int vi_scanout(ViState &vi, const uint32_t *regs, const uint8_t *d_rdram, const uint8_t *d_hidden, uint32_t rdram_size,
-               const uint8_t *d_gamma, hipStream_t stream, const rdp_mi_scanout_options &opts, uint32_t *out,
-               unsigned *width, unsigned *height, bool sync);
+// With upscaling the image is up to (640 x 576) << (2 * scale_log2) words.
+int vi_scanout(ViState &vi, const uint32_t *regs, const ViSource &src, const uint8_t *d_gamma, hipStream_t stream,
+               const rdp_mi_scanout_options &opts, uint32_t *out, unsigned *width, unsigned *height, bool sync);
 
 // Gamma table (luts.hpp:1034-1555) from its closed form, 256 + 16384 bytes.
 void vi_gamma_table(uint8_t *out);

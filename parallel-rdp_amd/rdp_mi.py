@@ -30,6 +30,9 @@ class RDPError(RuntimeError):
 ERROR_TIMEOUT = -5
 ERROR_LIMIT = -6
 
+# rdp_mi_create flags (rdp_device.hpp:43-51).
+FLAG_UPSCALING = {2: 1 << 2, 4: 1 << 3, 8: 1 << 4}
+
 # Words per command by opcode (the RDP's command lengths).
 COMMAND_LENGTHS = np.array([
     2, 0, 0, 0, 2, 0, 0, 0, 8, 12, 24, 28, 24, 28, 40, 44,
@@ -106,6 +109,11 @@ def load_library():
     for name in ("rdp_mi_begin_read_rdram", "rdp_mi_begin_read_hidden_rdram", "rdp_mi_get_tmem"):
         getattr(lib, name).restype = vp
         getattr(lib, name).argtypes = [vp]
+    lib.rdp_mi_get_upscaling_factor.restype = cu
+    lib.rdp_mi_get_upscaling_factor.argtypes = [vp]
+    for name in ("rdp_mi_read_upscaled_rdram", "rdp_mi_read_upscaled_hidden_rdram"):
+        getattr(lib, name).restype = ctypes.c_int
+        getattr(lib, name).argtypes = [vp, sz, vp, sz]
     lib.rdp_mi_end_write_rdram.argtypes = [vp]
     lib.rdp_mi_end_write_hidden_rdram.argtypes = [vp]
     lib.rdp_mi_get_rdram_size.restype = sz
@@ -320,6 +328,27 @@ class CommandProcessor:
         np.frombuffer(buf, dtype=np.uint8)[:] = np.asarray(data, dtype=np.uint8)
         self._lib.rdp_mi_end_write_hidden_rdram(self._h)
 
+    # -- upscaled domain -----------------------------------------------------
+    @property
+    def upscaling_factor(self) -> int:
+        return int(self._lib.rdp_mi_get_upscaling_factor(self._h))
+
+    def read_upscaled_rdram(self) -> np.ndarray:
+        """The factor^2 RDRAM sample planes as (planes, rdram_size // 4)
+        words; plane sy * factor + sx holds sample (sx, sy) of each pixel.
+        Call after idle() or a timeline wait."""
+        planes = self.upscaling_factor ** 2
+        out = np.empty((planes, self.rdram_size // 4), dtype=np.uint32)
+        _check(self._lib.rdp_mi_read_upscaled_rdram(self._h, 0, out.ctypes.data, out.nbytes), "read_upscaled_rdram")
+        return out
+
+    def read_upscaled_hidden_rdram(self) -> np.ndarray:
+        planes = self.upscaling_factor ** 2
+        out = np.empty((planes, self.rdram_size // 2), dtype=np.uint8)
+        _check(self._lib.rdp_mi_read_upscaled_hidden_rdram(self._h, 0, out.ctypes.data, out.nbytes),
+               "read_upscaled_hidden_rdram")
+        return out
+
     def read_tmem(self) -> np.ndarray:
         p = self._lib.rdp_mi_get_tmem(self._h)
         if not p:
@@ -334,7 +363,7 @@ class CommandProcessor:
         """VI scanout as a (height, width) array of RGBA8 words (R in the low byte);
         (0, 0) for a blank or invalid frame."""
         opts = options if options is not None else ScanoutOptions.default()
-        out = np.zeros(640 * 576, dtype=np.uint32)
+        out = np.zeros(640 * 576 * self.upscaling_factor ** 2, dtype=np.uint32)
         w = ctypes.c_uint(0)
         h = ctypes.c_uint(0)
         _check(self._lib.rdp_mi_scanout_sync_ex(self._h, ctypes.byref(opts), out.ctypes.data, ctypes.byref(w),
