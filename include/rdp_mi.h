@@ -45,14 +45,22 @@ enum {
  * planes at factor times the size (rdp_device.cpp:188-203, rdp_renderer.cpp:
  * 545-601, 2223-2342, video_interface.cpp:1215-1318).  Native RDRAM is
  * rendered exactly as without the flag.  Not with host-coherent RDRAM
- * (RDP_MI_ERROR_UNSUPPORTED).  Super-sampled readback into native RDRAM
- * and the VI's downscale / upscale-deinterlace blits are not offered. */
+ * (RDP_MI_ERROR_UNSUPPORTED).  With RDP_MI_FLAG_SUPER_SAMPLED_READ_BACK as
+ * well, every pass renders at scale only and is resolved into native RDRAM
+ * (the mean of each written pixel's samples, depth from its first sample,
+ * hidden RDRAM untouched; RDP_MI_FLAG_SUPER_SAMPLED_DITHER adds 4x4 ordered
+ * dither to 16-bit colour; update_upscaled_domain_resolve.comp); without an
+ * upscaling flag it is rejected with RDP_MI_ERROR_UNSUPPORTED.  The VI's
+ * downscale / upscale-deinterlace blits (float presentation) are not
+ * offered. */
 enum {
 	RDP_MI_FLAG_HOST_VISIBLE_HIDDEN_RDRAM = 1 << 0,
 	RDP_MI_FLAG_HOST_VISIBLE_TMEM = 1 << 1,
 	RDP_MI_FLAG_UPSCALING_2X = 1 << 2,
 	RDP_MI_FLAG_UPSCALING_4X = 1 << 3,
 	RDP_MI_FLAG_UPSCALING_8X = 1 << 4,
+	RDP_MI_FLAG_SUPER_SAMPLED_READ_BACK = 1 << 5,
+	RDP_MI_FLAG_SUPER_SAMPLED_DITHER = 1 << 6,
 };
 
 /* VI registers (rdp_common.hpp:324-341). */

@@ -97,6 +97,8 @@ def _bind(lib):
     lib.rdpo_command_length.argtypes = [u32]
     lib.rdpo_enable_upscaling.restype = ctypes.c_int
     lib.rdpo_enable_upscaling.argtypes = [vp, ctypes.c_uint]
+    lib.rdpo_enable_super_sampled_readback.restype = ctypes.c_int
+    lib.rdpo_enable_super_sampled_readback.argtypes = [vp, ctypes.c_int]
     lib.rdpo_upscaled_rdram.restype = vp
     lib.rdpo_upscaled_rdram.argtypes = [vp]
     lib.rdpo_upscaled_hidden_rdram.restype = vp
@@ -138,6 +140,12 @@ class Oracle:
         hbuf = (ctypes.c_uint8 * (planes * (n // 2))).from_address(self._lib.rdpo_upscaled_hidden_rdram(self._h))
         self.upscaled_hidden = np.frombuffer(hbuf, dtype=np.uint8).reshape(planes, n // 2)
         self.upscaling = factor
+
+    def enable_super_sampled_readback(self, dither: bool = False):
+        """After enable_upscaling: passes render at scale only and resolve
+        into native RDRAM (the reference's SUPER_SAMPLED_READ_BACK flag)."""
+        if self._lib.rdpo_enable_super_sampled_readback(self._h, int(dither)) != 0:
+            raise ValueError("super-sampled readback needs upscaling, before the first command")
 
     def close(self):
         if self._h:

@@ -240,6 +240,12 @@ struct rdpo {
 	 * upscaled render; replayed at scale when the pass ends. */
 	struct up_draw *up_draws;
 	unsigned up_num;
+	/* Super-sampled readback (rdpo_enable_super_sampled_readback): every
+	 * pass renders at scale only and is resolved into native RDRAM;
+	 * up_wmask[native key] holds bit 0 colour written, bit 1 depth written
+	 * by any sample of the pass. */
+	int up_ssaa, up_ssaa_dither;
+	uint8_t *up_wmask;
 
 	/* Video interface state (frame counter, previous image, per-scanline
 	 * register latches). */
@@ -459,6 +465,18 @@ int rdpo_enable_upscaling(rdpo *o, unsigned factor)
 	return 0;
 }
 
+int rdpo_enable_super_sampled_readback(rdpo *o, int dither)
+{
+	if (!o->up_log2 || o->up_ssaa || o->prim_counter)
+		return -1;
+	o->up_wmask = (uint8_t *)calloc((size_t)MAX_FB_LINES * MAX_FB_WIDTH, 1);
+	if (!o->up_wmask)
+		abort();
+	o->up_ssaa = 1;
+	o->up_ssaa_dither = dither != 0;
+	return 0;
+}
+
 uint8_t *rdpo_upscaled_rdram(rdpo *o) { return o->up_rdram; }
 uint8_t *rdpo_upscaled_hidden_rdram(rdpo *o) { return o->up_hidden; }
 unsigned rdpo_upscaling_factor(const rdpo *o) { return 1u << o->up_log2; }
@@ -470,6 +488,7 @@ void rdpo_destroy(rdpo *o)
 	if (o->upx)
 		upx_release(o);
 	free(o->up_draws);
+	free(o->up_wmask);
 	free(o->upx);
 	free(o->upx_touched);
 	free(o->up_rdram);
@@ -2844,6 +2863,9 @@ static void up_write_back(rdpo *o)
 			pixel_state_t *p = &blk[slice];
 			if (!p->loaded)
 				continue;
+			/* emit_scaled_write_masks, memory_interfacing.h:305-336 */
+			if (o->up_ssaa)
+				o->up_wmask[o->upx_touched[i]] |= (uint8_t)((p->color_dirty ? 1 : 0) | (p->depth_dirty ? 2 : 0));
 			up_select_plane(o, slice, nr, nh);
 			if (p->color_dirty)
 				store_color(o, fmt, p->color_index, p->color);
@@ -2915,6 +2937,97 @@ static unsigned fb_pixel_size_log2(uint32_t fmt)
 	return fmt == FB_RGBA8888 ? 2 : (fmt == FB_RGBA5551 || fmt == FB_IA88) ? 1 : 0;
 }
 
+static uint16_t host16(const uint8_t *p, size_t index)
+{
+	uint16_t v;
+	memcpy(&v, p + 2 * index, 2);
+	return v;
+}
+
+static void set_host16(uint8_t *p, size_t index, uint16_t v) { memcpy(p + 2 * index, &v, 2); }
+
+/* The 4x4 ordered-dither offsets of the 16-bit resolve
+ * (update_upscaled_domain_resolve.comp:126-130). */
+static const uint8_t ssaa_dither_lut[16] = { 0, 4, 1, 5, 4, 0, 5, 1, 3, 7, 2, 6, 7, 3, 6, 2 };
+
+/* update_upscaled_domain_resolve.comp:165-276 over width x height native
+ * pixels: a pixel any sample wrote gets the average of its samples in
+ * native RDRAM and the domain's reference copy (8-bit: plane 0's byte;
+ * depth: plane 0's value); hidden RDRAM is left alone. */
+static void up_ssaa_resolve(rdpo *o, uint32_t width, uint32_t height)
+{
+	const uint32_t N = 1u << (2 * o->up_log2);
+	const unsigned size_log2 = fb_pixel_size_log2(o->fb_fmt);
+	const uint32_t fb_index = o->fb_addr >> size_log2, depth_index = o->depth_addr >> 1;
+	const int alias = o->fb_addr == o->depth_addr;
+	const size_t R = o->rdram_size;
+	for (uint32_t y = 0; y < height; y++) {
+		for (uint32_t x = 0; x < width; x++) {
+			const uint32_t key = y * MAX_FB_WIDTH + x;
+			const uint8_t m = o->up_wmask[key];
+			o->up_wmask[key] = 0;
+			const uint32_t index = y * width + x;
+			if (m & 1) {
+				if (size_log2 == 0) {
+					const uint32_t ci = ((fb_index + index) & (uint32_t)(R - 1)) ^ 3u;
+					uint32_t r = 0;
+					for (uint32_t i = 0; i < N; i++)
+						r += o->up_rdram[ci];
+					r = (r + (N >> 1)) / N;
+					o->rdram[ci] = o->up_ref[ci] = (uint8_t)r;
+				} else if (size_log2 == 1) {
+					const uint32_t ci = ((fb_index + index) & (uint32_t)(R / 2 - 1)) ^ 1u;
+					uint32_t c[4] = { 0, 0, 0, 0 };
+					for (uint32_t i = 0; i < N; i++) {
+						const uint32_t w = host16(o->up_rdram + (size_t)i * R, ci);
+						c[0] += (w >> 11) & 31;
+						c[1] += (w >> 6) & 31;
+						c[2] += (w >> 1) & 31;
+						c[3] += w & 1;
+					}
+					for (int k = 0; k < 4; k++) {
+						if (o->up_ssaa_dither)
+							c[k] = (8 * c[k] + ssaa_dither_lut[PLANTED(39) ? (x & 3) * 4 + (y & 3) : (y & 3) * 4 + (x & 3)] * N) /
+							       (8 * N);
+						else
+							c[k] = (c[k] + (PLANTED(38) ? 0 : (N >> 1))) / N;
+					}
+					const uint16_t e = (uint16_t)((c[0] << 11) | (c[1] << 6) | (c[2] << 1) | c[3]);
+					set_host16(o->rdram, ci, e);
+					set_host16(o->up_ref, ci, e);
+				} else {
+					const uint32_t ci = (fb_index + index) & (uint32_t)(R / 4 - 1);
+					uint32_t c[4] = { 0, 0, 0, 0 };
+					for (uint32_t i = 0; i < N; i++) {
+						uint32_t w;
+						memcpy(&w, o->up_rdram + (size_t)i * R + 4 * (size_t)ci, 4);
+						for (int k = 0; k < 4; k++)
+							c[k] += (w >> (24 - 8 * k)) & 0xff;
+					}
+					uint32_t e = 0;
+					for (int k = 0; k < 4; k++)
+						e |= ((c[k] + (N >> 1)) / N) << (24 - 8 * k);
+					memcpy(o->rdram + 4 * (size_t)ci, &e, 4);
+					memcpy(o->up_ref + 4 * (size_t)ci, &e, 4);
+				}
+			}
+			if (!alias && (m & 2)) {
+				const uint32_t di = ((depth_index + index) & (uint32_t)(R / 2 - 1)) ^ 1u;
+				const uint16_t v = host16(o->up_rdram + (PLANTED(40) ? R : 0), di);
+				set_host16(o->rdram, di, v);
+				set_host16(o->up_ref, di, v);
+			}
+		}
+	}
+}
+
+static void pass_discard(rdpo *o)
+{
+	for (size_t i = 0; i < o->num_touched; i++)
+		o->px_loaded[o->px_touched[i]] = 0;
+	o->num_touched = 0;
+}
+
 static void render_draw(rdpo *o, const tri_t *setup, const attr_t *attr, const draw_t *d, const tile_t *tiles,
                         const uint32_t scissor[4], int ylo, int yhi, int L);
 
@@ -2924,25 +3037,45 @@ static void render_draw(rdpo *o, const tri_t *setup, const attr_t *attr, const d
  * raster, the pre update over the framebuffer region; after it, for a pass
  * that qualifies, the post update and the same draws rendered at scale into
  * the sample planes.  native_write_back runs the native raster's merge. */
+static void up_render_recorded(rdpo *o)
+{
+	uint16_t saved[2048];
+	memcpy(saved, o->tmem, sizeof(saved));
+	for (unsigned i = 0; i < o->up_num; i++) {
+		up_draw_t *r = &o->up_draws[i];
+		memcpy(o->tmem, r->tmem, sizeof(saved));
+		render_draw(o, &r->tri, &r->attr, &r->d, r->tiles, r->scissor, r->ylo, r->yhi, (int)o->up_log2);
+	}
+	memcpy(o->tmem, saved, sizeof(saved));
+}
+
 static void up_pass_end(rdpo *o, void (*native_write_back)(rdpo *))
 {
 	int need_render_pass = o->fb_width && o->pass.deduced_height && o->pass.num_span_jobs;
 	uint32_t num_pixels = o->fb_width * o->pass.deduced_height;
 	unsigned size_log2 = fb_pixel_size_log2(o->fb_fmt);
+	if (o->up_ssaa) {
+		/* Super-sampled readback (flush_queues without the native pass,
+		 * rdp_renderer.cpp:2908-2985, then submit_render_pass_upscaled): a
+		 * pass with span jobs or TMEM uploads runs the pre update, renders
+		 * at scale with write masks and resolves into native RDRAM. */
+		pass_discard(o);
+		if (o->pass.num_span_jobs || o->pass.num_uploads) {
+			up_domain_update(o, 0, o->fb_addr, o->depth_addr, num_pixels, size_log2);
+			up_render_recorded(o);
+			up_write_back(o);
+			up_ssaa_resolve(o, o->fb_width, o->pass.deduced_height);
+		}
+		o->up_num = 0;
+		return;
+	}
 	if (need_render_pass)
 		up_domain_update(o, 0, o->fb_addr, o->depth_addr, num_pixels, size_log2);
 	native_write_back(o);
 	if (need_render_pass && up_pass_eligible(o)) {
 		if (!PLANTED(33))
 			up_domain_update(o, 1, o->fb_addr, o->depth_addr, num_pixels, size_log2);
-		uint16_t saved[2048];
-		memcpy(saved, o->tmem, sizeof(saved));
-		for (unsigned i = 0; i < o->up_num; i++) {
-			up_draw_t *r = &o->up_draws[i];
-			memcpy(o->tmem, r->tmem, sizeof(saved));
-			render_draw(o, &r->tri, &r->attr, &r->d, r->tiles, r->scissor, r->ylo, r->yhi, (int)o->up_log2);
-		}
-		memcpy(o->tmem, saved, sizeof(saved));
+		up_render_recorded(o);
 		up_write_back(o);
 	}
 	o->up_num = 0;
@@ -3075,7 +3208,7 @@ static void draw_primitive(rdpo *o, tri_t *setup, const attr_t *attr_in)
 	}
 
 	render_draw(o, setup, &attr, &d, o->tiles, scissor, ylo, yhi, 0);
-	if (o->up_log2 && up_pass_eligible(o))
+	if (o->up_log2 && (o->up_ssaa || up_pass_eligible(o)))
 		up_record(o, setup, &attr, &d, scissor, ylo, yhi);
 	pass_account_draw(o, setup);
 }

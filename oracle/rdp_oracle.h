@@ -46,6 +46,12 @@ void rdpo_destroy(rdpo *o);
  * hidden RDRAM (rdram_size / 2 each); sample (sx, sy) of a pixel lives in
  * plane sy * factor + sx at the pixel's native address. */
 int rdpo_enable_upscaling(rdpo *o, unsigned factor);
+/* COMMAND_PROCESSOR_FLAG_SUPER_SAMPLED_READ_BACK_BIT (and _DITHER_BIT):
+ * after rdpo_enable_upscaling, before the first command.  Every pass then
+ * renders at scale only and is resolved into native RDRAM
+ * (update_upscaled_domain_resolve.comp).  Returns 0, or -1 without
+ * upscaling or on a second call. */
+int rdpo_enable_super_sampled_readback(rdpo *o, int dither);
 uint8_t *rdpo_upscaled_rdram(rdpo *o);
 uint8_t *rdpo_upscaled_hidden_rdram(rdpo *o);
 unsigned rdpo_upscaling_factor(const rdpo *o);

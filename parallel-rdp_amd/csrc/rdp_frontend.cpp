@@ -34,7 +34,29 @@ void Frontend::flush()
 		cur.fb_fmt = fb_fmt;
 		cur.depth_addr = depth_addr;
 		cur.marks.clear();
-		if (scale_log2) {
+		if (scale_log2 && ssaa) {
+			// Super-sampled readback (flush_queues without the native pass,
+			// rdp_renderer.cpp:2908-2985, and submit_render_pass_upscaled):
+			// a pass with span jobs or TMEM uploads runs the pre update over
+			// its region, renders at scale only, then resolves.
+			const bool work = ref.span_jobs || ref.uploads;
+			cur.domain_height = (work && fb_width) ? ref.deduced_height : 0;
+			cur.domain_pre = cur.ssaa_clear = cur.domain_height != 0;
+			HostPass up;
+			if (work && !cur.prims.empty()) {
+				up = upscaled_copy(cur);
+				up.domain_height = cur.domain_height;
+				up.ssaa_resolve = cur.domain_height != 0;
+			}
+			// The native pass keeps its uploads (the TMEM snapshots the
+			// upscaled pass reads) and draws nothing.
+			cur.prims.clear();
+			cur.words.clear();
+			cur.states.clear();
+			cur.num_spans = cur.num_tilerows = cur.fb_height = cur.raster_class = 0;
+			done.push_back(std::move(cur));
+			if (!up.prims.empty()) done.push_back(std::move(up));
+		} else if (scale_log2) {
 			// render_pass_is_upscaled / should_render_upscaled
 			// (rdp_renderer.cpp:2988-3010): a pass that renders runs the
 			// domain's pre update over its region; a 16/32-bit colour pass at

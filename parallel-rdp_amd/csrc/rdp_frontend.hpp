@@ -55,6 +55,11 @@ struct HostPass {
 	bool domain_pre = false, domain_post = false;
 	uint32_t domain_height = 0;
 	uint32_t scale_log2 = 0, native_uploads = 0;
+	// Super-sampled readback: the native pass renders nothing and clears the
+	// per-pixel write masks with its pre update (ssaa_clear); its upscaled
+	// copy records which pixels it writes and resolves them into native
+	// RDRAM over fb_width / 2^scale_log2 x domain_height (ssaa_resolve).
+	bool ssaa_clear = false, ssaa_resolve = false;
 	bool empty() const { return prims.empty() && uploads.empty(); }
 };
 
@@ -113,9 +118,10 @@ public:
 	// pass of the reference's renderer that held a draw (for tests).
 	std::vector<uint64_t> *pass_end_log = nullptr;
 
-	// log2 of the upscaling factor (0: native only); set before the first
-	// command.
+	// log2 of the upscaling factor (0: native only) and super-sampled
+	// readback; set before the first command.
 	uint32_t scale_log2 = 0;
+	bool ssaa = false;
 
 private:
 	uint32_t rdram_size;

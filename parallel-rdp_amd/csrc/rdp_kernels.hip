@@ -2309,6 +2309,12 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 		store_color(me, fmt, ci, px);
 		if (wm) mark_color(wm, pe->rdram_mask, fmt, ci);
 	}
+	if constexpr (UP) {
+		// emit_scaled_write_masks (memory_interfacing.h:305-336).
+		const uint32_t bits = (px.color_dirty ? 1u : 0u) | (px.depth_dirty ? 2u : 0u);
+		if (in_fb && bits && (pe->domain_flags & DOMAIN_SSAA_RESOLVE))
+			atomicOr(pe->up_wmask + (size_t)ny * (pe->fb_width >> L) + nx, bits);
+	}
 	if (in_fb && !alias && px.depth_dirty) {
 		const uint32_t de = (pe->depth_addr >> 1) + (pe->fb_width >> L) * ny + nx;
 		me.wr16i(de, (px.depth << 2) | (px.dz >> 2));
@@ -2596,6 +2602,88 @@ __global__ __launch_bounds__(256) void k_domain(const PassDesc *__restrict__ pas
 	const DomainMem d{ gptr(P.rdram), gptr(P.hidden), gptr(P.up_ref), gptr(P.up_rdram), gptr(P.up_hidden),
 		               P.rdram_mask + 1, 1u << (2 * P.domain_log2) };
 	domain_update(d, post != 0, P.fb_addr, P.depth_addr, P.fb_width * P.domain_height, size_log2, i);
+	// clear_super_sampled_write_mask.comp
+	if (!post && (P.domain_flags & DOMAIN_SSAA_CLEAR) && i < P.fb_width * P.domain_height) gptr(P.up_wmask)[i] = 0;
+}
+
+// update_upscaled_domain_resolve.comp:165-276, one lane per native pixel of
+// the region (fb_width >> scale_log2) x domain_height of an upscaled pass:
+// a pixel any sample wrote gets the mean of its samples in native RDRAM and
+// the reference copy (8-bit: plane 0's byte; depth: plane 0's value).
+__constant__ uint8_t c_ssaa_dither[16] = { 0, 4, 1, 5, 4, 0, 5, 1, 3, 7, 2, 6, 7, 3, 6, 2 };
+
+__global__ __launch_bounds__(256) void k_ssaa_resolve(const PassDesc *__restrict__ passes,
+                                                      const uint32_t *__restrict__ items)
+{
+	const uint32_t item = items[blockIdx.x];
+	const PassDesc &P = passes[item >> 17];
+	const uint32_t width = P.fb_width >> P.scale_log2;
+	const uint32_t index = (item & 0x1ffff) * 256 + threadIdx.x;
+	if (index >= width * P.domain_height) return;
+	const uint32_t x = index % width, y = index / width;
+	const uint32_t m = gptr(P.up_wmask)[index];
+	if (!m) return;
+	const uint32_t fmt = P.fb_fmt;
+	const uint32_t size_log2 = fmt == FB_RGBA8888 ? 2 : (fmt == FB_RGBA5551 || fmt == FB_IA88) ? 1 : 0;
+	const uint32_t N = 1u << (2 * P.scale_log2);
+	const size_t R = (size_t)P.rdram_mask + 1;
+	// An upscaled pass's rdram points at the planes.
+	const AS_GLOBAL uint8_t *planes = gptr(P.rdram);
+	AS_GLOBAL uint8_t *native = gptr(P.resolve_rdram);
+	AS_GLOBAL uint8_t *ref = gptr(P.up_ref);
+	if (m & 1u) {
+		const uint32_t fb_index = P.fb_addr >> size_log2;
+		if (size_log2 == 0) {
+			const uint32_t ci = ((fb_index + index) & (uint32_t)(R - 1)) ^ 3u;
+			uint32_t r = 0;
+			for (uint32_t i = 0; i < N; i++) r += planes[ci];
+			r = (r + (N >> 1)) / N;
+			native[ci] = ref[ci] = (uint8_t)r;
+		} else if (size_log2 == 1) {
+			const uint32_t ci = ((fb_index + index) & (uint32_t)(R / 2 - 1)) ^ 1u;
+			uint32_t c[4] = { 0, 0, 0, 0 };
+			for (uint32_t i = 0; i < N; i++) {
+				const uint32_t w = ((const AS_GLOBAL uint16_t *)(planes + i * R))[ci];
+				c[0] += (w >> 11) & 31;
+				c[1] += (w >> 6) & 31;
+				c[2] += (w >> 1) & 31;
+				c[3] += w & 1;
+			}
+			const bool dither = (P.domain_flags & DOMAIN_SSAA_DITHER) != 0;
+			const uint32_t d = c_ssaa_dither[(y & 3) * 4 + (x & 3)] * N;
+#pragma unroll
+			for (int k = 0; k < 4; k++) c[k] = dither ? (8 * c[k] + d) / (8 * N) : (c[k] + (N >> 1)) / N;
+			const uint16_t e = (uint16_t)((c[0] << 11) | (c[1] << 6) | (c[2] << 1) | c[3]);
+			((AS_GLOBAL uint16_t *)native)[ci] = e;
+			((AS_GLOBAL uint16_t *)ref)[ci] = e;
+		} else {
+			const uint32_t ci = (fb_index + index) & (uint32_t)(R / 4 - 1);
+			uint32_t c[4] = { 0, 0, 0, 0 };
+			for (uint32_t i = 0; i < N; i++) {
+				const uint32_t w = ((const AS_GLOBAL uint32_t *)(planes + i * R))[ci];
+#pragma unroll
+				for (int k = 0; k < 4; k++) c[k] += (w >> (24 - 8 * k)) & 0xff;
+			}
+			uint32_t e = 0;
+#pragma unroll
+			for (int k = 0; k < 4; k++) e |= ((c[k] + (N >> 1)) / N) << (24 - 8 * k);
+			((AS_GLOBAL uint32_t *)native)[ci] = e;
+			((AS_GLOBAL uint32_t *)ref)[ci] = e;
+		}
+	}
+	if (P.fb_addr != P.depth_addr && (m & 2u)) {
+		const uint32_t di = (((P.depth_addr >> 1) + index) & (uint32_t)(R / 2 - 1)) ^ 1u;
+		const uint16_t v = ((const AS_GLOBAL uint16_t *)planes)[di];
+		((AS_GLOBAL uint16_t *)native)[di] = v;
+		((AS_GLOBAL uint16_t *)ref)[di] = v;
+	}
+}
+
+hipError_t launch_ssaa_resolve(const PassDesc *passes, const uint32_t *items, uint32_t n, hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_ssaa_resolve, dim3(n), dim3(256), 0, s, passes, items);
+	return hipGetLastError();
 }
 
 // One range, colour only (submit_update_upscaled_domain_external,

@@ -48,6 +48,7 @@ hipError_t launch_raster_round(uint32_t cls, const PassDesc *, const uint32_t *,
                                const uint32_t *, uint32_t, uint32_t, hipStream_t);
 hipError_t launch_raster_up(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 hipError_t launch_domain(const PassDesc *, const uint32_t *, uint32_t, bool post, hipStream_t);
+hipError_t launch_ssaa_resolve(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_domain_range(const uint8_t *rdram, const uint8_t *hidden, uint8_t *ref, uint8_t *planes,
                                uint8_t *hplanes, uint32_t rdram_size, uint32_t num_planes, uint32_t addr,
                                uint32_t num_pixels, uint32_t size_log2, hipStream_t s);
@@ -374,6 +375,10 @@ struct Processor {
 	// reference copy of RDRAM the domain last agreed with.
 	uint32_t up_log2 = 0;
 	uint8_t *d_up_rdram = nullptr, *d_up_hidden = nullptr, *d_up_ref = nullptr;
+	// Super-sampled readback (RDP_MI_FLAG_SUPER_SAMPLED_READ_BACK): a write
+	// mask word per native pixel of a pass's region (1024 x 1024 at most).
+	bool ssaa = false, ssaa_dither = false;
+	uint32_t *d_up_wmask = nullptr;
 
 	explicit Processor(uint32_t size) : fe(size), rdram_size(size) { memset(tmem_mirror, 0, sizeof(tmem_mirror)); }
 	uint8_t *host_rdram() { return user_rdram ? user_rdram : rdram_mirror.data(); }
@@ -438,6 +443,8 @@ struct Launch {
 	uint32_t domain_pre_n, domain_post_n;
 	const uint32_t *up_items[kRasterClasses];
 	uint32_t up_n[kRasterClasses];
+	const uint32_t *resolve_items;  // the round's upscaled passes with a resolve, after their raster
+	uint32_t resolve_n;
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
 	bool tmem_early;  // no upload reads what the previous round's raster writes
 	uint64_t raster_bytes;  // algorithmic framebuffer bytes read + written by k_raster
@@ -486,6 +493,12 @@ static uint32_t domain_blocks(const HostPass &hp)
 	const uint32_t align = 4u >> size_log2;
 	const uint32_t n = (hp.fb_width * hp.domain_height + align - 1u) & ~(align - 1u);
 	return (n + 255u) / 256u;
+}
+
+// Blocks of 256 native pixels of an upscaled pass's resolve region.
+static uint32_t resolve_blocks(const HostPass &hp)
+{
+	return ((hp.fb_width >> hp.scale_log2) * hp.domain_height + 255u) / 256u;
 }
 
 // A pass whose every primitive is a FILL-mode primitive.
@@ -644,6 +657,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		HIP_CHECK(launch_domain(L.descs, L.domain_post_items, L.domain_post_n, true, s));
 		for (uint32_t c = 0; c < kRasterClasses; c++)
 			HIP_CHECK(launch_raster_up(c, L.descs, L.up_items[c], L.up_n[c], cfg.xcd_chunk, s));
+		HIP_CHECK(launch_ssaa_resolve(L.descs, L.resolve_items, L.resolve_n, s));
 		if (stage_events) HIP_CHECK(hipEventRecord(stage_events[r * 4 + 3], s));
 	}
 	return 0;
@@ -667,7 +681,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		size_t descs, items_setup, items_tmem, items_bin, items_raster, items_domain, items_up;
 		uint32_t n_setup, n_span, n_tmem, n_tmem_gather, n_bin, n_raster, n_fill, n_quad;
 		uint32_t n_raster_class[kRasterClasses], n_quad_class[kRasterClasses];
-		uint32_t n_dom_pre, n_dom_post, n_up, n_up_class[kRasterClasses];
+		uint32_t n_dom_pre, n_dom_post, n_up, n_up_class[kRasterClasses], n_resolve;
 		size_t scratch;
 	};
 	std::vector<RoundLayout> rl(rounds.size());
@@ -698,7 +712,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			return set_error(RDP_MI_ERROR_UNSUPPORTED, "%zu passes in one round (at most 32768)", rounds[r].size());
 		RoundLayout &L = rl[r];
 		L.n_setup = L.n_span = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = L.n_fill = L.n_quad = 0;
-		L.n_dom_pre = L.n_dom_post = L.n_up = 0;
+		L.n_dom_pre = L.n_dom_post = L.n_up = L.n_resolve = 0;
 		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = L.n_quad_class[c] = L.n_up_class[c] = 0;
 		size_t scr = scratch_base;
 		pl[r].resize(rounds[r].size());
@@ -720,6 +734,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			}
 			if (hp.domain_pre) L.n_dom_pre += domain_blocks(hp);
 			if (hp.domain_post) L.n_dom_post += domain_blocks(hp);
+			if (hp.ssaa_resolve) L.n_resolve += resolve_blocks(hp);
 			PassInputs &pi = pl[r][i];
 			pi.fill = fill_only(hp);
 			if (!hp.prims.empty()) {
@@ -755,7 +770,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		L.items_raster = off;
 		off = align_up(off + (L.n_raster + L.n_fill + L.n_quad) * 4ull, 256);
 		L.items_domain = off;
-		off = align_up(off + (L.n_dom_pre + L.n_dom_post) * 4ull, 256);
+		off = align_up(off + (L.n_dom_pre + L.n_dom_post + L.n_resolve) * 4ull, 256);
 		L.items_up = off;
 		off = align_up(off + L.n_up * 4ull, 256);
 		for (size_t i = 0; i < rounds[r].size(); i++) {
@@ -802,7 +817,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		uint32_t *it_raster = reinterpret_cast<uint32_t *>(h + L.items_raster);
 		uint32_t *it_domain = reinterpret_cast<uint32_t *>(h + L.items_domain);
 		uint32_t *it_up = reinterpret_cast<uint32_t *>(h + L.items_up);
-		uint32_t ns = 0, nsp = 0, nt = 0, ntg = 0, nb = 0, ndp = 0, ndq = 0;
+		uint32_t ns = 0, nsp = 0, nt = 0, ntg = 0, nb = 0, ndp = 0, ndq = 0, nrs = 0;
 		uint32_t upbase[kRasterClasses], nu[kRasterClasses];
 		for (uint32_t c = 0, acc = 0; c < kRasterClasses; c++) {
 			upbase[c] = acc;
@@ -850,6 +865,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		LA.domain_pre_n = L.n_dom_pre;
 		LA.domain_post_items = LA.domain_pre_items + L.n_dom_pre;
 		LA.domain_post_n = L.n_dom_post;
+		LA.resolve_items = LA.domain_post_items + L.n_dom_post;
+		LA.resolve_n = L.n_resolve;
 		LA.raster_bytes = 0;
 		LA.fill_bytes = 0;
 		LA.fill_tiles = 0;
@@ -915,6 +932,10 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			D.up_ref = pr.d_up_ref;
 			D.domain_height = hp.domain_height;
 			D.domain_log2 = pr.up_log2;
+			D.up_wmask = pr.d_up_wmask;
+			D.domain_flags = (hp.ssaa_clear ? DOMAIN_SSAA_CLEAR : 0u) | (hp.ssaa_resolve ? DOMAIN_SSAA_RESOLVE : 0u) |
+			                 (pr.ssaa_dither ? DOMAIN_SSAA_DITHER : 0u);
+			D.resolve_rdram = pr.d_rdram;
 			if (hp.scale_log2) {
 				// The upscaled pass renders into the sample planes with the
 				// TMEM snapshots of its native pass.
@@ -944,6 +965,9 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 				for (uint32_t b = 0; b < domain_blocks(hp); b++) it_domain[ndp++] = (pidx << 17) | b;
 			if (hp.domain_post)
 				for (uint32_t b = 0; b < domain_blocks(hp); b++) it_domain[L.n_dom_pre + ndq++] = (pidx << 17) | b;
+			if (hp.ssaa_resolve)
+				for (uint32_t b = 0; b < resolve_blocks(hp); b++)
+					it_domain[L.n_dom_pre + L.n_dom_post + nrs++] = (pidx << 17) | b;
 		}
 		// Raster items, per class, passes with the most primitives per tile
 		// first: their tiles have the longest chains, so they start early
@@ -1894,12 +1918,14 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "rdram_size must be a power of two");
 	if (hidden_rdram_size != rdram_size / 2)
 		return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "hidden_rdram_size must be rdram_size / 2");
-	const uint32_t up_flags = flags & (RDP_MI_FLAG_UPSCALING_2X | RDP_MI_FLAG_UPSCALING_4X | RDP_MI_FLAG_UPSCALING_8X);
-	const uint32_t up_log2 = up_flags == RDP_MI_FLAG_UPSCALING_2X ? 1 : up_flags == RDP_MI_FLAG_UPSCALING_4X ? 2
-	                         : up_flags == RDP_MI_FLAG_UPSCALING_8X ? 3 : 0;
-	if (up_flags && !up_log2) return set_error(RDP_MI_ERROR_INVALID_ARGUMENT, "more than one upscaling flag");
+	// The largest factor whose flag is set (rdp_device.cpp:188-195).
+	const uint32_t up_log2 = (flags & RDP_MI_FLAG_UPSCALING_8X) ? 3 : (flags & RDP_MI_FLAG_UPSCALING_4X) ? 2
+	                         : (flags & RDP_MI_FLAG_UPSCALING_2X) ? 1 : 0;
 	if (up_log2 && rdram_ptr)
 		return set_error(RDP_MI_ERROR_UNSUPPORTED, "upscaling with host-coherent RDRAM is not supported");
+	const bool ssaa = (flags & RDP_MI_FLAG_SUPER_SAMPLED_READ_BACK) != 0;
+	if (ssaa && !up_log2)
+		return set_error(RDP_MI_ERROR_UNSUPPORTED, "super-sampled readback needs an upscaling flag");
 	Context &c = ctx();
 	std::lock_guard<std::mutex> g(c.lock);
 	int err = c.init();
@@ -1912,6 +1938,7 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 		if (p.d_up_rdram) (void)hipFree(p.d_up_rdram);
 		if (p.d_up_hidden) (void)hipFree(p.d_up_hidden);
 		if (p.d_up_ref) (void)hipFree(p.d_up_ref);
+		if (p.d_up_wmask) (void)hipFree(p.d_up_wmask);
 		if (p.d_rdram) (void)hipFree(p.d_rdram);
 		if (p.d_hidden) (void)hipFree(p.d_hidden);
 		if (p.d_tmem) (void)hipFree(p.d_tmem);
@@ -1960,6 +1987,13 @@ int rdp_mi_create(rdp_mi_processor **out, void *rdram_ptr, size_t rdram_offset, 
 		    hipMemsetAsync(p.d_up_hidden, 0x03, planes * hidden_rdram_size, c.stream) != hipSuccess ||
 		    hipMemsetAsync(p.d_up_ref, 0, rdram_size, c.stream) != hipSuccess)
 			return fail(RDP_MI_ERROR_DEVICE, "upscaled domain clear failed");
+		if (ssaa) {
+			if (hipMalloc(&p.d_up_wmask, (size_t)kMaxWidth * kMaxHeight * sizeof(uint32_t)) != hipSuccess)
+				return fail(RDP_MI_ERROR_OUT_OF_MEMORY, "write mask allocation failed");
+			p.ssaa = true;
+			p.ssaa_dither = (flags & RDP_MI_FLAG_SUPER_SAMPLED_DITHER) != 0;
+			p.fe.ssaa = true;
+		}
 		p.up_log2 = up_log2;
 		p.fe.scale_log2 = up_log2;
 	}
@@ -1994,6 +2028,7 @@ void rdp_mi_destroy(rdp_mi_processor *h)
 		(void)hipFree(p.d_up_rdram);
 		(void)hipFree(p.d_up_hidden);
 		(void)hipFree(p.d_up_ref);
+		if (p.d_up_wmask) (void)hipFree(p.d_up_wmask);
 	}
 	if (p.coherent) {
 		(void)hipFree(p.d_wmask);

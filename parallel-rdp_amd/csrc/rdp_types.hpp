@@ -425,6 +425,17 @@ struct PassDesc {
 	// the region height; domain_log2 is the factor of the domain.
 	uint8_t *up_rdram, *up_hidden, *up_ref;
 	uint32_t domain_height, domain_log2;
+	// Super-sampled readback: per native pixel of the region, bit 0 colour
+	// and bit 1 depth written by any sample; DOMAIN_* flags.
+	uint32_t *up_wmask;
+	uint32_t domain_flags;
+	uint8_t *resolve_rdram;  // an upscaled pass with DOMAIN_SSAA_RESOLVE: native RDRAM
+};
+
+enum : uint32_t {
+	DOMAIN_SSAA_CLEAR = 1u << 0,    // the pre update also clears the write masks of the region
+	DOMAIN_SSAA_RESOLVE = 1u << 1,  // the upscaled raster records write masks; the pass is resolved
+	DOMAIN_SSAA_DITHER = 1u << 2,   // ordered dither in the 16-bit resolve
 };
 
 }  // namespace rdp_mi

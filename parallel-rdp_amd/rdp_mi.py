@@ -32,6 +32,8 @@ ERROR_LIMIT = -6
 
 # rdp_mi_create flags (rdp_device.hpp:43-51).
 FLAG_UPSCALING = {2: 1 << 2, 4: 1 << 3, 8: 1 << 4}
+FLAG_SUPER_SAMPLED_READ_BACK = 1 << 5
+FLAG_SUPER_SAMPLED_DITHER = 1 << 6
 
 # Words per command by opcode (the RDP's command lengths).
 COMMAND_LENGTHS = np.array([

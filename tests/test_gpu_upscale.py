@@ -41,10 +41,13 @@ def assert_same(gpu, ref, what):
     assert not bad, f"{what}: " + "; ".join(bad)
 
 
-def oracle_state(stream, factor, scanouts=None, opts=None):
+def oracle_state(stream, factor, scanouts=None, opts=None, ssaa=None):
+    """ssaa: None, or the dither switch of super-sampled readback."""
     import oracle
     o = oracle.Oracle(rs.RDRAM_SIZE)
     o.enable_upscaling(factor)
+    if ssaa is not None:
+        o.enable_super_sampled_readback(ssaa)
     if stream.rdram_init is not None:
         o.rdram[:] = stream.rdram_init
     o.hidden[:] = stream.hidden_init
@@ -60,8 +63,11 @@ def oracle_state(stream, factor, scanouts=None, opts=None):
     return out
 
 
-def new_proc(rdp_mi, stream, factor, hold=False):
-    p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=rdp_mi.FLAG_UPSCALING[factor])
+def new_proc(rdp_mi, stream, factor, hold=False, ssaa=None):
+    flags = rdp_mi.FLAG_UPSCALING[factor]
+    if ssaa is not None:
+        flags |= rdp_mi.FLAG_SUPER_SAMPLED_READ_BACK | (rdp_mi.FLAG_SUPER_SAMPLED_DITHER if ssaa else 0)
+    p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=flags)
     assert p.upscaling_factor == factor
     if hold:
         p.hold_submissions()
@@ -75,8 +81,8 @@ def gpu_state(p):
             p.read_upscaled_hidden_rdram())
 
 
-def run_gpu(rdp_mi, stream, factor, scanouts=None, opts=None):
-    p = new_proc(rdp_mi, stream, factor)
+def run_gpu(rdp_mi, stream, factor, scanouts=None, opts=None, ssaa=None):
+    p = new_proc(rdp_mi, stream, factor, ssaa=ssaa)
     for f in stream.frames:
         p.enqueue_commands(f.words)
         if scanouts is not None and f.vi:
@@ -194,11 +200,36 @@ def test_upscaled_scanouts_of_rendered_frames(rdp_mi, factor):
         assert np.array_equal(a, b), f"scanout {i}: {int(np.count_nonzero(a != b))} pixels differ"
 
 
+def test_upscaled_graph_replays(rdp_mi):
+    """Frames that start with a clear give the same native and upscaled
+    state on every replay: one eager replay, then graph replays, each
+    checked against the oracle's single run (2x, 4x and a readback
+    processor in one batched recording)."""
+    streams = [rs.batch_stream(i, max_tris=300) for i in (1, 4, 8)]
+    modes = [(2, None), (4, None), (2, True)]
+    say("[case] upscaled graph replays")
+    procs = []
+    for s, (f, ssaa) in zip(streams, modes):
+        p = new_proc(rdp_mi, s, f, hold=True, ssaa=ssaa)
+        for fr in s.frames:
+            p.enqueue_commands(fr.words)
+        procs.append(p)
+    rec = rdp_mi.Recording(procs, batched=True)
+    refs = [oracle_state(s, f, ssaa=ssaa) for s, (f, ssaa) in zip(streams, modes)]
+    rec.replay(1, use_graph=False)
+    for k in range(3):
+        for p, s, ref in zip(procs, streams, refs):
+            assert_same(gpu_state(p), ref, f"{s.name} replay {k}")
+        rec.replay(1, use_graph=True)
+    rec.close()
+    for p in procs:
+        p.close()
+
+
 @pytest.mark.parametrize("batched", [True, False])
-def test_upscaled_recording_and_graph_replay(rdp_mi, batched):
+def test_upscaled_recording(rdp_mi, batched):
     """Upscaled passes follow their native passes round by round in a
-    recording of several processors (some 2x, some 4x); each graph replay
-    of frames that start with a clear must give the eager result."""
+    recording of several processors (some 2x, some 4x)."""
     streams = [rs.batch_stream(i, max_tris=300) for i in (0, 2, 7, 10)] + \
               [rs.fuzz_stream(7100 + i, width=320, height=64) for i in range(2)]
     factors = [2, 4, 2, 2, 4, 2]
@@ -216,3 +247,41 @@ def test_upscaled_recording_and_graph_replay(rdp_mi, batched):
     rec.close()
     for p in procs:
         p.close()
+
+
+@pytest.mark.parametrize("factor,dither", [(2, False), (2, True), (4, False), (8, True)])
+def test_super_sampled_readback_config0(rdp_mi, factor, dither):
+    """RDP_MI_FLAG_SUPER_SAMPLED_READ_BACK: passes render at scale only and
+    are resolved into native RDRAM."""
+    s = rs.config0_stream(num_triangles=64)
+    say(f"[case] {s.name} ssaa x{factor} dither={dither}")
+    assert_same(run_gpu(rdp_mi, s, factor, ssaa=dither), oracle_state(s, factor, ssaa=dither),
+                f"{s.name} ssaa x{factor}")
+
+
+@pytest.mark.parametrize("index,factor,dither", [(1, 2, True), (2, 2, False), (3, 4, True), (6, 2, False)])
+def test_super_sampled_readback_batch(rdp_mi, index, factor, dither):
+    s = rs.batch_stream(index, max_tris=300)
+    say(f"[case] {s.name} ssaa x{factor} dither={dither}")
+    assert_same(run_gpu(rdp_mi, s, factor, ssaa=dither), oracle_state(s, factor, ssaa=dither),
+                f"{s.name} ssaa x{factor}")
+
+
+@pytest.mark.parametrize("seed", range(0, 12, 4))
+def test_super_sampled_readback_fuzz(rdp_mi, seed):
+    """Every framebuffer format and width renders at scale and resolves
+    (8-bit colour from the first sample, depth from the first sample)."""
+    for k in range(seed, seed + 4):
+        s = rs.fuzz_stream(7300 + k, width=int((320, 160, 256, 96)[k % 4]), height=64)
+        say(f"[case] {s.name} ssaa x2")
+        assert_same(run_gpu(rdp_mi, s, 2, ssaa=bool(k & 1)), oracle_state(s, 2, ssaa=bool(k & 1)), f"{s.name} ssaa")
+
+
+def test_super_sampled_readback_flags(rdp_mi):
+    """Without an upscaling flag the readback flag is refused; with several
+    upscaling flags the largest factor wins (rdp_device.cpp:188-195)."""
+    with pytest.raises(rdp_mi.RDPError):
+        rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=rdp_mi.FLAG_SUPER_SAMPLED_READ_BACK)
+    p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=rdp_mi.FLAG_UPSCALING[2] | rdp_mi.FLAG_UPSCALING[4])
+    assert p.upscaling_factor == 4
+    p.close()

@@ -367,6 +367,133 @@ def check_vi_planes(make, factor):
 
 
 # ---------------------------------------------------------------------------
+# Super-sampled readback: passes render at scale only and resolve into
+# native RDRAM (update_upscaled_domain_resolve.comp).
+# ---------------------------------------------------------------------------
+def _ssaa(make, factor, dither=False):
+    o = make(factor)
+    o.enable_super_sampled_readback(dither)
+    return o
+
+
+def _mean_check(native, planes, dither):
+    """native (H, W) RGBA5551 words against the rounded mean of the plane
+    samples (N, H, W) per 5-bit channel and the alpha bit; with dither the
+    4x4 ordered offsets b / 8 (b = 0..7 from the Bayer-style table) are
+    added before the floor."""
+    n = planes.shape[0]
+    Y, X = np.mgrid[0:native.shape[0], 0:native.shape[1]]
+    bayer = np.array([0, 4, 1, 5, 4, 0, 5, 1, 3, 7, 2, 6, 7, 3, 6, 2])[(Y & 3) * 4 + (X & 3)]
+    for shift, bits in ((11, 5), (6, 5), (1, 5), (0, 1)):
+        mean = ((planes.astype(np.int64) >> shift) & ((1 << bits) - 1)).sum(axis=0) / n
+        exp = np.floor(mean + (bayer / 8.0 if dither else 0.5)).astype(np.int64)
+        got = (native.astype(np.int64) >> shift) & ((1 << bits) - 1)
+        assert np.array_equal(got, exp), (shift, int(np.count_nonzero(got != exp)))
+
+
+def check_ssaa_flat(make, factor, size):
+    """Pixel-aligned flat rectangles: all samples of a pixel agree, so the
+    resolved native colour is exactly the native rendering's; hidden RDRAM
+    is not written by the resolve."""
+    rng = np.random.default_rng(9000 + factor + size)
+    cl = CommandList()
+    _bind(cl, size=size)
+    cl.set_other_modes(ONE_CYCLE)
+    cl.set_combine(*PRIM)
+    for _ in range(16):
+        x0, y0 = int(rng.integers(0, W - 8)), int(rng.integers(0, H - 8))
+        x1, y1 = int(rng.integers(x0 + 1, W)), int(rng.integers(y0 + 1, H))
+        cl.set_prim_color(int(rng.integers(0, 1 << 32)))
+        cl.texture_rectangle(0, x0 << 2, y0 << 2, x1 << 2, y1 << 2, 0, 0, 1 << 10, 1 << 10)
+    cl.sync_full()
+    o = _ssaa(make, factor)
+    o.process(cl.words())
+    o.flush()
+    ref = make(1)
+    ref.process(cl.words())
+    ref.flush()
+    assert np.count_nonzero(_fb(o.rdram, size)) > W * H // 4
+    assert np.array_equal(_fb(o.rdram, size), _fb(ref.rdram, size))
+    assert np.all(o.hidden == 3)
+    o.close()
+    ref.close()
+
+
+def check_ssaa_mean(make, factor, dither):
+    """Gouraud triangles into RGBA5551: after the pass every native pixel of
+    the framebuffer is the (dithered) mean of its samples.  Pixels no sample
+    wrote hold what the pre update copied to every plane, so the mean is
+    their own value."""
+    rng = np.random.default_rng(9100 + factor + dither)
+    cl = CommandList()
+    _bind(cl)
+    cl.set_other_modes(ONE_CYCLE)
+    cl.set_combine(*SHADE)
+    xy = np.stack([rng.integers(0, W * 4, size=(6, 3)), rng.integers(0, H * 4, size=(6, 3))], axis=-1)
+    rgba = rng.integers(0, 256, size=(6, 3, 4)).astype(np.float64)
+    cl.extend_array(rs.triangle_commands(xy, rgba=rgba))
+    cl.sync_full()
+    o = _ssaa(make, factor, dither)
+    o.process(cl.words())
+    o.flush()
+    nat = _fb(o.rdram, rs.SIZ_16)
+    planes = np.stack([_fb(o.upscaled_rdram[p], rs.SIZ_16) for p in range(factor * factor)])
+    assert len(np.unique(nat)) > 64
+    _mean_check(nat, planes, dither)
+    o.close()
+
+
+def check_ssaa_depth(make, factor):
+    """Z-buffered triangles: the resolved depth is the first sample's."""
+    rng = np.random.default_rng(9200 + factor)
+    cl = CommandList()
+    # Clear Z through a fill pass (at scale too), then draw.
+    _bind(cl, addr=rs.DEPTH_ADDR)
+    cl.set_other_modes(OtherModes(cycle_type=3))
+    cl.set_fill_color(0xFFFCFFFC)
+    cl.fill_rectangle(0, 0, (W - 1) << 2, (H - 1) << 2)
+    cl.sync_pipe()
+    _bind(cl)
+    cl.set_other_modes(OtherModes(cycle_type=0, depth_test=True, depth_write=True, blend=((0, 0, 1, 0), (0, 0, 1, 0))))
+    cl.set_combine(*SHADE)
+    xy = np.stack([rng.integers(0, W * 4, size=(8, 3)), rng.integers(0, H * 4, size=(8, 3))], axis=-1)
+    rgba = rng.integers(0, 256, size=(8, 3, 4)).astype(np.float64)
+    z = rng.integers(0, 0x40000, size=(8, 3)).astype(np.float64)
+    cl.extend_array(rs.triangle_commands(xy, rgba=rgba, z=z))
+    cl.sync_full()
+    o = _ssaa(make, factor)
+    o.process(cl.words())
+    o.flush()
+    nat = _fb(o.rdram, rs.SIZ_16, addr=rs.DEPTH_ADDR)
+    assert np.count_nonzero(nat != 0xFFFC) > 1000
+    assert np.array_equal(nat, _fb(o.upscaled_rdram[0], rs.SIZ_16, addr=rs.DEPTH_ADDR))
+    o.close()
+
+
+def check_ssaa_host_writes(make):
+    """Memory the pass does not draw keeps what the host wrote."""
+    rng = np.random.default_rng(9300)
+    o = _ssaa(make, 2)
+    host = rng.integers(1, 1 << 16, size=W * H).astype(np.uint16)
+    rs.rdram_write16(o.rdram, COLOR, host)
+    cl = CommandList()
+    _bind(cl)
+    cl.set_other_modes(ONE_CYCLE)
+    cl.set_combine(*PRIM)
+    cl.set_prim_color(0xFF00FFFF)
+    cl.texture_rectangle(0, 40 << 2, 40 << 2, 60 << 2, 50 << 2, 0, 0, 1 << 10, 1 << 10)
+    cl.sync_full()
+    o.process(cl.words())
+    o.flush()
+    nat = _fb(o.rdram, rs.SIZ_16)
+    keep = np.ones((H, W), dtype=bool)
+    keep[40:50, 40:60] = False
+    assert np.array_equal(nat[keep], host.reshape(H, W)[keep])
+    assert np.all(nat[40:50, 40:60] >> 1 == 0xF83E >> 1)  # magenta in RGBA5551
+    o.close()
+
+
+# ---------------------------------------------------------------------------
 # The pins, and a planted mistake for each that its pin must catch.
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("factor", [2, 4, 8])
@@ -414,6 +541,26 @@ def test_upscaled_scanout_reads_sample_planes(oracle_mod, factor):
     check_vi_planes(_maker(oracle_mod), factor)
 
 
+@pytest.mark.parametrize("factor", [2, 4])
+@pytest.mark.parametrize("size", [rs.SIZ_16, rs.SIZ_32])
+def test_ssaa_flat_rectangles_resolve_to_native(oracle_mod, factor, size):
+    check_ssaa_flat(_maker(oracle_mod), factor, size)
+
+
+@pytest.mark.parametrize("factor,dither", [(2, False), (2, True), (4, False), (4, True)])
+def test_ssaa_resolve_is_the_sample_mean(oracle_mod, factor, dither):
+    check_ssaa_mean(_maker(oracle_mod), factor, dither)
+
+
+@pytest.mark.parametrize("factor", [2, 4])
+def test_ssaa_depth_is_the_first_sample(oracle_mod, factor):
+    check_ssaa_depth(_maker(oracle_mod), factor)
+
+
+def test_ssaa_keeps_undrawn_host_memory(oracle_mod):
+    check_ssaa_host_writes(_maker(oracle_mod))
+
+
 PLANTS = {
     30: lambda m: check_gradient(m, 2),              # colour x gradient not split by the factor
     31: lambda m: check_flat_rectangles(m, 2, rs.SIZ_16),  # H edge left at native scale
@@ -423,6 +570,9 @@ PLANTS = {
     35: lambda m: check_copy_blit(m, 2),             # copy-mode texel steps at S-times rate
     36: lambda m: check_vi_planes(m, 2),             # VI fetch reads transposed planes
     37: lambda m: check_texture_columns(m, 2),       # s/t x gradient not split by the factor
+    38: lambda m: check_ssaa_mean(m, 2, False),      # resolve mean truncated instead of rounded
+    39: lambda m: check_ssaa_mean(m, 2, True),       # dither offsets transposed
+    40: lambda m: check_ssaa_depth(m, 2),            # resolved depth from the second sample
 }
 
 
