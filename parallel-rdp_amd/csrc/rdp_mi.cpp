@@ -619,8 +619,10 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			// With both forked, prep_done covers the first two and the side
 			// stream's own order the third; otherwise the main stream has
 			// reached all of them here.
+			// The domain's pre update (main stream) must read native RDRAM
+			// before any raster of the round writes it.
 			hipEvent_t ready = E.prep_done;
-			if (!(ahead && fork_tmem)) {
+			if (!(ahead && fork_tmem) || L.domain_pre_n) {
 				HIP_CHECK(hipEventRecord(E.bin_done, s));
 				ready = E.bin_done;
 			}

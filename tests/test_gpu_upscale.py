@@ -226,6 +226,28 @@ def test_upscaled_graph_replays(rdp_mi):
         p.close()
 
 
+def test_upscaled_wide_rounds(rdp_mi):
+    """Rounds of many passes run setup ahead on a second stream and the
+    heavier raster classes beside the light one on a third; the domain's
+    pre update must still see native RDRAM before any of the round's
+    raster writes it (12 processors, mixed occupancy classes, 2x)."""
+    streams = [rs.batch_stream(i, max_tris=300) for i in range(12)]
+    say("[case] upscaled wide rounds")
+    procs = []
+    for s in streams:
+        p = new_proc(rdp_mi, s, 2, hold=True)
+        for fr in s.frames:
+            p.enqueue_commands(fr.words)
+        procs.append(p)
+    rec = rdp_mi.Recording(procs, batched=True)
+    rec.replay(1, use_graph=True)
+    for p, s in zip(procs, streams):
+        assert_same(gpu_state(p), oracle_state(s, 2), f"{s.name} x2")
+    rec.close()
+    for p in procs:
+        p.close()
+
+
 @pytest.mark.parametrize("batched", [True, False])
 def test_upscaled_recording(rdp_mi, batched):
     """Upscaled passes follow their native passes round by round in a
