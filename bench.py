@@ -48,10 +48,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def setup_processors(rdp_mi, streams):
+def setup_processors(rdp_mi, streams, flags=0):
     procs = []
     for s in streams:
-        p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE)
+        p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=flags)
         # Every enqueued pass stays pending for the recording (no idle-time
         # submission behind the recording's back).
         p.hold_submissions()
@@ -232,6 +232,60 @@ def serving(rdp_mi, streams, refs, args):
     return out
 
 
+def oracle_upscaled(s, factor, ssaa):
+    """The oracle's upscaled run of one stream: native RDRAM, hidden RDRAM,
+    TMEM, the RDRAM and hidden sample planes."""
+    o = _oracle_module().Oracle(rs.RDRAM_SIZE)
+    o.enable_upscaling(factor)
+    if ssaa is not None:
+        o.enable_super_sampled_readback(ssaa)
+    if s.rdram_init is not None:
+        o.rdram[:] = s.rdram_init
+    o.hidden[:] = s.hidden_init
+    for f in s.frames:
+        o.process(f.words)
+    o.flush()
+    out = (o.rdram.copy(), o.hidden.copy(), o.tmem.copy(), o.upscaled_rdram.copy(), o.upscaled_hidden.copy())
+    o.close()
+    return out
+
+
+def upscaled_rows(rdp_mi, streams, args):
+    """The batch rendered with upscaling (2x, 4x) and with super-sampled
+    readback (2x, dithered), replayed as one graph per step like the
+    headline form.  A sample of the streams is checked against the oracle's
+    upscaled run, native and sample-plane memory byte for byte; frames/s
+    counts the step's frames only when every sampled stream is exact."""
+    rows = {}
+    sample = [i for i in (0, 17, 33, 50) if i < len(streams)]
+    for name, factor, ssaa in (("2x", 2, None), ("4x", 4, None), ("2x_ssaa_dither", 2, True)):
+        flags = rdp_mi.FLAG_UPSCALING[factor]
+        if ssaa is not None:
+            flags |= rdp_mi.FLAG_SUPER_SAMPLED_READ_BACK | (rdp_mi.FLAG_SUPER_SAMPLED_DITHER if ssaa else 0)
+        procs = setup_processors(rdp_mi, streams, flags)
+        enqueue_all(procs, streams)
+        rec = rdp_mi.Recording(procs, batched=True)
+        rec.replay(max(args.warmup, 1), use_graph=True)
+        ms = float(rec.replay(max(3, args.steps // 5), use_graph=True).mean())
+        exact = 0
+        for i in sample:
+            p = procs[i]
+            gpu = (p.read_rdram(), p.read_hidden_rdram(), p.read_tmem(), p.read_upscaled_rdram(),
+                   p.read_upscaled_hidden_rdram())
+            ref = oracle_upscaled(streams[i], factor, ssaa)
+            exact += all(np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+                         for a, b in zip(gpu, ref))
+        rows[name] = {"factor": factor, "super_sampled_readback": ssaa is not None, "dither": bool(ssaa),
+                      "ms_per_step": round(ms, 4),
+                      "frames_per_s": round(len(streams) / (ms / 1e3), 1) if exact == len(sample) else 0.0,
+                      "bit_exact_sample": f"{exact}/{len(sample)}", "passes": rec.passes, "rounds": rec.rounds}
+        log(f"[upscaled] {name}: {rows[name]}")
+        rec.close()
+        for p in procs:
+            p.close()
+    return rows
+
+
 def config_rows(rdp_mi, pending, args):
     """Configs 0-2 on their own: each one stream (one processor), its frames
     recorded and replayed from device memory as one hipGraph per replay; the
@@ -305,6 +359,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison (profiling runs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the per-config rows (configs 0-2 on their own)")
+    ap.add_argument("--no-upscaled", action="store_true", help="skip the upscaled and super-sampled rows")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (profiling runs: the tuner's trial launches stay out)")
     ap.add_argument("--instances", type=int, default=256)
@@ -413,6 +468,7 @@ def main():
 
     serve = None if args.no_serve else serving(rdp_mi, streams, refs, args)
     configs = None if args.no_configs else config_rows(rdp_mi, pending, args)
+    upscaled = None if args.no_upscaled else upscaled_rows(rdp_mi, streams, args)
     bg.shutdown()
 
     frames_per_step = len(streams)
@@ -519,6 +575,7 @@ def main():
         "cpu_baseline": cpu,
         "configs": configs,
         "serving": serve,
+        "upscaled": upscaled,
     }
     print(json.dumps(out), flush=True)
 

@@ -26,51 +26,69 @@ unsigned Frontend::command_length(uint32_t first_word)
 	return lengths[(first_word >> 24) & 63];
 }
 
+// The framebuffer rows the upscaled domain updates for the reference pass
+// now ending (0: none).  Without super-sampled readback, a pass that renders
+// (render_pass_is_upscaled / submit_render_pass, rdp_renderer.cpp:2223-2246,
+// 2988-3010); with it, every pass with span jobs or TMEM uploads (flush_queues,
+// rdp_renderer.cpp:2908-2916, then submit_render_pass_upscaled).
+uint32_t Frontend::ref_domain_height() const
+{
+	if (!scale_log2 || !fb_width) return 0;
+	const bool work = ssaa ? (ref.span_jobs || ref.uploads) : (ref.span_jobs != 0);
+	return work ? ref.deduced_height : 0;
+}
+
+// Hands a completed device pass over.  domain_height: the largest region of
+// the reference passes it holds (ref_domain_height).  Merged
+// reference passes of one framebuffer update the union of their regions:
+// each one's update leaves the rows an earlier one already synchronised
+// unchanged, and a pass writes no row past its own region.
+void Frontend::complete_pass(HostPass &&p, uint32_t domain_height)
+{
+	p.fb_addr = fb_addr;
+	p.fb_width = fb_width;
+	p.fb_fmt = fb_fmt;
+	p.depth_addr = depth_addr;
+	p.marks.clear();
+	if (scale_log2 && ssaa) {
+		// Super-sampled readback: the pre update over the region, the draws
+		// at scale only (with write masks), then the resolve.
+		p.domain_height = domain_height;
+		p.domain_pre = p.ssaa_clear = domain_height != 0;
+		HostPass up;
+		if (domain_height && !p.prims.empty()) {
+			up = upscaled_copy(p);
+			up.domain_height = domain_height;
+			up.ssaa_resolve = true;
+		}
+		// The native pass keeps its uploads (the TMEM snapshots the upscaled
+		// pass reads) and draws nothing.
+		p.prims.clear();
+		p.words.clear();
+		p.states.clear();
+		p.num_spans = p.num_tilerows = p.fb_height = p.raster_class = 0;
+		done.push_back(std::move(p));
+		if (!up.prims.empty()) done.push_back(std::move(up));
+	} else if (scale_log2) {
+		// A pass that renders runs the domain's pre update over its region; a
+		// 16/32-bit colour pass at least 256 pixels wide is then rendered
+		// again at scale (should_render_upscaled, rdp_renderer.cpp:2999-3010).
+		const bool upscaled =
+		    domain_height && (fb_fmt == FB_RGBA5551 || fb_fmt == FB_RGBA8888) && fb_width >= 256;
+		p.domain_height = domain_height;
+		p.domain_pre = domain_height != 0;
+		p.domain_post = upscaled;
+		done.push_back(std::move(p));
+		if (upscaled) done.push_back(upscaled_copy(done.back()));
+	} else
+		done.push_back(std::move(p));
+}
+
 void Frontend::flush()
 {
-	if (!cur.empty()) {
-		cur.fb_addr = fb_addr;
-		cur.fb_width = fb_width;
-		cur.fb_fmt = fb_fmt;
-		cur.depth_addr = depth_addr;
-		cur.marks.clear();
-		if (scale_log2 && ssaa) {
-			// Super-sampled readback (flush_queues without the native pass,
-			// rdp_renderer.cpp:2908-2985, and submit_render_pass_upscaled):
-			// a pass with span jobs or TMEM uploads runs the pre update over
-			// its region, renders at scale only, then resolves.
-			const bool work = ref.span_jobs || ref.uploads;
-			cur.domain_height = (work && fb_width) ? ref.deduced_height : 0;
-			cur.domain_pre = cur.ssaa_clear = cur.domain_height != 0;
-			HostPass up;
-			if (work && !cur.prims.empty()) {
-				up = upscaled_copy(cur);
-				up.domain_height = cur.domain_height;
-				up.ssaa_resolve = cur.domain_height != 0;
-			}
-			// The native pass keeps its uploads (the TMEM snapshots the
-			// upscaled pass reads) and draws nothing.
-			cur.prims.clear();
-			cur.words.clear();
-			cur.states.clear();
-			cur.num_spans = cur.num_tilerows = cur.fb_height = cur.raster_class = 0;
-			done.push_back(std::move(cur));
-			if (!up.prims.empty()) done.push_back(std::move(up));
-		} else if (scale_log2) {
-			// render_pass_is_upscaled / should_render_upscaled
-			// (rdp_renderer.cpp:2988-3010): a pass that renders runs the
-			// domain's pre update over its region; a 16/32-bit colour pass at
-			// least 256 pixels wide is then rendered again at scale.
-			const bool renders = fb_width && ref.deduced_height && ref.span_jobs;
-			const bool upscaled = renders && (fb_fmt == FB_RGBA5551 || fb_fmt == FB_RGBA8888) && fb_width >= 256;
-			cur.domain_height = renders ? ref.deduced_height : 0;
-			cur.domain_pre = renders;
-			cur.domain_post = upscaled;
-			done.push_back(std::move(cur));
-			if (upscaled) done.push_back(upscaled_copy(done.back()));
-		} else
-			done.push_back(std::move(cur));
-	}
+	const uint32_t dh = std::max(domain_height, ref_domain_height());
+	if (!cur.empty()) complete_pass(std::move(cur), dh);
+	domain_height = 0;
 	cur = HostPass();
 	state_dirty = true;
 	color_write_pending = false;
@@ -349,14 +367,14 @@ void Frontend::add_ref_keys(unsigned setup_tile)
 // The reference ends a render pass here (flush_queues).
 void Frontend::ref_pass_end()
 {
-	// With upscaling, every reference pass is a pass of its own: the domain
-	// updates run over each pass's own framebuffer region.
-	if (fb_fmt == FB_I8 || fb_addr == depth_addr || scale_log2) {
+	if (fb_fmt == FB_I8 || fb_addr == depth_addr) {
 		flush();
 		return;
 	}
+	domain_height = std::max(domain_height, ref_domain_height());
 	if (!cur.empty()) {
 		PassMark m;
+		m.domain_height = domain_height;
 		m.prims = (uint32_t)cur.prims.size();
 		m.uploads = (uint32_t)cur.uploads.size();
 		m.words = (uint32_t)cur.words.size();
@@ -390,7 +408,9 @@ void Frontend::split_at_last_mark()
 	head.num_spans = m.spans;
 	head.num_tilerows = m.tilerows;
 	for (const PrimRecord &r : head.prims) head.raster_class = std::max<uint32_t>(head.raster_class, r.raster_class);
-	if (!head.empty()) done.push_back(std::move(head));
+	if (!head.empty()) complete_pass(std::move(head), m.domain_height);
+	// What stays open is the reference pass after the mark.
+	domain_height = 0;
 
 	HostPass tail;
 	tail.words.assign(cur.words.begin() + m.words, cur.words.end());

@@ -35,6 +35,7 @@ enum ValidationError : uint32_t {
 struct PassMark {
 	uint32_t prims, uploads, words, states, spans, tilerows, fb_height;
 	bool color_written, depth_written;
+	uint32_t domain_height;  // upscaled domain rows of the reference passes up to here
 };
 
 struct HostPass {
@@ -154,6 +155,8 @@ private:
 	bool color_write_pending = false, depth_write_pending = false;
 
 	HostPass cur;
+	// Upscaled domain rows of the reference passes that ended inside cur.
+	uint32_t domain_height = 0;
 	std::vector<HostPass> done;
 	RefPassState ref;
 	std::function<void(uint32_t, const char *)> validation;
@@ -171,6 +174,8 @@ private:
 	void split_at_last_mark();
 	void split_or_flush();
 	HostPass upscaled_copy(const HostPass &native) const;
+	uint32_t ref_domain_height() const;
+	void complete_pass(HostPass &&p, uint32_t domain_height);
 	void validate_draw() const;
 	void report(uint32_t err, const char *msg) const
 	{

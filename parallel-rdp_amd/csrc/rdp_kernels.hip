@@ -2310,10 +2310,28 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 		if (wm) mark_color(wm, pe->rdram_mask, fmt, ci);
 	}
 	if constexpr (UP) {
-		// emit_scaled_write_masks (memory_interfacing.h:305-336).
-		const uint32_t bits = (px.color_dirty ? 1u : 0u) | (px.depth_dirty ? 2u : 0u);
-		if (in_fb && bits && (pe->domain_flags & DOMAIN_SSAA_RESOLVE))
-			atomicOr(pe->up_wmask + (size_t)ny * (pe->fb_width >> L) + nx, bits);
+		// emit_scaled_write_masks (memory_interfacing.h:305-336).  At 4x
+		// and 8x the 16 or 64 lanes of one native pixel merge their bits
+		// first and one lane per pixel sets them; at 2x (four lanes per
+		// pixel) each lane sets its own.
+		if (pe->domain_flags & DOMAIN_SSAA_RESOLVE) {
+			const uint32_t bits = in_fb ? (px.color_dirty ? 1u : 0u) | (px.depth_dirty ? 2u : 0u) : 0u;
+			const uint32_t key = ny * (pe->fb_width >> L) + nx;
+			uint64_t todo = __ballot(bits != 0);
+			if (L == 1) {
+				if (bits) atomicOr(pe->up_wmask + key, bits);
+				todo = 0;
+			}
+			while (todo) {
+				const int leader = __ffsll((unsigned long long)todo) - 1;
+				const uint32_t k = __builtin_amdgcn_readlane(key, leader);
+				const uint64_t same = __ballot(key == k && bits != 0);
+				const uint32_t merged = (__ballot(key == k && (bits & 1u)) ? 1u : 0u) |
+				                        (__ballot(key == k && (bits & 2u)) ? 2u : 0u);
+				if (lane == leader) atomicOr(pe->up_wmask + k, merged);
+				todo &= ~same;
+			}
+		}
 	}
 	if (in_fb && !alias && px.depth_dirty) {
 		const uint32_t de = (pe->depth_addr >> 1) + (pe->fb_width >> L) * ny + nx;

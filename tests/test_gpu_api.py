@@ -90,10 +90,12 @@ def test_validation_reports_fill_mode_depth_test(proc):
     assert any("depth test" in m for _, m in proc.validation_errors)
 
 
-def test_upscaling_flag_is_rejected():
+def test_upscaling_with_host_coherent_rdram_is_rejected():
+    """Upscaling needs device-resident RDRAM (RDP_MI_ERROR_UNSUPPORTED)."""
     import rdp_mi
+    host = np.zeros(rs.RDRAM_SIZE // 4, dtype=np.uint32)
     with pytest.raises(rdp_mi.RDPError):
-        rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=1 << 2)
+        rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=rdp_mi.FLAG_UPSCALING[2], host_rdram=host)
 
 
 def test_small_rdram_wraps_like_the_oracle():
