@@ -689,6 +689,18 @@ DEV int32_t clamp_z(int32_t z)
 // entry 6 which the hardware table holds one lower; slopes are -4 * (r_i - r_i+1).
 DEV int32_t persp_base(int i) { return ((1 << 20) + (64 + i) / 2) / (64 + i) - (i == 6); }
 
+// The reciprocal table of perspective_divide (perspective.h:27-44), 65
+// entries from the closed form above, in LDS: the raster kernels fill it at
+// their start (persp_table_init), so the per-lane lookup is one LDS read
+// instead of an integer division.
+__shared__ int32_t s_persp_base[65];
+
+DEV void persp_table_init()
+{
+	for (int k = (int)threadIdx.x; k < 65; k += (int)blockDim.x) s_persp_base[k] = persp_base(k);
+	__syncthreads();
+}
+
 DEV void perspective_divide(int32_t s, int32_t t, int32_t w, int32_t &os, int32_t &ot, bool &overflow)
 {
 	bool w_carry = w <= 0;
@@ -697,8 +709,8 @@ DEV void perspective_divide(int32_t s, int32_t t, int32_t w, int32_t &os, int32_
 	int32_t normout = (w << shift) & 0x3fff;
 	int32_t wnorm = normout & 0xff;
 	int i = normout >> 8;
-	int32_t base = persp_base(i);
-	int32_t slope = -4 * (base - persp_base(i + 1));
+	int32_t base = s_persp_base[i];
+	int32_t slope = -4 * (base - s_persp_base[i + 1]);
 	int32_t rcp = ((slope * wnorm) >> 10) + base;
 	int32_t prod[2] = { s * rcp, t * rcp };
 	int32_t temp_mask = ((1 << 30) - 1) & -((1 << 29) >> shift);
@@ -2349,6 +2361,7 @@ __global__ __launch_bounds__(QUAD ? 256 : 64) __attribute__((amdgpu_waves_per_eu
 void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
 {
 	__shared__ uint16_t s_tmem_all[(QUAD ? 4 : 1) * 2048];
+	persp_table_init();
 	raster_tile<CLS, QUAD, UP>(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_tmem_all);
 }
 
@@ -2432,6 +2445,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
 {
 	__shared__ uint16_t s_unused[2];
+	// Fill-mode primitives never divide; the table is filled all the same so
+	// every kernel that can reach perspective_divide has it.
+	persp_table_init();
 	fill_item(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_unused);
 }
 
@@ -2449,6 +2465,7 @@ void k_raster_round(const PassDesc *__restrict__ passes, const uint32_t *__restr
                     uint32_t n_fill)
 {
 	__shared__ uint16_t s_tmem[2048];
+	persp_table_init();
 	const uint32_t total = n_hi + n_lo + n_fill;
 	for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
 		if (k < n_hi)
