@@ -2200,6 +2200,29 @@ DEV void staging_sync()
 // UP: the pass renders into the sample planes at 2^scale_log2 times the
 // resolution (submit_render_pass_upscaled, rdp_renderer.cpp:2287-2342), and
 // work items are pass << 20 | tile x << 10 | tile y.
+// emit_scaled_write_masks (memory_interfacing.h:305-336): bit 0 (colour)
+// and bit 1 (depth) of the native pixel `key` for a sample this lane wrote.
+// At 4x and 8x the 16 or 64 lanes of one native pixel merge their bits first
+// and one lane per pixel sets them; at 2x (four lanes per pixel) each lane
+// sets its own.
+DEV void emit_scaled_write_masks(uint32_t *wmask, uint32_t key, uint32_t bits, int L, int lane)
+{
+	uint64_t todo = __ballot(bits != 0);
+	if (L == 1) {
+		if (bits) atomicOr(wmask + key, bits);
+		todo = 0;
+	}
+	while (todo) {
+		const int leader = __ffsll((unsigned long long)todo) - 1;
+		const uint32_t k = __builtin_amdgcn_readlane(key, leader);
+		const uint64_t same = __ballot(key == k && bits != 0);
+		const uint32_t merged =
+		    (__ballot(key == k && (bits & 1u)) ? 1u : 0u) | (__ballot(key == k && (bits & 2u)) ? 2u : 0u);
+		if (lane == leader) atomicOr(wmask + k, merged);
+		todo &= ~same;
+	}
+}
+
 template <uint32_t CLS, bool QUAD, bool UP = false>
 DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_t *s_tmem_all)
 {
@@ -2322,28 +2345,9 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 		if (wm) mark_color(wm, pe->rdram_mask, fmt, ci);
 	}
 	if constexpr (UP) {
-		// emit_scaled_write_masks (memory_interfacing.h:305-336).  At 4x
-		// and 8x the 16 or 64 lanes of one native pixel merge their bits
-		// first and one lane per pixel sets them; at 2x (four lanes per
-		// pixel) each lane sets its own.
-		if (pe->domain_flags & DOMAIN_SSAA_RESOLVE) {
-			const uint32_t bits = in_fb ? (px.color_dirty ? 1u : 0u) | (px.depth_dirty ? 2u : 0u) : 0u;
-			const uint32_t key = ny * (pe->fb_width >> L) + nx;
-			uint64_t todo = __ballot(bits != 0);
-			if (L == 1) {
-				if (bits) atomicOr(pe->up_wmask + key, bits);
-				todo = 0;
-			}
-			while (todo) {
-				const int leader = __ffsll((unsigned long long)todo) - 1;
-				const uint32_t k = __builtin_amdgcn_readlane(key, leader);
-				const uint64_t same = __ballot(key == k && bits != 0);
-				const uint32_t merged = (__ballot(key == k && (bits & 1u)) ? 1u : 0u) |
-				                        (__ballot(key == k && (bits & 2u)) ? 2u : 0u);
-				if (lane == leader) atomicOr(pe->up_wmask + k, merged);
-				todo &= ~same;
-			}
-		}
+		if (pe->domain_flags & DOMAIN_SSAA_RESOLVE)
+			emit_scaled_write_masks(pe->up_wmask, ny * (pe->fb_width >> L) + nx,
+			                        in_fb ? (px.color_dirty ? 1u : 0u) | (px.depth_dirty ? 2u : 0u) : 0u, L, lane);
 	}
 	if (in_fb && !alias && px.depth_dirty) {
 		const uint32_t de = (pe->depth_addr >> 1) + (pe->fb_width >> L) * ny + nx;
@@ -2373,14 +2377,24 @@ void k_raster(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ 
 constexpr uint32_t kFillTiles = kFillTilesPerItem;
 using FillVariant = RasterVariant<RAS_FILL | RAS_COPY, RAS_FILL, (int)COMB_ANY, TEXFMT_ANY, (int)DBP_ANY>;
 
+// UP: an upscaled pass's item (pass << 20 | tile group << 10 | tile row),
+// every lane a sample of its native pixel in the sample planes, as in
+// raster_tile.
+template <bool UP>
 DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t *s_unused)
 {
-	const PassDesc &P = passes[item >> 14];
-	const uint32_t tx0 = ((item >> 7) & 0x7f) * kFillTiles, ty = item & 0x7f;
+	const PassDesc &P = passes[UP ? (item >> 20) : (item >> 14)];
+	const uint32_t tx0 = (UP ? (item >> 10) & 0x3ff : (item >> 7) & 0x7f) * kFillTiles, ty = UP ? item & 0x3ff : item & 0x7f;
 	const int lane = threadIdx.x & 63;
 	const uint32_t fmt = P.fb_fmt;
 	const bool alias = P.fb_addr == P.depth_addr;
-	const Mem m{ gptr(P.rdram), gptr(P.hidden), P.rdram_mask };
+	// Tiles start at multiples of 8, so a lane's sample slot (x mod S,
+	// y mod S) is the same in every tile of the item.
+	const int L = UP ? (int)uniform_u32(P.scale_log2) : 0;
+	const uint32_t smask = (1u << L) - 1;
+	const uint32_t slice = UP ? ((((uint32_t)lane >> 3) & smask) << L) + ((uint32_t)lane & smask) : 0;
+	const size_t plane8 = UP ? (size_t)slice * ((size_t)P.rdram_mask + 1) : 0;
+	const Mem m{ gptr(P.rdram) + plane8, gptr(P.hidden) + (plane8 >> 1), P.rdram_mask };
 	const AS_CONST RasterPrim *rprims = cptr(P.rprims);
 	const AS_CONST DrawState *states = cptr(P.states);
 	const AS_GLOBAL SpanSetup *spans = gptr(P.spans);
@@ -2409,7 +2423,8 @@ DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t 
 		case FB_IA88: addr_index = P.fb_addr >> 1; break;
 		default: addr_index = P.fb_addr >> 2; break;
 		}
-		const uint32_t ci = addr_index + P.fb_width * y + x;
+		const uint32_t nx = x >> L, ny = y >> L;
+		const uint32_t ci = addr_index + (P.fb_width >> L) * ny + nx;
 		Pixel px;
 		px.c[0] = px.c[1] = px.c[2] = px.c[3] = 0;
 		px.depth = px.dz = 0;
@@ -2430,13 +2445,20 @@ DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t 
 					const AS_CONST RasterPrim *ps = rprims + ((gbase + (uint32_t)g) * 64 + (uint32_t)bit);
 					const RasterPrimHead h = ldc(reinterpret_cast<const AS_CONST RasterPrimHead *>(&ps->ylo));
 					if (!in_fb || (int)y < h.ylo || (int)y > h.yhi) continue;
-					raster_prim<FillVariant>(ps, h, states, spans, tex, lut, fmt, alias, ci, (int)x, (int)y, px);
+					raster_prim<FillVariant>(ps, h, states, spans, tex, lut, fmt, alias, ci, (int)x, (int)y, px, L);
 				}
 			}
 		}
 		if (in_fb && px.color_dirty) {
 			store_color(m, fmt, ci, px);
 			if (P.wmask) mark_color(gptr(P.wmask), P.rdram_mask, fmt, ci);
+		}
+		// A fill writes no depth field (with colour and depth aliased the
+		// colour store is the depth store).
+		if constexpr (UP) {
+			if (P.domain_flags & DOMAIN_SSAA_RESOLVE)
+				emit_scaled_write_masks(P.up_wmask, ny * (P.fb_width >> L) + nx, in_fb && px.color_dirty ? 1u : 0u, L,
+				                        lane);
 		}
 	}
 }
@@ -2448,7 +2470,16 @@ void k_raster_fill(const PassDesc *__restrict__ passes, const uint32_t *__restri
 	// Fill-mode primitives never divide; the table is filled all the same so
 	// every kernel that can reach perspective_divide has it.
 	persp_table_init();
-	fill_item(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_unused);
+	fill_item<false>(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_unused);
+}
+
+// The FILL-only passes of an upscaled round: k_raster_fill in the sample planes.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+void k_raster_fill_up(const PassDesc *__restrict__ passes, const uint32_t *__restrict__ items, int32_t xcd_chunk)
+{
+	__shared__ uint16_t s_unused[2];
+	persp_table_init();
+	fill_item<true>(passes, items[xcd_local_item(blockIdx.x, gridDim.x, xcd_chunk)], s_unused);
 }
 
 // One launch for every raster work item of a round (persistent): the grid
@@ -2473,7 +2504,7 @@ void k_raster_round(const PassDesc *__restrict__ passes, const uint32_t *__restr
 		else if (k < n_hi + n_lo)
 			raster_tile<CLS, false>(passes, items_lo[k - n_hi], s_tmem);
 		else
-			fill_item(passes, fill_items[k - n_hi - n_lo], s_tmem);
+			fill_item<false>(passes, fill_items[k - n_hi - n_lo], s_tmem);
 		// The next item's staging must not overwrite texels this item's
 		// lanes still read.
 		__syncthreads();
@@ -2509,6 +2540,14 @@ hipError_t launch_raster_fill(const PassDesc *passes, const uint32_t *items, uin
 {
 	if (!n) return hipSuccess;
 	hipLaunchKernelGGL(k_raster_fill, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk);
+	return hipGetLastError();
+}
+
+hipError_t launch_raster_fill_up(const PassDesc *passes, const uint32_t *items, uint32_t n, int32_t xcd_chunk,
+                                 hipStream_t s)
+{
+	if (!n) return hipSuccess;
+	hipLaunchKernelGGL(k_raster_fill_up, dim3(n), dim3(64), 0, s, passes, items, xcd_chunk);
 	return hipGetLastError();
 }
 

@@ -47,6 +47,7 @@ hipError_t launch_raster_fill(const PassDesc *, const uint32_t *, uint32_t, int3
 hipError_t launch_raster_round(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, const uint32_t *, uint32_t,
                                const uint32_t *, uint32_t, uint32_t, hipStream_t);
 hipError_t launch_raster_up(uint32_t cls, const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
+hipError_t launch_raster_fill_up(const PassDesc *, const uint32_t *, uint32_t, int32_t, hipStream_t);
 hipError_t launch_domain(const PassDesc *, const uint32_t *, uint32_t, bool post, hipStream_t);
 hipError_t launch_ssaa_resolve(const PassDesc *, const uint32_t *, uint32_t, hipStream_t);
 hipError_t launch_domain_range(const uint8_t *rdram, const uint8_t *hidden, uint8_t *ref, uint8_t *planes,
@@ -443,6 +444,8 @@ struct Launch {
 	uint32_t domain_pre_n, domain_post_n;
 	const uint32_t *up_items[kRasterClasses];
 	uint32_t up_n[kRasterClasses];
+	const uint32_t *up_fill_items;  // upscaled passes of FILL primitives only: k_raster_fill_up
+	uint32_t up_fill_n;
 	const uint32_t *resolve_items;  // the round's upscaled passes with a resolve, after their raster
 	uint32_t resolve_n;
 	bool prep_ahead;  // every round has its own scratch: setup/binning may run ahead
@@ -657,6 +660,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		// After the native raster: the post updates, then the upscaled passes
 		// (rdp_renderer.cpp:2269-2278, 2287-2342, in flush_queues' order).
 		HIP_CHECK(launch_domain(L.descs, L.domain_post_items, L.domain_post_n, true, s));
+		HIP_CHECK(launch_raster_fill_up(L.descs, L.up_fill_items, L.up_fill_n, cfg.xcd_chunk, s));
 		for (uint32_t c = 0; c < kRasterClasses; c++)
 			HIP_CHECK(launch_raster_up(c, L.descs, L.up_items[c], L.up_n[c], cfg.xcd_chunk, s));
 		HIP_CHECK(launch_ssaa_resolve(L.descs, L.resolve_items, L.resolve_n, s));
@@ -683,7 +687,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		size_t descs, items_setup, items_tmem, items_bin, items_raster, items_domain, items_up;
 		uint32_t n_setup, n_span, n_tmem, n_tmem_gather, n_bin, n_raster, n_fill, n_quad;
 		uint32_t n_raster_class[kRasterClasses], n_quad_class[kRasterClasses];
-		uint32_t n_dom_pre, n_dom_post, n_up, n_up_class[kRasterClasses], n_resolve;
+		uint32_t n_dom_pre, n_dom_post, n_up, n_up_class[kRasterClasses], n_up_fill, n_resolve;
 		size_t scratch;
 	};
 	std::vector<RoundLayout> rl(rounds.size());
@@ -714,7 +718,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			return set_error(RDP_MI_ERROR_UNSUPPORTED, "%zu passes in one round (at most 32768)", rounds[r].size());
 		RoundLayout &L = rl[r];
 		L.n_setup = L.n_span = L.n_tmem = L.n_tmem_gather = L.n_bin = L.n_raster = L.n_fill = L.n_quad = 0;
-		L.n_dom_pre = L.n_dom_post = L.n_up = L.n_resolve = 0;
+		L.n_dom_pre = L.n_dom_post = L.n_up = L.n_up_fill = L.n_resolve = 0;
 		for (uint32_t c = 0; c < kRasterClasses; c++) L.n_raster_class[c] = L.n_quad_class[c] = L.n_up_class[c] = 0;
 		size_t scr = scratch_base;
 		pl[r].resize(rounds[r].size());
@@ -741,7 +745,9 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			pi.fill = fill_only(hp);
 			if (!hp.prims.empty()) {
 				L.n_bin += ty * (groups / 8);
-				if (up) {
+				if (up && pi.fill) {
+					L.n_up_fill += ty * ((tx + kFillTilesPerItem - 1) / kFillTilesPerItem);
+				} else if (up) {
 					L.n_up += tx * ty;
 					L.n_up_class[std::min(hp.raster_class, kRasterClasses - 1)] += tx * ty;
 				} else if (pi.fill) {
@@ -774,7 +780,7 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 		L.items_domain = off;
 		off = align_up(off + (L.n_dom_pre + L.n_dom_post + L.n_resolve) * 4ull, 256);
 		L.items_up = off;
-		off = align_up(off + L.n_up * 4ull, 256);
+		off = align_up(off + (L.n_up + L.n_up_fill) * 4ull, 256);
 		for (size_t i = 0; i < rounds[r].size(); i++) {
 			const HostPass &hp = *rounds[r][i].pass;
 			PassInputs &pi = pl[r][i];
@@ -863,6 +869,8 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			LA.up_items[c] = reinterpret_cast<const uint32_t *>(d + L.items_up) + upbase[c];
 			LA.up_n[c] = L.n_up_class[c];
 		}
+		LA.up_fill_items = reinterpret_cast<const uint32_t *>(d + L.items_up) + L.n_up;
+		LA.up_fill_n = L.n_up_fill;
 		LA.domain_pre_items = reinterpret_cast<const uint32_t *>(d + L.items_domain);
 		LA.domain_pre_n = L.n_dom_pre;
 		LA.domain_post_items = LA.domain_pre_items + L.n_dom_pre;
@@ -982,11 +990,18 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			return tiles ? (double)hp.prims.size() / (double)tiles : 0.0;
 		};
 		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return density(a) > density(b); });
-		uint32_t nf = 0;
+		uint32_t nf = 0, nuf = 0;
 		for (uint32_t pidx : order) {
 			const HostPass &hp = *rounds[r][pidx].pass;
 			if (hp.prims.empty()) continue;
 			const uint32_t tiles_x = (hp.fb_width + 7) / 8, tiles_y = (hp.fb_height + 7) / 8;
+			if (hp.scale_log2 && pl[r][pidx].fill) {
+				uint32_t *out = it_up + L.n_up;
+				for (uint32_t y = 0; y < tiles_y; y++)
+					for (uint32_t x = 0; x < (tiles_x + kFillTilesPerItem - 1) / kFillTilesPerItem; x++)
+						out[nuf++] = (pidx << 20) | (x << 10) | y;
+				continue;
+			}
 			if (hp.scale_log2) {
 				const uint32_t cls = std::min(hp.raster_class, kRasterClasses - 1);
 				uint32_t *out = it_up + upbase[cls];

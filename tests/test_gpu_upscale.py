@@ -307,3 +307,43 @@ def test_super_sampled_readback_flags(rdp_mi):
     p = rdp_mi.CommandProcessor(rs.RDRAM_SIZE, flags=rdp_mi.FLAG_UPSCALING[2] | rdp_mi.FLAG_UPSCALING[4])
     assert p.upscaling_factor == 4
     p.close()
+
+
+def fill_only_stream(seed, frames=3):
+    """Passes of FILL-mode rectangles only (the upscaled fill kernel's work):
+    random scissors, fractional rectangle edges, 8/16/32-bit colour images,
+    IA88 fills and a colour image that is also the depth image."""
+    rng = np.random.default_rng(seed)
+    out = []
+    width, height = 320, 240
+    for _ in range(frames):
+        cl = rs.CommandList()
+        cl.set_other_modes(rs.OtherModes(cycle_type=3))
+        for k in range(4):
+            size, fmt = [(rs.SIZ_16, rs.FMT_RGBA), (rs.SIZ_32, rs.FMT_RGBA), (rs.SIZ_8, rs.FMT_I),
+                         (rs.SIZ_16, rs.FMT_IA)][int(rng.integers(4))]
+            addr = rs.DEPTH_ADDR if k == 3 else rs.COLOR_ADDR
+            cl.set_color_image(addr, width, size, fmt)
+            if k == 3:
+                cl.set_mask_image(addr)
+            x0, y0 = int(rng.integers(0, 40)), int(rng.integers(0, 40))
+            cl.set_scissor(x0 << 2, y0 << 2, int(rng.integers(200, width + 1)) << 2,
+                           int(rng.integers(150, height + 1)) << 2)
+            for _ in range(int(rng.integers(1, 6))):
+                cl.set_fill_color(int(rng.integers(0, 1 << 32)))
+                a = rng.integers(0, (width << 2), 2)
+                b = rng.integers(0, (height << 2), 2)
+                cl.fill_rectangle(int(a.min()), int(b.min()), int(a.max()), int(b.max()))
+            cl.sync_pipe()
+        cl.sync_full()
+        out.append(rs.Frame(words=cl.words()))
+    return rs.Stream(name=f"fill_only_{seed}", seed=seed, frames=out, color_addr=rs.COLOR_ADDR,
+                     depth_addr=rs.DEPTH_ADDR, width=width, height=height)
+
+
+@pytest.mark.parametrize("seed,factor,ssaa", [(1, 2, None), (2, 4, None), (3, 8, None), (4, 2, True), (5, 4, False)])
+def test_fill_only_passes_upscaled(rdp_mi, seed, factor, ssaa):
+    stream = fill_only_stream(seed)
+    say(f"fill-only stream {seed} at {factor}x, super-sampled readback {ssaa}")
+    assert_same(run_gpu(rdp_mi, stream, factor, ssaa=ssaa), oracle_state(stream, factor, ssaa=ssaa),
+                f"fill-only stream {seed} at {factor}x")
