@@ -160,6 +160,7 @@ struct PinnedBuffer {
 
 struct RoundEvents {
 	hipEvent_t fork = nullptr, tmem_done = nullptr, bin_done = nullptr, join = nullptr, prep_done = nullptr;
+	hipEvent_t prep_heavy = nullptr;  // setup and binning of the passes above class 0 done
 	hipEvent_t pre_raster = nullptr;  // the round's stream just before its raster launches
 };
 
@@ -198,6 +199,7 @@ struct Lane {
 			HIP_CHECK(hipEventCreateWithFlags(&e.bin_done, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.join, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.prep_done, hipEventDisableTiming));
+			HIP_CHECK(hipEventCreateWithFlags(&e.prep_heavy, hipEventDisableTiming));
 			HIP_CHECK(hipEventCreateWithFlags(&e.pre_raster, hipEventDisableTiming));
 			events.push_back(e);
 		}
@@ -213,7 +215,7 @@ struct Lane {
 		if (staging.ptr) (void)hipHostFree(staging.ptr);
 		if (staging_done) (void)hipEventDestroy(staging_done);
 		for (auto &e : events)
-			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join, e.prep_done, e.pre_raster })
+			for (hipEvent_t ev : { e.fork, e.tmem_done, e.bin_done, e.join, e.prep_done, e.prep_heavy, e.pre_raster })
 				(void)hipEventDestroy(ev);
 		if (start) (void)hipEventDestroy(start);
 		if (side) (void)hipStreamDestroy(side);
@@ -431,6 +433,10 @@ struct Launch {
 	uint32_t tmem_scan_n;
 	const uint32_t *span_items;  // second setup launch: one item per 8 tile-row entries
 	uint32_t span_n;
+	// The setup, span and bin items of passes above occupancy class 0 come
+	// first in their lists: run ahead on their own, they let the heavier
+	// classes' raster start while the light passes are still being set up.
+	uint32_t setup_hi, span_hi, bin_hi;
 	const uint32_t *raster_items[kRasterClasses];  // 8x8 tiles
 	uint32_t raster_n[kRasterClasses];
 	const uint32_t *quad_items[kRasterClasses];  // the same passes as 16x16 tiles
@@ -561,8 +567,13 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 		HIP_CHECK(hipStreamWaitEvent(prep, lane.start, 0));
 		for (size_t r = 0; r < launches.size(); r++) {
 			const Launch &L = launches[r];
-			HIP_CHECK(launch_setup(L.descs, L.items[0], L.n[0], L.span_items, L.span_n, prep));
-			HIP_CHECK(launch_bin(L.descs, L.items[2], L.n[2], prep));
+			// The heavier classes' passes first (prep_heavy), then the rest.
+			HIP_CHECK(launch_setup(L.descs, L.items[0], L.setup_hi, L.span_items, L.span_hi, prep));
+			HIP_CHECK(launch_bin(L.descs, L.items[2], L.bin_hi, prep));
+			HIP_CHECK(hipEventRecord(lane.events[r].prep_heavy, prep));
+			HIP_CHECK(launch_setup(L.descs, L.items[0] + L.setup_hi, L.n[0] - L.setup_hi, L.span_items + L.span_hi,
+			                       L.span_n - L.span_hi, prep));
+			HIP_CHECK(launch_bin(L.descs, L.items[2] + L.bin_hi, L.n[2] - L.bin_hi, prep));
 			HIP_CHECK(hipEventRecord(lane.events[r].prep_done, prep));
 		}
 	}
@@ -619,12 +630,13 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			                              L.fill_items, L.fill_n, std::min(total, g_persistent_grid), s));
 		} else if (fork_raster) {
 			// The heavy launch needs setup, binning and the TMEM replay.
-			// With both forked, prep_done covers the first two and the side
-			// stream's own order the third; otherwise the main stream has
-			// reached all of them here.
+			// With both forked, prep_heavy covers the first two (the heavy
+			// launch's classes are all above class 0) and the side stream's
+			// own order the third; otherwise the main stream has reached all
+			// of them here.
 			// The domain's pre update (main stream) must read native RDRAM
 			// before any raster of the round writes it.
-			hipEvent_t ready = E.prep_done;
+			hipEvent_t ready = E.prep_heavy;
 			if (!(ahead && fork_tmem) || L.domain_pre_n) {
 				HIP_CHECK(hipEventRecord(E.bin_done, s));
 				ready = E.bin_done;
@@ -978,6 +990,15 @@ static int prepare_rounds(Lane &c, const std::vector<std::vector<PassRef>> &roun
 			if (hp.ssaa_resolve)
 				for (uint32_t b = 0; b < resolve_blocks(hp); b++)
 					it_domain[L.n_dom_pre + L.n_dom_post + nrs++] = (pidx << 17) | b;
+		}
+		// Setup, span and bin items: the passes above class 0 first.
+		{
+			auto heavy = [&](uint32_t pidx) { return rounds[r][pidx].pass->raster_class > 0; };
+			LA.setup_hi = (uint32_t)(std::stable_partition(it_setup, it_setup + L.n_setup,
+			                                               [&](uint32_t it) { return heavy(it >> 7); }) - it_setup);
+			uint32_t *sp = it_setup + L.n_setup;
+			LA.span_hi = (uint32_t)(std::stable_partition(sp, sp + L.n_span, [&](uint32_t it) { return heavy(it >> 17); }) - sp);
+			LA.bin_hi = (uint32_t)(std::stable_partition(it_bin, it_bin + L.n_bin, [&](uint32_t it) { return heavy(it >> 17); }) - it_bin);
 		}
 		// Raster items, per class, passes with the most primitives per tile
 		// first: their tiles have the longest chains, so they start early
