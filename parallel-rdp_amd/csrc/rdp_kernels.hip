@@ -1375,8 +1375,14 @@ static_assert(offsetof(RasterPrim, variant) - offsetof(RasterPrim, ylo) == offse
 struct Pixel {
 	int32_t c[4];
 	uint32_t depth, dz;
-	bool color_dirty, depth_dirty;
+	// PIXEL_COLOR_DIRTY | PIXEL_DEPTH_DIRTY, kept in a vector register
+	// (raster_prim): as booleans the two flags live in scalar lane masks,
+	// merged with scalar instructions at the end of every divergent branch
+	// that sets them.
+	uint32_t dirty;
 };
+#define PIXEL_COLOR_DIRTY 1u
+#define PIXEL_DEPTH_DIRTY 2u
 
 // The Z comparison terms of depth_test.h:41-144 for a fragment at depth z
 // with primitive dz against the pixel's stored depth and dz code; shared by
@@ -1432,7 +1438,7 @@ DEV void write_color(Pixel &p, uint32_t fmt, int32_t r, int32_t g, int32_t b, in
 	p.c[1] = g;
 	p.c[2] = b;
 	if (fmt != FB_I4) p.c[3] = a;
-	p.color_dirty = true;
+	p.dirty |= PIXEL_COLOR_DIRTY;
 }
 
 DEV void alias_color_to_depth(Pixel &p, uint32_t fmt)
@@ -1463,7 +1469,7 @@ DEV void alias_depth_to_color(Pixel &p, uint32_t fmt)
 		p.c[0] = (int32_t)((w >> 10) & 0xff);
 		p.c[3] = (int32_t)((w >> 2) & 0xff);
 	}
-	p.color_dirty = true;
+	p.dirty |= PIXEL_COLOR_DIRTY;
 }
 
 template <typename Modes>
@@ -2020,7 +2026,7 @@ DEV void depth_blend(const AS_GLOBAL uint8_t *lut, const AS_CONST RasterPrim *ps
 		if (z_update) {
 			p.depth = z_compress(z);
 			p.dz = (uint32_t)h.dz_compressed;
-			p.depth_dirty = true;
+			p.dirty |= PIXEL_DEPTH_DIRTY;
 			if (alias) alias_depth_to_color(p, fmt);
 		} else if (alias)
 			alias_color_to_depth(p, fmt);
@@ -2058,7 +2064,7 @@ DEV void copy_pixel(Pixel &p, uint32_t fmt, bool alias, uint32_t word)
 	switch (fmt) {
 	case FB_I4:
 		p.c[0] = p.c[1] = p.c[2] = p.c[3] = 0;
-		p.color_dirty = true;
+		p.dirty |= PIXEL_COLOR_DIRTY;
 		break;
 	case FB_I8: {
 		int32_t v = (int32_t)(word & 0xff);
@@ -2112,13 +2118,16 @@ DEV void raster_prim(const AS_CONST RasterPrim *ps, const RasterPrimHead &h, con
 			for (int i = 0; i < 6; i++) asm volatile("" ::"s"(gt[i]));
 		}
 	}
-	if (!shade_pixel<V>(ps, h, states[h.state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh, L)) return;
-	if (sh.coverage & COVERAGE_FILL)
-		fill_pixel(px, fmt, alias, ci, h.fill_color);
-	else if (sh.coverage & COVERAGE_COPY)
-		copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
-	else
-		depth_blend<V>(lut, ps, h, fmt, alias, px, sh);
+	asm volatile("" : "+v"(px.dirty));
+	if (shade_pixel<V>(ps, h, states[h.state_index].tiles, tex, fmt, span, sp + 1, x, y, px, sh, L)) {
+		if (sh.coverage & COVERAGE_FILL)
+			fill_pixel(px, fmt, alias, ci, h.fill_color);
+		else if (sh.coverage & COVERAGE_COPY)
+			copy_pixel(px, fmt, alias, (uint32_t)sh.z_dith);
+		else
+			depth_blend<V>(lut, ps, h, fmt, alias, px, sh);
+	}
+	asm volatile("" : "+v"(px.dirty));
 }
 
 // Waves per SIMD each occupancy class is compiled for (the register budget
@@ -2271,7 +2280,7 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 	const uint32_t di = (P.depth_addr >> 1) + nrow * ny + nx;
 
 	Pixel px;
-	px.color_dirty = px.depth_dirty = false;
+	px.dirty = 0;
 	load_color(m, fmt, ci, px);
 	{
 		uint32_t w = m.rd16i(di);
@@ -2340,16 +2349,16 @@ DEV void raster_tile(const PassDesc *__restrict__ passes, uint32_t item, uint16_
 	asm volatile("" : "+s"(pe));
 	const Mem me{ gptr(pe->rdram) + plane8, gptr(pe->hidden) + (plane8 >> 1), pe->rdram_mask };
 	AS_GLOBAL uint8_t *wm = gptr(pe->wmask);
-	if (in_fb && px.color_dirty) {
+	if (in_fb && (px.dirty & PIXEL_COLOR_DIRTY)) {
 		store_color(me, fmt, ci, px);
 		if (wm) mark_color(wm, pe->rdram_mask, fmt, ci);
 	}
 	if constexpr (UP) {
 		if (pe->domain_flags & DOMAIN_SSAA_RESOLVE)
 			emit_scaled_write_masks(pe->up_wmask, ny * (pe->fb_width >> L) + nx,
-			                        in_fb ? (px.color_dirty ? 1u : 0u) | (px.depth_dirty ? 2u : 0u) : 0u, L, lane);
+			                        in_fb ? px.dirty : 0u, L, lane);
 	}
-	if (in_fb && !alias && px.depth_dirty) {
+	if (in_fb && !alias && (px.dirty & PIXEL_DEPTH_DIRTY)) {
 		const uint32_t de = (pe->depth_addr >> 1) + (pe->fb_width >> L) * ny + nx;
 		me.wr16i(de, (px.depth << 2) | (px.dz >> 2));
 		me.set_hid(de, px.dz & 3);
@@ -2428,7 +2437,7 @@ DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t 
 		Pixel px;
 		px.c[0] = px.c[1] = px.c[2] = px.c[3] = 0;
 		px.depth = px.dz = 0;
-		px.color_dirty = px.depth_dirty = false;
+		px.dirty = 0;
 		for (uint32_t gbase = 0; gbase < G; gbase += 64) {
 			const uint64_t my_mask =
 			    gbase == 0 ? first[t] : (gbase + (uint32_t)lane < G ? bins[(size_t)(ty * tiles_x + tx) * G + gbase + lane] : 0);
@@ -2449,7 +2458,7 @@ DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t 
 				}
 			}
 		}
-		if (in_fb && px.color_dirty) {
+		if (in_fb && (px.dirty & PIXEL_COLOR_DIRTY)) {
 			store_color(m, fmt, ci, px);
 			if (P.wmask) mark_color(gptr(P.wmask), P.rdram_mask, fmt, ci);
 		}
@@ -2457,7 +2466,7 @@ DEV void fill_item(const PassDesc *__restrict__ passes, uint32_t item, uint16_t 
 		// colour store is the depth store).
 		if constexpr (UP) {
 			if (P.domain_flags & DOMAIN_SSAA_RESOLVE)
-				emit_scaled_write_masks(P.up_wmask, ny * (P.fb_width >> L) + nx, in_fb && px.color_dirty ? 1u : 0u, L,
+				emit_scaled_write_masks(P.up_wmask, ny * (P.fb_width >> L) + nx, in_fb ? px.dirty & PIXEL_COLOR_DIRTY : 0u, L,
 				                        lane);
 		}
 	}
