@@ -773,14 +773,14 @@ struct Tex {
 	MDEV uint32_t t8(uint32_t i) const { return reinterpret_cast<const AS_LDS uint8_t *>(tmem)[i & 0xfff]; }
 };
 
+// Branch-free on the tile's (uniform) mask and mirror bits: with no mask
+// both operations reduce to the identity (mirror bit 0, mask all ones).
 DEV int mask_coord(int v, uint32_t mask_bits, bool mirror)
 {
-	if (mask_bits) {
-		int mask = 1 << mask_bits;
-		if (mirror) v ^= imax((v & mask) - 1, 0);
-		v &= mask - 1;
-	}
-	return v;
+	const int bit = mask_bits ? (1 << mask_bits) : 0;
+	const int keep = mask_bits ? bit - 1 : -1;
+	v ^= imax((v & (mirror ? bit : 0)) - 1, 0);
+	return v & keep;
 }
 
 DEV void rgba16_to(uint32_t w, int32_t o[4])
@@ -910,26 +910,26 @@ DEV void fetch_yuv16(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, u
 	o[2] = o[3] = (int32_t)luma;
 }
 
-DEV int clamp_and_shift_coord(bool clamp_bit, int c, int lo, int hi, int shift)
+// The tile's shift (uniform): right by 0-10, or left by 1-5 with the
+// result kept to 16 bits (shifts 11-15).  Both forms are computed and one
+// selected, which costs fewer instructions than a branch per coordinate.
+DEV int tile_shift(int c, int shift)
 {
 	c = iclamp(c, -0x8000, 0x7fff);
-	if (shift < 11) c >>= shift;
-	else c = (c << (32 - shift)) >> 16;
-	if (clamp_bit) {
-		if ((c >> 3) >= hi) c = (((hi >> 2) - (lo >> 2)) & 0x3ff) << 5;
-		else c = imax(c - (lo << 3), 0);
-	} else
-		c -= lo << 3;
-	return c;
+	const int right = c >> (shift < 11 ? shift : 0);
+	const int left = (int)((uint32_t)c << ((32 - shift) & 31)) >> 16;
+	return shift < 11 ? right : left;
 }
 
-DEV int shift_coord(int c, int lo, int shift)
+DEV int clamp_and_shift_coord(bool clamp_bit, int c, int lo, int hi, int shift)
 {
-	c = iclamp(c, -0x8000, 0x7fff);
-	if (shift < 11) c >>= shift;
-	else c = (c << (32 - shift)) >> 16;
-	return c - (lo << 3);
+	c = tile_shift(c, shift);
+	const int base = c - (lo << 3);
+	const int clamped = (c >> 3) >= hi ? (((hi >> 2) - (lo >> 2)) & 0x3ff) << 5 : imax(base, 0);
+	return clamp_bit ? clamped : base;
 }
+
+DEV int shift_coord(int c, int lo, int shift) { return tile_shift(c, shift) - (lo << 3); }
 
 DEV void convert_factors(const int32_t in[4], const int32_t f[4], int32_t o[4])
 {
@@ -1331,19 +1331,14 @@ DEV void dither_coefficients(int x, int y, int mode_rgb, int mode_alpha, uint32_
 	                       01ull << 21 | 03ull << 24 | 07ull << 27 | 02ull << 30 | 06ull << 33 | 07ull << 36 |
 	                       03ull << 39 | 06ull << 42 | 02ull << 45;
 	const int idx = (y & 3) * 4 + (x & 3);
-	const int mag = (int)((magic >> (3 * idx)) & 7);
-	const int bay = (int)((bayer >> (3 * idx)) & 7);
-	if (mode_rgb == 0) rgb_dith = mag * 0x49;
-	else if (mode_rgb == 1) rgb_dith = bay * 0x49;
-	else if (mode_rgb == 2) rgb_dith = (int)(noise & 0x1ff);
-	else rgb_dith = 0;
-	if (mode_alpha == 3) alpha_dith = 0;
-	else if (mode_alpha == 2) alpha_dith = (int)(noise & 7);
-	else {
-		int a = mode_rgb >= 2 ? ((mode_rgb & 1) ? bay : mag) : (rgb_dith & 7);
-		if (mode_alpha == 1) a = ~a & 7;
-		alpha_dith = a;
-	}
+	// The matrix entry every mode that uses one reads: Bayer for the odd RGB
+	// modes, magic square for the even ones (the alpha pattern follows the
+	// RGB matrix, and in the noise and off RGB modes the same parity rule).
+	// Selects on the primitive's modes rather than branches: the modes are
+	// uniform, and branches on them cost scalar instructions per pixel.
+	const int m = (int)(((mode_rgb & 1) ? bayer : magic) >> (3 * idx)) & 7;
+	rgb_dith = mode_rgb < 2 ? m * 0x49 : (mode_rgb == 2 ? (int)(noise & 0x1ff) : 0);
+	alpha_dith = mode_alpha == 3 ? 0 : (mode_alpha == 2 ? (int)(noise & 7) : (mode_alpha == 1 ? (~m & 7) : m));
 }
 
 // ---- depth / blend: z_encode.h, depth_test.h, blender.h, coverage.h --------
