@@ -149,12 +149,19 @@ def _per_dispatch(path: str, counter: str, kernel_filter: str = ""):
 
 
 def short_name(kname: str) -> str:
-    """'void rdp_mi::k_raster<0u, false>(...)' -> 'k_raster<0>' (8x8 tiles);
-    'void rdp_mi::k_raster<0u, true>(...)' -> 'k_raster<0> 16x16'."""
+    """'void rdp_mi::k_raster<0u, false, false>(...)' -> 'k_raster<0>' (8x8
+    tiles); 'void rdp_mi::k_raster<0u, true, false>(...)' -> 'k_raster<0>
+    16x16'; 'void rdp_mi::k_raster<0u, false, true>(...)' -> 'k_raster<0>
+    upscaled'.  Names without the third argument (older builds) map alike."""
     n = kname.split("(")[0].replace("void ", "").replace("rdp_mi::", "")
-    quad = ", true>" in n
-    n = n.replace(", false>", ">").replace(", true>", ">").replace("u>", ">")
-    return n + " 16x16" if quad else n
+    if "<" not in n:
+        return n
+    base, args = n.split("<", 1)
+    parts = [a.strip() for a in args.rstrip(">").split(",")]
+    quad = len(parts) > 1 and parts[1] == "true"
+    up = len(parts) > 2 and parts[2] == "true"
+    n = f"{base}<{parts[0].rstrip('u')}>"
+    return n + (" 16x16" if quad else "") + (" upscaled" if up else "")
 
 
 def traffic(fetch_db, write_db, copyf_db, copyw_db, sq_db, copy_bytes, out, kernel_filter="k_raster"):
