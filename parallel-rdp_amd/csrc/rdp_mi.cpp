@@ -577,6 +577,7 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			HIP_CHECK(hipEventRecord(lane.events[r].prep_done, prep));
 		}
 	}
+	bool prev_tmem_forked = false;
 	for (size_t r = 0; r < launches.size(); r++) {
 		const Launch &L = launches[r];
 		const bool big = overlap && L.passes >= kOverlapMinPasses;
@@ -601,11 +602,18 @@ static int execute_launches(const std::vector<Launch> &launches, Lane &lane, con
 			// none of its uploads reads what that round's raster writes:
 			// then it starts before that raster (`early`).  E.fork marks the
 			// end of the previous round's raster launches on the main stream.
+			// In round 1 of a submission whose setup runs ahead, an early
+			// replay needs only what precedes the submission (lane.start)
+			// when round 0's own TMEM replay is ahead of it on the side
+			// stream.
 			HIP_CHECK(hipEventRecord(E.fork, s));
-			HIP_CHECK(hipStreamWaitEvent(side, early ? lane.events[r - 1].pre_raster : E.fork, 0));
+			hipEvent_t after = E.fork;
+			if (early) after = (r == 1 && ahead && prev_tmem_forked) ? lane.start : lane.events[r - 1].pre_raster;
+			HIP_CHECK(hipStreamWaitEvent(side, after, 0));
 			HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, side));
 			HIP_CHECK(hipEventRecord(E.tmem_done, side));
 		}
+		prev_tmem_forked = fork_tmem || !L.tmem_scan_n;
 		if (ahead) {
 			if (!fork_tmem) HIP_CHECK(launch_tmem(L.descs, L.items[1], L.n[1], L.tmem_scan_items, L.tmem_scan_n, s));
 			HIP_CHECK(hipStreamWaitEvent(s, E.prep_done, 0));
