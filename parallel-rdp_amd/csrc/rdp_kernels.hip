@@ -1214,6 +1214,11 @@ struct CombIn {
 
 // Selector decoding follows the reference's select_* helpers: values that
 // do not name a per-pixel input read the constant slot.
+// The constant inputs are the primitive's resolved constants (k, per
+// selector slot: prim / env colour, key, convert and LOD terms), which the
+// host fills only for the selectors that name one and leaves 0 for every
+// other; those selectors return 0 here without reading k, so a combiner
+// known at compile time drops the loads and arithmetic of its zero terms.
 DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const AS_CONST uint8_t *k)
 {
 	switch (which) {
@@ -1225,7 +1230,9 @@ DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const AS_CONST
 		case 4: return in.shade[ch];
 		case 6: return 0x100;
 		case 7: return in.noise;
-		default: return k[ch];
+		case 3:
+		case 5: return k[ch];
+		default: return 0;
 		}
 	case 1:  // mulsub
 		switch (sel) {
@@ -1234,7 +1241,10 @@ DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const AS_CONST
 		case 2: return in.texel1[ch];
 		case 4: return in.shade[ch];
 		case 7: return ((int32_t)k[1] << 8) | k[2];
-		default: return k[ch];
+		case 3:
+		case 5:
+		case 6: return k[ch];
+		default: return 0;
 		}
 	case 2:  // mul
 		switch (sel) {
@@ -1248,7 +1258,13 @@ DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const AS_CONST
 		case 11: return in.shade[3];
 		case 13: return in.lod_frac;
 		case 15: return ((int32_t)k[1] << 8) | k[2];
-		default: return k[ch];
+		case 3:
+		case 5:
+		case 6:
+		case 10:
+		case 12:
+		case 14: return k[ch];
+		default: return 0;
 		}
 	default:  // add
 		switch (sel) {
@@ -1257,7 +1273,9 @@ DEV int32_t sel_rgb(int which, int sel, int ch, const CombIn &in, const AS_CONST
 		case 2: return in.texel1[ch];
 		case 4: return in.shade[ch];
 		case 6: return 0x100;
-		default: return k[ch];
+		case 3:
+		case 5: return k[ch];
+		default: return 0;
 		}
 	}
 }
@@ -1270,7 +1288,10 @@ DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const AS_CONST uint8
 		case 1: return in.texel0[3];
 		case 2: return in.texel1[3];
 		case 4: return in.shade[3];
-		default: return k[3];
+		case 3:
+		case 5:
+		case 6: return k[3];
+		default: return 0;
 		}
 	}
 	switch (sel) {
@@ -1279,7 +1300,9 @@ DEV int32_t sel_alpha(int which, int sel, const CombIn &in, const AS_CONST uint8
 	case 2: return in.texel1[3];
 	case 4: return in.shade[3];
 	case 6: return 0x100;
-	default: return k[3];
+	case 3:
+	case 5: return k[3];
+	default: return 0;
 	}
 }
 
