@@ -201,7 +201,7 @@ constexpr uint32_t RAS_2CYC_SPRITE_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RA
 	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                \
 	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                \
 	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY, DBP_Z_OPAQUE, 0)                                                   \
-	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                          \
+	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY, DBP_ANY, 1)                                                          \
 	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_Z_FORCE_BLEND, 0)        \
 	X(14, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_ANY, 0)                 \
 	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY, DBP_ANY, 1)                                      \
