@@ -711,8 +711,11 @@ DEV void perspective_divide(int32_t s, int32_t t, int32_t w, int32_t &os, int32_
 	int i = normout >> 8;
 	int32_t base = s_persp_base[i];
 	int32_t slope = -4 * (base - s_persp_base[i + 1]);
-	int32_t rcp = ((slope * wnorm) >> 10) + base;
-	int32_t prod[2] = { s * rcp, t * rcp };
+	// s and t are 16-bit, the reciprocal below 2^15 and the slope term
+	// small: every operand fits 24 bits, so the full-rate 24-bit multiply
+	// gives the exact 32-bit product.
+	int32_t rcp = (__mul24(slope, wnorm) >> 10) + base;
+	int32_t prod[2] = { __mul24(s, rcp), __mul24(t, rcp) };
 	int32_t temp_mask = ((1 << 30) - 1) & -((1 << 29) >> shift);
 	int32_t out[2];
 #pragma unroll
@@ -799,15 +802,21 @@ DEV void ia16_to(uint32_t w, int32_t o[4])
 	o[3] = (int32_t)(w & 0xff);
 }
 
+// A texel row's TMEM base: offset + stride * row.  Every TMEM address keeps
+// at most its low 13 bits, which depend only on the low 24 bits of the
+// operands, so the 24-bit multiply (full rate, where a 32-bit one issues at a
+// quarter of it) gives the same address.
+DEV uint32_t row_base(const TileDesc &t, uint32_t row) { return t.offset + __umul24(t.stride, row); }
+
 DEV uint32_t tidx8(const TileDesc &t, uint32_t v, uint32_t wrap, uint32_t add)
 {
-	uint32_t b = (t.offset + t.stride * v + add) & wrap;
+	uint32_t b = (row_base(t, v) + add) & wrap;
 	return (b ^ ((v & 1) << 2)) ^ 3;
 }
 
 DEV uint32_t tidx16(const TileDesc &t, uint32_t s, uint32_t v, uint32_t wrap)
 {
-	uint32_t b = (t.offset + t.stride * v + s * 2) & wrap;
+	uint32_t b = (row_base(t, v) + s * 2) & wrap;
 	return ((b >> 1) ^ ((v & 1) << 1)) ^ 1;
 }
 
@@ -899,7 +908,7 @@ DEV void fetch_texel(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, b
 
 DEV void fetch_yuv16(const Tex &tx, const TileDesc &t, uint32_t s, uint32_t v, uint32_t cx, int32_t o[4])
 {
-	uint32_t bo = t.offset + t.stride * v;
+	uint32_t bo = row_base(t, v);
 	uint32_t bl = (bo + s) & 0x7ff, bc = (bo + cx * 2) & 0x7ff;
 	uint32_t il = (bl ^ ((v & 1) << 2)) ^ 3;
 	uint32_t ic = ((bc >> 1) ^ ((v & 1) << 1)) ^ 1;
@@ -934,9 +943,10 @@ DEV int shift_coord(int c, int lo, int shift) { return tile_shift(c, shift) - (l
 DEV void convert_factors(const int32_t in[4], const int32_t f[4], int32_t o[4])
 {
 	int32_t t0 = sext(in[0], 9), t1 = sext(in[1], 9), t2 = sext(in[2], 9);
-	o[0] = t2 + ((f[0] * t1 + 0x80) >> 8);
-	o[1] = t2 + ((f[1] * t0 + f[2] * t1 + 0x80) >> 8);
-	o[2] = t2 + ((f[3] * t0 + 0x80) >> 8);
+	// 16-bit factors times 9-bit terms: exact with 24-bit multiplies.
+	o[0] = t2 + ((__mul24(f[0], t1) + 0x80) >> 8);
+	o[1] = t2 + ((__mul24(f[1], t0) + __mul24(f[2], t1) + 0x80) >> 8);
+	o[2] = t2 + ((__mul24(f[3], t0) + 0x80) >> 8);
 	o[3] = t2;
 }
 
@@ -949,7 +959,7 @@ DEV void bilinear_3tap2(const int32_t *t00, const int32_t *t10, const int32_t *t
 	int ffy = sum >= 32 ? 32 - fx : fy;
 #pragma unroll
 	for (int i = 0; i < 2; i++)
-		o[i] = ((((t10[i] - b[i]) * ffx + (t01[i] - b[i]) * ffy) + 0x10) >> 5) + b[i];
+		o[i] = (((__mul24(t10[i] - b[i], ffx) + __mul24(t01[i] - b[i], ffy)) + 0x10) >> 5) + b[i];
 }
 
 // texture.h:498-923
@@ -1165,7 +1175,7 @@ DEV int sample_copy_word(const Tex &tx, const TileDesc &tile, int st_s, int st_t
 	if (replicate) {
 		int sx = st_s + 2 * s_offset;
 		int t = mask_coord(st_t, tile.mask_t, mt);
-		uint32_t tbase = tile.offset + tile.stride * (uint32_t)t;
+		uint32_t tbase = row_base(tile, (uint32_t)t);
 		uint32_t sam[2];
 #pragma unroll
 		for (int i = 0; i < 2; i++) {
@@ -1185,7 +1195,7 @@ DEV int sample_copy_word(const Tex &tx, const TileDesc &tile, int st_s, int st_t
 	}
 	int sv = mask_coord(st_s + s_offset, tile.mask_s, ms);
 	int t = mask_coord(st_t, tile.mask_t, mt);
-	uint32_t tbase = tile.offset + tile.stride * (uint32_t)t;
+	uint32_t tbase = row_base(tile, (uint32_t)t);
 	uint32_t nib = (tbase * 2 + ((uint32_t)sv << shamt)) & 0x1fff;
 	nib ^= ((uint32_t)t & 1) * 8;
 	int samp = (int)tx.t16(((nib >> 2) & idx_mask) ^ 1);
@@ -1340,7 +1350,7 @@ DEV void combine(const CombIn &in, const Sel &sel, const AS_CONST uint8_t (*k)[4
 		a = special_expand(a);
 		b = special_expand(b);
 		d = special_expand(d);
-		out[ch] = (((a - b) * c + 0x80) >> 8) + d;
+		out[ch] = ((__mul24(a - b, c) + 0x80) >> 8) + d;
 	}
 }
 
@@ -1544,12 +1554,12 @@ DEV void blender(const AS_GLOBAL uint8_t *lut, const int32_t pixel[4], const int
 	}
 	if (!final_cycle || force_blend) {
 #pragma unroll
-		for (int i = 0; i < 3; i++) out[i] = ((src0[i] * a0 + src1[i] * (a1 + 1)) >> 5) & 0xff;
+		for (int i = 0; i < 3; i++) out[i] = ((__mul24(src0[i], a0) + __mul24(src1[i], a1 + 1)) >> 5) & 0xff;
 	} else {
 		const uint32_t sum = (uint32_t)((a0 >> 2) + (a1 >> 2) + 1);
 #pragma unroll
 		for (int i = 0; i < 3; i++) {
-			uint32_t b = (uint32_t)((src0[i] * a0 + src1[i] * (a1 + 1)) >> 2) & 0x7ff;
+			uint32_t b = (uint32_t)((__mul24(src0[i], a0) + __mul24(src1[i], a1 + 1)) >> 2) & 0x7ff;
 			uint32_t idx = (sum << 11) | b;
 			out[i] = idx < 0x8000 ? lut[idx] : 0;
 		}
