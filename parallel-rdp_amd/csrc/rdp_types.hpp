@@ -176,6 +176,12 @@ constexpr int TEXFMT_CI8 = 2 * 4 + 1;
 
 constexpr uint32_t RAS_STRUCT_1CYC = RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL | RAS_USES_TEXEL0 | RAS_USES_TEXEL1 |
                                      RAS_USES_PIPELINED_TEXEL1 | RAS_USES_LOD;
+// Coverage-to-alpha, alpha test, AA and noise all off: the plain shaded
+// state, with every one of those paths folded away at compile time.
+constexpr uint32_t RAS_PLAIN_KNOWN = RAS_AA | RAS_ALPHA_TEST | RAS_CVG_TIMES_ALPHA | RAS_ALPHA_CVG_SELECT | RAS_NEED_NOISE;
+// Coverage AA with alpha from coverage (alpha-coverage select), the other
+// RAS_PLAIN_KNOWN paths off.
+constexpr uint32_t RAS_AA_CVG_SELECT = RAS_AA | RAS_ALPHA_CVG_SELECT;
 constexpr uint32_t RAS_SAMPLING = RAS_PERSPECTIVE | RAS_SAMPLE_QUAD | RAS_BILERP0 | RAS_BILERP1 | RAS_MID_TEXEL |
                                   RAS_CONVERT_ONE | RAS_TLUT | RAS_TLUT_TYPE | RAS_TEX_LOD | RAS_SHARPEN_LOD |
                                   RAS_DETAIL_LOD;
@@ -200,15 +206,21 @@ constexpr uint32_t RAS_2CYC_SPRITE_TLUT = RAS_MULTI_CYCLE | RAS_USES_TEXEL0 | RA
 #define RDP_RASTER_VARIANTS(X)                                                                                        \
 	X(1, RAS_FILL | RAS_COPY, RAS_FILL, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                \
 	X(2, RAS_FILL | RAS_COPY, RAS_COPY, COMB_ANY, TEXFMT_ANY, DBP_ANY, 0)                                                \
+	X(15, RAS_STRUCT_1CYC | RAS_PLAIN_KNOWN, 0u, COMB_SHADE, TEXFMT_ANY, DBP_Z_OPAQUE, 0)                                \
 	X(3, RAS_STRUCT_1CYC, 0u, COMB_SHADE, TEXFMT_ANY, DBP_Z_OPAQUE, 0)                                                   \
 	X(4, RAS_STRUCT_1CYC, 0u, COMB_ANY, TEXFMT_ANY, DBP_ANY, 1)                                                          \
+	X(16, RAS_STRUCT_1CYC | RAS_SAMPLING | RAS_PLAIN_KNOWN, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_Z_FORCE_BLEND, 0) \
 	X(5, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_Z_FORCE_BLEND, 0)        \
 	X(14, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_TEX0_BILINEAR, COMB_TEX0_SHADE, TEXFMT_RGBA16, DBP_ANY, 0)                 \
 	X(6, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0_SHADE, TEXFMT_ANY, DBP_ANY, 1)                                      \
 	X(7, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_TEX0, TEXFMT_ANY, DBP_ANY, 1)                                            \
 	X(8, RAS_STRUCT_1CYC, RAS_USES_TEXEL0, COMB_ANY, TEXFMT_ANY, DBP_ANY, 1)                                             \
+	X(17, RAS_STRUCT_1CYC | RAS_SAMPLING | RAS_PLAIN_KNOWN, RAS_2CYC_LOD_TLUT | RAS_AA_CVG_SELECT, COMB_LOD_SHADE, TEXFMT_CI8, DBP_Z_AA_FOG, 1) \
+	X(18, RAS_STRUCT_1CYC | RAS_SAMPLING | RAS_PLAIN_KNOWN, RAS_2CYC_LOD_TLUT | RAS_AA_CVG_SELECT, COMB_LOD_SHADE, TEXFMT_CI4, DBP_Z_AA_FOG, 1) \
 	X(9, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI8, DBP_Z_AA_FOG, 1)                 \
 	X(10, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_LOD_TLUT, COMB_LOD_SHADE, TEXFMT_CI4, DBP_Z_AA_FOG, 1)                \
+	X(19, RAS_STRUCT_1CYC | RAS_SAMPLING | RAS_PLAIN_KNOWN, RAS_2CYC_SPRITE_TLUT | RAS_AA_CVG_SELECT, COMB_TEX0_ENV, TEXFMT_CI8, DBP_AA_SPRITE, 1) \
+	X(20, RAS_STRUCT_1CYC | RAS_SAMPLING | RAS_PLAIN_KNOWN, RAS_2CYC_SPRITE_TLUT | RAS_AA_CVG_SELECT, COMB_TEX0_ENV, TEXFMT_CI4, DBP_AA_SPRITE, 1) \
 	X(12, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI8, DBP_AA_SPRITE, 1)             \
 	X(13, RAS_STRUCT_1CYC | RAS_SAMPLING, RAS_2CYC_SPRITE_TLUT, COMB_TEX0_ENV, TEXFMT_CI4, DBP_AA_SPRITE, 1)             \
 	X(11, RAS_MULTI_CYCLE | RAS_COPY | RAS_FILL, RAS_MULTI_CYCLE, COMB_ANY, TEXFMT_ANY, DBP_ANY, 2)
